@@ -1,0 +1,264 @@
+/*
+ * sdx_fem.h - C ABI of the MI355X implicit soft-body FEM step.
+ *
+ * Two levels, both plain C: opaque handles, POD parameter structs, host
+ * pointers plus sizes, integer status codes.  No C++ or torch types cross
+ * this boundary.
+ *
+ *   L1  sdx_pcg_bsr3_solve: block-Jacobi PCG on a 3x3-block BSR matrix.  It
+ *       replaces the CudaCG branch of LinearSolver<T>::Solve
+ *       (mochi_core/include/mochi_core/solvers/linear_solver.h:480-484, 615-662).
+ *   L2  sdx_world_*: soft tet actors, pinned nodes, kinematic plane/sphere
+ *       colliders and one backward-Euler Newton step per call.  It replaces
+ *       the soft-body path of Scene::Step
+ *       (mochi_physics/src/mochi_scene.cpp:594 -> mochi_step.cpp:547-569 ->
+ *       mochi_solve.cpp:1090-1160 -> newton_solver.cpp:603-876).
+ *
+ * Reference paths are relative to superdex_physics/libraries/mochi/.
+ * Every function returns SDX_OK (0) or a negative sdx_status; the text of
+ * the last error of the calling thread is returned by sdx_last_error().
+ * One sdx_world must be used by one host thread at a time; it owns its
+ * device buffers and its HIP stream.
+ */
+#ifndef SDX_FEM_H_
+#define SDX_FEM_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t sdx_status;
+enum {
+  SDX_OK = 0,
+  SDX_ERR_INVALID_ARGUMENT = -1,
+  SDX_ERR_DEVICE = -2,       /* a HIP call failed (message names it) */
+  SDX_ERR_OUT_OF_MEMORY = -3,
+  SDX_ERR_STATE = -4,        /* call not allowed in the current world state */
+  SDX_ERR_UNSUPPORTED = -5,
+};
+
+/* ConvergenceStatus / LinearSolverConvergenceStatus, ordered by severity as in
+ * mochi_core/include/mochi_core/solvers/nonlinear_solver_params.h:28-55 and
+ * mochi_core/include/mochi_core/solvers/krylov_solver.h:85-116. */
+enum {
+  SDX_CONV_NONE = 0,
+  SDX_CONV_CONVERGED = 1,
+  SDX_CONV_STOPPED = 2,
+  SDX_CONV_DIVERGED = 3,
+};
+
+/* LinearSolverConvergenceNorm (linear_solver_params.h:303-311). */
+enum {
+  SDX_NORM_RESIDUAL_L2 = 0,
+  SDX_NORM_PRECONDITIONED_RESIDUAL_L2 = 1,
+  SDX_NORM_RESIDUAL_PRECONDITIONER_INDUCED = 2,
+};
+
+/* PreconditionerType subset (linear_solver_params.h:270-283). */
+enum {
+  SDX_PREC_NONE = 0,
+  SDX_PREC_JACOBI = 1,
+  SDX_PREC_BLOCK_JACOBI = 2,
+};
+
+/* LineSearchType subset (nonlinear_solver_params.h:137-194): None = 0, ResidualNorm = 5. */
+enum {
+  SDX_LS_NONE = 0,
+  SDX_LS_RESIDUAL_NORM = 5,
+};
+
+/* PsdProjectionMode (nonlinear_solver_params.h:69-75); IfFail* are not implemented. */
+enum {
+  SDX_PSD_NEVER = 0,
+  SDX_PSD_ALWAYS = 1,
+};
+
+/* Arithmetic of the CG (extension: FP32 matvec with FP64 dots + refresh). */
+enum {
+  SDX_PREC_MODE_FP32 = 0,     /* FP32 vectors, matvec and dots */
+  SDX_PREC_MODE_MIXED = 1,    /* FP32 vectors/matvec, FP64 dots, FP64 residual refresh */
+  SDX_PREC_MODE_FP64 = 2,     /* everything FP64 (reference SUPERDEX_PRECISION=fp64) */
+};
+
+/* CG initial guess (extension). */
+enum {
+  SDX_WARM_START_ZERO = 0,
+  SDX_WARM_START_LEARNED = 1,
+};
+
+/* BSR value layout accepted by sdx_pcg_bsr3_solve. */
+enum {
+  SDX_BSR_BLOCK_ROWMAJOR = 0, /* values[9*k + 3*r + c] = block k, row r, col c */
+  SDX_BSR_REF_ROW_STRIP = 1,  /* BlockSparseMatrix strip: each block row is a row-major
+                                 3 x (3*nnz_row) matrix (block_sparse_matrix.h:1238) */
+};
+
+/* KrylovSolverParams + LinearSolverParams subset (krylov_solver.h:43-82,
+ * linear_solver_params.h:318-345). */
+typedef struct sdx_krylov_params {
+  double abs_tol;
+  double rel_tol;
+  double rel_div_tol;
+  int32_t max_iter;
+  int32_t norm_type;        /* SDX_NORM_* */
+  int32_t preconditioner;   /* SDX_PREC_* */
+  int32_t abort_if_not_spd;
+  int32_t initial_guess_zero; /* 1: x is ignored and treated as 0 (InitialGuessHint::Zero) */
+  int32_t precision_mode;   /* SDX_PREC_MODE_* */
+  int32_t refresh_every;    /* MIXED: recompute r = b - A x in FP64 every k iterations (0: never) */
+  int32_t reserved;
+} sdx_krylov_params;
+
+/* LinearSolverStatus (krylov_solver.h:106-116). */
+typedef struct sdx_linear_status {
+  int32_t num_iter_done;
+  int32_t convergence;       /* SDX_CONV_* */
+  double residual_norm;
+  double relative_residual_norm;
+} sdx_linear_status;
+
+/* SolverParams (mochi_physics/include/mochi_physics/cpp_api/mochi_structs.h:70-81)
+ * mapped the way GetIslandNewtonParams does (mochi_physics/src/mochi_simulation.cpp:34-135),
+ * plus the extensions at the end. */
+typedef struct sdx_solver_params {
+  /* NonLinearSolverParams (nonlinear_solver_params.h:322-445) */
+  int32_t newton_max_iter;        /* default 4 */
+  int32_t psd_proj_mode;          /* SDX_PSD_* (default Always) */
+  double newton_abs_tol;          /* default 1e-3 */
+  double newton_rel_tol;          /* default 1e-6 */
+  double rel_step_tol;            /* default 10*eps(real) */
+  int32_t explosion_control;      /* default 1 */
+  int32_t line_search_type;       /* SDX_LS_* (default ResidualNorm) */
+  double abs_div_tol;             /* default 1e9 */
+  double rel_div_tol;             /* default 1e4 */
+  int32_t line_search_max_iter;   /* default 4 */
+  int32_t linear_tolerance_constant; /* 1: Constant (the only strategy implemented) */
+  double line_search_alpha;       /* default 0.5 */
+  /* LinearSolverParams */
+  sdx_krylov_params linear;
+  /* scene */
+  double gravity[3];              /* default (0, -9.8, 0) */
+  /* extensions */
+  int32_t warm_start;             /* SDX_WARM_START_* */
+  int32_t schedule;               /* element schedule id, -1 = autotuned */
+} sdx_solver_params;
+
+/* Per-actor result of one step: NewtonSolverStatus fields that
+ * StageSolverStats drops, kept here (newton_solver_status.h:40-90). */
+typedef struct sdx_step_stats {
+  int32_t newton_iters;
+  int32_t linear_iters;           /* totalNumLinearIterDone */
+  int32_t line_search_iters;      /* totalNumLSIterDone */
+  int32_t convergence;            /* SDX_CONV_* of the stage */
+  int32_t linear_convergence;     /* of the last linear solve */
+  int32_t warm_start_accepted;    /* learned warm starts kept / tried */
+  int32_t warm_start_tried;
+  int32_t fp64_refreshes;
+  double res_norm0;
+  double res_norm;
+  double linear_rel_res;          /* last linear solve ||r||/||b|| (as the stop test measured it) */
+} sdx_step_stats;
+
+/* SoftActorParams subset (cpp_api/mochi_structs.h:174-197). */
+typedef struct sdx_soft_params {
+  double youngs_modulus;          /* homogeneous; per-element fields via sdx_set_element_materials */
+  double poisson_ratio;
+  double density;
+  double mass_damping;            /* massDampingCoefficient [1/s] */
+  /* ContactParams (contact/contact_params.h:96-160) */
+  double penalty_coefficient;     /* [Pa/m] */
+  double penalty_smoothing_half_distance; /* default 0.005 m */
+  double penalty_threshold;       /* default 0.001 m */
+  double max_alignment_normals;   /* default 0 */
+} sdx_soft_params;
+
+typedef struct sdx_world sdx_world;
+
+/* Fill a struct with the reference defaults. */
+void sdx_default_solver_params(sdx_solver_params* out);
+void sdx_default_soft_params(sdx_soft_params* out);
+const char* sdx_last_error(void);
+
+/* ---------------- L1: linear solve (CudaIterativeSolve seam) ---------------- */
+
+/* Solve A x = b with PCG (Polak-Ribiere beta, pcg.h:70-238) on device `device`.
+ * A: n_block_rows x n_block_rows, nnzb blocks, host arrays; block_row_ptr has
+ * n_block_rows+1 entries, block_col_idx nnzb entries, values 9*nnzb reals
+ * laid out as `layout`.  b, x: 3*n_block_rows doubles (x is the initial guess
+ * unless params->initial_guess_zero).  Replaces
+ * LinearSolver<T>::CudaIterativeSolve<3> (linear_solver.h:615-662). */
+sdx_status sdx_pcg_bsr3_solve(int32_t device, int32_t n_block_rows, int32_t nnzb,
+                              const int32_t* block_row_ptr, const int32_t* block_col_idx,
+                              const double* values, int32_t layout, const double* b, double* x,
+                              const sdx_krylov_params* params, sdx_linear_status* out);
+
+/* ---------------- L2: world / step (Scene::Step seam) ---------------- */
+
+/* Context/Scene creation (cpp_api/mochi_context.h, mochi_scene.h).
+ * precision_mode chooses the storage of the state (FP64 for SDX_PREC_MODE_FP64). */
+sdx_status sdx_world_create(int32_t device, int32_t precision_mode, sdx_world** out);
+sdx_status sdx_world_destroy(sdx_world* world);
+
+/* CreateTetMeshShape + CreateSoftActor (mochi_context.h:142, mochi_scene.h:99).
+ * X: 3*n_verts rest positions, tets: 4*n_tets node indices (positive volume). */
+sdx_status sdx_add_soft_tet_mesh(sdx_world* world, const double* X, int32_t n_verts,
+                                 const int32_t* tets, int32_t n_tets, const sdx_soft_params* params,
+                                 int32_t* out_actor);
+
+/* SetSoftMaterialParamsField: per-element Young's modulus and Poisson ratio. */
+sdx_status sdx_set_element_materials(sdx_world* world, int32_t actor, const double* E,
+                                     const double* nu, int32_t n_tets);
+
+/* AddBoundaryConditionNodesWorldPermanent (mochi_actor.h:202): pin all three
+ * DoFs of the nodes at their current displacement. */
+sdx_status sdx_pin_nodes(sdx_world* world, int32_t actor, const int32_t* nodes, int32_t n);
+
+/* CreateRigidActor with a plane / sphere shape (kinematic).  actor = -1: the
+ * collider touches every soft actor, else only that one.  plane: point p,
+ * unit normal n pointing to the free side.  sphere: centre c, radius r. */
+sdx_status sdx_add_rigid_plane(sdx_world* world, const double p[3], const double n[3],
+                               int32_t actor, int32_t* out_collider);
+sdx_status sdx_add_rigid_sphere(sdx_world* world, const double c[3], double r, int32_t actor,
+                                int32_t* out_collider);
+/* SetRootTransform translation: plane point or sphere centre. */
+sdx_status sdx_set_rigid_position(sdx_world* world, int32_t collider, const double pos[3]);
+
+/* Scene::Step(dt): one backward-Euler stage per actor, each actor its own
+ * island and Newton solve.  stats: n_actor entries (may be NULL). */
+sdx_status sdx_step(sdx_world* world, double dt, const sdx_solver_params* params,
+                    sdx_step_stats* stats, int32_t n_stats);
+
+/* State access (Actor::GetDofValues / SetNodeVelocities analogues): 3*n_verts doubles. */
+sdx_status sdx_get_displacements(sdx_world* world, int32_t actor, double* out, int32_t n);
+sdx_status sdx_set_displacements(sdx_world* world, int32_t actor, const double* in, int32_t n);
+sdx_status sdx_get_velocities(sdx_world* world, int32_t actor, double* out, int32_t n);
+sdx_status sdx_set_velocities(sdx_world* world, int32_t actor, const double* in, int32_t n);
+sdx_status sdx_get_positions_world(sdx_world* world, int32_t actor, double* out, int32_t n);
+
+/* World totals: counts over all actors (vertices, tets, BSR blocks, boundary triangles). */
+sdx_status sdx_world_info(sdx_world* world, int64_t out[8]);
+
+/* Assembly of one actor at its current displacement, copied to the host:
+ * residual (3*n_verts), BSR pattern and values in SDX_BSR_BLOCK_ROWMAJOR with
+ * Dirichlet rows/columns applied, as AssembleIslandPipeline leaves them
+ * (mochi_solve.cpp:595-771).  Any output pointer may be NULL.  The pattern
+ * sizes are given by sdx_world_info / sdx_actor_bsr_size. */
+sdx_status sdx_actor_bsr_size(sdx_world* world, int32_t actor, int32_t* n_block_rows,
+                              int32_t* nnzb);
+sdx_status sdx_assemble(sdx_world* world, int32_t actor, double dt, const sdx_solver_params* params,
+                        double* out_residual, int32_t* out_row_ptr, int32_t* out_col_idx,
+                        double* out_values);
+
+/* Timing of one kernel class inside sdx_step, measured with HIP events on the
+ * world's stream.  kernel: 0 = CG matvec (SpMV + p update + p.Ap),
+ * 1 = CG vector update, 2 = element Hessian, 3 = BSR gather.  enable 1 clears
+ * the accumulators.  out: {launches, total milliseconds}. */
+sdx_status sdx_kernel_timing(sdx_world* world, int32_t kernel, int32_t enable, double out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDX_FEM_H_ */

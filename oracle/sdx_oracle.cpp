@@ -1,0 +1,1219 @@
+/*
+ * sdx_oracle.cpp - CPU restatement of the reference soft-body FEM step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product (project_superdex_amd) never
+ * links, loads or calls it.
+ *
+ * It restates, function by function, the algorithm of the reference
+ * (paths relative to superdex_physics/libraries/mochi/):
+ *   SmithNeoHookean ............ mochi_core/src/materials/batched_smith_neo_hookean.cpp:29-269
+ *   Lame constants ............. mochi_core/include/mochi_core/materials/material_utils.h:331-339
+ *   P1 tet map / dBasis ........ mochi_core/include/mochi_core/elements/tetrahedral/finite_element.h:95-125
+ *   Q1 / Q4 / trace Q3 rules ... .../elements/tetrahedral/simplex_quadrature.h:33-42,117;
+ *                                .../elements/triangular/simplex_quadrature.h:36-43
+ *   StressWork ................. mochi_core/include/mochi_core/element_operations/fem_stress.h:107-262
+ *   InertiaWork, mass matrix ... .../element_operations/fem_inertia.h:55-130,199-250
+ *   GravityWork ................ .../element_operations/fem_gravity.h:44-125
+ *   predicted target ........... .../element_operations/batched_element_utils.h:90-120
+ *   body assembly, damping ..... mochi_physics/src/mochi_soft.cpp:202-318, mochi_soft.h:274-292
+ *   contact penalty ............ mochi_core/include/mochi_core/contact/contact_utils.h:498-570,
+ *                                .../utils/activations.h:113-128, mochi_core/src/contact/contact_utils.cpp:304-440,
+ *                                alignment mask contact_utils.h:1080-1095, TractionWork fem_traction.h:58-170
+ *   Dirichlet BCs .............. mochi_physics/src/mochi_solve.cpp:337-422
+ *   block Jacobi ............... mochi_core/include/mochi_core/linear_algebra/krylov/block_jacobi_prec.h:35-151
+ *   PCG + stop tests ........... .../linear_algebra/krylov/pcg.h:70-238, stopping_criterion.h:31-333
+ *   Newton + line search ....... mochi_core/src/solvers/newton_solver.cpp:603-876, line_search.h:300-335
+ *   backward-Euler stage ....... mochi_physics/src/mochi_soft.cpp:94-163
+ *
+ * Two deliberate differences of method (same mathematics):
+ *  - the PSD projection of the Smith tangent is done by a cyclic Jacobi
+ *    eigen-decomposition of the full 9x9 tangent in double precision, where
+ *    the reference uses the analytic eigensystem (rotation-variant SVD).  The
+ *    projected matrix is unique, so both give the same result; the HIP code
+ *    uses the analytic eigensystem, so the two are independent checks.
+ *  - dot products accumulate in 8 partial lanes of the working precision T,
+ *    chunked so that results do not depend on the thread count.
+ */
+#include "../include/sdx_fem.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace sdxo {
+
+// ------------------------------------------------------------------ utilities
+
+template <class T>
+static inline T Det3(const T F[9]) {
+  return F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+         F[2] * (F[3] * F[7] - F[4] * F[6]);
+}
+
+// cof[i][j] = dJ/dF_ij
+template <class T>
+static inline void Cofactor3(const T F[9], T cof[9]) {
+  cof[0] = F[4] * F[8] - F[5] * F[7];
+  cof[1] = F[5] * F[6] - F[3] * F[8];
+  cof[2] = F[3] * F[7] - F[4] * F[6];
+  cof[3] = F[2] * F[7] - F[1] * F[8];
+  cof[4] = F[0] * F[8] - F[2] * F[6];
+  cof[5] = F[1] * F[6] - F[0] * F[7];
+  cof[6] = F[1] * F[5] - F[2] * F[4];
+  cof[7] = F[2] * F[3] - F[0] * F[5];
+  cof[8] = F[0] * F[4] - F[1] * F[3];
+}
+
+static inline int Levi(int i, int j, int k) {
+  if (i == j || j == k || i == k) return 0;
+  return ((i + 1) % 3 == j) ? 1 : -1;
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major, double).
+static void JacobiEigen(int n, double* A, double* Q, double* lam) {
+  for (int i = 0; i < n * n; ++i) Q[i] = 0.0;
+  for (int i = 0; i < n; ++i) Q[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) (i == j ? diag : off) += A[i * n + j] * A[i * n + j];
+    if (off <= 1e-30 * (diag + 1e-300)) break;
+    for (int p = 0; p < n - 1; ++p) {
+      for (int q = p + 1; q < n; ++q) {
+        double apq = A[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        double app = A[p * n + p], aqq = A[q * n + q];
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          double qkp = Q[k * n + p], qkq = Q[k * n + q];
+          Q[k * n + p] = c * qkp - s * qkq;
+          Q[k * n + q] = s * qkp + c * qkq;
+        }
+      }
+    }
+  }
+  for (int i = 0; i < n; ++i) lam[i] = A[i * n + i];
+}
+
+// ------------------------------------------------------------------ material
+
+// Smith et al. 2018 stable Neo-Hookean, batched_smith_neo_hookean.cpp:29-269.
+// C[(i*3+j)*9 + k*3+l] = dP_ij / dF_kl.
+template <class T>
+void SmithNH(T mu, T lambda, const T F[9], double* energy, T* P, T* C, bool projectPsd) {
+  const T one = T(1);
+  const T muHat = mu * T(4.0 / 3.0);
+  const T lambdaHat = lambda + mu * T(5.0 / 6.0);
+  const T alpha = one + T(3.0 / 4.0) * muHat / lambdaHat;
+  T Ic = 0;
+  for (int i = 0; i < 9; ++i) Ic += F[i] * F[i];
+  const T IcPlus1 = Ic + one;
+  const T IcPlus1Inv = one / IcPlus1;
+  const T J = Det3(F);
+  const T Jma = J - alpha;
+  if (energy) {
+    *energy = double(T(0.5) * muHat * (Ic - T(3)) + T(0.5) * lambdaHat * Jma * Jma -
+                     T(0.5) * muHat * std::log(IcPlus1));
+  }
+  T cof[9];
+  Cofactor3(F, cof);
+  if (P) {
+    const T c0 = muHat * (one - IcPlus1Inv);
+    const T c1 = lambdaHat * Jma;
+    for (int i = 0; i < 9; ++i) P[i] = c0 * F[i] + c1 * cof[i];
+  }
+  if (!C) return;
+  const T c2 = muHat * (one - IcPlus1Inv);
+  const T c3 = T(2) * muHat * IcPlus1Inv * IcPlus1Inv;
+  const T d2J = lambdaHat * Jma;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) {
+          T hJ = 0;
+          for (int m = 0; m < 3; ++m)
+            for (int n = 0; n < 3; ++n) {
+              int e = Levi(i, k, m) * Levi(j, l, n);
+              if (e) hJ += T(e) * F[m * 3 + n];
+            }
+          T v = c3 * F[i * 3 + j] * F[k * 3 + l] + lambdaHat * cof[i * 3 + j] * cof[k * 3 + l] +
+                d2J * hJ;
+          if (i == k && j == l) v += c2;
+          C[(i * 3 + j) * 9 + k * 3 + l] = v;
+        }
+  if (!projectPsd) return;
+  // PSD oracle "Correct" (:99-106): indefinite iff
+  //   lambdaHat^2 (J-alpha)^2 max_i sigma_i^2 > (muHat (1 - 1/(Ic+1)))^2  or  J < 0.
+  double FtF[9], Q3[9], l3[3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double s = 0;
+      for (int r = 0; r < 3; ++r) s += double(F[r * 3 + a]) * double(F[r * 3 + b]);
+      FtF[a * 3 + b] = s;
+    }
+  JacobiEigen(3, FtF, Q3, l3);
+  const T maxSigmaSq = T(std::max(l3[0], std::max(l3[1], l3[2])));
+  const T muHatKSqr = (muHat * (one - IcPlus1Inv)) * (muHat * (one - IcPlus1Inv));
+  const T lhsBase = (lambdaHat * Jma) * (lambdaHat * Jma);
+  const bool indefinite = (lhsBase * maxSigmaSq > muHatKSqr) || (J < T(0));
+  if (!indefinite) return;
+  // Projection: clamp the eigenvalues at kMinProjectedEigenvalue = eps(real)
+  // (material_params_utils.h:66), batched_smith_neo_hookean.cpp:192-196.
+  double A[81], Q[81], lam[9];
+  for (int i = 0; i < 81; ++i) A[i] = double(C[i]);
+  for (int i = 0; i < 9; ++i)
+    for (int j = i + 1; j < 9; ++j) A[i * 9 + j] = A[j * 9 + i] = 0.5 * (A[i * 9 + j] + A[j * 9 + i]);
+  JacobiEigen(9, A, Q, lam);
+  const double eps = double(std::numeric_limits<T>::epsilon());
+  for (int i = 0; i < 9; ++i) lam[i] = std::max(lam[i], eps);
+  for (int i = 0; i < 9; ++i)
+    for (int j = 0; j < 9; ++j) {
+      double s = 0;
+      for (int k = 0; k < 9; ++k) s += Q[i * 9 + k] * lam[k] * Q[j * 9 + k];
+      C[i * 9 + j] = T(s);
+    }
+}
+
+// PolyReLU, activations.h:113-128.
+template <class T>
+void PolyReLU(T x, T t, T shift, T& f, T& df, T& ddf) {
+  x -= shift;
+  const T xt = x / t, x2t2 = xt * xt, x3t3 = x2t2 * xt;
+  const bool mask = (t <= T(0)) || (std::fabs(x) > t);
+  T fv = mask ? x : T(3.0 / 16.0) * t + (T(0.5) + T(3.0 / 8.0) * xt - T(1.0 / 16.0) * x3t3) * x;
+  f = std::max(T(0), fv);
+  T dfv = mask ? (x < T(0) ? T(0) : T(1)) : T(0.5) + T(0.75) * xt - T(0.25) * x3t3;
+  df = std::min(T(1), std::max(T(0), dfv));
+  ddf = std::max(T(0), mask ? T(0) : T(0.75) * (T(1) - x2t2) / t);
+}
+
+// ------------------------------------------------------------------ element geometry
+
+template <class T>
+struct TetGeom {
+  T dN[4][3];  // physical basis gradients
+  T vol;       // Q1 quad weight = det(dMap)/6
+};
+
+template <class T>
+static TetGeom<T> MakeTetGeom(const T* X, const int32_t* tet) {
+  // dMap columns = X_f - X_0 (finite_element.h:95-110), dBasis = dMap^{-T} dN_param.
+  T D[9];
+  for (int d = 0; d < 3; ++d)
+    for (int f = 0; f < 3; ++f) D[d * 3 + f] = X[3 * tet[f + 1] + d] - X[3 * tet[0] + d];
+  const T det = Det3(D);
+  T cof[9];
+  Cofactor3(D, cof);  // D^{-T} = cof / det
+  TetGeom<T> g;
+  static const T dNp[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int f = 0; f < 4; ++f)
+    for (int r = 0; r < 3; ++r) {
+      T s = 0;
+      for (int c = 0; c < 3; ++c) s += cof[r * 3 + c] * dNp[f][c];
+      g.dN[f][r] = s / det;
+    }
+  g.vol = det / T(6);
+  return g;
+}
+
+// Q4 points and weights (simplex_quadrature.h:37-42), barycentric N for each.
+static const double kQ4a = 0.58541019662496852, kQ4b = 0.13819660112501051;
+
+template <class T>
+static void Q4Basis(int q, T N[4]) {
+  T p[3] = {T(kQ4b), T(kQ4b), T(kQ4b)};
+  if (q < 3) p[q] = T(kQ4a);
+  N[0] = T(1) - p[0] - p[1] - p[2];
+  N[1] = p[0];
+  N[2] = p[1];
+  N[3] = p[2];
+}
+
+// Stress residual (12) and Hessian (144) of one tet, fem_stress.h:148-259.
+template <class T>
+static void StressElement(const TetGeom<T>& g, const T u[12], T mu, T lambda, bool wantH,
+                          bool projectPsd, T res[12], T H[144]) {
+  T F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int f = 0; f < 4; ++f)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) F[r * 3 + c] += u[f * 3 + r] * g.dN[f][c];
+  T P[9], C[81];
+  SmithNH<T>(mu, lambda, F, nullptr, P, wantH ? C : nullptr, projectPsd);
+  for (int f = 0; f < 4; ++f)
+    for (int r = 0; r < 3; ++r) {
+      res[f * 3 + r] += P[r * 3 + 0] * (g.dN[f][0] * g.vol) + P[r * 3 + 1] * (g.dN[f][1] * g.vol) +
+                        P[r * 3 + 2] * (g.dN[f][2] * g.vol);
+    }
+  if (!wantH) return;
+  for (int f = 0; f < 4; ++f)
+    for (int r = 0; r < 3; ++r) {
+      T Cf[3][3];
+      for (int rr = 0; rr < 3; ++rr)
+        for (int cc = 0; cc < 3; ++cc) {
+          T s = 0;
+          for (int a = 0; a < 3; ++a) s += (g.dN[f][a] * g.vol) * C[(r * 3 + a) * 9 + rr * 3 + cc];
+          Cf[rr][cc] = s;
+        }
+      for (int gg = f; gg < 4; ++gg)
+        for (int c = 0; c < 3; ++c) {
+          T s = Cf[c][0] * g.dN[gg][0] + Cf[c][1] * g.dN[gg][1] + Cf[c][2] * g.dN[gg][2];
+          H[(f * 3 + r) * 12 + gg * 3 + c] += s;
+        }
+    }
+  for (int f = 0; f < 4; ++f)
+    for (int gg = f + 1; gg < 4; ++gg)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) H[(gg * 3 + c) * 12 + f * 3 + r] = H[(f * 3 + r) * 12 + gg * 3 + c];
+}
+
+// ------------------------------------------------------------------ BSR
+
+template <class T>
+struct Bsr {
+  int nb = 0;
+  std::vector<int32_t> ptr, col;
+  std::vector<T> val;  // 9 per block, row-major 3x3
+  std::vector<int32_t> diag;  // slot of (i,i)
+  int Slot(int i, int j) const {
+    auto b = col.begin() + ptr[i], e = col.begin() + ptr[i + 1];
+    auto it = std::lower_bound(b, e, j);
+    return int(it - col.begin());
+  }
+};
+
+// Deterministic dot: fixed chunks of 4096 entries, 8 lanes per chunk.
+template <class T>
+static T Dot(const std::vector<T>& a, const std::vector<T>& b) {
+  const int64_t n = int64_t(a.size());
+  const int64_t kChunk = 4096;
+  const int64_t nc = (n + kChunk - 1) / kChunk;
+  std::vector<T> part(size_t(std::max<int64_t>(nc, 1)), T(0));
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < nc; ++c) {
+    T lanes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t e = std::min(n, (c + 1) * kChunk);
+    for (int64_t i = c * kChunk; i < e; ++i) lanes[i & 7] += a[size_t(i)] * b[size_t(i)];
+    part[size_t(c)] = ((lanes[0] + lanes[4]) + (lanes[1] + lanes[5])) +
+                      ((lanes[2] + lanes[6]) + (lanes[3] + lanes[7]));
+  }
+  T s = 0;
+  for (auto v : part) s += v;
+  return s;
+}
+
+template <class T>
+static void SpMV(const Bsr<T>& A, const std::vector<T>& x, std::vector<T>& y) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < A.nb; ++i) {
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+      const T* v = &A.val[size_t(9) * k];
+      const T* xj = &x[size_t(3) * A.col[k]];
+      s0 += v[0] * xj[0] + v[1] * xj[1] + v[2] * xj[2];
+      s1 += v[3] * xj[0] + v[4] * xj[1] + v[5] * xj[2];
+      s2 += v[6] * xj[0] + v[7] * xj[1] + v[8] * xj[2];
+    }
+    y[3 * i] = s0;
+    y[3 * i + 1] = s1;
+    y[3 * i + 2] = s2;
+  }
+}
+
+// Symmetric 3x3 inverse by cofactors (BatchedInverse<kIsSymmetric>, matrix_operations.h:1540).
+template <class T>
+static void Inverse3(const T* a, T* out) {
+  T cof[9];
+  Cofactor3(a, cof);
+  T det = a[0] * cof[0] + a[1] * cof[1] + a[2] * cof[2];
+  T inv = T(1) / det;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) out[r * 3 + c] = cof[c * 3 + r] * inv;
+}
+
+// ------------------------------------------------------------------ PCG
+
+// Generic linear operator interface for PCG: y = A x.
+template <class T>
+struct LinOp {
+  virtual ~LinOp() = default;
+  virtual void Apply(const std::vector<T>& x, std::vector<T>& y) const = 0;
+  virtual int N() const = 0;
+};
+
+template <class T>
+struct BsrOp : LinOp<T> {
+  const Bsr<T>* A;
+  explicit BsrOp(const Bsr<T>* a) : A(a) {}
+  void Apply(const std::vector<T>& x, std::vector<T>& y) const override { SpMV(*A, x, y); }
+  int N() const override { return 3 * A->nb; }
+};
+
+template <class T>
+struct CsrOp : LinOp<T> {
+  int n;
+  const int32_t* ptr;
+  const int32_t* col;
+  std::vector<T> val;
+  void Apply(const std::vector<T>& x, std::vector<T>& y) const override {
+    for (int i = 0; i < n; ++i) {
+      T s = 0;
+      for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[size_t(k)] * x[size_t(col[k])];
+      y[size_t(i)] = s;
+    }
+  }
+  int N() const override { return n; }
+};
+
+// Block-diagonal preconditioner with block size bs (1 = Jacobi, 3 = block Jacobi).
+template <class T>
+struct Prec {
+  int bs = 0;  // 0: identity
+  std::vector<T> inv;  // bs*bs per block
+  void Solve(const std::vector<T>& r, std::vector<T>& z) const {
+    const size_t n = r.size();
+    if (bs == 0) {
+      z = r;
+    } else if (bs == 1) {
+      for (size_t i = 0; i < n; ++i) z[i] = inv[i] * r[i];
+    } else {
+#pragma omp parallel for schedule(static)
+      for (int64_t b = 0; b < int64_t(n / 3); ++b) {
+        const T* m = &inv[size_t(9) * b];
+        const T* x = &r[size_t(3) * b];
+        T* y = &z[size_t(3) * b];
+        y[0] = m[0] * x[0] + m[1] * x[1] + m[2] * x[2];
+        y[1] = m[3] * x[0] + m[4] * x[1] + m[5] * x[2];
+        y[2] = m[6] * x[0] + m[7] * x[1] + m[8] * x[2];
+      }
+    }
+  }
+};
+
+struct LinStatus {
+  int iters = 0;
+  int conv = SDX_CONV_NONE;
+  double resNorm = 0, relResNorm = 0;
+};
+
+// StopTest (stopping_criterion.h:31-101).
+template <class T>
+struct StopTest {
+  T absTol, relTol, relDivTol;
+  T start = 0, resSq = 0, conv = 0, div = 0;
+  void SetScaling(T s) {
+    start = s;
+    div = (start * relDivTol) * relDivTol;
+    conv = std::max(absTol * absTol, (start * relTol) * relTol);
+  }
+  // returns 0 active, 1 converged, 3 diverged
+  int Check(T latest) {
+    resSq = latest;
+    if (!std::isfinite(double(resSq)) || resSq > div || resSq < -conv) return 3;
+    if (resSq < T(0)) resSq = T(0);
+    if (resSq <= conv) return 1;
+    return 0;
+  }
+  double Norm() const { return std::sqrt(double(resSq)); }
+  double Rel() const { return start == T(0) ? 0.0 : std::sqrt(double(resSq / start)); }
+};
+
+// krylov::PCG (pcg.h:70-238) with usePolakRibiere = true.
+template <class T>
+LinStatus PCG(const LinOp<T>& A, const std::vector<T>& b, std::vector<T>& x, const Prec<T>& M,
+              const sdx_krylov_params& p) {
+  const size_t n = b.size();
+  std::vector<T> r = b, Ap(n, T(0)), pv(n, T(0)), z(n, T(0));
+  StopTest<T> st{T(p.abs_tol), T(p.rel_tol), T(p.rel_div_tol)};
+  const int norm = p.norm_type;
+  const bool needPrec = norm != SDX_NORM_RESIDUAL_L2;
+  // SetScaling
+  if (norm == SDX_NORM_RESIDUAL_L2) {
+    st.SetScaling(Dot(b, b));
+  } else {
+    M.Solve(b, z);
+    st.SetScaling(norm == SDX_NORM_PRECONDITIONED_RESIDUAL_L2 ? Dot(z, z) : Dot(b, z));
+  }
+  const bool zeroGuess = p.initial_guess_zero != 0;
+  if (zeroGuess) {
+    std::fill(x.begin(), x.end(), T(0));
+  } else {
+    A.Apply(x, Ap);
+    for (size_t i = 0; i < n; ++i) r[i] -= Ap[i];
+  }
+  auto check = [&](void) -> int {
+    T v = norm == SDX_NORM_RESIDUAL_L2 ? Dot(r, r)
+          : norm == SDX_NORM_PRECONDITIONED_RESIDUAL_L2 ? Dot(z, z) : Dot(r, z);
+    return st.Check(v);
+  };
+  int status;
+  if (needPrec) {
+    if (!zeroGuess) M.Solve(r, z);
+    status = check();
+  } else {
+    status = check();
+    if (status == 0) M.Solve(r, z);
+  }
+  LinStatus out;
+  auto finish = [&](int iters, int s) {
+    out.iters = iters;
+    out.resNorm = st.Norm();
+    out.relResNorm = st.Rel();
+    out.conv = s == 1 ? SDX_CONV_CONVERGED : (s == 3 ? SDX_CONV_DIVERGED : SDX_CONV_STOPPED);
+    return out;
+  };
+  if (status != 0) return finish(0, status);
+  pv = z;
+  T rTz = Dot(r, z);
+  for (int iter = 1; iter <= p.max_iter; ++iter) {
+    A.Apply(pv, Ap);
+    const T pAp = Dot(pv, Ap);
+    if (pAp <= T(0) && p.abort_if_not_spd) return finish(iter, 3);
+    const T alpha = rTz / pAp;
+    for (size_t i = 0; i < n; ++i) {
+      x[i] += alpha * pv[i];
+      r[i] -= alpha * Ap[i];
+    }
+    T beta = 0;
+    if (needPrec) {
+      if (iter > 1) beta = Dot(r, z);
+      M.Solve(r, z);
+      status = check();
+    } else {
+      status = check();
+      if (status == 0) {
+        if (iter > 1) beta = Dot(r, z);
+        M.Solve(r, z);
+      }
+    }
+    if (status != 0) return finish(iter, status);
+    const T rTzOld = rTz;
+    rTz = Dot(r, z);
+    if (rTz == T(0)) return finish(iter, 3);
+    beta = (rTz - beta) / rTzOld;
+    for (size_t i = 0; i < n; ++i) pv[i] = z[i] + beta * pv[i];
+  }
+  out.iters = p.max_iter;
+  out.resNorm = st.Norm();
+  out.relResNorm = st.Rel();
+  out.conv = SDX_CONV_STOPPED;
+  return out;
+}
+
+// ------------------------------------------------------------------ scene
+
+struct Collider {
+  int kind;  // 0 plane, 1 sphere
+  double p[3], n[3], r;
+  int actor;
+};
+
+template <class T>
+struct Actor {
+  int nV = 0, nT = 0;
+  std::vector<T> X;
+  std::vector<int32_t> tets;
+  std::vector<TetGeom<T>> geom;
+  std::vector<T> mu, lambda;  // per tet
+  T density = 1000, massDamping = 0;
+  T kPen = 1e6, hPen = 0.005, thrPen = 0.001, maxAlign = 0;
+  std::vector<uint8_t> pinned;  // per DoF
+  std::vector<T> pinValue;      // per DoF
+  std::vector<T> u, v;
+  Bsr<T> M;       // consistent mass (Q4), same pattern as the stiffness
+  Bsr<T> K;       // dres
+  std::vector<int32_t> tetSlots;  // 16 per tet: slot of (a,b)
+  // boundary triangles (outward) and rest area
+  std::vector<int32_t> tris;
+  std::vector<T> triArea;
+};
+
+template <class T>
+struct Scene {
+  std::vector<Actor<T>> actors;
+  std::vector<Collider> colliders;
+
+  int AddActor(const double* X, int nV, const int32_t* tets, int nT, const sdx_soft_params& sp) {
+    Actor<T> a;
+    a.nV = nV;
+    a.nT = nT;
+    a.X.assign(X, X + 3 * size_t(nV));
+    a.tets.assign(tets, tets + 4 * size_t(nT));
+    a.geom.resize(size_t(nT));
+    for (int t = 0; t < nT; ++t) a.geom[size_t(t)] = MakeTetGeom<T>(a.X.data(), &a.tets[4 * size_t(t)]);
+    double lam = sp.youngs_modulus * sp.poisson_ratio /
+                 ((1.0 + sp.poisson_ratio) * (1.0 - 2.0 * sp.poisson_ratio));
+    double mu = sp.youngs_modulus / (2.0 * (1.0 + sp.poisson_ratio));
+    a.mu.assign(size_t(nT), T(mu));
+    a.lambda.assign(size_t(nT), T(lam));
+    a.density = T(sp.density);
+    a.massDamping = T(sp.mass_damping);
+    a.kPen = T(sp.penalty_coefficient);
+    a.hPen = T(sp.penalty_smoothing_half_distance);
+    a.thrPen = T(sp.penalty_threshold);
+    a.maxAlign = T(sp.max_alignment_normals);
+    a.pinned.assign(3 * size_t(nV), 0);
+    a.pinValue.assign(3 * size_t(nV), T(0));
+    a.u.assign(3 * size_t(nV), T(0));
+    a.v.assign(3 * size_t(nV), T(0));
+    BuildPattern(a);
+    BuildMass(a);
+    BuildBoundary(a);
+    actors.push_back(std::move(a));
+    return int(actors.size()) - 1;
+  }
+
+  static void BuildPattern(Actor<T>& a) {
+    std::vector<std::vector<int32_t>> adj(size_t(a.nV));
+    for (int t = 0; t < a.nT; ++t)
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) adj[size_t(a.tets[4 * t + i])].push_back(a.tets[4 * t + j]);
+    Bsr<T>& K = a.K;
+    K.nb = a.nV;
+    K.ptr.assign(size_t(a.nV) + 1, 0);
+    for (int i = 0; i < a.nV; ++i) {
+      auto& v = adj[size_t(i)];
+      v.push_back(i);
+      std::sort(v.begin(), v.end());
+      v.erase(std::unique(v.begin(), v.end()), v.end());
+      K.ptr[size_t(i) + 1] = K.ptr[size_t(i)] + int32_t(v.size());
+    }
+    K.col.resize(size_t(K.ptr.back()));
+    for (int i = 0; i < a.nV; ++i)
+      std::copy(adj[size_t(i)].begin(), adj[size_t(i)].end(), K.col.begin() + K.ptr[size_t(i)]);
+    K.val.assign(9 * K.col.size(), T(0));
+    K.diag.resize(size_t(a.nV));
+    for (int i = 0; i < a.nV; ++i) K.diag[size_t(i)] = K.Slot(i, i);
+    a.tetSlots.resize(16 * size_t(a.nT));
+    for (int t = 0; t < a.nT; ++t)
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+          a.tetSlots[16 * size_t(t) + 4 * i + j] = K.Slot(a.tets[4 * t + i], a.tets[4 * t + j]);
+    a.M = K;
+  }
+
+  // ComputeMassMatrixPerElement (fem_inertia.h:199-250) with the Q4 rule, assembled.
+  static void BuildMass(Actor<T>& a) {
+    std::fill(a.M.val.begin(), a.M.val.end(), T(0));
+    for (int t = 0; t < a.nT; ++t) {
+      const auto& g = a.geom[size_t(t)];
+      T mm[4][4] = {};
+      for (int q = 0; q < 4; ++q) {
+        T N[4];
+        Q4Basis<T>(q, N);
+        const T w = g.vol * T(1.0 / 24.0) * T(6);  // weight 1/24 times det(dMap) = 6 vol
+        for (int f = 0; f < 4; ++f)
+          for (int gg = f; gg < 4; ++gg) mm[f][gg] += a.density * N[f] * N[gg] * w;
+      }
+      for (int f = 0; f < 4; ++f)
+        for (int gg = 0; gg < f; ++gg) mm[f][gg] = mm[gg][f];
+      for (int f = 0; f < 4; ++f)
+        for (int gg = 0; gg < 4; ++gg) {
+          T* blk = &a.M.val[9 * size_t(a.tetSlots[16 * size_t(t) + 4 * f + gg])];
+          for (int d = 0; d < 3; ++d) blk[d * 4] += mm[f][gg];
+        }
+    }
+  }
+
+  static void BuildBoundary(Actor<T>& a) {
+    struct Face {
+      int32_t k[3];
+      int32_t n[3];
+      int32_t opp;
+    };
+    std::vector<Face> faces;
+    faces.reserve(4 * size_t(a.nT));
+    static const int fv[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    for (int t = 0; t < a.nT; ++t)
+      for (int f = 0; f < 4; ++f) {
+        Face F;
+        for (int i = 0; i < 3; ++i) F.n[i] = a.tets[4 * t + fv[f][i]];
+        F.opp = a.tets[4 * t + f];
+        std::copy(F.n, F.n + 3, F.k);
+        std::sort(F.k, F.k + 3);
+        faces.push_back(F);
+      }
+    std::sort(faces.begin(), faces.end(), [](const Face& x, const Face& y) {
+      return std::lexicographical_compare(x.k, x.k + 3, y.k, y.k + 3);
+    });
+    for (size_t i = 0; i < faces.size();) {
+      size_t j = i;
+      while (j < faces.size() && std::equal(faces[i].k, faces[i].k + 3, faces[j].k)) ++j;
+      if (j - i == 1) {
+        Face F = faces[i];
+        const T* X = a.X.data();
+        T e1[3], e2[3], nn[3], w[3];
+        for (int d = 0; d < 3; ++d) {
+          e1[d] = X[3 * F.n[1] + d] - X[3 * F.n[0] + d];
+          e2[d] = X[3 * F.n[2] + d] - X[3 * F.n[0] + d];
+          w[d] = X[3 * F.opp + d] - X[3 * F.n[0] + d];
+        }
+        nn[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        nn[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        nn[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        if (nn[0] * w[0] + nn[1] * w[1] + nn[2] * w[2] > T(0)) std::swap(F.n[1], F.n[2]);
+        for (int d = 0; d < 3; ++d) a.tris.push_back(F.n[d]);
+        a.triArea.push_back(T(0.5) * std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]));
+      }
+      i = j;
+    }
+  }
+
+  // AssembleBodyImpl + AssembleAsyncContact + ApplyBoundaryConditionsToProblem for one actor.
+  // u: current displacement; u0/v0: stage start; collider positions as stored.
+  void Assemble(Actor<T>& a, int ai, const std::vector<T>& u, const std::vector<T>& u0,
+                const std::vector<T>& v0, T dt, const T grav[3], bool wantH, bool projectPsd,
+                std::vector<T>& res) {
+    const T dtfi2 = T(1) / (dt * dt);
+    const T massDampingScale = a.massDamping > T(0) ? a.massDamping / dt : T(0);
+    const T massScale = dtfi2 + massDampingScale;
+    std::fill(res.begin(), res.end(), T(0));
+    if (wantH)
+      for (size_t i = 0; i < a.K.val.size(); ++i) a.K.val[i] = massScale * a.M.val[i];
+    std::vector<T> eres(12 * size_t(a.nT));
+    std::vector<T> eH(wantH ? 144 * size_t(a.nT) : 0);
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < a.nT; ++t) {
+      const auto& g = a.geom[size_t(t)];
+      const int32_t* tv = &a.tets[4 * size_t(t)];
+      T ue[12], tgt[12], st[12];
+      for (int f = 0; f < 4; ++f)
+        for (int d = 0; d < 3; ++d) {
+          ue[3 * f + d] = u[3 * size_t(tv[f]) + d];
+          tgt[3 * f + d] = u0[3 * size_t(tv[f]) + d] + dt * v0[3 * size_t(tv[f]) + d];
+          st[3 * f + d] = u0[3 * size_t(tv[f]) + d];
+        }
+      T* r = &eres[12 * size_t(t)];
+      T* H = wantH ? &eH[144 * size_t(t)] : nullptr;
+      for (int i = 0; i < 12; ++i) r[i] = 0;
+      if (H)
+        for (int i = 0; i < 144; ++i) H[i] = 0;
+      StressElement<T>(g, ue, a.mu[size_t(t)], a.lambda[size_t(t)], wantH, projectPsd, r, H);
+      // GravityWork, Q1: res_f -= rho g N_f(centroid) w
+      for (int f = 0; f < 4; ++f)
+        for (int d = 0; d < 3; ++d) r[3 * f + d] -= (grav[d] * a.density) * (T(0.25) * g.vol);
+      // InertiaWork with Q4 (and mass damping with stage-start target, mochi_soft.h:400-420)
+      for (int q = 0; q < 4; ++q) {
+        T N[4];
+        Q4Basis<T>(q, N);
+        const T w = g.vol * T(0.25);
+        T qv[3] = {0, 0, 0}, qd[3] = {0, 0, 0};
+        for (int f = 0; f < 4; ++f)
+          for (int d = 0; d < 3; ++d) {
+            qv[d] += N[f] * (ue[3 * f + d] - tgt[3 * f + d]);
+            qd[d] += N[f] * (ue[3 * f + d] - st[3 * f + d]);
+          }
+        const T coeff = dtfi2 * a.density;
+        for (int f = 0; f < 4; ++f) {
+          const T wb = coeff * N[f] * w;
+          for (int d = 0; d < 3; ++d) r[3 * f + d] += wb * qv[d];
+        }
+        if (massDampingScale > T(0)) {
+          const T coeffD = massDampingScale * a.density;
+          for (int f = 0; f < 4; ++f) {
+            const T wb = coeffD * N[f] * w;
+            for (int d = 0; d < 3; ++d) r[3 * f + d] += wb * qd[d];
+          }
+        }
+      }
+    }
+    // scatter in element order (deterministic)
+    for (int t = 0; t < a.nT; ++t) {
+      const int32_t* tv = &a.tets[4 * size_t(t)];
+      for (int f = 0; f < 4; ++f)
+        for (int d = 0; d < 3; ++d) res[3 * size_t(tv[f]) + d] += eres[12 * size_t(t) + 3 * f + d];
+      if (wantH) {
+        const T* H = &eH[144 * size_t(t)];
+        for (int f = 0; f < 4; ++f)
+          for (int gg = 0; gg < 4; ++gg) {
+            T* blk = &a.K.val[9 * size_t(a.tetSlots[16 * size_t(t) + 4 * f + gg])];
+            for (int r = 0; r < 3; ++r)
+              for (int c = 0; c < 3; ++c) blk[r * 3 + c] += H[(f * 3 + r) * 12 + gg * 3 + c];
+          }
+      }
+    }
+    AssembleContact(a, ai, u, wantH, res);
+    ApplyBc(a, wantH, res);
+  }
+
+  void AssembleContact(Actor<T>& a, int ai, const std::vector<T>& u, bool wantH, std::vector<T>& res) {
+    static const T kBary[3][3] = {{T(1.0 / 6.0), T(2.0 / 3.0), T(1.0 / 6.0)},
+                                  {T(1.0 / 6.0), T(1.0 / 6.0), T(2.0 / 3.0)},
+                                  {T(2.0 / 3.0), T(1.0 / 6.0), T(1.0 / 6.0)}};
+    const int nTri = int(a.triArea.size());
+    for (size_t c = 0; c < colliders.size(); ++c) {
+      const Collider& col = colliders[c];
+      if (col.actor >= 0 && col.actor != ai) continue;
+      for (int tr = 0; tr < nTri; ++tr) {
+        const int32_t* nd = &a.tris[3 * size_t(tr)];
+        T xs[3][3];
+        for (int i = 0; i < 3; ++i)
+          for (int d = 0; d < 3; ++d) xs[i][d] = a.X[3 * size_t(nd[i]) + d] + u[3 * size_t(nd[i]) + d];
+        // current outward face normal (SetupActiveCollisionNormals, mochi_deformable.cpp:160-176)
+        T e1[3], e2[3], fn[3];
+        for (int d = 0; d < 3; ++d) {
+          e1[d] = xs[1][d] - xs[0][d];
+          e2[d] = xs[2][d] - xs[0][d];
+        }
+        fn[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        fn[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        fn[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        T fl = std::sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+        for (int d = 0; d < 3; ++d) fn[d] /= fl;
+        const T w = a.triArea[size_t(tr)] / T(3);
+        for (int q = 0; q < 3; ++q) {
+          T x[3] = {0, 0, 0};
+          for (int i = 0; i < 3; ++i)
+            for (int d = 0; d < 3; ++d) x[d] += kBary[q][i] * xs[i][d];
+          T dist, grad[3];
+          if (col.kind == 0) {
+            dist = 0;
+            for (int d = 0; d < 3; ++d) {
+              grad[d] = T(col.n[d]);
+              dist += grad[d] * (x[d] - T(col.p[d]));
+            }
+          } else {
+            T dv[3], l = 0;
+            for (int d = 0; d < 3; ++d) {
+              dv[d] = x[d] - T(col.p[d]);
+              l += dv[d] * dv[d];
+            }
+            l = std::sqrt(l);
+            for (int d = 0; d < 3; ++d) grad[d] = dv[d] / l;
+            dist = l - T(col.r);
+          }
+          // detection tolerance = penalty threshold (mochi_contact.cpp:950-953)
+          if (!(dist <= a.thrPen)) continue;
+          const T align = grad[0] * fn[0] + grad[1] * fn[1] + grad[2] * fn[2];
+          if (!(align <= a.maxAlign)) continue;
+          T pen, dpen, ddpen;
+          PolyReLU<T>(-dist, a.hPen, a.hPen - a.thrPen, pen, dpen, ddpen);
+          const T fnorm = a.kPen * pen * dpen;
+          const T dfn = -a.kPen * (dpen * dpen + pen * ddpen);
+          for (int i = 0; i < 3; ++i) {
+            const T bs = kBary[q][i] * w;
+            for (int d = 0; d < 3; ++d) res[3 * size_t(nd[i]) + d] -= fnorm * grad[d] * bs;
+          }
+          if (!wantH) continue;
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+              T* blk = &a.K.val[9 * size_t(a.K.Slot(nd[i], nd[j]))];
+              const T bb = kBary[q][i] * w * kBary[q][j];
+              for (int r = 0; r < 3; ++r)
+                for (int cc = 0; cc < 3; ++cc) blk[r * 3 + cc] -= dfn * grad[r] * grad[cc] * bb;
+            }
+        }
+      }
+    }
+  }
+
+  static void ApplyBc(Actor<T>& a, bool wantH, std::vector<T>& res) {
+    for (int i = 0; i < 3 * a.nV; ++i)
+      if (a.pinned[size_t(i)]) res[size_t(i)] = T(0);
+    if (!wantH) return;
+    for (int i = 0; i < a.nV; ++i)
+      for (int k = a.K.ptr[size_t(i)]; k < a.K.ptr[size_t(i) + 1]; ++k) {
+        int j = a.K.col[size_t(k)];
+        T* blk = &a.K.val[9 * size_t(k)];
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            if (a.pinned[3 * size_t(i) + r] || a.pinned[3 * size_t(j) + c]) {
+              blk[r * 3 + c] = (i == j && r == c) ? T(1) : T(0);
+            }
+          }
+      }
+  }
+
+  static T Norm(const std::vector<T>& v) { return std::sqrt(Dot(v, v)); }
+
+  // NewtonSolver<T>::Solve (newton_solver.cpp:603-876) for one actor's island.
+  void NewtonSolve(Actor<T>& a, int ai, const std::vector<T>& u0, const std::vector<T>& v0, T dt,
+                   const sdx_solver_params& sp, sdx_step_stats& stats) {
+    const T grav[3] = {T(sp.gravity[0]), T(sp.gravity[1]), T(sp.gravity[2])};
+    const size_t n = 3 * size_t(a.nV);
+    std::vector<T> res(n), dx(n, T(0)), sol0;
+    std::vector<T>& x = a.u;
+    const bool projectPsd = sp.psd_proj_mode == SDX_PSD_ALWAYS;
+    int numIterDone = 0;
+    T resNorm = 0, resNorm0 = 0;
+    double dxNorm = std::numeric_limits<double>::infinity();
+    bool haveDx = false;
+    stats = sdx_step_stats{};
+    while (true) {
+      const bool first = numIterDone == 0;
+      Assemble(a, ai, x, u0, v0, dt, grav, first, projectPsd, res);
+      resNorm = Norm(res);
+      if (first) resNorm0 = resNorm;
+      // stop criteria: ResNorm, MaxIters, (MaxElapsedTime off), RelStepNorm
+      const bool absConv = resNorm <= T(sp.newton_abs_tol);
+      const bool relConv = numIterDone > 0 && resNorm <= T(sp.newton_rel_tol) * resNorm0;
+      if (absConv || relConv) {
+        stats.convergence = SDX_CONV_CONVERGED;
+        break;
+      }
+      if (numIterDone >= sp.newton_max_iter) {
+        stats.convergence = SDX_CONV_STOPPED;
+        break;
+      }
+      if (numIterDone > 0 && haveDx && dxNorm <= double(T(sp.rel_step_tol)) * double(Norm(x))) {
+        stats.convergence = SDX_CONV_STOPPED;
+        break;
+      }
+      numIterDone++;
+      if (!first) Assemble(a, ai, x, u0, v0, dt, grav, true, projectPsd, res);
+      // linear solve with the global BSR and block Jacobi
+      Prec<T> M;
+      if (sp.linear.preconditioner == SDX_PREC_BLOCK_JACOBI) {
+        M.bs = 3;
+        M.inv.resize(9 * size_t(a.nV));
+        for (int i = 0; i < a.nV; ++i)
+          Inverse3(&a.K.val[9 * size_t(a.K.diag[size_t(i)])], &M.inv[9 * size_t(i)]);
+      } else if (sp.linear.preconditioner == SDX_PREC_JACOBI) {
+        M.bs = 1;
+        M.inv.resize(n);
+        for (int i = 0; i < a.nV; ++i)
+          for (int d = 0; d < 3; ++d) M.inv[3 * size_t(i) + d] = T(1) / a.K.val[9 * size_t(a.K.diag[size_t(i)]) + 4 * d];
+      }
+      sdx_krylov_params kp = sp.linear;
+      kp.initial_guess_zero = 1;
+      std::fill(dx.begin(), dx.end(), T(0));
+      BsrOp<T> op(&a.K);
+      LinStatus ls = PCG<T>(op, res, dx, M, kp);
+      stats.linear_iters += ls.iters;
+      stats.linear_convergence = ls.conv;
+      stats.linear_rel_res = ls.relResNorm;
+      dxNorm = double(Norm(dx));
+      haveDx = true;
+      // Line search along -dx.  A converged solve takes the step; a failed line
+      // search or an unconverged solve retries once with an identical
+      // operator (fitted Hessian already has no effect without friction),
+      // which reproduces the same step, and then keeps the final iterate
+      // (newton_solver.cpp:783-854).
+      const int lsMax = sp.line_search_type == SDX_LS_RESIDUAL_NORM ? sp.line_search_max_iter : 1;
+      sol0 = x;
+      const T lsRes0 = resNorm;
+      bool improved = false;
+      int it = 0;
+      T scale = T(1);
+      for (it = 0; it < lsMax; ++it) {
+        if (it > 0) scale *= T(sp.line_search_alpha);
+        for (size_t i = 0; i < n; ++i) x[i] = sol0[i] - scale * dx[i];
+        if (sp.line_search_type != SDX_LS_RESIDUAL_NORM) {
+          improved = true;
+          break;
+        }
+        Assemble(a, ai, x, u0, v0, dt, grav, false, projectPsd, res);
+        if (Norm(res) <= lsRes0) {
+          improved = true;
+          break;
+        }
+      }
+      const int lsIters = (it < lsMax) ? it : lsMax;
+      stats.line_search_iters += lsIters;
+      const bool linConv = ls.conv == SDX_CONV_CONVERGED;
+      if (!improved || !linConv) {
+        // the retry repeats the same assembly, solve and line search
+        if (linConv) stats.line_search_iters += lsIters;
+        stats.linear_iters += ls.iters;
+      }
+      if (sp.explosion_control) {
+        Assemble(a, ai, x, u0, v0, dt, grav, false, projectPsd, res);
+        resNorm = Norm(res);
+        if (resNorm > T(sp.rel_div_tol) * resNorm0 || resNorm > T(sp.abs_div_tol) ||
+            !std::isfinite(double(resNorm))) {
+          stats.convergence = SDX_CONV_DIVERGED;
+          break;
+        }
+      }
+    }
+    stats.newton_iters = numIterDone;
+    stats.res_norm0 = double(resNorm0);
+    stats.res_norm = double(resNorm);
+  }
+
+  // Scene::Step for every actor: EntityIncrementStep, PreStage, Newton, PostStage.
+  void Step(T dt, const sdx_solver_params& sp, sdx_step_stats* stats, int nStats) {
+    for (size_t ai = 0; ai < actors.size(); ++ai) {
+      Actor<T>& a = actors[ai];
+      const std::vector<T> u0 = a.u, v0 = a.v;
+      for (size_t i = 0; i < a.u.size(); ++i)
+        if (a.pinned[i]) a.u[i] = a.pinValue[i];
+      sdx_step_stats st{};
+      NewtonSolve(a, int(ai), u0, v0, dt, sp, st);
+      for (size_t i = 0; i < a.u.size(); ++i) a.v[i] = (a.u[i] - u0[i]) * (T(1) / dt);
+      if (st.convergence == SDX_CONV_DIVERGED) {
+        std::fill(a.u.begin(), a.u.end(), T(0));
+        std::fill(a.v.begin(), a.v.end(), T(0));
+      }
+      if (stats && int(ai) < nStats) stats[ai] = st;
+    }
+  }
+};
+
+struct SceneBase {
+  virtual ~SceneBase() = default;
+};
+template <class T>
+struct SceneHolder : SceneBase {
+  Scene<T> s;
+};
+
+}  // namespace sdxo
+
+using namespace sdxo;
+
+struct sdxo_scene {
+  int fp64;
+  std::unique_ptr<SceneHolder<float>> f;
+  std::unique_ptr<SceneHolder<double>> d;
+};
+
+template <class F>
+static int Dispatch(sdxo_scene* s, F&& fn) {
+  if (!s) return SDX_ERR_INVALID_ARGUMENT;
+  if (s->fp64) return fn(s->d->s);
+  return fn(s->f->s);
+}
+
+extern "C" {
+
+void* sdxo_scene_create(int fp64) {
+  auto* s = new sdxo_scene;
+  s->fp64 = fp64;
+  if (fp64)
+    s->d.reset(new SceneHolder<double>);
+  else
+    s->f.reset(new SceneHolder<float>);
+  return s;
+}
+
+void sdxo_scene_destroy(void* s) { delete static_cast<sdxo_scene*>(s); }
+
+int sdxo_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
+
+int sdxo_add_soft_tet_mesh(void* sv, const double* X, int nV, const int32_t* tets, int nT,
+                           const sdx_soft_params* sp) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) { return sc.AddActor(X, nV, tets, nT, *sp); });
+}
+
+int sdxo_set_element_materials(void* sv, int actor, const double* E, const double* nu, int nT) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    using T = typename std::remove_reference_t<decltype(sc.actors[0].X)>::value_type;
+    auto& a = sc.actors[size_t(actor)];
+    if (nT != a.nT) return int(SDX_ERR_INVALID_ARGUMENT);
+    for (int t = 0; t < nT; ++t) {
+      double lam = E[t] * nu[t] / ((1.0 + nu[t]) * (1.0 - 2.0 * nu[t]));
+      double mu = E[t] / (2.0 * (1.0 + nu[t]));
+      a.mu[size_t(t)] = T(mu);
+      a.lambda[size_t(t)] = T(lam);
+    }
+    return int(SDX_OK);
+  });
+}
+
+int sdxo_pin_nodes(void* sv, int actor, const int32_t* nodes, int n) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    auto& a = sc.actors[size_t(actor)];
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) {
+        a.pinned[3 * size_t(nodes[i]) + d] = 1;
+        a.pinValue[3 * size_t(nodes[i]) + d] = a.u[3 * size_t(nodes[i]) + d];
+      }
+    return int(SDX_OK);
+  });
+}
+
+int sdxo_add_collider(void* sv, int kind, const double* p, const double* nrm, double r, int actor) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    Collider c{};
+    c.kind = kind;
+    for (int d = 0; d < 3; ++d) {
+      c.p[d] = p[d];
+      c.n[d] = nrm ? nrm[d] : 0.0;
+    }
+    c.r = r;
+    c.actor = actor;
+    sc.colliders.push_back(c);
+    return int(sc.colliders.size()) - 1;
+  });
+}
+
+int sdxo_set_collider_position(void* sv, int id, const double* p) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    for (int d = 0; d < 3; ++d) sc.colliders[size_t(id)].p[d] = p[d];
+    return int(SDX_OK);
+  });
+}
+
+int sdxo_step(void* sv, double dt, const sdx_solver_params* sp, sdx_step_stats* stats, int nStats) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    using T = typename std::remove_reference_t<decltype(sc.actors[0].X)>::value_type;
+    sc.Step(T(dt), *sp, stats, nStats);
+    return int(SDX_OK);
+  });
+}
+
+int sdxo_get_state(void* sv, int actor, int which, double* out) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    auto& a = sc.actors[size_t(actor)];
+    const auto& v = which == 0 ? a.u : a.v;
+    for (size_t i = 0; i < v.size(); ++i) out[i] = double(v[i]);
+    return int(SDX_OK);
+  });
+}
+
+int sdxo_set_state(void* sv, int actor, int which, const double* in) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    using T = typename std::remove_reference_t<decltype(sc.actors[0].X)>::value_type;
+    auto& a = sc.actors[size_t(actor)];
+    auto& v = which == 0 ? a.u : a.v;
+    for (size_t i = 0; i < v.size(); ++i) v[i] = T(in[i]);
+    return int(SDX_OK);
+  });
+}
+
+// sizes: out[0]=nV, out[1]=nT, out[2]=nnzb, out[3]=nTri
+int sdxo_actor_info(void* sv, int actor, int64_t* out) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    auto& a = sc.actors[size_t(actor)];
+    out[0] = a.nV;
+    out[1] = a.nT;
+    out[2] = int64_t(a.K.col.size());
+    out[3] = int64_t(a.triArea.size());
+    return int(SDX_OK);
+  });
+}
+
+// Full assembly at the actor's current displacement with stage start = (u0, v0)
+// given explicitly (NULL: zeros).  Outputs in double.
+int sdxo_assemble(void* sv, int actor, double dt, const sdx_solver_params* sp, const double* u0in,
+                  const double* v0in, double* res, int32_t* ptr, int32_t* col, double* val) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    using T = typename std::remove_reference_t<decltype(sc.actors[0].X)>::value_type;
+    auto& a = sc.actors[size_t(actor)];
+    const size_t n = 3 * size_t(a.nV);
+    std::vector<T> u0(n, T(0)), v0(n, T(0)), r(n);
+    if (u0in)
+      for (size_t i = 0; i < n; ++i) u0[i] = T(u0in[i]);
+    if (v0in)
+      for (size_t i = 0; i < n; ++i) v0[i] = T(v0in[i]);
+    const T grav[3] = {T(sp->gravity[0]), T(sp->gravity[1]), T(sp->gravity[2])};
+    sc.Assemble(a, actor, a.u, u0, v0, T(dt), grav, true, sp->psd_proj_mode == SDX_PSD_ALWAYS, r);
+    if (res)
+      for (size_t i = 0; i < n; ++i) res[i] = double(r[i]);
+    if (ptr) std::copy(a.K.ptr.begin(), a.K.ptr.end(), ptr);
+    if (col) std::copy(a.K.col.begin(), a.K.col.end(), col);
+    if (val)
+      for (size_t i = 0; i < a.K.val.size(); ++i) val[i] = double(a.K.val[i]);
+    return int(SDX_OK);
+  });
+}
+
+// Boundary triangles (3 per row) and rest areas of one actor.
+int sdxo_boundary(void* sv, int actor, int32_t* tris, double* area) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    auto& a = sc.actors[size_t(actor)];
+    if (tris) std::copy(a.tris.begin(), a.tris.end(), tris);
+    if (area)
+      for (size_t i = 0; i < a.triArea.size(); ++i) area[i] = double(a.triArea[i]);
+    return int(SDX_OK);
+  });
+}
+
+// Material response at one F (double or float arithmetic).
+void sdxo_smith_nh(int fp64, double mu, double lambda, const double* F, double* energy, double* P,
+                   double* C, int project) {
+  if (fp64) {
+    SmithNH<double>(mu, lambda, F, energy, P, C, project != 0);
+  } else {
+    float Ff[9], Pf[9], Cf[81];
+    for (int i = 0; i < 9; ++i) Ff[i] = float(F[i]);
+    SmithNH<float>(float(mu), float(lambda), Ff, energy, P ? Pf : nullptr, C ? Cf : nullptr, project != 0);
+    if (P)
+      for (int i = 0; i < 9; ++i) P[i] = Pf[i];
+    if (C)
+      for (int i = 0; i < 81; ++i) C[i] = Cf[i];
+  }
+}
+
+void sdxo_polyrelu(double x, double t, double shift, double* out3) {
+  double f, df, ddf;
+  PolyReLU<double>(x, t, shift, f, df, ddf);
+  out3[0] = f;
+  out3[1] = df;
+  out3[2] = ddf;
+}
+
+// Stress residual/Hessian of one tet given rest nodes X (12) and displacement u (12).
+void sdxo_stress_element(const double* X, const double* u, double mu, double lambda, int project,
+                         double* res, double* H) {
+  const int32_t tv[4] = {0, 1, 2, 3};
+  TetGeom<double> g = MakeTetGeom<double>(X, tv);
+  double r[12] = {}, h[144] = {};
+  StressElement<double>(g, u, mu, lambda, true, project != 0, r, h);
+  std::copy(r, r + 12, res);
+  std::copy(h, h + 144, H);
+}
+
+// PCG on a CSR matrix (scalar rows) with preconditioner block size bs in {0,1,3}.
+int sdxo_pcg_csr(int fp64, int n, const int32_t* ptr, const int32_t* col, const double* val,
+                 const double* b, double* x, int bs, const sdx_krylov_params* kp, int* iters, int* conv,
+                 double* resNorm) {
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    CsrOp<T> op;
+    op.n = n;
+    op.ptr = ptr;
+    op.col = col;
+    op.val.assign(val, val + ptr[n]);
+    std::vector<T> bb(b, b + n), xx(x, x + n);
+    Prec<T> M;
+    M.bs = bs;
+    if (bs == 1) {
+      M.inv.resize(size_t(n));
+      for (int i = 0; i < n; ++i)
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+          if (col[k] == i) M.inv[size_t(i)] = T(1) / T(val[k]);
+    } else if (bs == 3) {
+      M.inv.assign(3 * size_t(n), T(0));
+      std::vector<T> blk(9 * size_t(n / 3), T(0));
+      for (int i = 0; i < n; ++i)
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+          if (col[k] / 3 == i / 3) blk[9 * size_t(i / 3) + 3 * (i % 3) + col[k] % 3] = T(val[k]);
+      for (int bI = 0; bI < n / 3; ++bI) Inverse3(&blk[9 * size_t(bI)], &M.inv[9 * size_t(bI)]);
+    }
+    LinStatus s = PCG<T>(op, bb, xx, M, *kp);
+    for (int i = 0; i < n; ++i) x[i] = double(xx[size_t(i)]);
+    *iters = s.iters;
+    *conv = s.conv;
+    *resNorm = s.resNorm;
+    return 0;
+  };
+  return fp64 ? run(double(0)) : run(float(0));
+}
+
+}  // extern "C"
