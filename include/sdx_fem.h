@@ -179,7 +179,11 @@ typedef struct sdx_world sdx_world;
 /* Fill a struct with the reference defaults. */
 void sdx_default_solver_params(sdx_solver_params* out);
 void sdx_default_soft_params(sdx_soft_params* out);
+void sdx_default_krylov(sdx_krylov_params* out);
 const char* sdx_last_error(void);
+/* sizeof of the ABI structs (0 krylov, 1 linear status, 2 solver params,
+ * 3 step stats, 4 soft params) so bindings can check their layouts. */
+int32_t sdx_abi_struct_size(int32_t which);
 
 /* ---------------- L1: linear solve (CudaIterativeSolve seam) ---------------- */
 
@@ -253,9 +257,14 @@ sdx_status sdx_assemble(sdx_world* world, int32_t actor, double dt, const sdx_so
 
 /* Timing of one kernel class inside sdx_step, measured with HIP events on the
  * world's stream.  kernel: 0 = CG matvec (SpMV + p update + p.Ap),
- * 1 = CG vector update, 2 = element Hessian, 3 = BSR gather.  enable 1 clears
- * the accumulators.  out: {launches, total milliseconds}. */
+ * 1 = CG vector update, 2 = element Hessian, 3 = BSR gather.  enable N > 0
+ * clears the accumulators and times every N-th launch of the class, 0 stops
+ * timing, -1 leaves the setting unchanged.  out (may be NULL): {timed
+ * launches, total milliseconds}. */
 sdx_status sdx_kernel_timing(sdx_world* world, int32_t kernel, int32_t enable, double out[2]);
+
+/* CG iterations launched between host convergence checks (default 8). */
+sdx_status sdx_set_cg_burst(sdx_world* world, int32_t burst);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,328 @@
+// C ABI of include/sdx_fem.h: argument checking, error codes and dispatch
+// to World<T> / the standalone PCG solve.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sdx_fem.h"
+#include "sdx_kernels.h"
+#include "sdx_world.h"
+
+using sdx::set_error;
+
+struct sdx_world {
+  std::unique_ptr<sdx::WorldBase> impl;
+};
+
+namespace {
+
+template <class F>
+sdx_status guarded(F&& fn) {
+  try {
+    return fn();
+  } catch (const sdx::DeviceError& e) {
+    return set_error(SDX_ERR_DEVICE, e.what());
+  } catch (const std::bad_alloc&) {
+    return set_error(SDX_ERR_OUT_OF_MEMORY, "out of device or host memory");
+  } catch (const std::out_of_range& e) {
+    return set_error(SDX_ERR_INVALID_ARGUMENT, e.what());
+  } catch (const std::invalid_argument& e) {
+    return set_error(SDX_ERR_INVALID_ARGUMENT, e.what());
+  } catch (const std::exception& e) {
+    return set_error(SDX_ERR_STATE, e.what());
+  }
+}
+
+#define SDX_REQUIRE(cond, msg) \
+  do {                         \
+    if (!(cond)) return set_error(SDX_ERR_INVALID_ARGUMENT, msg); \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+const char* sdx_last_error(void) { return sdx::g_last_error.c_str(); }
+
+void sdx_default_krylov(sdx_krylov_params* k) {
+  k->abs_tol = 1e-9;
+  k->rel_tol = 1e-5;
+  k->rel_div_tol = 1e10;
+  k->max_iter = 500;
+  k->norm_type = SDX_NORM_PRECONDITIONED_RESIDUAL_L2;
+  k->preconditioner = SDX_PREC_BLOCK_JACOBI;
+  k->abort_if_not_spd = 0;
+  k->initial_guess_zero = 1;
+  k->precision_mode = SDX_PREC_MODE_FP32;
+  k->refresh_every = 0;
+  k->reserved = 0;
+}
+
+void sdx_default_solver_params(sdx_solver_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->newton_max_iter = 4;
+  p->psd_proj_mode = SDX_PSD_ALWAYS;
+  p->newton_abs_tol = 1e-3;
+  p->newton_rel_tol = 1e-6;
+  p->rel_step_tol = 10.0 * double(std::numeric_limits<float>::epsilon());
+  p->explosion_control = 1;
+  p->line_search_type = SDX_LS_RESIDUAL_NORM;
+  p->abs_div_tol = 1e9;
+  p->rel_div_tol = 1e4;
+  p->line_search_max_iter = 4;
+  p->linear_tolerance_constant = 1;
+  p->line_search_alpha = 0.5;
+  sdx_default_krylov(&p->linear);
+  p->gravity[0] = 0.0;
+  p->gravity[1] = -9.8;
+  p->gravity[2] = 0.0;
+  p->warm_start = SDX_WARM_START_ZERO;
+  p->schedule = -1;
+}
+
+void sdx_default_soft_params(sdx_soft_params* s) {
+  s->youngs_modulus = 1e5;
+  s->poisson_ratio = 0.45;
+  s->density = 1000.0;
+  s->mass_damping = 0.0;
+  s->penalty_coefficient = 1e9;
+  s->penalty_smoothing_half_distance = 0.005;
+  s->penalty_threshold = 0.001;
+  s->max_alignment_normals = 0.0;
+}
+
+int32_t sdx_abi_struct_size(int32_t which) {
+  switch (which) {
+    case 0: return int32_t(sizeof(sdx_krylov_params));
+    case 1: return int32_t(sizeof(sdx_linear_status));
+    case 2: return int32_t(sizeof(sdx_solver_params));
+    case 3: return int32_t(sizeof(sdx_step_stats));
+    case 4: return int32_t(sizeof(sdx_soft_params));
+    default: return -1;
+  }
+}
+
+sdx_status sdx_world_create(int32_t device, int32_t precision_mode, sdx_world** out) {
+  SDX_REQUIRE(out, "out is NULL");
+  SDX_REQUIRE(precision_mode >= SDX_PREC_MODE_FP32 && precision_mode <= SDX_PREC_MODE_FP64, "bad precision mode");
+  return guarded([&]() -> sdx_status {
+    int n = 0;
+    sdx::hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    SDX_REQUIRE(device >= 0 && device < n, "no such HIP device");
+    auto* w = new sdx_world;
+    if (precision_mode == SDX_PREC_MODE_FP64)
+      w->impl.reset(new sdx::World<double>(device, precision_mode));
+    else
+      w->impl.reset(new sdx::World<float>(device, precision_mode));
+    *out = w;
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_world_destroy(sdx_world* world) {
+  delete world;
+  return SDX_OK;
+}
+
+sdx_status sdx_add_soft_tet_mesh(sdx_world* world, const double* X, int32_t n_verts, const int32_t* tets,
+                                 int32_t n_tets, const sdx_soft_params* params, int32_t* out_actor) {
+  SDX_REQUIRE(world && X && tets && params, "NULL argument");
+  SDX_REQUIRE(n_verts > 0 && n_tets > 0, "empty mesh");
+  SDX_REQUIRE(params->youngs_modulus > 0 && params->poisson_ratio > -1 && params->poisson_ratio < 0.5,
+              "invalid Young's modulus or Poisson ratio");
+  SDX_REQUIRE(params->density > 0, "density must be positive");
+  SDX_REQUIRE(params->penalty_coefficient > 0 && params->penalty_smoothing_half_distance >= 0,
+              "invalid contact parameters");
+  for (int64_t k = 0; k < 4 * int64_t(n_tets); ++k)
+    SDX_REQUIRE(tets[k] >= 0 && tets[k] < n_verts, "tet node index out of range");
+  return guarded([&]() -> sdx_status {
+    sdx::HostActor a;
+    a.nV = n_verts;
+    a.nT = n_tets;
+    a.X.assign(X, X + 3 * size_t(n_verts));
+    a.tets.assign(tets, tets + 4 * size_t(n_tets));
+    a.params = *params;
+    a.pinned.assign(3 * size_t(n_verts), 0);
+    a.pin_value.assign(3 * size_t(n_verts), 0.0);
+    a.u.assign(3 * size_t(n_verts), 0.0);
+    a.v.assign(3 * size_t(n_verts), 0.0);
+    const int id = world->impl->add_actor(std::move(a));
+    if (out_actor) *out_actor = id;
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_set_element_materials(sdx_world* world, int32_t actor, const double* E, const double* nu,
+                                     int32_t n_tets) {
+  SDX_REQUIRE(world && E && nu, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    sdx::HostActor& a = world->impl->actor(actor);
+    SDX_REQUIRE(n_tets == a.nT, "n_tets does not match the actor");
+    for (int t = 0; t < n_tets; ++t)
+      SDX_REQUIRE(E[t] > 0 && nu[t] > -1 && nu[t] < 0.5, "invalid per-element material");
+    world->impl->invalidate();
+    sdx::HostActor& b = world->impl->actor(actor);
+    b.E.assign(E, E + n_tets);
+    b.nu.assign(nu, nu + n_tets);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_pin_nodes(sdx_world* world, int32_t actor, const int32_t* nodes, int32_t n) {
+  SDX_REQUIRE(world && (nodes || n == 0), "NULL argument");
+  return guarded([&]() -> sdx_status {
+    world->impl->invalidate();
+    sdx::HostActor& a = world->impl->actor(actor);
+    for (int i = 0; i < n; ++i) SDX_REQUIRE(nodes[i] >= 0 && nodes[i] < a.nV, "node index out of range");
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) {
+        a.pinned[3 * size_t(nodes[i]) + size_t(d)] = 1;
+        a.pin_value[3 * size_t(nodes[i]) + size_t(d)] = a.u[3 * size_t(nodes[i]) + size_t(d)];
+      }
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_add_rigid_plane(sdx_world* world, const double p[3], const double n[3], int32_t actor,
+                               int32_t* out_collider) {
+  SDX_REQUIRE(world && p && n, "NULL argument");
+  const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  SDX_REQUIRE(l > 0 && std::isfinite(l), "plane normal must be non-zero");
+  return guarded([&]() -> sdx_status {
+    SDX_REQUIRE(actor >= -1 && actor < world->impl->num_actors(), "no such actor");
+    sdx::HostCollider c;
+    c.kind = 0;
+    for (int d = 0; d < 3; ++d) {
+      c.p[d] = p[d];
+      c.n[d] = n[d] / l;
+    }
+    c.actor = actor;
+    const int id = world->impl->add_collider(c);
+    if (out_collider) *out_collider = id;
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_add_rigid_sphere(sdx_world* world, const double c[3], double r, int32_t actor,
+                                int32_t* out_collider) {
+  SDX_REQUIRE(world && c, "NULL argument");
+  SDX_REQUIRE(r > 0, "sphere radius must be positive");
+  return guarded([&]() -> sdx_status {
+    SDX_REQUIRE(actor >= -1 && actor < world->impl->num_actors(), "no such actor");
+    sdx::HostCollider s;
+    s.kind = 1;
+    for (int d = 0; d < 3; ++d) s.p[d] = c[d];
+    s.r = r;
+    s.actor = actor;
+    const int id = world->impl->add_collider(s);
+    if (out_collider) *out_collider = id;
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_set_rigid_position(sdx_world* world, int32_t collider, const double pos[3]) {
+  SDX_REQUIRE(world && pos, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    sdx::HostCollider& c = world->impl->collider(collider);
+    for (int d = 0; d < 3; ++d) c.p[d] = pos[d];
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_step(sdx_world* world, double dt, const sdx_solver_params* params, sdx_step_stats* stats,
+                    int32_t n_stats) {
+  SDX_REQUIRE(world && params, "NULL argument");
+  SDX_REQUIRE(dt > 0 && std::isfinite(dt), "dt must be positive");
+  SDX_REQUIRE(params->newton_max_iter >= 0 && params->linear.max_iter > 0, "invalid iteration limits");
+  SDX_REQUIRE(params->line_search_type == SDX_LS_NONE || params->line_search_type == SDX_LS_RESIDUAL_NORM,
+              "only the None and ResidualNorm line searches are implemented");
+  SDX_REQUIRE(params->psd_proj_mode == SDX_PSD_ALWAYS || params->psd_proj_mode == SDX_PSD_NEVER,
+              "only PSD projection modes Always and Never are implemented");
+  SDX_REQUIRE(params->linear.preconditioner >= SDX_PREC_NONE && params->linear.preconditioner <= SDX_PREC_BLOCK_JACOBI,
+              "unsupported preconditioner");
+  SDX_REQUIRE(params->linear.norm_type >= 0 && params->linear.norm_type <= 2, "unsupported norm type");
+  SDX_REQUIRE(params->warm_start == SDX_WARM_START_ZERO || params->warm_start == SDX_WARM_START_LEARNED,
+              "unknown warm start");
+  return guarded([&]() -> sdx_status {
+    world->impl->step(dt, *params, stats, stats ? n_stats : 0);
+    return SDX_OK;
+  });
+}
+
+static sdx_status get_state(sdx_world* world, int32_t actor, int which, double* out, int32_t n) {
+  SDX_REQUIRE(world && out, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    SDX_REQUIRE(n == 3 * world->impl->actor(actor).nV, "size must be 3*n_verts");
+    world->impl->get_state(actor, which, out);
+    return SDX_OK;
+  });
+}
+
+static sdx_status set_state(sdx_world* world, int32_t actor, int which, const double* in, int32_t n) {
+  SDX_REQUIRE(world && in, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    SDX_REQUIRE(n == 3 * world->impl->actor(actor).nV, "size must be 3*n_verts");
+    world->impl->set_state(actor, which, in);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_get_displacements(sdx_world* w, int32_t a, double* out, int32_t n) { return get_state(w, a, 0, out, n); }
+sdx_status sdx_set_displacements(sdx_world* w, int32_t a, const double* in, int32_t n) { return set_state(w, a, 0, in, n); }
+sdx_status sdx_get_velocities(sdx_world* w, int32_t a, double* out, int32_t n) { return get_state(w, a, 1, out, n); }
+sdx_status sdx_set_velocities(sdx_world* w, int32_t a, const double* in, int32_t n) { return set_state(w, a, 1, in, n); }
+sdx_status sdx_get_positions_world(sdx_world* w, int32_t a, double* out, int32_t n) { return get_state(w, a, 2, out, n); }
+
+sdx_status sdx_world_info(sdx_world* world, int64_t out[8]) {
+  SDX_REQUIRE(world && out, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    world->impl->info(out);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_actor_bsr_size(sdx_world* world, int32_t actor, int32_t* n_block_rows, int32_t* nnzb) {
+  SDX_REQUIRE(world, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    world->impl->actor_bsr_size(actor, n_block_rows, nnzb);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_assemble(sdx_world* world, int32_t actor, double dt, const sdx_solver_params* params,
+                        double* out_residual, int32_t* out_row_ptr, int32_t* out_col_idx, double* out_values) {
+  SDX_REQUIRE(world && params, "NULL argument");
+  SDX_REQUIRE(dt > 0, "dt must be positive");
+  return guarded([&]() -> sdx_status {
+    world->impl->assemble_actor(actor, dt, *params, out_residual, out_row_ptr, out_col_idx, out_values);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_kernel_timing(sdx_world* world, int32_t kernel, int32_t enable, double out[2]) {
+  SDX_REQUIRE(world, "NULL argument");
+  SDX_REQUIRE(kernel >= 0 && kernel < 4, "kernel class must be 0..3");
+  return guarded([&]() -> sdx_status {
+    if (enable >= 0) world->impl->timing_enable(kernel, enable);
+    if (out) {
+      auto r = world->impl->timing_read(kernel);
+      out[0] = r.first;
+      out[1] = r.second;
+    }
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_set_cg_burst(sdx_world* world, int32_t burst) {
+  SDX_REQUIRE(world && burst > 0 && burst <= 256, "burst must be in 1..256");
+  world->impl->cg_burst_ = burst;
+  return SDX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+// Launch interface of the HIP kernels of the soft-body FEM step.
+// Host code (sdx_world.cpp, sdx_pcg.cpp) only calls these wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sdx {
+
+// Per-environment CG state, one per island, in device memory.
+enum : int32_t { kCgActive = 0, kCgConverged = 1, kCgStopped = 2, kCgDiverged = 3, kCgIdle = 4 };
+
+struct EnvCG {
+  double rz;       // r.z of the current iterate
+  double alpha;
+  double beta;
+  double start;    // stop-test scaling value (squared norm of b in the chosen norm)
+  double conv;     // squared convergence threshold
+  double div;      // squared divergence threshold
+  double latest;   // latest squared stop-test norm
+  double pAp;
+  int32_t iter;    // completed CG iterations
+  int32_t status;  // kCg*
+  int32_t refreshes;
+  int32_t pad;
+};
+
+struct CgConfig {
+  int32_t n_env;          // environments in the world (accumulator layout)
+  int32_t norm_type;      // SDX_NORM_*
+  int32_t prec;           // SDX_PREC_*
+  int32_t max_iter;
+  int32_t refresh_every;  // 0: never
+  int32_t abort_if_not_spd;
+  int32_t zero_guess;
+  int32_t pad;
+  double abs_tol, rel_tol, rel_div_tol;
+};
+
+// Tiles: contiguous row ranges that never cross an environment.
+struct Tiles {
+  const int32_t* env;    // [n] environment of the tile
+  const int32_t* row0;   // [n] first row
+  const int32_t* row1;   // [n] one past the last row
+  const int32_t* first;  // [nEnv+1] tile range of each environment
+  int32_t n;
+};
+
+constexpr int kSpmvLanes = 16;    // lanes per block row in the CG matvec
+constexpr int kSpmvThreads = 256;
+constexpr int kSpmvRows = kSpmvThreads / kSpmvLanes;  // 64 rows per matvec tile
+constexpr int kVecThreads = 256;  // rows per vector tile
+constexpr int kHeStride = 80;     // packed element Hessian (78 used) per tet
+constexpr int kReStride = 12;     // element residual per tet
+// Cross-block accumulators: per environment and set, kSlots lines of
+// kSlotStride values (one 128-byte line per slot for doubles).
+constexpr int kSlots = 16;
+constexpr int kSlotStride = 16;
+constexpr int kCgSets = 6;
+
+// Geometry planes of a tet, stored SoA with stride nT.
+enum : int { kGdN = 0, kGVol = 9, kGMu = 10, kGLambda = 11, kGMass = 12, kGAlpha = 13, kGeomPlanes = 14 };
+
+struct MeshView {
+  int32_t nV, nT, nF, nB, nEnv;
+  const int32_t* tets;      // [4 nT]
+  const int32_t* tet_env;   // [nT]
+  const int32_t* tris;      // [3 nF]
+  const int32_t* tri_env;   // [nF]
+  const int32_t* row_env;   // [nV]
+  const int32_t* row_ptr;   // [nV+1]
+  const int32_t* col;       // [nB]
+  const int32_t* slot_row;  // [nB]
+  const int32_t* diag_slot; // [nV]
+  const int32_t* slot_ptr;  // [nB+1] element contributions per slot
+  const uint32_t* slot_code;   // e*16 + a*4 + b
+  const int32_t* slot_ptr_c;   // [nB+1] contact contributions per slot
+  const uint32_t* slot_code_c; // tri*16 + a*4 + b
+  const int32_t* vtx_ptr;   // [nV+1]
+  const uint32_t* vtx_code; // e*4 + a
+  const int32_t* vtx_ptr_c; // [nV+1]
+  const uint32_t* vtx_code_c; // tri*4 + a
+  const uint8_t* pinned;    // [3 nV]
+  const int32_t* coll_first; // [nEnv+1] into coll
+  const double* coll;        // 8 doubles per collider: kind, p(3), n(3), r
+  const int32_t* env_mask;   // [nEnv] 1: assemble/solve this environment
+};
+
+struct ContactParamsDev {
+  double k, h, thr, max_align;
+};
+
+template <class T>
+struct FieldPtrs {
+  const T* X;   // rest positions [3nV]
+  const T* geom;  // [kGeomPlanes * nT]
+  const double* contact;  // [4 nEnv] k, h, thr, maxAlign
+};
+
+// ---- assembly ----
+template <class T>
+void launch_element(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0,
+                    const T* v0, double dt, const double grav[3], int wantH, int project, T* Re, T* He);
+template <class T>
+void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, int wantH, T* Rc,
+                    T* Hc);
+template <class T>
+void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* He, const T* Hc, int prec, T* vals,
+                       T* dinv);
+// acc: set 0 of the accumulators (zeroed by the caller); per-env ||res||^2.
+template <class T, class A>
+void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Re, const T* Rc, T* res, A* acc);
+template <class A>
+void launch_fill(hipStream_t s, int64_t n, A v, A* d);
+// out[e * nv + k] = sum over slots of set 0, k < nv.
+template <class A>
+void launch_acc_finalize(hipStream_t s, int nEnv, int nv, const A* acc, double* out);
+
+// ---- CG ----
+template <class A>
+void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, A* acc);
+template <class T, class A>
+void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const T* ax0, const T* dinv,
+                    int32_t nV, T* x, T* r, T* z, EnvCG* env, A* acc);
+template <class T, class A>
+void launch_cg_spmv(hipStream_t s, const Tiles& ta, const CgConfig& cfg, int j, const int32_t* row_ptr,
+                    const int32_t* col, const T* vals, int32_t nB, const T* z, T* p0, T* p1, const T* x, const T* b,
+                    T* Ap, double* rref, EnvCG* env, A* acc, int32_t* num_active);
+template <class T, class A>
+void launch_cg_update(hipStream_t s, const Tiles& tv, const CgConfig& cfg, int j, const T* dinv, int32_t nV,
+                      const T* p0, const T* p1, const T* Ap, const double* rref, T* x, T* r, T* z, EnvCG* env, A* acc,
+                      int32_t* num_active);
+// y = A x over all rows (plain matvec).
+template <class T>
+void launch_spmv_plain(hipStream_t s, const Tiles& ta, const int32_t* row_ptr, const int32_t* col,
+                       const T* vals, int32_t nB, const T* x, T* y);
+// Diagonal-block inverse of a BSR matrix (L1 entry; assembly fuses it).
+template <class T>
+void launch_block_inverse(hipStream_t s, int32_t nV, const int32_t* diag_slot, const T* vals, int32_t nB,
+                          int prec, T* dinv);
+
+// ---- vector helpers ----
+template <class T>
+void launch_copy(hipStream_t s, int64_t n, const T* src, T* dst);
+template <class T>
+void launch_apply_pins(hipStream_t s, int64_t n, const uint8_t* pinned, const T* value, T* u);
+// x = sol0 - scale[env] * dx on rows whose env has ls_mask set.
+template <class T>
+void launch_ls_update(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* ls_mask,
+                      const double* scale, const T* sol0, const T* dx, T* x);
+// v = (u - u0) / dt; environments with reset[env] get u = v = 0.
+template <class T>
+void launch_post_step(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* reset, double dt,
+                      const T* u0, T* u, T* v);
+// Per-environment |a|^2, |b|^2 added into set 0 of acc (zeroed by the caller).
+template <class T>
+void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, const T* b, double* acc);
+
+}  // namespace sdx

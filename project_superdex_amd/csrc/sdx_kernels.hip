@@ -1,0 +1,1107 @@
+// HIP kernels of the implicit soft-body FEM step for gfx950 (CDNA4).
+//
+// Data layout in HBM (V vertices, T tets, F boundary triangles, B BSR blocks):
+//   DoF vectors ............ [3V] node-major xyz, FP32 (FP64 in fp64 mode)
+//   tet geometry ........... 14 SoA planes of T reals (coalesced per tet)
+//   element outputs ........ Re [T][12], He [T][80] (78 packed Hessian entries)
+//   BSR values ............. 9 SoA planes of B reals (vals[c*B + slot]); one
+//                            matvec lane per block reads each plane coalesced
+//   block-Jacobi inverse ... 9 SoA planes of V reals
+// Assembly is atomic-free: every BSR slot / vertex owns a precomputed,
+// sorted list of (element, local block) contributions and sums it in a
+// fixed order, so results are deterministic.
+//
+// CG (pcg.h:70-238, Polak-Ribiere) takes two kernels per iteration:
+//   cg_spmv:   p = z + beta p_old on the fly, Ap = A p, partial p.Ap
+//   cg_update: x += alpha p, r -= alpha Ap, z = D^-1 r, partial r.r, r.z, z.z, r.z_old
+// Per-environment scalars (p.Ap, r.r, r.z, z.z, r.z_old) are summed by spread
+// atomic accumulators; every block of the next kernel derives alpha, beta and
+// the stop test from them.  Tiles never straddle environments, so hundreds of
+// uneven environments are solved by the same launches.
+#include "sdx_kernels.h"
+#include "sdx_math.h"
+
+namespace sdx {
+
+// ------------------------------------------------------------------ reductions
+//
+// Cross-block sums use spread atomic accumulators instead of a last-block
+// reduction: on a multi-XCD CDNA device an agent-scope fence per block (L2
+// write-back / invalidate) costs far more than the kernel's own traffic,
+// whereas FP64 atomic adds spread over kSlots separate 128-byte lines cost
+// almost nothing.  Consumers (the next kernel, or the host) sum the slots in
+// a fixed order; kernel boundaries order producers and consumers.
+
+template <class A>
+__device__ __forceinline__ A wave_sum(A v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum NV values over the block; result valid in thread 0.
+template <class A, int NV>
+__device__ __forceinline__ void block_sum(A (&v)[NV]) {
+  __shared__ A sh[NV][16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[k][wid] = v[k];
+  __syncthreads();
+  if (wid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      A t = lane < nw ? sh[k][lane] : A(0);
+      v[k] = wave_sum(t);
+    }
+  }
+  __syncthreads();
+}
+
+// Accumulator set `set` of environment e: kSlots lines of kSlotStride values.
+template <class A>
+__device__ __forceinline__ A* acc_ptr(A* acc, int set, int nEnv, int e) {
+  return acc + ((size_t)(set * nEnv + e) * kSlots) * kSlotStride;
+}
+
+// Block-sum v (one barrier: wave sums, then thread 0 adds the per-wave
+// values) and add it into slot (tile_in_env % kSlots) of `dst`.  Uses its own
+// LDS, so a kernel may call it more than once.
+template <class A, int NV>
+__device__ __forceinline__ void acc_add(A (&v)[NV], A* dst, int tile_in_env) {
+  __shared__ A sh[NV][16];
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[k][threadIdx.x >> 6] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    A* d = dst + (tile_in_env & (kSlots - 1)) * kSlotStride;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      A t = sh[k][0];
+      for (int w = 1; w < nw; ++w) t += sh[k][w];
+      atomicAdd(d + k, t);
+    }
+  }
+}
+
+template <class A, int NV>
+__device__ __forceinline__ void acc_read(const A* src, double (&out)[NV]) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) out[k] = 0.0;
+  for (int sl = 0; sl < kSlots; ++sl)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] += double(src[sl * kSlotStride + k]);
+}
+
+// Zero one accumulator set of one environment (kSlots * kSlotStride values).
+template <class A>
+__device__ __forceinline__ void acc_zero(A* dst) {
+  for (int k = threadIdx.x; k < kSlots * kSlotStride; k += blockDim.x) dst[k] = A(0);
+}
+
+// ------------------------------------------------------------------ element
+
+template <class T>
+__global__ void __launch_bounds__(128) k_element(MeshView m, FieldPtrs<T> f, const T* __restrict__ u,
+                                                 const T* __restrict__ u0, const T* __restrict__ v0, T dt,
+                                                 T gx, T gy, T gz, int wantH, int project, T* __restrict__ Re,
+                                                 T* __restrict__ He) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.nT) return;
+  const int env = m.tet_env[t];
+  if (!m.env_mask[env]) return;
+  const int nT = m.nT;
+  const int4 tv = reinterpret_cast<const int4*>(m.tets)[t];
+  const int nd[4] = {tv.x, tv.y, tv.z, tv.w};
+  const T* g = f.geom;
+  V3<T> dN[4];
+  dN[1] = mk3(g[0 * nT + t], g[1 * nT + t], g[2 * nT + t]);
+  dN[2] = mk3(g[3 * nT + t], g[4 * nT + t], g[5 * nT + t]);
+  dN[3] = mk3(g[6 * nT + t], g[7 * nT + t], g[8 * nT + t]);
+  dN[0] = scl3(T(-1), add3(add3(dN[1], dN[2]), dN[3]));
+  const T vol = g[kGVol * nT + t], mu = g[kGMu * nT + t], lam = g[kGLambda * nT + t];
+  const T mass = g[kGMass * nT + t], alpha = g[kGAlpha * nT + t];
+  T ue[12], w[12], wd[12];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T uu = u[3 * nd[a] + d], s0 = u0[3 * nd[a] + d];
+      ue[3 * a + d] = uu;
+      w[3 * a + d] = uu - (s0 + dt * v0[3 * nd[a] + d]);  // u - predicted target (batched_element_utils.h:90-120)
+      wd[3 * a + d] = uu - s0;
+    }
+  T res[12];
+  T H[78];
+  smith_element<T>(dN, ue, vol, mu, lam, wantH != 0, project != 0, res, H);
+  // gravity (fem_gravity.h:108-118): res_f -= rho g vol / 4
+  const T gq = T(0.25) * mass;
+  // consistent mass M_ab = m/20 (1 + delta_ab) (Q4 integrates P1 x P1 exactly)
+  const T idt2 = T(1) / (dt * dt);
+  const T mdamp = alpha > T(0) ? alpha / dt : T(0);
+  const T m20 = mass * T(0.05);
+  T sw[3] = {w[0] + w[3] + w[6] + w[9], w[1] + w[4] + w[7] + w[10], w[2] + w[5] + w[8] + w[11]};
+  T sd[3] = {wd[0] + wd[3] + wd[6] + wd[9], wd[1] + wd[4] + wd[7] + wd[10], wd[2] + wd[5] + wd[8] + wd[11]};
+  const T gv[3] = {gx, gy, gz};
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T r = res[3 * a + d] - gv[d] * gq + idt2 * m20 * (w[3 * a + d] + sw[d]);
+      if (mdamp > T(0)) r += mdamp * m20 * (wd[3 * a + d] + sd[d]);
+      res[3 * a + d] = r;
+    }
+  T* re = Re + (size_t)t * kReStride;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) re[k] = res[k];
+  if (!wantH) return;
+  const T ms = (idt2 + mdamp) * m20;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = a; b < 4; ++b) {
+      T* blk = H + he_block_offset(a, b);
+      const T add = a == b ? T(2) * ms : ms;
+      if (a == b) {
+        blk[0] += add;
+        blk[1] += add;
+        blk[2] += add;
+      } else {
+        blk[0] += add;
+        blk[4] += add;
+        blk[8] += add;
+      }
+    }
+  T* he = He + (size_t)t * kHeStride;
+#pragma unroll
+  for (int k = 0; k < 78; ++k) he[k] = H[k];
+}
+
+// ------------------------------------------------------------------ contact
+
+__host__ __device__ constexpr int hc_block(int a, int b) { return a == 0 ? b : (a == 1 ? 2 + b : 5); }
+constexpr int kHcStride = 36;
+constexpr int kRcStride = 9;
+
+// Penalty contact of the 3 trace-Q3 samples of a boundary triangle against
+// the analytic colliders of its environment (contact_utils.h:498-570,
+// fem_traction.h:58-170).  Outputs per triangle: 9 residual entries and 6
+// symmetric 3x3 blocks.
+template <class T>
+__global__ void __launch_bounds__(256) k_contact(MeshView m, FieldPtrs<T> f, const T* __restrict__ u, int wantH,
+                                                 T* __restrict__ Rc, T* __restrict__ Hc) {
+  const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tr >= m.nF) return;
+  const int env = m.tri_env[tr];
+  if (!m.env_mask[env]) return;
+  T rc[9];
+  T hc[36];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) rc[k] = T(0);
+  if (wantH)
+#pragma unroll
+    for (int k = 0; k < 36; ++k) hc[k] = T(0);
+  const int c0 = m.coll_first[env], c1 = m.coll_first[env + 1];
+  if (c1 > c0) {
+    const int nd[3] = {m.tris[3 * tr], m.tris[3 * tr + 1], m.tris[3 * tr + 2]};
+    V3<T> xs[3], Xr[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      Xr[i] = mk3(f.X[3 * nd[i]], f.X[3 * nd[i] + 1], f.X[3 * nd[i] + 2]);
+      xs[i] = add3(Xr[i], mk3(u[3 * nd[i]], u[3 * nd[i] + 1], u[3 * nd[i] + 2]));
+    }
+    // rest area -> quad weight A0/3 ; current outward normal for the alignment mask
+    const V3<T> n0 = cross3(sub3(Xr[1], Xr[0]), sub3(Xr[2], Xr[0]));
+    const T w = T(0.5) * dsqrt(dot3(n0, n0)) / T(3);
+    V3<T> fnv = cross3(sub3(xs[1], xs[0]), sub3(xs[2], xs[0]));
+    fnv = scl3(T(1) / dsqrt(dot3(fnv, fnv)), fnv);
+    const double* cp = f.contact + 4 * env;
+    const T kp = T(cp[0]), hp = T(cp[1]), thr = T(cp[2]), maxAlign = T(cp[3]);
+    const T kSix = T(1.0 / 6.0), kTwoThirds = T(2.0 / 3.0);
+    for (int c = c0; c < c1; ++c) {
+      const double* cd = m.coll + 8 * c;
+      const int kind = (int)cd[0];
+      const V3<T> P = mk3(T(cd[1]), T(cd[2]), T(cd[3]));
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        T N[3];
+        N[0] = q == 2 ? kTwoThirds : kSix;
+        N[1] = q == 0 ? kTwoThirds : kSix;
+        N[2] = q == 1 ? kTwoThirds : kSix;
+        const V3<T> x = add3(add3(scl3(N[0], xs[0]), scl3(N[1], xs[1])), scl3(N[2], xs[2]));
+        T dist;
+        V3<T> gr;
+        if (kind == 0) {
+          gr = mk3(T(cd[4]), T(cd[5]), T(cd[6]));
+          dist = dot3(gr, sub3(x, P));
+        } else {
+          const V3<T> dv = sub3(x, P);
+          const T l = dsqrt(dot3(dv, dv));
+          gr = scl3(T(1) / l, dv);
+          dist = l - T(cd[7]);
+        }
+        if (!(dist <= thr)) continue;
+        if (!(dot3(gr, fnv) <= maxAlign)) continue;
+        T pen, dpen, ddpen;
+        poly_relu<T>(-dist, hp, hp - thr, pen, dpen, ddpen);
+        const T fn = kp * pen * dpen;
+        const T dfn = -kp * (dpen * dpen + pen * ddpen);
+        const T gv[3] = {gr.x, gr.y, gr.z};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) rc[3 * i + d] -= fn * gv[d] * (N[i] * w);
+        if (wantH) {
+          const T gg[6] = {gv[0] * gv[0], gv[1] * gv[1], gv[2] * gv[2], gv[0] * gv[1], gv[0] * gv[2], gv[1] * gv[2]};
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = a; b < 3; ++b) {
+              const T s = -dfn * N[a] * N[b] * w;
+              T* blk = hc + 6 * hc_block(a, b);
+#pragma unroll
+              for (int k = 0; k < 6; ++k) blk[k] += s * gg[k];
+            }
+        }
+      }
+    }
+  }
+  T* ro = Rc + (size_t)tr * kRcStride;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) ro[k] = rc[k];
+  if (wantH) {
+    T* ho = Hc + (size_t)tr * kHcStride;
+#pragma unroll
+    for (int k = 0; k < 36; ++k) ho[k] = hc[k];
+  }
+}
+
+// ------------------------------------------------------------------ gathers
+
+template <class T>
+__device__ __forceinline__ void inv3(const T a[9], T o[9]) {
+  const T c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+  const T c10 = a[2] * a[7] - a[1] * a[8], c11 = a[0] * a[8] - a[2] * a[6], c12 = a[1] * a[6] - a[0] * a[7];
+  const T c20 = a[1] * a[5] - a[2] * a[4], c21 = a[2] * a[3] - a[0] * a[5], c22 = a[0] * a[4] - a[1] * a[3];
+  const T id = T(1) / (a[0] * c00 + a[1] * c01 + a[2] * c02);
+  o[0] = c00 * id;
+  o[1] = c10 * id;
+  o[2] = c20 * id;
+  o[3] = c01 * id;
+  o[4] = c11 * id;
+  o[5] = c21 * id;
+  o[6] = c02 * id;
+  o[7] = c12 * id;
+  o[8] = c22 * id;
+}
+
+// One thread per BSR slot: sum its element and contact contributions, apply
+// Dirichlet rows/columns (mochi_solve.cpp:337-396), write the 9 SoA planes,
+// and on diagonal slots the preconditioner block.
+template <class T>
+__global__ void __launch_bounds__(256) k_gather_bsr(MeshView m, const T* __restrict__ He, const T* __restrict__ Hc,
+                                                    int prec, T* __restrict__ vals, T* __restrict__ dinv) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m.nB) return;
+  const int i = m.slot_row[s];
+  if (!m.env_mask[m.row_env[i]]) return;
+  const int j = m.col[s];
+  T acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = T(0);
+  for (int c = m.slot_ptr[s]; c < m.slot_ptr[s + 1]; ++c) {
+    const uint32_t code = m.slot_code[c];
+    const uint32_t e = code >> 4;
+    const int a = (code >> 2) & 3, b = code & 3;
+    const T* h = He + (size_t)e * kHeStride;
+    if (a == b) {
+      const T* q = h + he_block_offset(a, a);
+      acc[0] += q[0];
+      acc[4] += q[1];
+      acc[8] += q[2];
+      acc[1] += q[3];
+      acc[3] += q[3];
+      acc[2] += q[4];
+      acc[6] += q[4];
+      acc[5] += q[5];
+      acc[7] += q[5];
+    } else if (a < b) {
+      const T* q = h + he_block_offset(a, b);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) acc[k] += q[k];
+    } else {
+      const T* q = h + he_block_offset(b, a);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) acc[r * 3 + cc] += q[cc * 3 + r];
+    }
+  }
+  for (int c = m.slot_ptr_c[s]; c < m.slot_ptr_c[s + 1]; ++c) {
+    const uint32_t code = m.slot_code_c[c];
+    const uint32_t tr = code >> 4;
+    int a = (code >> 2) & 3, b = code & 3;
+    if (a > b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const T* q = Hc + (size_t)tr * kHcStride + 6 * hc_block(a, b);
+    acc[0] += q[0];
+    acc[4] += q[1];
+    acc[8] += q[2];
+    acc[1] += q[3];
+    acc[3] += q[3];
+    acc[2] += q[4];
+    acc[6] += q[4];
+    acc[5] += q[5];
+    acc[7] += q[5];
+  }
+  const uint8_t* pin = m.pinned;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (pin[3 * i + r] | pin[3 * j + c]) acc[r * 3 + c] = (i == j && r == c) ? T(1) : T(0);
+  const int nB = m.nB;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) vals[(size_t)k * nB + s] = acc[k];
+  if (i == j) {
+    T o[9];
+    if (prec == 2) {
+      inv3(acc, o);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) o[k] = T(0);
+      if (prec == 1) {
+        o[0] = T(1) / acc[0];
+        o[4] = T(1) / acc[4];
+        o[8] = T(1) / acc[8];
+      } else {
+        o[0] = o[4] = o[8] = T(1);
+      }
+    }
+    const int nV = m.nV;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dinv[(size_t)k * nV + i] = o[k];
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_block_inverse(int nV, const int32_t* __restrict__ diag_slot,
+                                                       const T* __restrict__ vals, int nB, int prec,
+                                                       T* __restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV) return;
+  const int s = diag_slot[i];
+  T a[9], o[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) a[k] = vals[(size_t)k * nB + s];
+  if (prec == 2) {
+    inv3(a, o);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[k] = T(0);
+    if (prec == 1) {
+      o[0] = T(1) / a[0];
+      o[4] = T(1) / a[4];
+      o[8] = T(1) / a[8];
+    } else {
+      o[0] = o[4] = o[8] = T(1);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dinv[(size_t)k * nV + i] = o[k];
+}
+
+// One thread per vertex: residual = sum of element and contact
+// contributions, Dirichlet rows zeroed; per-environment ||res||^2 added into
+// acc (set 0, zeroed by the caller).
+template <class T, class A>
+__global__ void __launch_bounds__(kVecThreads) k_gather_res(MeshView m, Tiles tv, const T* __restrict__ Re,
+                                                            const T* __restrict__ Rc, T* __restrict__ res, A* acc) {
+  const int tile = blockIdx.x;
+  const int env = tv.env[tile];
+  if (!m.env_mask[env]) return;
+  const int i = tv.row0[tile] + threadIdx.x;
+  A part[1] = {A(0)};
+  if (i < tv.row1[tile]) {
+    T r0 = T(0), r1 = T(0), r2 = T(0);
+    for (int c = m.vtx_ptr[i]; c < m.vtx_ptr[i + 1]; ++c) {
+      const uint32_t code = m.vtx_code[c];
+      const T* q = Re + (size_t)(code >> 2) * kReStride + 3 * (code & 3);
+      r0 += q[0];
+      r1 += q[1];
+      r2 += q[2];
+    }
+    for (int c = m.vtx_ptr_c[i]; c < m.vtx_ptr_c[i + 1]; ++c) {
+      const uint32_t code = m.vtx_code_c[c];
+      const T* q = Rc + (size_t)(code >> 2) * kRcStride + 3 * (code & 3);
+      r0 += q[0];
+      r1 += q[1];
+      r2 += q[2];
+    }
+    if (m.pinned[3 * i]) r0 = T(0);
+    if (m.pinned[3 * i + 1]) r1 = T(0);
+    if (m.pinned[3 * i + 2]) r2 = T(0);
+    res[3 * i] = r0;
+    res[3 * i + 1] = r1;
+    res[3 * i + 2] = r2;
+    part[0] = A(r0) * A(r0) + A(r1) * A(r1) + A(r2) * A(r2);
+  }
+  acc_add<A, 1>(part, acc_ptr(acc, 0, m.nEnv, env), tile - tv.first[env]);
+}
+
+// ------------------------------------------------------------------ CG
+//
+// Accumulator sets per environment (kCgSets of them):
+//   0      scaling of the stop test from b: b.b, b.M^-1 b, |M^-1 b|^2
+//   1, 2   p.Ap of CG iteration j in set 1 + j % 2
+//   3..5   r.r, r.z, z.z, r.z_old after iteration j in set 3 + (j + 1) % 3;
+//          the initial residual (j = -1) in set 3
+// cg_update of iteration j zeroes the sets that iteration j + 1 writes.
+
+constexpr int kSetScale = 0, kSetPAp = 1, kSetRes = 3;
+__host__ __device__ constexpr int res_set(int j) { return kSetRes + (j + 1 + 3) % 3; }
+
+template <class T>
+__device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int nV, int i, int prec, T r0, T r1, T r2,
+                                           T& z0, T& z1, T& z2) {
+  if (prec == 2) {
+    const T* d = dinv + i;
+    const size_t s = nV;
+    z0 = d[0] * r0 + d[1 * s] * r1 + d[2 * s] * r2;
+    z1 = d[3 * s] * r0 + d[4 * s] * r1 + d[5 * s] * r2;
+    z2 = d[6 * s] * r0 + d[7 * s] * r1 + d[8 * s] * r2;
+  } else if (prec == 1) {
+    z0 = dinv[i] * r0;
+    z1 = dinv[4 * (size_t)nV + i] * r1;
+    z2 = dinv[8 * (size_t)nV + i] * r2;
+  } else {
+    z0 = r0;
+    z1 = r1;
+    z2 = r2;
+  }
+}
+
+// StopTest (stopping_criterion.h:53-98): scaling from b, status of `latest`.
+// Returns kCgActive, kCgConverged or kCgDiverged.
+__device__ __forceinline__ int stop_test(const CgConfig& cfg, double start, double latest) {
+  const double div = (start * cfg.rel_div_tol) * cfg.rel_div_tol;
+  const double c1 = cfg.abs_tol * cfg.abs_tol, c2 = (start * cfg.rel_tol) * cfg.rel_tol;
+  const double conv = c1 > c2 ? c1 : c2;
+  if (!(latest == latest) || isinf(latest) || latest > div || latest < -conv) return kCgDiverged;
+  if ((latest < 0.0 ? 0.0 : latest) <= conv) return kCgConverged;
+  return kCgActive;
+}
+
+__device__ __forceinline__ double pick_norm(int norm_type, double rr, double rz, double zz) {
+  return norm_type == 0 ? rr : (norm_type == 1 ? zz : rz);
+}
+
+// One block per environment: zero its accumulator sets and (re)start its CG state.
+template <class A>
+__global__ void k_cg_reset(int nEnv, const int32_t* __restrict__ solve, EnvCG* env, A* acc) {
+  const int e = blockIdx.x;
+  for (int set = 0; set < kCgSets; ++set) acc_zero(acc_ptr(acc, set, nEnv, e));
+  if (threadIdx.x == 0) {
+    EnvCG E = {};
+    E.status = solve[e] ? kCgActive : kCgIdle;
+    env[e] = E;
+  }
+}
+
+// r0 = b - A x0 (or b), z0 = M^-1 r0; scaling from b (SetScaling).
+template <class T, class A>
+__global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg, const T* __restrict__ b,
+                                                         const T* __restrict__ ax0, const T* __restrict__ dinv,
+                                                         int nV, T* __restrict__ x, T* __restrict__ r,
+                                                         T* __restrict__ z, EnvCG* env, A* acc) {
+  const int tile = blockIdx.x;
+  const int e = tv.env[tile];
+  if (env[e].status != kCgActive) return;
+  const int i = tv.row0[tile] + threadIdx.x;
+  A sc[3] = {A(0), A(0), A(0)};        // b.b, b.Mb, Mb.Mb
+  A rs[4] = {A(0), A(0), A(0), A(0)};  // r.r, r.z, z.z, 0
+  if (i < tv.row1[tile]) {
+    const T b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
+    T r0 = b0, r1 = b1, r2 = b2;
+    if (cfg.zero_guess) {
+      x[3 * i] = T(0);
+      x[3 * i + 1] = T(0);
+      x[3 * i + 2] = T(0);
+    } else {
+      r0 -= ax0[3 * i];
+      r1 -= ax0[3 * i + 1];
+      r2 -= ax0[3 * i + 2];
+    }
+    T zb0, zb1, zb2, z0, z1, z2;
+    apply_dinv(dinv, nV, i, cfg.prec, b0, b1, b2, zb0, zb1, zb2);
+    apply_dinv(dinv, nV, i, cfg.prec, r0, r1, r2, z0, z1, z2);
+    r[3 * i] = r0;
+    r[3 * i + 1] = r1;
+    r[3 * i + 2] = r2;
+    z[3 * i] = z0;
+    z[3 * i + 1] = z1;
+    z[3 * i + 2] = z2;
+    sc[0] = A(b0) * A(b0) + A(b1) * A(b1) + A(b2) * A(b2);
+    sc[1] = A(b0) * A(zb0) + A(b1) * A(zb1) + A(b2) * A(zb2);
+    sc[2] = A(zb0) * A(zb0) + A(zb1) * A(zb1) + A(zb2) * A(zb2);
+    rs[0] = A(r0) * A(r0) + A(r1) * A(r1) + A(r2) * A(r2);
+    rs[1] = A(r0) * A(z0) + A(r1) * A(z1) + A(r2) * A(z2);
+    rs[2] = A(z0) * A(z0) + A(z1) * A(z1) + A(z2) * A(z2);
+  }
+  const int te = tile - tv.first[e];
+  acc_add<A, 3>(sc, acc_ptr(acc, kSetScale, cfg.n_env, e), te);
+  acc_add<A, 4>(rs, acc_ptr(acc, res_set(-1), cfg.n_env, e), te);
+}
+
+// Sum over the kSlots lines of an accumulator set, one line per lane of each
+// 16-lane group, finished with shuffles: every lane ends with the totals.
+template <class A, int NV>
+__device__ __forceinline__ void acc_read_lanes(const A* src, double (&out)[NV]) {
+  const int sl = threadIdx.x & (kSlots - 1);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) out[k] = double(src[sl * kSlotStride + k]);
+#pragma unroll
+  for (int off = kSlots / 2; off >= 1; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] += __shfl_xor(out[k], off, kSlots);
+}
+
+// Finish an accumulator read started with one slot load per lane.
+__device__ __forceinline__ double slot_reduce(double v) {
+#pragma unroll
+  for (int off = kSlots / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kSlots);
+  return v;
+}
+
+// CG iteration j, first half.  Every block re-derives from the accumulators
+// the stop test after iteration j (pcg.h:186-222) and beta_j; the first tile
+// of a finished environment records its status.  Then Ap = A p_j with
+// p_j = z_j + beta_j p_{j-1} (accumulated as A z + beta A p_{j-1}, so the
+// matrix and vector loads do not wait for beta), p_j is stored, and p_j.Ap
+// goes into set 1 + j % 2.  On refresh iterations rref = b - A x_j is also
+// accumulated in FP64.
+template <class T, class A, bool kFirst, bool kRefresh>
+__global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(Tiles ta, CgConfig cfg, int j,
+                                                          const int32_t* __restrict__ row_ptr,
+                                                          const int32_t* __restrict__ col, const T* __restrict__ vals,
+                                                          int nB, const T* __restrict__ z, T* __restrict__ p0,
+                                                          T* __restrict__ p1, const T* __restrict__ x,
+                                                          const T* __restrict__ b, T* __restrict__ Ap,
+                                                          double* __restrict__ rref, EnvCG* env, A* acc,
+                                                          int32_t* num_active) {
+  const int tile = blockIdx.x;
+  const int e = ta.env[tile];
+  const int status = env[e].status;  // branched on after the row loads are issued
+  const int nEnv = cfg.n_env;
+  constexpr bool first = kFirst;
+  T* __restrict__ pnew = (j & 1) ? p1 : p0;
+  const T* __restrict__ pold = (j & 1) ? p0 : p1;
+  constexpr bool refresh = kRefresh;
+  // raw accumulator slots (one per lane of each 16-lane group), issued first:
+  // scaling norm, latest norm, r.z, r.z_old after iteration j; r.z after j-1
+  const int sl = (threadIdx.x & (kSlots - 1)) * kSlotStride;
+  const int nsel = cfg.norm_type == 0 ? 0 : (cfg.norm_type == 1 ? 2 : 1);
+  const A* s_sc = acc_ptr(acc, kSetScale, nEnv, e) + sl;
+  const A* s_rs = acc_ptr(acc, res_set(j - 1), nEnv, e) + sl;
+  double w_start = double(s_sc[nsel]);
+  double w_latest = double(s_rs[nsel]);
+  double w_rz = double(s_rs[1]);
+  double w_rzpr = j >= 2 ? double(s_rs[3]) : 0.0;
+  double w_prev = j >= 1 ? double(acc_ptr(acc, res_set(j - 2), nEnv, e)[sl + 1]) : 0.0;
+  const int i = ta.row0[tile] + (threadIdx.x / kSpmvLanes);
+  const int lane = threadIdx.x % kSpmvLanes;
+  const bool valid = i < ta.row1[tile];
+  // matrix rows: A z and A p_old
+  T a0 = T(0), a1 = T(0), a2 = T(0), c0 = T(0), c1 = T(0), c2 = T(0);
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+  T zi0 = T(0), zi1 = T(0), zi2 = T(0), oi0 = T(0), oi1 = T(0), oi2 = T(0);
+  if (valid) {
+    const int k1 = row_ptr[i + 1];
+    for (int k = row_ptr[i] + lane; k < k1; k += kSpmvLanes) {
+      const int jc = col[k];
+      const T v0 = vals[k], v1 = vals[(size_t)nB + k], v2 = vals[2 * (size_t)nB + k];
+      const T v3 = vals[3 * (size_t)nB + k], v4 = vals[4 * (size_t)nB + k], v5 = vals[5 * (size_t)nB + k];
+      const T v6 = vals[6 * (size_t)nB + k], v7 = vals[7 * (size_t)nB + k], v8 = vals[8 * (size_t)nB + k];
+      const T zj0 = z[3 * jc], zj1 = z[3 * jc + 1], zj2 = z[3 * jc + 2];
+      a0 += v0 * zj0 + v1 * zj1 + v2 * zj2;
+      a1 += v3 * zj0 + v4 * zj1 + v5 * zj2;
+      a2 += v6 * zj0 + v7 * zj1 + v8 * zj2;
+      if (!first) {
+        const T oj0 = pold[3 * jc], oj1 = pold[3 * jc + 1], oj2 = pold[3 * jc + 2];
+        c0 += v0 * oj0 + v1 * oj1 + v2 * oj2;
+        c1 += v3 * oj0 + v4 * oj1 + v5 * oj2;
+        c2 += v6 * oj0 + v7 * oj1 + v8 * oj2;
+      }
+      if (refresh) {
+        const double x0 = x[3 * jc], x1 = x[3 * jc + 1], x2 = x[3 * jc + 2];
+        q0 += double(v0) * x0 + double(v1) * x1 + double(v2) * x2;
+        q1 += double(v3) * x0 + double(v4) * x1 + double(v5) * x2;
+        q2 += double(v6) * x0 + double(v7) * x1 + double(v8) * x2;
+      }
+    }
+    if (lane == 0) {
+      zi0 = z[3 * i];
+      zi1 = z[3 * i + 1];
+      zi2 = z[3 * i + 2];
+      if (!first) {
+        oi0 = pold[3 * i];
+        oi1 = pold[3 * i + 1];
+        oi2 = pold[3 * i + 2];
+      }
+    }
+  }
+  if (status != kCgActive) return;
+  // environment scalars: stop test after iteration j and beta_j
+  const double start = slot_reduce(w_start);
+  const double latest = slot_reduce(w_latest);
+  const double rz = slot_reduce(w_rz);
+  int st = stop_test(cfg, start, latest);
+  double beta = 0.0;
+  if (j >= 1) {
+    const double rzpr = slot_reduce(w_rzpr);  // Polak-Ribiere term, zero on the first iteration
+    const double prev = slot_reduce(w_prev);
+    if (st == kCgActive) {
+      if (rz == 0.0)
+        st = kCgDiverged;
+      else
+        beta = (rz - rzpr) / prev;
+    }
+  }
+  if (st == kCgActive && j >= cfg.max_iter) st = kCgStopped;
+  if (st != kCgActive) {
+    if (tile == ta.first[e] && threadIdx.x == 0) {
+      EnvCG& E = env[e];
+      E.status = st;
+      E.iter = j;
+      E.latest = latest < 0.0 ? 0.0 : latest;
+      E.start = start;
+      E.rz = rz;
+      atomicSub(num_active, 1);
+    }
+    return;
+  }
+  const T bt = T(beta);
+  a0 += bt * c0;
+  a1 += bt * c1;
+  a2 += bt * c2;
+#pragma unroll
+  for (int off = kSpmvLanes / 2; off >= 1; off >>= 1) {
+    a0 += __shfl_xor(a0, off, kSpmvLanes);
+    a1 += __shfl_xor(a1, off, kSpmvLanes);
+    a2 += __shfl_xor(a2, off, kSpmvLanes);
+    if (refresh) {
+      q0 += __shfl_xor(q0, off, kSpmvLanes);
+      q1 += __shfl_xor(q1, off, kSpmvLanes);
+      q2 += __shfl_xor(q2, off, kSpmvLanes);
+    }
+  }
+  A part[1] = {A(0)};
+  if (valid && lane == 0) {
+    const T pi0 = zi0 + bt * oi0, pi1 = zi1 + bt * oi1, pi2 = zi2 + bt * oi2;
+    pnew[3 * i] = pi0;
+    pnew[3 * i + 1] = pi1;
+    pnew[3 * i + 2] = pi2;
+    Ap[3 * i] = a0;
+    Ap[3 * i + 1] = a1;
+    Ap[3 * i + 2] = a2;
+    part[0] = A(pi0) * A(a0) + A(pi1) * A(a1) + A(pi2) * A(a2);
+    if (refresh) {
+      rref[3 * i] = double(b[3 * i]) - q0;
+      rref[3 * i + 1] = double(b[3 * i + 1]) - q1;
+      rref[3 * i + 2] = double(b[3 * i + 2]) - q2;
+    }
+  }
+  acc_add<A, 1>(part, acc_ptr(acc, kSetPAp + (j & 1), nEnv, e), tile - ta.first[e]);
+}
+
+// CG iteration j, second half: alpha_j = r.z / p.Ap; x += alpha p; r -= alpha Ap
+// (or r = rref - alpha Ap on refresh iterations); z = M^-1 r; r.r, r.z, z.z and
+// r.z_old into set res_set(j).  Row loads are issued before alpha is known.
+template <class T, class A, bool kRefresh>
+__global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, CgConfig cfg, int j, const T* __restrict__ dinv,
+                                                           int nV, const T* __restrict__ p0, const T* __restrict__ p1,
+                                                           const T* __restrict__ Ap, const double* __restrict__ rref,
+                                                           T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                           EnvCG* env, A* acc, int32_t* num_active) {
+  const int tile = blockIdx.x;
+  const int e = tv.env[tile];
+  const int status = env[e].status;  // branched on after the row loads are issued
+  const int nEnv = cfg.n_env;
+  const T* __restrict__ p = (j & 1) ? p1 : p0;
+  constexpr bool refresh = kRefresh;
+  const int sl = (threadIdx.x & (kSlots - 1)) * kSlotStride;
+  double w_pap = double(acc_ptr(acc, kSetPAp + (j & 1), nEnv, e)[sl]);
+  double w_rz = double(acc_ptr(acc, res_set(j - 1), nEnv, e)[sl + 1]);
+  const int i = tv.row0[tile] + threadIdx.x;
+  const bool valid = i < tv.row1[tile];
+  T x0 = T(0), x1 = T(0), x2 = T(0), pp0 = T(0), pp1 = T(0), pp2 = T(0), ap0 = T(0), ap1 = T(0), ap2 = T(0);
+  double ro0 = 0.0, ro1 = 0.0, ro2 = 0.0;
+  T zo0 = T(0), zo1 = T(0), zo2 = T(0);
+  if (valid) {
+    x0 = x[3 * i];
+    x1 = x[3 * i + 1];
+    x2 = x[3 * i + 2];
+    pp0 = p[3 * i];
+    pp1 = p[3 * i + 1];
+    pp2 = p[3 * i + 2];
+    ap0 = Ap[3 * i];
+    ap1 = Ap[3 * i + 1];
+    ap2 = Ap[3 * i + 2];
+    if (refresh) {
+      ro0 = rref[3 * i];
+      ro1 = rref[3 * i + 1];
+      ro2 = rref[3 * i + 2];
+    } else {
+      ro0 = r[3 * i];
+      ro1 = r[3 * i + 1];
+      ro2 = r[3 * i + 2];
+    }
+    zo0 = z[3 * i];
+    zo1 = z[3 * i + 1];
+    zo2 = z[3 * i + 2];
+  }
+  if (status != kCgActive) return;
+  const double pap = slot_reduce(w_pap);
+  const double rzc = slot_reduce(w_rz);
+  const bool lead = tile == tv.first[e];
+  if (pap <= 0.0 && cfg.abort_if_not_spd) {
+    // pcg.h:152-169: not SPD, abort with iteration j + 1
+    if (lead && threadIdx.x == 0) {
+      EnvCG& E = env[e];
+      E.status = kCgDiverged;
+      E.iter = j + 1;
+      atomicSub(num_active, 1);
+    }
+    return;
+  }
+  if (lead) {
+    // the sets iteration j + 1 writes
+    acc_zero(acc_ptr(acc, res_set(j + 1), nEnv, e));
+    acc_zero(acc_ptr(acc, kSetPAp + ((j + 1) & 1), nEnv, e));
+  }
+  const double alpha = rzc / pap;
+  const T al = T(alpha);
+  A part[4] = {A(0), A(0), A(0), A(0)};  // r.r, r.z, z.z, r.z_old
+  if (valid) {
+    x[3 * i] = x0 + al * pp0;
+    x[3 * i + 1] = x1 + al * pp1;
+    x[3 * i + 2] = x2 + al * pp2;
+    T r0, r1, r2;
+    if (refresh) {
+      r0 = T(ro0 - alpha * double(ap0));
+      r1 = T(ro1 - alpha * double(ap1));
+      r2 = T(ro2 - alpha * double(ap2));
+    } else {
+      r0 = T(ro0) - al * ap0;
+      r1 = T(ro1) - al * ap1;
+      r2 = T(ro2) - al * ap2;
+    }
+    r[3 * i] = r0;
+    r[3 * i + 1] = r1;
+    r[3 * i + 2] = r2;
+    T z0, z1, z2;
+    apply_dinv(dinv, nV, i, cfg.prec, r0, r1, r2, z0, z1, z2);
+    z[3 * i] = z0;
+    z[3 * i + 1] = z1;
+    z[3 * i + 2] = z2;
+    part[0] = A(r0) * A(r0) + A(r1) * A(r1) + A(r2) * A(r2);
+    part[1] = A(r0) * A(z0) + A(r1) * A(z1) + A(r2) * A(z2);
+    part[2] = A(z0) * A(z0) + A(z1) * A(z1) + A(z2) * A(z2);
+    part[3] = A(r0) * A(zo0) + A(r1) * A(zo1) + A(r2) * A(zo2);
+  }
+  acc_add<A, 4>(part, acc_ptr(acc, res_set(j), nEnv, e), tile - tv.first[e]);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kSpmvThreads) k_spmv_plain(Tiles ta, const int32_t* __restrict__ row_ptr,
+                                                             const int32_t* __restrict__ col,
+                                                             const T* __restrict__ vals, int nB,
+                                                             const T* __restrict__ x, T* __restrict__ y) {
+  const int tile = blockIdx.x;
+  const int i = ta.row0[tile] + (threadIdx.x / kSpmvLanes);
+  const int lane = threadIdx.x % kSpmvLanes;
+  T a0 = T(0), a1 = T(0), a2 = T(0);
+  const bool valid = i < ta.row1[tile];
+  if (valid) {
+    for (int k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += kSpmvLanes) {
+      const int j = col[k];
+      const T x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+      a0 += vals[k] * x0 + vals[(size_t)nB + k] * x1 + vals[2 * (size_t)nB + k] * x2;
+      a1 += vals[3 * (size_t)nB + k] * x0 + vals[4 * (size_t)nB + k] * x1 + vals[5 * (size_t)nB + k] * x2;
+      a2 += vals[6 * (size_t)nB + k] * x0 + vals[7 * (size_t)nB + k] * x1 + vals[8 * (size_t)nB + k] * x2;
+    }
+  }
+#pragma unroll
+  for (int off = kSpmvLanes / 2; off >= 1; off >>= 1) {
+    a0 += __shfl_xor(a0, off, kSpmvLanes);
+    a1 += __shfl_xor(a1, off, kSpmvLanes);
+    a2 += __shfl_xor(a2, off, kSpmvLanes);
+  }
+  if (valid && lane == 0) {
+    y[3 * i] = a0;
+    y[3 * i + 1] = a1;
+    y[3 * i + 2] = a2;
+  }
+}
+
+// ------------------------------------------------------------------ vectors
+
+template <class T>
+__global__ void k_copy(int64_t n, const T* __restrict__ s, T* __restrict__ d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = s[i];
+}
+
+template <class T>
+__global__ void k_apply_pins(int64_t n, const uint8_t* __restrict__ pin, const T* __restrict__ val, T* __restrict__ u) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && pin[i]) u[i] = val[i];
+}
+
+template <class T>
+__global__ void k_ls_update(int nV, const int32_t* __restrict__ row_env, const int32_t* __restrict__ mask,
+                            const double* __restrict__ scale, const T* __restrict__ sol0, const T* __restrict__ dx,
+                            T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV) return;
+  const int e = row_env[i];
+  if (!mask[e]) return;
+  const T s = T(scale[e]);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) x[3 * i + d] = sol0[3 * i + d] - s * dx[3 * i + d];
+}
+
+template <class T>
+__global__ void k_post_step(int nV, const int32_t* __restrict__ row_env, const int32_t* __restrict__ reset, T idt,
+                            const T* __restrict__ u0, T* __restrict__ u, T* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV) return;
+  const bool rs = reset[row_env[i]] != 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const T uu = u[3 * i + d];
+    v[3 * i + d] = rs ? T(0) : (uu - u0[3 * i + d]) * idt;
+    if (rs) u[3 * i + d] = T(0);
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kVecThreads) k_env_norms(Tiles tv, int nEnv, const T* __restrict__ a,
+                                                           const T* __restrict__ b, double* acc) {
+  const int tile = blockIdx.x;
+  const int e = tv.env[tile];
+  const int i = tv.row0[tile] + threadIdx.x;
+  double part[2] = {0.0, 0.0};
+  if (i < tv.row1[tile]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double av = a[3 * i + d], bv = b[3 * i + d];
+      part[0] += av * av;
+      part[1] += bv * bv;
+    }
+  }
+  acc_add<double, 2>(part, acc_ptr(acc, 0, nEnv, e), tile - tv.first[e]);
+}
+
+// Sum the kSlots lines of set 0 of each environment: out[e * nv + k].
+template <class A>
+__global__ void k_acc_finalize(int nEnv, int nv, const A* __restrict__ acc, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nEnv) return;
+  const A* src = acc + (size_t)e * kSlots * kSlotStride;
+  for (int k = 0; k < nv; ++k) {
+    double s = 0.0;
+    for (int sl = 0; sl < kSlots; ++sl) s += double(src[sl * kSlotStride + k]);
+    out[(size_t)e * nv + k] = s;
+  }
+}
+
+template <class A>
+__global__ void k_fill(int64_t n, A v, A* __restrict__ d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = v;
+}
+
+// ------------------------------------------------------------------ launchers
+
+static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <class T>
+void launch_element(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0, const T* v0,
+                    double dt, const double grav[3], int wantH, int project, T* Re, T* He) {
+  if (m.nT == 0) return;
+  hipLaunchKernelGGL(k_element<T>, dim3(blocks_for(m.nT, 128)), dim3(128), 0, s, m, f, u, u0, v0, T(dt),
+                     T(grav[0]), T(grav[1]), T(grav[2]), wantH, project, Re, He);
+}
+
+template <class T>
+void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, int wantH, T* Rc, T* Hc) {
+  if (m.nF == 0) return;
+  hipLaunchKernelGGL(k_contact<T>, dim3(blocks_for(m.nF, 256)), dim3(256), 0, s, m, f, u, wantH, Rc, Hc);
+}
+
+template <class T>
+void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* He, const T* Hc, int prec, T* vals, T* dinv) {
+  if (m.nB == 0) return;
+  hipLaunchKernelGGL(k_gather_bsr<T>, dim3(blocks_for(m.nB, 256)), dim3(256), 0, s, m, He, Hc, prec, vals, dinv);
+}
+
+template <class T, class A>
+void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Re, const T* Rc, T* res, A* acc) {
+  if (tv.n == 0) return;
+  hipLaunchKernelGGL((k_gather_res<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, m, tv, Re, Rc, res, acc);
+}
+
+template <class A>
+void launch_fill(hipStream_t s, int64_t n, A v, A* d) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill<A>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, v, d);
+}
+
+template <class A>
+void launch_acc_finalize(hipStream_t s, int nEnv, int nv, const A* acc, double* out) {
+  if (nEnv == 0) return;
+  hipLaunchKernelGGL(k_acc_finalize<A>, dim3(blocks_for(nEnv, 64)), dim3(64), 0, s, nEnv, nv, acc, out);
+}
+
+template <class A>
+void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, A* acc) {
+  if (nEnv == 0) return;
+  hipLaunchKernelGGL(k_cg_reset<A>, dim3(nEnv), dim3(256), 0, s, nEnv, solve, env, acc);
+}
+
+template <class T, class A>
+void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const T* ax0, const T* dinv,
+                    int32_t nV, T* x, T* r, T* z, EnvCG* env, A* acc) {
+  if (tv.n == 0) return;
+  hipLaunchKernelGGL((k_cg_init<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, b, ax0, dinv, nV, x, r, z,
+                     env, acc);
+}
+
+template <class T, class A>
+void launch_cg_spmv(hipStream_t s, const Tiles& ta, const CgConfig& cfg, int j, const int32_t* row_ptr,
+                    const int32_t* col, const T* vals, int32_t nB, const T* z, T* p0, T* p1, const T* x, const T* b,
+                    T* Ap, double* rref, EnvCG* env, A* acc, int32_t* num_active) {
+  if (ta.n == 0) return;
+  const bool refresh = cfg.refresh_every > 0 && ((j + 1) % cfg.refresh_every) == 0;
+#define SDX_SPMV(F, R)                                                                                          \
+  hipLaunchKernelGGL((k_cg_spmv<T, A, F, R>), dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, cfg, j, row_ptr, col, vals, \
+                     nB, z, p0, p1, x, b, Ap, rref, env, acc, num_active)
+  if (j == 0) {
+    if (refresh) SDX_SPMV(true, true); else SDX_SPMV(true, false);
+  } else {
+    if (refresh) SDX_SPMV(false, true); else SDX_SPMV(false, false);
+  }
+#undef SDX_SPMV
+}
+
+template <class T, class A>
+void launch_cg_update(hipStream_t s, const Tiles& tv, const CgConfig& cfg, int j, const T* dinv, int32_t nV,
+                      const T* p0, const T* p1, const T* Ap, const double* rref, T* x, T* r, T* z, EnvCG* env, A* acc,
+                      int32_t* num_active) {
+  if (tv.n == 0) return;
+  const bool refresh = cfg.refresh_every > 0 && ((j + 1) % cfg.refresh_every) == 0;
+  if (refresh)
+    hipLaunchKernelGGL((k_cg_update<T, A, true>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, j, dinv, nV, p0, p1,
+                       Ap, rref, x, r, z, env, acc, num_active);
+  else
+    hipLaunchKernelGGL((k_cg_update<T, A, false>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, j, dinv, nV, p0, p1,
+                       Ap, rref, x, r, z, env, acc, num_active);
+}
+
+template <class T>
+void launch_spmv_plain(hipStream_t s, const Tiles& ta, const int32_t* row_ptr, const int32_t* col, const T* vals,
+                       int32_t nB, const T* x, T* y) {
+  if (ta.n == 0) return;
+  hipLaunchKernelGGL(k_spmv_plain<T>, dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, row_ptr, col, vals, nB, x, y);
+}
+
+template <class T>
+void launch_block_inverse(hipStream_t s, int32_t nV, const int32_t* diag_slot, const T* vals, int32_t nB, int prec,
+                          T* dinv) {
+  if (nV == 0) return;
+  hipLaunchKernelGGL(k_block_inverse<T>, dim3(blocks_for(nV, 256)), dim3(256), 0, s, nV, diag_slot, vals, nB, prec,
+                     dinv);
+}
+
+template <class T>
+void launch_copy(hipStream_t s, int64_t n, const T* src, T* dst) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_copy<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, src, dst);
+}
+
+template <class T>
+void launch_apply_pins(hipStream_t s, int64_t n, const uint8_t* pinned, const T* value, T* u) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_apply_pins<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, pinned, value, u);
+}
+
+template <class T>
+void launch_ls_update(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* ls_mask, const double* scale,
+                      const T* sol0, const T* dx, T* x) {
+  if (nV == 0) return;
+  hipLaunchKernelGGL(k_ls_update<T>, dim3(blocks_for(nV, 256)), dim3(256), 0, s, nV, row_env, ls_mask, scale, sol0,
+                     dx, x);
+}
+
+template <class T>
+void launch_post_step(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* reset, double dt,
+                      const T* u0, T* u, T* v) {
+  if (nV == 0) return;
+  hipLaunchKernelGGL(k_post_step<T>, dim3(blocks_for(nV, 256)), dim3(256), 0, s, nV, row_env, reset,
+                     T(1.0 / dt), u0, u, v);
+}
+
+template <class T>
+void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, const T* b, double* acc) {
+  if (tv.n == 0) return;
+  hipLaunchKernelGGL(k_env_norms<T>, dim3(tv.n), dim3(kVecThreads), 0, s, tv, nEnv, a, b, acc);
+}
+
+#define SDX_INST_T(T)                                                                                            \
+  template void launch_element<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, const T*,        \
+                                  const T*, double, const double*, int, int, T*, T*);                           \
+  template void launch_contact<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, int, T*, T*);    \
+  template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const T*, const T*, int, T*, T*);            \
+  template void launch_spmv_plain<T>(hipStream_t, const Tiles&, const int32_t*, const int32_t*, const T*,       \
+                                     int32_t, const T*, T*);                                                    \
+  template void launch_block_inverse<T>(hipStream_t, int32_t, const int32_t*, const T*, int32_t, int, T*);     \
+  template void launch_copy<T>(hipStream_t, int64_t, const T*, T*);                                             \
+  template void launch_apply_pins<T>(hipStream_t, int64_t, const uint8_t*, const T*, T*);                       \
+  template void launch_ls_update<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, const double*,        \
+                                    const T*, const T*, T*);                                                    \
+  template void launch_post_step<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, double, const T*, T*, \
+                                    T*);                                                                        \
+  template void launch_env_norms<T>(hipStream_t, const Tiles&, int, const T*, const T*, double*);              \
+  template void launch_fill<T>(hipStream_t, int64_t, T, T*);                                                    \
+  template void launch_acc_finalize<T>(hipStream_t, int, int, const T*, double*);                               \
+  template void launch_cg_reset<T>(hipStream_t, int, const int32_t*, EnvCG*, T*);
+
+#define SDX_INST_TA(T, A)                                                                                         \
+  template void launch_gather_res<T, A>(hipStream_t, const MeshView&, const Tiles&, const T*, const T*, T*, A*); \
+  template void launch_cg_init<T, A>(hipStream_t, const Tiles&, const CgConfig&, const T*, const T*, const T*,   \
+                                     int32_t, T*, T*, T*, EnvCG*, A*);                                           \
+  template void launch_cg_spmv<T, A>(hipStream_t, const Tiles&, const CgConfig&, int, const int32_t*,            \
+                                     const int32_t*, const T*, int32_t, const T*, T*, T*, const T*, const T*,    \
+                                     T*, double*, EnvCG*, A*, int32_t*);                                         \
+  template void launch_cg_update<T, A>(hipStream_t, const Tiles&, const CgConfig&, int, const T*, int32_t,       \
+                                       const T*, const T*, const T*, const double*, T*, T*, T*, EnvCG*, A*,      \
+                                       int32_t*);
+
+SDX_INST_T(float)
+SDX_INST_T(double)
+SDX_INST_TA(float, float)
+SDX_INST_TA(float, double)
+SDX_INST_TA(double, double)
+
+}  // namespace sdx

@@ -1,0 +1,838 @@
+// Host side of the MI355X soft-body FEM step: world set-up, the Newton /
+// line-search / CG drivers, and the C ABI of include/sdx_fem.h.
+//
+// Reference call surface replaced (paths under superdex_physics/libraries/mochi/):
+//   Scene::Step ............... mochi_physics/src/mochi_scene.cpp:594
+//   StepIslandNewtonAsync ..... mochi_physics/src/mochi_solve.cpp:1090-1160
+//   NewtonSolver<T>::Solve .... mochi_core/src/solvers/newton_solver.cpp:603-876
+//   LineSearch::ResidualNorm .. mochi_core/include/mochi_core/solvers/line_search.h:300-335
+//   LinearSolver::Solve (CG) .. mochi_core/include/mochi_core/solvers/linear_solver.h:321-496
+//   soft stage integration .... mochi_physics/src/mochi_soft.cpp:94-163
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sdx_fem.h"
+#include "sdx_kernels.h"
+#include "sdx_world.h"
+
+namespace sdx {
+
+thread_local std::string g_last_error;
+
+sdx_status set_error(sdx_status code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ set-up helpers
+
+namespace {
+
+struct Pattern {
+  std::vector<int32_t> ptr, col;
+};
+
+// BSR pattern of one actor: node-to-node adjacency through tets, sorted.
+Pattern BuildPattern(int nV, const std::vector<int32_t>& tets) {
+  const int nT = int(tets.size() / 4);
+  std::vector<std::vector<int32_t>> adj(static_cast<size_t>(nV));
+  for (int i = 0; i < nV; ++i) adj[size_t(i)].push_back(i);
+  for (int t = 0; t < nT; ++t)
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) adj[size_t(tets[4 * t + a])].push_back(tets[4 * t + b]);
+  Pattern p;
+  p.ptr.assign(size_t(nV) + 1, 0);
+  for (int i = 0; i < nV; ++i) {
+    auto& v = adj[size_t(i)];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    p.ptr[size_t(i) + 1] = p.ptr[size_t(i)] + int32_t(v.size());
+  }
+  p.col.reserve(size_t(p.ptr.back()));
+  for (int i = 0; i < nV; ++i) p.col.insert(p.col.end(), adj[size_t(i)].begin(), adj[size_t(i)].end());
+  return p;
+}
+
+int FindSlot(const Pattern& p, int i, int j) {
+  auto b = p.col.begin() + p.ptr[size_t(i)], e = p.col.begin() + p.ptr[size_t(i) + 1];
+  auto it = std::lower_bound(b, e, j);
+  if (it == e || *it != j) throw std::runtime_error("BSR slot not in pattern");
+  return int(it - p.col.begin());
+}
+
+// Boundary triangles: tet faces used once, oriented with the normal away from
+// the opposite vertex.
+void BuildBoundary(const std::vector<double>& X, const std::vector<int32_t>& tets, std::vector<int32_t>& tris) {
+  struct Face {
+    std::array<int32_t, 3> key, nodes;
+    int32_t opp;
+  };
+  const int nT = int(tets.size() / 4);
+  static const int fv[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+  std::vector<Face> faces(size_t(4) * size_t(nT));
+  for (int t = 0; t < nT; ++t)
+    for (int f = 0; f < 4; ++f) {
+      Face& F = faces[size_t(4 * t + f)];
+      for (int i = 0; i < 3; ++i) F.nodes[size_t(i)] = tets[size_t(4 * t + fv[f][i])];
+      F.opp = tets[size_t(4 * t + f)];
+      F.key = F.nodes;
+      std::sort(F.key.begin(), F.key.end());
+    }
+  std::sort(faces.begin(), faces.end(), [](const Face& a, const Face& b) { return a.key < b.key; });
+  for (size_t i = 0; i < faces.size();) {
+    size_t j = i;
+    while (j < faces.size() && faces[j].key == faces[i].key) ++j;
+    if (j - i == 1) {
+      Face F = faces[i];
+      double e1[3], e2[3], w[3];
+      for (int d = 0; d < 3; ++d) {
+        e1[d] = X[size_t(3 * F.nodes[1] + d)] - X[size_t(3 * F.nodes[0] + d)];
+        e2[d] = X[size_t(3 * F.nodes[2] + d)] - X[size_t(3 * F.nodes[0] + d)];
+        w[d] = X[size_t(3 * F.opp + d)] - X[size_t(3 * F.nodes[0] + d)];
+      }
+      const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                           e1[0] * e2[1] - e1[1] * e2[0]};
+      if (n[0] * w[0] + n[1] * w[1] + n[2] * w[2] > 0) std::swap(F.nodes[1], F.nodes[2]);
+      tris.insert(tris.end(), F.nodes.begin(), F.nodes.end());
+    }
+    i = j;
+  }
+}
+
+// Stable counting sort of (key, code) pairs into CSR form.
+void CountingCsr(int nKeys, const std::vector<int32_t>& keys, const std::vector<uint32_t>& codes,
+                 std::vector<int32_t>& ptr, std::vector<uint32_t>& out) {
+  ptr.assign(size_t(nKeys) + 1, 0);
+  for (int32_t k : keys) ptr[size_t(k) + 1]++;
+  for (int i = 0; i < nKeys; ++i) ptr[size_t(i) + 1] += ptr[size_t(i)];
+  out.resize(codes.size());
+  std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+  for (size_t i = 0; i < keys.size(); ++i) out[size_t(pos[size_t(keys[i])]++)] = codes[i];
+}
+
+void BuildTiles(const std::vector<int32_t>& env_v0, int nEnv, int rows_per_tile, HostTiles& t) {
+  t.env.clear();
+  t.row0.clear();
+  t.row1.clear();
+  t.first.assign(size_t(nEnv) + 1, 0);
+  for (int e = 0; e < nEnv; ++e) {
+    t.first[size_t(e)] = int32_t(t.env.size());
+    for (int r = env_v0[size_t(e)]; r < env_v0[size_t(e) + 1]; r += rows_per_tile) {
+      t.env.push_back(e);
+      t.row0.push_back(r);
+      t.row1.push_back(std::min(r + rows_per_tile, env_v0[size_t(e) + 1]));
+    }
+  }
+  t.first[size_t(nEnv)] = int32_t(t.env.size());
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ World<T>
+
+template <class T>
+World<T>::World(int device, int precision) : WorldBase(device, precision) {}
+
+template <class T>
+World<T>::~World() {
+  release_device();
+}
+
+template <class T>
+void World<T>::release_device() {
+  for (auto* b : bufs_) b->free();
+  finalized_ = false;
+}
+
+template <class T>
+void World<T>::pull_state() {
+  if (!finalized_) return;
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  std::vector<T> hu(size_t(3 * nV_)), hv(size_t(3 * nV_));
+  u_.download(hu);
+  v_.download(hv);
+  for (size_t a = 0; a < actors_.size(); ++a) {
+    auto& A = actors_[a];
+    for (int k = 0; k < 3 * A.nV; ++k) {
+      A.u[size_t(k)] = double(hu[size_t(3 * env_v0_[a] + k)]);
+      A.v[size_t(k)] = double(hv[size_t(3 * env_v0_[a] + k)]);
+    }
+  }
+}
+
+template <class T>
+void World<T>::push_state() {
+  std::vector<T> hu(size_t(3 * nV_)), hv(size_t(3 * nV_));
+  for (size_t a = 0; a < actors_.size(); ++a) {
+    auto& A = actors_[a];
+    for (int k = 0; k < 3 * A.nV; ++k) {
+      hu[size_t(3 * env_v0_[a] + k)] = T(A.u[size_t(k)]);
+      hv[size_t(3 * env_v0_[a] + k)] = T(A.v[size_t(k)]);
+    }
+  }
+  u_.upload(hu);
+  v_.upload(hv);
+}
+
+template <class T>
+void World<T>::finalize() {
+  if (finalized_) return;
+  if (actors_.empty()) throw std::invalid_argument("world has no soft actors");
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  const int nEnv = int(actors_.size());
+  nEnv_ = nEnv;
+  env_v0_.assign(size_t(nEnv) + 1, 0);
+  std::vector<int32_t> env_t0(size_t(nEnv) + 1, 0);
+  for (int e = 0; e < nEnv; ++e) {
+    env_v0_[size_t(e) + 1] = env_v0_[size_t(e)] + actors_[size_t(e)].nV;
+    env_t0[size_t(e) + 1] = env_t0[size_t(e)] + actors_[size_t(e)].nT;
+  }
+  nV_ = env_v0_.back();
+  nT_ = env_t0.back();
+  // global pattern, tets, geometry
+  std::vector<int32_t> row_ptr(1, 0), col, slot_row, diag_slot(static_cast<size_t>(nV_)), row_env(static_cast<size_t>(nV_));
+  std::vector<int32_t> tets(size_t(4) * size_t(nT_)), tet_env(static_cast<size_t>(nT_));
+  std::vector<T> geom(size_t(kGeomPlanes) * size_t(nT_));
+  std::vector<T> X(size_t(3) * size_t(nV_));
+  std::vector<uint8_t> pinned(size_t(3) * size_t(nV_), 0);
+  std::vector<T> pin_value(size_t(3) * size_t(nV_), T(0));
+  std::vector<int32_t> tris, tri_env;
+  std::vector<int32_t> slot_keys;
+  std::vector<uint32_t> slot_codes;
+  std::vector<int32_t> vtx_keys;
+  std::vector<uint32_t> vtx_codes;
+  std::vector<int32_t> slot_keys_c, vtx_keys_c;
+  std::vector<uint32_t> slot_codes_c, vtx_codes_c;
+  std::vector<double> contact(size_t(4) * size_t(nEnv));
+  for (int e = 0; e < nEnv; ++e) {
+    auto& A = actors_[size_t(e)];
+    const int v0 = env_v0_[size_t(e)], t0 = env_t0[size_t(e)];
+    Pattern P = BuildPattern(A.nV, A.tets);
+    const int b0 = int(col.size());
+    for (int i = 0; i < A.nV; ++i) {
+      row_env[size_t(v0 + i)] = e;
+      for (int k = P.ptr[size_t(i)]; k < P.ptr[size_t(i) + 1]; ++k) {
+        col.push_back(v0 + P.col[size_t(k)]);
+        slot_row.push_back(v0 + i);
+        if (P.col[size_t(k)] == i) diag_slot[size_t(v0 + i)] = b0 + k;
+      }
+      row_ptr.push_back(int32_t(col.size()));
+    }
+    for (int k = 0; k < 3 * A.nV; ++k) {
+      X[size_t(3 * v0 + k)] = T(A.X[size_t(k)]);
+      pinned[size_t(3 * v0 + k)] = A.pinned[size_t(k)];
+      pin_value[size_t(3 * v0 + k)] = T(A.pin_value[size_t(k)]);
+    }
+    const double lam0 = A.params.youngs_modulus * A.params.poisson_ratio /
+                        ((1 + A.params.poisson_ratio) * (1 - 2 * A.params.poisson_ratio));
+    const double mu0 = A.params.youngs_modulus / (2 * (1 + A.params.poisson_ratio));
+    for (int t = 0; t < A.nT; ++t) {
+      const int gt = t0 + t;
+      tet_env[size_t(gt)] = e;
+      const int32_t* tv = &A.tets[size_t(4 * t)];
+      for (int a = 0; a < 4; ++a) tets[size_t(4 * gt + a)] = v0 + tv[a];
+      // dMap columns X_f - X_0, dBasis = dMap^{-T} dN_param (finite_element.h:95-125)
+      double D[9];
+      for (int d = 0; d < 3; ++d)
+        for (int f = 0; f < 3; ++f)
+          D[d * 3 + f] = A.X[size_t(3 * tv[f + 1] + d)] - A.X[size_t(3 * tv[0] + d)];
+      const double det = D[0] * (D[4] * D[8] - D[5] * D[7]) - D[1] * (D[3] * D[8] - D[5] * D[6]) +
+                         D[2] * (D[3] * D[7] - D[4] * D[6]);
+      if (!(det > 0)) throw std::invalid_argument("tet with non-positive volume in actor " + std::to_string(e));
+      const double cof[9] = {D[4] * D[8] - D[5] * D[7], D[5] * D[6] - D[3] * D[8], D[3] * D[7] - D[4] * D[6],
+                             D[2] * D[7] - D[1] * D[8], D[0] * D[8] - D[2] * D[6], D[1] * D[6] - D[0] * D[7],
+                             D[1] * D[5] - D[2] * D[4], D[2] * D[3] - D[0] * D[5], D[0] * D[4] - D[1] * D[3]};
+      // dN_f (f = 1..3) = row f-1 of dMap^{-1} = column of cof / det
+      for (int f = 0; f < 3; ++f)
+        for (int r = 0; r < 3; ++r) geom[size_t(kGdN + 3 * f + r) * size_t(nT_) + size_t(gt)] = T(cof[r * 3 + f] / det);
+      const double vol = det / 6.0;
+      double mu = mu0, lam = lam0;
+      if (!A.E.empty()) {
+        const double E = A.E[size_t(t)], nu = A.nu[size_t(t)];
+        lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+        mu = E / (2 * (1 + nu));
+      }
+      geom[size_t(kGVol) * size_t(nT_) + size_t(gt)] = T(vol);
+      geom[size_t(kGMu) * size_t(nT_) + size_t(gt)] = T(mu);
+      geom[size_t(kGLambda) * size_t(nT_) + size_t(gt)] = T(lam);
+      geom[size_t(kGMass) * size_t(nT_) + size_t(gt)] = T(A.params.density * vol);
+      geom[size_t(kGAlpha) * size_t(nT_) + size_t(gt)] = T(A.params.mass_damping);
+      for (int a = 0; a < 4; ++a) {
+        vtx_keys.push_back(v0 + tv[a]);
+        vtx_codes.push_back(uint32_t(gt) * 4u + uint32_t(a));
+        for (int b = 0; b < 4; ++b) {
+          slot_keys.push_back(b0 + FindSlot(P, tv[a], tv[b]));
+          slot_codes.push_back(uint32_t(gt) * 16u + uint32_t(a) * 4u + uint32_t(b));
+        }
+      }
+    }
+    std::vector<int32_t> atris;
+    BuildBoundary(A.X, A.tets, atris);
+    const int f0 = int(tri_env.size());
+    for (size_t f = 0; f < atris.size() / 3; ++f) {
+      const int gf = f0 + int(f);
+      tri_env.push_back(e);
+      for (int a = 0; a < 3; ++a) {
+        tris.push_back(v0 + atris[3 * f + size_t(a)]);
+        vtx_keys_c.push_back(v0 + atris[3 * f + size_t(a)]);
+        vtx_codes_c.push_back(uint32_t(gf) * 4u + uint32_t(a));
+        for (int b = 0; b < 3; ++b) {
+          slot_keys_c.push_back(b0 + FindSlot(P, atris[3 * f + size_t(a)], atris[3 * f + size_t(b)]));
+          slot_codes_c.push_back(uint32_t(gf) * 16u + uint32_t(a) * 4u + uint32_t(b));
+        }
+      }
+    }
+    contact[size_t(4 * e + 0)] = A.params.penalty_coefficient;
+    contact[size_t(4 * e + 1)] = A.params.penalty_smoothing_half_distance;
+    contact[size_t(4 * e + 2)] = A.params.penalty_threshold;
+    contact[size_t(4 * e + 3)] = A.params.max_alignment_normals;
+  }
+  nB_ = int(col.size());
+  nF_ = int(tri_env.size());
+  h_row_ptr_ = row_ptr;
+  h_col_ = col;
+  if (int64_t(nT_) * 16 >= (int64_t(1) << 32) || int64_t(nF_) * 16 >= (int64_t(1) << 32))
+    throw std::invalid_argument("mesh too large for 32-bit contribution codes");
+  std::vector<int32_t> slot_ptr, vtx_ptr, slot_ptr_c, vtx_ptr_c;
+  std::vector<uint32_t> slot_code, vtx_code, slot_code_c, vtx_code_c;
+  CountingCsr(nB_, slot_keys, slot_codes, slot_ptr, slot_code);
+  CountingCsr(nV_, vtx_keys, vtx_codes, vtx_ptr, vtx_code);
+  CountingCsr(nB_, slot_keys_c, slot_codes_c, slot_ptr_c, slot_code_c);
+  CountingCsr(nV_, vtx_keys_c, vtx_codes_c, vtx_ptr_c, vtx_code_c);
+  BuildTiles(env_v0_, nEnv, kVecThreads, tiles_v_);
+  BuildTiles(env_v0_, nEnv, kSpmvRows, tiles_a_);
+
+  // upload
+  d_tets_.upload(tets);
+  d_tet_env_.upload(tet_env);
+  d_tris_.upload(tris);
+  d_tri_env_.upload(tri_env);
+  d_row_env_.upload(row_env);
+  d_row_ptr_.upload(row_ptr);
+  d_col_.upload(col);
+  d_slot_row_.upload(slot_row);
+  d_diag_slot_.upload(diag_slot);
+  d_slot_ptr_.upload(slot_ptr);
+  d_slot_code_.upload(slot_code);
+  d_slot_ptr_c_.upload(slot_ptr_c);
+  d_slot_code_c_.upload(slot_code_c);
+  d_vtx_ptr_.upload(vtx_ptr);
+  d_vtx_code_.upload(vtx_code);
+  d_vtx_ptr_c_.upload(vtx_ptr_c);
+  d_vtx_code_c_.upload(vtx_code_c);
+  d_pinned_.upload(pinned);
+  d_pin_value_.upload(pin_value);
+  d_contact_.upload(contact);
+  d_X_.upload(X);
+  d_geom_.upload(geom);
+  upload_tiles(tiles_v_, dt_v_);
+  upload_tiles(tiles_a_, dt_a_);
+  const size_t n3 = size_t(3) * size_t(nV_);
+  u_.alloc(n3);
+  v_.alloc(n3);
+  u0_.alloc(n3);
+  v0_.alloc(n3);
+  res_.alloc(n3);
+  dx_.alloc(n3);
+  sol0_.alloc(n3);
+  r_.alloc(n3);
+  z_.alloc(n3);
+  p0_.alloc(n3);
+  p1_.alloc(n3);
+  Ap_.alloc(n3);
+  rref_.alloc(n3);
+  Re_.alloc(size_t(kReStride) * size_t(nT_));
+  He_.alloc(size_t(kHeStride) * size_t(nT_));
+  Rc_.alloc(size_t(9) * size_t(std::max(nF_, 1)));
+  Hc_.alloc(size_t(36) * size_t(std::max(nF_, 1)));
+  vals_.alloc(size_t(9) * size_t(nB_));
+  dinv_.alloc(size_t(9) * size_t(nV_));
+  env_cg_.alloc(static_cast<size_t>(nEnv));
+  acc_.alloc(size_t(kCgSets) * size_t(nEnv) * size_t(kSlots) * size_t(kSlotStride));
+  solve_mask_.alloc(static_cast<size_t>(nEnv));
+  num_active_.alloc(1);
+  env_mask_.alloc(static_cast<size_t>(nEnv));
+  ls_mask_.alloc(static_cast<size_t>(nEnv));
+  ls_scale_.alloc(static_cast<size_t>(nEnv));
+  reset_.alloc(static_cast<size_t>(nEnv));
+  env_sq_.alloc(static_cast<size_t>(nEnv));
+  norms2_.alloc(size_t(2) * size_t(nEnv));
+  hip_check(hipMemsetAsync(Rc_.p, 0, Rc_.bytes(), stream_), "hipMemsetAsync");
+  hip_check(hipMemsetAsync(Hc_.p, 0, Hc_.bytes(), stream_), "hipMemsetAsync");
+  finalized_ = true;
+  push_state();
+  colliders_dirty_ = true;
+  mesh_.nV = nV_;
+  mesh_.nT = nT_;
+  mesh_.nF = nF_;
+  mesh_.nB = nB_;
+  mesh_.nEnv = nEnv;
+  mesh_.tets = d_tets_.p;
+  mesh_.tet_env = d_tet_env_.p;
+  mesh_.tris = d_tris_.p;
+  mesh_.tri_env = d_tri_env_.p;
+  mesh_.row_env = d_row_env_.p;
+  mesh_.row_ptr = d_row_ptr_.p;
+  mesh_.col = d_col_.p;
+  mesh_.slot_row = d_slot_row_.p;
+  mesh_.diag_slot = d_diag_slot_.p;
+  mesh_.slot_ptr = d_slot_ptr_.p;
+  mesh_.slot_code = d_slot_code_.p;
+  mesh_.slot_ptr_c = d_slot_ptr_c_.p;
+  mesh_.slot_code_c = d_slot_code_c_.p;
+  mesh_.vtx_ptr = d_vtx_ptr_.p;
+  mesh_.vtx_code = d_vtx_code_.p;
+  mesh_.vtx_ptr_c = d_vtx_ptr_c_.p;
+  mesh_.vtx_code_c = d_vtx_code_c_.p;
+  mesh_.pinned = d_pinned_.p;
+  mesh_.env_mask = env_mask_.p;
+  fields_.X = d_X_.p;
+  fields_.geom = d_geom_.p;
+  fields_.contact = d_contact_.p;
+  upload_colliders();
+}
+
+template <class T>
+void World<T>::upload_tiles(const HostTiles& h, DevTiles& d) {
+  d.env.upload(h.env);
+  d.row0.upload(h.row0);
+  d.row1.upload(h.row1);
+  d.first.upload(h.first);
+  d.view.env = d.env.p;
+  d.view.row0 = d.row0.p;
+  d.view.row1 = d.row1.p;
+  d.view.first = d.first.p;
+  d.view.n = int32_t(h.env.size());
+}
+
+template <class T>
+void World<T>::upload_colliders() {
+  std::vector<int32_t> first(size_t(nEnv_) + 1, 0);
+  std::vector<double> cd;
+  for (int e = 0; e < nEnv_; ++e) {
+    first[size_t(e)] = int32_t(cd.size() / 8);
+    for (const auto& c : colliders_) {
+      if (c.actor >= 0 && c.actor != e) continue;
+      cd.push_back(double(c.kind));
+      cd.insert(cd.end(), c.p, c.p + 3);
+      cd.insert(cd.end(), c.n, c.n + 3);
+      cd.push_back(c.r);
+    }
+  }
+  first[size_t(nEnv_)] = int32_t(cd.size() / 8);
+  if (cd.empty()) cd.assign(8, 0.0);
+  if (d_coll_.n < cd.size()) d_coll_.alloc(cd.size());
+  d_coll_.upload_async(cd, stream_);
+  if (d_coll_first_.n < first.size()) d_coll_first_.alloc(first.size());
+  d_coll_first_.upload_async(first, stream_);
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  mesh_.coll_first = d_coll_first_.p;
+  mesh_.coll = d_coll_.p;
+  colliders_dirty_ = false;
+}
+
+template <class T>
+void World<T>::set_env_mask(const std::vector<int32_t>& mask) {
+  env_mask_.upload_async(mask, stream_);
+}
+
+// Assemble residual (and BSR + preconditioner if wantH) for the environments
+// in the current mask; per-environment ||res||^2 lands in env_sq_.
+template <class T>
+void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH) {
+  const int project = sp.psd_proj_mode == SDX_PSD_ALWAYS ? 1 : 0;
+  timer_begin(2);
+  launch_element<T>(stream_, mesh_, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, Re_.p,
+                    He_.p);
+  timer_end(2);
+  launch_contact<T>(stream_, mesh_, fields_, u_.p, wantH ? 1 : 0, Rc_.p, Hc_.p);
+  if (wantH) {
+    timer_begin(3);
+    launch_gather_bsr<T>(stream_, mesh_, He_.p, Hc_.p, sp.linear.preconditioner, vals_.p, dinv_.p);
+    timer_end(3);
+  }
+  const int64_t set0 = int64_t(nEnv_) * kSlots * kSlotStride;
+  if (precision_ == SDX_PREC_MODE_FP32) {
+    T* acc = reinterpret_cast<T*>(acc_.p);
+    launch_fill<T>(stream_, set0, T(0), acc);
+    launch_gather_res<T, T>(stream_, mesh_, dt_v_.view, Re_.p, Rc_.p, res_.p, acc);
+    launch_acc_finalize<T>(stream_, nEnv_, 1, acc, env_sq_.p);
+  } else {
+    launch_fill<double>(stream_, set0, 0.0, acc_.p);
+    launch_gather_res<T, double>(stream_, mesh_, dt_v_.view, Re_.p, Rc_.p, res_.p, acc_.p);
+    launch_acc_finalize<double>(stream_, nEnv_, 1, acc_.p, env_sq_.p);
+  }
+}
+
+template <class T>
+std::vector<double> World<T>::read_env_sq() {
+  std::vector<double> h(static_cast<size_t>(nEnv_));
+  hip_check(hipMemcpyAsync(h.data(), env_sq_.p, sizeof(double) * size_t(nEnv_), hipMemcpyDeviceToHost, stream_),
+            "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  flush_timers();
+  return h;
+}
+
+// Block-Jacobi PCG on the assembled matrix for the environments with
+// solve[e] != 0.  Solution in dx_; returns the per-environment CG state.
+template <class T>
+std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve) {
+  CgConfig cfg{};
+  cfg.n_env = nEnv_;
+  cfg.norm_type = kp.norm_type;
+  cfg.prec = kp.preconditioner;
+  cfg.max_iter = kp.max_iter;
+  cfg.refresh_every = precision_ == SDX_PREC_MODE_MIXED ? kp.refresh_every : 0;
+  cfg.abort_if_not_spd = kp.abort_if_not_spd;
+  cfg.zero_guess = 1;
+  cfg.abs_tol = kp.abs_tol;
+  cfg.rel_tol = kp.rel_tol;
+  cfg.rel_div_tol = kp.rel_div_tol;
+  std::vector<EnvCG> h(static_cast<size_t>(nEnv_));
+  int active = 0;
+  for (int e = 0; e < nEnv_; ++e) {
+    std::memset(&h[size_t(e)], 0, sizeof(EnvCG));
+    h[size_t(e)].status = solve[size_t(e)] ? kCgActive : kCgIdle;
+    active += solve[size_t(e)] ? 1 : 0;
+  }
+  if (active == 0) return h;
+  solve_mask_.upload_async(solve, stream_);
+  std::vector<int32_t> na(1, active);
+  num_active_.upload_async(na, stream_);
+  const bool dd = precision_ != SDX_PREC_MODE_FP32;
+  T* accf = reinterpret_cast<T*>(acc_.p);
+  if (dd) {
+    launch_cg_reset<double>(stream_, nEnv_, solve_mask_.p, env_cg_.p, acc_.p);
+    launch_cg_init<T, double>(stream_, dt_v_.view, cfg, res_.p, nullptr, dinv_.p, nV_, dx_.p, r_.p, z_.p, env_cg_.p,
+                              acc_.p);
+  } else {
+    launch_cg_reset<T>(stream_, nEnv_, solve_mask_.p, env_cg_.p, accf);
+    launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, nullptr, dinv_.p, nV_, dx_.p, r_.p, z_.p, env_cg_.p, accf);
+  }
+  const int burst = cg_burst_;
+  int j = 0;
+  int32_t* hna = pinned_active_;
+  while (true) {
+    for (int k = 0; k < burst; ++k, ++j) {
+      timer_begin(0);
+      if (dd)
+        launch_cg_spmv<T, double>(stream_, dt_a_.view, cfg, j, d_row_ptr_.p, d_col_.p, vals_.p, nB_, z_.p, p0_.p,
+                                  p1_.p, dx_.p, res_.p, Ap_.p, rref_.p, env_cg_.p, acc_.p, num_active_.p);
+      else
+        launch_cg_spmv<T, T>(stream_, dt_a_.view, cfg, j, d_row_ptr_.p, d_col_.p, vals_.p, nB_, z_.p, p0_.p, p1_.p,
+                             dx_.p, res_.p, Ap_.p, rref_.p, env_cg_.p, accf, num_active_.p);
+      timer_end(0);
+      timer_begin(1);
+      if (dd)
+        launch_cg_update<T, double>(stream_, dt_v_.view, cfg, j, dinv_.p, nV_, p0_.p, p1_.p, Ap_.p, rref_.p, dx_.p,
+                                    r_.p, z_.p, env_cg_.p, acc_.p, num_active_.p);
+      else
+        launch_cg_update<T, T>(stream_, dt_v_.view, cfg, j, dinv_.p, nV_, p0_.p, p1_.p, Ap_.p, rref_.p, dx_.p, r_.p,
+                               z_.p, env_cg_.p, accf, num_active_.p);
+      timer_end(1);
+    }
+    hip_check(hipMemcpyAsync(hna, num_active_.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    flush_timers();
+    if (*hna <= 0 || j > kp.max_iter + 1) break;
+  }
+  env_cg_.download(h);
+  if (cfg.refresh_every > 0)
+    for (auto& e : h) e.refreshes = e.iter / cfg.refresh_every;
+  return h;
+}
+
+template <class T>
+void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stats, int n_stats) {
+  if (!(dt > 0)) throw std::invalid_argument("dt must be positive");
+  finalize();
+  if (colliders_dirty_) upload_colliders();
+  const int nE = nEnv_;
+  const size_t n3 = size_t(3) * size_t(nV_);
+  // EntityIncrementStep / PreStage: stage start = previous step.
+  launch_copy<T>(stream_, int64_t(n3), u_.p, u0_.p);
+  launch_copy<T>(stream_, int64_t(n3), v_.p, v0_.p);
+  launch_apply_pins<T>(stream_, int64_t(n3), d_pinned_.p, d_pin_value_.p, u_.p);
+
+  struct EnvState {
+    bool active = true;
+    int iters = 0;
+    double res0 = 0, res = 0;
+    int conv = SDX_CONV_NONE;
+    int lin_iters = 0, ls_iters = 0, lin_conv = SDX_CONV_NONE, refreshes = 0;
+    double lin_rel = 0;
+    double dx_norm = std::numeric_limits<double>::infinity();
+    double ls_res0 = 0;
+    bool have_dx = false;
+  };
+  std::vector<EnvState> S(static_cast<size_t>(nE));
+  std::vector<int32_t> mask(size_t(nE), 1);
+  bool res_valid = false;  // res_ and S.res hold the residual at the current u
+  // |dx| is needed by the RelStepNorm test of the next Newton iteration only
+  const bool need_dx = sp.newton_max_iter > 1 && sp.rel_step_tol > 0;
+  while (true) {
+    bool h_valid = false;
+    for (int e = 0; e < nE; ++e) mask[size_t(e)] = S[size_t(e)].active ? 1 : 0;
+    if (!res_valid) {
+      // first Newton iteration: residual + dresidual (newton_solver.cpp:667-686)
+      set_env_mask(mask);
+      assemble(dt, sp, true);
+      h_valid = true;
+      auto sq = read_env_sq();
+      for (int e = 0; e < nE; ++e)
+        if (S[size_t(e)].active) S[size_t(e)].res = std::sqrt(sq[size_t(e)]);
+    }
+    // stop criteria (newton_solver.cpp:304-314): ResNorm, MaxIters, RelStepNorm
+    bool need_norms = false;
+    for (int e = 0; e < nE; ++e)
+      if (S[size_t(e)].active && S[size_t(e)].iters > 0 && S[size_t(e)].have_dx && sp.rel_step_tol > 0)
+        need_norms = true;
+    std::vector<double> n2;
+    if (need_norms) {
+      launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
+      launch_env_norms<T>(stream_, dt_v_.view, nEnv_, u_.p, dx_.p, acc_.p);
+      launch_acc_finalize<double>(stream_, nEnv_, 2, acc_.p, norms2_.p);
+      n2.resize(size_t(2 * nE));
+      norms2_.download(n2);
+    }
+    int n_active = 0;
+    for (int e = 0; e < nE; ++e) {
+      EnvState& s = S[size_t(e)];
+      if (!s.active) continue;
+      if (s.iters == 0) s.res0 = s.res;
+      const bool absConv = s.res <= sp.newton_abs_tol;
+      const bool relConv = s.iters > 0 && s.res <= sp.newton_rel_tol * s.res0;
+      if (absConv || relConv) {
+        s.conv = SDX_CONV_CONVERGED;
+        s.active = false;
+      } else if (s.iters >= sp.newton_max_iter) {
+        s.conv = SDX_CONV_STOPPED;
+        s.active = false;
+      } else if (need_norms && s.iters > 0 && s.have_dx &&
+                 s.dx_norm <= double(T(sp.rel_step_tol)) * std::sqrt(n2[size_t(2 * e)])) {
+        s.conv = SDX_CONV_STOPPED;
+        s.active = false;
+      }
+      if (s.active) ++n_active;
+    }
+    if (n_active == 0) break;
+    for (int e = 0; e < nE; ++e) mask[size_t(e)] = S[size_t(e)].active ? 1 : 0;
+    set_env_mask(mask);
+    if (!h_valid) assemble(dt, sp, true);
+    for (int e = 0; e < nE; ++e)
+      if (S[size_t(e)].active) S[size_t(e)].iters++;
+    // linear solve (zero initial guess), solution in dx_
+    auto cg = cg_solve(sp.linear, mask);
+    std::vector<int32_t> lin_ok(size_t(nE), 0);
+    for (int e = 0; e < nE; ++e) {
+      EnvState& s = S[size_t(e)];
+      if (!s.active) continue;
+      const EnvCG& c = cg[size_t(e)];
+      s.lin_iters += c.iter;
+      s.lin_conv = c.status == kCgConverged ? SDX_CONV_CONVERGED
+                   : c.status == kCgDiverged ? SDX_CONV_DIVERGED
+                                             : SDX_CONV_STOPPED;
+      s.lin_rel = c.start > 0 ? std::sqrt(c.latest / c.start) : 0.0;
+      s.refreshes += c.refreshes;
+      s.have_dx = true;
+      lin_ok[size_t(e)] = s.lin_conv == SDX_CONV_CONVERGED;
+    }
+    if (need_dx) {
+      launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
+      launch_env_norms<T>(stream_, dt_v_.view, nEnv_, dx_.p, dx_.p, acc_.p);
+      launch_acc_finalize<double>(stream_, nEnv_, 2, acc_.p, norms2_.p);
+      std::vector<double> m2(size_t(2 * nE));
+      norms2_.download(m2);
+      for (int e = 0; e < nE; ++e)
+        if (S[size_t(e)].active) S[size_t(e)].dx_norm = std::sqrt(m2[size_t(2 * e)]);
+    }
+    // line search along -dx (line_search.h:300-335)
+    for (int e = 0; e < nE; ++e) S[size_t(e)].ls_res0 = S[size_t(e)].res;
+    launch_copy<T>(stream_, int64_t(n3), u_.p, sol0_.p);
+    std::vector<double> scale(size_t(nE), 1.0);
+    std::vector<int32_t> searching(mask);
+    std::vector<int> accepted_it(size_t(nE), -1);
+    const bool ls_on = sp.line_search_type == SDX_LS_RESIDUAL_NORM;
+    const int ls_max = ls_on ? std::max(1, sp.line_search_max_iter) : 1;
+    for (int it = 0; it < ls_max; ++it) {
+      if (it > 0)
+        for (int e = 0; e < nE; ++e)
+          if (searching[size_t(e)]) scale[size_t(e)] *= sp.line_search_alpha;
+      ls_scale_.upload_async(scale, stream_);
+      ls_mask_.upload_async(searching, stream_);
+      launch_ls_update<T>(stream_, nV_, d_row_env_.p, ls_mask_.p, ls_scale_.p, sol0_.p, dx_.p, u_.p);
+      // residual at the trial point (also the explosion-control residual and
+      // the next iteration's residual: same u, same assembly)
+      set_env_mask(searching);
+      assemble(dt, sp, false);
+      auto sq = read_env_sq();
+      int still = 0;
+      for (int e = 0; e < nE; ++e) {
+        if (!searching[size_t(e)]) continue;
+        const double rn = std::sqrt(sq[size_t(e)]);
+        S[size_t(e)].res = rn;  // residual at the current trial
+        if (!ls_on || rn <= S[size_t(e)].ls_res0) {
+          accepted_it[size_t(e)] = it;
+          searching[size_t(e)] = 0;
+        } else {
+          ++still;
+        }
+      }
+      if (still == 0) break;
+    }
+    for (int e = 0; e < nE; ++e) {
+      EnvState& s = S[size_t(e)];
+      if (!s.active) continue;
+      const bool improved = accepted_it[size_t(e)] >= 0;
+      const int lsIters = improved ? accepted_it[size_t(e)] : ls_max;
+      s.ls_iters += lsIters;
+      // a failed line search or an unconverged solve is retried once with an
+      // identical operator, which repeats the same solve and line search
+      // (newton_solver.cpp:783-854); count that work, keep the final iterate
+      if (!improved || !lin_ok[size_t(e)]) {
+        if (lin_ok[size_t(e)]) s.ls_iters += lsIters;
+        s.lin_iters += cg[size_t(e)].iter;
+      }
+      // explosion control (newton_solver.cpp:856-874)
+      if (sp.explosion_control) {
+        if (s.res > sp.rel_div_tol * s.res0 || s.res > sp.abs_div_tol || !std::isfinite(s.res)) {
+          s.conv = SDX_CONV_DIVERGED;
+          s.active = false;
+        }
+      }
+    }
+    // the residual of each env's last line-search trial is its residual at u
+    res_valid = true;
+  }
+  // PostStage: v = (u - u0) / dt, reset on divergence (mochi_soft.cpp:135-163)
+  std::vector<int32_t> reset(size_t(nE), 0);
+  for (int e = 0; e < nE; ++e) reset[size_t(e)] = S[size_t(e)].conv == SDX_CONV_DIVERGED ? 1 : 0;
+  reset_.upload_async(reset, stream_);
+  launch_post_step<T>(stream_, nV_, d_row_env_.p, reset_.p, dt, u0_.p, u_.p, v_.p);
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  flush_timers();
+  if (stats) {
+    for (int e = 0; e < std::min(nE, n_stats); ++e) {
+      const EnvState& s = S[size_t(e)];
+      sdx_step_stats& o = stats[e];
+      std::memset(&o, 0, sizeof(o));
+      o.newton_iters = s.iters;
+      o.linear_iters = s.lin_iters;
+      o.line_search_iters = s.ls_iters;
+      o.convergence = s.conv;
+      o.linear_convergence = s.lin_conv;
+      o.fp64_refreshes = s.refreshes;
+      o.res_norm0 = s.res0;
+      o.res_norm = s.res;
+      o.linear_rel_res = s.lin_rel;
+    }
+  }
+}
+
+template <class T>
+void World<T>::get_state(int a, int which, double* out) {
+  HostActor& A = actor(a);
+  if (!finalized_) {
+    const std::vector<double>& src = which == 1 ? A.v : A.u;
+    for (int k = 0; k < 3 * A.nV; ++k) out[k] = src[size_t(k)] + (which == 2 ? A.X[size_t(k)] : 0.0);
+    return;
+  }
+  std::vector<T> h(size_t(3 * A.nV));
+  const T* src = (which == 1 ? v_.p : u_.p) + size_t(3) * size_t(env_v0_[size_t(a)]);
+  hip_check(hipMemcpyAsync(h.data(), src, h.size() * sizeof(T), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  for (int k = 0; k < 3 * A.nV; ++k) out[k] = double(h[size_t(k)]) + (which == 2 ? A.X[size_t(k)] : 0.0);
+}
+
+template <class T>
+void World<T>::set_state(int a, int which, const double* in) {
+  HostActor& A = actor(a);
+  std::vector<double>& dst = which == 1 ? A.v : A.u;
+  for (int k = 0; k < 3 * A.nV; ++k) dst[size_t(k)] = in[k];
+  if (!finalized_) return;
+  std::vector<T> h(size_t(3 * A.nV));
+  for (int k = 0; k < 3 * A.nV; ++k) h[size_t(k)] = T(in[k]);
+  T* d = (which == 1 ? v_.p : u_.p) + size_t(3) * size_t(env_v0_[size_t(a)]);
+  hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream_), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+template <class T>
+void World<T>::actor_bsr_size(int a, int32_t* nrows, int32_t* nnzb) {
+  finalize();
+  actor(a);
+  const int v0 = env_v0_[size_t(a)], v1 = env_v0_[size_t(a) + 1];
+  if (nrows) *nrows = v1 - v0;
+  if (nnzb) *nnzb = h_row_ptr_[size_t(v1)] - h_row_ptr_[size_t(v0)];
+}
+
+template <class T>
+void World<T>::assemble_actor(int a, double dt, const sdx_solver_params& sp, double* res, int32_t* ptr, int32_t* col,
+                              double* vals) {
+  finalize();
+  actor(a);
+  if (colliders_dirty_) upload_colliders();
+  // stage start u0 = 0, v0 = 0 (inertia acts on u itself), as the oracle's assemble entry
+  hip_check(hipMemsetAsync(u0_.p, 0, u0_.bytes(), stream_), "hipMemsetAsync");
+  hip_check(hipMemsetAsync(v0_.p, 0, v0_.bytes(), stream_), "hipMemsetAsync");
+  std::vector<int32_t> mask(size_t(nEnv_), 0);
+  mask[size_t(a)] = 1;
+  set_env_mask(mask);
+  assemble(dt, sp, true);
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  const int v0 = env_v0_[size_t(a)], v1 = env_v0_[size_t(a) + 1];
+  const int b0 = h_row_ptr_[size_t(v0)], b1 = h_row_ptr_[size_t(v1)];
+  if (res) {
+    std::vector<T> h(size_t(3 * (v1 - v0)));
+    hip_check(hipMemcpy(h.data(), res_.p + size_t(3) * size_t(v0), h.size() * sizeof(T), hipMemcpyDeviceToHost),
+              "hipMemcpy");
+    for (size_t k = 0; k < h.size(); ++k) res[k] = double(h[k]);
+  }
+  if (ptr)
+    for (int i = v0; i <= v1; ++i) ptr[i - v0] = h_row_ptr_[size_t(i)] - b0;
+  if (col)
+    for (int k = b0; k < b1; ++k) col[k - b0] = h_col_[size_t(k)] - v0;
+  if (vals) {
+    std::vector<T> h(size_t(b1 - b0));
+    for (int c = 0; c < 9; ++c) {
+      hip_check(hipMemcpy(h.data(), vals_.p + size_t(c) * size_t(nB_) + size_t(b0), h.size() * sizeof(T),
+                          hipMemcpyDeviceToHost),
+                "hipMemcpy");
+      for (size_t k = 0; k < h.size(); ++k) vals[9 * k + size_t(c)] = double(h[k]);
+    }
+  }
+}
+
+template <class T>
+void World<T>::info(int64_t out[8]) {
+  finalize();
+  std::memset(out, 0, sizeof(int64_t) * 8);
+  out[0] = nV_;
+  out[1] = nT_;
+  out[2] = nB_;
+  out[3] = nF_;
+  out[4] = nEnv_;
+  out[5] = int64_t(tiles_a_.env.size());
+  out[6] = int64_t(tiles_v_.env.size());
+}
+
+template class World<float>;
+template class World<double>;
+
+}  // namespace sdx

@@ -1,0 +1,319 @@
+// World state of the MI355X soft-body FEM step (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/sdx_fem.h"
+#include "sdx_kernels.h"
+
+namespace sdx {
+
+struct DeviceError : std::runtime_error {
+  explicit DeviceError(const std::string& m) : std::runtime_error(m) {}
+};
+
+extern thread_local std::string g_last_error;
+sdx_status set_error(sdx_status code, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+
+struct DBufBase {
+  virtual ~DBufBase() = default;
+  virtual void free() = 0;
+};
+
+// Device buffer owned by a world; registers itself for release.
+template <class T>
+struct DBuf : DBufBase {
+  T* p = nullptr;
+  size_t n = 0;
+  hipStream_t* stream = nullptr;
+  DBuf(std::vector<DBufBase*>& reg, hipStream_t* s) : stream(s) { reg.push_back(this); }
+  ~DBuf() override { free(); }
+  void alloc(size_t count) {
+    free();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+    hip_check(e, "hipMalloc");
+    n = count;
+  }
+  void free() override {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+  void upload(const std::vector<T>& h) {
+    if (!p || n < h.size()) alloc(h.size());
+    if (!h.empty())
+      hip_check(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, *stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(*stream), "hipStreamSynchronize");
+  }
+  // The host vector must stay alive until the stream reaches this copy.
+  void upload_async(const std::vector<T>& h, hipStream_t s) {
+    if (!p || n < h.size()) throw std::logic_error("device buffer too small");
+    if (!h.empty())
+      hip_check(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+  }
+  void download(std::vector<T>& h) {
+    if (!h.empty())
+      hip_check(hipMemcpyAsync(h.data(), p, h.size() * sizeof(T), hipMemcpyDeviceToHost, *stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(*stream), "hipStreamSynchronize");
+  }
+};
+
+struct HostActor {
+  int nV = 0, nT = 0;
+  std::vector<double> X;
+  std::vector<int32_t> tets;
+  sdx_soft_params params{};
+  std::vector<double> E, nu;  // per-element fields (empty: homogeneous)
+  std::vector<uint8_t> pinned;
+  std::vector<double> pin_value;
+  std::vector<double> u, v;
+};
+
+struct HostCollider {
+  int kind = 0;  // 0 plane, 1 sphere
+  double p[3] = {0, 0, 0}, n[3] = {0, 1, 0}, r = 0;
+  int actor = -1;
+};
+
+struct HostTiles {
+  std::vector<int32_t> env, row0, row1, first;
+};
+
+class WorldBase {
+ public:
+  WorldBase(int device, int precision) : device_(device), precision_(precision) {
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&pinned_active_), sizeof(int32_t)), "hipHostMalloc");
+  }
+  virtual ~WorldBase() {
+    for (auto& kv : timers_)
+      for (auto& ev : kv.events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+      }
+    if (pinned_active_) (void)hipHostFree(pinned_active_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+  int device() const { return device_; }
+  int precision() const { return precision_; }
+
+  int add_actor(HostActor a) {
+    invalidate();
+    actors_.push_back(std::move(a));
+    return int(actors_.size()) - 1;
+  }
+  HostActor& actor(int a) {
+    if (a < 0 || a >= int(actors_.size())) throw std::out_of_range("no such actor");
+    return actors_[size_t(a)];
+  }
+  int num_actors() const { return int(actors_.size()); }
+  int add_collider(const HostCollider& c) {
+    colliders_.push_back(c);
+    colliders_dirty_ = true;
+    return int(colliders_.size()) - 1;
+  }
+  HostCollider& collider(int c) {
+    if (c < 0 || c >= int(colliders_.size())) throw std::out_of_range("no such collider");
+    colliders_dirty_ = true;
+    return colliders_[size_t(c)];
+  }
+  // Bring device state back to host mirrors and drop device buffers; the next
+  // step rebuilds them (topology or material change).
+  void invalidate() {
+    pull_state();
+    release_device();
+  }
+
+  virtual void finalize() = 0;
+  virtual void step(double dt, const sdx_solver_params& sp, sdx_step_stats* stats, int n_stats) = 0;
+  virtual void pull_state() = 0;
+  virtual void release_device() = 0;
+  virtual void get_state(int actor, int which, double* out) = 0;
+  virtual void set_state(int actor, int which, const double* in) = 0;
+  virtual void assemble_actor(int actor, double dt, const sdx_solver_params& sp, double* res, int32_t* ptr,
+                              int32_t* col, double* vals) = 0;
+  virtual void info(int64_t out[8]) = 0;
+  virtual void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) = 0;
+
+  // kernel timing: class 0 CG matvec, 1 CG update, 2 element, 3 BSR gather
+  // on > 0: time every on-th launch of the class (1: every launch); 0: off.
+  void timing_enable(int k, int on) {
+    auto& t = timers_[size_t(k)];
+    t.enabled = on > 0;
+    t.every = on > 0 ? on : 1;
+    t.counter = 0;
+    t.launches = 0;
+    t.ms = 0.0;
+    t.used = 0;
+  }
+  std::pair<double, double> timing_read(int k) {
+    flush_timers();
+    return {double(timers_[size_t(k)].launches), timers_[size_t(k)].ms};
+  }
+  int cg_burst_ = 8;
+
+ protected:
+  struct Timer {
+    bool enabled = false;
+    bool active = false;  // the current launch is being timed
+    int every = 1;
+    int64_t counter = 0;
+    size_t used = 0;
+    int64_t launches = 0;
+    double ms = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  };
+  void timer_begin(int k) {
+    Timer& t = timers_[size_t(k)];
+    t.active = t.enabled && (t.counter++ % t.every) == 0;
+    if (!t.active) return;
+    if (t.used == t.events.size()) {
+      std::pair<hipEvent_t, hipEvent_t> ev;
+      hip_check(hipEventCreate(&ev.first), "hipEventCreate");
+      hip_check(hipEventCreate(&ev.second), "hipEventCreate");
+      t.events.push_back(ev);
+    }
+    hip_check(hipEventRecord(t.events[t.used].first, stream_), "hipEventRecord");
+  }
+  void timer_end(int k) {
+    Timer& t = timers_[size_t(k)];
+    if (!t.active) return;
+    hip_check(hipEventRecord(t.events[t.used].second, stream_), "hipEventRecord");
+    t.used++;
+  }
+  // Call after a stream synchronisation: folds recorded event pairs into totals.
+  void flush_timers() {
+    for (auto& t : timers_) {
+      if (!t.enabled || t.used == 0) continue;
+      hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+      for (size_t i = 0; i < t.used; ++i) {
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, t.events[i].first, t.events[i].second), "hipEventElapsedTime");
+        t.ms += double(ms);
+      }
+      t.launches += int64_t(t.used);
+      t.used = 0;
+    }
+  }
+
+  int device_;
+  int precision_;
+  hipStream_t stream_ = nullptr;
+  int32_t* pinned_active_ = nullptr;
+  std::vector<HostActor> actors_;
+  std::vector<HostCollider> colliders_;
+  bool colliders_dirty_ = true;
+  bool finalized_ = false;
+  Timer timers_[4];
+};
+
+template <class T>
+class World final : public WorldBase {
+ public:
+  World(int device, int precision);
+  ~World() override;
+  void finalize() override;
+  void step(double dt, const sdx_solver_params& sp, sdx_step_stats* stats, int n_stats) override;
+  void pull_state() override;
+  void release_device() override;
+  void get_state(int actor, int which, double* out) override;
+  void set_state(int actor, int which, const double* in) override;
+  void assemble_actor(int actor, double dt, const sdx_solver_params& sp, double* res, int32_t* ptr, int32_t* col,
+                      double* vals) override;
+  void info(int64_t out[8]) override;
+  void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) override;
+
+ private:
+  struct DevTiles {
+    DBuf<int32_t> env, row0, row1, first;
+    Tiles view{};
+    DevTiles(std::vector<DBufBase*>& r, hipStream_t* s) : env(r, s), row0(r, s), row1(r, s), first(r, s) {}
+  };
+  void push_state();
+  void upload_tiles(const HostTiles& h, DevTiles& d);
+  void upload_colliders();
+  void set_env_mask(const std::vector<int32_t>& mask);
+  void assemble(double dt, const sdx_solver_params& sp, bool wantH);
+  std::vector<double> read_env_sq();
+  std::vector<EnvCG> cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve);
+
+  std::vector<DBufBase*> bufs_;
+  int nEnv_ = 0, nV_ = 0, nT_ = 0, nB_ = 0, nF_ = 0;
+  std::vector<int32_t> env_v0_;
+  std::vector<int32_t> env_b0_;
+  std::vector<int32_t> h_row_ptr_, h_col_;
+  HostTiles tiles_v_, tiles_a_;
+  MeshView mesh_{};
+  FieldPtrs<T> fields_{};
+#define SDX_BUF(type, name) DBuf<type> name{bufs_, &stream_};
+  SDX_BUF(int32_t, d_tets_)
+  SDX_BUF(int32_t, d_tet_env_)
+  SDX_BUF(int32_t, d_tris_)
+  SDX_BUF(int32_t, d_tri_env_)
+  SDX_BUF(int32_t, d_row_env_)
+  SDX_BUF(int32_t, d_row_ptr_)
+  SDX_BUF(int32_t, d_col_)
+  SDX_BUF(int32_t, d_slot_row_)
+  SDX_BUF(int32_t, d_diag_slot_)
+  SDX_BUF(int32_t, d_slot_ptr_)
+  SDX_BUF(uint32_t, d_slot_code_)
+  SDX_BUF(int32_t, d_slot_ptr_c_)
+  SDX_BUF(uint32_t, d_slot_code_c_)
+  SDX_BUF(int32_t, d_vtx_ptr_)
+  SDX_BUF(uint32_t, d_vtx_code_)
+  SDX_BUF(int32_t, d_vtx_ptr_c_)
+  SDX_BUF(uint32_t, d_vtx_code_c_)
+  SDX_BUF(uint8_t, d_pinned_)
+  SDX_BUF(T, d_pin_value_)
+  SDX_BUF(double, d_contact_)
+  SDX_BUF(double, d_coll_)
+  SDX_BUF(int32_t, d_coll_first_)
+  SDX_BUF(T, d_X_)
+  SDX_BUF(T, d_geom_)
+  SDX_BUF(T, u_)
+  SDX_BUF(T, v_)
+  SDX_BUF(T, u0_)
+  SDX_BUF(T, v0_)
+  SDX_BUF(T, res_)
+  SDX_BUF(T, dx_)
+  SDX_BUF(T, sol0_)
+  SDX_BUF(T, r_)
+  SDX_BUF(T, z_)
+  SDX_BUF(T, p0_)
+  SDX_BUF(T, p1_)
+  SDX_BUF(T, Ap_)
+  SDX_BUF(double, rref_)
+  SDX_BUF(T, Re_)
+  SDX_BUF(T, He_)
+  SDX_BUF(T, Rc_)
+  SDX_BUF(T, Hc_)
+  SDX_BUF(T, vals_)
+  SDX_BUF(T, dinv_)
+  SDX_BUF(EnvCG, env_cg_)
+  SDX_BUF(double, acc_)
+  SDX_BUF(int32_t, solve_mask_)
+  SDX_BUF(int32_t, num_active_)
+  SDX_BUF(int32_t, env_mask_)
+  SDX_BUF(int32_t, ls_mask_)
+  SDX_BUF(double, ls_scale_)
+  SDX_BUF(int32_t, reset_)
+  SDX_BUF(double, env_sq_)
+  SDX_BUF(double, norms2_)
+#undef SDX_BUF
+  DevTiles dt_v_{bufs_, &stream_};
+  DevTiles dt_a_{bufs_, &stream_};
+};
+
+}  // namespace sdx

@@ -1,0 +1,202 @@
+"""Python mirror of the reference's soft-body call surface, bound to libsdxfem.
+
+Mirrors the shape of the reference API (paths under
+superdex_physics/libraries/mochi/mochi_physics/include/mochi_physics/cpp_api/):
+  Scene::Step(dt) ......................... Scene.step                 (mochi_scene.h:63)
+  Scene::CreateSoftActor(SoftActorParams) . Scene.create_soft_actor    (mochi_scene.h:99)
+  Actor::AddBoundaryConditionNodesWorldPermanent
+                                             SoftActor.add_boundary_condition_nodes (mochi_actor.h:202)
+  CreateRigidActor(plane / sphere shape) .. Scene.create_rigid_plane / create_rigid_sphere
+  Actor::SetRootTransform (translation) ... RigidActor.set_position     (mochi_actor.h:79)
+  Actor::GetDofValues / velocities ........ SoftActor.displacements / velocities
+Every call goes through the C ABI of include/sdx_fem.h; there is no
+Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._lib import check, lib
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_dp)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(_ip)
+
+
+class SoftActor:
+    def __init__(self, scene: "Scene", index: int, n_verts: int, n_tets: int):
+        self.scene, self.index, self.n_verts, self.n_tets = scene, index, n_verts, n_tets
+
+    def add_boundary_condition_nodes(self, nodes):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        check(lib().sdx_pin_nodes(self.scene._w, self.index, _i(nodes), len(nodes)))
+
+    def set_element_materials(self, E, nu):
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        nu = np.ascontiguousarray(nu, dtype=np.float64)
+        check(lib().sdx_set_element_materials(self.scene._w, self.index, _d(E), _d(nu), len(E)))
+
+    def _get(self, fn):
+        out = np.zeros(3 * self.n_verts)
+        check(fn(self.scene._w, self.index, out.ctypes.data_as(_dp), out.size))
+        return out.reshape(-1, 3)
+
+    def displacements(self):
+        return self._get(lib().sdx_get_displacements)
+
+    def velocities(self):
+        return self._get(lib().sdx_get_velocities)
+
+    def positions_world(self):
+        return self._get(lib().sdx_get_positions_world)
+
+    def set_displacements(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        check(lib().sdx_set_displacements(self.scene._w, self.index, u.ctypes.data_as(_dp), u.size))
+
+    def set_velocities(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        check(lib().sdx_set_velocities(self.scene._w, self.index, v.ctypes.data_as(_dp), v.size))
+
+    def bsr_size(self):
+        nr, nb = C.c_int32(), C.c_int32()
+        check(lib().sdx_actor_bsr_size(self.scene._w, self.index, C.byref(nr), C.byref(nb)))
+        return nr.value, nb.value
+
+    def assemble(self, dt: float, params: _abi.SolverParams):
+        """Residual and Dirichlet-applied BSR (row-major 3x3 blocks) at the current displacement."""
+        nr, nb = self.bsr_size()
+        res = np.zeros(3 * nr)
+        ptr = np.zeros(nr + 1, dtype=np.int32)
+        col = np.zeros(nb, dtype=np.int32)
+        val = np.zeros(9 * nb)
+        check(lib().sdx_assemble(self.scene._w, self.index, dt, C.byref(params), res.ctypes.data_as(_dp),
+                                 ptr.ctypes.data_as(_ip), col.ctypes.data_as(_ip), val.ctypes.data_as(_dp)))
+        return res, ptr, col, val.reshape(nb, 3, 3)
+
+
+class RigidActor:
+    def __init__(self, scene: "Scene", index: int):
+        self.scene, self.index = scene, index
+
+    def set_position(self, pos):
+        check(lib().sdx_set_rigid_position(self.scene._w, self.index, _d(np.asarray(pos, float))))
+
+
+class Scene:
+    """One world on one HIP device: soft actors, kinematic colliders, step()."""
+
+    def __init__(self, device: int = 0, precision: int = _abi.PREC_MODE_FP32):
+        w = C.c_void_p()
+        check(lib().sdx_world_create(device, precision, C.byref(w)))
+        self._w = w
+        self.precision = precision
+        self.actors: list[SoftActor] = []
+        self.colliders: list[RigidActor] = []
+
+    def close(self):
+        if getattr(self, "_w", None):
+            lib().sdx_world_destroy(self._w)
+            self._w = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def create_soft_actor(self, X, tets, params: _abi.SoftParams | None = None) -> SoftActor:
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        tets = np.ascontiguousarray(tets, dtype=np.int32)
+        params = params or _abi.default_soft_params()
+        idx = C.c_int32()
+        check(lib().sdx_add_soft_tet_mesh(self._w, _d(X), X.shape[0], _i(tets), tets.shape[0], C.byref(params),
+                                          C.byref(idx)))
+        a = SoftActor(self, idx.value, X.shape[0], tets.shape[0])
+        self.actors.append(a)
+        return a
+
+    def create_rigid_plane(self, point, normal, actor: int = -1) -> RigidActor:
+        idx = C.c_int32()
+        check(lib().sdx_add_rigid_plane(self._w, _d(np.asarray(point, float)), _d(np.asarray(normal, float)),
+                                        actor, C.byref(idx)))
+        r = RigidActor(self, idx.value)
+        self.colliders.append(r)
+        return r
+
+    def create_rigid_sphere(self, center, radius: float, actor: int = -1) -> RigidActor:
+        idx = C.c_int32()
+        check(lib().sdx_add_rigid_sphere(self._w, _d(np.asarray(center, float)), float(radius), actor,
+                                         C.byref(idx)))
+        r = RigidActor(self, idx.value)
+        self.colliders.append(r)
+        return r
+
+    def step(self, dt: float, params: _abi.SolverParams | None = None):
+        params = params or _abi.default_solver_params()
+        stats = (_abi.StepStats * max(1, len(self.actors)))()
+        check(lib().sdx_step(self._w, dt, C.byref(params), stats, len(self.actors)))
+        return [s.as_dict() for s in stats[:len(self.actors)]]
+
+    def info(self):
+        out = (C.c_int64 * 8)()
+        check(lib().sdx_world_info(self._w, out))
+        return {"n_verts": out[0], "n_tets": out[1], "nnzb": out[2], "n_tris": out[3], "n_envs": out[4],
+                "matvec_tiles": out[5], "vector_tiles": out[6]}
+
+    def set_cg_burst(self, burst: int):
+        check(lib().sdx_set_cg_burst(self._w, burst))
+
+    def kernel_timing(self, kernel: int, enable: int = -1):
+        """enable 1 clears and starts timing kernel class `kernel`; returns (launches, total ms)."""
+        out = (C.c_double * 2)()
+        check(lib().sdx_kernel_timing(self._w, kernel, enable, out))
+        return out[0], out[1]
+
+
+def pcg_bsr3_solve(indptr, indices, blocks, b, x0=None, params: _abi.KrylovParams | None = None,
+                   layout: int = _abi.BSR_BLOCK_ROWMAJOR, device: int = 0):
+    """L1: solve A x = b on the GPU; blocks (nnzb, 3, 3) row-major (or strip layout values)."""
+    indptr = np.ascontiguousarray(indptr, dtype=np.int32)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    vals = np.ascontiguousarray(blocks, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
+    params = params or _abi.default_krylov_params()
+    st = _abi.LinearStatus()
+    check(lib().sdx_pcg_bsr3_solve(device, len(indptr) - 1, len(indices), _i(indptr), _i(indices),
+                                   vals.ctypes.data_as(_dp), layout, b.ctypes.data_as(_dp), x.ctypes.data_as(_dp),
+                                   C.byref(params), C.byref(st)))
+    return x, {"num_iter_done": st.num_iter_done, "convergence": st.convergence,
+               "residual_norm": st.residual_norm, "relative_residual_norm": st.relative_residual_norm}
+
+
+def build_scene(spec, device: int = 0, precision: int = _abi.PREC_MODE_FP32):
+    """Scene of a mesh.SceneSpec; returns (scene, rigid actors in spec order)."""
+    sc = Scene(device, precision)
+    for b in spec.bodies:
+        E0 = float(np.mean(b.E)) if np.ndim(b.E) else float(b.E)
+        nu0 = float(np.mean(b.nu)) if np.ndim(b.nu) else float(b.nu)
+        a = sc.create_soft_actor(b.X, b.tets, _abi.default_soft_params(E0, nu0, b.density, b.penalty,
+                                                                      b.mass_damping))
+        if np.ndim(b.E):
+            a.set_element_materials(b.E, b.nu)
+        if len(b.pinned):
+            a.add_boundary_condition_nodes(b.pinned)
+    rigid = []
+    for c in spec.colliders:
+        if c.kind == "plane":
+            rigid.append(sc.create_rigid_plane(c.point, c.normal, c.actor))
+        else:
+            rigid.append(sc.create_rigid_sphere(c.point, c.radius, c.actor))
+    return sc, rigid
