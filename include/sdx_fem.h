@@ -142,7 +142,8 @@ typedef struct sdx_solver_params {
   double gravity[3];              /* default (0, -9.8, 0) */
   /* extensions */
   int32_t warm_start;             /* SDX_WARM_START_* */
-  int32_t schedule;               /* element schedule id, -1 = autotuned */
+  int32_t schedule;               /* -1: autotune the world's schedule at its first step (sdx_autotune);
+                                     0: use the world's schedule as set (default) */
 } sdx_solver_params;
 
 /* Per-actor result of one step: NewtonSolverStatus fields that
@@ -159,6 +160,8 @@ typedef struct sdx_step_stats {
   double res_norm0;
   double res_norm;
   double linear_rel_res;          /* last linear solve ||r||/||b|| (as the stop test measured it) */
+  double linear_true_rel_res;     /* last linear solve ||b - A x||/||b|| recomputed in FP64 */
+  double warm_start_rel_res;      /* ||b - A x0||/||b|| of the last learned warm start (0: none) */
 } sdx_step_stats;
 
 /* SoftActorParams subset (cpp_api/mochi_structs.h:174-197). */
@@ -255,16 +258,37 @@ sdx_status sdx_assemble(sdx_world* world, int32_t actor, double dt, const sdx_so
                         double* out_residual, int32_t* out_row_ptr, int32_t* out_col_idx,
                         double* out_values);
 
-/* Timing of one kernel class inside sdx_step, measured with HIP events on the
- * world's stream.  kernel: 0 = CG matvec (SpMV + p update + p.Ap),
- * 1 = CG vector update, 2 = element Hessian, 3 = BSR gather.  enable N > 0
- * clears the accumulators and times every N-th launch of the class, 0 stops
- * timing, -1 leaves the setting unchanged.  out (may be NULL): {timed
- * launches, total milliseconds}. */
+/* Timing of one kernel class inside sdx_step, measured with HIP events
+ * recorded at the kernel's own dispatch on the world's stream.  kernel: 0 =
+ * CG matvec (w = A z + z.w), 1 = CG vector update, 2 = element Hessian, 3 =
+ * BSR gather.  enable N > 0 clears the accumulators and times every N-th
+ * launch of an assembly class, or every launch of every N-th CG burst (the
+ * other bursts replay as HIP graphs); 0 stops timing, -1 leaves the setting
+ * unchanged.  out (may be NULL): {timed launches, total milliseconds}. */
 sdx_status sdx_kernel_timing(sdx_world* world, int32_t kernel, int32_t enable, double out[2]);
 
 /* CG iterations launched between host convergence checks (default 8). */
 sdx_status sdx_set_cg_burst(sdx_world* world, int32_t burst);
+
+/* Launch schedule of a world.  No counterpart in the reference (its CPU
+ * path has no launch schedule); the batched autotuner searches this space. */
+typedef struct sdx_schedule {
+  int32_t cg_burst;       /* CG iterations per graph replay / host check, 1..256 (default 8) */
+  int32_t use_graphs;     /* 1: each CG burst is captured once and replayed as a HIP graph (default 1) */
+  int32_t spmv_lanes;     /* lanes per block row of the CG matvec: 4, 8 or 16 (default 16) */
+  int32_t element_block;  /* threads per workgroup of the element kernel: 64, 128 or 256 (default 128) */
+} sdx_schedule;
+
+sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* schedule);
+sdx_status sdx_get_schedule(sdx_world* world, sdx_schedule* out);
+
+/* Autotuner: times sdx_step of the world's current state under candidate
+ * schedules (coordinate search over matvec lanes, element workgroup size,
+ * CG burst length and graph replay; the best of `trials` steps each),
+ * restores the state after every trial and keeps the fastest schedule.
+ * out / out_ms (may be NULL): the chosen schedule and its step time. */
+sdx_status sdx_autotune(sdx_world* world, double dt, const sdx_solver_params* params, int32_t trials,
+                        sdx_schedule* out, double* out_ms);
 
 #ifdef __cplusplus
 }

@@ -73,6 +73,18 @@ class SolverParams(C.Structure):
     ]
 
 
+class Schedule(C.Structure):
+    _fields_ = [
+        ("cg_burst", C.c_int32),
+        ("use_graphs", C.c_int32),
+        ("spmv_lanes", C.c_int32),
+        ("element_block", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class StepStats(C.Structure):
     _fields_ = [
         ("newton_iters", C.c_int32),
@@ -86,6 +98,8 @@ class StepStats(C.Structure):
         ("res_norm0", C.c_double),
         ("res_norm", C.c_double),
         ("linear_rel_res", C.c_double),
+        ("linear_true_rel_res", C.c_double),
+        ("warm_start_rel_res", C.c_double),
     ]
 
     def as_dict(self):
@@ -131,7 +145,7 @@ def default_solver_params() -> SolverParams:
     p.linear = default_krylov_params()
     p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -9.8, 0.0
     p.warm_start = WARM_START_ZERO
-    p.schedule = -1
+    p.schedule = 0
     return p
 
 

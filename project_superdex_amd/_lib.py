@@ -46,6 +46,10 @@ SIGNATURES = {
     "sdx_assemble": (C.c_int32, [_vp, C.c_int32, C.c_double, C.POINTER(_abi.SolverParams), _dp, _ip, _ip, _dp]),
     "sdx_kernel_timing": (C.c_int32, [_vp, C.c_int32, C.c_int32, _dp]),
     "sdx_set_cg_burst": (C.c_int32, [_vp, C.c_int32]),
+    "sdx_set_schedule": (C.c_int32, [_vp, C.POINTER(_abi.Schedule)]),
+    "sdx_get_schedule": (C.c_int32, [_vp, C.POINTER(_abi.Schedule)]),
+    "sdx_autotune": (C.c_int32, [_vp, C.c_double, C.POINTER(_abi.SolverParams), C.c_int32,
+                                 C.POINTER(_abi.Schedule), _dp]),
 }
 
 _lib = None
@@ -69,7 +73,7 @@ def lib():
             fn.restype = res
             fn.argtypes = args
         for idx, cls in enumerate([_abi.KrylovParams, _abi.LinearStatus, _abi.SolverParams, _abi.StepStats,
-                                   _abi.SoftParams]):
+                                   _abi.SoftParams, _abi.Schedule]):
             if L.sdx_abi_struct_size(idx) != C.sizeof(cls):
                 raise RuntimeError(f"ABI mismatch for {cls.__name__}: C {L.sdx_abi_struct_size(idx)} vs "
                                    f"Python {C.sizeof(cls)}")

@@ -83,7 +83,7 @@ void sdx_default_solver_params(sdx_solver_params* p) {
   p->gravity[1] = -9.8;
   p->gravity[2] = 0.0;
   p->warm_start = SDX_WARM_START_ZERO;
-  p->schedule = -1;
+  p->schedule = 0;
 }
 
 void sdx_default_soft_params(sdx_soft_params* s) {
@@ -104,6 +104,7 @@ int32_t sdx_abi_struct_size(int32_t which) {
     case 2: return int32_t(sizeof(sdx_solver_params));
     case 3: return int32_t(sizeof(sdx_step_stats));
     case 4: return int32_t(sizeof(sdx_soft_params));
+    case 5: return int32_t(sizeof(sdx_schedule));
     default: return -1;
   }
 }
@@ -235,10 +236,30 @@ sdx_status sdx_set_rigid_position(sdx_world* world, int32_t collider, const doub
   });
 }
 
+static sdx_status validate_solver_params(const sdx_solver_params& p) {
+  const sdx_solver_params* params = &p;
+  SDX_REQUIRE(params->newton_max_iter >= 0 && params->linear.max_iter > 0, "invalid iteration limits");
+  SDX_REQUIRE(params->line_search_type == SDX_LS_NONE || params->line_search_type == SDX_LS_RESIDUAL_NORM,
+              "only the None and ResidualNorm line searches are implemented");
+  SDX_REQUIRE(params->psd_proj_mode == SDX_PSD_ALWAYS || params->psd_proj_mode == SDX_PSD_NEVER,
+              "only PSD projection modes Always and Never are implemented");
+  SDX_REQUIRE(params->linear.preconditioner >= SDX_PREC_NONE && params->linear.preconditioner <= SDX_PREC_BLOCK_JACOBI,
+              "unsupported preconditioner");
+  SDX_REQUIRE(params->linear.norm_type >= 0 && params->linear.norm_type <= 2, "unsupported norm type");
+  SDX_REQUIRE(params->warm_start == SDX_WARM_START_ZERO || params->warm_start == SDX_WARM_START_LEARNED,
+              "unknown warm start");
+  SDX_REQUIRE(params->schedule == 0 || params->schedule == -1, "schedule must be 0 or -1");
+  return SDX_OK;
+}
+
 sdx_status sdx_step(sdx_world* world, double dt, const sdx_solver_params* params, sdx_step_stats* stats,
                     int32_t n_stats) {
   SDX_REQUIRE(world && params, "NULL argument");
   SDX_REQUIRE(dt > 0 && std::isfinite(dt), "dt must be positive");
+  {
+    const sdx_status v = validate_solver_params(*params);
+    if (v != SDX_OK) return v;
+  }
   SDX_REQUIRE(params->newton_max_iter >= 0 && params->linear.max_iter > 0, "invalid iteration limits");
   SDX_REQUIRE(params->line_search_type == SDX_LS_NONE || params->line_search_type == SDX_LS_RESIDUAL_NORM,
               "only the None and ResidualNorm line searches are implemented");
@@ -321,8 +342,46 @@ sdx_status sdx_kernel_timing(sdx_world* world, int32_t kernel, int32_t enable, d
 
 sdx_status sdx_set_cg_burst(sdx_world* world, int32_t burst) {
   SDX_REQUIRE(world && burst > 0 && burst <= 256, "burst must be in 1..256");
-  world->impl->cg_burst_ = burst;
+  sdx_schedule sc = world->impl->schedule();
+  sc.cg_burst = burst;
+  return guarded([&]() -> sdx_status {
+    world->impl->set_schedule(sc);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* sc) {
+  SDX_REQUIRE(world && sc, "NULL argument");
+  SDX_REQUIRE(sc->cg_burst > 0 && sc->cg_burst <= 256, "cg_burst must be in 1..256");
+  SDX_REQUIRE(sc->use_graphs == 0 || sc->use_graphs == 1, "use_graphs must be 0 or 1");
+  SDX_REQUIRE(sc->spmv_lanes == 4 || sc->spmv_lanes == 8 || sc->spmv_lanes == 16, "spmv_lanes must be 4, 8 or 16");
+  SDX_REQUIRE(sc->element_block == 64 || sc->element_block == 128 || sc->element_block == 256,
+              "element_block must be 64, 128 or 256");
+  return guarded([&]() -> sdx_status {
+    world->impl->set_schedule(*sc);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_get_schedule(sdx_world* world, sdx_schedule* out) {
+  SDX_REQUIRE(world && out, "NULL argument");
+  *out = world->impl->schedule();
   return SDX_OK;
+}
+
+sdx_status sdx_autotune(sdx_world* world, double dt, const sdx_solver_params* params, int32_t trials,
+                        sdx_schedule* out, double* out_ms) {
+  SDX_REQUIRE(world && params, "NULL argument");
+  SDX_REQUIRE(dt > 0, "dt must be positive");
+  SDX_REQUIRE(trials >= 1 && trials <= 16, "trials must be in 1..16");
+  const sdx_status v = validate_solver_params(*params);
+  if (v != SDX_OK) return v;
+  return guarded([&]() -> sdx_status {
+    const double ms = world->impl->autotune(dt, *params, trials);
+    if (out) *out = world->impl->schedule();
+    if (out_ms) *out_ms = ms;
+    return SDX_OK;
+  });
 }
 
 }  // extern "C"

@@ -12,16 +12,14 @@ namespace sdx {
 enum : int32_t { kCgActive = 0, kCgConverged = 1, kCgStopped = 2, kCgDiverged = 3, kCgIdle = 4 };
 
 struct EnvCG {
-  double rz;       // r.z of the current iterate
-  double alpha;
-  double beta;
-  double start;    // stop-test scaling value (squared norm of b in the chosen norm)
-  double conv;     // squared convergence threshold
-  double div;      // squared divergence threshold
-  double latest;   // latest squared stop-test norm
-  double pAp;
-  int32_t iter;    // completed CG iterations
-  int32_t status;  // kCg*
+  double start;     // stop-test scaling value (squared norm of b in the chosen norm)
+  double latest;    // squared stop-test norm when the solve ended
+  double rz;        // r.z when the solve ended
+  double gamma[2];  // r_k.z_k of iteration k in slot k % 2
+  double alpha[2];
+  double mu[2];     // p_k.A p_k
+  int32_t iter;     // completed CG iterations
+  int32_t status;   // kCg*
   int32_t refreshes;
   int32_t pad;
 };
@@ -33,9 +31,20 @@ struct CgConfig {
   int32_t max_iter;
   int32_t refresh_every;  // 0: never
   int32_t abort_if_not_spd;
-  int32_t zero_guess;
-  int32_t pad;
+  int32_t pad0;
+  int32_t pad1;
   double abs_tol, rel_tol, rel_div_tol;
+};
+
+// Iteration k of the CG replaces the recursive residual by the FP64 true
+// residual (formed in the matvec of iteration k) every refresh_every iterations.
+__host__ __device__ inline bool cg_refresh(const CgConfig& cfg, int k) {
+  return cfg.refresh_every > 0 && ((k + 1) % cfg.refresh_every) == 0;
+}
+
+// Event pair recorded around one kernel dispatch.
+struct KernelEvents {
+  hipEvent_t start, stop;
 };
 
 // Tiles: contiguous row ranges that never cross an environment.
@@ -47,9 +56,9 @@ struct Tiles {
   int32_t n;
 };
 
-constexpr int kSpmvLanes = 16;    // lanes per block row in the CG matvec
+constexpr int kSpmvLanes = 16;    // lanes per block row of the plain matvec (CG matvec: 4, 8 or 16)
 constexpr int kSpmvThreads = 256;
-constexpr int kSpmvRows = kSpmvThreads / kSpmvLanes;  // 64 rows per matvec tile
+constexpr int kSpmvRows = kSpmvThreads / kSpmvLanes;  // 16 rows per plain matvec tile
 constexpr int kVecThreads = 256;  // rows per vector tile
 constexpr int kHeStride = 80;     // packed element Hessian (78 used) per tet
 constexpr int kReStride = 12;     // element residual per tet
@@ -101,13 +110,14 @@ struct FieldPtrs {
 // ---- assembly ----
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0,
-                    const T* v0, double dt, const double grav[3], int wantH, int project, T* Re, T* He);
+                    const T* v0, double dt, const double grav[3], int wantH, int project, T* Re, T* He,
+                    int block = 128, const KernelEvents* ev = nullptr);
 template <class T>
 void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, int wantH, T* Rc,
                     T* Hc);
 template <class T>
 void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* He, const T* Hc, int prec, T* vals,
-                       T* dinv);
+                       T* dinv, const KernelEvents* ev = nullptr);
 // acc: set 0 of the accumulators (zeroed by the caller); per-env ||res||^2.
 template <class T, class A>
 void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Re, const T* Rc, T* res, A* acc);
@@ -120,20 +130,36 @@ void launch_acc_finalize(hipStream_t s, int nEnv, int nv, const A* acc, double* 
 // ---- CG ----
 template <class A>
 void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, A* acc);
+// r0, z0, scaling and R_0.  warm (may be NULL): environments that keep x as
+// the initial guess and take r0 from r64 = b - A x (launch_residual64).
 template <class T, class A>
-void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const T* ax0, const T* dinv,
-                    int32_t nV, T* x, T* r, T* z, EnvCG* env, A* acc);
+void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const int32_t* warm,
+                    const double* r64, const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env, A* acc);
+// CG iteration k: w = A z (+ FP64 true residual on refresh iterations), with
+// `lanes` (4, 8, 16) lanes per block row and kSpmvThreads / lanes rows per tile.
 template <class T, class A>
-void launch_cg_spmv(hipStream_t s, const Tiles& ta, const CgConfig& cfg, int j, const int32_t* row_ptr,
-                    const int32_t* col, const T* vals, int32_t nB, const T* z, T* p0, T* p1, const T* x, const T* b,
-                    T* Ap, double* rref, EnvCG* env, A* acc, int32_t* num_active);
+void launch_cg_spmv(hipStream_t s, const Tiles& ta, const CgConfig& cfg, int k, int lanes, const int32_t* row_ptr,
+                    const int32_t* col, const T* vals, int32_t nB, const T* z, const A* x, const T* b, T* w,
+                    double* rref, EnvCG* env, A* acc, int32_t* num_active, const KernelEvents* ev = nullptr);
+// CG iteration k: p, s = A p, x, r, z updates.  The iterate x is kept in the
+// accumulator type A (FP64 in mixed precision).
 template <class T, class A>
-void launch_cg_update(hipStream_t s, const Tiles& tv, const CgConfig& cfg, int j, const T* dinv, int32_t nV,
-                      const T* p0, const T* p1, const T* Ap, const double* rref, T* x, T* r, T* z, EnvCG* env, A* acc,
-                      int32_t* num_active);
-// y = A x over all rows (plain matvec).
+void launch_cg_update(hipStream_t s, const Tiles& tv, const CgConfig& cfg, int k, const T* dinv, int32_t nV,
+                      const T* w, const double* rref, A* x, T* r, T* z, T* p, T* sv, EnvCG* env, A* acc,
+                      int32_t* num_active, const KernelEvents* ev = nullptr);
+// FP64 residual b - A x of the environments with mask[e]: into r64 when
+// given; |b - A x|^2 and |b|^2 into set 0 of acc when given (zeroed by the
+// caller).  Matvec tiles of kSpmvThreads / lanes rows.
+template <class T, class X>
+void launch_residual64(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
+                       const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* x,
+                       const T* b, double* r64, double* acc);
+// dst[i] = D(src[i])
+template <class S, class D>
+void launch_convert(hipStream_t s, int64_t n, const S* src, D* dst);
+// y = A x over all rows (plain matvec), `lanes` lanes per block row.
 template <class T>
-void launch_spmv_plain(hipStream_t s, const Tiles& ta, const int32_t* row_ptr, const int32_t* col,
+void launch_spmv_plain(hipStream_t s, const Tiles& ta, int lanes, const int32_t* row_ptr, const int32_t* col,
                        const T* vals, int32_t nB, const T* x, T* y);
 // Diagonal-block inverse of a BSR matrix (L1 entry; assembly fuses it).
 template <class T>

@@ -12,12 +12,15 @@
 // fixed order, so results are deterministic.
 //
 // CG (pcg.h:70-238, Polak-Ribiere) takes two kernels per iteration:
-//   cg_spmv:   p = z + beta p_old on the fly, Ap = A p, partial p.Ap
-//   cg_update: x += alpha p, r -= alpha Ap, z = D^-1 r, partial r.r, r.z, z.z, r.z_old
-// Per-environment scalars (p.Ap, r.r, r.z, z.z, r.z_old) are summed by spread
-// atomic accumulators; every block of the next kernel derives alpha, beta and
-// the stop test from them.  Tiles never straddle environments, so hundreds of
-// uneven environments are solved by the same launches.
+//   cg_spmv:   w = A z (the only matvec), partial z.w
+//   cg_update: p = z + beta p, s = w + beta s (= A p), x += alpha p,
+//              r -= alpha s, z = D^-1 r, partial r.r, r.z, z.z, r.z_old
+// Per-environment scalars are summed by spread atomic accumulators; every
+// block of the next kernel derives alpha, beta and the stop test from them.
+// Tiles never straddle environments, so hundreds of uneven environments are
+// solved by the same launches.
+#include <hip/hip_ext.h>
+
 #include "sdx_kernels.h"
 #include "sdx_math.h"
 
@@ -109,7 +112,7 @@ __device__ __forceinline__ void acc_zero(A* dst) {
 // ------------------------------------------------------------------ element
 
 template <class T>
-__global__ void __launch_bounds__(128) k_element(MeshView m, FieldPtrs<T> f, const T* __restrict__ u,
+__global__ void __launch_bounds__(256) k_element(MeshView m, FieldPtrs<T> f, const T* __restrict__ u,
                                                  const T* __restrict__ u0, const T* __restrict__ v0, T dt,
                                                  T gx, T gy, T gz, int wantH, int project, T* __restrict__ Re,
                                                  T* __restrict__ He) {
@@ -461,15 +464,27 @@ __global__ void __launch_bounds__(kVecThreads) k_gather_res(MeshView m, Tiles tv
 
 // ------------------------------------------------------------------ CG
 //
+// Single-gather Polak-Ribiere PCG.  With w = A z and s = A p kept by the
+// recurrence s_k = w_k + beta_k s_{k-1}, iteration k needs only one matvec
+// (w_k = A z_k) and one reduction point:
+//   gamma_k = r_k.z_k, rho_k = r_k.z_{k-1}, delta_k = z_k.w_k
+//   beta_k  = (gamma_k - [k >= 2] rho_k) / gamma_{k-1}          (pcg.h:205-219)
+//   mu_k    = p_k.s_k = delta_k + 2 beta_k (rho_k - gamma_k) / alpha_{k-1}
+//             + beta_k^2 mu_{k-1}                (exact for s = A p, symmetric A, M)
+//   alpha_k = gamma_k / mu_k
+// which is the reference iteration rearranged: p_k, x, r, z are the same
+// vectors in exact arithmetic.
+//
 // Accumulator sets per environment (kCgSets of them):
 //   0      scaling of the stop test from b: b.b, b.M^-1 b, |M^-1 b|^2
-//   1, 2   p.Ap of CG iteration j in set 1 + j % 2
-//   3..5   r.r, r.z, z.z, r.z_old after iteration j in set 3 + (j + 1) % 3;
-//          the initial residual (j = -1) in set 3
-// cg_update of iteration j zeroes the sets that iteration j + 1 writes.
+//   1, 2   R_k in set 1 + k % 2: r_k.r_k, r_k.z_k, z_k.z_k, r_k.z_{k-1}
+//   3, 4   D_k in set 3 + k % 2: z_k.w_k
+// cg_init writes R_0; cg_spmv(k) writes D_k and zeroes R_{k+1}; cg_update(k)
+// writes R_{k+1} and zeroes D_{k+1}.  Both kernels of iteration k re-derive
+// the same stop decision from R_k, so finished environments drop out of the
+// matvec as well as the update.
 
-constexpr int kSetScale = 0, kSetPAp = 1, kSetRes = 3;
-__host__ __device__ constexpr int res_set(int j) { return kSetRes + (j + 1 + 3) % 3; }
+constexpr int kSetScale = 0, kSetR = 1, kSetD = 3;
 
 template <class T>
 __device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int nV, int i, int prec, T r0, T r1, T r2,
@@ -502,8 +517,13 @@ __device__ __forceinline__ int stop_test(const CgConfig& cfg, double start, doub
   return kCgActive;
 }
 
-__device__ __forceinline__ double pick_norm(int norm_type, double rr, double rz, double zz) {
-  return norm_type == 0 ? rr : (norm_type == 1 ? zz : rz);
+// Status of an environment before update k (pcg.h:186-222): stop test on
+// r_k, a vanishing r.z, then the iteration budget.
+__device__ __forceinline__ int cg_decide(const CgConfig& cfg, int k, double start, double latest, double gamma) {
+  int st = stop_test(cfg, start, latest);
+  if (st == kCgActive && k >= 1 && gamma == 0.0) st = kCgDiverged;
+  if (st == kCgActive && k >= cfg.max_iter) st = kCgStopped;
+  return st;
 }
 
 // One block per environment: zero its accumulator sets and (re)start its CG state.
@@ -518,29 +538,33 @@ __global__ void k_cg_reset(int nEnv, const int32_t* __restrict__ solve, EnvCG* e
   }
 }
 
-// r0 = b - A x0 (or b), z0 = M^-1 r0; scaling from b (SetScaling).
+// r0 = b - A x0 or b, z0 = M^-1 r0; scaling from b (SetScaling); R_0.
+// Environments with warm[e] (warm may be NULL: none) keep x and take r0 from
+// r64 = b - A x0 formed in FP64 (k_residual64); the others start from x = 0.
 template <class T, class A>
 __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg, const T* __restrict__ b,
-                                                         const T* __restrict__ ax0, const T* __restrict__ dinv,
-                                                         int nV, T* __restrict__ x, T* __restrict__ r,
+                                                         const int32_t* __restrict__ warm,
+                                                         const double* __restrict__ r64, const T* __restrict__ dinv,
+                                                         int nV, A* __restrict__ x, T* __restrict__ r,
                                                          T* __restrict__ z, EnvCG* env, A* acc) {
   const int tile = blockIdx.x;
   const int e = tv.env[tile];
   if (env[e].status != kCgActive) return;
+  const bool w = warm && warm[e];
   const int i = tv.row0[tile] + threadIdx.x;
   A sc[3] = {A(0), A(0), A(0)};        // b.b, b.Mb, Mb.Mb
   A rs[4] = {A(0), A(0), A(0), A(0)};  // r.r, r.z, z.z, 0
   if (i < tv.row1[tile]) {
     const T b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
     T r0 = b0, r1 = b1, r2 = b2;
-    if (cfg.zero_guess) {
-      x[3 * i] = T(0);
-      x[3 * i + 1] = T(0);
-      x[3 * i + 2] = T(0);
+    if (w) {
+      r0 = T(r64[3 * i]);
+      r1 = T(r64[3 * i + 1]);
+      r2 = T(r64[3 * i + 2]);
     } else {
-      r0 -= ax0[3 * i];
-      r1 -= ax0[3 * i + 1];
-      r2 -= ax0[3 * i + 2];
+      x[3 * i] = A(0);
+      x[3 * i + 1] = A(0);
+      x[3 * i + 2] = A(0);
     }
     T zb0, zb1, zb2, z0, z1, z2;
     apply_dinv(dinv, nV, i, cfg.prec, b0, b1, b2, zb0, zb1, zb2);
@@ -560,20 +584,7 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg,
   }
   const int te = tile - tv.first[e];
   acc_add<A, 3>(sc, acc_ptr(acc, kSetScale, cfg.n_env, e), te);
-  acc_add<A, 4>(rs, acc_ptr(acc, res_set(-1), cfg.n_env, e), te);
-}
-
-// Sum over the kSlots lines of an accumulator set, one line per lane of each
-// 16-lane group, finished with shuffles: every lane ends with the totals.
-template <class A, int NV>
-__device__ __forceinline__ void acc_read_lanes(const A* src, double (&out)[NV]) {
-  const int sl = threadIdx.x & (kSlots - 1);
-#pragma unroll
-  for (int k = 0; k < NV; ++k) out[k] = double(src[sl * kSlotStride + k]);
-#pragma unroll
-  for (int off = kSlots / 2; off >= 1; off >>= 1)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] += __shfl_xor(out[k], off, kSlots);
+  acc_add<A, 4>(rs, acc_ptr(acc, kSetR, cfg.n_env, e), te);
 }
 
 // Finish an accumulator read started with one slot load per lane.
@@ -583,66 +594,56 @@ __device__ __forceinline__ double slot_reduce(double v) {
   return v;
 }
 
-// CG iteration j, first half.  Every block re-derives from the accumulators
-// the stop test after iteration j (pcg.h:186-222) and beta_j; the first tile
-// of a finished environment records its status.  Then Ap = A p_j with
-// p_j = z_j + beta_j p_{j-1} (accumulated as A z + beta A p_{j-1}, so the
-// matrix and vector loads do not wait for beta), p_j is stored, and p_j.Ap
-// goes into set 1 + j % 2.  On refresh iterations rref = b - A x_j is also
-// accumulated in FP64.
-template <class T, class A, bool kFirst, bool kRefresh>
-__global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(Tiles ta, CgConfig cfg, int j,
+// Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
+// run of tiles so the vector entries its matvec rows gather stay in its own L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  constexpr int kXcd = 8;
+  const int q = n / kXcd, rem = n % kXcd;
+  const int x = b % kXcd, i = b / kXcd;
+  return x * q + (x < rem ? x : rem) + i;
+}
+
+// CG iteration k, matvec half.  Every block re-derives the stop decision on
+// r_k from R_k; the first tile of a finished environment records it.  Then
+// w_k = A z_k is stored and z_k.w_k goes into D_k.  On refresh iterations the
+// true residual rref = b - A x_k is also formed in FP64 (residual replacement).
+template <class T, class A, bool kRefresh, int L>
+__global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(Tiles ta, CgConfig cfg, int k,
                                                           const int32_t* __restrict__ row_ptr,
                                                           const int32_t* __restrict__ col, const T* __restrict__ vals,
-                                                          int nB, const T* __restrict__ z, T* __restrict__ p0,
-                                                          T* __restrict__ p1, const T* __restrict__ x,
-                                                          const T* __restrict__ b, T* __restrict__ Ap,
+                                                          int nB, const T* __restrict__ z, const A* __restrict__ x,
+                                                          const T* __restrict__ b, T* __restrict__ w,
                                                           double* __restrict__ rref, EnvCG* env, A* acc,
                                                           int32_t* num_active) {
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int e = ta.env[tile];
   const int status = env[e].status;  // branched on after the row loads are issued
   const int nEnv = cfg.n_env;
-  constexpr bool first = kFirst;
-  T* __restrict__ pnew = (j & 1) ? p1 : p0;
-  const T* __restrict__ pold = (j & 1) ? p0 : p1;
-  constexpr bool refresh = kRefresh;
-  // raw accumulator slots (one per lane of each 16-lane group), issued first:
-  // scaling norm, latest norm, r.z, r.z_old after iteration j; r.z after j-1
+  // raw accumulator slots (one per lane of each 16-lane group), issued first
   const int sl = (threadIdx.x & (kSlots - 1)) * kSlotStride;
   const int nsel = cfg.norm_type == 0 ? 0 : (cfg.norm_type == 1 ? 2 : 1);
-  const A* s_sc = acc_ptr(acc, kSetScale, nEnv, e) + sl;
-  const A* s_rs = acc_ptr(acc, res_set(j - 1), nEnv, e) + sl;
-  double w_start = double(s_sc[nsel]);
-  double w_latest = double(s_rs[nsel]);
-  double w_rz = double(s_rs[1]);
-  double w_rzpr = j >= 2 ? double(s_rs[3]) : 0.0;
-  double w_prev = j >= 1 ? double(acc_ptr(acc, res_set(j - 2), nEnv, e)[sl + 1]) : 0.0;
-  const int i = ta.row0[tile] + (threadIdx.x / kSpmvLanes);
-  const int lane = threadIdx.x % kSpmvLanes;
+  const A* s_r = acc_ptr(acc, kSetR + (k & 1), nEnv, e) + sl;
+  double w_start = double(acc_ptr(acc, kSetScale, nEnv, e)[sl + nsel]);
+  double w_latest = double(s_r[nsel]);
+  double w_gamma = double(s_r[1]);
+  const int i = ta.row0[tile] + (threadIdx.x / L);
+  const int lane = threadIdx.x % L;
   const bool valid = i < ta.row1[tile];
-  // matrix rows: A z and A p_old
-  T a0 = T(0), a1 = T(0), a2 = T(0), c0 = T(0), c1 = T(0), c2 = T(0);
+  T a0 = T(0), a1 = T(0), a2 = T(0);
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;
-  T zi0 = T(0), zi1 = T(0), zi2 = T(0), oi0 = T(0), oi1 = T(0), oi2 = T(0);
+  T zi0 = T(0), zi1 = T(0), zi2 = T(0);
   if (valid) {
     const int k1 = row_ptr[i + 1];
-    for (int k = row_ptr[i] + lane; k < k1; k += kSpmvLanes) {
-      const int jc = col[k];
-      const T v0 = vals[k], v1 = vals[(size_t)nB + k], v2 = vals[2 * (size_t)nB + k];
-      const T v3 = vals[3 * (size_t)nB + k], v4 = vals[4 * (size_t)nB + k], v5 = vals[5 * (size_t)nB + k];
-      const T v6 = vals[6 * (size_t)nB + k], v7 = vals[7 * (size_t)nB + k], v8 = vals[8 * (size_t)nB + k];
+    for (int kk = row_ptr[i] + lane; kk < k1; kk += L) {
+      const int jc = col[kk];
+      const T v0 = vals[kk], v1 = vals[(size_t)nB + kk], v2 = vals[2 * (size_t)nB + kk];
+      const T v3 = vals[3 * (size_t)nB + kk], v4 = vals[4 * (size_t)nB + kk], v5 = vals[5 * (size_t)nB + kk];
+      const T v6 = vals[6 * (size_t)nB + kk], v7 = vals[7 * (size_t)nB + kk], v8 = vals[8 * (size_t)nB + kk];
       const T zj0 = z[3 * jc], zj1 = z[3 * jc + 1], zj2 = z[3 * jc + 2];
       a0 += v0 * zj0 + v1 * zj1 + v2 * zj2;
       a1 += v3 * zj0 + v4 * zj1 + v5 * zj2;
       a2 += v6 * zj0 + v7 * zj1 + v8 * zj2;
-      if (!first) {
-        const T oj0 = pold[3 * jc], oj1 = pold[3 * jc + 1], oj2 = pold[3 * jc + 2];
-        c0 += v0 * oj0 + v1 * oj1 + v2 * oj2;
-        c1 += v3 * oj0 + v4 * oj1 + v5 * oj2;
-        c2 += v6 * oj0 + v7 * oj1 + v8 * oj2;
-      }
-      if (refresh) {
+      if (kRefresh) {
         const double x0 = x[3 * jc], x1 = x[3 * jc + 1], x2 = x[3 * jc + 2];
         q0 += double(v0) * x0 + double(v1) * x1 + double(v2) * x2;
         q1 += double(v3) * x0 + double(v4) * x1 + double(v5) * x2;
@@ -653,111 +654,104 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(Tiles ta, CgConfig cfg
       zi0 = z[3 * i];
       zi1 = z[3 * i + 1];
       zi2 = z[3 * i + 2];
-      if (!first) {
-        oi0 = pold[3 * i];
-        oi1 = pold[3 * i + 1];
-        oi2 = pold[3 * i + 2];
-      }
     }
   }
   if (status != kCgActive) return;
-  // environment scalars: stop test after iteration j and beta_j
   const double start = slot_reduce(w_start);
   const double latest = slot_reduce(w_latest);
-  const double rz = slot_reduce(w_rz);
-  int st = stop_test(cfg, start, latest);
-  double beta = 0.0;
-  if (j >= 1) {
-    const double rzpr = slot_reduce(w_rzpr);  // Polak-Ribiere term, zero on the first iteration
-    const double prev = slot_reduce(w_prev);
-    if (st == kCgActive) {
-      if (rz == 0.0)
-        st = kCgDiverged;
-      else
-        beta = (rz - rzpr) / prev;
-    }
-  }
-  if (st == kCgActive && j >= cfg.max_iter) st = kCgStopped;
+  const double gamma = slot_reduce(w_gamma);
+  const int st = cg_decide(cfg, k, start, latest, gamma);
+  const bool lead = tile == ta.first[e];
   if (st != kCgActive) {
-    if (tile == ta.first[e] && threadIdx.x == 0) {
+    if (lead && threadIdx.x == 0) {
       EnvCG& E = env[e];
       E.status = st;
-      E.iter = j;
+      E.iter = k;
       E.latest = latest < 0.0 ? 0.0 : latest;
       E.start = start;
-      E.rz = rz;
+      E.rz = gamma;
       atomicSub(num_active, 1);
     }
     return;
   }
-  const T bt = T(beta);
-  a0 += bt * c0;
-  a1 += bt * c1;
-  a2 += bt * c2;
+  if (lead) acc_zero(acc_ptr(acc, kSetR + ((k + 1) & 1), nEnv, e));
 #pragma unroll
-  for (int off = kSpmvLanes / 2; off >= 1; off >>= 1) {
-    a0 += __shfl_xor(a0, off, kSpmvLanes);
-    a1 += __shfl_xor(a1, off, kSpmvLanes);
-    a2 += __shfl_xor(a2, off, kSpmvLanes);
-    if (refresh) {
-      q0 += __shfl_xor(q0, off, kSpmvLanes);
-      q1 += __shfl_xor(q1, off, kSpmvLanes);
-      q2 += __shfl_xor(q2, off, kSpmvLanes);
+  for (int off = L / 2; off >= 1; off >>= 1) {
+    a0 += __shfl_xor(a0, off, L);
+    a1 += __shfl_xor(a1, off, L);
+    a2 += __shfl_xor(a2, off, L);
+    if (kRefresh) {
+      q0 += __shfl_xor(q0, off, L);
+      q1 += __shfl_xor(q1, off, L);
+      q2 += __shfl_xor(q2, off, L);
     }
   }
   A part[1] = {A(0)};
   if (valid && lane == 0) {
-    const T pi0 = zi0 + bt * oi0, pi1 = zi1 + bt * oi1, pi2 = zi2 + bt * oi2;
-    pnew[3 * i] = pi0;
-    pnew[3 * i + 1] = pi1;
-    pnew[3 * i + 2] = pi2;
-    Ap[3 * i] = a0;
-    Ap[3 * i + 1] = a1;
-    Ap[3 * i + 2] = a2;
-    part[0] = A(pi0) * A(a0) + A(pi1) * A(a1) + A(pi2) * A(a2);
-    if (refresh) {
+    w[3 * i] = a0;
+    w[3 * i + 1] = a1;
+    w[3 * i + 2] = a2;
+    part[0] = A(zi0) * A(a0) + A(zi1) * A(a1) + A(zi2) * A(a2);
+    if (kRefresh) {
       rref[3 * i] = double(b[3 * i]) - q0;
       rref[3 * i + 1] = double(b[3 * i + 1]) - q1;
       rref[3 * i + 2] = double(b[3 * i + 2]) - q2;
     }
   }
-  acc_add<A, 1>(part, acc_ptr(acc, kSetPAp + (j & 1), nEnv, e), tile - ta.first[e]);
+  acc_add<A, 1>(part, acc_ptr(acc, kSetD + (k & 1), nEnv, e), tile - ta.first[e]);
 }
 
-// CG iteration j, second half: alpha_j = r.z / p.Ap; x += alpha p; r -= alpha Ap
-// (or r = rref - alpha Ap on refresh iterations); z = M^-1 r; r.r, r.z, z.z and
-// r.z_old into set res_set(j).  Row loads are issued before alpha is known.
+// CG iteration k, update half: beta_k, mu_k, alpha_k from R_k, D_k and the
+// previous iteration's scalars; p_k = z_k + beta p, s_k = w_k + beta s,
+// x += alpha p_k, r -= alpha s_k (r = rref - alpha s_k on refresh
+// iterations), z = M^-1 r; R_{k+1} partials.  Row loads are issued before
+// the scalars are known.
 template <class T, class A, bool kRefresh>
-__global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, CgConfig cfg, int j, const T* __restrict__ dinv,
-                                                           int nV, const T* __restrict__ p0, const T* __restrict__ p1,
-                                                           const T* __restrict__ Ap, const double* __restrict__ rref,
-                                                           T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                                           EnvCG* env, A* acc, int32_t* num_active) {
+__global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, CgConfig cfg, int k, const T* __restrict__ dinv,
+                                                           int nV, const T* __restrict__ w,
+                                                           const double* __restrict__ rref, A* __restrict__ x,
+                                                           T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
+                                                           T* __restrict__ s, EnvCG* env, A* acc,
+                                                           int32_t* num_active) {
   const int tile = blockIdx.x;
   const int e = tv.env[tile];
   const int status = env[e].status;  // branched on after the row loads are issued
   const int nEnv = cfg.n_env;
-  const T* __restrict__ p = (j & 1) ? p1 : p0;
-  constexpr bool refresh = kRefresh;
   const int sl = (threadIdx.x & (kSlots - 1)) * kSlotStride;
-  double w_pap = double(acc_ptr(acc, kSetPAp + (j & 1), nEnv, e)[sl]);
-  double w_rz = double(acc_ptr(acc, res_set(j - 1), nEnv, e)[sl + 1]);
+  const int nsel = cfg.norm_type == 0 ? 0 : (cfg.norm_type == 1 ? 2 : 1);
+  const A* s_r = acc_ptr(acc, kSetR + (k & 1), nEnv, e) + sl;
+  double w_start = double(acc_ptr(acc, kSetScale, nEnv, e)[sl + nsel]);
+  double w_latest = double(s_r[nsel]);
+  double w_gamma = double(s_r[1]);
+  double w_rho = double(s_r[3]);
+  double w_delta = double(acc_ptr(acc, kSetD + (k & 1), nEnv, e)[sl]);
+  const int pk = (k + 1) & 1;  // slot of the previous iteration's scalars
+  const double g_prev = env[e].gamma[pk], a_prev = env[e].alpha[pk], m_prev = env[e].mu[pk];
   const int i = tv.row0[tile] + threadIdx.x;
   const bool valid = i < tv.row1[tile];
-  T x0 = T(0), x1 = T(0), x2 = T(0), pp0 = T(0), pp1 = T(0), pp2 = T(0), ap0 = T(0), ap1 = T(0), ap2 = T(0);
+  A x0 = A(0), x1 = A(0), x2 = A(0);
+  T z0 = T(0), z1 = T(0), z2 = T(0), w0 = T(0), w1 = T(0), w2 = T(0);
+  T p0 = T(0), p1 = T(0), p2 = T(0), s0 = T(0), s1 = T(0), s2 = T(0);
   double ro0 = 0.0, ro1 = 0.0, ro2 = 0.0;
-  T zo0 = T(0), zo1 = T(0), zo2 = T(0);
   if (valid) {
     x0 = x[3 * i];
     x1 = x[3 * i + 1];
     x2 = x[3 * i + 2];
-    pp0 = p[3 * i];
-    pp1 = p[3 * i + 1];
-    pp2 = p[3 * i + 2];
-    ap0 = Ap[3 * i];
-    ap1 = Ap[3 * i + 1];
-    ap2 = Ap[3 * i + 2];
-    if (refresh) {
+    z0 = z[3 * i];
+    z1 = z[3 * i + 1];
+    z2 = z[3 * i + 2];
+    w0 = w[3 * i];
+    w1 = w[3 * i + 1];
+    w2 = w[3 * i + 2];
+    if (k > 0) {
+      p0 = p[3 * i];
+      p1 = p[3 * i + 1];
+      p2 = p[3 * i + 2];
+      s0 = s[3 * i];
+      s1 = s[3 * i + 1];
+      s2 = s[3 * i + 2];
+    }
+    if (kRefresh) {
       ro0 = rref[3 * i];
       ro1 = rref[3 * i + 1];
       ro2 = rref[3 * i + 2];
@@ -766,74 +760,154 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, CgConfig cf
       ro1 = r[3 * i + 1];
       ro2 = r[3 * i + 2];
     }
-    zo0 = z[3 * i];
-    zo1 = z[3 * i + 1];
-    zo2 = z[3 * i + 2];
   }
   if (status != kCgActive) return;
-  const double pap = slot_reduce(w_pap);
-  const double rzc = slot_reduce(w_rz);
+  const double start = slot_reduce(w_start);
+  const double latest = slot_reduce(w_latest);
+  const double gamma = slot_reduce(w_gamma);
+  if (cg_decide(cfg, k, start, latest, gamma) != kCgActive) return;  // recorded by cg_spmv(k)
+  const double rho = slot_reduce(w_rho);
+  const double delta = slot_reduce(w_delta);
+  double beta = 0.0, mu = delta;
+  if (k >= 1) {
+    beta = (gamma - (k >= 2 ? rho : 0.0)) / g_prev;
+    mu = delta + 2.0 * beta * (rho - gamma) / a_prev + beta * beta * m_prev;
+  }
   const bool lead = tile == tv.first[e];
-  if (pap <= 0.0 && cfg.abort_if_not_spd) {
-    // pcg.h:152-169: not SPD, abort with iteration j + 1
+  if (mu <= 0.0 && cfg.abort_if_not_spd) {
+    // pcg.h:152-169: not SPD, abort with iteration k + 1
     if (lead && threadIdx.x == 0) {
       EnvCG& E = env[e];
       E.status = kCgDiverged;
-      E.iter = j + 1;
+      E.iter = k + 1;
+      E.latest = latest < 0.0 ? 0.0 : latest;
+      E.start = start;
+      E.rz = gamma;
       atomicSub(num_active, 1);
     }
     return;
   }
+  const double alpha = gamma / mu;
   if (lead) {
-    // the sets iteration j + 1 writes
-    acc_zero(acc_ptr(acc, res_set(j + 1), nEnv, e));
-    acc_zero(acc_ptr(acc, kSetPAp + ((j + 1) & 1), nEnv, e));
+    acc_zero(acc_ptr(acc, kSetD + ((k + 1) & 1), nEnv, e));
+    if (threadIdx.x == 0) {
+      EnvCG& E = env[e];
+      E.gamma[k & 1] = gamma;
+      E.alpha[k & 1] = alpha;
+      E.mu[k & 1] = mu;
+    }
   }
-  const double alpha = rzc / pap;
-  const T al = T(alpha);
+  const T al = T(alpha), bt = T(beta);
   A part[4] = {A(0), A(0), A(0), A(0)};  // r.r, r.z, z.z, r.z_old
   if (valid) {
-    x[3 * i] = x0 + al * pp0;
-    x[3 * i + 1] = x1 + al * pp1;
-    x[3 * i + 2] = x2 + al * pp2;
+    p0 = z0 + bt * p0;
+    p1 = z1 + bt * p1;
+    p2 = z2 + bt * p2;
+    s0 = w0 + bt * s0;
+    s1 = w1 + bt * s1;
+    s2 = w2 + bt * s2;
+    p[3 * i] = p0;
+    p[3 * i + 1] = p1;
+    p[3 * i + 2] = p2;
+    s[3 * i] = s0;
+    s[3 * i + 1] = s1;
+    s[3 * i + 2] = s2;
+    // the iterate accumulates in the dot-product type (FP64 in mixed precision)
+    x[3 * i] = x0 + A(alpha) * A(p0);
+    x[3 * i + 1] = x1 + A(alpha) * A(p1);
+    x[3 * i + 2] = x2 + A(alpha) * A(p2);
     T r0, r1, r2;
-    if (refresh) {
-      r0 = T(ro0 - alpha * double(ap0));
-      r1 = T(ro1 - alpha * double(ap1));
-      r2 = T(ro2 - alpha * double(ap2));
+    if (kRefresh) {
+      r0 = T(ro0 - alpha * double(s0));
+      r1 = T(ro1 - alpha * double(s1));
+      r2 = T(ro2 - alpha * double(s2));
     } else {
-      r0 = T(ro0) - al * ap0;
-      r1 = T(ro1) - al * ap1;
-      r2 = T(ro2) - al * ap2;
+      r0 = T(ro0) - al * s0;
+      r1 = T(ro1) - al * s1;
+      r2 = T(ro2) - al * s2;
     }
     r[3 * i] = r0;
     r[3 * i + 1] = r1;
     r[3 * i + 2] = r2;
-    T z0, z1, z2;
-    apply_dinv(dinv, nV, i, cfg.prec, r0, r1, r2, z0, z1, z2);
-    z[3 * i] = z0;
-    z[3 * i + 1] = z1;
-    z[3 * i + 2] = z2;
+    T n0, n1, n2;
+    apply_dinv(dinv, nV, i, cfg.prec, r0, r1, r2, n0, n1, n2);
+    z[3 * i] = n0;
+    z[3 * i + 1] = n1;
+    z[3 * i + 2] = n2;
+    if (kRefresh) {
+      // the mu recurrence of the next iteration needs z_{k+1}.(r_k this update
+      // started from) = r_{k+1}.M^-1 rref: replace z_k by M^-1 rref
+      apply_dinv(dinv, nV, i, cfg.prec, T(ro0), T(ro1), T(ro2), z0, z1, z2);
+    }
     part[0] = A(r0) * A(r0) + A(r1) * A(r1) + A(r2) * A(r2);
-    part[1] = A(r0) * A(z0) + A(r1) * A(z1) + A(r2) * A(z2);
-    part[2] = A(z0) * A(z0) + A(z1) * A(z1) + A(z2) * A(z2);
-    part[3] = A(r0) * A(zo0) + A(r1) * A(zo1) + A(r2) * A(zo2);
+    part[1] = A(r0) * A(n0) + A(r1) * A(n1) + A(r2) * A(n2);
+    part[2] = A(n0) * A(n0) + A(n1) * A(n1) + A(n2) * A(n2);
+    part[3] = A(r0) * A(z0) + A(r1) * A(z1) + A(r2) * A(z2);
   }
-  acc_add<A, 4>(part, acc_ptr(acc, res_set(j), nEnv, e), tile - tv.first[e]);
+  acc_add<A, 4>(part, acc_ptr(acc, kSetR + ((k + 1) & 1), nEnv, e), tile - tv.first[e]);
 }
 
-template <class T>
+// FP64 residual b - A x of the environments with mask[e]: stored in r64
+// when given, and |b - A x|^2, |b|^2 added into set 0 of acc when given
+// (zeroed by the caller).
+template <class T, class X, int L>
+__global__ void __launch_bounds__(kSpmvThreads) k_residual64(Tiles ta, int nEnv, const int32_t* __restrict__ mask,
+                                                             const int32_t* __restrict__ row_ptr,
+                                                             const int32_t* __restrict__ col,
+                                                             const T* __restrict__ vals, int nB,
+                                                             const X* __restrict__ x, const T* __restrict__ b,
+                                                             double* __restrict__ r64, double* acc) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int e = ta.env[tile];
+  if (!mask[e]) return;
+  const int i = ta.row0[tile] + (threadIdx.x / L);
+  const int lane = threadIdx.x % L;
+  const bool valid = i < ta.row1[tile];
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+  if (valid) {
+    for (int kk = row_ptr[i] + lane; kk < row_ptr[i + 1]; kk += L) {
+      const int jc = col[kk];
+      const double x0 = x[3 * jc], x1 = x[3 * jc + 1], x2 = x[3 * jc + 2];
+      q0 += double(vals[kk]) * x0 + double(vals[(size_t)nB + kk]) * x1 + double(vals[2 * (size_t)nB + kk]) * x2;
+      q1 += double(vals[3 * (size_t)nB + kk]) * x0 + double(vals[4 * (size_t)nB + kk]) * x1 +
+            double(vals[5 * (size_t)nB + kk]) * x2;
+      q2 += double(vals[6 * (size_t)nB + kk]) * x0 + double(vals[7 * (size_t)nB + kk]) * x1 +
+            double(vals[8 * (size_t)nB + kk]) * x2;
+    }
+  }
+#pragma unroll
+  for (int off = L / 2; off >= 1; off >>= 1) {
+    q0 += __shfl_xor(q0, off, L);
+    q1 += __shfl_xor(q1, off, L);
+    q2 += __shfl_xor(q2, off, L);
+  }
+  double part[2] = {0.0, 0.0};
+  if (valid && lane == 0) {
+    const double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
+    const double d0 = b0 - q0, d1 = b1 - q1, d2 = b2 - q2;
+    if (r64) {
+      r64[3 * i] = d0;
+      r64[3 * i + 1] = d1;
+      r64[3 * i + 2] = d2;
+    }
+    part[0] = d0 * d0 + d1 * d1 + d2 * d2;
+    part[1] = b0 * b0 + b1 * b1 + b2 * b2;
+  }
+  if (acc) acc_add<double, 2>(part, acc_ptr(acc, 0, nEnv, e), tile - ta.first[e]);
+}
+
+template <class T, int L>
 __global__ void __launch_bounds__(kSpmvThreads) k_spmv_plain(Tiles ta, const int32_t* __restrict__ row_ptr,
                                                              const int32_t* __restrict__ col,
                                                              const T* __restrict__ vals, int nB,
                                                              const T* __restrict__ x, T* __restrict__ y) {
   const int tile = blockIdx.x;
-  const int i = ta.row0[tile] + (threadIdx.x / kSpmvLanes);
-  const int lane = threadIdx.x % kSpmvLanes;
+  const int i = ta.row0[tile] + (threadIdx.x / L);
+  const int lane = threadIdx.x % L;
   T a0 = T(0), a1 = T(0), a2 = T(0);
   const bool valid = i < ta.row1[tile];
   if (valid) {
-    for (int k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += kSpmvLanes) {
+    for (int k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += L) {
       const int j = col[k];
       const T x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
       a0 += vals[k] * x0 + vals[(size_t)nB + k] * x1 + vals[2 * (size_t)nB + k] * x2;
@@ -842,10 +916,10 @@ __global__ void __launch_bounds__(kSpmvThreads) k_spmv_plain(Tiles ta, const int
     }
   }
 #pragma unroll
-  for (int off = kSpmvLanes / 2; off >= 1; off >>= 1) {
-    a0 += __shfl_xor(a0, off, kSpmvLanes);
-    a1 += __shfl_xor(a1, off, kSpmvLanes);
-    a2 += __shfl_xor(a2, off, kSpmvLanes);
+  for (int off = L / 2; off >= 1; off >>= 1) {
+    a0 += __shfl_xor(a0, off, L);
+    a1 += __shfl_xor(a1, off, L);
+    a2 += __shfl_xor(a2, off, L);
   }
   if (valid && lane == 0) {
     y[3 * i] = a0;
@@ -936,12 +1010,23 @@ __global__ void k_fill(int64_t n, A v, A* __restrict__ d) {
 
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Plain launch, or an event-bracketed one (hipExtLaunchKernelGGL records the
+// events at the kernel's own dispatch) when ev is given.
+#define SDX_LAUNCH(ev, kern, grid, block, s, ...)                                                        \
+  do {                                                                                                 \
+    if (ev)                                                                                            \
+      hipExtLaunchKernelGGL(kern, grid, block, 0, s, (ev)->start, (ev)->stop, 0, __VA_ARGS__);         \
+    else                                                                                               \
+      hipLaunchKernelGGL(kern, grid, block, 0, s, __VA_ARGS__);                                        \
+  } while (0)
+
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0, const T* v0,
-                    double dt, const double grav[3], int wantH, int project, T* Re, T* He) {
+                    double dt, const double grav[3], int wantH, int project, T* Re, T* He, int block,
+                    const KernelEvents* ev) {
   if (m.nT == 0) return;
-  hipLaunchKernelGGL(k_element<T>, dim3(blocks_for(m.nT, 128)), dim3(128), 0, s, m, f, u, u0, v0, T(dt),
-                     T(grav[0]), T(grav[1]), T(grav[2]), wantH, project, Re, He);
+  SDX_LAUNCH(ev, k_element<T>, dim3(blocks_for(m.nT, block)), dim3(block), s, m, f, u, u0, v0, T(dt), T(grav[0]),
+             T(grav[1]), T(grav[2]), wantH, project, Re, He);
 }
 
 template <class T>
@@ -951,9 +1036,10 @@ void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, con
 }
 
 template <class T>
-void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* He, const T* Hc, int prec, T* vals, T* dinv) {
+void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* He, const T* Hc, int prec, T* vals, T* dinv,
+                       const KernelEvents* ev) {
   if (m.nB == 0) return;
-  hipLaunchKernelGGL(k_gather_bsr<T>, dim3(blocks_for(m.nB, 256)), dim3(256), 0, s, m, He, Hc, prec, vals, dinv);
+  SDX_LAUNCH(ev, k_gather_bsr<T>, dim3(blocks_for(m.nB, 256)), dim3(256), s, m, He, Hc, prec, vals, dinv);
 }
 
 template <class T, class A>
@@ -981,49 +1067,79 @@ void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, 
 }
 
 template <class T, class A>
-void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const T* ax0, const T* dinv,
-                    int32_t nV, T* x, T* r, T* z, EnvCG* env, A* acc) {
+void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const int32_t* warm,
+                    const double* r64, const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env, A* acc) {
   if (tv.n == 0) return;
-  hipLaunchKernelGGL((k_cg_init<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, b, ax0, dinv, nV, x, r, z,
-                     env, acc);
+  hipLaunchKernelGGL((k_cg_init<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, b, warm, r64, dinv, nV, x, r,
+                     z, env, acc);
 }
 
 template <class T, class A>
-void launch_cg_spmv(hipStream_t s, const Tiles& ta, const CgConfig& cfg, int j, const int32_t* row_ptr,
-                    const int32_t* col, const T* vals, int32_t nB, const T* z, T* p0, T* p1, const T* x, const T* b,
-                    T* Ap, double* rref, EnvCG* env, A* acc, int32_t* num_active) {
+void launch_cg_spmv(hipStream_t s, const Tiles& ta, const CgConfig& cfg, int k, int lanes, const int32_t* row_ptr,
+                    const int32_t* col, const T* vals, int32_t nB, const T* z, const A* x, const T* b, T* w,
+                    double* rref, EnvCG* env, A* acc, int32_t* num_active, const KernelEvents* ev) {
   if (ta.n == 0) return;
-  const bool refresh = cfg.refresh_every > 0 && ((j + 1) % cfg.refresh_every) == 0;
-#define SDX_SPMV(F, R)                                                                                          \
-  hipLaunchKernelGGL((k_cg_spmv<T, A, F, R>), dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, cfg, j, row_ptr, col, vals, \
-                     nB, z, p0, p1, x, b, Ap, rref, env, acc, num_active)
-  if (j == 0) {
-    if (refresh) SDX_SPMV(true, true); else SDX_SPMV(true, false);
-  } else {
-    if (refresh) SDX_SPMV(false, true); else SDX_SPMV(false, false);
+  const bool refresh = cg_refresh(cfg, k);
+#define SDX_SPMV(R, L)                                                                                           \
+  SDX_LAUNCH(ev, (k_cg_spmv<T, A, R, L>), dim3(ta.n), dim3(kSpmvThreads), s, ta, cfg, k, row_ptr, col, vals, nB, z, \
+             x, b, w, rref, env, acc, num_active)
+  switch (lanes) {
+    case 4:
+      if (refresh) SDX_SPMV(true, 4); else SDX_SPMV(false, 4);
+      break;
+    case 8:
+      if (refresh) SDX_SPMV(true, 8); else SDX_SPMV(false, 8);
+      break;
+    default:
+      if (refresh) SDX_SPMV(true, 16); else SDX_SPMV(false, 16);
+      break;
   }
 #undef SDX_SPMV
 }
 
 template <class T, class A>
-void launch_cg_update(hipStream_t s, const Tiles& tv, const CgConfig& cfg, int j, const T* dinv, int32_t nV,
-                      const T* p0, const T* p1, const T* Ap, const double* rref, T* x, T* r, T* z, EnvCG* env, A* acc,
-                      int32_t* num_active) {
+void launch_cg_update(hipStream_t s, const Tiles& tv, const CgConfig& cfg, int k, const T* dinv, int32_t nV,
+                      const T* w, const double* rref, A* x, T* r, T* z, T* p, T* sv, EnvCG* env, A* acc,
+                      int32_t* num_active, const KernelEvents* ev) {
   if (tv.n == 0) return;
-  const bool refresh = cfg.refresh_every > 0 && ((j + 1) % cfg.refresh_every) == 0;
-  if (refresh)
-    hipLaunchKernelGGL((k_cg_update<T, A, true>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, j, dinv, nV, p0, p1,
-                       Ap, rref, x, r, z, env, acc, num_active);
+  if (cg_refresh(cfg, k))
+    SDX_LAUNCH(ev, (k_cg_update<T, A, true>), dim3(tv.n), dim3(kVecThreads), s, tv, cfg, k, dinv, nV, w, rref, x, r,
+               z, p, sv, env, acc, num_active);
   else
-    hipLaunchKernelGGL((k_cg_update<T, A, false>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, j, dinv, nV, p0, p1,
-                       Ap, rref, x, r, z, env, acc, num_active);
+    SDX_LAUNCH(ev, (k_cg_update<T, A, false>), dim3(tv.n), dim3(kVecThreads), s, tv, cfg, k, dinv, nV, w, rref, x,
+               r, z, p, sv, env, acc, num_active);
+}
+
+template <class T, class X>
+void launch_residual64(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
+                       const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* x,
+                       const T* b, double* r64, double* acc) {
+  if (ta.n == 0) return;
+  auto kern = lanes == 4   ? k_residual64<T, X, 4>
+              : lanes == 8 ? k_residual64<T, X, 8>
+                           : k_residual64<T, X, 16>;
+  hipLaunchKernelGGL(kern, dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, nEnv, mask, row_ptr, col, vals, nB, x, b, r64,
+                     acc);
+}
+
+template <class S, class D>
+__global__ void k_convert(int64_t n, const S* __restrict__ s, D* __restrict__ d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = D(s[i]);
+}
+
+template <class S, class D>
+void launch_convert(hipStream_t s, int64_t n, const S* src, D* dst) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_convert<S, D>), dim3(blocks_for(n, 256)), dim3(256), 0, s, n, src, dst);
 }
 
 template <class T>
-void launch_spmv_plain(hipStream_t s, const Tiles& ta, const int32_t* row_ptr, const int32_t* col, const T* vals,
-                       int32_t nB, const T* x, T* y) {
+void launch_spmv_plain(hipStream_t s, const Tiles& ta, int lanes, const int32_t* row_ptr, const int32_t* col,
+                       const T* vals, int32_t nB, const T* x, T* y) {
   if (ta.n == 0) return;
-  hipLaunchKernelGGL(k_spmv_plain<T>, dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, row_ptr, col, vals, nB, x, y);
+  auto kern = lanes == 4 ? k_spmv_plain<T, 4> : lanes == 8 ? k_spmv_plain<T, 8> : k_spmv_plain<T, 16>;
+  hipLaunchKernelGGL(kern, dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, row_ptr, col, vals, nB, x, y);
 }
 
 template <class T>
@@ -1070,10 +1186,12 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
 
 #define SDX_INST_T(T)                                                                                            \
   template void launch_element<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, const T*,        \
-                                  const T*, double, const double*, int, int, T*, T*);                           \
+                                  const T*, double, const double*, int, int, T*, T*, int,                       \
+                                  const KernelEvents*);                                                         \
   template void launch_contact<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, int, T*, T*);    \
-  template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const T*, const T*, int, T*, T*);            \
-  template void launch_spmv_plain<T>(hipStream_t, const Tiles&, const int32_t*, const int32_t*, const T*,       \
+  template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const T*, const T*, int, T*, T*,             \
+                                     const KernelEvents*);                                                      \
+  template void launch_spmv_plain<T>(hipStream_t, const Tiles&, int, const int32_t*, const int32_t*, const T*,  \
                                      int32_t, const T*, T*);                                                    \
   template void launch_block_inverse<T>(hipStream_t, int32_t, const int32_t*, const T*, int32_t, int, T*);     \
   template void launch_copy<T>(hipStream_t, int64_t, const T*, T*);                                             \
@@ -1089,19 +1207,26 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
 
 #define SDX_INST_TA(T, A)                                                                                         \
   template void launch_gather_res<T, A>(hipStream_t, const MeshView&, const Tiles&, const T*, const T*, T*, A*); \
-  template void launch_cg_init<T, A>(hipStream_t, const Tiles&, const CgConfig&, const T*, const T*, const T*,   \
-                                     int32_t, T*, T*, T*, EnvCG*, A*);                                           \
-  template void launch_cg_spmv<T, A>(hipStream_t, const Tiles&, const CgConfig&, int, const int32_t*,            \
-                                     const int32_t*, const T*, int32_t, const T*, T*, T*, const T*, const T*,    \
-                                     T*, double*, EnvCG*, A*, int32_t*);                                         \
+  template void launch_cg_init<T, A>(hipStream_t, const Tiles&, const CgConfig&, const T*, const int32_t*,       \
+                                     const double*, const T*, int32_t, A*, T*, T*, EnvCG*, A*);                  \
+  template void launch_residual64<T, A>(hipStream_t, const Tiles&, int, int, const int32_t*, const int32_t*,    \
+                                        const int32_t*, const T*, int32_t, const A*, const T*, double*,         \
+                                        double*);                                                               \
+  template void launch_cg_spmv<T, A>(hipStream_t, const Tiles&, const CgConfig&, int, int, const int32_t*,       \
+                                     const int32_t*, const T*, int32_t, const T*, const A*, const T*, T*,        \
+                                     double*, EnvCG*, A*, int32_t*, const KernelEvents*);                        \
   template void launch_cg_update<T, A>(hipStream_t, const Tiles&, const CgConfig&, int, const T*, int32_t,       \
-                                       const T*, const T*, const T*, const double*, T*, T*, T*, EnvCG*, A*,      \
-                                       int32_t*);
+                                       const T*, const double*, A*, T*, T*, T*, T*, EnvCG*, A*, int32_t*,        \
+                                       const KernelEvents*);
 
 SDX_INST_T(float)
 SDX_INST_T(double)
 SDX_INST_TA(float, float)
 SDX_INST_TA(float, double)
 SDX_INST_TA(double, double)
+template void launch_convert<float, float>(hipStream_t, int64_t, const float*, float*);
+template void launch_convert<float, double>(hipStream_t, int64_t, const float*, double*);
+template void launch_convert<double, float>(hipStream_t, int64_t, const double*, float*);
+template void launch_convert<double, double>(hipStream_t, int64_t, const double*, double*);
 
 }  // namespace sdx

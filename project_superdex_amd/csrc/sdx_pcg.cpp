@@ -80,13 +80,15 @@ sdx_status solve(int32_t device, int32_t nbr, int32_t nnzb, const int32_t* ptr, 
   make_tiles(nbr, kVecThreads, te, t0, t1, tf);
   make_tiles(nbr, kSpmvRows, ae, a0, a1, af);
   const size_t n3 = size_t(3) * size_t(nbr);
-  std::vector<T> hb(n3), hx(n3);
+  std::vector<T> hb(n3);
+  std::vector<A> hx(n3);  // the iterate is kept in the accumulator type
   for (size_t k = 0; k < n3; ++k) {
     hb[k] = T(b[k]);
-    hx[k] = T(x[k]);
+    hx[k] = A(x[k]);
   }
   Tmp<int32_t> dptr(size_t(nbr) + 1), dcol(static_cast<size_t>(nnzb)), ddiag(static_cast<size_t>(nbr));
-  Tmp<T> dvals(vals.size()), ddinv(size_t(9) * size_t(nbr)), db(n3), dx(n3), dr(n3), dz(n3), dp0(n3), dp1(n3), dAp(n3);
+  Tmp<T> dvals(vals.size()), ddinv(size_t(9) * size_t(nbr)), db(n3), dr(n3), dz(n3), dp(n3), dsv(n3), dw(n3);
+  Tmp<A> dx(n3);
   Tmp<double> drref(n3);
   Tmp<int32_t> dte(te.size()), dt0(t0.size()), dt1(t1.size()), dtf(2), dae(ae.size()), da0(a0.size()),
       da1(a1.size()), daf(2);
@@ -118,22 +120,24 @@ sdx_status solve(int32_t device, int32_t nbr, int32_t nnzb, const int32_t* ptr, 
   cfg.max_iter = kp.max_iter;
   cfg.refresh_every = kp.precision_mode == SDX_PREC_MODE_MIXED ? kp.refresh_every : 0;
   cfg.abort_if_not_spd = kp.abort_if_not_spd;
-  cfg.zero_guess = kp.initial_guess_zero ? 1 : 0;
   cfg.abs_tol = kp.abs_tol;
   cfg.rel_tol = kp.rel_tol;
   cfg.rel_div_tol = kp.rel_div_tol;
   launch_block_inverse<T>(s, nbr, ddiag.p, dvals.p, nnzb, kp.preconditioner, ddinv.p);
-  if (!cfg.zero_guess) launch_spmv_plain<T>(s, ta, dptr.p, dcol.p, dvals.p, nnzb, dx.p, dAp.p);
+  const bool warm = !kp.initial_guess_zero;
+  if (warm)  // r0 = b - A x0 in FP64
+    launch_residual64<T, A>(s, ta, kSpmvLanes, 1, dsolve.p, dptr.p, dcol.p, dvals.p, nnzb, dx.p, db.p, drref.p, nullptr);
   A* acc = reinterpret_cast<A*>(dacc.p);
   launch_cg_reset<A>(s, 1, dsolve.p, denv.p, acc);
-  launch_cg_init<T, A>(s, tv, cfg, db.p, dAp.p, ddinv.p, nbr, dx.p, dr.p, dz.p, denv.p, acc);
+  launch_cg_init<T, A>(s, tv, cfg, db.p, warm ? dsolve.p : nullptr, drref.p, ddinv.p, nbr, dx.p, dr.p, dz.p, denv.p,
+                       acc);
   int j = 0;
   int32_t na = 1;
   while (true) {
     for (int k = 0; k < 8; ++k, ++j) {
-      launch_cg_spmv<T, A>(s, ta, cfg, j, dptr.p, dcol.p, dvals.p, nnzb, dz.p, dp0.p, dp1.p, dx.p, db.p, dAp.p,
-                           drref.p, denv.p, acc, dna.p);
-      launch_cg_update<T, A>(s, tv, cfg, j, ddinv.p, nbr, dp0.p, dp1.p, dAp.p, drref.p, dx.p, dr.p, dz.p, denv.p, acc,
+      launch_cg_spmv<T, A>(s, ta, cfg, j, kSpmvLanes, dptr.p, dcol.p, dvals.p, nnzb, dz.p, dx.p, db.p, dw.p, drref.p, denv.p,
+                           acc, dna.p);
+      launch_cg_update<T, A>(s, tv, cfg, j, ddinv.p, nbr, dw.p, drref.p, dx.p, dr.p, dz.p, dp.p, dsv.p, denv.p, acc,
                              dna.p);
     }
     hip_check(hipMemcpyAsync(&na, dna.p, sizeof(int32_t), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
@@ -142,7 +146,7 @@ sdx_status solve(int32_t device, int32_t nbr, int32_t nnzb, const int32_t* ptr, 
   }
   EnvCG e;
   hip_check(hipMemcpyAsync(&e, denv.p, sizeof(EnvCG), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-  hip_check(hipMemcpyAsync(hx.data(), dx.p, n3 * sizeof(T), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  hip_check(hipMemcpyAsync(hx.data(), dx.p, n3 * sizeof(A), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   for (size_t k = 0; k < n3; ++k) x[k] = double(hx[k]);
   if (out) {

@@ -8,6 +8,7 @@
 //   LineSearch::ResidualNorm .. mochi_core/include/mochi_core/solvers/line_search.h:300-335
 //   LinearSolver::Solve (CG) .. mochi_core/include/mochi_core/solvers/linear_solver.h:321-496
 //   soft stage integration .... mochi_physics/src/mochi_soft.cpp:94-163
+#include <chrono>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -155,6 +156,8 @@ World<T>::~World() {
 
 template <class T>
 void World<T>::release_device() {
+  drop_graphs();
+  tuned_ = false;
   for (auto* b : bufs_) b->free();
   finalized_ = false;
 }
@@ -315,7 +318,7 @@ void World<T>::finalize() {
   CountingCsr(nB_, slot_keys_c, slot_codes_c, slot_ptr_c, slot_code_c);
   CountingCsr(nV_, vtx_keys_c, vtx_codes_c, vtx_ptr_c, vtx_code_c);
   BuildTiles(env_v0_, nEnv, kVecThreads, tiles_v_);
-  BuildTiles(env_v0_, nEnv, kSpmvRows, tiles_a_);
+  BuildTiles(env_v0_, nEnv, kSpmvThreads / sched_.spmv_lanes, tiles_a_);
 
   // upload
   d_tets_.upload(tets);
@@ -349,12 +352,13 @@ void World<T>::finalize() {
   v0_.alloc(n3);
   res_.alloc(n3);
   dx_.alloc(n3);
+  xcg_.alloc(n3);
   sol0_.alloc(n3);
   r_.alloc(n3);
   z_.alloc(n3);
-  p0_.alloc(n3);
-  p1_.alloc(n3);
-  Ap_.alloc(n3);
+  p_.alloc(n3);
+  sv_.alloc(n3);
+  w_.alloc(n3);
   rref_.alloc(n3);
   Re_.alloc(size_t(kReStride) * size_t(nT_));
   He_.alloc(size_t(kHeStride) * size_t(nT_));
@@ -365,6 +369,7 @@ void World<T>::finalize() {
   env_cg_.alloc(static_cast<size_t>(nEnv));
   acc_.alloc(size_t(kCgSets) * size_t(nEnv) * size_t(kSlots) * size_t(kSlotStride));
   solve_mask_.alloc(static_cast<size_t>(nEnv));
+  warm_mask_.alloc(static_cast<size_t>(nEnv));
   num_active_.alloc(1);
   env_mask_.alloc(static_cast<size_t>(nEnv));
   ls_mask_.alloc(static_cast<size_t>(nEnv));
@@ -456,15 +461,12 @@ void World<T>::set_env_mask(const std::vector<int32_t>& mask) {
 template <class T>
 void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH) {
   const int project = sp.psd_proj_mode == SDX_PSD_ALWAYS ? 1 : 0;
-  timer_begin(2);
   launch_element<T>(stream_, mesh_, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, Re_.p,
-                    He_.p);
-  timer_end(2);
+                    He_.p, sched_.element_block, sampled_events(2));
   launch_contact<T>(stream_, mesh_, fields_, u_.p, wantH ? 1 : 0, Rc_.p, Hc_.p);
   if (wantH) {
-    timer_begin(3);
-    launch_gather_bsr<T>(stream_, mesh_, He_.p, Hc_.p, sp.linear.preconditioner, vals_.p, dinv_.p);
-    timer_end(3);
+    launch_gather_bsr<T>(stream_, mesh_, He_.p, Hc_.p, sp.linear.preconditioner, vals_.p, dinv_.p,
+                         sampled_events(3));
   }
   const int64_t set0 = int64_t(nEnv_) * kSlots * kSlotStride;
   if (precision_ == SDX_PREC_MODE_FP32) {
@@ -489,10 +491,156 @@ std::vector<double> World<T>::read_env_sq() {
   return h;
 }
 
-// Block-Jacobi PCG on the assembled matrix for the environments with
-// solve[e] != 0.  Solution in dx_; returns the per-environment CG state.
+// Launch CG iterations [k0, k0 + n) on the stream; with `timed`, every
+// matvec and update launch records its own dispatch events.
 template <class T>
-std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve) {
+void World<T>::issue_cg(const CgConfig& cfg, int k0, int n, bool timed) {
+  const bool dd = precision_ != SDX_PREC_MODE_FP32;
+  T* accf = reinterpret_cast<T*>(acc_.p);
+  for (int k = k0; k < k0 + n; ++k) {
+    const KernelEvents* e0 = timed ? timer_events(0) : nullptr;
+    if (dd)
+      launch_cg_spmv<T, double>(stream_, dt_a_.view, cfg, k, sched_.spmv_lanes, d_row_ptr_.p, d_col_.p, vals_.p, nB_, z_.p, xcg<double>(),
+                                res_.p, w_.p, rref_.p, env_cg_.p, acc_.p, num_active_.p, e0);
+    else
+      launch_cg_spmv<T, T>(stream_, dt_a_.view, cfg, k, sched_.spmv_lanes, d_row_ptr_.p, d_col_.p, vals_.p, nB_, z_.p, xcg<T>(), res_.p,
+                           w_.p, rref_.p, env_cg_.p, accf, num_active_.p, e0);
+    const KernelEvents* e1 = timed ? timer_events(1) : nullptr;
+    if (dd)
+      launch_cg_update<T, double>(stream_, dt_v_.view, cfg, k, dinv_.p, nV_, w_.p, rref_.p, xcg<double>(), r_.p, z_.p, p_.p,
+                                  sv_.p, env_cg_.p, acc_.p, num_active_.p, e1);
+    else
+      launch_cg_update<T, T>(stream_, dt_v_.view, cfg, k, dinv_.p, nV_, w_.p, rref_.p, xcg<T>(), r_.p, z_.p, p_.p,
+                             sv_.p, env_cg_.p, accf, num_active_.p, e1);
+  }
+}
+
+// CG burst m (iterations [m B, m B + B)) as a replayable graph, captured on
+// first use for the current configuration.
+template <class T>
+hipGraphExec_t World<T>::cg_graph(const CgConfig& cfg, int m, int B) {
+  if (std::memcmp(&graph_cfg_, &cfg, sizeof(CgConfig)) != 0 || graph_burst_ != B) {
+    drop_graphs();
+    graph_cfg_ = cfg;
+    graph_burst_ = B;
+  }
+  if (size_t(m) >= graphs_.size()) graphs_.resize(size_t(m) + 1, nullptr);
+  if (!graphs_[size_t(m)]) {
+    hipGraph_t g = nullptr;
+    hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    issue_cg(cfg, m * B, B, false);
+    hip_check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    const hipError_t err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hip_check(err, "hipGraphInstantiate");
+    graphs_[size_t(m)] = ex;
+  }
+  return graphs_[size_t(m)];
+}
+
+template <class T>
+void World<T>::drop_graphs() {
+  for (auto& g : graphs_)
+    if (g) (void)hipGraphExecDestroy(g);
+  graphs_.clear();
+}
+
+// One CG run for the environments with solve[e]; those with warm[e] keep
+// dx_ as the initial guess.  Returns the per-environment CG state.
+template <class T>
+std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve,
+                                    const std::vector<int32_t>& warm) {
+  std::vector<EnvCG> h(static_cast<size_t>(nEnv_));
+  int active = 0, n_warm = 0;
+  for (int e = 0; e < nEnv_; ++e) {
+    std::memset(&h[size_t(e)], 0, sizeof(EnvCG));
+    h[size_t(e)].status = solve[size_t(e)] ? kCgActive : kCgIdle;
+    active += solve[size_t(e)] ? 1 : 0;
+    n_warm += solve[size_t(e)] && warm[size_t(e)] ? 1 : 0;
+  }
+  if (active == 0) return h;
+  solve_mask_.upload_async(solve, stream_);
+  std::vector<int32_t> na(1, active);
+  num_active_.upload_async(na, stream_);
+  const int32_t* wm = n_warm ? warm_mask_.p : nullptr;
+  if (precision_ != SDX_PREC_MODE_FP32) {
+    if (n_warm) {
+      warm_mask_.upload_async(warm, stream_);
+      launch_residual64<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
+                                   d_col_.p, vals_.p, nB_, xcg<double>(), res_.p, rref_.p, nullptr);
+    }
+    launch_cg_reset<double>(stream_, nEnv_, solve_mask_.p, env_cg_.p, acc_.p);
+    launch_cg_init<T, double>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, dinv_.p, nV_, xcg<double>(), r_.p,
+                              z_.p, env_cg_.p, acc_.p);
+  } else {
+    if (n_warm) {
+      warm_mask_.upload_async(warm, stream_);
+      launch_residual64<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p, d_col_.p,
+                              vals_.p, nB_, xcg<T>(), res_.p, rref_.p, nullptr);
+    }
+    T* accf = reinterpret_cast<T*>(acc_.p);
+    launch_cg_reset<T>(stream_, nEnv_, solve_mask_.p, env_cg_.p, accf);
+    launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, dinv_.p, nV_, xcg<T>(), r_.p, z_.p,
+                         env_cg_.p, accf);
+  }
+  // Bursts of B iterations are issued one ahead of the host's check of the
+  // active count, so the device never waits for the host between bursts; a
+  // burst issued after every environment finished exits at once.
+  const int B = std::max(1, sched_.cg_burst);
+  int32_t* hna = pinned_active_;
+  for (int m = 0;; ++m) {
+    const bool issue = int64_t(m) * B <= int64_t(cfg.max_iter);
+    if (issue) {
+      const bool timed = burst_timed();
+      if (sched_.use_graphs && !timed)
+        hip_check(hipGraphLaunch(cg_graph(cfg, m, B), stream_), "hipGraphLaunch");
+      else
+        issue_cg(cfg, m * B, B, timed);
+      hip_check(hipMemcpyAsync(hna + (m & 1), num_active_.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream_),
+                "hipMemcpyAsync");
+      hip_check(hipEventRecord(burst_done_[m & 1], stream_), "hipEventRecord");
+    }
+    if (m > 0) {
+      hip_check(hipEventSynchronize(burst_done_[(m - 1) & 1]), "hipEventSynchronize");
+      if (hna[(m - 1) & 1] <= 0) break;
+    }
+    if (!issue) break;
+  }
+  env_cg_.download(h);
+  return h;
+}
+
+// FP64 ||b - A x|| / ||b|| of every environment with mask[e].
+template <class T>
+std::vector<double> World<T>::true_residual(const std::vector<int32_t>& mask) {
+  std::vector<double> q(size_t(2 * nEnv_)), out(size_t(nEnv_), 0.0);
+  solve_mask_.upload_async(mask, stream_);
+  launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
+  if (precision_ != SDX_PREC_MODE_FP32)
+    launch_residual64<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, solve_mask_.p, d_row_ptr_.p,
+                                 d_col_.p, vals_.p, nB_, xcg<double>(), res_.p, nullptr, acc_.p);
+  else
+    launch_residual64<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, solve_mask_.p, d_row_ptr_.p, d_col_.p,
+                            vals_.p, nB_, xcg<T>(), res_.p, nullptr, acc_.p);
+  launch_acc_finalize<double>(stream_, nEnv_, 2, acc_.p, norms2_.p);
+  norms2_.download(q);
+  for (int e = 0; e < nEnv_; ++e)
+    if (mask[size_t(e)] && q[size_t(2 * e + 1)] > 0) out[size_t(e)] = std::sqrt(q[size_t(2 * e)] / q[size_t(2 * e + 1)]);
+  return out;
+}
+
+// Block-Jacobi PCG on the assembled matrix for the environments with
+// solve[e] != 0, from x0 = 0 (or dx_ where warm[e] is set).  The iterate is
+// kept in the accumulator type (xcg_: FP64 in mixed precision); the solution
+// is rounded into dx_ at the end.
+// In mixed precision with the ResidualL2 norm, a solve that converged on the
+// recursive residual is confirmed against the FP64 true residual and, if
+// that misses the tolerance, continued from its iterate (at most twice).
+// true_rel receives the FP64 ||b - A x|| / ||b|| of every solved environment.
+template <class T>
+std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve,
+                                      std::vector<double>* true_rel, const std::vector<int32_t>* warm_in) {
   CgConfig cfg{};
   cfg.n_env = nEnv_;
   cfg.norm_type = kp.norm_type;
@@ -500,59 +648,50 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
   cfg.max_iter = kp.max_iter;
   cfg.refresh_every = precision_ == SDX_PREC_MODE_MIXED ? kp.refresh_every : 0;
   cfg.abort_if_not_spd = kp.abort_if_not_spd;
-  cfg.zero_guess = 1;
   cfg.abs_tol = kp.abs_tol;
   cfg.rel_tol = kp.rel_tol;
   cfg.rel_div_tol = kp.rel_div_tol;
-  std::vector<EnvCG> h(static_cast<size_t>(nEnv_));
-  int active = 0;
-  for (int e = 0; e < nEnv_; ++e) {
-    std::memset(&h[size_t(e)], 0, sizeof(EnvCG));
-    h[size_t(e)].status = solve[size_t(e)] ? kCgActive : kCgIdle;
-    active += solve[size_t(e)] ? 1 : 0;
-  }
-  if (active == 0) return h;
-  solve_mask_.upload_async(solve, stream_);
-  std::vector<int32_t> na(1, active);
-  num_active_.upload_async(na, stream_);
+  std::vector<int32_t> warm = warm_in ? *warm_in : std::vector<int32_t>(size_t(nEnv_), 0);
+  std::vector<int32_t> todo = solve;
+  const int64_t n3 = int64_t(3) * nV_;
   const bool dd = precision_ != SDX_PREC_MODE_FP32;
-  T* accf = reinterpret_cast<T*>(acc_.p);
-  if (dd) {
-    launch_cg_reset<double>(stream_, nEnv_, solve_mask_.p, env_cg_.p, acc_.p);
-    launch_cg_init<T, double>(stream_, dt_v_.view, cfg, res_.p, nullptr, dinv_.p, nV_, dx_.p, r_.p, z_.p, env_cg_.p,
-                              acc_.p);
-  } else {
-    launch_cg_reset<T>(stream_, nEnv_, solve_mask_.p, env_cg_.p, accf);
-    launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, nullptr, dinv_.p, nV_, dx_.p, r_.p, z_.p, env_cg_.p, accf);
+  if (warm_in) {  // initial guesses arrive in dx_
+    if (dd)
+      launch_convert<T, double>(stream_, n3, dx_.p, xcg<double>());
+    else
+      launch_convert<T, T>(stream_, n3, dx_.p, xcg<T>());
   }
-  const int burst = cg_burst_;
-  int j = 0;
-  int32_t* hna = pinned_active_;
-  while (true) {
-    for (int k = 0; k < burst; ++k, ++j) {
-      timer_begin(0);
-      if (dd)
-        launch_cg_spmv<T, double>(stream_, dt_a_.view, cfg, j, d_row_ptr_.p, d_col_.p, vals_.p, nB_, z_.p, p0_.p,
-                                  p1_.p, dx_.p, res_.p, Ap_.p, rref_.p, env_cg_.p, acc_.p, num_active_.p);
-      else
-        launch_cg_spmv<T, T>(stream_, dt_a_.view, cfg, j, d_row_ptr_.p, d_col_.p, vals_.p, nB_, z_.p, p0_.p, p1_.p,
-                             dx_.p, res_.p, Ap_.p, rref_.p, env_cg_.p, accf, num_active_.p);
-      timer_end(0);
-      timer_begin(1);
-      if (dd)
-        launch_cg_update<T, double>(stream_, dt_v_.view, cfg, j, dinv_.p, nV_, p0_.p, p1_.p, Ap_.p, rref_.p, dx_.p,
-                                    r_.p, z_.p, env_cg_.p, acc_.p, num_active_.p);
-      else
-        launch_cg_update<T, T>(stream_, dt_v_.view, cfg, j, dinv_.p, nV_, p0_.p, p1_.p, Ap_.p, rref_.p, dx_.p, r_.p,
-                               z_.p, env_cg_.p, accf, num_active_.p);
-      timer_end(1);
+  std::vector<EnvCG> h = cg_run(cfg, todo, warm);
+  std::vector<double> tr = true_residual(solve);
+  const bool confirm = precision_ == SDX_PREC_MODE_MIXED && kp.norm_type == SDX_NORM_RESIDUAL_L2;
+  for (int round = 0; confirm && round < 2; ++round) {
+    int n = 0;
+    for (int e = 0; e < nEnv_; ++e) {
+      const EnvCG& c = h[size_t(e)];
+      const bool redo = todo[size_t(e)] && c.status == kCgConverged && tr[size_t(e)] > kp.rel_tol &&
+                        tr[size_t(e)] * std::sqrt(c.start) > kp.abs_tol && c.iter < kp.max_iter;
+      todo[size_t(e)] = redo ? 1 : 0;
+      warm[size_t(e)] = todo[size_t(e)];
+      n += redo ? 1 : 0;
     }
-    hip_check(hipMemcpyAsync(hna, num_active_.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
-    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    flush_timers();
-    if (*hna <= 0 || j > kp.max_iter + 1) break;
+    if (n == 0) break;
+    CgConfig c2 = cfg;
+    std::vector<EnvCG> h2 = cg_run(c2, todo, warm);
+    std::vector<double> tr2 = true_residual(todo);
+    for (int e = 0; e < nEnv_; ++e) {
+      if (!todo[size_t(e)]) continue;
+      const int prev = h[size_t(e)].iter;
+      h[size_t(e)] = h2[size_t(e)];
+      h[size_t(e)].iter += prev;
+      tr[size_t(e)] = tr2[size_t(e)];
+    }
   }
-  env_cg_.download(h);
+  if (dd)  // the Newton update takes the solution in the state type
+    launch_convert<double, T>(stream_, n3, xcg<double>(), dx_.p);
+  else
+    launch_convert<T, T>(stream_, n3, xcg<T>(), dx_.p);
+  flush_timers();
+  if (true_rel) *true_rel = tr;
   if (cfg.refresh_every > 0)
     for (auto& e : h) e.refreshes = e.iter / cfg.refresh_every;
   return h;
@@ -562,6 +701,7 @@ template <class T>
 void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stats, int n_stats) {
   if (!(dt > 0)) throw std::invalid_argument("dt must be positive");
   finalize();
+  if (sp.schedule == -1 && !tuned_) autotune(dt, sp, 2);
   if (colliders_dirty_) upload_colliders();
   const int nE = nEnv_;
   const size_t n3 = size_t(3) * size_t(nV_);
@@ -576,7 +716,7 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
     double res0 = 0, res = 0;
     int conv = SDX_CONV_NONE;
     int lin_iters = 0, ls_iters = 0, lin_conv = SDX_CONV_NONE, refreshes = 0;
-    double lin_rel = 0;
+    double lin_rel = 0, lin_true_rel = 0;
     double dx_norm = std::numeric_limits<double>::infinity();
     double ls_res0 = 0;
     bool have_dx = false;
@@ -638,7 +778,8 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
     for (int e = 0; e < nE; ++e)
       if (S[size_t(e)].active) S[size_t(e)].iters++;
     // linear solve (zero initial guess), solution in dx_
-    auto cg = cg_solve(sp.linear, mask);
+    std::vector<double> true_rel;
+    auto cg = cg_solve(sp.linear, mask, &true_rel, nullptr);
     std::vector<int32_t> lin_ok(size_t(nE), 0);
     for (int e = 0; e < nE; ++e) {
       EnvState& s = S[size_t(e)];
@@ -649,6 +790,7 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
                    : c.status == kCgDiverged ? SDX_CONV_DIVERGED
                                              : SDX_CONV_STOPPED;
       s.lin_rel = c.start > 0 ? std::sqrt(c.latest / c.start) : 0.0;
+      s.lin_true_rel = true_rel[size_t(e)];
       s.refreshes += c.refreshes;
       s.have_dx = true;
       lin_ok[size_t(e)] = s.lin_conv == SDX_CONV_CONVERGED;
@@ -741,6 +883,7 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       o.res_norm0 = s.res0;
       o.res_norm = s.res;
       o.linear_rel_res = s.lin_rel;
+      o.linear_true_rel_res = s.lin_true_rel;
     }
   }
 }
@@ -817,6 +960,86 @@ void World<T>::assemble_actor(int a, double dt, const sdx_solver_params& sp, dou
       for (size_t k = 0; k < h.size(); ++k) vals[9 * k + size_t(c)] = double(h[k]);
     }
   }
+}
+
+template <class T>
+void World<T>::set_schedule(const sdx_schedule& sc) {
+  const bool lanes_changed = sc.spmv_lanes != sched_.spmv_lanes;
+  sched_ = sc;
+  drop_graphs();
+  if (finalized_ && lanes_changed) {
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    BuildTiles(env_v0_, nEnv_, kSpmvThreads / sched_.spmv_lanes, tiles_a_);
+    upload_tiles(tiles_a_, dt_a_);
+  }
+}
+
+// Coordinate search over the schedule: each candidate runs `trials` timed
+// steps (after one untimed step that captures its graphs) from the same saved
+// state; the state is restored afterwards.
+template <class T>
+double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials) {
+  finalize();
+  if (colliders_dirty_) upload_colliders();
+  sdx_solver_params sp = sp_in;
+  sp.schedule = 0;
+  const int64_t n3 = int64_t(3) * nV_;
+  std::vector<DBufBase*> reg;
+  DBuf<T> su(reg, &stream_), sv(reg, &stream_);
+  su.alloc(size_t(n3));
+  sv.alloc(size_t(n3));
+  launch_copy<T>(stream_, n3, u_.p, su.p);
+  launch_copy<T>(stream_, n3, v_.p, sv.p);
+  auto restore = [&]() {
+    launch_copy<T>(stream_, n3, su.p, u_.p);
+    launch_copy<T>(stream_, n3, sv.p, v_.p);
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  };
+  auto measure = [&](const sdx_schedule& c) {
+    set_schedule(c);
+    double best = std::numeric_limits<double>::infinity();
+    for (int t = 0; t <= trials; ++t) {
+      restore();
+      const auto t0 = std::chrono::steady_clock::now();
+      step(dt, sp, nullptr, 0);
+      const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      if (t > 0) best = std::min(best, ms);
+    }
+    return best;
+  };
+  sdx_schedule cur = sched_;
+  double cur_ms = measure(cur);
+  auto consider = [&](sdx_schedule c) {
+    const double ms = measure(c);
+    if (ms < 0.98 * cur_ms) {
+      cur = c;
+      cur_ms = ms;
+    }
+  };
+  for (int L : {4, 8, 16})
+    if (L != cur.spmv_lanes) {
+      sdx_schedule c = cur;
+      c.spmv_lanes = L;
+      consider(c);
+    }
+  for (int eb : {64, 128, 256})
+    if (eb != cur.element_block) {
+      sdx_schedule c = cur;
+      c.element_block = eb;
+      consider(c);
+    }
+  for (int g : {1, 0})
+    for (int B : {4, 8, 16, 32})
+      if (g != cur.use_graphs || B != cur.cg_burst) {
+        sdx_schedule c = cur;
+        c.use_graphs = g;
+        c.cg_burst = B;
+        consider(c);
+      }
+  set_schedule(cur);
+  restore();
+  tuned_ = true;
+  return cur_ms;
 }
 
 template <class T>

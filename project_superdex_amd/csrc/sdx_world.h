@@ -95,14 +95,17 @@ class WorldBase {
   WorldBase(int device, int precision) : device_(device), precision_(precision) {
     hip_check(hipSetDevice(device), "hipSetDevice");
     hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&pinned_active_), sizeof(int32_t)), "hipHostMalloc");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&pinned_active_), 2 * sizeof(int32_t)), "hipHostMalloc");
+    for (auto& ev : burst_done_) hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
   }
   virtual ~WorldBase() {
     for (auto& kv : timers_)
       for (auto& ev : kv.events) {
-        (void)hipEventDestroy(ev.first);
-        (void)hipEventDestroy(ev.second);
+        (void)hipEventDestroy(ev.start);
+        (void)hipEventDestroy(ev.stop);
       }
+    for (auto& ev : burst_done_)
+      if (ev) (void)hipEventDestroy(ev);
     if (pinned_active_) (void)hipHostFree(pinned_active_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
@@ -146,9 +149,13 @@ class WorldBase {
                               int32_t* col, double* vals) = 0;
   virtual void info(int64_t out[8]) = 0;
   virtual void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) = 0;
+  virtual void set_schedule(const sdx_schedule& s) = 0;
+  virtual double autotune(double dt, const sdx_solver_params& sp, int trials) = 0;
+  const sdx_schedule& schedule() const { return sched_; }
 
-  // kernel timing: class 0 CG matvec, 1 CG update, 2 element, 3 BSR gather
-  // on > 0: time every on-th launch of the class (1: every launch); 0: off.
+  // kernel timing: class 0 CG matvec, 1 CG update, 2 element, 3 BSR gather.
+  // on > 0: time every on-th launch of an assembly class, and every launch of
+  // every on-th CG burst (bursts otherwise replay as graphs); 0: off.
   void timing_enable(int k, int on) {
     auto& t = timers_[size_t(k)];
     t.enabled = on > 0;
@@ -162,36 +169,43 @@ class WorldBase {
     flush_timers();
     return {double(timers_[size_t(k)].launches), timers_[size_t(k)].ms};
   }
-  int cg_burst_ = 8;
+  sdx_schedule sched_{8, 1, 16, 128};
+  bool tuned_ = false;  // the schedule was autotuned for the current topology
 
  protected:
   struct Timer {
     bool enabled = false;
-    bool active = false;  // the current launch is being timed
     int every = 1;
     int64_t counter = 0;
     size_t used = 0;
     int64_t launches = 0;
     double ms = 0.0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<KernelEvents> events;
   };
-  void timer_begin(int k) {
+  // Next event pair of class k (nullptr when the class is not timed).
+  const KernelEvents* timer_events(int k) {
     Timer& t = timers_[size_t(k)];
-    t.active = t.enabled && (t.counter++ % t.every) == 0;
-    if (!t.active) return;
+    if (!t.enabled) return nullptr;
     if (t.used == t.events.size()) {
-      std::pair<hipEvent_t, hipEvent_t> ev;
-      hip_check(hipEventCreate(&ev.first), "hipEventCreate");
-      hip_check(hipEventCreate(&ev.second), "hipEventCreate");
+      KernelEvents ev{};
+      hip_check(hipEventCreate(&ev.start), "hipEventCreate");
+      hip_check(hipEventCreate(&ev.stop), "hipEventCreate");
       t.events.push_back(ev);
     }
-    hip_check(hipEventRecord(t.events[t.used].first, stream_), "hipEventRecord");
+    return &t.events[t.used++];
   }
-  void timer_end(int k) {
+  // Sampled launch of an assembly class: events for every `every`-th launch.
+  const KernelEvents* sampled_events(int k) {
     Timer& t = timers_[size_t(k)];
-    if (!t.active) return;
-    hip_check(hipEventRecord(t.events[t.used].second, stream_), "hipEventRecord");
-    t.used++;
+    if (!t.enabled || (t.counter++ % t.every) != 0) return nullptr;
+    return timer_events(k);
+  }
+  // Whether the next CG burst is launched eagerly with its launches timed.
+  bool burst_timed() {
+    Timer& t = timers_[0];
+    const bool on = timers_[0].enabled || timers_[1].enabled;
+    if (!on) return false;
+    return (t.counter++ % t.every) == 0;
   }
   // Call after a stream synchronisation: folds recorded event pairs into totals.
   void flush_timers() {
@@ -200,7 +214,7 @@ class WorldBase {
       hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
       for (size_t i = 0; i < t.used; ++i) {
         float ms = 0.f;
-        hip_check(hipEventElapsedTime(&ms, t.events[i].first, t.events[i].second), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&ms, t.events[i].start, t.events[i].stop), "hipEventElapsedTime");
         t.ms += double(ms);
       }
       t.launches += int64_t(t.used);
@@ -211,7 +225,8 @@ class WorldBase {
   int device_;
   int precision_;
   hipStream_t stream_ = nullptr;
-  int32_t* pinned_active_ = nullptr;
+  int32_t* pinned_active_ = nullptr;  // two slots: active count after bursts m - 1, m
+  hipEvent_t burst_done_[2] = {nullptr, nullptr};
   std::vector<HostActor> actors_;
   std::vector<HostCollider> colliders_;
   bool colliders_dirty_ = true;
@@ -234,6 +249,8 @@ class World final : public WorldBase {
                       double* vals) override;
   void info(int64_t out[8]) override;
   void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) override;
+  void set_schedule(const sdx_schedule& s) override;
+  double autotune(double dt, const sdx_solver_params& sp, int trials) override;
 
  private:
   struct DevTiles {
@@ -247,7 +264,20 @@ class World final : public WorldBase {
   void set_env_mask(const std::vector<int32_t>& mask);
   void assemble(double dt, const sdx_solver_params& sp, bool wantH);
   std::vector<double> read_env_sq();
-  std::vector<EnvCG> cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve);
+  std::vector<EnvCG> cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve,
+                              std::vector<double>* true_rel, const std::vector<int32_t>* warm);
+  std::vector<EnvCG> cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve,
+                            const std::vector<int32_t>& warm);
+  std::vector<double> true_residual(const std::vector<int32_t>& mask);
+  void issue_cg(const CgConfig& cfg, int k0, int n, bool timed);
+  hipGraphExec_t cg_graph(const CgConfig& cfg, int m, int B);
+  void drop_graphs();
+  template <class A>
+  A* xcg() { return reinterpret_cast<A*>(xcg_.p); }
+
+  std::vector<hipGraphExec_t> graphs_;
+  CgConfig graph_cfg_{};
+  int graph_burst_ = 0;
 
   std::vector<DBufBase*> bufs_;
   int nEnv_ = 0, nV_ = 0, nT_ = 0, nB_ = 0, nF_ = 0;
@@ -288,12 +318,13 @@ class World final : public WorldBase {
   SDX_BUF(T, v0_)
   SDX_BUF(T, res_)
   SDX_BUF(T, dx_)
+  SDX_BUF(double, xcg_)  // CG iterate in the accumulator type (float in FP32 mode)
   SDX_BUF(T, sol0_)
   SDX_BUF(T, r_)
   SDX_BUF(T, z_)
-  SDX_BUF(T, p0_)
-  SDX_BUF(T, p1_)
-  SDX_BUF(T, Ap_)
+  SDX_BUF(T, p_)
+  SDX_BUF(T, sv_)
+  SDX_BUF(T, w_)
   SDX_BUF(double, rref_)
   SDX_BUF(T, Re_)
   SDX_BUF(T, He_)
@@ -304,6 +335,7 @@ class World final : public WorldBase {
   SDX_BUF(EnvCG, env_cg_)
   SDX_BUF(double, acc_)
   SDX_BUF(int32_t, solve_mask_)
+  SDX_BUF(int32_t, warm_mask_)
   SDX_BUF(int32_t, num_active_)
   SDX_BUF(int32_t, env_mask_)
   SDX_BUF(int32_t, ls_mask_)

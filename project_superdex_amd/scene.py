@@ -157,6 +157,26 @@ class Scene:
     def set_cg_burst(self, burst: int):
         check(lib().sdx_set_cg_burst(self._w, burst))
 
+    def schedule(self) -> dict:
+        sc = _abi.Schedule()
+        check(lib().sdx_get_schedule(self._w, C.byref(sc)))
+        return sc.as_dict()
+
+    def set_schedule(self, **kw):
+        sc = _abi.Schedule()
+        check(lib().sdx_get_schedule(self._w, C.byref(sc)))
+        for k, v in kw.items():
+            setattr(sc, k, int(v))
+        check(lib().sdx_set_schedule(self._w, C.byref(sc)))
+
+    def autotune(self, dt: float, params: _abi.SolverParams, trials: int = 2):
+        """Time candidate schedules on the current state (restored afterwards);
+        returns (chosen schedule, its step time in ms)."""
+        sc = _abi.Schedule()
+        ms = C.c_double()
+        check(lib().sdx_autotune(self._w, dt, C.byref(params), trials, C.byref(sc), C.byref(ms)))
+        return sc.as_dict(), ms.value
+
     def kernel_timing(self, kernel: int, enable: int = -1):
         """enable 1 clears and starts timing kernel class `kernel`; returns (launches, total ms)."""
         out = (C.c_double * 2)()
