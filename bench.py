@@ -1,17 +1,28 @@
 """Throughput of the implicit soft-body FEM step on one MI355X.
 
-Workload (default): one soft cube of 40^3 hexes split 6 tets/hex (384,000
-tets, 68,921 vertices), Neo-Hookean E=1e5 Pa, nu=0.4, rho=1000, bottom face
-pinned, gravity -Y, rigid plate pressing 2 mm down over 20 steps (penalty
-k=1e6 Pa/m); backward Euler dt=1e-3, 1 Newton iteration per step,
-block-Jacobi PCG to relative residual 1e-6 (ResidualL2) with FP32 matvecs,
-FP64 dot products and an FP64 residual refresh every 10 iterations.
+Workloads (all synthetic, seeded, resident in HBM before the timed region):
 
-A "step" is one Scene::Step of that workload; everything it needs is
-resident in HBM before the timed region.  The JSON line reports
+  batch512 (default)  512 independent soft cubes, n ~ U{6..16} hexes per side
+                      (h = 1 cm, 6 tets/hex), E log-uniform in [1e4, 1e6] Pa,
+                      nu ~ U(0.3, 0.45), bottom face pinned, each pressed by a
+                      sphere indenter (r = 2 cm, 5 cm/s down, penalty 1e6),
+                      solved together: one set of launches per CG iteration for
+                      all environments, CG bursts replayed as HIP graphs, launch
+                      schedule chosen by the autotuner on the first step.
+  press40             one 40^3-hex cube (384,000 tets) under a 2 mm plate press.
+  full4096            4096 cubes, n ~ U{4..12}, 1-3 indenters with random
+                      velocities |v| <= 5 cm/s, 2 Newton iterations per step.
+
+Common: backward Euler dt = 1e-3, block-Jacobi PCG to relative residual 1e-6
+(ResidualL2, confirmed by an FP64 recompute of |b - A x| at exit), FP32 matvec
+with FP64 dots, FP64 iterate and an FP64 residual refresh every 10 iterations.
+
+A "step" is one Scene::Step of the workload.  The JSON line reports
 tet-steps/s, the roofline of the dominant kernel (the CG matvec, timed with
-HIP events on the world's stream during the timed region) and the CPU
-oracle timed on a bounded sample of the same workload on this host.
+HIP events at its own dispatch during the timed region), the plain GPU PCG
+path on a sample of the same environments (one environment per launch
+sequence, zero initial guess, eager launches), the position error against an
+FP64 run of the same steps, and the CPU oracle on a bounded sample.
 """
 from __future__ import annotations
 
@@ -30,53 +41,132 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "tet-steps/sec at rel. residual 1e-6, learned+batched vs plain PCG, 1 GPU"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cg_spmv_press40_traffic.txt")
 
 
 def matvec_bytes(nB: int, nV: int, s: int = 4) -> int:
-    """Algorithmic HBM bytes of one CG matvec launch (k_cg_spmv, w = A z):
-    block values and column indices once ((9s+4) B), row pointers (4(V+1)),
-    z read once (3sV), w written once (3sV)."""
+    """Algorithmic HBM bytes of one CG matvec launch (k_cg_spmv, w = A z) over
+    every row: block values and column indices once ((9s+4) B), row pointers
+    (4(V+1)), z read once (3sV), w written once (3sV)."""
     return (9 * s + 4) * nB + 4 * (nV + 1) + 6 * s * nV
 
 
-def step_bytes(nT, nB, nV, cg_iters, ls_trials, s=4):
-    """Reference-equivalent algorithmic bytes of one step with 1 Newton
-    iteration: full assembly + k_CG CG iterations + (1 + L + e) residual
-    assemblies + final residual (e = 1, explosion control on)."""
+def step_bytes(nT, nB, nV, cg_iters, ls_trials, newton=1, s=4):
+    """Reference-equivalent algorithmic bytes of one step: per Newton
+    iteration a full assembly, its CG iterations and (1 + L + e) residual
+    assemblies (e = 1, explosion control on), plus the final residual."""
     bytes_cg = (9 * s + 4) * nB + 4 * (nV + 1) + 9 * s * nV + 30 * s * nV
     bytes_asm = nT * (16 + 15 * s) + 9 * s * nB + 6 * s * nV
     bytes_res = nT * (16 + 15 * s) + 6 * s * nV
-    return bytes_asm + cg_iters * bytes_cg + (1 + ls_trials + 1) * bytes_res + bytes_res
+    return (newton * (bytes_asm + 2 * bytes_res) + ls_trials * bytes_res + cg_iters * bytes_cg + bytes_res)
 
 
-def read_traffic():
-    if not os.path.exists(TRAFFIC_FILE):
+def traffic_file(workload):
+    return os.path.join(ROOT, "profiles", f"cg_spmv_{workload}_traffic.txt")
+
+
+def read_traffic(workload):
+    path = traffic_file(workload)
+    if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(TRAFFIC_FILE).read())
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(spec, params, seconds_budget: float = 20.0):
+def make_spec(workload: str, steps: int, seed: int):
+    from project_superdex_amd import mesh
+
+    if workload == "press40":
+        return mesh.plate_press_scene(n=40, steps=steps)
+    if workload == "batch512":
+        return mesh.batched_indent_scene(512, seed, 6, 16, h=0.01, radius=0.02, speed=0.05, steps=steps,
+                                         indenters=(1, 1), newton_iters=1)
+    if workload == "full4096":
+        return mesh.batched_indent_scene(4096, seed, 4, 12, h=0.01, radius=0.02, speed=0.05, steps=steps,
+                                         indenters=(1, 3), newton_iters=2)
+    raise ValueError(workload)
+
+
+def solver_params(spec, mode, autotune=False):
+    from project_superdex_amd import _abi
+
+    p = _abi.pinned_solver_params(spec.newton_iters, lin_rel_tol=1e-6, precision_mode=mode,
+                                  refresh_every=10 if mode == _abi.PREC_MODE_MIXED else 0)
+    p.schedule = -1 if autotune else 0
+    return p
+
+
+def run_steps(sc, rigid, spec, params, s0, s1, stats=None):
+    for s in range(s0, s1):
+        for r, c in zip(rigid, spec.colliders):
+            r.set_position(c.position(s, spec.dt))
+        st = sc.step(spec.dt, params)
+        if stats is not None:
+            stats.append(st)
+
+
+def plain_gpu(spec, mode, steps, warmup, sample):
+    """The plain GPU PCG path: every environment of `sample` stepped on its own
+    (one environment per launch sequence), zero initial guess, eager launches."""
+    from project_superdex_amd import scene as gs
+
+    tets, secs = 0, 0.0
+    for e in sample:
+        sub = spec.subset([e])
+        sc, rigid = gs.build_scene(sub, precision=mode)
+        sc.set_schedule(use_graphs=0)
+        p = solver_params(sub, mode)
+        run_steps(sc, rigid, sub, p, 0, warmup)
+        t0 = time.perf_counter()
+        run_steps(sc, rigid, sub, p, warmup, warmup + steps)
+        secs += time.perf_counter() - t0
+        tets += sub.num_tets * steps
+        sc.close()
+    return tets / secs, secs
+
+
+def position_error_vs_fp64(spec, total, sc32):
+    """Max over vertices of |x - x_fp64| / |x_fp64| after the same steps run in FP64."""
+    from project_superdex_amd import _abi
+    from project_superdex_amd import scene as gs
+
+    sc, rigid = gs.build_scene(spec, precision=_abi.PREC_MODE_FP64)
+    p = solver_params(spec, _abi.PREC_MODE_FP64)
+    run_steps(sc, rigid, spec, p, 0, total)
+    worst = 0.0
+    for a, b in zip(sc32.actors, sc.actors):
+        x32, x64 = a.positions_world(), b.positions_world()
+        worst = max(worst, float(np.max(np.linalg.norm(x32 - x64, axis=1) /
+                                        np.maximum(np.linalg.norm(x64, axis=1), 1e-9))))
+    sc.close()
+    return worst
+
+
+def cpu_baseline(spec, seconds_budget: float):
     """The oracle (C++ restatement, OpenMP) on the same workload, bounded to
-    about `seconds_budget` of stepping; returns a cpu_baseline object."""
+    about `seconds_budget` of stepping: the first 16 environments of a batch
+    (the whole mesh for press40), stepped from rest."""
     from oracle import oracle_api
+    from project_superdex_amd import _abi
 
     threads = min(16, os.cpu_count() or 1)
     oracle_api.set_threads(threads)
-    sc, ids = oracle_api.build_scene(spec, fp64=False)
+    n_env = len(spec.bodies)
+    envs = list(range(min(n_env, 16)))
+    sub = spec.subset(envs) if n_env > 1 else spec
+    params = _abi.pinned_solver_params(spec.newton_iters, lin_rel_tol=1e-6)
+    sc, ids = oracle_api.build_scene(sub, fp64=False)
     steps, t_used = 0, 0.0
     while t_used < seconds_budget and steps < spec.steps:
-        for cid, c in zip(ids, spec.colliders):
+        for cid, c in zip(ids, sub.colliders):
             sc.set_collider_position(cid, c.position(steps, spec.dt))
         t0 = time.perf_counter()
         sc.step(spec.dt, params)
         t_used += time.perf_counter() - t0
         steps += 1
-    rate = spec.num_tets * steps / t_used
-    return {"value": rate, "unit": "tet-steps/s", "cores": threads, "kind": "port",
-            "sample": f"first {steps} steps of the same workload (FP32 oracle, {threads} OpenMP threads), "
-                      f"{t_used:.1f} s of stepping"}
+    what = f"{len(envs)} environments" if n_env > 1 else "the whole mesh"
+    return {"value": sub.num_tets * steps / t_used, "unit": "tet-steps/s", "cores": threads, "kind": "port",
+            "sample": f"first {steps} steps of {what} ({sub.num_tets} tets), FP32 oracle with {threads} "
+                      f"OpenMP threads, {t_used:.1f} s of stepping"}
 
 
 def copy_probe(mib: int) -> int:
@@ -95,18 +185,25 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
+DEFAULTS = {"press40": (50, 3), "batch512": (20, 3), "full4096": (20, 2)}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=40, help="hexes per cube side")
+    ap.add_argument("--workload", choices=sorted(DEFAULTS), default="batch512")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU PCG comparison")
+    ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP64 position-error run")
+    ap.add_argument("--plain-envs", type=int, default=32, help="environments timed on the plain path")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--burst", type=int, default=0, help="CG iterations between host checks (0: default)")
-    ap.add_argument("--time-every", type=int, default=16,
-                    help="time every N-th CG matvec launch with HIP events (0: no kernel timing)")
+    ap.add_argument("--time-every", type=int, default=1,
+                    help="time the first CG burst of every N-th solve with HIP events (0: no kernel timing)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 elementwise copies of this many MiB (read + write), the "
                          "calibration kernel for the HBM byte counters, and exit")
@@ -114,88 +211,114 @@ def main(argv=None) -> int:
 
     if args.copy_probe_mib:
         return copy_probe(args.copy_probe_mib)
-
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU; run with --gpus 1",
               file=sys.stderr)
         return 2
 
-    from project_superdex_amd import _abi, mesh
+    from project_superdex_amd import _abi
     from project_superdex_amd import scene as gs
 
+    steps = args.steps if args.steps is not None else DEFAULTS[args.workload][0]
+    warmup = args.warmup if args.warmup is not None else DEFAULTS[args.workload][1]
+    warmup = max(warmup, 1)
     mode = {"fp32": _abi.PREC_MODE_FP32, "mixed": _abi.PREC_MODE_MIXED, "fp64": _abi.PREC_MODE_FP64}[args.precision]
-    total = args.warmup + args.steps
-    spec = mesh.plate_press_scene(n=args.n, steps=total)
-    params = _abi.pinned_solver_params(1, lin_rel_tol=1e-6, precision_mode=mode,
-                                       refresh_every=10 if mode == _abi.PREC_MODE_MIXED else 0)
+    total = warmup + steps
+    spec = make_spec(args.workload, total, args.seed)
+    batched = len(spec.bodies) > 1
+    autotune = not args.no_autotune
+    print(f"[bench] {args.workload}: {len(spec.bodies)} envs, {spec.num_tets} tets", file=sys.stderr, flush=True)
     sc, rigid = gs.build_scene(spec, device=0, precision=mode)
-    if args.burst:
-        sc.set_cg_burst(args.burst)
     info = sc.info()
 
-    def do_step(s):
-        for r, c in zip(rigid, spec.colliders):
-            r.set_position(c.position(s, spec.dt))
-        return sc.step(spec.dt, params)[0]
-
-    for s in range(args.warmup):
-        do_step(s)
+    # warm-up (the first step autotunes the launch schedule)
+    run_steps(sc, rigid, spec, solver_params(spec, mode, autotune), 0, 1)
+    params = solver_params(spec, mode)
+    run_steps(sc, rigid, spec, params, 1, warmup)
+    print(f"[bench] warm, schedule {sc.schedule()}", file=sys.stderr, flush=True)
     if args.time_every > 0:
         sc.kernel_timing(0, enable=args.time_every)
     stats = []
     t0 = time.perf_counter()
-    for s in range(args.warmup, total):
-        stats.append(do_step(s))
+    run_steps(sc, rigid, spec, params, warmup, total, stats)
     elapsed = time.perf_counter() - t0
     launches, ms = sc.kernel_timing(0) if args.time_every > 0 else (0, 0.0)
     sc.kernel_timing(0, enable=0)
+    print(f"[bench] timed {steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
 
     nT, nV, nB = info["n_tets"], info["n_verts"], info["nnzb"]
-    cg = float(np.mean([s["linear_iters"] for s in stats]))
-    ls = float(np.mean([s["line_search_iters"] for s in stats]))
+    per_env = [s for st in stats for s in st]
     ok = all(s["linear_convergence"] == _abi.CONV_CONVERGED and s["linear_rel_res"] <= 1e-6
-             and s["linear_true_rel_res"] <= 1e-6 for s in stats)
-    true_rel = max(s["linear_true_rel_res"] for s in stats)
-    value = nT * args.steps / elapsed
+             and s["linear_true_rel_res"] <= 1e-6 for s in per_env)
+    true_rel = max(s["linear_true_rel_res"] for s in per_env)
+    sizes = np.array([b.tets.shape[0] for b in spec.bodies], dtype=np.float64)
+    it = np.array([[s["linear_iters"] for s in st] for st in stats], dtype=np.float64)
+    ls = np.array([[s["line_search_iters"] for s in st] for st in stats], dtype=np.float64)
+    value = nT * steps / elapsed
     s_bytes = 8 if mode == _abi.PREC_MODE_FP64 else 4
     per_launch = matvec_bytes(nB, nV, s_bytes)
     avg_ms = ms / launches if launches else None
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms else None
-    traffic = read_traffic()
-    step_gbs = step_bytes(nT, nB, nV, cg, ls, s_bytes) / (elapsed / args.steps) / 1e9
-
+    # reference-equivalent bytes per step, environment by environment
+    step_b = 0.0
+    for e, b in enumerate(spec.bodies):
+        nv_e = b.X.shape[0]
+        nb_e = nB * nv_e / nV  # blocks per vertex are nearly uniform on these grids
+        step_b += step_bytes(b.tets.shape[0], nb_e, nv_e, it[:, e].mean(), ls[:, e].mean(), spec.newton_iters,
+                             s_bytes)
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "tet-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "mixed": "f32 (f64 dots + refresh)", "fp64": "f64"}[args.precision],
+        "dtype": {"fp32": "f32", "mixed": "f32 matvec, f64 dots/iterate/refresh", "fp64": "f64"}[args.precision],
         "data": "synthetic",
         "config": {
-            "workload": f"{args.n}^3-hex soft cube ({nT} tets, {nV} verts, {nB} BSR blocks) under a 2 mm plate "
-                        f"press, 1 Newton iter/step, block-Jacobi PCG rel 1e-6, precision {args.precision}",
-            "tets": nT, "verts": nV, "nnzb": nB, "dt": spec.dt,
+            "workload": args.workload,
+            "description": {"press40": "40^3-hex soft cube under a 2 mm plate press",
+                            "batch512": "512 soft cubes n~U{6..16}, sphere indenters, one batched solve",
+                            "full4096": "4096 soft cubes n~U{4..12}, 1-3 indenters, 2 Newton iters"}[args.workload],
+            "envs": len(spec.bodies), "tets": nT, "verts": nV, "nnzb": nB, "dt": spec.dt,
+            "newton_iters": spec.newton_iters, "lin_rel_tol": 1e-6, "precision": args.precision,
+            "warm_start": "zero", "schedule": sc.schedule(),
         },
         "roofline": {
-            "bound": "hbm", "kernel": "k_cg_spmv (CG matvec w = A z + z.w)",
+            "bound": "hbm", "kernel": "k_cg_spmv (CG matvec w = A z and z.w)",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": traffic, "bytes_per_launch": per_launch, "avg_launch_us": 1e3 * avg_ms if avg_ms else None,
-            "timed_launches": int(launches), "timed_every": args.time_every,
+            "traffic": read_traffic(args.workload), "bytes_per_launch": per_launch,
+            "avg_launch_us": 1e3 * avg_ms if avg_ms else None, "timed_launches": int(launches),
+            "timed": ("every launch of the first CG burst (all environments active) of every "
+                      f"{args.time_every}-th solve") if args.time_every > 0 else None,
         },
-        "step": {"cg_iters": cg, "ls_iters": ls, "all_converged": ok, "max_true_rel_res": true_rel,
-                 "reference_equivalent_GBps": step_gbs},
+        "step": {"cg_iters_mean_per_env": float(it.mean()),
+                 "cg_iters_tet_weighted": float((it * sizes).sum(axis=1).mean() / sizes.sum()),
+                 "cg_iters_max": int(it.max()), "ls_iters_mean": float(ls.mean()),
+                 "all_converged": ok, "max_true_rel_res": true_rel,
+                 "reference_equivalent_GBps": step_b / (elapsed / steps) / 1e9},
     }
+    if batched and not args.no_plain:
+        rng = np.random.default_rng(args.seed + 1)
+        sample = sorted(int(e) for e in rng.choice(len(spec.bodies), size=min(args.plain_envs, len(spec.bodies)),
+                                                   replace=False))
+        rate, secs = plain_gpu(spec, mode, steps, warmup, sample)
+        out["plain_gpu"] = {"value": rate, "unit": "tet-steps/s",
+                            "sample": f"{len(sample)} of the {len(spec.bodies)} environments, each stepped on its "
+                                      f"own ({steps} timed steps after {warmup}), zero initial guess, eager "
+                                      f"launches; {secs:.1f} s",
+                            "speedup": value / rate}
+        print(f"[bench] plain path done", file=sys.stderr, flush=True)
+    if not args.no_fp64_check and mode != _abi.PREC_MODE_FP64:
+        out["position_error_vs_fp64"] = position_error_vs_fp64(spec, total, sc)
+        print(f"[bench] fp64 check done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        cpu_spec = mesh.plate_press_scene(n=args.n, steps=total)
-        cpu_params = _abi.pinned_solver_params(1, lin_rel_tol=1e-6)
-        out["cpu_baseline"] = cpu_baseline(cpu_spec, cpu_params, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(make_spec(args.workload, total, args.seed), args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

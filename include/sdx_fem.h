@@ -262,9 +262,10 @@ sdx_status sdx_assemble(sdx_world* world, int32_t actor, double dt, const sdx_so
  * recorded at the kernel's own dispatch on the world's stream.  kernel: 0 =
  * CG matvec (w = A z + z.w), 1 = CG vector update, 2 = element Hessian, 3 =
  * BSR gather.  enable N > 0 clears the accumulators and times every N-th
- * launch of an assembly class, or every launch of every N-th CG burst (the
- * other bursts replay as HIP graphs); 0 stops timing, -1 leaves the setting
- * unchanged.  out (may be NULL): {timed launches, total milliseconds}. */
+ * launch of an assembly class, or every launch of the first CG burst (all
+ * environments active) of every N-th linear solve (that burst is launched
+ * eagerly, the others replay as HIP graphs); 0 stops timing, -1 leaves the
+ * setting unchanged.  out (may be NULL): {timed launches, total milliseconds}. */
 sdx_status sdx_kernel_timing(sdx_world* world, int32_t kernel, int32_t enable, double out[2]);
 
 /* CG iterations launched between host convergence checks (default 8). */
@@ -277,6 +278,9 @@ typedef struct sdx_schedule {
   int32_t use_graphs;     /* 1: each CG burst is captured once and replayed as a HIP graph (default 1) */
   int32_t spmv_lanes;     /* lanes per block row of the CG matvec: 4, 8 or 16 (default 16) */
   int32_t element_block;  /* threads per workgroup of the element kernel: 64, 128 or 256 (default 128) */
+  int32_t active_lists;   /* 1: each CG burst first compacts the tiles of the environments still
+                             active into work lists and runs persistent CG kernels over them
+                             (default 1) */
 } sdx_schedule;
 
 sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* schedule);
@@ -284,7 +288,7 @@ sdx_status sdx_get_schedule(sdx_world* world, sdx_schedule* out);
 
 /* Autotuner: times sdx_step of the world's current state under candidate
  * schedules (coordinate search over matvec lanes, element workgroup size,
- * CG burst length and graph replay; the best of `trials` steps each),
+ * work lists, CG burst length and graph replay; the best of `trials` steps each),
  * restores the state after every trial and keeps the fastest schedule.
  * out / out_ms (may be NULL): the chosen schedule and its step time. */
 sdx_status sdx_autotune(sdx_world* world, double dt, const sdx_solver_params* params, int32_t trials,

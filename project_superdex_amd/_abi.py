@@ -79,6 +79,7 @@ class Schedule(C.Structure):
         ("use_graphs", C.c_int32),
         ("spmv_lanes", C.c_int32),
         ("element_block", C.c_int32),
+        ("active_lists", C.c_int32),
     ]
 
     def as_dict(self):

@@ -357,6 +357,7 @@ sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* sc) {
   SDX_REQUIRE(sc->spmv_lanes == 4 || sc->spmv_lanes == 8 || sc->spmv_lanes == 16, "spmv_lanes must be 4, 8 or 16");
   SDX_REQUIRE(sc->element_block == 64 || sc->element_block == 128 || sc->element_block == 256,
               "element_block must be 64, 128 or 256");
+  SDX_REQUIRE(sc->active_lists == 0 || sc->active_lists == 1, "active_lists must be 0 or 1");
   return guarded([&]() -> sdx_status {
     world->impl->set_schedule(*sc);
     return SDX_OK;

@@ -42,6 +42,13 @@ __host__ __device__ inline bool cg_refresh(const CgConfig& cfg, int k) {
   return cfg.refresh_every > 0 && ((k + 1) % cfg.refresh_every) == 0;
 }
 
+// Work list of tiles (of environments still active) for the persistent CG
+// kernels; list == nullptr: one tile per workgroup over all tiles.
+struct WorkList {
+  const int32_t* list;
+  const int32_t* count;
+};
+
 // Event pair recorded around one kernel dispatch.
 struct KernelEvents {
   hipEvent_t start, stop;
@@ -136,17 +143,23 @@ template <class T, class A>
 void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const int32_t* warm,
                     const double* r64, const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env, A* acc);
 // CG iteration k: w = A z (+ FP64 true residual on refresh iterations), with
-// `lanes` (4, 8, 16) lanes per block row and kSpmvThreads / lanes rows per tile.
+// `lanes` (4, 8, 16) lanes per block row and kSpmvThreads / lanes rows per
+// tile; over all tiles, or persistent (`grid` >= 8 workgroups) over a list.
 template <class T, class A>
-void launch_cg_spmv(hipStream_t s, const Tiles& ta, const CgConfig& cfg, int k, int lanes, const int32_t* row_ptr,
-                    const int32_t* col, const T* vals, int32_t nB, const T* z, const A* x, const T* b, T* w,
-                    double* rref, EnvCG* env, A* acc, int32_t* num_active, const KernelEvents* ev = nullptr);
+void launch_cg_spmv(hipStream_t s, const Tiles& ta, const WorkList& wl, int grid, const CgConfig& cfg, int k,
+                    int lanes, const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const T* z,
+                    const A* x, const T* b, T* w, double* rref, EnvCG* env, A* acc, int32_t* num_active,
+                    const KernelEvents* ev = nullptr);
 // CG iteration k: p, s = A p, x, r, z updates.  The iterate x is kept in the
 // accumulator type A (FP64 in mixed precision).
 template <class T, class A>
-void launch_cg_update(hipStream_t s, const Tiles& tv, const CgConfig& cfg, int k, const T* dinv, int32_t nV,
-                      const T* w, const double* rref, A* x, T* r, T* z, T* p, T* sv, EnvCG* env, A* acc,
-                      int32_t* num_active, const KernelEvents* ev = nullptr);
+void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int grid, const CgConfig& cfg, int k,
+                      const T* dinv, int32_t nV, const T* w, const double* rref, A* x, T* r, T* z, T* p, T* sv,
+                      EnvCG* env, A* acc, int32_t* num_active, const KernelEvents* ev = nullptr);
+// Work lists of the tiles of active environments (slot `par` of counts,
+// which must start at zero; slot par ^ 1 is cleared for the next burst).
+void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, const EnvCG* env, int32_t* la,
+                          int32_t* lv, int32_t* counts, int par);
 // FP64 residual b - A x of the environments with mask[e]: into r64 when
 // given; |b - A x|^2 and |b|^2 into set 0 of acc when given (zeroed by the
 // caller).  Matvec tiles of kSpmvThreads / lanes rows.

@@ -608,14 +608,11 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // w_k = A z_k is stored and z_k.w_k goes into D_k.  On refresh iterations the
 // true residual rref = b - A x_k is also formed in FP64 (residual replacement).
 template <class T, class A, bool kRefresh, int L>
-__global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(Tiles ta, CgConfig cfg, int k,
-                                                          const int32_t* __restrict__ row_ptr,
-                                                          const int32_t* __restrict__ col, const T* __restrict__ vals,
-                                                          int nB, const T* __restrict__ z, const A* __restrict__ x,
-                                                          const T* __restrict__ b, T* __restrict__ w,
-                                                          double* __restrict__ rref, EnvCG* env, A* acc,
-                                                          int32_t* num_active) {
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const CgConfig& cfg, int k,
+                                             const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                                             const T* __restrict__ vals, int nB, const T* __restrict__ z,
+                                             const A* __restrict__ x, const T* __restrict__ b, T* __restrict__ w,
+                                             double* __restrict__ rref, EnvCG* env, A* acc, int32_t* num_active) {
   const int e = ta.env[tile];
   const int status = env[e].status;  // branched on after the row loads are issued
   const int nEnv = cfg.n_env;
@@ -707,13 +704,11 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(Tiles ta, CgConfig cfg
 // iterations), z = M^-1 r; R_{k+1} partials.  Row loads are issued before
 // the scalars are known.
 template <class T, class A, bool kRefresh>
-__global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, CgConfig cfg, int k, const T* __restrict__ dinv,
-                                                           int nV, const T* __restrict__ w,
-                                                           const double* __restrict__ rref, A* __restrict__ x,
-                                                           T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
-                                                           T* __restrict__ s, EnvCG* env, A* acc,
-                                                           int32_t* num_active) {
-  const int tile = blockIdx.x;
+__device__ __forceinline__ void cg_update_tile(int tile, const Tiles& tv, const CgConfig& cfg, int k,
+                                               const T* __restrict__ dinv, int nV, const T* __restrict__ w,
+                                               const double* __restrict__ rref, A* __restrict__ x,
+                                               T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
+                                               T* __restrict__ s, EnvCG* env, A* acc, int32_t* num_active) {
   const int e = tv.env[tile];
   const int status = env[e].status;  // branched on after the row loads are issued
   const int nEnv = cfg.n_env;
@@ -845,6 +840,94 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, CgConfig cf
     part[3] = A(r0) * A(z0) + A(r1) * A(z1) + A(r2) * A(z2);
   }
   acc_add<A, 4>(part, acc_ptr(acc, kSetR + ((k + 1) & 1), nEnv, e), tile - tv.first[e]);
+}
+
+// A CG kernel runs one tile per workgroup (list == NULL: tiles in XCD-
+// contiguous order), or is persistent over a work list of the tiles of the
+// environments still active: the list is split into 8 contiguous chunks, one
+// per XCD, and each workgroup strides over its XCD's chunk.
+template <class F>
+__device__ __forceinline__ void for_each_tile(const WorkList& wl, int n_tiles, F&& f) {
+  if (!wl.list) {
+    f(xcd_tile(blockIdx.x, n_tiles));
+    return;
+  }
+  constexpr int kXcd = 8;
+  const int n = *wl.count;
+  const int xc = blockIdx.x % kXcd, i = blockIdx.x / kXcd;
+  const int nb = (int(gridDim.x) - xc + kXcd - 1) / kXcd;  // workgroups on this XCD (grid >= 8)
+  const int q1 = int((int64_t)n * (xc + 1) / kXcd);
+  for (int q = int((int64_t)n * xc / kXcd) + i; q < q1; q += nb) {
+    f(wl.list[q]);
+    __syncthreads();  // the tile's block reduction reuses LDS
+  }
+}
+
+template <class T, class A, bool kRefresh, int L>
+__global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(Tiles ta, WorkList wl, CgConfig cfg, int k,
+                                                          const int32_t* __restrict__ row_ptr,
+                                                          const int32_t* __restrict__ col, const T* __restrict__ vals,
+                                                          int nB, const T* __restrict__ z, const A* __restrict__ x,
+                                                          const T* __restrict__ b, T* __restrict__ w,
+                                                          double* __restrict__ rref, EnvCG* env, A* acc,
+                                                          int32_t* num_active) {
+  for_each_tile(wl, wl.list ? 0 : int(gridDim.x), [&](int tile) {
+    cg_spmv_tile<T, A, kRefresh, L>(tile, ta, cfg, k, row_ptr, col, vals, nB, z, x, b, w, rref, env, acc,
+                                    num_active);
+  });
+}
+
+template <class T, class A, bool kRefresh>
+__global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, WorkList wl, CgConfig cfg, int k,
+                                                           const T* __restrict__ dinv, int nV,
+                                                           const T* __restrict__ w, const double* __restrict__ rref,
+                                                           A* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                           T* __restrict__ p, T* __restrict__ s, EnvCG* env, A* acc,
+                                                           int32_t* num_active) {
+  if (!wl.list) {  // vector tiles need no XCD placement
+    cg_update_tile<T, A, kRefresh>(blockIdx.x, tv, cfg, k, dinv, nV, w, rref, x, r, z, p, s, env, acc, num_active);
+    return;
+  }
+  for_each_tile(wl, 0, [&](int tile) {
+    cg_update_tile<T, A, kRefresh>(tile, tv, cfg, k, dinv, nV, w, rref, x, r, z, p, s, env, acc, num_active);
+  });
+}
+
+// Work lists of the matvec and vector tiles whose environment is still
+// active, for the bursts' persistent CG kernels.  Blocks [0, nba) scan the
+// matvec tiles, the rest the vector tiles; each reserves a contiguous run of
+// its list with one atomic (list order follows block arrival).  counts:
+// {matvec list, matvec list (other parity), vector list, vector list (other
+// parity)}; this launch fills slot `par` and clears slot par ^ 1 for the next
+// burst.
+__global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, const EnvCG* __restrict__ env,
+                                                      int32_t* __restrict__ la, int32_t* __restrict__ lv,
+                                                      int32_t* counts, int par) {
+  const int nba = (ta.n + 255) / 256;
+  const bool isA = int(blockIdx.x) < nba;
+  const Tiles& t = isA ? ta : tv;
+  const int i = (isA ? int(blockIdx.x) : int(blockIdx.x) - nba) * 256 + int(threadIdx.x);
+  int32_t* list = isA ? la : lv;
+  int32_t* cnt = counts + (isA ? 0 : 2) + par;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    counts[par ^ 1] = 0;
+    counts[2 + (par ^ 1)] = 0;
+  }
+  const bool keep = i < t.n && env[t.env[i]].status == kCgActive;
+  const unsigned long long m = __ballot(keep);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ int ws[4];
+  __shared__ int base;
+  if (lane == 0) ws[wv] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int tot = ws[0] + ws[1] + ws[2] + ws[3];
+    base = tot ? atomicAdd(cnt, tot) : 0;
+  }
+  __syncthreads();
+  int off = base;
+  for (int q = 0; q < wv; ++q) off += ws[q];
+  if (keep) list[off + __popcll(m & ((1ull << lane) - 1ull))] = i;
 }
 
 // FP64 residual b - A x of the environments with mask[e]: stored in r64
@@ -1075,14 +1158,16 @@ void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T
 }
 
 template <class T, class A>
-void launch_cg_spmv(hipStream_t s, const Tiles& ta, const CgConfig& cfg, int k, int lanes, const int32_t* row_ptr,
-                    const int32_t* col, const T* vals, int32_t nB, const T* z, const A* x, const T* b, T* w,
-                    double* rref, EnvCG* env, A* acc, int32_t* num_active, const KernelEvents* ev) {
+void launch_cg_spmv(hipStream_t s, const Tiles& ta, const WorkList& wl, int grid, const CgConfig& cfg, int k,
+                    int lanes, const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const T* z,
+                    const A* x, const T* b, T* w, double* rref, EnvCG* env, A* acc, int32_t* num_active,
+                    const KernelEvents* ev) {
   if (ta.n == 0) return;
+  const int g = wl.list ? grid : ta.n;
   const bool refresh = cg_refresh(cfg, k);
 #define SDX_SPMV(R, L)                                                                                           \
-  SDX_LAUNCH(ev, (k_cg_spmv<T, A, R, L>), dim3(ta.n), dim3(kSpmvThreads), s, ta, cfg, k, row_ptr, col, vals, nB, z, \
-             x, b, w, rref, env, acc, num_active)
+  SDX_LAUNCH(ev, (k_cg_spmv<T, A, R, L>), dim3(g), dim3(kSpmvThreads), s, ta, wl, cfg, k, row_ptr, col, vals, nB, \
+             z, x, b, w, rref, env, acc, num_active)
   switch (lanes) {
     case 4:
       if (refresh) SDX_SPMV(true, 4); else SDX_SPMV(false, 4);
@@ -1098,16 +1183,24 @@ void launch_cg_spmv(hipStream_t s, const Tiles& ta, const CgConfig& cfg, int k, 
 }
 
 template <class T, class A>
-void launch_cg_update(hipStream_t s, const Tiles& tv, const CgConfig& cfg, int k, const T* dinv, int32_t nV,
-                      const T* w, const double* rref, A* x, T* r, T* z, T* p, T* sv, EnvCG* env, A* acc,
-                      int32_t* num_active, const KernelEvents* ev) {
+void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int grid, const CgConfig& cfg, int k,
+                      const T* dinv, int32_t nV, const T* w, const double* rref, A* x, T* r, T* z, T* p, T* sv,
+                      EnvCG* env, A* acc, int32_t* num_active, const KernelEvents* ev) {
   if (tv.n == 0) return;
+  const int g = wl.list ? grid : tv.n;
   if (cg_refresh(cfg, k))
-    SDX_LAUNCH(ev, (k_cg_update<T, A, true>), dim3(tv.n), dim3(kVecThreads), s, tv, cfg, k, dinv, nV, w, rref, x, r,
-               z, p, sv, env, acc, num_active);
-  else
-    SDX_LAUNCH(ev, (k_cg_update<T, A, false>), dim3(tv.n), dim3(kVecThreads), s, tv, cfg, k, dinv, nV, w, rref, x,
+    SDX_LAUNCH(ev, (k_cg_update<T, A, true>), dim3(g), dim3(kVecThreads), s, tv, wl, cfg, k, dinv, nV, w, rref, x,
                r, z, p, sv, env, acc, num_active);
+  else
+    SDX_LAUNCH(ev, (k_cg_update<T, A, false>), dim3(g), dim3(kVecThreads), s, tv, wl, cfg, k, dinv, nV, w, rref, x,
+               r, z, p, sv, env, acc, num_active);
+}
+
+void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, const EnvCG* env, int32_t* la,
+                          int32_t* lv, int32_t* counts, int par) {
+  const int nb = (ta.n + 255) / 256 + (tv.n + 255) / 256;
+  if (nb == 0) return;
+  hipLaunchKernelGGL(k_compact_tiles, dim3(nb), dim3(256), 0, s, ta, tv, env, la, lv, counts, par);
 }
 
 template <class T, class X>
@@ -1212,12 +1305,12 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
   template void launch_residual64<T, A>(hipStream_t, const Tiles&, int, int, const int32_t*, const int32_t*,    \
                                         const int32_t*, const T*, int32_t, const A*, const T*, double*,         \
                                         double*);                                                               \
-  template void launch_cg_spmv<T, A>(hipStream_t, const Tiles&, const CgConfig&, int, int, const int32_t*,       \
-                                     const int32_t*, const T*, int32_t, const T*, const A*, const T*, T*,        \
-                                     double*, EnvCG*, A*, int32_t*, const KernelEvents*);                        \
-  template void launch_cg_update<T, A>(hipStream_t, const Tiles&, const CgConfig&, int, const T*, int32_t,       \
-                                       const T*, const double*, A*, T*, T*, T*, T*, EnvCG*, A*, int32_t*,        \
-                                       const KernelEvents*);
+  template void launch_cg_spmv<T, A>(hipStream_t, const Tiles&, const WorkList&, int, const CgConfig&, int, int, \
+                                     const int32_t*, const int32_t*, const T*, int32_t, const T*, const A*,      \
+                                     const T*, T*, double*, EnvCG*, A*, int32_t*, const KernelEvents*);          \
+  template void launch_cg_update<T, A>(hipStream_t, const Tiles&, const WorkList&, int, const CgConfig&, int,    \
+                                       const T*, int32_t, const T*, const double*, A*, T*, T*, T*, T*, EnvCG*,   \
+                                       A*, int32_t*, const KernelEvents*);
 
 SDX_INST_T(float)
 SDX_INST_T(double)
@@ -1225,6 +1318,7 @@ SDX_INST_TA(float, float)
 SDX_INST_TA(float, double)
 SDX_INST_TA(double, double)
 template void launch_convert<float, float>(hipStream_t, int64_t, const float*, float*);
+template void launch_fill<int32_t>(hipStream_t, int64_t, int32_t, int32_t*);
 template void launch_convert<float, double>(hipStream_t, int64_t, const float*, double*);
 template void launch_convert<double, float>(hipStream_t, int64_t, const double*, float*);
 template void launch_convert<double, double>(hipStream_t, int64_t, const double*, double*);

@@ -135,9 +135,9 @@ sdx_status solve(int32_t device, int32_t nbr, int32_t nnzb, const int32_t* ptr, 
   int32_t na = 1;
   while (true) {
     for (int k = 0; k < 8; ++k, ++j) {
-      launch_cg_spmv<T, A>(s, ta, cfg, j, kSpmvLanes, dptr.p, dcol.p, dvals.p, nnzb, dz.p, dx.p, db.p, dw.p, drref.p, denv.p,
+      launch_cg_spmv<T, A>(s, ta, WorkList{nullptr, nullptr}, 0, cfg, j, kSpmvLanes, dptr.p, dcol.p, dvals.p, nnzb, dz.p, dx.p, db.p, dw.p, drref.p, denv.p,
                            acc, dna.p);
-      launch_cg_update<T, A>(s, tv, cfg, j, ddinv.p, nbr, dw.p, drref.p, dx.p, dr.p, dz.p, dp.p, dsv.p, denv.p, acc,
+      launch_cg_update<T, A>(s, tv, WorkList{nullptr, nullptr}, 0, cfg, j, ddinv.p, nbr, dw.p, drref.p, dx.p, dr.p, dz.p, dp.p, dsv.p, denv.p, acc,
                              dna.p);
     }
     hip_check(hipMemcpyAsync(&na, dna.p, sizeof(int32_t), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");

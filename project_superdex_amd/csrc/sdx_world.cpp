@@ -345,6 +345,7 @@ void World<T>::finalize() {
   d_geom_.upload(geom);
   upload_tiles(tiles_v_, dt_v_);
   upload_tiles(tiles_a_, dt_a_);
+  alloc_lists();
   const size_t n3 = size_t(3) * size_t(nV_);
   u_.alloc(n3);
   v_.alloc(n3);
@@ -370,6 +371,7 @@ void World<T>::finalize() {
   acc_.alloc(size_t(kCgSets) * size_t(nEnv) * size_t(kSlots) * size_t(kSlotStride));
   solve_mask_.alloc(static_cast<size_t>(nEnv));
   warm_mask_.alloc(static_cast<size_t>(nEnv));
+  list_count_.alloc(4);
   num_active_.alloc(1);
   env_mask_.alloc(static_cast<size_t>(nEnv));
   ls_mask_.alloc(static_cast<size_t>(nEnv));
@@ -410,6 +412,12 @@ void World<T>::finalize() {
   fields_.geom = d_geom_.p;
   fields_.contact = d_contact_.p;
   upload_colliders();
+}
+
+template <class T>
+void World<T>::alloc_lists() {
+  list_a_.alloc(std::max<size_t>(tiles_a_.env.size(), 1));
+  list_v_.alloc(std::max<size_t>(tiles_v_.env.size(), 1));
 }
 
 template <class T>
@@ -491,27 +499,40 @@ std::vector<double> World<T>::read_env_sq() {
   return h;
 }
 
-// Launch CG iterations [k0, k0 + n) on the stream; with `timed`, every
-// matvec and update launch records its own dispatch events.
+// Launch CG burst m, iterations [k0, k0 + n), on the stream; with `timed`,
+// every matvec and update launch records its own dispatch events.  With
+// work lists, the burst starts by compacting the tiles of the environments
+// still active and its kernels are persistent over those lists.
 template <class T>
-void World<T>::issue_cg(const CgConfig& cfg, int k0, int n, bool timed) {
+void World<T>::issue_cg(const CgConfig& cfg, int m, int k0, int n, bool timed) {
   const bool dd = precision_ != SDX_PREC_MODE_FP32;
   T* accf = reinterpret_cast<T*>(acc_.p);
+  WorkList wa{nullptr, nullptr}, wv{nullptr, nullptr};
+  int ga = 0, gv = 0;
+  if (sched_.active_lists) {
+    const int par = m & 1;
+    launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, env_cg_.p, list_a_.p, list_v_.p, list_count_.p, par);
+    wa = WorkList{list_a_.p, list_count_.p + par};
+    wv = WorkList{list_v_.p, list_count_.p + 2 + par};
+    ga = std::max(8, std::min(dt_a_.view.n, kPersistentGridA));
+    gv = std::max(8, std::min(dt_v_.view.n, kPersistentGridV));
+  }
   for (int k = k0; k < k0 + n; ++k) {
     const KernelEvents* e0 = timed ? timer_events(0) : nullptr;
     if (dd)
-      launch_cg_spmv<T, double>(stream_, dt_a_.view, cfg, k, sched_.spmv_lanes, d_row_ptr_.p, d_col_.p, vals_.p, nB_, z_.p, xcg<double>(),
-                                res_.p, w_.p, rref_.p, env_cg_.p, acc_.p, num_active_.p, e0);
+      launch_cg_spmv<T, double>(stream_, dt_a_.view, wa, ga, cfg, k, sched_.spmv_lanes, d_row_ptr_.p, d_col_.p,
+                                vals_.p, nB_, z_.p, xcg<double>(), res_.p, w_.p, rref_.p, env_cg_.p, acc_.p,
+                                num_active_.p, e0);
     else
-      launch_cg_spmv<T, T>(stream_, dt_a_.view, cfg, k, sched_.spmv_lanes, d_row_ptr_.p, d_col_.p, vals_.p, nB_, z_.p, xcg<T>(), res_.p,
-                           w_.p, rref_.p, env_cg_.p, accf, num_active_.p, e0);
+      launch_cg_spmv<T, T>(stream_, dt_a_.view, wa, ga, cfg, k, sched_.spmv_lanes, d_row_ptr_.p, d_col_.p, vals_.p,
+                           nB_, z_.p, xcg<T>(), res_.p, w_.p, rref_.p, env_cg_.p, accf, num_active_.p, e0);
     const KernelEvents* e1 = timed ? timer_events(1) : nullptr;
     if (dd)
-      launch_cg_update<T, double>(stream_, dt_v_.view, cfg, k, dinv_.p, nV_, w_.p, rref_.p, xcg<double>(), r_.p, z_.p, p_.p,
-                                  sv_.p, env_cg_.p, acc_.p, num_active_.p, e1);
+      launch_cg_update<T, double>(stream_, dt_v_.view, wv, gv, cfg, k, dinv_.p, nV_, w_.p, rref_.p, xcg<double>(),
+                                  r_.p, z_.p, p_.p, sv_.p, env_cg_.p, acc_.p, num_active_.p, e1);
     else
-      launch_cg_update<T, T>(stream_, dt_v_.view, cfg, k, dinv_.p, nV_, w_.p, rref_.p, xcg<T>(), r_.p, z_.p, p_.p,
-                             sv_.p, env_cg_.p, accf, num_active_.p, e1);
+      launch_cg_update<T, T>(stream_, dt_v_.view, wv, gv, cfg, k, dinv_.p, nV_, w_.p, rref_.p, xcg<T>(), r_.p, z_.p,
+                             p_.p, sv_.p, env_cg_.p, accf, num_active_.p, e1);
   }
 }
 
@@ -528,7 +549,7 @@ hipGraphExec_t World<T>::cg_graph(const CgConfig& cfg, int m, int B) {
   if (!graphs_[size_t(m)]) {
     hipGraph_t g = nullptr;
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    issue_cg(cfg, m * B, B, false);
+    issue_cg(cfg, m, m * B, B, false);
     hip_check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
     hipGraphExec_t ex = nullptr;
     const hipError_t err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -564,6 +585,7 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
   std::vector<int32_t> na(1, active);
   num_active_.upload_async(na, stream_);
   const int32_t* wm = n_warm ? warm_mask_.p : nullptr;
+  launch_fill<int32_t>(stream_, 4, 0, list_count_.p);
   if (precision_ != SDX_PREC_MODE_FP32) {
     if (n_warm) {
       warm_mask_.upload_async(warm, stream_);
@@ -592,11 +614,11 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
   for (int m = 0;; ++m) {
     const bool issue = int64_t(m) * B <= int64_t(cfg.max_iter);
     if (issue) {
-      const bool timed = burst_timed();
+      const bool timed = burst_timed(m);
       if (sched_.use_graphs && !timed)
         hip_check(hipGraphLaunch(cg_graph(cfg, m, B), stream_), "hipGraphLaunch");
       else
-        issue_cg(cfg, m * B, B, timed);
+        issue_cg(cfg, m, m * B, B, timed);
       hip_check(hipMemcpyAsync(hna + (m & 1), num_active_.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream_),
                 "hipMemcpyAsync");
       hip_check(hipEventRecord(burst_done_[m & 1], stream_), "hipEventRecord");
@@ -971,6 +993,7 @@ void World<T>::set_schedule(const sdx_schedule& sc) {
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     BuildTiles(env_v0_, nEnv_, kSpmvThreads / sched_.spmv_lanes, tiles_a_);
     upload_tiles(tiles_a_, dt_a_);
+    alloc_lists();
   }
 }
 
@@ -1022,6 +1045,11 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
       c.spmv_lanes = L;
       consider(c);
     }
+  {
+    sdx_schedule c = cur;
+    c.active_lists = 1 - cur.active_lists;
+    consider(c);
+  }
   for (int eb : {64, 128, 256})
     if (eb != cur.element_block) {
       sdx_schedule c = cur;

@@ -155,7 +155,8 @@ class WorldBase {
 
   // kernel timing: class 0 CG matvec, 1 CG update, 2 element, 3 BSR gather.
   // on > 0: time every on-th launch of an assembly class, and every launch of
-  // every on-th CG burst (bursts otherwise replay as graphs); 0: off.
+  // the first CG burst of every on-th solve (other bursts replay as graphs);
+  // 0: off.
   void timing_enable(int k, int on) {
     auto& t = timers_[size_t(k)];
     t.enabled = on > 0;
@@ -169,7 +170,7 @@ class WorldBase {
     flush_timers();
     return {double(timers_[size_t(k)].launches), timers_[size_t(k)].ms};
   }
-  sdx_schedule sched_{8, 1, 16, 128};
+  sdx_schedule sched_{8, 1, 16, 128, 1};
   bool tuned_ = false;  // the schedule was autotuned for the current topology
 
  protected:
@@ -200,11 +201,12 @@ class WorldBase {
     if (!t.enabled || (t.counter++ % t.every) != 0) return nullptr;
     return timer_events(k);
   }
-  // Whether the next CG burst is launched eagerly with its launches timed.
-  bool burst_timed() {
+  // Whether CG burst m of the current solve is launched eagerly with its
+  // launches timed: the first burst (every environment still active) of
+  // every `every`-th solve.
+  bool burst_timed(int m) {
+    if (m != 0 || !(timers_[0].enabled || timers_[1].enabled)) return false;
     Timer& t = timers_[0];
-    const bool on = timers_[0].enabled || timers_[1].enabled;
-    if (!on) return false;
     return (t.counter++ % t.every) == 0;
   }
   // Call after a stream synchronisation: folds recorded event pairs into totals.
@@ -260,6 +262,9 @@ class World final : public WorldBase {
   };
   void push_state();
   void upload_tiles(const HostTiles& h, DevTiles& d);
+  void alloc_lists();
+  // workgroups of the persistent CG kernels (full occupancy of 256 CUs)
+  static constexpr int kPersistentGridA = 4096, kPersistentGridV = 2048;
   void upload_colliders();
   void set_env_mask(const std::vector<int32_t>& mask);
   void assemble(double dt, const sdx_solver_params& sp, bool wantH);
@@ -269,7 +274,7 @@ class World final : public WorldBase {
   std::vector<EnvCG> cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve,
                             const std::vector<int32_t>& warm);
   std::vector<double> true_residual(const std::vector<int32_t>& mask);
-  void issue_cg(const CgConfig& cfg, int k0, int n, bool timed);
+  void issue_cg(const CgConfig& cfg, int m, int k0, int n, bool timed);
   hipGraphExec_t cg_graph(const CgConfig& cfg, int m, int B);
   void drop_graphs();
   template <class A>
@@ -336,6 +341,9 @@ class World final : public WorldBase {
   SDX_BUF(double, acc_)
   SDX_BUF(int32_t, solve_mask_)
   SDX_BUF(int32_t, warm_mask_)
+  SDX_BUF(int32_t, list_a_)      // work list of active matvec tiles
+  SDX_BUF(int32_t, list_v_)      // work list of active vector tiles
+  SDX_BUF(int32_t, list_count_)  // list lengths, two parities each
   SDX_BUF(int32_t, num_active_)
   SDX_BUF(int32_t, env_mask_)
   SDX_BUF(int32_t, ls_mask_)

@@ -137,6 +137,21 @@ class SceneSpec:
     def num_verts(self) -> int:
         return int(sum(b.X.shape[0] for b in self.bodies))
 
+    def subset(self, envs) -> "SceneSpec":
+        """The scene restricted to the soft bodies `envs` (with their own
+        colliders and the global ones), actors renumbered in that order."""
+        envs = list(envs)
+        remap = {e: i for i, e in enumerate(envs)}
+        cols = []
+        for c in self.colliders:
+            if c.actor == -1:
+                cols.append(c)
+            elif c.actor in remap:
+                cols.append(Collider(kind=c.kind, point=c.point, normal=c.normal, radius=c.radius,
+                                     velocity=c.velocity, ramp_steps=c.ramp_steps, actor=remap[c.actor]))
+        return SceneSpec(bodies=[self.bodies[e] for e in envs], colliders=cols, dt=self.dt, steps=self.steps,
+                         newton_iters=self.newton_iters, name=f"{self.name}[{len(envs)}]")
+
 
 def soft_cube(n: int, edge: float, E=1e5, nu=0.4, density=1000.0, penalty=1e6, origin=(0.0, 0.0, 0.0)):
     X, tets = hex_grid_tets(n, edge / n, origin)

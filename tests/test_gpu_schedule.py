@@ -5,7 +5,7 @@
   at or below 1e-6, iteration counts stay within 15% of the oracle's
   (standard PCG) and positions match the oracle as in the parity tests;
 - every launch schedule (matvec lanes, element workgroup, burst length,
-  graph replay) steps to the same state: displacements within 1e-3 of the
+  graph replay, persistent kernels over work lists of active tiles) steps to the same state: displacements within 1e-3 of the
   largest displacement (FP32 solves at rel. residual 1e-6 differ at the
   1e-4 level when only the summation order changes; a wrong result is off
   by O(1)), iteration counts within 2;
@@ -58,6 +58,8 @@ def state(sc):
     dict(spmv_lanes=8, cg_burst=3),
     dict(element_block=64, cg_burst=32),
     dict(element_block=256, use_graphs=0, cg_burst=1),
+    dict(active_lists=0),
+    dict(active_lists=0, use_graphs=0, spmv_lanes=8),
 ])
 def test_schedules_step_to_the_same_state(sched):
     # one Newton iteration per step: a second one starts from a residual at
