@@ -161,7 +161,8 @@ typedef struct sdx_step_stats {
   double res_norm;
   double linear_rel_res;          /* last linear solve ||r||/||b|| (as the stop test measured it) */
   double linear_true_rel_res;     /* last linear solve ||b - A x||/||b|| recomputed in FP64 */
-  double warm_start_rel_res;      /* ||b - A x0||/||b|| of the last learned warm start (0: none) */
+  double warm_start_rel_res;      /* ||b - a A x0||/||b|| of the last learned guess x0 with its
+                                     least-residual factor a (0: none tried) */
 } sdx_step_stats;
 
 /* SoftActorParams subset (cpp_api/mochi_structs.h:174-197). */
@@ -293,6 +294,71 @@ sdx_status sdx_get_schedule(sdx_world* world, sdx_schedule* out);
  * out / out_ms (may be NULL): the chosen schedule and its step time. */
 sdx_status sdx_autotune(sdx_world* world, double dt, const sdx_solver_params* params, int32_t trials,
                         sdx_schedule* out, double* out_ms);
+
+/* ---------------- learned CG warm start ----------------
+ * No counterpart in the reference: its PCG starts from zero
+ * (pcg.h:109-130).  A per-vertex network 15 -> 128 -> 128 -> 128 -> 3 (GELU)
+ * maps features of the assembled Newton system to a guess of the Newton
+ * step; sdx_step uses it when sdx_solver_params.warm_start is
+ * SDX_WARM_START_LEARNED and a model is set.  Per environment the guess is
+ * scaled by the factor with the least residual, a = b.Ax0 / |Ax0|^2, and kept
+ * only if that residual is below 0.2 |b|; else the environment starts from
+ * zero, as the plain method does.  Inference runs in BF16 on the matrix cores, training
+ * in FP32 on the device.
+ *
+ * Feature row (16 floats, the last zero) of vertex i of environment e, with
+ * b the residual, D_i the 3x3 diagonal block, d_i = trace(D_i) / 3,
+ * y = b / d, dprev the previous Newton step, s_e = sqrt(max(|dprev|^2,
+ * |y|^2) / n_e) over the environment:
+ *   y_i / s_e (3), (D00, D11, D22, D01, D02, D12) / d_i (6), dprev_i / s_e (3),
+ *   mean of y_j / s_e over the 1-ring neighbours j (3).
+ * Target / output: the Newton step of vertex i divided by s_e.
+ * Parameters (FP32, SDX_MLP_PARAMS values): W1[128][15] b1[128] W2[128][128]
+ * b2[128] W3[128][128] b3[128] W4[3][128] b4[3], layer y = W x + b. */
+#define SDX_MLP_FEATURES 15
+#define SDX_MLP_ROW 16
+#define SDX_MLP_OUT 3
+#define SDX_MLP_PARAMS 35459
+
+typedef struct sdx_mlp sdx_mlp;
+
+typedef struct sdx_train_params {
+  int32_t epochs;
+  int32_t batch;
+  float lr;      /* Adam (Kingma & Ba 2015) */
+  float beta1;
+  float beta2;
+  float eps;
+  uint64_t seed; /* row order of every epoch */
+} sdx_train_params;
+
+void sdx_default_train_params(sdx_train_params* out);
+/* Random initialisation (He normal) from `seed`. */
+sdx_status sdx_mlp_create(int32_t device, uint64_t seed, sdx_mlp** out);
+sdx_status sdx_mlp_destroy(sdx_mlp* mlp);
+sdx_status sdx_mlp_set_params(sdx_mlp* mlp, const float* params, int32_t n);
+sdx_status sdx_mlp_get_params(sdx_mlp* mlp, float* params, int32_t n);
+/* y[n_rows][3] = net(x[n_rows][16]); bf16 = 1: the matrix-core inference
+ * kernel, 0: the FP32 training forward. */
+sdx_status sdx_mlp_forward(sdx_mlp* mlp, const float* x, int64_t n_rows, int32_t bf16, float* y);
+/* loss = mean over rows of |net(x) - t|^2 and its gradient (grad may be NULL). */
+sdx_status sdx_mlp_loss_grad(sdx_mlp* mlp, const float* x, const float* t, int32_t n_rows, float* grad,
+                             double* loss);
+/* Adam training on host rows; epoch_loss (may be NULL): mean loss of each epoch. */
+sdx_status sdx_mlp_train(sdx_mlp* mlp, const float* x, const float* t, int64_t n_rows,
+                         const sdx_train_params* params, double* epoch_loss);
+/* Adam training on the samples a world collected (same device, no copies). */
+sdx_status sdx_mlp_train_collected(sdx_mlp* mlp, sdx_world* world, const sdx_train_params* params,
+                                   double* epoch_loss);
+/* The world takes a snapshot of the model's parameters (NULL: no model). */
+sdx_status sdx_world_set_warm_start_model(sdx_world* world, sdx_mlp* mlp);
+/* Collect (feature row, Newton step / s_e) pairs of `per_env` vertices per
+ * environment from every linear solve of the following sdx_step calls, up
+ * to `capacity` rows (capacity 0: stop and free). */
+sdx_status sdx_world_collect(sdx_world* world, int64_t capacity, int32_t per_env);
+/* Rows collected so far in *n_rows; copies min(n, cap_rows) rows into x
+ * ([n][16]) and t ([n][3]) when they are not NULL. */
+sdx_status sdx_world_collected(sdx_world* world, int64_t* n_rows, float* x, float* t, int64_t cap_rows);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,21 @@ class Schedule(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class TrainParams(C.Structure):
+    _fields_ = [
+        ("epochs", C.c_int32),
+        ("batch", C.c_int32),
+        ("lr", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("eps", C.c_float),
+        ("seed", C.c_uint64),
+    ]
+
+
+MLP_FEATURES, MLP_ROW, MLP_OUT, MLP_PARAMS = 15, 16, 3, 35459
+
+
 class StepStats(C.Structure):
     _fields_ = [
         ("newton_iters", C.c_int32),

@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsdxfem.so")
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
 _vp = C.c_void_p
@@ -50,6 +51,18 @@ SIGNATURES = {
     "sdx_get_schedule": (C.c_int32, [_vp, C.POINTER(_abi.Schedule)]),
     "sdx_autotune": (C.c_int32, [_vp, C.c_double, C.POINTER(_abi.SolverParams), C.c_int32,
                                  C.POINTER(_abi.Schedule), _dp]),
+    "sdx_default_train_params": (None, [C.POINTER(_abi.TrainParams)]),
+    "sdx_mlp_create": (C.c_int32, [C.c_int32, C.c_uint64, C.POINTER(_vp)]),
+    "sdx_mlp_destroy": (C.c_int32, [_vp]),
+    "sdx_mlp_set_params": (C.c_int32, [_vp, _fp, C.c_int32]),
+    "sdx_mlp_get_params": (C.c_int32, [_vp, _fp, C.c_int32]),
+    "sdx_mlp_forward": (C.c_int32, [_vp, _fp, C.c_int64, C.c_int32, _fp]),
+    "sdx_mlp_loss_grad": (C.c_int32, [_vp, _fp, _fp, C.c_int32, _fp, _dp]),
+    "sdx_mlp_train": (C.c_int32, [_vp, _fp, _fp, C.c_int64, C.POINTER(_abi.TrainParams), _dp]),
+    "sdx_mlp_train_collected": (C.c_int32, [_vp, _vp, C.POINTER(_abi.TrainParams), _dp]),
+    "sdx_world_set_warm_start_model": (C.c_int32, [_vp, _vp]),
+    "sdx_world_collect": (C.c_int32, [_vp, C.c_int64, C.c_int32]),
+    "sdx_world_collected": (C.c_int32, [_vp, C.POINTER(C.c_int64), _fp, _fp, C.c_int64]),
 }
 
 _lib = None
@@ -73,7 +86,7 @@ def lib():
             fn.restype = res
             fn.argtypes = args
         for idx, cls in enumerate([_abi.KrylovParams, _abi.LinearStatus, _abi.SolverParams, _abi.StepStats,
-                                   _abi.SoftParams, _abi.Schedule]):
+                                   _abi.SoftParams, _abi.Schedule, _abi.TrainParams]):
             if L.sdx_abi_struct_size(idx) != C.sizeof(cls):
                 raise RuntimeError(f"ABI mismatch for {cls.__name__}: C {L.sdx_abi_struct_size(idx)} vs "
                                    f"Python {C.sizeof(cls)}")

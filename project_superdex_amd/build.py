@@ -18,8 +18,8 @@ OUT = os.path.join(HERE, "libsdxfem.so")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("SDX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["sdx_kernels.hip", "sdx_world.cpp", "sdx_api.cpp", "sdx_pcg.cpp"]
-HEADERS = ["sdx_kernels.h", "sdx_math.h", "sdx_world.h"]
+SOURCES = ["sdx_kernels.hip", "sdx_mlp.hip", "sdx_world.cpp", "sdx_api.cpp", "sdx_pcg.cpp", "sdx_mlp_host.cpp"]
+HEADERS = ["sdx_kernels.h", "sdx_math.h", "sdx_world.h", "sdx_device.h", "sdx_mlp.h", "sdx_mlp_host.h"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -42,7 +42,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    with ThreadPoolExecutor(max_workers=6) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]

@@ -12,12 +12,17 @@
 
 #include "../../include/sdx_fem.h"
 #include "sdx_kernels.h"
+#include "sdx_mlp_host.h"
 #include "sdx_world.h"
 
 using sdx::set_error;
 
 struct sdx_world {
   std::unique_ptr<sdx::WorldBase> impl;
+};
+
+struct sdx_mlp {
+  std::unique_ptr<sdx::Mlp> impl;
 };
 
 namespace {
@@ -105,6 +110,7 @@ int32_t sdx_abi_struct_size(int32_t which) {
     case 3: return int32_t(sizeof(sdx_step_stats));
     case 4: return int32_t(sizeof(sdx_soft_params));
     case 5: return int32_t(sizeof(sdx_schedule));
+    case 6: return int32_t(sizeof(sdx_train_params));
     default: return -1;
   }
 }
@@ -381,6 +387,180 @@ sdx_status sdx_autotune(sdx_world* world, double dt, const sdx_solver_params* pa
     const double ms = world->impl->autotune(dt, *params, trials);
     if (out) *out = world->impl->schedule();
     if (out_ms) *out_ms = ms;
+    return SDX_OK;
+  });
+}
+
+void sdx_default_train_params(sdx_train_params* p) {
+  p->epochs = 10;
+  p->batch = 4096;
+  p->lr = 1e-3f;
+  p->beta1 = 0.9f;
+  p->beta2 = 0.999f;
+  p->eps = 1e-8f;
+  p->seed = 1;
+}
+
+static sdx_status check_train(const sdx_train_params* p) {
+  SDX_REQUIRE(p, "NULL train params");
+  SDX_REQUIRE(p->epochs > 0 && p->epochs <= 100000, "epochs must be in 1..100000");
+  SDX_REQUIRE(p->batch > 0 && p->batch <= (1 << 20), "batch must be in 1..2^20");
+  SDX_REQUIRE(p->lr > 0 && p->beta1 >= 0 && p->beta1 < 1 && p->beta2 >= 0 && p->beta2 < 1 && p->eps > 0,
+              "invalid Adam parameters");
+  return SDX_OK;
+}
+
+static sdx::TrainConfig train_config(const sdx_train_params* p) {
+  sdx::TrainConfig c;
+  c.epochs = p->epochs;
+  c.batch = p->batch;
+  c.lr = p->lr;
+  c.beta1 = p->beta1;
+  c.beta2 = p->beta2;
+  c.eps = p->eps;
+  c.seed = p->seed;
+  return c;
+}
+
+sdx_status sdx_mlp_create(int32_t device, uint64_t seed, sdx_mlp** out) {
+  SDX_REQUIRE(out, "out is NULL");
+  return guarded([&]() -> sdx_status {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+      return set_error(SDX_ERR_DEVICE, "no such HIP device");
+    auto m = std::make_unique<sdx_mlp>();
+    m->impl = std::make_unique<sdx::Mlp>(device, seed);
+    *out = m.release();
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_mlp_destroy(sdx_mlp* mlp) {
+  delete mlp;
+  return SDX_OK;
+}
+
+sdx_status sdx_mlp_set_params(sdx_mlp* mlp, const float* params, int32_t n) {
+  SDX_REQUIRE(mlp && params, "NULL argument");
+  SDX_REQUIRE(n == SDX_MLP_PARAMS, "n must be SDX_MLP_PARAMS");
+  for (int32_t i = 0; i < n; ++i) SDX_REQUIRE(std::isfinite(params[i]), "parameters must be finite");
+  return guarded([&]() -> sdx_status {
+    mlp->impl->set_params(params);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_mlp_get_params(sdx_mlp* mlp, float* params, int32_t n) {
+  SDX_REQUIRE(mlp && params, "NULL argument");
+  SDX_REQUIRE(n == SDX_MLP_PARAMS, "n must be SDX_MLP_PARAMS");
+  return guarded([&]() -> sdx_status {
+    mlp->impl->get_params(params);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_mlp_forward(sdx_mlp* mlp, const float* x, int64_t n_rows, int32_t bf16, float* y) {
+  SDX_REQUIRE(mlp && x && y, "NULL argument");
+  SDX_REQUIRE(n_rows > 0 && n_rows < (int64_t(1) << 31), "n_rows must be in 1..2^31-1");
+  return guarded([&]() -> sdx_status {
+    mlp->impl->forward(x, n_rows, bf16 != 0, y);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_mlp_loss_grad(sdx_mlp* mlp, const float* x, const float* t, int32_t n_rows, float* grad,
+                             double* loss) {
+  SDX_REQUIRE(mlp && x && t, "NULL argument");
+  SDX_REQUIRE(n_rows > 0 && n_rows <= (1 << 22), "n_rows must be in 1..2^22");
+  return guarded([&]() -> sdx_status {
+    const double l = mlp->impl->loss_grad(x, t, n_rows, grad);
+    if (loss) *loss = l;
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_mlp_train(sdx_mlp* mlp, const float* x, const float* t, int64_t n_rows,
+                         const sdx_train_params* params, double* epoch_loss) {
+  SDX_REQUIRE(mlp && x && t, "NULL argument");
+  SDX_REQUIRE(n_rows > 0 && n_rows < (int64_t(1) << 31), "n_rows must be in 1..2^31-1");
+  const sdx_status v = check_train(params);
+  if (v != SDX_OK) return v;
+  return guarded([&]() -> sdx_status {
+    sdx::hip_check(hipSetDevice(mlp->impl->device()), "hipSetDevice");
+    float *dx = nullptr, *dt = nullptr;
+    sdx::hip_check(hipMalloc(reinterpret_cast<void**>(&dx), sizeof(float) * size_t(n_rows) * SDX_MLP_ROW),
+                   "hipMalloc");
+    const hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&dt), sizeof(float) * size_t(n_rows) * SDX_MLP_OUT);
+    if (e2 != hipSuccess) {
+      (void)hipFree(dx);
+      sdx::hip_check(e2, "hipMalloc");
+    }
+    std::vector<double> losses;
+    try {
+      sdx::hip_check(hipMemcpy(dx, x, sizeof(float) * size_t(n_rows) * SDX_MLP_ROW, hipMemcpyHostToDevice), "memcpy");
+      sdx::hip_check(hipMemcpy(dt, t, sizeof(float) * size_t(n_rows) * SDX_MLP_OUT, hipMemcpyHostToDevice), "memcpy");
+      losses = mlp->impl->train(dx, dt, n_rows, train_config(params));
+    } catch (...) {
+      (void)hipFree(dx);
+      (void)hipFree(dt);
+      throw;
+    }
+    (void)hipFree(dx);
+    (void)hipFree(dt);
+    if (epoch_loss)
+      for (size_t i = 0; i < losses.size(); ++i) epoch_loss[i] = losses[i];
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_mlp_train_collected(sdx_mlp* mlp, sdx_world* world, const sdx_train_params* params,
+                                   double* epoch_loss) {
+  SDX_REQUIRE(mlp && world, "NULL argument");
+  SDX_REQUIRE(mlp->impl->device() == world->impl->device(), "model and world must be on the same device");
+  const sdx_status v = check_train(params);
+  if (v != SDX_OK) return v;
+  return guarded([&]() -> sdx_status {
+    const float *x = nullptr, *t = nullptr;
+    int64_t n = 0;
+    world->impl->collected_device(&x, &t, &n);
+    SDX_REQUIRE(n > 0, "the world has collected no samples");
+    const std::vector<double> losses = mlp->impl->train(x, t, n, train_config(params));
+    if (epoch_loss)
+      for (size_t i = 0; i < losses.size(); ++i) epoch_loss[i] = losses[i];
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_world_set_warm_start_model(sdx_world* world, sdx_mlp* mlp) {
+  SDX_REQUIRE(world, "NULL argument");
+  SDX_REQUIRE(!mlp || mlp->impl->device() == world->impl->device(), "model and world must be on the same device");
+  return guarded([&]() -> sdx_status {
+    if (!mlp) {
+      world->impl->set_ws_params(nullptr);
+      return SDX_OK;
+    }
+    std::vector<float> p(SDX_MLP_PARAMS);
+    mlp->impl->get_params(p.data());
+    world->impl->set_ws_params(p.data());
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_world_collect(sdx_world* world, int64_t capacity, int32_t per_env) {
+  SDX_REQUIRE(world, "NULL argument");
+  SDX_REQUIRE(capacity >= 0 && capacity < (int64_t(1) << 31), "capacity must be in 0..2^31-1");
+  SDX_REQUIRE(capacity == 0 || (per_env > 0 && per_env <= 65536), "per_env must be in 1..65536");
+  return guarded([&]() -> sdx_status {
+    world->impl->collect(capacity, per_env);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_world_collected(sdx_world* world, int64_t* n_rows, float* x, float* t, int64_t cap_rows) {
+  SDX_REQUIRE(world && n_rows, "NULL argument");
+  SDX_REQUIRE(cap_rows >= 0, "cap_rows must be non-negative");
+  return guarded([&]() -> sdx_status {
+    *n_rows = world->impl->collected(x, t, cap_rows);
     return SDX_OK;
   });
 }

@@ -138,10 +138,12 @@ void launch_acc_finalize(hipStream_t s, int nEnv, int nv, const A* acc, double* 
 template <class A>
 void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, A* acc);
 // r0, z0, scaling and R_0.  warm (may be NULL): environments that keep x as
-// the initial guess and take r0 from r64 = b - A x (launch_residual64).
+// the initial guess and take r0 from r64 = b - A x (launch_residual64);
+// alpha (may be NULL): per-environment factor applied to the kept guess.
 template <class T, class A>
 void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const int32_t* warm,
-                    const double* r64, const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env, A* acc);
+                    const double* r64, const double* alpha, const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env,
+                    A* acc);
 // CG iteration k: w = A z (+ FP64 true residual on refresh iterations), with
 // `lanes` (4, 8, 16) lanes per block row and kSpmvThreads / lanes rows per
 // tile; over all tiles, or persistent (`grid` >= 8 workgroups) over a list.
@@ -161,8 +163,8 @@ void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int gr
 void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, const EnvCG* env, int32_t* la,
                           int32_t* lv, int32_t* counts, int par);
 // FP64 residual b - A x of the environments with mask[e]: into r64 when
-// given; |b - A x|^2 and |b|^2 into set 0 of acc when given (zeroed by the
-// caller).  Matvec tiles of kSpmvThreads / lanes rows.
+// given; |b - A x|^2, |b|^2, b.Ax, |Ax|^2 into set 0 of acc when given
+// (zeroed by the caller).  Matvec tiles of kSpmvThreads / lanes rows.
 template <class T, class X>
 void launch_residual64(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
                        const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* x,
@@ -188,10 +190,14 @@ void launch_apply_pins(hipStream_t s, int64_t n, const uint8_t* pinned, const T*
 template <class T>
 void launch_ls_update(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* ls_mask,
                       const double* scale, const T* sol0, const T* dx, T* x);
-// v = (u - u0) / dt; environments with reset[env] get u = v = 0.
+// v = (u - u0) / dt; environments with reset[env] get u = v = 0 (and
+// prev = 0 when given).
 template <class T>
 void launch_post_step(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* reset, double dt,
-                      const T* u0, T* u, T* v);
+                      const T* u0, T* u, T* v, T* prev = nullptr);
+// dst = src on the rows of environments with mask[env].
+template <class T>
+void launch_env_copy(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* mask, const T* src, T* dst);
 // Per-environment |a|^2, |b|^2 added into set 0 of acc (zeroed by the caller).
 template <class T>
 void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, const T* b, double* acc);

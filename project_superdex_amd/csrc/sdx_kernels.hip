@@ -21,93 +21,11 @@
 // solved by the same launches.
 #include <hip/hip_ext.h>
 
+#include "sdx_device.h"
 #include "sdx_kernels.h"
 #include "sdx_math.h"
 
 namespace sdx {
-
-// ------------------------------------------------------------------ reductions
-//
-// Cross-block sums use spread atomic accumulators instead of a last-block
-// reduction: on a multi-XCD CDNA device an agent-scope fence per block (L2
-// write-back / invalidate) costs far more than the kernel's own traffic,
-// whereas FP64 atomic adds spread over kSlots separate 128-byte lines cost
-// almost nothing.  Consumers (the next kernel, or the host) sum the slots in
-// a fixed order; kernel boundaries order producers and consumers.
-
-template <class A>
-__device__ __forceinline__ A wave_sum(A v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Sum NV values over the block; result valid in thread 0.
-template <class A, int NV>
-__device__ __forceinline__ void block_sum(A (&v)[NV]) {
-  __shared__ A sh[NV][16];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) sh[k][wid] = v[k];
-  __syncthreads();
-  if (wid == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      A t = lane < nw ? sh[k][lane] : A(0);
-      v[k] = wave_sum(t);
-    }
-  }
-  __syncthreads();
-}
-
-// Accumulator set `set` of environment e: kSlots lines of kSlotStride values.
-template <class A>
-__device__ __forceinline__ A* acc_ptr(A* acc, int set, int nEnv, int e) {
-  return acc + ((size_t)(set * nEnv + e) * kSlots) * kSlotStride;
-}
-
-// Block-sum v (one barrier: wave sums, then thread 0 adds the per-wave
-// values) and add it into slot (tile_in_env % kSlots) of `dst`.  Uses its own
-// LDS, so a kernel may call it more than once.
-template <class A, int NV>
-__device__ __forceinline__ void acc_add(A (&v)[NV], A* dst, int tile_in_env) {
-  __shared__ A sh[NV][16];
-  const int nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-  if ((threadIdx.x & 63) == 0)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) sh[k][threadIdx.x >> 6] = v[k];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    A* d = dst + (tile_in_env & (kSlots - 1)) * kSlotStride;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      A t = sh[k][0];
-      for (int w = 1; w < nw; ++w) t += sh[k][w];
-      atomicAdd(d + k, t);
-    }
-  }
-}
-
-template <class A, int NV>
-__device__ __forceinline__ void acc_read(const A* src, double (&out)[NV]) {
-#pragma unroll
-  for (int k = 0; k < NV; ++k) out[k] = 0.0;
-  for (int sl = 0; sl < kSlots; ++sl)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] += double(src[sl * kSlotStride + k]);
-}
-
-// Zero one accumulator set of one environment (kSlots * kSlotStride values).
-template <class A>
-__device__ __forceinline__ void acc_zero(A* dst) {
-  for (int k = threadIdx.x; k < kSlots * kSlotStride; k += blockDim.x) dst[k] = A(0);
-}
 
 // ------------------------------------------------------------------ element
 
@@ -539,12 +457,14 @@ __global__ void k_cg_reset(int nEnv, const int32_t* __restrict__ solve, EnvCG* e
 }
 
 // r0 = b - A x0 or b, z0 = M^-1 r0; scaling from b (SetScaling); R_0.
-// Environments with warm[e] (warm may be NULL: none) keep x and take r0 from
-// r64 = b - A x0 formed in FP64 (k_residual64); the others start from x = 0.
+// Environments with warm[e] (warm may be NULL: none) keep x (scaled by
+// alpha[e] when alpha is given) and take r0 from r64 = b - A x0 formed in
+// FP64 (k_residual64); the others start from x = 0.
 template <class T, class A>
 __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg, const T* __restrict__ b,
                                                          const int32_t* __restrict__ warm,
-                                                         const double* __restrict__ r64, const T* __restrict__ dinv,
+                                                         const double* __restrict__ r64,
+                                                         const double* __restrict__ alpha, const T* __restrict__ dinv,
                                                          int nV, A* __restrict__ x, T* __restrict__ r,
                                                          T* __restrict__ z, EnvCG* env, A* acc) {
   const int tile = blockIdx.x;
@@ -558,9 +478,19 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg,
     const T b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
     T r0 = b0, r1 = b1, r2 = b2;
     if (w) {
-      r0 = T(r64[3 * i]);
-      r1 = T(r64[3 * i + 1]);
-      r2 = T(r64[3 * i + 2]);
+      if (alpha) {  // x0 <- a x0, so r0 = b - a A x0 = a r64 + (1 - a) b
+        const double a = alpha[e];
+        r0 = T(a * r64[3 * i] + (1.0 - a) * double(b0));
+        r1 = T(a * r64[3 * i + 1] + (1.0 - a) * double(b1));
+        r2 = T(a * r64[3 * i + 2] + (1.0 - a) * double(b2));
+        x[3 * i] = A(a) * x[3 * i];
+        x[3 * i + 1] = A(a) * x[3 * i + 1];
+        x[3 * i + 2] = A(a) * x[3 * i + 2];
+      } else {
+        r0 = T(r64[3 * i]);
+        r1 = T(r64[3 * i + 1]);
+        r2 = T(r64[3 * i + 2]);
+      }
     } else {
       x[3 * i] = A(0);
       x[3 * i + 1] = A(0);
@@ -931,8 +861,8 @@ __global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, const
 }
 
 // FP64 residual b - A x of the environments with mask[e]: stored in r64
-// when given, and |b - A x|^2, |b|^2 added into set 0 of acc when given
-// (zeroed by the caller).
+// when given, and |b - A x|^2, |b|^2, b.Ax, |Ax|^2 added into set 0 of acc
+// when given (zeroed by the caller).
 template <class T, class X, int L>
 __global__ void __launch_bounds__(kSpmvThreads) k_residual64(Tiles ta, int nEnv, const int32_t* __restrict__ mask,
                                                              const int32_t* __restrict__ row_ptr,
@@ -964,7 +894,7 @@ __global__ void __launch_bounds__(kSpmvThreads) k_residual64(Tiles ta, int nEnv,
     q1 += __shfl_xor(q1, off, L);
     q2 += __shfl_xor(q2, off, L);
   }
-  double part[2] = {0.0, 0.0};
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
   if (valid && lane == 0) {
     const double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
     const double d0 = b0 - q0, d1 = b1 - q1, d2 = b2 - q2;
@@ -975,8 +905,10 @@ __global__ void __launch_bounds__(kSpmvThreads) k_residual64(Tiles ta, int nEnv,
     }
     part[0] = d0 * d0 + d1 * d1 + d2 * d2;
     part[1] = b0 * b0 + b1 * b1 + b2 * b2;
+    part[2] = b0 * q0 + b1 * q1 + b2 * q2;  // b.Ax and |Ax|^2: the best multiple of x
+    part[3] = q0 * q0 + q1 * q1 + q2 * q2;
   }
-  if (acc) acc_add<double, 2>(part, acc_ptr(acc, 0, nEnv, e), tile - ta.first[e]);
+  if (acc) acc_add<double, 4>(part, acc_ptr(acc, 0, nEnv, e), tile - ta.first[e]);
 }
 
 template <class T, int L>
@@ -1040,7 +972,7 @@ __global__ void k_ls_update(int nV, const int32_t* __restrict__ row_env, const i
 
 template <class T>
 __global__ void k_post_step(int nV, const int32_t* __restrict__ row_env, const int32_t* __restrict__ reset, T idt,
-                            const T* __restrict__ u0, T* __restrict__ u, T* __restrict__ v) {
+                            const T* __restrict__ u0, T* __restrict__ u, T* __restrict__ v, T* __restrict__ prev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nV) return;
   const bool rs = reset[row_env[i]] != 0;
@@ -1048,8 +980,20 @@ __global__ void k_post_step(int nV, const int32_t* __restrict__ row_env, const i
   for (int d = 0; d < 3; ++d) {
     const T uu = u[3 * i + d];
     v[3 * i + d] = rs ? T(0) : (uu - u0[3 * i + d]) * idt;
-    if (rs) u[3 * i + d] = T(0);
+    if (rs) {
+      u[3 * i + d] = T(0);
+      if (prev) prev[3 * i + d] = T(0);
+    }
   }
+}
+
+template <class T>
+__global__ void k_env_copy(int nV, const int32_t* __restrict__ row_env, const int32_t* __restrict__ mask,
+                           const T* __restrict__ src, T* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV || !mask[row_env[i]]) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) dst[3 * i + d] = src[3 * i + d];
 }
 
 template <class T>
@@ -1151,10 +1095,11 @@ void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, 
 
 template <class T, class A>
 void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const int32_t* warm,
-                    const double* r64, const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env, A* acc) {
+                    const double* r64, const double* alpha, const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env,
+                    A* acc) {
   if (tv.n == 0) return;
-  hipLaunchKernelGGL((k_cg_init<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, b, warm, r64, dinv, nV, x, r,
-                     z, env, acc);
+  hipLaunchKernelGGL((k_cg_init<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, b, warm, r64, alpha, dinv, nV,
+                     x, r, z, env, acc);
 }
 
 template <class T, class A>
@@ -1265,10 +1210,16 @@ void launch_ls_update(hipStream_t s, int32_t nV, const int32_t* row_env, const i
 
 template <class T>
 void launch_post_step(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* reset, double dt,
-                      const T* u0, T* u, T* v) {
+                      const T* u0, T* u, T* v, T* prev) {
   if (nV == 0) return;
   hipLaunchKernelGGL(k_post_step<T>, dim3(blocks_for(nV, 256)), dim3(256), 0, s, nV, row_env, reset,
-                     T(1.0 / dt), u0, u, v);
+                     T(1.0 / dt), u0, u, v, prev);
+}
+
+template <class T>
+void launch_env_copy(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* mask, const T* src, T* dst) {
+  if (nV == 0) return;
+  hipLaunchKernelGGL(k_env_copy<T>, dim3(blocks_for(nV, 256)), dim3(256), 0, s, nV, row_env, mask, src, dst);
 }
 
 template <class T>
@@ -1292,7 +1243,8 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
   template void launch_ls_update<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, const double*,        \
                                     const T*, const T*, T*);                                                    \
   template void launch_post_step<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, double, const T*, T*, \
-                                    T*);                                                                        \
+                                    T*, T*);                                                                    \
+  template void launch_env_copy<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, const T*, T*);        \
   template void launch_env_norms<T>(hipStream_t, const Tiles&, int, const T*, const T*, double*);              \
   template void launch_fill<T>(hipStream_t, int64_t, T, T*);                                                    \
   template void launch_acc_finalize<T>(hipStream_t, int, int, const T*, double*);                               \
@@ -1301,7 +1253,7 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
 #define SDX_INST_TA(T, A)                                                                                         \
   template void launch_gather_res<T, A>(hipStream_t, const MeshView&, const Tiles&, const T*, const T*, T*, A*); \
   template void launch_cg_init<T, A>(hipStream_t, const Tiles&, const CgConfig&, const T*, const int32_t*,       \
-                                     const double*, const T*, int32_t, A*, T*, T*, EnvCG*, A*);                  \
+                                     const double*, const double*, const T*, int32_t, A*, T*, T*, EnvCG*, A*);   \
   template void launch_residual64<T, A>(hipStream_t, const Tiles&, int, int, const int32_t*, const int32_t*,    \
                                         const int32_t*, const T*, int32_t, const A*, const T*, double*,         \
                                         double*);                                                               \

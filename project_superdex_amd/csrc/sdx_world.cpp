@@ -372,6 +372,14 @@ void World<T>::finalize() {
   solve_mask_.alloc(static_cast<size_t>(nEnv));
   warm_mask_.alloc(static_cast<size_t>(nEnv));
   list_count_.alloc(4);
+  prev_dx_.alloc(n3);
+  hip_check(hipMemsetAsync(prev_dx_.p, 0, prev_dx_.bytes(), stream_), "hipMemsetAsync");
+  ws_y_.alloc(n3);
+  ws_F_.alloc(size_t(kMlpInPad) * size_t(nV_));
+  ws_scale_.alloc(static_cast<size_t>(nEnv));
+  ws_ratio_.alloc(static_cast<size_t>(nEnv));
+  ws_alpha_.alloc(static_cast<size_t>(nEnv));
+  d_env_v0_.upload(env_v0_);
   num_active_.alloc(1);
   env_mask_.alloc(static_cast<size_t>(nEnv));
   ls_mask_.alloc(static_cast<size_t>(nEnv));
@@ -571,7 +579,7 @@ void World<T>::drop_graphs() {
 // dx_ as the initial guess.  Returns the per-environment CG state.
 template <class T>
 std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve,
-                                    const std::vector<int32_t>& warm) {
+                                    const std::vector<int32_t>& warm, bool decide) {
   std::vector<EnvCG> h(static_cast<size_t>(nEnv_));
   int active = 0, n_warm = 0;
   for (int e = 0; e < nEnv_; ++e) {
@@ -586,25 +594,30 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
   num_active_.upload_async(na, stream_);
   const int32_t* wm = n_warm ? warm_mask_.p : nullptr;
   launch_fill<int32_t>(stream_, 4, 0, list_count_.p);
+  // decide: keep an environment's guess only if |b - A x0| < |b| (on device)
+  double* dacc = decide ? acc_.p : nullptr;
+  if (n_warm && decide) launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
   if (precision_ != SDX_PREC_MODE_FP32) {
     if (n_warm) {
       warm_mask_.upload_async(warm, stream_);
       launch_residual64<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
-                                   d_col_.p, vals_.p, nB_, xcg<double>(), res_.p, rref_.p, nullptr);
+                                   d_col_.p, vals_.p, nB_, xcg<double>(), res_.p, rref_.p, dacc);
+      if (decide) launch_ws_decide(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p);
     }
     launch_cg_reset<double>(stream_, nEnv_, solve_mask_.p, env_cg_.p, acc_.p);
-    launch_cg_init<T, double>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, dinv_.p, nV_, xcg<double>(), r_.p,
-                              z_.p, env_cg_.p, acc_.p);
+    launch_cg_init<T, double>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, decide ? ws_alpha_.p : nullptr,
+                              dinv_.p, nV_, xcg<double>(), r_.p, z_.p, env_cg_.p, acc_.p);
   } else {
     if (n_warm) {
       warm_mask_.upload_async(warm, stream_);
       launch_residual64<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p, d_col_.p,
-                              vals_.p, nB_, xcg<T>(), res_.p, rref_.p, nullptr);
+                              vals_.p, nB_, xcg<T>(), res_.p, rref_.p, dacc);
+      if (decide) launch_ws_decide(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p);
     }
     T* accf = reinterpret_cast<T*>(acc_.p);
     launch_cg_reset<T>(stream_, nEnv_, solve_mask_.p, env_cg_.p, accf);
-    launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, dinv_.p, nV_, xcg<T>(), r_.p, z_.p,
-                         env_cg_.p, accf);
+    launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, decide ? ws_alpha_.p : nullptr, dinv_.p, nV_,
+                         xcg<T>(), r_.p, z_.p, env_cg_.p, accf);
   }
   // Bursts of B iterations are issued one ahead of the host's check of the
   // active count, so the device never waits for the host between bursts; a
@@ -662,7 +675,8 @@ std::vector<double> World<T>::true_residual(const std::vector<int32_t>& mask) {
 // true_rel receives the FP64 ||b - A x|| / ||b|| of every solved environment.
 template <class T>
 std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve,
-                                      std::vector<double>* true_rel, const std::vector<int32_t>* warm_in) {
+                                      std::vector<double>* true_rel, const std::vector<int32_t>* warm_in,
+                                      WarmStats* learned) {
   CgConfig cfg{};
   cfg.n_env = nEnv_;
   cfg.norm_type = kp.norm_type;
@@ -683,7 +697,13 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
     else
       launch_convert<T, T>(stream_, n3, dx_.p, xcg<T>());
   }
-  std::vector<EnvCG> h = cg_run(cfg, todo, warm);
+  std::vector<EnvCG> h = cg_run(cfg, todo, warm, learned != nullptr);
+  if (learned) {
+    learned->accepted.assign(size_t(nEnv_), 0);
+    learned->ratio.assign(size_t(nEnv_), 0.0);
+    warm_mask_.download(learned->accepted);
+    ws_ratio_.download(learned->ratio);
+  }
   std::vector<double> tr = true_residual(solve);
   const bool confirm = precision_ == SDX_PREC_MODE_MIXED && kp.norm_type == SDX_NORM_RESIDUAL_L2;
   for (int round = 0; confirm && round < 2; ++round) {
@@ -698,7 +718,7 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
     }
     if (n == 0) break;
     CgConfig c2 = cfg;
-    std::vector<EnvCG> h2 = cg_run(c2, todo, warm);
+    std::vector<EnvCG> h2 = cg_run(c2, todo, warm, false);
     std::vector<double> tr2 = true_residual(todo);
     for (int e = 0; e < nEnv_; ++e) {
       if (!todo[size_t(e)]) continue;
@@ -717,6 +737,77 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
   if (cfg.refresh_every > 0)
     for (auto& e : h) e.refreshes = e.iter / cfg.refresh_every;
   return h;
+}
+
+// Features of the learned warm start for every vertex, from the assembled
+// residual and matrix and the previous Newton step.
+template <class T>
+void World<T>::ws_features() {
+  launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
+  launch_ws_scaled_residual<T>(stream_, mesh_, dt_v_.view, res_.p, vals_.p, prev_dx_.p, ws_y_.p, acc_.p);
+  launch_ws_env_scale(stream_, nEnv_, acc_.p, ws_scale_.p);
+  launch_ws_features<T>(stream_, mesh_, dt_v_.view, vals_.p, ws_y_.p, prev_dx_.p, ws_scale_.p, ws_F_.p);
+}
+
+template <class T>
+void World<T>::set_ws_params(const float* params) {
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  if (!params) {
+    ws_on_ = false;
+    return;
+  }
+  if (!ws_params_.p) {
+    ws_params_.alloc(kMlpParams);
+    ws_wpack_.alloc(kBf16W1 + 2 * kBf16W2 + kBf16W4);
+    ws_bpack_.alloc(3 * kMlpHid + 16);
+  }
+  std::vector<float> h(params, params + kMlpParams);
+  ws_params_.upload(h);
+  uint16_t* w = ws_wpack_.p;
+  float* b = ws_bpack_.p;
+  launch_mlp_pack(stream_, ws_params_.p, w, w + kBf16W1, w + kBf16W1 + kBf16W2, w + kBf16W1 + 2 * kBf16W2, b,
+                  b + kMlpHid, b + 2 * kMlpHid, b + 3 * kMlpHid);
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  ws_w_ = MlpBf16{w, w + kBf16W1, w + kBf16W1 + kBf16W2, w + kBf16W1 + 2 * kBf16W2, b, b + kMlpHid, b + 2 * kMlpHid,
+                  b + 3 * kMlpHid};
+  ws_on_ = true;
+}
+
+template <class T>
+void World<T>::collect(int64_t capacity, int per_env) {
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  coll_n_ = 0;
+  coll_salt_ = 0;
+  if (capacity <= 0) {
+    coll_x_.free();
+    coll_t_.free();
+    coll_cap_ = 0;
+    coll_per_env_ = 0;
+    return;
+  }
+  coll_x_.alloc(size_t(capacity) * kMlpInPad);
+  coll_t_.alloc(size_t(capacity) * kMlpOut);
+  coll_cap_ = capacity;
+  coll_per_env_ = per_env;
+}
+
+template <class T>
+int64_t World<T>::collected(float* x, float* t, int64_t cap_rows) {
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  const int64_t n = std::min(coll_n_, cap_rows);
+  if (x && n > 0)
+    hip_check(hipMemcpy(x, coll_x_.p, sizeof(float) * size_t(n) * kMlpInPad, hipMemcpyDeviceToHost), "hipMemcpy");
+  if (t && n > 0)
+    hip_check(hipMemcpy(t, coll_t_.p, sizeof(float) * size_t(n) * kMlpOut, hipMemcpyDeviceToHost), "hipMemcpy");
+  return coll_n_;
+}
+
+template <class T>
+void World<T>::collected_device(const float** x, const float** t, int64_t* n) {
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  *x = coll_x_.p;
+  *t = coll_t_.p;
+  *n = coll_n_;
 }
 
 template <class T>
@@ -739,6 +830,8 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
     int conv = SDX_CONV_NONE;
     int lin_iters = 0, ls_iters = 0, lin_conv = SDX_CONV_NONE, refreshes = 0;
     double lin_rel = 0, lin_true_rel = 0;
+    int ws_tried = 0, ws_accepted = 0;
+    double ws_ratio = 0;
     double dx_norm = std::numeric_limits<double>::infinity();
     double ls_res0 = 0;
     bool have_dx = false;
@@ -799,14 +892,32 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
     if (!h_valid) assemble(dt, sp, true);
     for (int e = 0; e < nE; ++e)
       if (S[size_t(e)].active) S[size_t(e)].iters++;
-    // linear solve (zero initial guess), solution in dx_
+    // linear solve, solution in dx_: from zero, or from the learned warm
+    // start where its guess lowers the residual
+    const bool learned = sp.warm_start == SDX_WARM_START_LEARNED && ws_on_;
+    const bool collecting = coll_per_env_ > 0 && coll_n_ + int64_t(nE) * coll_per_env_ <= coll_cap_;
+    if (learned || collecting) ws_features();
+    if (learned)
+      launch_mlp_infer<T>(stream_, ws_w_, ws_F_.p, nV_, d_row_env_.p, ws_scale_.p, d_pinned_.p, nullptr, dx_.p);
     std::vector<double> true_rel;
-    auto cg = cg_solve(sp.linear, mask, &true_rel, nullptr);
+    WarmStats ws;
+    auto cg = cg_solve(sp.linear, mask, &true_rel, learned ? &mask : nullptr, learned ? &ws : nullptr);
+    if (collecting) {  // (features, Newton step / scale) pairs of this solve
+      launch_ws_collect<T>(stream_, nE, d_env_v0_.p, env_mask_.p, coll_per_env_, coll_salt_++, ws_F_.p, dx_.p,
+                           ws_scale_.p, coll_n_, coll_x_.p, coll_t_.p);
+      coll_n_ += int64_t(nE) * coll_per_env_;
+    }
+    launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, dx_.p, prev_dx_.p);
     std::vector<int32_t> lin_ok(size_t(nE), 0);
     for (int e = 0; e < nE; ++e) {
       EnvState& s = S[size_t(e)];
       if (!s.active) continue;
       const EnvCG& c = cg[size_t(e)];
+      if (learned) {
+        s.ws_tried++;
+        s.ws_accepted += ws.accepted[size_t(e)] ? 1 : 0;
+        s.ws_ratio = ws.ratio[size_t(e)];
+      }
       s.lin_iters += c.iter;
       s.lin_conv = c.status == kCgConverged ? SDX_CONV_CONVERGED
                    : c.status == kCgDiverged ? SDX_CONV_DIVERGED
@@ -888,7 +999,7 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
   std::vector<int32_t> reset(size_t(nE), 0);
   for (int e = 0; e < nE; ++e) reset[size_t(e)] = S[size_t(e)].conv == SDX_CONV_DIVERGED ? 1 : 0;
   reset_.upload_async(reset, stream_);
-  launch_post_step<T>(stream_, nV_, d_row_env_.p, reset_.p, dt, u0_.p, u_.p, v_.p);
+  launch_post_step<T>(stream_, nV_, d_row_env_.p, reset_.p, dt, u0_.p, u_.p, v_.p, prev_dx_.p);
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   flush_timers();
   if (stats) {
@@ -906,6 +1017,9 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       o.res_norm = s.res;
       o.linear_rel_res = s.lin_rel;
       o.linear_true_rel_res = s.lin_true_rel;
+      o.warm_start_tried = s.ws_tried;
+      o.warm_start_accepted = s.ws_accepted;
+      o.warm_start_rel_res = s.ws_ratio;
     }
   }
 }
@@ -1008,14 +1122,21 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
   sp.schedule = 0;
   const int64_t n3 = int64_t(3) * nV_;
   std::vector<DBufBase*> reg;
-  DBuf<T> su(reg, &stream_), sv(reg, &stream_);
+  DBuf<T> su(reg, &stream_), sv(reg, &stream_), sp0(reg, &stream_);
   su.alloc(size_t(n3));
   sv.alloc(size_t(n3));
+  sp0.alloc(size_t(n3));
   launch_copy<T>(stream_, n3, u_.p, su.p);
   launch_copy<T>(stream_, n3, v_.p, sv.p);
+  launch_copy<T>(stream_, n3, prev_dx_.p, sp0.p);
+  const int64_t coll_n0 = coll_n_;
+  const uint32_t coll_salt0 = coll_salt_;
   auto restore = [&]() {
     launch_copy<T>(stream_, n3, su.p, u_.p);
     launch_copy<T>(stream_, n3, sv.p, v_.p);
+    launch_copy<T>(stream_, n3, sp0.p, prev_dx_.p);
+    coll_n_ = coll_n0;  // trial steps add no training samples
+    coll_salt_ = coll_salt0;
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   };
   auto measure = [&](const sdx_schedule& c) {

@@ -12,6 +12,7 @@
 
 #include "../../include/sdx_fem.h"
 #include "sdx_kernels.h"
+#include "sdx_mlp.h"
 
 namespace sdx {
 
@@ -150,6 +151,15 @@ class WorldBase {
   virtual void info(int64_t out[8]) = 0;
   virtual void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) = 0;
   virtual void set_schedule(const sdx_schedule& s) = 0;
+  // Learned warm start: a snapshot of the network parameters (FP32, kMlpParams
+  // values; NULL removes the model).
+  virtual void set_ws_params(const float* params) = 0;
+  // Training samples: start collecting (capacity rows, per_env vertices per
+  // environment and solve; capacity 0 stops and frees), read them back.
+  virtual void collect(int64_t capacity, int per_env) = 0;
+  virtual int64_t collected(float* x, float* t, int64_t cap_rows) = 0;
+  // Device rows of the collected samples (after a stream synchronisation).
+  virtual void collected_device(const float** x, const float** t, int64_t* n) = 0;
   virtual double autotune(double dt, const sdx_solver_params& sp, int trials) = 0;
   const sdx_schedule& schedule() const { return sched_; }
 
@@ -253,6 +263,10 @@ class World final : public WorldBase {
   void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) override;
   void set_schedule(const sdx_schedule& s) override;
   double autotune(double dt, const sdx_solver_params& sp, int trials) override;
+  void set_ws_params(const float* params) override;
+  void collect(int64_t capacity, int per_env) override;
+  int64_t collected(float* x, float* t, int64_t cap_rows) override;
+  void collected_device(const float** x, const float** t, int64_t* n) override;
 
  private:
   struct DevTiles {
@@ -269,10 +283,16 @@ class World final : public WorldBase {
   void set_env_mask(const std::vector<int32_t>& mask);
   void assemble(double dt, const sdx_solver_params& sp, bool wantH);
   std::vector<double> read_env_sq();
+  struct WarmStats {
+    std::vector<int32_t> accepted;  // learned warm start kept (per environment)
+    std::vector<double> ratio;      // |b - A x0| / |b| of the learned guess
+  };
   std::vector<EnvCG> cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve,
-                              std::vector<double>* true_rel, const std::vector<int32_t>* warm);
+                              std::vector<double>* true_rel, const std::vector<int32_t>* warm,
+                              WarmStats* learned = nullptr);
   std::vector<EnvCG> cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve,
-                            const std::vector<int32_t>& warm);
+                            const std::vector<int32_t>& warm, bool decide);
+  void ws_features();
   std::vector<double> true_residual(const std::vector<int32_t>& mask);
   void issue_cg(const CgConfig& cfg, int m, int k0, int n, bool timed);
   hipGraphExec_t cg_graph(const CgConfig& cfg, int m, int B);
@@ -341,6 +361,13 @@ class World final : public WorldBase {
   SDX_BUF(double, acc_)
   SDX_BUF(int32_t, solve_mask_)
   SDX_BUF(int32_t, warm_mask_)
+  SDX_BUF(T, prev_dx_)           // last Newton step of each environment
+  SDX_BUF(float, ws_y_)          // Jacobi-scaled residual
+  SDX_BUF(float, ws_F_)          // feature rows [nV][16]
+  SDX_BUF(double, ws_scale_)     // per-environment scale of features and output
+  SDX_BUF(double, ws_ratio_)     // relative residual of the learned guess
+  SDX_BUF(double, ws_alpha_)     // least-residual multiple of the learned guess
+  SDX_BUF(int32_t, d_env_v0_)    // first vertex of each environment [nEnv+1]
   SDX_BUF(int32_t, list_a_)      // work list of active matvec tiles
   SDX_BUF(int32_t, list_v_)      // work list of active vector tiles
   SDX_BUF(int32_t, list_count_)  // list lengths, two parities each
@@ -354,6 +381,18 @@ class World final : public WorldBase {
 #undef SDX_BUF
   DevTiles dt_v_{bufs_, &stream_};
   DevTiles dt_a_{bufs_, &stream_};
+  // kept across topology changes: the warm-start network and collected samples
+  std::vector<DBufBase*> keep_;
+  DBuf<float> ws_params_{keep_, &stream_};
+  DBuf<uint16_t> ws_wpack_{keep_, &stream_};
+  DBuf<float> ws_bpack_{keep_, &stream_};
+  MlpBf16 ws_w_{};
+  bool ws_on_ = false;
+  DBuf<float> coll_x_{keep_, &stream_};
+  DBuf<float> coll_t_{keep_, &stream_};
+  int64_t coll_cap_ = 0, coll_n_ = 0;
+  int coll_per_env_ = 0;
+  uint32_t coll_salt_ = 0;
 };
 
 }  // namespace sdx

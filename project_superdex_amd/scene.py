@@ -169,6 +169,26 @@ class Scene:
             setattr(sc, k, int(v))
         check(lib().sdx_set_schedule(self._w, C.byref(sc)))
 
+    def set_warm_start_model(self, model):
+        """Snapshot `model`'s parameters as the learned CG warm start (None removes it)."""
+        check(lib().sdx_world_set_warm_start_model(self._w, model._m if model is not None else None))
+
+    def collect(self, capacity: int, per_env: int = 64):
+        """Collect (feature row, Newton step / scale) pairs during the next steps."""
+        check(lib().sdx_world_collect(self._w, int(capacity), int(per_env)))
+
+    def collected(self, copy: bool = True):
+        """Rows collected so far: (X[n][16], T[n][3]) or their count."""
+        n = C.c_int64()
+        check(lib().sdx_world_collected(self._w, C.byref(n), None, None, 0))
+        if not copy:
+            return n.value
+        X = np.zeros((n.value, _abi.MLP_ROW), dtype=np.float32)
+        T = np.zeros((n.value, _abi.MLP_OUT), dtype=np.float32)
+        check(lib().sdx_world_collected(self._w, C.byref(n), X.ctypes.data_as(C.POINTER(C.c_float)),
+                                        T.ctypes.data_as(C.POINTER(C.c_float)), X.shape[0]))
+        return X, T
+
     def autotune(self, dt: float, params: _abi.SolverParams, trials: int = 2):
         """Time candidate schedules on the current state (restored afterwards);
         returns (chosen schedule, its step time in ms)."""

@@ -36,7 +36,7 @@ def test_every_declared_symbol_is_exported(L):
 
 def test_struct_layouts_match(L):
     for idx, cls in enumerate([_abi.KrylovParams, _abi.LinearStatus, _abi.SolverParams, _abi.StepStats,
-                               _abi.SoftParams, _abi.Schedule]):
+                               _abi.SoftParams, _abi.Schedule, _abi.TrainParams]):
         assert L.sdx_abi_struct_size(idx) == C.sizeof(cls)
 
 

@@ -1,0 +1,192 @@
+"""GPU: the learned CG warm start (no reference counterpart; the plain method is the judge).
+
+- gradients of the device training kernels against the FP64 oracle network
+  (loss within 1e-5 relative, gradient within 1e-3 of its norm: FP32 sums);
+- BF16 matrix-core inference against the FP64 oracle forward (BF16 weights
+  and activations: within 3e-2 of the output scale) and the FP32 forward;
+- training lowers the loss on a learnable synthetic target;
+- the feature rows and targets the world collects equal the oracle's
+  features of the oracle-assembled system and its exact Newton step;
+- on a held-out seed a trained model predicts the Newton step better than
+  the zero guess; the CG iterations with and without it are reported (the
+  gate keeps a guess only where it removes 80% of the residual, so the count
+  may not grow by more than 3%) and the result stays that of the plain solver
+  (positions within 1e-3 of the largest displacement); a bad model is
+  rejected environment by environment, leaving the plain result and
+  iteration counts.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sps
+
+from oracle import mlp_oracle as mo
+from project_superdex_amd import _abi, mesh
+from project_superdex_amd import scene as gs
+from project_superdex_amd.learned import WarmStartModel
+
+pytestmark = pytest.mark.gpu
+
+
+def random_rows(rng, n, scale=1.0):
+    X = (rng.normal(size=(n, 16)) * scale).astype(np.float32)
+    X[:, 15] = 0.0
+    return X
+
+
+def test_device_gradients_match_oracle():
+    rng = np.random.default_rng(3)
+    m = WarmStartModel(seed=5)
+    p = m.params.astype(np.float64)
+    X = random_rows(rng, 257)
+    T = rng.normal(size=(257, 3)).astype(np.float32)
+    loss, g = m.loss_grad(X, T)
+    lo, go = mo.loss_grad(p, X, T)
+    assert abs(loss - lo) <= 1e-5 * lo
+    assert np.linalg.norm(g - go) <= 1e-3 * np.linalg.norm(go)
+    # every layer's block, not only the large ones
+    for a, b in [(mo.O_W1, mo.O_B1), (mo.O_B1, mo.O_W2), (mo.O_W4, mo.O_B4), (mo.O_B4, mo.N_PARAMS)]:
+        assert np.linalg.norm(g[a:b] - go[a:b]) <= 2e-3 * np.linalg.norm(go[a:b]) + 1e-7
+
+
+def test_bf16_inference_matches_oracle_and_fp32():
+    rng = np.random.default_rng(4)
+    m = WarmStartModel(seed=7)
+    X = random_rows(rng, 1000)  # not a multiple of the 32-row tile
+    yb = m.forward(X, bf16=True)
+    yf = m.forward(X, bf16=False)
+    yo = mo.forward(m.params, X)
+    scale = np.abs(yo).max()
+    assert np.abs(yf - yo).max() <= 1e-4 * scale
+    assert np.abs(yb - yo).max() <= 3e-2 * scale
+    assert np.corrcoef(yb.ravel(), yo.ravel())[0, 1] > 0.999
+
+
+def test_training_lowers_the_loss():
+    rng = np.random.default_rng(5)
+    X = random_rows(rng, 20000)
+    A = rng.normal(size=(15, 3)) * 0.3
+    T = (np.tanh(X[:, :15] @ A) + 0.1 * X[:, :3] ** 2).astype(np.float32)
+    m = WarmStartModel(seed=1)
+    l0, _ = m.loss_grad(X[:4096], T[:4096])
+    losses = m.train(X, T, epochs=30, batch=1024, lr=2e-3)
+    l1, _ = m.loss_grad(X[:4096], T[:4096])
+    assert losses[-1] < losses[0] and l1 < 0.1 * l0, (l0, l1, losses)
+
+
+def oracle_dense(res, ptr, col, blocks):
+    n = len(ptr) - 1
+    A = sps.bsr_matrix((blocks, col, ptr), shape=(3 * n, 3 * n)).toarray()
+    return A, np.linalg.solve(A, res)
+
+
+def test_collected_features_and_targets_match_oracle(oracle):
+    spec = mesh.plate_press_scene(n=4, steps=3, ramp_steps=3)
+    p = _abi.pinned_solver_params(1, lin_rel_tol=1e-10, precision_mode=_abi.PREC_MODE_FP64)
+    sc, rigid = gs.build_scene(spec, precision=_abi.PREC_MODE_FP64)
+    nv = spec.bodies[0].X.shape[0]
+    sc.collect(capacity=10 * nv, per_env=nv)
+    so, ids = oracle.build_scene(spec, fp64=True)
+    dprev = np.zeros(3 * nv)
+    for s in range(2):
+        for r, cid, c in zip(rigid, ids, spec.colliders):
+            r.set_position(c.position(s, spec.dt))
+            so.set_collider_position(cid, c.position(s, spec.dt))
+        a = sc.actors[0]
+        u, v = a.displacements().ravel(), a.velocities().ravel()
+        so.set_displacements(0, u)
+        so.set_velocities(0, v)
+        res, ptr, col, blocks = so.assemble(0, spec.dt, p, u0=u, v0=v)
+        F, scale = mo.features(res, ptr, col, blocks, dprev)
+        _, dx = oracle_dense(res, ptr, col, blocks)
+        sc.step(spec.dt, p)
+        X, T = sc.collected()
+        X, T = X[s * nv:(s + 1) * nv], T[s * nv:(s + 1) * nv]
+        np.testing.assert_allclose(X, F, rtol=1e-4, atol=1e-5 * np.abs(F).max())
+        np.testing.assert_allclose(T, (dx / scale).reshape(-1, 3), atol=1e-6 * np.abs(dx / scale).max())
+        dprev = T.astype(np.float64).ravel() * scale
+
+
+def batched(seed, steps):
+    return mesh.batched_indent_scene(num_envs=16, seed=seed, n_min=4, n_max=8, steps=steps, speed=0.05)
+
+
+def run(spec, params, model=None, collect=None):
+    sc, rigid = gs.build_scene(spec, precision=_abi.PREC_MODE_MIXED)
+    if model is not None:
+        sc.set_warm_start_model(model)
+    if collect:
+        sc.collect(*collect)
+    stats = []
+    for s in range(spec.steps):
+        for r, c in zip(rigid, spec.colliders):
+            r.set_position(c.position(s, spec.dt))
+        stats.append(sc.step(spec.dt, params))
+    return sc, stats
+
+
+@pytest.fixture(scope="module")
+def trained_model():
+    p = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
+    X, T = [], []
+    for seed in range(1, 7):
+        sc, _ = run(batched(seed, 16), p, collect=(16 * 16 * 64, 64))
+        x, t = sc.collected()
+        X.append(x)
+        T.append(t)
+    X, T = np.concatenate(X), np.concatenate(T)
+    m = WarmStartModel(seed=2)
+    m.train(X, T, epochs=40, batch=2048, lr=2e-3)
+    return m
+
+
+def test_learned_warm_start_on_held_out_seed(trained_model):
+    """What the trained network gains on a held-out seed: its guesses beat the
+    zero guess, and the CG iteration count with the learned warm start (kept
+    only where its best multiple removes 80% of the residual) is reported and
+    may not exceed the plain method's by more than 3%; the result is the plain
+    solver's."""
+    spec = batched(100, 16)  # held-out seed
+    zero = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
+    learned = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
+    learned.warm_start = _abi.WARM_START_LEARNED
+    s0, st0 = run(spec, zero, collect=(16 * 16 * 64, 64))
+    X, T = s0.collected()
+    Y = trained_model.forward(X, bf16=True)
+    err = np.linalg.norm(Y - T) / np.linalg.norm(T)
+    s1, st1 = run(spec, learned, model=trained_model)
+    it0 = sum(e["linear_iters"] for st in st0 for e in st)
+    it1 = sum(e["linear_iters"] for st in st1 for e in st)
+    tried = sum(e["warm_start_tried"] for st in st1 for e in st)
+    kept = sum(e["warm_start_accepted"] for st in st1 for e in st)
+    ratio = np.median([e["warm_start_rel_res"] for st in st1 for e in st])
+    print(f"held-out: |net - dx|/|dx| = {err:.3f}; CG iterations zero init {it0}, learned {it1} "
+          f"(x{it1 / it0:.3f}); guesses kept {kept}/{tried}, median residual ratio {ratio:.3f}")
+    assert err < 0.9  # the network predicts the Newton step better than zero does
+    assert tried == 16 * 16 and ratio < 1.0
+    assert it1 <= 1.03 * it0, (it0, it1)
+    for st in st1:
+        for e in st:
+            assert e["linear_convergence"] == _abi.CONV_CONVERGED and e["linear_true_rel_res"] <= 1e-6
+    for a, b in zip(s0.actors, s1.actors):
+        ua, ub = a.displacements(), b.displacements()
+        assert np.abs(ua - ub).max() <= 1e-3 * np.abs(ua).max()
+
+
+def test_bad_model_falls_back_to_zero_init():
+    spec = batched(101, 6)
+    zero = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
+    learned = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
+    learned.warm_start = _abi.WARM_START_LEARNED
+    bad = WarmStartModel(seed=3)
+    bad.params = bad.params * 8.0  # guesses of vertex-to-vertex noise: no multiple of them helps
+
+    s0, st0 = run(spec, zero)
+    s1, st1 = run(spec, learned, model=bad)
+    assert sum(e["warm_start_accepted"] for st in st1 for e in st) == 0
+    assert all(e["warm_start_rel_res"] >= 0.2 for st in st1 for e in st)
+    for a, b in zip(st0, st1):
+        for ea, eb in zip(a, b):
+            assert abs(ea["linear_iters"] - eb["linear_iters"]) <= 2
+    for a, b in zip(s0.actors, s1.actors):
+        ua, ub = a.displacements(), b.displacements()
+        assert np.abs(ua - ub).max() <= 1e-4 * np.abs(ua).max()
