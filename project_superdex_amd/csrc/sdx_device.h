@@ -90,4 +90,14 @@ __device__ __forceinline__ void acc_zero(A* dst) {
   for (int k = threadIdx.x; k < kSlots * kSlotStride; k += blockDim.x) dst[k] = A(0);
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin; block b of a grid of n
+// gets work item xcd_tile(b, n) so that each XCD takes a contiguous run of
+// the items and the neighbouring data they gather stays in its own L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  constexpr int kXcd = 8;
+  const int q = n / kXcd, rem = n % kXcd;
+  const int x = b % kXcd, i = b / kXcd;
+  return x * q + (x < rem ? x : rem) + i;
+}
+
 }  // namespace sdx

@@ -67,8 +67,9 @@ constexpr int kSpmvLanes = 16;    // lanes per block row of the plain matvec (CG
 constexpr int kSpmvThreads = 256;
 constexpr int kSpmvRows = kSpmvThreads / kSpmvLanes;  // 16 rows per plain matvec tile
 constexpr int kVecThreads = 256;  // rows per vector tile
-constexpr int kHeStride = 80;     // packed element Hessian (78 used) per tet
-constexpr int kReStride = 12;     // element residual per tet
+// Off-diagonal local pairs (a, b), a < b, of a tet in the order of their index k.
+constexpr int kPairA[6] = {0, 0, 0, 1, 1, 2};
+constexpr int kPairB[6] = {1, 2, 3, 2, 3, 3};
 // Cross-block accumulators: per environment and set, kSlots lines of
 // kSlotStride values (one 128-byte line per slot for doubles).
 constexpr int kSlots = 16;
@@ -89,12 +90,16 @@ struct MeshView {
   const int32_t* col;       // [nB]
   const int32_t* slot_row;  // [nB]
   const int32_t* diag_slot; // [nV]
-  const int32_t* slot_ptr;  // [nB+1] element contributions per slot
-  const uint32_t* slot_code;   // e*16 + a*4 + b
+  int32_t n_items;          // diagonal slots, then upper slots
+  const int32_t* item_a;    // [4 n_items] slot, mirror slot, first and end element record
+  const int32_t* item_b;    // [4 n_items] row, column, first and end contact contribution
+  const int32_t* item_env;  // [n_items] environment
+  const int32_t* eptr;      // [nB+1] off-diagonal element blocks of each upper slot (pair-sorted)
+  const int32_t* pos_v;     // [4 nT] vertex-sorted position of contribution (tet, a)
+  const uint32_t* pos_e;    // [6 nT] pair-sorted position of pair k of a tet; top bit: transposed
   const int32_t* slot_ptr_c;   // [nB+1] contact contributions per slot
   const uint32_t* slot_code_c; // tri*16 + a*4 + b
-  const int32_t* vtx_ptr;   // [nV+1]
-  const uint32_t* vtx_code; // e*4 + a
+  const int32_t* vtx_ptr;   // [nV+1] element contributions (tet, a) of each vertex
   const int32_t* vtx_ptr_c; // [nV+1]
   const uint32_t* vtx_code_c; // tri*4 + a
   const uint8_t* pinned;    // [3 nV]
@@ -115,19 +120,25 @@ struct FieldPtrs {
 };
 
 // ---- assembly ----
+// Element residuals and Hessians written straight to their destination
+// order: residual and diagonal block of (tet, a) at pos_v[4 tet + a] of Rv
+// ([.][3]) and Dv ([.][6]: 00 11 22 01 02 12), off-diagonal block k at
+// pos_e[6 tet + k] of Ev ([.][9], row-major for the (min, max) vertex pair).
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0,
-                    const T* v0, double dt, const double grav[3], int wantH, int project, T* Re, T* He,
+                    const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv, T* Ev,
                     int block = 128, const KernelEvents* ev = nullptr);
 template <class T>
 void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, int wantH, T* Rc,
                     T* Hc);
+// BSR values (and the preconditioner blocks) from the element and contact
+// contributions, over the work items of the mesh (diagonal and upper slots).
 template <class T>
-void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* He, const T* Hc, int prec, T* vals,
+void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* Dv, const T* Ev, const T* Hc, int prec, T* vals,
                        T* dinv, const KernelEvents* ev = nullptr);
 // acc: set 0 of the accumulators (zeroed by the caller); per-env ||res||^2.
 template <class T, class A>
-void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Re, const T* Rc, T* res, A* acc);
+void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Rv, const T* Rc, T* res, A* acc);
 template <class A>
 void launch_fill(hipStream_t s, int64_t n, A v, A* d);
 // out[e * nv + k] = sum over slots of set 0, k < nv.

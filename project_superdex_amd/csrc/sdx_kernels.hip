@@ -32,9 +32,9 @@ namespace sdx {
 template <class T>
 __global__ void __launch_bounds__(256) k_element(MeshView m, FieldPtrs<T> f, const T* __restrict__ u,
                                                  const T* __restrict__ u0, const T* __restrict__ v0, T dt,
-                                                 T gx, T gy, T gz, int wantH, int project, T* __restrict__ Re,
-                                                 T* __restrict__ He) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 T gx, T gy, T gz, int wantH, int project, T* __restrict__ Rv,
+                                                 T* __restrict__ Dv, T* __restrict__ Ev) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= m.nT) return;
   const int env = m.tet_env[t];
   if (!m.env_mask[env]) return;
@@ -79,9 +79,15 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, FieldPtrs<T> f, con
       if (mdamp > T(0)) r += mdamp * m20 * (wd[3 * a + d] + sd[d]);
       res[3 * a + d] = r;
     }
-  T* re = Re + (size_t)t * kReStride;
+  int pv[4];
 #pragma unroll
-  for (int k = 0; k < 12; ++k) re[k] = res[k];
+  for (int a = 0; a < 4; ++a) {
+    pv[a] = m.pos_v[4 * t + a];
+    T* re = Rv + (size_t)pv[a] * 3;
+    re[0] = res[3 * a];
+    re[1] = res[3 * a + 1];
+    re[2] = res[3 * a + 2];
+  }
   if (!wantH) return;
   const T ms = (idt2 + mdamp) * m20;
 #pragma unroll
@@ -100,9 +106,30 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, FieldPtrs<T> f, con
         blk[8] += add;
       }
     }
-  T* he = He + (size_t)t * kHeStride;
+  // diagonal blocks (6 entries) to the vertex-sorted slots, off-diagonal
+  // blocks to the pair-sorted slots, transposed where the pair runs backwards
 #pragma unroll
-  for (int k = 0; k < 78; ++k) he[k] = H[k];
+  for (int a = 0; a < 4; ++a) {
+    const T* q = H + he_block_offset(a, a);
+    T* d = Dv + (size_t)pv[a] * 6;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d[k] = q[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const uint32_t pe = m.pos_e[6 * t + k];
+    const T* q = H + he_block_offset(kPairA[k], kPairB[k]);
+    T* d = Ev + (size_t)(pe & 0x7fffffffu) * 9;
+    if (pe >> 31) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) d[3 * r + c] = q[3 * c + r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 9; ++r) d[r] = q[r];
+    }
+  }
 }
 
 // ------------------------------------------------------------------ contact
@@ -223,96 +250,94 @@ __device__ __forceinline__ void inv3(const T a[9], T o[9]) {
   o[8] = c22 * id;
 }
 
-// One thread per BSR slot: sum its element and contact contributions, apply
-// Dirichlet rows/columns (mochi_solve.cpp:337-396), write the 9 SoA planes,
-// and on diagonal slots the preconditioner block.
+// Every BSR slot: sum its element and contact contributions, apply Dirichlet
+// rows/columns (mochi_solve.cpp:337-396), write the 9 SoA planes, and on
+// diagonal slots the preconditioner block.
+// Entry (r, c) of a symmetric 3x3 block stored as 00 11 22 01 02 12.
+__device__ __forceinline__ int sym_entry(int r, int c) { return r == c ? r : 2 + r + c; }
+
 template <class T>
-__global__ void __launch_bounds__(256) k_gather_bsr(MeshView m, const T* __restrict__ He, const T* __restrict__ Hc,
-                                                    int prec, T* __restrict__ vals, T* __restrict__ dinv) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= m.nB) return;
-  const int i = m.slot_row[s];
-  if (!m.env_mask[m.row_env[i]]) return;
-  const int j = m.col[s];
-  T acc[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) acc[k] = T(0);
-  for (int c = m.slot_ptr[s]; c < m.slot_ptr[s + 1]; ++c) {
-    const uint32_t code = m.slot_code[c];
-    const uint32_t e = code >> 4;
-    const int a = (code >> 2) & 3, b = code & 3;
-    const T* h = He + (size_t)e * kHeStride;
-    if (a == b) {
-      const T* q = h + he_block_offset(a, a);
-      acc[0] += q[0];
-      acc[4] += q[1];
-      acc[8] += q[2];
-      acc[1] += q[3];
-      acc[3] += q[3];
-      acc[2] += q[4];
-      acc[6] += q[4];
-      acc[5] += q[5];
-      acc[7] += q[5];
-    } else if (a < b) {
-      const T* q = h + he_block_offset(a, b);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) acc[k] += q[k];
-    } else {
-      const T* q = h + he_block_offset(b, a);
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) acc[r * 3 + cc] += q[cc * 3 + r];
-    }
-  }
-  for (int c = m.slot_ptr_c[s]; c < m.slot_ptr_c[s + 1]; ++c) {
-    const uint32_t code = m.slot_code_c[c];
-    const uint32_t tr = code >> 4;
-    int a = (code >> 2) & 3, b = code & 3;
-    if (a > b) {
-      const int t = a;
-      a = b;
-      b = t;
-    }
-    const T* q = Hc + (size_t)tr * kHcStride + 6 * hc_block(a, b);
-    acc[0] += q[0];
-    acc[4] += q[1];
-    acc[8] += q[2];
-    acc[1] += q[3];
-    acc[3] += q[3];
-    acc[2] += q[4];
-    acc[6] += q[4];
-    acc[5] += q[5];
-    acc[7] += q[5];
-  }
-  const uint8_t* pin = m.pinned;
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      if (pin[3 * i + r] | pin[3 * j + c]) acc[r * 3 + c] = (i == j && r == c) ? T(1) : T(0);
-  const int nB = m.nB;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) vals[(size_t)k * nB + s] = acc[k];
+__global__ void __launch_bounds__(256) k_gather_bsr(MeshView m, const T* __restrict__ Dv, const T* __restrict__ Ev,
+                                                    const T* __restrict__ Hc, int prec, T* __restrict__ vals,
+                                                    T* __restrict__ dinv) {
+  // Nine lanes per work item, lane q owning entry q = 3 r + c of the 3x3
+  // block, so the item's lanes read every element record as one contiguous
+  // run.  Items: the diagonal slots (first) and the upper slots (i < j); an
+  // upper slot also writes its mirror (j, i) transposed, so each pair's
+  // records are read once.  Seven items per wave (lane 63 idles).  An item's
+  // description is two int4 (slot, mirror, record range; row, column,
+  // contact range) plus its environment, so no load waits on another.
+  const int wave = xcd_tile(blockIdx.x, gridDim.x) * (256 / 64) + int(threadIdx.x) / 64;
+  const int lane = int(threadIdx.x) % 64;
+  const int g = lane / 9, q = lane % 9;
+  const int item = wave * 7 + g;
+  if (g >= 7 || item >= m.n_items) return;
+  const int4 ia = reinterpret_cast<const int4*>(m.item_a)[item];  // slot, mirror, first, end record
+  const int4 ib = reinterpret_cast<const int4*>(m.item_b)[item];  // row, column, first, end contact
+  if (!m.env_mask[m.item_env[item]]) return;
+  const int s = ia.x, i = ib.x, j = ib.y;
+  const int r = q / 3, c = q % 3;
+  const int e = sym_entry(r, c);
+  T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
   if (i == j) {
-    T o[9];
-    if (prec == 2) {
-      inv3(acc, o);
-    } else {
-#pragma unroll
-      for (int k = 0; k < 9; ++k) o[k] = T(0);
-      if (prec == 1) {
-        o[0] = T(1) / acc[0];
-        o[4] = T(1) / acc[4];
-        o[8] = T(1) / acc[8];
-      } else {
-        o[0] = o[4] = o[8] = T(1);
-      }
+    int k = ia.z;
+    for (; k + 3 < ia.w; k += 4) {
+      a0 += Dv[(size_t)k * 6 + e];
+      a1 += Dv[(size_t)(k + 1) * 6 + e];
+      a2 += Dv[(size_t)(k + 2) * 6 + e];
+      a3 += Dv[(size_t)(k + 3) * 6 + e];
     }
-    const int nV = m.nV;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) dinv[(size_t)k * nV + i] = o[k];
+    for (; k < ia.w; ++k) a0 += Dv[(size_t)k * 6 + e];
+  } else {
+    int k = ia.z;
+    for (; k + 1 < ia.w; k += 2) {
+      a0 += Ev[(size_t)k * 9 + q];
+      a1 += Ev[(size_t)(k + 1) * 9 + q];
+    }
+    if (k < ia.w) a2 += Ev[(size_t)k * 9 + q];
   }
+  T acc = (a0 + a1) + (a2 + a3);
+  // contact blocks are symmetric and the same triangles serve (i, j) and (j, i)
+  for (int k = ib.z; k < ib.w; ++k) {
+    const uint32_t code = m.slot_code_c[k];
+    int a = (code >> 2) & 3, bb = code & 3;
+    if (a > bb) {
+      const int t = a;
+      a = bb;
+      bb = t;
+    }
+    acc += Hc[(size_t)(code >> 4) * kHcStride + 6 * hc_block(a, bb) + e];
+  }
+  // Dirichlet rows / columns (mochi_solve.cpp:337-396); the test is symmetric in (i r, j c)
+  const uint8_t* pin = m.pinned;
+  if (pin[3 * i + r] | pin[3 * j + c]) acc = (i == j && r == c) ? T(1) : T(0);
+  const int nB = m.nB;
+  vals[(size_t)q * nB + s] = acc;
+  if (i != j) {
+    vals[(size_t)(3 * c + r) * nB + ia.y] = acc;
+    return;
+  }
+  // diagonal block: every lane of the item gathers the 9 entries and writes
+  // its entry of the preconditioner block
+  const int base = g * 9;
+  T blk[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) blk[k] = __shfl(acc, base + k, 64);
+  T o[9];
+  if (prec == 2) {
+    inv3(blk, o);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[k] = T(0);
+    if (prec == 1) {
+      o[0] = T(1) / blk[0];
+      o[4] = T(1) / blk[4];
+      o[8] = T(1) / blk[8];
+    } else {
+      o[0] = o[4] = o[8] = T(1);
+    }
+  }
+  dinv[(size_t)q * m.nV + i] = o[q];
 }
 
 template <class T>
@@ -346,18 +371,17 @@ __global__ void __launch_bounds__(256) k_block_inverse(int nV, const int32_t* __
 // contributions, Dirichlet rows zeroed; per-environment ||res||^2 added into
 // acc (set 0, zeroed by the caller).
 template <class T, class A>
-__global__ void __launch_bounds__(kVecThreads) k_gather_res(MeshView m, Tiles tv, const T* __restrict__ Re,
+__global__ void __launch_bounds__(kVecThreads) k_gather_res(MeshView m, Tiles tv, const T* __restrict__ Rv,
                                                             const T* __restrict__ Rc, T* __restrict__ res, A* acc) {
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int env = tv.env[tile];
   if (!m.env_mask[env]) return;
   const int i = tv.row0[tile] + threadIdx.x;
   A part[1] = {A(0)};
   if (i < tv.row1[tile]) {
     T r0 = T(0), r1 = T(0), r2 = T(0);
-    for (int c = m.vtx_ptr[i]; c < m.vtx_ptr[i + 1]; ++c) {
-      const uint32_t code = m.vtx_code[c];
-      const T* q = Re + (size_t)(code >> 2) * kReStride + 3 * (code & 3);
+    for (int c = m.vtx_ptr[i]; c < m.vtx_ptr[i + 1]; ++c) {  // contiguous run
+      const T* q = Rv + (size_t)c * 3;
       r0 += q[0];
       r1 += q[1];
       r2 += q[2];
@@ -524,14 +548,6 @@ __device__ __forceinline__ double slot_reduce(double v) {
   return v;
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
-// run of tiles so the vector entries its matvec rows gather stay in its own L2.
-__device__ __forceinline__ int xcd_tile(int b, int n) {
-  constexpr int kXcd = 8;
-  const int q = n / kXcd, rem = n % kXcd;
-  const int x = b % kXcd, i = b / kXcd;
-  return x * q + (x < rem ? x : rem) + i;
-}
 
 // CG iteration k, matvec half.  Every block re-derives the stop decision on
 // r_k from R_k; the first tile of a finished environment records it.  Then
@@ -1049,11 +1065,11 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs
 
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0, const T* v0,
-                    double dt, const double grav[3], int wantH, int project, T* Re, T* He, int block,
+                    double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv, T* Ev, int block,
                     const KernelEvents* ev) {
   if (m.nT == 0) return;
   SDX_LAUNCH(ev, k_element<T>, dim3(blocks_for(m.nT, block)), dim3(block), s, m, f, u, u0, v0, T(dt), T(grav[0]),
-             T(grav[1]), T(grav[2]), wantH, project, Re, He);
+             T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev);
 }
 
 template <class T>
@@ -1063,16 +1079,17 @@ void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, con
 }
 
 template <class T>
-void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* He, const T* Hc, int prec, T* vals, T* dinv,
-                       const KernelEvents* ev) {
-  if (m.nB == 0) return;
-  SDX_LAUNCH(ev, k_gather_bsr<T>, dim3(blocks_for(m.nB, 256)), dim3(256), s, m, He, Hc, prec, vals, dinv);
+void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* Dv, const T* Ev, const T* Hc, int prec, T* vals,
+                       T* dinv, const KernelEvents* ev) {
+  if (m.n_items == 0) return;
+  const int64_t waves = (int64_t(m.n_items) + 6) / 7;
+  SDX_LAUNCH(ev, k_gather_bsr<T>, dim3(unsigned((waves + 3) / 4)), dim3(256), s, m, Dv, Ev, Hc, prec, vals, dinv);
 }
 
 template <class T, class A>
-void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Re, const T* Rc, T* res, A* acc) {
+void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Rv, const T* Rc, T* res, A* acc) {
   if (tv.n == 0) return;
-  hipLaunchKernelGGL((k_gather_res<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, m, tv, Re, Rc, res, acc);
+  hipLaunchKernelGGL((k_gather_res<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, m, tv, Rv, Rc, res, acc);
 }
 
 template <class A>
@@ -1230,10 +1247,10 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
 
 #define SDX_INST_T(T)                                                                                            \
   template void launch_element<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, const T*,        \
-                                  const T*, double, const double*, int, int, T*, T*, int,                       \
+                                  const T*, double, const double*, int, int, T*, T*, T*, int,                   \
                                   const KernelEvents*);                                                         \
   template void launch_contact<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, int, T*, T*);    \
-  template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const T*, const T*, int, T*, T*,             \
+  template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const T*, const T*, const T*, int, T*, T*,   \
                                      const KernelEvents*);                                                      \
   template void launch_spmv_plain<T>(hipStream_t, const Tiles&, int, const int32_t*, const int32_t*, const T*,  \
                                      int32_t, const T*, T*);                                                    \

@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(kVecThreads) k_ws_scaled_residual(MeshView m, 
                                                                     const T* __restrict__ vals,
                                                                     const T* __restrict__ dprev, float* __restrict__ y,
                                                                     double* acc) {
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int e = tv.env[tile];
   const int i = tv.row0[tile] + threadIdx.x;
   double part[3] = {0.0, 0.0, 0.0};
@@ -466,7 +466,7 @@ template <class T>
 __global__ void __launch_bounds__(kVecThreads) k_ws_features(MeshView m, Tiles tv, const T* __restrict__ vals,
                                                              const float* __restrict__ y, const T* __restrict__ dprev,
                                                              const double* __restrict__ scale, float* __restrict__ F) {
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int e = tv.env[tile];
   const int i = tv.row0[tile] + threadIdx.x;
   if (i >= tv.row1[tile]) return;

@@ -215,8 +215,8 @@ void World<T>::finalize() {
   std::vector<uint8_t> pinned(size_t(3) * size_t(nV_), 0);
   std::vector<T> pin_value(size_t(3) * size_t(nV_), T(0));
   std::vector<int32_t> tris, tri_env;
-  std::vector<int32_t> slot_keys;
-  std::vector<uint32_t> slot_codes;
+  std::vector<int32_t> edge_keys;
+  std::vector<uint32_t> edge_codes;
   std::vector<int32_t> vtx_keys;
   std::vector<uint32_t> vtx_codes;
   std::vector<int32_t> slot_keys_c, vtx_keys_c;
@@ -278,10 +278,14 @@ void World<T>::finalize() {
       for (int a = 0; a < 4; ++a) {
         vtx_keys.push_back(v0 + tv[a]);
         vtx_codes.push_back(uint32_t(gt) * 4u + uint32_t(a));
-        for (int b = 0; b < 4; ++b) {
-          slot_keys.push_back(b0 + FindSlot(P, tv[a], tv[b]));
-          slot_codes.push_back(uint32_t(gt) * 16u + uint32_t(a) * 4u + uint32_t(b));
-        }
+      }
+      // off-diagonal pairs (a < b) go to the upper slot (min, max) of their
+      // vertex pair; the flag bit marks blocks stored transposed
+      for (int k = 0; k < 6; ++k) {
+        const int a = kPairA[k], b = kPairB[k];
+        const int va = tv[a], vb = tv[b];
+        edge_keys.push_back(b0 + FindSlot(P, std::min(va, vb), std::max(va, vb)));
+        edge_codes.push_back((uint32_t(gt) * 6u + uint32_t(k)) | (va > vb ? 0x80000000u : 0u));
       }
     }
     std::vector<int32_t> atris;
@@ -309,14 +313,40 @@ void World<T>::finalize() {
   nF_ = int(tri_env.size());
   h_row_ptr_ = row_ptr;
   h_col_ = col;
-  if (int64_t(nT_) * 16 >= (int64_t(1) << 32) || int64_t(nF_) * 16 >= (int64_t(1) << 32))
+  if (int64_t(nT_) * 6 >= (int64_t(1) << 31) || int64_t(nF_) * 16 >= (int64_t(1) << 32))
     throw std::invalid_argument("mesh too large for 32-bit contribution codes");
-  std::vector<int32_t> slot_ptr, vtx_ptr, slot_ptr_c, vtx_ptr_c;
-  std::vector<uint32_t> slot_code, vtx_code, slot_code_c, vtx_code_c;
-  CountingCsr(nB_, slot_keys, slot_codes, slot_ptr, slot_code);
+  std::vector<int32_t> eptr, vtx_ptr, slot_ptr_c, vtx_ptr_c;
+  std::vector<uint32_t> ecode, vtx_code, slot_code_c, vtx_code_c;
+  CountingCsr(nB_, edge_keys, edge_codes, eptr, ecode);
   CountingCsr(nV_, vtx_keys, vtx_codes, vtx_ptr, vtx_code);
   CountingCsr(nB_, slot_keys_c, slot_codes_c, slot_ptr_c, slot_code_c);
   CountingCsr(nV_, vtx_keys_c, vtx_codes_c, vtx_ptr_c, vtx_code_c);
+  // destination of every element contribution: the vertex-sorted position of
+  // (tet, a) for residuals and diagonal blocks, the pair-sorted position of
+  // (tet, pair) for off-diagonal blocks (flag bit: stored transposed)
+  std::vector<int32_t> pos_v(size_t(4) * size_t(nT_));
+  for (size_t q = 0; q < vtx_code.size(); ++q) pos_v[size_t(vtx_code[q])] = int32_t(q);
+  std::vector<uint32_t> pos_e(size_t(6) * size_t(nT_));
+  for (size_t q = 0; q < ecode.size(); ++q) pos_e[size_t(ecode[q] & 0x7fffffffu)] = uint32_t(q) | (ecode[q] & 0x80000000u);
+  // gather work items: the diagonal slots, then the upper slots (i < j),
+  // each with its mirror (j, i), record range and contact range; lower slots
+  // are written by the item of their upper mirror
+  std::vector<int32_t> item_a, item_b, item_env;
+  item_a.reserve(size_t(4) * (size_t(nV_) + size_t(nB_ - nV_) / 2));
+  item_b.reserve(item_a.capacity());
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < nV_; ++i)
+      for (int k = row_ptr[size_t(i)]; k < row_ptr[size_t(i) + 1]; ++k) {
+        const int j = col[size_t(k)];
+        if (pass == 0 ? j != i : j <= i) continue;
+        auto b = col.begin() + row_ptr[size_t(j)], e2 = col.begin() + row_ptr[size_t(j) + 1];
+        const int mirror = int32_t(std::lower_bound(b, e2, i) - col.begin());  // slot (j, i)
+        const int r0 = pass == 0 ? vtx_ptr[size_t(i)] : eptr[size_t(k)];
+        const int r1 = pass == 0 ? vtx_ptr[size_t(i) + 1] : eptr[size_t(k) + 1];
+        item_a.insert(item_a.end(), {k, mirror, r0, r1});
+        item_b.insert(item_b.end(), {i, j, slot_ptr_c[size_t(k)], slot_ptr_c[size_t(k) + 1]});
+        item_env.push_back(row_env[size_t(i)]);
+      }
   BuildTiles(env_v0_, nEnv, kVecThreads, tiles_v_);
   BuildTiles(env_v0_, nEnv, kSpmvThreads / sched_.spmv_lanes, tiles_a_);
 
@@ -330,12 +360,16 @@ void World<T>::finalize() {
   d_col_.upload(col);
   d_slot_row_.upload(slot_row);
   d_diag_slot_.upload(diag_slot);
-  d_slot_ptr_.upload(slot_ptr);
-  d_slot_code_.upload(slot_code);
+  d_eptr_.upload(eptr);
+  d_item_a_.upload(item_a);
+  d_item_b_.upload(item_b);
+  d_item_env_.upload(item_env);
+  n_items_ = int(item_env.size());
+  d_pos_v_.upload(pos_v);
+  d_pos_e_.upload(pos_e);
   d_slot_ptr_c_.upload(slot_ptr_c);
   d_slot_code_c_.upload(slot_code_c);
   d_vtx_ptr_.upload(vtx_ptr);
-  d_vtx_code_.upload(vtx_code);
   d_vtx_ptr_c_.upload(vtx_ptr_c);
   d_vtx_code_c_.upload(vtx_code_c);
   d_pinned_.upload(pinned);
@@ -361,8 +395,9 @@ void World<T>::finalize() {
   sv_.alloc(n3);
   w_.alloc(n3);
   rref_.alloc(n3);
-  Re_.alloc(size_t(kReStride) * size_t(nT_));
-  He_.alloc(size_t(kHeStride) * size_t(nT_));
+  cres_.alloc(size_t(12) * size_t(nT_));
+  cdiag_.alloc(size_t(24) * size_t(nT_));
+  coff_.alloc(size_t(54) * size_t(nT_));
   Rc_.alloc(size_t(9) * size_t(std::max(nF_, 1)));
   Hc_.alloc(size_t(36) * size_t(std::max(nF_, 1)));
   vals_.alloc(size_t(9) * size_t(nB_));
@@ -406,12 +441,16 @@ void World<T>::finalize() {
   mesh_.col = d_col_.p;
   mesh_.slot_row = d_slot_row_.p;
   mesh_.diag_slot = d_diag_slot_.p;
-  mesh_.slot_ptr = d_slot_ptr_.p;
-  mesh_.slot_code = d_slot_code_.p;
+  mesh_.eptr = d_eptr_.p;
+  mesh_.item_a = d_item_a_.p;
+  mesh_.item_b = d_item_b_.p;
+  mesh_.item_env = d_item_env_.p;
+  mesh_.n_items = n_items_;
+  mesh_.pos_v = d_pos_v_.p;
+  mesh_.pos_e = d_pos_e_.p;
   mesh_.slot_ptr_c = d_slot_ptr_c_.p;
   mesh_.slot_code_c = d_slot_code_c_.p;
   mesh_.vtx_ptr = d_vtx_ptr_.p;
-  mesh_.vtx_code = d_vtx_code_.p;
   mesh_.vtx_ptr_c = d_vtx_ptr_c_.p;
   mesh_.vtx_code_c = d_vtx_code_c_.p;
   mesh_.pinned = d_pinned_.p;
@@ -477,22 +516,22 @@ void World<T>::set_env_mask(const std::vector<int32_t>& mask) {
 template <class T>
 void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH) {
   const int project = sp.psd_proj_mode == SDX_PSD_ALWAYS ? 1 : 0;
-  launch_element<T>(stream_, mesh_, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, Re_.p,
-                    He_.p, sched_.element_block, sampled_events(2));
+  launch_element<T>(stream_, mesh_, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, cres_.p,
+                    cdiag_.p, coff_.p, sched_.element_block, sampled_events(2));
   launch_contact<T>(stream_, mesh_, fields_, u_.p, wantH ? 1 : 0, Rc_.p, Hc_.p);
   if (wantH) {
-    launch_gather_bsr<T>(stream_, mesh_, He_.p, Hc_.p, sp.linear.preconditioner, vals_.p, dinv_.p,
+    launch_gather_bsr<T>(stream_, mesh_, cdiag_.p, coff_.p, Hc_.p, sp.linear.preconditioner, vals_.p, dinv_.p,
                          sampled_events(3));
   }
   const int64_t set0 = int64_t(nEnv_) * kSlots * kSlotStride;
   if (precision_ == SDX_PREC_MODE_FP32) {
     T* acc = reinterpret_cast<T*>(acc_.p);
     launch_fill<T>(stream_, set0, T(0), acc);
-    launch_gather_res<T, T>(stream_, mesh_, dt_v_.view, Re_.p, Rc_.p, res_.p, acc);
+    launch_gather_res<T, T>(stream_, mesh_, dt_v_.view, cres_.p, Rc_.p, res_.p, acc);
     launch_acc_finalize<T>(stream_, nEnv_, 1, acc, env_sq_.p);
   } else {
     launch_fill<double>(stream_, set0, 0.0, acc_.p);
-    launch_gather_res<T, double>(stream_, mesh_, dt_v_.view, Re_.p, Rc_.p, res_.p, acc_.p);
+    launch_gather_res<T, double>(stream_, mesh_, dt_v_.view, cres_.p, Rc_.p, res_.p, acc_.p);
     launch_acc_finalize<double>(stream_, nEnv_, 1, acc_.p, env_sq_.p);
   }
 }

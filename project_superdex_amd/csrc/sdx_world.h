@@ -305,7 +305,7 @@ class World final : public WorldBase {
   int graph_burst_ = 0;
 
   std::vector<DBufBase*> bufs_;
-  int nEnv_ = 0, nV_ = 0, nT_ = 0, nB_ = 0, nF_ = 0;
+  int nEnv_ = 0, nV_ = 0, nT_ = 0, nB_ = 0, nF_ = 0, n_items_ = 0;
   std::vector<int32_t> env_v0_;
   std::vector<int32_t> env_b0_;
   std::vector<int32_t> h_row_ptr_, h_col_;
@@ -322,12 +322,15 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, d_col_)
   SDX_BUF(int32_t, d_slot_row_)
   SDX_BUF(int32_t, d_diag_slot_)
-  SDX_BUF(int32_t, d_slot_ptr_)
-  SDX_BUF(uint32_t, d_slot_code_)
+  SDX_BUF(int32_t, d_eptr_)
+  SDX_BUF(int32_t, d_item_a_)
+  SDX_BUF(int32_t, d_item_b_)
+  SDX_BUF(int32_t, d_item_env_)
+  SDX_BUF(int32_t, d_pos_v_)
+  SDX_BUF(uint32_t, d_pos_e_)
   SDX_BUF(int32_t, d_slot_ptr_c_)
   SDX_BUF(uint32_t, d_slot_code_c_)
   SDX_BUF(int32_t, d_vtx_ptr_)
-  SDX_BUF(uint32_t, d_vtx_code_)
   SDX_BUF(int32_t, d_vtx_ptr_c_)
   SDX_BUF(uint32_t, d_vtx_code_c_)
   SDX_BUF(uint8_t, d_pinned_)
@@ -351,8 +354,9 @@ class World final : public WorldBase {
   SDX_BUF(T, sv_)
   SDX_BUF(T, w_)
   SDX_BUF(double, rref_)
-  SDX_BUF(T, Re_)
-  SDX_BUF(T, He_)
+  SDX_BUF(T, cres_)   // element residuals, vertex-sorted [4 nT][3]
+  SDX_BUF(T, cdiag_)  // element diagonal blocks, vertex-sorted [4 nT][6]
+  SDX_BUF(T, coff_)   // element off-diagonal blocks, pair-sorted [6 nT][9]
   SDX_BUF(T, Rc_)
   SDX_BUF(T, Hc_)
   SDX_BUF(T, vals_)
