@@ -65,6 +65,7 @@ struct Tiles {
 
 constexpr int kSpmvLanes = 16;    // lanes per block row of the plain matvec (CG matvec: 4, 8 or 16)
 constexpr int kSpmvThreads = 256;
+constexpr int kSpmvChunk = 16;    // blocks of a row whose loads the CG matvec issues together
 constexpr int kSpmvRows = kSpmvThreads / kSpmvLanes;  // 16 rows per plain matvec tile
 constexpr int kVecThreads = 256;  // rows per vector tile
 // Off-diagonal local pairs (a, b), a < b, of a tet in the order of their index k.
@@ -148,6 +149,7 @@ void launch_acc_finalize(hipStream_t s, int nEnv, int nv, const A* acc, double* 
 // ---- CG ----
 template <class A>
 void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, A* acc);
+// Vectors of the CG are [3 nV] except z, stored [4 nV] (padded per vertex).
 // r0, z0, scaling and R_0.  warm (may be NULL): environments that keep x as
 // the initial guess and take r0 from r64 = b - A x (launch_residual64);
 // alpha (may be NULL): per-environment factor applied to the kept guess.

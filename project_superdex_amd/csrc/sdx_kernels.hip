@@ -428,6 +428,22 @@ __global__ void __launch_bounds__(kVecThreads) k_gather_res(MeshView m, Tiles tv
 
 constexpr int kSetScale = 0, kSetR = 1, kSetD = 3;
 
+// The preconditioned residual z is stored padded to 4 components per vertex
+// so that the matvec gathers a neighbour's z with one vector load.
+template <class T>
+struct alignas(4 * sizeof(T)) Z4 {
+  T v[4];
+};
+template <class T>
+__device__ __forceinline__ void store_z(T* z, int i, T a, T b, T c) {
+  Z4<T> q;
+  q.v[0] = a;
+  q.v[1] = b;
+  q.v[2] = c;
+  q.v[3] = T(0);
+  reinterpret_cast<Z4<T>*>(z)[i] = q;
+}
+
 template <class T>
 __device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int nV, int i, int prec, T r0, T r1, T r2,
                                            T& z0, T& z1, T& z2) {
@@ -526,9 +542,7 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg,
     r[3 * i] = r0;
     r[3 * i + 1] = r1;
     r[3 * i + 2] = r2;
-    z[3 * i] = z0;
-    z[3 * i + 1] = z1;
-    z[3 * i + 2] = z2;
+    store_z(z, i, z0, z1, z2);
     sc[0] = A(b0) * A(b0) + A(b1) * A(b1) + A(b2) * A(b2);
     sc[1] = A(b0) * A(zb0) + A(b1) * A(zb1) + A(b2) * A(zb2);
     sc[2] = A(zb0) * A(zb0) + A(zb1) * A(zb1) + A(zb2) * A(zb2);
@@ -576,27 +590,45 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;
   T zi0 = T(0), zi1 = T(0), zi2 = T(0);
   if (valid) {
-    const int k1 = row_ptr[i + 1];
-    for (int kk = row_ptr[i] + lane; kk < k1; kk += L) {
-      const int jc = col[kk];
-      const T v0 = vals[kk], v1 = vals[(size_t)nB + kk], v2 = vals[2 * (size_t)nB + kk];
-      const T v3 = vals[3 * (size_t)nB + kk], v4 = vals[4 * (size_t)nB + kk], v5 = vals[5 * (size_t)nB + kk];
-      const T v6 = vals[6 * (size_t)nB + kk], v7 = vals[7 * (size_t)nB + kk], v8 = vals[8 * (size_t)nB + kk];
-      const T zj0 = z[3 * jc], zj1 = z[3 * jc + 1], zj2 = z[3 * jc + 2];
-      a0 += v0 * zj0 + v1 * zj1 + v2 * zj2;
-      a1 += v3 * zj0 + v4 * zj1 + v5 * zj2;
-      a2 += v6 * zj0 + v7 * zj1 + v8 * zj2;
-      if (kRefresh) {
-        const double x0 = x[3 * jc], x1 = x[3 * jc + 1], x2 = x[3 * jc + 2];
-        q0 += double(v0) * x0 + double(v1) * x1 + double(v2) * x2;
-        q1 += double(v3) * x0 + double(v4) * x1 + double(v5) * x2;
-        q2 += double(v6) * x0 + double(v7) * x1 + double(v8) * x2;
+    const int k0 = row_ptr[i], k1 = row_ptr[i + 1];
+    // a lane's blocks of a chunk of U L blocks are loaded together: indices
+    // past the row end are clamped to its last block (always valid, a row
+    // holds its diagonal) and weighted by zero, so no load waits on a branch
+    constexpr int U = kSpmvChunk / L;
+    for (int kb = k0 + lane; kb < k1; kb += U * L) {
+      int kk[U], jc[U];
+      T wv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kr = kb + u * L;
+        kk[u] = kr < k1 ? kr : k1 - 1;
+        wv[u] = kr < k1 ? T(1) : T(0);
+        jc[u] = col[kk[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int q = kk[u];
+        const T v0 = vals[q] * wv[u], v1 = vals[(size_t)nB + q] * wv[u], v2 = vals[2 * (size_t)nB + q] * wv[u];
+        const T v3 = vals[3 * (size_t)nB + q] * wv[u], v4 = vals[4 * (size_t)nB + q] * wv[u];
+        const T v5 = vals[5 * (size_t)nB + q] * wv[u], v6 = vals[6 * (size_t)nB + q] * wv[u];
+        const T v7 = vals[7 * (size_t)nB + q] * wv[u], v8 = vals[8 * (size_t)nB + q] * wv[u];
+        const Z4<T> zj = reinterpret_cast<const Z4<T>*>(z)[jc[u]];  // one 16-byte (FP64: 32-byte) load
+        const T zj0 = zj.v[0], zj1 = zj.v[1], zj2 = zj.v[2];
+        a0 += v0 * zj0 + v1 * zj1 + v2 * zj2;
+        a1 += v3 * zj0 + v4 * zj1 + v5 * zj2;
+        a2 += v6 * zj0 + v7 * zj1 + v8 * zj2;
+        if (kRefresh) {
+          const double x0 = x[3 * jc[u]], x1 = x[3 * jc[u] + 1], x2 = x[3 * jc[u] + 2];
+          q0 += double(v0) * x0 + double(v1) * x1 + double(v2) * x2;
+          q1 += double(v3) * x0 + double(v4) * x1 + double(v5) * x2;
+          q2 += double(v6) * x0 + double(v7) * x1 + double(v8) * x2;
+        }
       }
     }
     if (lane == 0) {
-      zi0 = z[3 * i];
-      zi1 = z[3 * i + 1];
-      zi2 = z[3 * i + 2];
+      zi0 = z[4 * i];
+      zi1 = z[4 * i + 1];
+      zi2 = z[4 * i + 2];
     }
   }
   if (status != kCgActive) return;
@@ -678,9 +710,9 @@ __device__ __forceinline__ void cg_update_tile(int tile, const Tiles& tv, const 
     x0 = x[3 * i];
     x1 = x[3 * i + 1];
     x2 = x[3 * i + 2];
-    z0 = z[3 * i];
-    z1 = z[3 * i + 1];
-    z2 = z[3 * i + 2];
+    z0 = z[4 * i];
+    z1 = z[4 * i + 1];
+    z2 = z[4 * i + 2];
     w0 = w[3 * i];
     w1 = w[3 * i + 1];
     w2 = w[3 * i + 2];
@@ -772,9 +804,7 @@ __device__ __forceinline__ void cg_update_tile(int tile, const Tiles& tv, const 
     r[3 * i + 2] = r2;
     T n0, n1, n2;
     apply_dinv(dinv, nV, i, cfg.prec, r0, r1, r2, n0, n1, n2);
-    z[3 * i] = n0;
-    z[3 * i + 1] = n1;
-    z[3 * i + 2] = n2;
+    store_z(z, i, n0, n1, n2);
     if (kRefresh) {
       // the mu recurrence of the next iteration needs z_{k+1}.(r_k this update
       // started from) = r_{k+1}.M^-1 rref: replace z_k by M^-1 rref

@@ -390,7 +390,7 @@ void World<T>::finalize() {
   xcg_.alloc(n3);
   sol0_.alloc(n3);
   r_.alloc(n3);
-  z_.alloc(n3);
+  z_.alloc(size_t(4) * size_t(nV_));  // padded to 4 components per vertex
   p_.alloc(n3);
   sv_.alloc(n3);
   w_.alloc(n3);
