@@ -91,18 +91,15 @@ struct MeshView {
   const int32_t* col;       // [nB]
   const int32_t* slot_row;  // [nB]
   const int32_t* diag_slot; // [nV]
-  int32_t n_items;          // diagonal slots, then upper slots
-  const int32_t* item_a;    // [4 n_items] slot, mirror slot, first and end element record
-  const int32_t* item_b;    // [4 n_items] row, column, first and end contact contribution
-  const int32_t* item_env;  // [n_items] environment
-  const int32_t* eptr;      // [nB+1] off-diagonal element blocks of each upper slot (pair-sorted)
-  const int32_t* pos_v;     // [4 nT] vertex-sorted position of contribution (tet, a)
-  const uint32_t* pos_e;    // [6 nT] pair-sorted position of pair k of a tet; top bit: transposed
-  const int32_t* slot_ptr_c;   // [nB+1] contact contributions per slot
-  const uint32_t* slot_code_c; // tri*16 + a*4 + b
-  const int32_t* vtx_ptr;   // [nV+1] element contributions (tet, a) of each vertex
-  const int32_t* vtx_ptr_c; // [nV+1]
-  const uint32_t* vtx_code_c; // tri*4 + a
+  int32_t n_items;          // diagonal slots (one per vertex, first), then upper slots
+  const int32_t* item_a;    // [4 n_items] slot, mirror slot, first and end record
+  const int32_t* item_b;    // [2 n_items] row | pin bits << kItemRowBits, environment
+  const int32_t* eptr;      // [nB+1] off-diagonal records of each upper slot (pair-sorted)
+  const int32_t* pos_v;     // [4 nT] vertex-sorted record of element contribution (tet, a)
+  const uint32_t* pos_e;    // [6 nT] pair-sorted record of pair k of a tet; top bit: transposed
+  const int32_t* pos_vc;    // [3 nF] vertex-sorted record of contact contribution (triangle, a)
+  const int32_t* pos_ec;    // [3 nF] pair-sorted record of pair k of a boundary triangle
+  const int32_t* vtx_ptr;   // [nV+1] records (element and contact) of each vertex
   const uint8_t* pinned;    // [3 nV]
   const int32_t* coll_first; // [nEnv+1] into coll
   const double* coll;        // 8 doubles per collider: kind, p(3), n(3), r
@@ -121,25 +118,32 @@ struct FieldPtrs {
 };
 
 // ---- assembly ----
-// Element residuals and Hessians written straight to their destination
-// order: residual and diagonal block of (tet, a) at pos_v[4 tet + a] of Rv
-// ([.][3]) and Dv ([.][6]: 00 11 22 01 02 12), off-diagonal block k at
-// pos_e[6 tet + k] of Ev ([.][9], row-major for the (min, max) vertex pair).
+// Element and contact residuals and Hessians are written straight to their
+// destination order (records): the residual and diagonal block of (tet, a) at
+// pos_v[4 tet + a] of Rv ([.][3]) and Dv ([.][6]: 00 11 22 01 02 12), its
+// off-diagonal block k at pos_e[6 tet + k] of Ev ([.][9], row-major for the
+// (min, max) vertex pair); those of boundary triangle f at pos_vc[3 f + a] and
+// pos_ec[3 f + k].  Every vertex / upper slot owns one contiguous run of
+// records.  Record counts: 4 nT + 3 nF (Rv, Dv), 6 nT + 3 nF (Ev); each array
+// is padded by one 16-byte vector for the gathers' vector loads.
+constexpr int kItemRowBits = 26;  // rows of the gather items below 2^26
+constexpr int kTriPairA[3] = {0, 0, 1};
+constexpr int kTriPairB[3] = {1, 2, 2};
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0,
                     const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv, T* Ev,
                     int block = 128, const KernelEvents* ev = nullptr);
 template <class T>
-void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, int wantH, T* Rc,
-                    T* Hc);
-// BSR values (and the preconditioner blocks) from the element and contact
-// contributions, over the work items of the mesh (diagonal and upper slots).
+void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, int wantH, T* Rv, T* Dv,
+                    T* Ev);
+// BSR values (and the preconditioner blocks) from the records, over the work
+// items of the mesh (diagonal and upper slots).
 template <class T>
-void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* Dv, const T* Ev, const T* Hc, int prec, T* vals,
-                       T* dinv, const KernelEvents* ev = nullptr);
+void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* Dv, const T* Ev, int prec, T* vals, T* dinv,
+                       const KernelEvents* ev = nullptr);
 // acc: set 0 of the accumulators (zeroed by the caller); per-env ||res||^2.
 template <class T, class A>
-void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Rv, const T* Rc, T* res, A* acc);
+void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Rv, T* res, A* acc);
 template <class A>
 void launch_fill(hipStream_t s, int64_t n, A v, A* d);
 // out[e * nv + k] = sum over slots of set 0, k < nv.

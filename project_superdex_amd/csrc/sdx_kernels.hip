@@ -134,17 +134,19 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, FieldPtrs<T> f, con
 
 // ------------------------------------------------------------------ contact
 
+// Symmetric 3x3 block (a, b), a <= b, of a triangle's 6 in hc[36].
 __host__ __device__ constexpr int hc_block(int a, int b) { return a == 0 ? b : (a == 1 ? 2 + b : 5); }
-constexpr int kHcStride = 36;
-constexpr int kRcStride = 9;
+// Entry (r, c) of a symmetric 3x3 block stored as 00 11 22 01 02 12.
+__device__ __forceinline__ int sym_entry(int r, int c) { return r == c ? r : 2 + r + c; }
 
 // Penalty contact of the 3 trace-Q3 samples of a boundary triangle against
 // the analytic colliders of its environment (contact_utils.h:498-570,
-// fem_traction.h:58-170).  Outputs per triangle: 9 residual entries and 6
-// symmetric 3x3 blocks.
+// fem_traction.h:58-170).  Outputs per triangle, written to their records
+// (zeros when nothing touches): 3 residual rows, 3 diagonal blocks and the 3
+// blocks of its vertex pairs (symmetric, so no transposition).
 template <class T>
 __global__ void __launch_bounds__(256) k_contact(MeshView m, FieldPtrs<T> f, const T* __restrict__ u, int wantH,
-                                                 T* __restrict__ Rc, T* __restrict__ Hc) {
+                                                 T* __restrict__ Rv, T* __restrict__ Dv, T* __restrict__ Ev) {
   const int tr = blockIdx.x * blockDim.x + threadIdx.x;
   if (tr >= m.nF) return;
   const int env = m.tri_env[tr];
@@ -221,13 +223,30 @@ __global__ void __launch_bounds__(256) k_contact(MeshView m, FieldPtrs<T> f, con
       }
     }
   }
-  T* ro = Rc + (size_t)tr * kRcStride;
+  int pv[3];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) ro[k] = rc[k];
-  if (wantH) {
-    T* ho = Hc + (size_t)tr * kHcStride;
+  for (int a = 0; a < 3; ++a) {
+    pv[a] = m.pos_vc[3 * tr + a];
+    T* re = Rv + (size_t)pv[a] * 3;
+    re[0] = rc[3 * a];
+    re[1] = rc[3 * a + 1];
+    re[2] = rc[3 * a + 2];
+  }
+  if (!wantH) return;
 #pragma unroll
-    for (int k = 0; k < 36; ++k) ho[k] = hc[k];
+  for (int a = 0; a < 3; ++a) {
+    T* d = Dv + (size_t)pv[a] * 6;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d[k] = hc[6 * hc_block(a, a) + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const T* q = hc + 6 * hc_block(kTriPairA[k], kTriPairB[k]);
+    T* d = Ev + (size_t)m.pos_ec[3 * tr + k] * 9;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) d[3 * r + c] = q[sym_entry(r, c)];
   }
 }
 
@@ -253,91 +272,183 @@ __device__ __forceinline__ void inv3(const T a[9], T o[9]) {
 // Every BSR slot: sum its element and contact contributions, apply Dirichlet
 // rows/columns (mochi_solve.cpp:337-396), write the 9 SoA planes, and on
 // diagonal slots the preconditioner block.
-// Entry (r, c) of a symmetric 3x3 block stored as 00 11 22 01 02 12.
-__device__ __forceinline__ int sym_entry(int r, int c) { return r == c ? r : 2 + r + c; }
+// 16-byte vector of T for staging copies: a native vector type, so an
+// unrolled array of them stays in registers (HIP's float4 struct does not).
+typedef float sdx_f4 __attribute__((ext_vector_type(4)));
+typedef double sdx_d2 __attribute__((ext_vector_type(2)));
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = sdx_f4;
+};
+template <>
+struct Vec16<double> {
+  using type = sdx_d2;
+};
+
+// LDS staging chunk of the gathers: 16 KB, four 16-byte loads per thread of
+// a 256-thread workgroup.
+template <class T>
+constexpr int kStage = 16384 / int(sizeof(T));
+
+// The workgroup copies src[c0, c1) (at most CH values) into lds with 16-byte
+// loads, all of a thread's loads issued before its first LDS store so the
+// copy waits on memory once; c0 is a multiple of the vector width and src is
+// padded by one vector, so the last vector may run past c1 but never past the
+// allocation.
+template <class T, int NT, int CH>
+__device__ __forceinline__ void stage_chunk(const T* __restrict__ src, int64_t c0, int64_t c1, T* lds) {
+  using V = typename Vec16<T>::type;
+  constexpr int kW = 16 / int(sizeof(T));
+  constexpr int kU = CH / (kW * NT);  // vectors per thread
+  static_assert(kU * kW * NT == CH, "chunk must be a whole number of vectors per thread");
+  const V* s = reinterpret_cast<const V*>(src + c0);
+  V* d = reinterpret_cast<V*>(lds);
+  const int nv = int((c1 - c0 + kW - 1) / kW);
+  V v[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int k = int(threadIdx.x) + u * NT;
+    if (k < nv) v[u] = s[k];
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int k = int(threadIdx.x) + u * NT;
+    if (k < nv) d[k] = v[u];
+  }
+}
+
+// Records k of [k0, k1) whose component e (of W per record) lies in the
+// chunk [c0, c1): sum of lds[k W + e - c0].
+template <class T, int W>
+__device__ __forceinline__ T sum_run(const T* lds, int k0, int k1, int e, int64_t c0, int64_t c1) {
+  const int64_t lo = c0 > e ? (c0 - e + W - 1) / W : 0;
+  const int64_t hi = c1 > e ? (c1 - e + W - 1) / W : 0;
+  const int a = int(lo > k0 ? lo : int64_t(k0)), b = int(hi < k1 ? hi : int64_t(k1));
+  const T* p = lds + (int64_t(a) * W + e - c0);
+  T s0 = T(0), s1 = T(0);
+  int k = a;
+  for (; k + 1 < b; k += 2, p += 2 * W) {
+    s0 += p[0];
+    s1 += p[W];
+  }
+  if (k < b) s0 += p[0];
+  return s0 + s1;
+}
+
+// BSR gather workgroups: four waves of seven 9-lane items per round; the
+// records of a workgroup's items are staged in chunks of 16 KB
+// (four 16-byte loads per thread).  Measured on 512 batched cubes: one round
+// of 28 items per workgroup is as fast as workgroups of 1 or 2 waves; 2 or 4
+// rounds per workgroup, or eight loads per thread, are 1.2x to 4x slower.
+// Chunks are kStage<T> values.
+constexpr int kGatherWaves = 4;
+constexpr int kGatherRounds = 1;
+constexpr int kGatherRoundItems = 7 * kGatherWaves;
+constexpr int kGatherItems = kGatherRoundItems * kGatherRounds;
+
+template <class T, int W>
+__device__ __forceinline__ void gather_rounds(const T* __restrict__ src, int rec0, int rec1,
+                                              const int4 (&ia)[kGatherRounds], const bool (&on)[kGatherRounds],
+                                              int e, T (&acc)[kGatherRounds], T* lds) {
+  constexpr int kW = 16 / int(sizeof(T));
+  const int64_t end = int64_t(rec1) * W;
+#pragma unroll
+  for (int t = 0; t < kGatherRounds; ++t) acc[t] = T(0);
+  for (int64_t c0 = (int64_t(rec0) * W) & ~int64_t(kW - 1); c0 < end; c0 += kStage<T>) {
+    const int64_t c1 = c0 + kStage<T> < end ? c0 + kStage<T> : end;
+    stage_chunk<T, 64 * kGatherWaves, kStage<T>>(src, c0, c1, lds);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kGatherRounds; ++t)
+      if (on[t]) acc[t] += sum_run<T, W>(lds, ia[t].z, ia[t].w, e, c0, c1);
+    __syncthreads();
+  }
+}
 
 template <class T>
-__global__ void __launch_bounds__(256) k_gather_bsr(MeshView m, const T* __restrict__ Dv, const T* __restrict__ Ev,
-                                                    const T* __restrict__ Hc, int prec, T* __restrict__ vals,
-                                                    T* __restrict__ dinv) {
+__global__ void __launch_bounds__(64 * kGatherWaves) k_gather_bsr(MeshView m, const T* __restrict__ Dv,
+                                                                  const T* __restrict__ Ev, int prec,
+                                                                  int n_diag_blocks, T* __restrict__ vals,
+                                                                  T* __restrict__ dinv) {
   // Nine lanes per work item, lane q owning entry q = 3 r + c of the 3x3
-  // block, so the item's lanes read every element record as one contiguous
-  // run.  Items: the diagonal slots (first) and the upper slots (i < j); an
-  // upper slot also writes its mirror (j, i) transposed, so each pair's
-  // records are read once.  Seven items per wave (lane 63 idles).  An item's
-  // description is two int4 (slot, mirror, record range; row, column,
-  // contact range) plus its environment, so no load waits on another.
-  const int wave = xcd_tile(blockIdx.x, gridDim.x) * (256 / 64) + int(threadIdx.x) / 64;
+  // block.  Items: the diagonal slots (the first nV items, one per vertex)
+  // and the upper slots (i < j); an upper slot also writes its mirror (j, i)
+  // transposed, so each pair's records are read once.  The records of
+  // consecutive items are contiguous (vertex-sorted diagonal records,
+  // pair-sorted off-diagonal records, element and contact alike), so a
+  // workgroup stages the records of its items through LDS with 16-byte
+  // loads and the lanes sum from LDS.  Workgroups [0, n_diag_blocks) take
+  // diagonal items, the rest upper items.  The descriptions of all rounds
+  // (slot, mirror, record range; row with the Dirichlet flags of its row and
+  // column DoFs, environment) are loaded with the workgroup's record range
+  // before anything waits, so a workgroup waits on memory twice: for the
+  // descriptions, then for the records and the environment masks.
+  __shared__ typename Vec16<T>::type lds_v[kStage<T> * sizeof(T) / 16];
+  T* lds = reinterpret_cast<T*>(lds_v);
+  const int blk = xcd_tile(blockIdx.x, gridDim.x);
+  const bool diag = blk < n_diag_blocks;
+  const int first = diag ? blk * kGatherItems : m.nV + (blk - n_diag_blocks) * kGatherItems;
+  const int last = min(first + kGatherItems, diag ? m.nV : m.n_items);  // exclusive
+  const int rec0 = m.item_a[4 * first + 2], rec1 = m.item_a[4 * (last - 1) + 3];
   const int lane = int(threadIdx.x) % 64;
   const int g = lane / 9, q = lane % 9;
-  const int item = wave * 7 + g;
-  if (g >= 7 || item >= m.n_items) return;
-  const int4 ia = reinterpret_cast<const int4*>(m.item_a)[item];  // slot, mirror, first, end record
-  const int4 ib = reinterpret_cast<const int4*>(m.item_b)[item];  // row, column, first, end contact
-  if (!m.env_mask[m.item_env[item]]) return;
-  const int s = ia.x, i = ib.x, j = ib.y;
+  const int item0 = first + (int(threadIdx.x) / 64) * 7 + g;
+  int4 ia[kGatherRounds];
+  int2 ib[kGatherRounds];
+  bool on[kGatherRounds];
+#pragma unroll
+  for (int t = 0; t < kGatherRounds; ++t) {
+    const int item = item0 + t * kGatherRoundItems;
+    const bool mine = g < 7 && item < last;
+    ia[t] = mine ? reinterpret_cast<const int4*>(m.item_a)[item] : make_int4(0, 0, 0, 0);
+    ib[t] = mine ? reinterpret_cast<const int2*>(m.item_b)[item] : make_int2(0, 0);
+    on[t] = mine;
+  }
+#pragma unroll
+  for (int t = 0; t < kGatherRounds; ++t) on[t] = on[t] && m.env_mask[ib[t].y] != 0;
   const int r = q / 3, c = q % 3;
-  const int e = sym_entry(r, c);
-  T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
-  if (i == j) {
-    int k = ia.z;
-    for (; k + 3 < ia.w; k += 4) {
-      a0 += Dv[(size_t)k * 6 + e];
-      a1 += Dv[(size_t)(k + 1) * 6 + e];
-      a2 += Dv[(size_t)(k + 2) * 6 + e];
-      a3 += Dv[(size_t)(k + 3) * 6 + e];
-    }
-    for (; k < ia.w; ++k) a0 += Dv[(size_t)k * 6 + e];
-  } else {
-    int k = ia.z;
-    for (; k + 1 < ia.w; k += 2) {
-      a0 += Ev[(size_t)k * 9 + q];
-      a1 += Ev[(size_t)(k + 1) * 9 + q];
-    }
-    if (k < ia.w) a2 += Ev[(size_t)k * 9 + q];
-  }
-  T acc = (a0 + a1) + (a2 + a3);
-  // contact blocks are symmetric and the same triangles serve (i, j) and (j, i)
-  for (int k = ib.z; k < ib.w; ++k) {
-    const uint32_t code = m.slot_code_c[k];
-    int a = (code >> 2) & 3, bb = code & 3;
-    if (a > bb) {
-      const int t = a;
-      a = bb;
-      bb = t;
-    }
-    acc += Hc[(size_t)(code >> 4) * kHcStride + 6 * hc_block(a, bb) + e];
-  }
-  // Dirichlet rows / columns (mochi_solve.cpp:337-396); the test is symmetric in (i r, j c)
-  const uint8_t* pin = m.pinned;
-  if (pin[3 * i + r] | pin[3 * j + c]) acc = (i == j && r == c) ? T(1) : T(0);
+  T acc[kGatherRounds];
+  if (diag)
+    gather_rounds<T, 6>(Dv, rec0, rec1, ia, on, sym_entry(r, c), acc, lds);
+  else
+    gather_rounds<T, 9>(Ev, rec0, rec1, ia, on, q, acc, lds);
   const int nB = m.nB;
-  vals[(size_t)q * nB + s] = acc;
-  if (i != j) {
-    vals[(size_t)(3 * c + r) * nB + ia.y] = acc;
-    return;
-  }
-  // diagonal block: every lane of the item gathers the 9 entries and writes
-  // its entry of the preconditioner block
-  const int base = g * 9;
-  T blk[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) blk[k] = __shfl(acc, base + k, 64);
-  T o[9];
-  if (prec == 2) {
-    inv3(blk, o);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) o[k] = T(0);
-    if (prec == 1) {
-      o[0] = T(1) / blk[0];
-      o[4] = T(1) / blk[4];
-      o[8] = T(1) / blk[8];
-    } else {
-      o[0] = o[4] = o[8] = T(1);
+  for (int t = 0; t < kGatherRounds; ++t) {
+    if (!on[t]) continue;
+    T v = acc[t];
+    // Dirichlet rows / columns (mochi_solve.cpp:337-396); the test is symmetric in (i r, j c)
+    const int pins = ib[t].x >> kItemRowBits;
+    if (((pins >> r) | (pins >> (3 + c))) & 1) v = (diag && r == c) ? T(1) : T(0);
+    vals[(size_t)q * nB + ia[t].x] = v;
+    if (!diag) {
+      vals[(size_t)(3 * c + r) * nB + ia[t].y] = v;
+      continue;
     }
+    // diagonal block: every lane of the item gathers the 9 entries and
+    // writes its entry of the preconditioner block
+    const int base = g * 9;
+    T blk9[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) blk9[k] = __shfl(v, base + k, 64);
+    T o[9];
+    if (prec == 2) {
+      inv3(blk9, o);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) o[k] = T(0);
+      if (prec == 1) {
+        o[0] = T(1) / blk9[0];
+        o[4] = T(1) / blk9[4];
+        o[8] = T(1) / blk9[8];
+      } else {
+        o[0] = o[4] = o[8] = T(1);
+      }
+    }
+    dinv[(size_t)q * m.nV + (ib[t].x & ((1 << kItemRowBits) - 1))] = o[q];
   }
-  dinv[(size_t)q * m.nV + i] = o[q];
 }
 
 template <class T>
@@ -367,32 +478,39 @@ __global__ void __launch_bounds__(256) k_block_inverse(int nV, const int32_t* __
   for (int k = 0; k < 9; ++k) dinv[(size_t)k * nV + i] = o[k];
 }
 
-// One thread per vertex: residual = sum of element and contact
-// contributions, Dirichlet rows zeroed; per-environment ||res||^2 added into
-// acc (set 0, zeroed by the caller).
+// One thread per vertex: residual = sum of its element and contact records,
+// Dirichlet rows zeroed; per-environment ||res||^2 added into acc (set 0,
+// zeroed by the caller).  The residual records of a tile's vertices are
+// contiguous (vertex-sorted), so the workgroup stages them through LDS with
+// 16-byte loads and each thread sums its run from LDS.
 template <class T, class A>
 __global__ void __launch_bounds__(kVecThreads) k_gather_res(MeshView m, Tiles tv, const T* __restrict__ Rv,
-                                                            const T* __restrict__ Rc, T* __restrict__ res, A* acc) {
+                                                            T* __restrict__ res, A* acc) {
+  __shared__ typename Vec16<T>::type lds_v[kStage<T> * sizeof(T) / 16];
+  T* lds = reinterpret_cast<T*>(lds_v);
+  constexpr int kW = 16 / int(sizeof(T));
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int env = tv.env[tile];
-  if (!m.env_mask[env]) return;
-  const int i = tv.row0[tile] + threadIdx.x;
+  if (!m.env_mask[env]) return;  // uniform over the tile
+  const int row0 = tv.row0[tile], row1 = tv.row1[tile];
+  const int i = row0 + int(threadIdx.x);
+  const bool mine = i < row1;
+  const int k0 = mine ? m.vtx_ptr[i] : 0, k1 = mine ? m.vtx_ptr[i + 1] : 0;
+  const int64_t end = int64_t(m.vtx_ptr[row1]) * 3;
+  T r0 = T(0), r1 = T(0), r2 = T(0);
+  for (int64_t c0 = (int64_t(m.vtx_ptr[row0]) * 3) & ~int64_t(kW - 1); c0 < end; c0 += kStage<T>) {
+    const int64_t c1 = c0 + kStage<T> < end ? c0 + kStage<T> : end;
+    stage_chunk<T, kVecThreads, kStage<T>>(Rv, c0, c1, lds);
+    __syncthreads();
+    if (mine) {
+      r0 += sum_run<T, 3>(lds, k0, k1, 0, c0, c1);
+      r1 += sum_run<T, 3>(lds, k0, k1, 1, c0, c1);
+      r2 += sum_run<T, 3>(lds, k0, k1, 2, c0, c1);
+    }
+    __syncthreads();
+  }
   A part[1] = {A(0)};
-  if (i < tv.row1[tile]) {
-    T r0 = T(0), r1 = T(0), r2 = T(0);
-    for (int c = m.vtx_ptr[i]; c < m.vtx_ptr[i + 1]; ++c) {  // contiguous run
-      const T* q = Rv + (size_t)c * 3;
-      r0 += q[0];
-      r1 += q[1];
-      r2 += q[2];
-    }
-    for (int c = m.vtx_ptr_c[i]; c < m.vtx_ptr_c[i + 1]; ++c) {
-      const uint32_t code = m.vtx_code_c[c];
-      const T* q = Rc + (size_t)(code >> 2) * kRcStride + 3 * (code & 3);
-      r0 += q[0];
-      r1 += q[1];
-      r2 += q[2];
-    }
+  if (mine) {
     if (m.pinned[3 * i]) r0 = T(0);
     if (m.pinned[3 * i + 1]) r1 = T(0);
     if (m.pinned[3 * i + 2]) r2 = T(0);
@@ -1103,23 +1221,25 @@ void launch_element(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, con
 }
 
 template <class T>
-void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, int wantH, T* Rc, T* Hc) {
+void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, int wantH, T* Rv, T* Dv,
+                    T* Ev) {
   if (m.nF == 0) return;
-  hipLaunchKernelGGL(k_contact<T>, dim3(blocks_for(m.nF, 256)), dim3(256), 0, s, m, f, u, wantH, Rc, Hc);
+  hipLaunchKernelGGL(k_contact<T>, dim3(blocks_for(m.nF, 256)), dim3(256), 0, s, m, f, u, wantH, Rv, Dv, Ev);
 }
 
 template <class T>
-void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* Dv, const T* Ev, const T* Hc, int prec, T* vals,
-                       T* dinv, const KernelEvents* ev) {
+void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* Dv, const T* Ev, int prec, T* vals, T* dinv,
+                       const KernelEvents* ev) {
   if (m.n_items == 0) return;
-  const int64_t waves = (int64_t(m.n_items) + 6) / 7;
-  SDX_LAUNCH(ev, k_gather_bsr<T>, dim3(unsigned((waves + 3) / 4)), dim3(256), s, m, Dv, Ev, Hc, prec, vals, dinv);
+  const int nd = (m.nV + kGatherItems - 1) / kGatherItems;
+  const int nu = (m.n_items - m.nV + kGatherItems - 1) / kGatherItems;
+  SDX_LAUNCH(ev, k_gather_bsr<T>, dim3(unsigned(nd + nu)), dim3(64 * kGatherWaves), s, m, Dv, Ev, prec, nd, vals, dinv);
 }
 
 template <class T, class A>
-void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Rv, const T* Rc, T* res, A* acc) {
+void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Rv, T* res, A* acc) {
   if (tv.n == 0) return;
-  hipLaunchKernelGGL((k_gather_res<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, m, tv, Rv, Rc, res, acc);
+  hipLaunchKernelGGL((k_gather_res<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, m, tv, Rv, res, acc);
 }
 
 template <class A>
@@ -1279,8 +1399,8 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
   template void launch_element<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, const T*,        \
                                   const T*, double, const double*, int, int, T*, T*, T*, int,                   \
                                   const KernelEvents*);                                                         \
-  template void launch_contact<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, int, T*, T*);    \
-  template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const T*, const T*, const T*, int, T*, T*,   \
+  template void launch_contact<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, int, T*, T*, T*); \
+  template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const T*, const T*, int, T*, T*,             \
                                      const KernelEvents*);                                                      \
   template void launch_spmv_plain<T>(hipStream_t, const Tiles&, int, const int32_t*, const int32_t*, const T*,  \
                                      int32_t, const T*, T*);                                                    \
@@ -1298,7 +1418,7 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
   template void launch_cg_reset<T>(hipStream_t, int, const int32_t*, EnvCG*, T*);
 
 #define SDX_INST_TA(T, A)                                                                                         \
-  template void launch_gather_res<T, A>(hipStream_t, const MeshView&, const Tiles&, const T*, const T*, T*, A*); \
+  template void launch_gather_res<T, A>(hipStream_t, const MeshView&, const Tiles&, const T*, T*, A*);          \
   template void launch_cg_init<T, A>(hipStream_t, const Tiles&, const CgConfig&, const T*, const int32_t*,       \
                                      const double*, const double*, const T*, int32_t, A*, T*, T*, EnvCG*, A*);   \
   template void launch_residual64<T, A>(hipStream_t, const Tiles&, int, int, const int32_t*, const int32_t*,    \

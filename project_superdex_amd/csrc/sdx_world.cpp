@@ -219,8 +219,6 @@ void World<T>::finalize() {
   std::vector<uint32_t> edge_codes;
   std::vector<int32_t> vtx_keys;
   std::vector<uint32_t> vtx_codes;
-  std::vector<int32_t> slot_keys_c, vtx_keys_c;
-  std::vector<uint32_t> slot_codes_c, vtx_codes_c;
   std::vector<double> contact(size_t(4) * size_t(nEnv));
   for (int e = 0; e < nEnv; ++e) {
     auto& A = actors_[size_t(e)];
@@ -294,14 +292,17 @@ void World<T>::finalize() {
     for (size_t f = 0; f < atris.size() / 3; ++f) {
       const int gf = f0 + int(f);
       tri_env.push_back(e);
+      // contact records join the element records of the same vertex / pair
+      // (flag bits 31 / 30 mark contact codes)
       for (int a = 0; a < 3; ++a) {
         tris.push_back(v0 + atris[3 * f + size_t(a)]);
-        vtx_keys_c.push_back(v0 + atris[3 * f + size_t(a)]);
-        vtx_codes_c.push_back(uint32_t(gf) * 4u + uint32_t(a));
-        for (int b = 0; b < 3; ++b) {
-          slot_keys_c.push_back(b0 + FindSlot(P, atris[3 * f + size_t(a)], atris[3 * f + size_t(b)]));
-          slot_codes_c.push_back(uint32_t(gf) * 16u + uint32_t(a) * 4u + uint32_t(b));
-        }
+        vtx_keys.push_back(v0 + atris[3 * f + size_t(a)]);
+        vtx_codes.push_back((uint32_t(gf) * 3u + uint32_t(a)) | 0x80000000u);
+      }
+      for (int k = 0; k < 3; ++k) {
+        const int va = atris[3 * f + size_t(kTriPairA[k])], vb = atris[3 * f + size_t(kTriPairB[k])];
+        edge_keys.push_back(b0 + FindSlot(P, std::min(va, vb), std::max(va, vb)));
+        edge_codes.push_back((uint32_t(gf) * 3u + uint32_t(k)) | 0x40000000u);
       }
     }
     contact[size_t(4 * e + 0)] = A.params.penalty_coefficient;
@@ -313,27 +314,40 @@ void World<T>::finalize() {
   nF_ = int(tri_env.size());
   h_row_ptr_ = row_ptr;
   h_col_ = col;
-  if (int64_t(nT_) * 6 >= (int64_t(1) << 31) || int64_t(nF_) * 16 >= (int64_t(1) << 32))
-    throw std::invalid_argument("mesh too large for 32-bit contribution codes");
-  std::vector<int32_t> eptr, vtx_ptr, slot_ptr_c, vtx_ptr_c;
-  std::vector<uint32_t> ecode, vtx_code, slot_code_c, vtx_code_c;
+  if (int64_t(nT_) * 6 + int64_t(nF_) * 3 >= (int64_t(1) << 30) || nV_ >= (1 << kItemRowBits))
+    throw std::invalid_argument("mesh too large for 32-bit record codes");
+  std::vector<int32_t> eptr, vtx_ptr;
+  std::vector<uint32_t> ecode, vtx_code;
   CountingCsr(nB_, edge_keys, edge_codes, eptr, ecode);
   CountingCsr(nV_, vtx_keys, vtx_codes, vtx_ptr, vtx_code);
-  CountingCsr(nB_, slot_keys_c, slot_codes_c, slot_ptr_c, slot_code_c);
-  CountingCsr(nV_, vtx_keys_c, vtx_codes_c, vtx_ptr_c, vtx_code_c);
-  // destination of every element contribution: the vertex-sorted position of
-  // (tet, a) for residuals and diagonal blocks, the pair-sorted position of
-  // (tet, pair) for off-diagonal blocks (flag bit: stored transposed)
-  std::vector<int32_t> pos_v(size_t(4) * size_t(nT_));
-  for (size_t q = 0; q < vtx_code.size(); ++q) pos_v[size_t(vtx_code[q])] = int32_t(q);
+  // record of every element and contact contribution: the vertex-sorted
+  // position of (tet, a) / (triangle, a) for residuals and diagonal blocks,
+  // the pair-sorted position of (tet, pair) / (triangle, pair) for
+  // off-diagonal blocks (flag bit 31 of a tet pair: stored transposed)
+  std::vector<int32_t> pos_v(size_t(4) * size_t(nT_)), pos_vc(size_t(3) * size_t(nF_));
+  for (size_t q = 0; q < vtx_code.size(); ++q) {
+    const uint32_t code = vtx_code[q];
+    if (code >> 31)
+      pos_vc[size_t(code & 0x7fffffffu)] = int32_t(q);
+    else
+      pos_v[size_t(code)] = int32_t(q);
+  }
   std::vector<uint32_t> pos_e(size_t(6) * size_t(nT_));
-  for (size_t q = 0; q < ecode.size(); ++q) pos_e[size_t(ecode[q] & 0x7fffffffu)] = uint32_t(q) | (ecode[q] & 0x80000000u);
+  std::vector<int32_t> pos_ec(size_t(3) * size_t(nF_));
+  for (size_t q = 0; q < ecode.size(); ++q) {
+    const uint32_t code = ecode[q];
+    if (code & 0x40000000u)
+      pos_ec[size_t(code & 0x3fffffffu)] = int32_t(q);
+    else
+      pos_e[size_t(code & 0x7fffffffu)] = uint32_t(q) | (code & 0x80000000u);
+  }
   // gather work items: the diagonal slots, then the upper slots (i < j),
-  // each with its mirror (j, i), record range and contact range; lower slots
-  // are written by the item of their upper mirror
-  std::vector<int32_t> item_a, item_b, item_env;
+  // each with its mirror (j, i), record range, row with the Dirichlet flags of
+  // the row and column DoFs, and environment; lower slots are written by the
+  // item of their upper mirror
+  std::vector<int32_t> item_a, item_b;
   item_a.reserve(size_t(4) * (size_t(nV_) + size_t(nB_ - nV_) / 2));
-  item_b.reserve(item_a.capacity());
+  item_b.reserve(item_a.capacity() / 2);
   for (int pass = 0; pass < 2; ++pass)
     for (int i = 0; i < nV_; ++i)
       for (int k = row_ptr[size_t(i)]; k < row_ptr[size_t(i) + 1]; ++k) {
@@ -343,9 +357,11 @@ void World<T>::finalize() {
         const int mirror = int32_t(std::lower_bound(b, e2, i) - col.begin());  // slot (j, i)
         const int r0 = pass == 0 ? vtx_ptr[size_t(i)] : eptr[size_t(k)];
         const int r1 = pass == 0 ? vtx_ptr[size_t(i) + 1] : eptr[size_t(k) + 1];
+        int pins = 0;
+        for (int d = 0; d < 3; ++d)
+          pins |= (pinned[size_t(3 * i + d)] ? 1 : 0) << d | (pinned[size_t(3 * j + d)] ? 1 : 0) << (3 + d);
         item_a.insert(item_a.end(), {k, mirror, r0, r1});
-        item_b.insert(item_b.end(), {i, j, slot_ptr_c[size_t(k)], slot_ptr_c[size_t(k) + 1]});
-        item_env.push_back(row_env[size_t(i)]);
+        item_b.insert(item_b.end(), {i | (pins << kItemRowBits), row_env[size_t(i)]});
       }
   BuildTiles(env_v0_, nEnv, kVecThreads, tiles_v_);
   BuildTiles(env_v0_, nEnv, kSpmvThreads / sched_.spmv_lanes, tiles_a_);
@@ -363,15 +379,12 @@ void World<T>::finalize() {
   d_eptr_.upload(eptr);
   d_item_a_.upload(item_a);
   d_item_b_.upload(item_b);
-  d_item_env_.upload(item_env);
-  n_items_ = int(item_env.size());
+  n_items_ = int(item_b.size() / 2);
   d_pos_v_.upload(pos_v);
   d_pos_e_.upload(pos_e);
-  d_slot_ptr_c_.upload(slot_ptr_c);
-  d_slot_code_c_.upload(slot_code_c);
+  d_pos_vc_.upload(pos_vc);
+  d_pos_ec_.upload(pos_ec);
   d_vtx_ptr_.upload(vtx_ptr);
-  d_vtx_ptr_c_.upload(vtx_ptr_c);
-  d_vtx_code_c_.upload(vtx_code_c);
   d_pinned_.upload(pinned);
   d_pin_value_.upload(pin_value);
   d_contact_.upload(contact);
@@ -395,11 +408,12 @@ void World<T>::finalize() {
   sv_.alloc(n3);
   w_.alloc(n3);
   rref_.alloc(n3);
-  cres_.alloc(size_t(12) * size_t(nT_));
-  cdiag_.alloc(size_t(24) * size_t(nT_));
-  coff_.alloc(size_t(54) * size_t(nT_));
-  Rc_.alloc(size_t(9) * size_t(std::max(nF_, 1)));
-  Hc_.alloc(size_t(36) * size_t(std::max(nF_, 1)));
+  // padded by one 16-byte vector: the gathers stage these with vector loads
+  const size_t nrec_v = size_t(4) * size_t(nT_) + size_t(3) * size_t(nF_);
+  const size_t nrec_e = size_t(6) * size_t(nT_) + size_t(3) * size_t(nF_);
+  cres_.alloc(3 * nrec_v + 4);
+  cdiag_.alloc(6 * nrec_v + 4);
+  coff_.alloc(9 * nrec_e + 4);
   vals_.alloc(size_t(9) * size_t(nB_));
   dinv_.alloc(size_t(9) * size_t(nV_));
   env_cg_.alloc(static_cast<size_t>(nEnv));
@@ -422,8 +436,6 @@ void World<T>::finalize() {
   reset_.alloc(static_cast<size_t>(nEnv));
   env_sq_.alloc(static_cast<size_t>(nEnv));
   norms2_.alloc(size_t(2) * size_t(nEnv));
-  hip_check(hipMemsetAsync(Rc_.p, 0, Rc_.bytes(), stream_), "hipMemsetAsync");
-  hip_check(hipMemsetAsync(Hc_.p, 0, Hc_.bytes(), stream_), "hipMemsetAsync");
   finalized_ = true;
   push_state();
   colliders_dirty_ = true;
@@ -444,15 +456,12 @@ void World<T>::finalize() {
   mesh_.eptr = d_eptr_.p;
   mesh_.item_a = d_item_a_.p;
   mesh_.item_b = d_item_b_.p;
-  mesh_.item_env = d_item_env_.p;
   mesh_.n_items = n_items_;
   mesh_.pos_v = d_pos_v_.p;
   mesh_.pos_e = d_pos_e_.p;
-  mesh_.slot_ptr_c = d_slot_ptr_c_.p;
-  mesh_.slot_code_c = d_slot_code_c_.p;
+  mesh_.pos_vc = d_pos_vc_.p;
+  mesh_.pos_ec = d_pos_ec_.p;
   mesh_.vtx_ptr = d_vtx_ptr_.p;
-  mesh_.vtx_ptr_c = d_vtx_ptr_c_.p;
-  mesh_.vtx_code_c = d_vtx_code_c_.p;
   mesh_.pinned = d_pinned_.p;
   mesh_.env_mask = env_mask_.p;
   fields_.X = d_X_.p;
@@ -518,20 +527,20 @@ void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH) {
   const int project = sp.psd_proj_mode == SDX_PSD_ALWAYS ? 1 : 0;
   launch_element<T>(stream_, mesh_, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, cres_.p,
                     cdiag_.p, coff_.p, sched_.element_block, sampled_events(2));
-  launch_contact<T>(stream_, mesh_, fields_, u_.p, wantH ? 1 : 0, Rc_.p, Hc_.p);
+  launch_contact<T>(stream_, mesh_, fields_, u_.p, wantH ? 1 : 0, cres_.p, cdiag_.p, coff_.p);
   if (wantH) {
-    launch_gather_bsr<T>(stream_, mesh_, cdiag_.p, coff_.p, Hc_.p, sp.linear.preconditioner, vals_.p, dinv_.p,
+    launch_gather_bsr<T>(stream_, mesh_, cdiag_.p, coff_.p, sp.linear.preconditioner, vals_.p, dinv_.p,
                          sampled_events(3));
   }
   const int64_t set0 = int64_t(nEnv_) * kSlots * kSlotStride;
   if (precision_ == SDX_PREC_MODE_FP32) {
     T* acc = reinterpret_cast<T*>(acc_.p);
     launch_fill<T>(stream_, set0, T(0), acc);
-    launch_gather_res<T, T>(stream_, mesh_, dt_v_.view, cres_.p, Rc_.p, res_.p, acc);
+    launch_gather_res<T, T>(stream_, mesh_, dt_v_.view, cres_.p, res_.p, acc);
     launch_acc_finalize<T>(stream_, nEnv_, 1, acc, env_sq_.p);
   } else {
     launch_fill<double>(stream_, set0, 0.0, acc_.p);
-    launch_gather_res<T, double>(stream_, mesh_, dt_v_.view, cres_.p, Rc_.p, res_.p, acc_.p);
+    launch_gather_res<T, double>(stream_, mesh_, dt_v_.view, cres_.p, res_.p, acc_.p);
     launch_acc_finalize<double>(stream_, nEnv_, 1, acc_.p, env_sq_.p);
   }
 }

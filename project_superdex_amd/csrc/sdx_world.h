@@ -325,14 +325,11 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, d_eptr_)
   SDX_BUF(int32_t, d_item_a_)
   SDX_BUF(int32_t, d_item_b_)
-  SDX_BUF(int32_t, d_item_env_)
   SDX_BUF(int32_t, d_pos_v_)
   SDX_BUF(uint32_t, d_pos_e_)
-  SDX_BUF(int32_t, d_slot_ptr_c_)
-  SDX_BUF(uint32_t, d_slot_code_c_)
+  SDX_BUF(int32_t, d_pos_vc_)
+  SDX_BUF(int32_t, d_pos_ec_)
   SDX_BUF(int32_t, d_vtx_ptr_)
-  SDX_BUF(int32_t, d_vtx_ptr_c_)
-  SDX_BUF(uint32_t, d_vtx_code_c_)
   SDX_BUF(uint8_t, d_pinned_)
   SDX_BUF(T, d_pin_value_)
   SDX_BUF(double, d_contact_)
@@ -354,11 +351,9 @@ class World final : public WorldBase {
   SDX_BUF(T, sv_)
   SDX_BUF(T, w_)
   SDX_BUF(double, rref_)
-  SDX_BUF(T, cres_)   // element residuals, vertex-sorted [4 nT][3]
-  SDX_BUF(T, cdiag_)  // element diagonal blocks, vertex-sorted [4 nT][6]
-  SDX_BUF(T, coff_)   // element off-diagonal blocks, pair-sorted [6 nT][9]
-  SDX_BUF(T, Rc_)
-  SDX_BUF(T, Hc_)
+  SDX_BUF(T, cres_)   // element and contact residual records, vertex-sorted [4 nT + 3 nF][3]
+  SDX_BUF(T, cdiag_)  // diagonal-block records, vertex-sorted [4 nT + 3 nF][6]
+  SDX_BUF(T, coff_)   // off-diagonal-block records, pair-sorted [6 nT + 3 nF][9]
   SDX_BUF(T, vals_)
   SDX_BUF(T, dinv_)
   SDX_BUF(EnvCG, env_cg_)
