@@ -31,7 +31,7 @@ struct CgConfig {
   int32_t max_iter;
   int32_t refresh_every;  // 0: never
   int32_t abort_if_not_spd;
-  int32_t pad0;
+  int32_t n_rows;         // block rows of the matrix (size of z)
   int32_t pad1;
   double abs_tol, rel_tol, rel_div_tol;
 };

@@ -21,6 +21,8 @@
 // solved by the same launches.
 #include <hip/hip_ext.h>
 
+#include <stdexcept>
+
 #include "sdx_device.h"
 #include "sdx_kernels.h"
 #include "sdx_math.h"
@@ -552,6 +554,57 @@ template <class T>
 struct alignas(4 * sizeof(T)) Z4 {
   T v[4];
 };
+
+// Raw buffer resources (gfx9 descriptor word 3 = 0x00020000: 32-bit data,
+// no swizzle): a load names its array by a scalar descriptor and its element
+// by a 32-bit byte offset in one register, where a global load needs a
+// 64-bit address per element.  Loads past `bytes` return zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), short(0), int(bytes), 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ float buf_ld<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double buf_ld<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ int buf_ld<int>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return int(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+// The first three components of Z4 entry j.
+__device__ __forceinline__ void buf_ld_z(__amdgpu_buffer_rsrc_t r, int j, float& a, float& b, float& c) {
+  const sdx_f4 q = __builtin_bit_cast(sdx_f4, __builtin_amdgcn_raw_buffer_load_b128(r, uint32_t(j) * 16u, 0, 0));
+  a = q.x;
+  b = q.y;
+  c = q.z;
+}
+__device__ __forceinline__ void buf_ld_z(__amdgpu_buffer_rsrc_t r, int j, double& a, double& b, double& c) {
+  const sdx_d2 q = __builtin_bit_cast(sdx_d2, __builtin_amdgcn_raw_buffer_load_b128(r, uint32_t(j) * 32u, 0, 0));
+  a = q.x;
+  b = q.y;
+  c = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, uint32_t(j) * 32u + 16u, 0, 0));
+}
+
+// Descriptors of the matvec operands: the 9 value planes, the column
+// indices and z (each below 4 GB; checked by the launcher).
+struct SpmvBufs {
+  __amdgpu_buffer_rsrc_t v[9];
+  __amdgpu_buffer_rsrc_t col, z;
+};
+template <class T>
+__device__ __forceinline__ SpmvBufs spmv_bufs(const int32_t* col, const T* vals, int nB, const T* z, int nV) {
+  SpmvBufs b;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) b.v[c] = make_rsrc(vals + (size_t)c * nB, uint32_t(nB) * uint32_t(sizeof(T)));
+  b.col = make_rsrc(col, uint32_t(nB) * 4u);
+  b.z = make_rsrc(z, uint32_t(nV) * uint32_t(4 * sizeof(T)));
+  return b;
+}
 template <class T>
 __device__ __forceinline__ void store_z(T* z, int i, T a, T b, T c) {
   Z4<T> q;
@@ -687,9 +740,9 @@ __device__ __forceinline__ double slot_reduce(double v) {
 // true residual rref = b - A x_k is also formed in FP64 (residual replacement).
 template <class T, class A, bool kRefresh, int L>
 __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const CgConfig& cfg, int k,
-                                             const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                                             const T* __restrict__ vals, int nB, const T* __restrict__ z,
-                                             const A* __restrict__ x, const T* __restrict__ b, T* __restrict__ w,
+                                             const int32_t* __restrict__ row_ptr, const SpmvBufs& bf,
+                                             const T* __restrict__ z, const A* __restrict__ x,
+                                             const T* __restrict__ b, T* __restrict__ w,
                                              double* __restrict__ rref, EnvCG* env, A* acc, int32_t* num_active) {
   const int e = ta.env[tile];
   const int status = env[e].status;  // branched on after the row loads are issued
@@ -721,17 +774,18 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
         const int kr = kb + u * L;
         kk[u] = kr < k1 ? kr : k1 - 1;
         wv[u] = kr < k1 ? T(1) : T(0);
-        jc[u] = col[kk[u]];
+        jc[u] = buf_ld<int>(bf.col, uint32_t(kk[u]) * 4u);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int q = kk[u];
-        const T v0 = vals[q] * wv[u], v1 = vals[(size_t)nB + q] * wv[u], v2 = vals[2 * (size_t)nB + q] * wv[u];
-        const T v3 = vals[3 * (size_t)nB + q] * wv[u], v4 = vals[4 * (size_t)nB + q] * wv[u];
-        const T v5 = vals[5 * (size_t)nB + q] * wv[u], v6 = vals[6 * (size_t)nB + q] * wv[u];
-        const T v7 = vals[7 * (size_t)nB + q] * wv[u], v8 = vals[8 * (size_t)nB + q] * wv[u];
-        const Z4<T> zj = reinterpret_cast<const Z4<T>*>(z)[jc[u]];  // one 16-byte (FP64: 32-byte) load
-        const T zj0 = zj.v[0], zj1 = zj.v[1], zj2 = zj.v[2];
+        const uint32_t o = uint32_t(kk[u]) * uint32_t(sizeof(T));
+        const T v0 = buf_ld<T>(bf.v[0], o) * wv[u], v1 = buf_ld<T>(bf.v[1], o) * wv[u];
+        const T v2 = buf_ld<T>(bf.v[2], o) * wv[u], v3 = buf_ld<T>(bf.v[3], o) * wv[u];
+        const T v4 = buf_ld<T>(bf.v[4], o) * wv[u], v5 = buf_ld<T>(bf.v[5], o) * wv[u];
+        const T v6 = buf_ld<T>(bf.v[6], o) * wv[u], v7 = buf_ld<T>(bf.v[7], o) * wv[u];
+        const T v8 = buf_ld<T>(bf.v[8], o) * wv[u];
+        T zj0, zj1, zj2;
+        buf_ld_z(bf.z, jc[u], zj0, zj1, zj2);  // one 16-byte (FP64: 24 of 32 bytes) gather
         a0 += v0 * zj0 + v1 * zj1 + v2 * zj2;
         a1 += v3 * zj0 + v4 * zj1 + v5 * zj2;
         a2 += v6 * zj0 + v7 * zj1 + v8 * zj2;
@@ -965,9 +1019,9 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(Tiles ta, WorkList wl,
                                                           const T* __restrict__ b, T* __restrict__ w,
                                                           double* __restrict__ rref, EnvCG* env, A* acc,
                                                           int32_t* num_active) {
+  const SpmvBufs bf = spmv_bufs<T>(col, vals, nB, z, cfg.n_rows);
   for_each_tile(wl, wl.list ? 0 : int(gridDim.x), [&](int tile) {
-    cg_spmv_tile<T, A, kRefresh, L>(tile, ta, cfg, k, row_ptr, col, vals, nB, z, x, b, w, rref, env, acc,
-                                    num_active);
+    cg_spmv_tile<T, A, kRefresh, L>(tile, ta, cfg, k, row_ptr, bf, z, x, b, w, rref, env, acc, num_active);
   });
 }
 
@@ -1275,6 +1329,8 @@ void launch_cg_spmv(hipStream_t s, const Tiles& ta, const WorkList& wl, int grid
                     const A* x, const T* b, T* w, double* rref, EnvCG* env, A* acc, int32_t* num_active,
                     const KernelEvents* ev) {
   if (ta.n == 0) return;
+  if (uint64_t(nB) * sizeof(T) >= (uint64_t(1) << 32) || uint64_t(cfg.n_rows) * 4 * sizeof(T) >= (uint64_t(1) << 32))
+    throw std::invalid_argument("matrix too large for 32-bit buffer offsets of the CG matvec");
   const int g = wl.list ? grid : ta.n;
   const bool refresh = cg_refresh(cfg, k);
 #define SDX_SPMV(R, L)                                                                                           \

@@ -116,6 +116,7 @@ sdx_status solve(int32_t device, int32_t nbr, int32_t nnzb, const int32_t* ptr, 
   Tiles ta{dae.p, da0.p, da1.p, daf.p, int32_t(ae.size())};
   CgConfig cfg{};
   cfg.n_env = 1;
+  cfg.n_rows = nbr;
   cfg.norm_type = kp.norm_type;
   cfg.prec = kp.preconditioner;
   cfg.max_iter = kp.max_iter;

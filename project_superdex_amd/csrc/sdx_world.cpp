@@ -727,6 +727,7 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
                                       WarmStats* learned) {
   CgConfig cfg{};
   cfg.n_env = nEnv_;
+  cfg.n_rows = nV_;
   cfg.norm_type = kp.norm_type;
   cfg.prec = kp.preconditioner;
   cfg.max_iter = kp.max_iter;
