@@ -61,15 +61,56 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, FieldPtrs<T> f, con
       w[3 * a + d] = uu - (s0 + dt * v0[3 * nd[a] + d]);  // u - predicted target (batched_element_utils.h:90-120)
       wd[3 * a + d] = uu - s0;
     }
-  T res[12];
-  T H[78];
-  smith_element<T>(dN, ue, vol, mu, lam, wantH != 0, project != 0, res, H);
-  // gravity (fem_gravity.h:108-118): res_f -= rho g vol / 4
-  const T gq = T(0.25) * mass;
+  // destination records of the tet, loaded before the element math
+  int pv[4];
+  uint32_t pe[6];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) pv[a] = m.pos_v[4 * t + a];
+  if (wantH)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pe[k] = m.pos_e[6 * t + k];
   // consistent mass M_ab = m/20 (1 + delta_ab) (Q4 integrates P1 x P1 exactly)
   const T idt2 = T(1) / (dt * dt);
   const T mdamp = alpha > T(0) ? alpha / dt : T(0);
   const T m20 = mass * T(0.05);
+  const T ms = (idt2 + mdamp) * m20;
+  // Hessian blocks go to their records as soon as they are formed, with the
+  // mass term added: diagonal blocks (6 entries) to the vertex-sorted
+  // records, off-diagonal blocks to the pair-sorted records, transposed
+  // where the pair runs backwards
+  auto emit = [&](int fa, int fb, const T* q) {
+    if (fa == fb) {
+      T* d = Dv + (size_t)pv[fa] * 6;
+      d[0] = q[0] + T(2) * ms;
+      d[1] = q[1] + T(2) * ms;
+      d[2] = q[2] + T(2) * ms;
+      d[3] = q[3];
+      d[4] = q[4];
+      d[5] = q[5];
+      return;
+    }
+    const uint32_t p = pe[fa == 0 ? fb - 1 : (fa == 1 ? fb + 1 : 5)];  // pair (kPairA, kPairB) index
+    T o[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) o[r] = q[r];
+    o[0] += ms;
+    o[4] += ms;
+    o[8] += ms;
+    T* d = Ev + (size_t)(p & 0x7fffffffu) * 9;
+    if (p >> 31) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) d[3 * r + c] = o[3 * c + r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 9; ++r) d[r] = o[r];
+    }
+  };
+  T res[12];
+  smith_element<T>(dN, ue, vol, mu, lam, wantH != 0, project != 0, res, emit);
+  // gravity (fem_gravity.h:108-118): res_f -= rho g vol / 4
+  const T gq = T(0.25) * mass;
   T sw[3] = {w[0] + w[3] + w[6] + w[9], w[1] + w[4] + w[7] + w[10], w[2] + w[5] + w[8] + w[11]};
   T sd[3] = {wd[0] + wd[3] + wd[6] + wd[9], wd[1] + wd[4] + wd[7] + wd[10], wd[2] + wd[5] + wd[8] + wd[11]};
   const T gv[3] = {gx, gy, gz};
@@ -81,56 +122,12 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, FieldPtrs<T> f, con
       if (mdamp > T(0)) r += mdamp * m20 * (wd[3 * a + d] + sd[d]);
       res[3 * a + d] = r;
     }
-  int pv[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    pv[a] = m.pos_v[4 * t + a];
     T* re = Rv + (size_t)pv[a] * 3;
     re[0] = res[3 * a];
     re[1] = res[3 * a + 1];
     re[2] = res[3 * a + 2];
-  }
-  if (!wantH) return;
-  const T ms = (idt2 + mdamp) * m20;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = a; b < 4; ++b) {
-      T* blk = H + he_block_offset(a, b);
-      const T add = a == b ? T(2) * ms : ms;
-      if (a == b) {
-        blk[0] += add;
-        blk[1] += add;
-        blk[2] += add;
-      } else {
-        blk[0] += add;
-        blk[4] += add;
-        blk[8] += add;
-      }
-    }
-  // diagonal blocks (6 entries) to the vertex-sorted slots, off-diagonal
-  // blocks to the pair-sorted slots, transposed where the pair runs backwards
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const T* q = H + he_block_offset(a, a);
-    T* d = Dv + (size_t)pv[a] * 6;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) d[k] = q[k];
-  }
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const uint32_t pe = m.pos_e[6 * t + k];
-    const T* q = H + he_block_offset(kPairA[k], kPairB[k]);
-    T* d = Ev + (size_t)(pe & 0x7fffffffu) * 9;
-    if (pe >> 31) {
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) d[3 * r + c] = q[3 * c + r];
-    } else {
-#pragma unroll
-      for (int r = 0; r < 9; ++r) d[r] = q[r];
-    }
   }
 }
 

@@ -239,8 +239,67 @@ __host__ __device__ constexpr int he_block_offset(int a, int b) {
                 : 72;
 }
 
-// Element kernel core: stress residual (12) and, if wantH, the packed
-// Hessian (78) of one tet; both scaled by the Q1 weight `vol`.
+// Rank-1 update of the packed Hessian by eigenpair MODE of Smith's
+// eigensystem (0-2 scaling, 3-8 twist / flip about axis (MODE - 3) / 2), as a
+// template so every array index is a compile-time constant.
+template <int MODE, class T>
+__device__ __forceinline__ void smith_mode(const T la[3], const V3<T> ae[3], const V3<T> w[4], const V3<T> U[3],
+                                           const T s[3], T c0, T c1, T vol, T eps, T H[78]) {
+  T lk;
+  V3<T> y[4];
+  if (MODE < 3) {
+    // scaling: Q = U diag(ae) V^T  ->  y_f = U (ae .* w_f)
+    lk = la[MODE < 3 ? MODE : 0];
+    const V3<T> e = ae[MODE < 3 ? MODE : 0];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const T t0 = e.x * w[f].x, t1 = e.y * w[f].y, t2 = e.z * w[f].z;
+      y[f] = add3(add3(scl3(t0, U[0]), scl3(t1, U[1])), scl3(t2, U[2]));
+    }
+  } else {
+    // twist / flip in the plane (j, k) orthogonal to axis i, oriented j < k
+    constexpr int i = MODE < 3 ? 0 : (MODE - 3) >> 1;
+    constexpr bool twist = MODE < 3 ? false : ((MODE - 3) & 1) == 0;
+    constexpr int j = (i + 1) % 3, k = (i + 2) % 3;
+    constexpr int jj = j < k ? j : k, kk = j < k ? k : j;
+    lk = twist ? (c0 + c1 * s[i]) : (c0 - c1 * s[i]);
+    const T sg = twist ? T(-1) : T(1);
+    const T isq2 = T(0.70710678118654752440);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const T wv[3] = {w[f].x, w[f].y, w[f].z};
+      // M = e_jj e_kk^T + sg e_kk e_jj^T  ->  M w = e_jj w_kk + sg e_kk w_jj
+      y[f] = scl3(isq2, add3(scl3(wv[kk], U[jj]), scl3(sg * wv[jj], U[kk])));
+    }
+  }
+  const T lam = (lk > eps ? lk : eps) * vol;
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int g = f; g < 4; ++g) {
+      T* blk = H + he_block_offset(f, g);
+      const T yf[3] = {y[f].x, y[f].y, y[f].z}, yg[3] = {y[g].x, y[g].y, y[g].z};
+      if (f == g) {
+        blk[0] += lam * yf[0] * yf[0];
+        blk[1] += lam * yf[1] * yf[1];
+        blk[2] += lam * yf[2] * yf[2];
+        blk[3] += lam * yf[0] * yf[1];
+        blk[4] += lam * yf[0] * yf[2];
+        blk[5] += lam * yf[1] * yf[2];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) blk[r * 3 + c] += lam * yf[r] * yg[c];
+      }
+    }
+}
+
+// Element kernel core: stress residual (12) and, if wantH, the 10 upper 3x3
+// blocks of the Hessian of one tet, both scaled by the Q1 weight `vol`; each
+// block goes to emit(f, g, blk) as soon as it is formed (f == g: 6 entries
+// 00 11 22 01 02 12, else 9 row-major), so the 78 values are never live
+// together on the direct path.
 //   F = I + sum_f u_f (x) dN_f
 //   res_f = vol * P dN_f
 //   direct tangent:   H_fg = vol [c3 a_f a_g^T + lh b_f b_g^T + c2 (dN_f.dN_g) I
@@ -248,9 +307,9 @@ __host__ __device__ constexpr int he_block_offset(int a, int b) {
 //     with a_f = F dN_f, b_f = cof(F) dN_f, skew(v)_rc = eps_rcm v_m
 //   projected tangent: H_fg = vol sum_k max(l_k, eps) (Q_k dN_f)(Q_k dN_g)^T
 //     over Smith's 9 eigenpairs (3 scaling, 3 twist, 3 flip).
-template <class T>
-__device__ void smith_element(const V3<T> dN[4], const T u[12], T vol, T mu, T lambda, bool wantH,
-                              bool projectPsd, T res[12], T H[78]) {
+template <class T, class Emit>
+__device__ __forceinline__ void smith_element(const V3<T> dN[4], const T u[12], T vol, T mu, T lambda, bool wantH,
+                                              bool projectPsd, T res[12], Emit&& emit) {
   M33<T> F;
   F.m[0] = T(1);
   F.m[1] = T(0);
@@ -314,7 +373,7 @@ __device__ void smith_element(const V3<T> dN[4], const T u[12], T vol, T mu, T l
         const T dd = dot3(dN[f], dN[g]);
         const V3<T> sk = mul(F, cross3(dN[f], dN[g]));
         const T skv[3] = {sk.x, sk.y, sk.z};
-        T* blk = H + he_block_offset(f, g);
+        T blk[9];
         if (f == g) {
           // entries 00,11,22,01,02,12 ; skew term vanishes on the diagonal block
           const int rr[6] = {0, 1, 2, 0, 0, 1}, cc[6] = {0, 1, 2, 1, 2, 2};
@@ -340,6 +399,7 @@ __device__ void smith_element(const V3<T> dN[4], const T u[12], T vol, T mu, T l
               blk[r * 3 + c] = vol * v;
             }
         }
+        emit(f, g, blk);
       }
     return;
   }
@@ -364,62 +424,22 @@ __device__ void smith_element(const V3<T> dN[4], const T u[12], T vol, T mu, T l
   V3<T> w[4];
 #pragma unroll
   for (int f = 0; f < 4; ++f) w[f] = mk3(dot3(Vv[0], dN[f]), dot3(Vv[1], dN[f]), dot3(Vv[2], dN[f]));
+  T H[78];
 #pragma unroll
   for (int k = 0; k < 78; ++k) H[k] = T(0);
-  const T isq2 = T(0.70710678118654752440);
+  smith_mode<0>(la, ae, w, U, s, c0, c1, vol, eps, H);
+  smith_mode<1>(la, ae, w, U, s, c0, c1, vol, eps, H);
+  smith_mode<2>(la, ae, w, U, s, c0, c1, vol, eps, H);
+  smith_mode<3>(la, ae, w, U, s, c0, c1, vol, eps, H);
+  smith_mode<4>(la, ae, w, U, s, c0, c1, vol, eps, H);
+  smith_mode<5>(la, ae, w, U, s, c0, c1, vol, eps, H);
+  smith_mode<6>(la, ae, w, U, s, c0, c1, vol, eps, H);
+  smith_mode<7>(la, ae, w, U, s, c0, c1, vol, eps, H);
+  smith_mode<8>(la, ae, w, U, s, c0, c1, vol, eps, H);
 #pragma unroll
-  for (int mode = 0; mode < 9; ++mode) {
-    T lk;
-    V3<T> y[4];
-    if (mode < 3) {
-      // scaling: Q = U diag(ae) V^T  ->  y_f = U (ae .* w_f)
-      lk = la[mode];
-      const V3<T> e = ae[mode];
+  for (int f = 0; f < 4; ++f)
 #pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        const T t0 = e.x * w[f].x, t1 = e.y * w[f].y, t2 = e.z * w[f].z;
-        y[f] = add3(add3(scl3(t0, U[0]), scl3(t1, U[1])), scl3(t2, U[2]));
-      }
-    } else {
-      // twist / flip in the plane (j, k) orthogonal to axis i
-      const int i = (mode - 3) >> 1;
-      const bool twist = ((mode - 3) & 1) == 0;
-      const int j = (i + 1) % 3, k = (i + 2) % 3;
-      // with the (j,k) orientation fixed so that j < k for the matrix entries
-      const int jj = j < k ? j : k, kk = j < k ? k : j;
-      lk = twist ? (c0 + c1 * s[i]) : (c0 - c1 * s[i]);
-      const T sg = twist ? T(-1) : T(1);
-#pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        const T wv[3] = {w[f].x, w[f].y, w[f].z};
-        // M = e_jj e_kk^T + sg e_kk e_jj^T  ->  M w = e_jj w_kk + sg e_kk w_jj
-        const V3<T> Uj = jj == 0 ? U[0] : (jj == 1 ? U[1] : U[2]);
-        const V3<T> Uk = kk == 0 ? U[0] : (kk == 1 ? U[1] : U[2]);
-        y[f] = scl3(isq2, add3(scl3(wv[kk], Uj), scl3(sg * wv[jj], Uk)));
-      }
-    }
-    const T lam = (lk > eps ? lk : eps) * vol;
-#pragma unroll
-    for (int f = 0; f < 4; ++f)
-#pragma unroll
-      for (int g = f; g < 4; ++g) {
-        T* blk = H + he_block_offset(f, g);
-        const T yf[3] = {y[f].x, y[f].y, y[f].z}, yg[3] = {y[g].x, y[g].y, y[g].z};
-        if (f == g) {
-          blk[0] += lam * yf[0] * yf[0];
-          blk[1] += lam * yf[1] * yf[1];
-          blk[2] += lam * yf[2] * yf[2];
-          blk[3] += lam * yf[0] * yf[1];
-          blk[4] += lam * yf[0] * yf[2];
-          blk[5] += lam * yf[1] * yf[2];
-        } else {
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) blk[r * 3 + c] += lam * yf[r] * yg[c];
-        }
-      }
-  }
+    for (int g = f; g < 4; ++g) emit(f, g, H + he_block_offset(f, g));
 }
 
 }  // namespace sdx

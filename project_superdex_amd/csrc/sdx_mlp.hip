@@ -69,25 +69,63 @@ void launch_mlp_pack(hipStream_t s, const float* params, uint16_t* w1, uint16_t*
 }
 
 // ------------------------------------------------------------------ inference
+//
+// One workgroup of kInferWaves waves per CU holds the BF16 weights in LDS;
+// each wave runs tiles of 32 vertices through the layers.  A layer computes
+// C[u][v] = sum_k W[u][k] act[v][k] on the matrix cores with W as the A
+// operand (rows = units) and the activations as the B operand (columns =
+// vertices), so the accumulator of lane l holds 4 consecutive units
+// 4 (l / 16) + e of vertex l % 16 (v_mfma_f32_16x16x*_bf16 output layout) and
+// bias + GELU + BF16 packing end in one 8-byte LDS store per 4 units.  Both
+// operands of an MFMA hold k = K (l / 16) + j in lane l, slot j (K = 4 or 8
+// slots); any one lane-to-k map used for both operands gives the same sum.
 
-constexpr int kLd = kMlpHid + 8;  // LDS row stride (BF16): rows land on different banks
-constexpr int kInferWaves = 4;
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+constexpr int kLd = kMlpHid + 8;  // LDS row stride (BF16) of W2, W3, W4 and the activations
+constexpr int kInferWaves = 8;
 constexpr int kRowsPerWave = 32;
 
-// Bias + GELU of a 32 x 128 accumulator block, stored as BF16 rows of `act`.
-// Accumulator (rt, ct) element e of lane l is row rt*16 + 4*(l/16) + e,
-// column ct*16 + l%16 (v_mfma_f32_16x16x16_bf16 output layout).
-__device__ __forceinline__ void store_act(f32x4 (&acc)[2][8], const float* __restrict__ bias,
+// GELU(z) = z Phi(z) = z (1 + erf(z / sqrt 2)) / 2 for inference, with erf
+// from Abramowitz & Stegun, Handbook of Mathematical Functions, 7.1.26:
+// erf(a) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-a^2),
+// t = 1 / (1 + p a), a >= 0, |error| <= 1.5e-7 (far below BF16 resolution).
+__device__ __forceinline__ float gelu_infer(float z) {
+  const float a = fabsf(z) * 0.7071067811865476f;
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, a, 1.0f));
+  const float poly =
+      t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f), 0.254829592f);
+  const float e = __builtin_amdgcn_exp2f(-(a * a) * 1.4426950408889634f);  // exp(-a^2)
+  const float erf_v = copysignf(fmaf(-poly, e, 1.0f), z);
+  const float hz = 0.5f * z;
+  return fmaf(hz, erf_v, hz);
+}
+
+// Two FP32 values to packed BF16 (round to nearest even, v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+}
+
+// Bias + GELU of the 128 x 32 accumulator block (units x vertices), stored
+// as BF16 activation rows act[v][u].
+__device__ __forceinline__ void store_act(f32x4 (&acc)[8][2], const float* __restrict__ bias,
                                           uint16_t* __restrict__ act, int lr, int lg) {
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int ut = 0; ut < 8; ++ut) {
+    const f32x4 b = *reinterpret_cast<const f32x4*>(bias + ut * 16 + 4 * lg);
 #pragma unroll
-    for (int ct = 0; ct < 8; ++ct) {
-      const int c = ct * 16 + lr;
-      const float bc = bias[c];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) act[(rt * 16 + 4 * lg + e) * kLd + c] = f2bf(gelu(acc[rt][ct][e] + bc));
+    for (int vt = 0; vt < 2; ++vt) {
+      const f32x4 z = acc[ut][vt] + b;
+      uint2 q;
+      q.x = pack_bf16(gelu_infer(z[0]), gelu_infer(z[1]));
+      q.y = pack_bf16(gelu_infer(z[2]), gelu_infer(z[3]));
+      *reinterpret_cast<uint2*>(act + (vt * 16 + lr) * kLd + ut * 16 + 4 * lg) = q;
     }
+  }
 }
 
 // Order one wave's LDS writes before its own later reads of other lanes' data.
@@ -97,24 +135,26 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One hidden layer: act (32 x 128, BF16) times W^T (W: 128 x 128 BF16 rows in
-// LDS) into acc.  A operand: lane l holds row l%16, k = 4*(l/16) + 0..3;
-// B operand: lane l holds column l%16 (row of W), the same k.
+__device__ __forceinline__ bf16x8_t lds_frag8(const uint16_t* p) {
+  return __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const s16x8*>(p));
+}
+
+// One hidden layer (K = 128): acc[ut][vt] = W[ut block] act[vt block]^T.
 __device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ act, const uint16_t* __restrict__ W,
-                                             f32x4 (&acc)[2][8], int lr, int lg) {
+                                             f32x4 (&acc)[8][2], int lr, int lg) {
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int ut = 0; ut < 8; ++ut)
 #pragma unroll
-    for (int ct = 0; ct < 8; ++ct) acc[rt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int ks = 0; ks < kMlpHid / 16; ++ks) {
-    const bf16x4 a0 = *reinterpret_cast<const bf16x4*>(act + lr * kLd + ks * 16 + 4 * lg);
-    const bf16x4 a1 = *reinterpret_cast<const bf16x4*>(act + (16 + lr) * kLd + ks * 16 + 4 * lg);
+    for (int vt = 0; vt < 2; ++vt) acc[ut][vt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ct = 0; ct < 8; ++ct) {
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(W + (ct * 16 + lr) * kLd + ks * 16 + 4 * lg);
-      acc[0][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b, acc[0][ct], 0, 0, 0);
-      acc[1][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b, acc[1][ct], 0, 0, 0);
+  for (int ks = 0; ks < kMlpHid / 32; ++ks) {
+    const bf16x8_t b0 = lds_frag8(act + lr * kLd + ks * 32 + 8 * lg);
+    const bf16x8_t b1 = lds_frag8(act + (16 + lr) * kLd + ks * 32 + 8 * lg);
+#pragma unroll
+    for (int ut = 0; ut < 8; ++ut) {
+      const bf16x8_t w = lds_frag8(W + (ut * 16 + lr) * kLd + ks * 32 + 8 * lg);
+      acc[ut][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, b0, acc[ut][0], 0, 0, 0);
+      acc[ut][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, b1, acc[ut][1], 0, 0, 0);
     }
   }
 }
@@ -129,7 +169,7 @@ __global__ void __launch_bounds__(kInferWaves * 64) k_mlp_infer(MlpBf16 w, const
   __shared__ __align__(16) uint16_t sW2[kMlpHid * kLd];
   __shared__ __align__(16) uint16_t sW3[kMlpHid * kLd];
   __shared__ __align__(16) uint16_t sW4[16 * kLd];
-  __shared__ float sB[3 * kMlpHid + 16];
+  __shared__ __align__(16) float sB[3 * kMlpHid + 16];
   __shared__ __align__(16) uint16_t sAct[kInferWaves][kRowsPerWave * kLd];
   for (int i = threadIdx.x; i < kMlpHid * kMlpInPad; i += blockDim.x) sW1[i] = w.w1[i];
   for (int i = threadIdx.x; i < kMlpHid * kMlpHid; i += blockDim.x) {
@@ -149,25 +189,24 @@ __global__ void __launch_bounds__(kInferWaves * 64) k_mlp_infer(MlpBf16 w, const
   const int lr = lane & 15, lg = lane >> 4;
   uint16_t* act = sAct[wave];
   const int n_tiles = (n + kRowsPerWave - 1) / kRowsPerWave;
-  f32x4 acc[2][8];
+  f32x4 acc[8][2];
   for (int tile = blockIdx.x * kInferWaves + wave; tile < n_tiles; tile += gridDim.x * kInferWaves) {
     const int row0 = tile * kRowsPerWave;
-    // layer 1 (K = 16): the A operand comes straight from the FP32 feature rows
+    // layer 1 (K = 16): the B operand comes straight from the FP32 feature rows
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt) {
-      const int r = row0 + rt * 16 + lr;
-      bf16x4 a = {0, 0, 0, 0};
+    for (int vt = 0; vt < 2; ++vt) {
+      const int r = row0 + vt * 16 + lr;
+      uint2 q = make_uint2(0u, 0u);
       if (r < n) {
         const float4 f = *reinterpret_cast<const float4*>(x + (size_t)r * kMlpInPad + 4 * lg);
-        a[0] = short(f2bf(f.x));
-        a[1] = short(f2bf(f.y));
-        a[2] = short(f2bf(f.z));
-        a[3] = short(f2bf(f.w));
+        q.x = pack_bf16(f.x, f.y);
+        q.y = pack_bf16(f.z, f.w);
       }
+      const bf16x4 bx = __builtin_bit_cast(bf16x4, q);
 #pragma unroll
-      for (int ct = 0; ct < 8; ++ct) {
-        const bf16x4 b = *reinterpret_cast<const bf16x4*>(sW1 + (ct * 16 + lr) * kMlpInPad + 4 * lg);
-        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      for (int ut = 0; ut < 8; ++ut) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(sW1 + (ut * 16 + lr) * kMlpInPad + 4 * lg);
+        acc[ut][vt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       }
     }
     store_act(acc, sB, act, lr, lg);
@@ -180,34 +219,33 @@ __global__ void __launch_bounds__(kInferWaves * 64) k_mlp_infer(MlpBf16 w, const
     wave_lds_sync();
     store_act(acc, sB + 2 * kMlpHid, act, lr, lg);
     wave_lds_sync();
-    // output layer (N = 3, padded to 16)
+    // output layer (3 units, padded to 16): lanes 0..15 hold units 0..3 of vertex lr
     f32x4 o[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
-    for (int ks = 0; ks < kMlpHid / 16; ++ks) {
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(sW4 + lr * kLd + ks * 16 + 4 * lg);
+    for (int ks = 0; ks < kMlpHid / 32; ++ks) {
+      const bf16x8_t wv = lds_frag8(sW4 + lr * kLd + ks * 32 + 8 * lg);
 #pragma unroll
-      for (int rt = 0; rt < 2; ++rt) {
-        const bf16x4 a = *reinterpret_cast<const bf16x4*>(act + (rt * 16 + lr) * kLd + ks * 16 + 4 * lg);
-        o[rt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, o[rt], 0, 0, 0);
-      }
+      for (int vt = 0; vt < 2; ++vt)
+        o[vt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, lds_frag8(act + (vt * 16 + lr) * kLd + ks * 32 + 8 * lg),
+                                                        o[vt], 0, 0, 0);
     }
     wave_lds_sync();  // the next tile overwrites act
-    if (lr < kMlpOut) {
-      const float bc = sB[3 * kMlpHid + lr];
+    if (lg == 0) {
 #pragma unroll
-      for (int rt = 0; rt < 2; ++rt)
+      for (int vt = 0; vt < 2; ++vt) {
+        const int r = row0 + vt * 16 + lr;
+        if (r >= n) continue;
+        const float sc = out_scale ? float(out_scale[row_env[r]]) : 1.0f;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int r = row0 + rt * 16 + 4 * lg + e;
-          if (r >= n) continue;
-          float v = o[rt][e] + bc;
-          if (out_scale) v *= float(out_scale[row_env[r]]);
-          if (pinned && pinned[3 * r + lr]) v = 0.0f;
+        for (int e = 0; e < kMlpOut; ++e) {
+          float v = (o[vt][e] + sB[3 * kMlpHid + e]) * sc;
+          if (pinned && pinned[3 * r + e]) v = 0.0f;
           if (yt)
-            yt[3 * r + lr] = T(v);
+            yt[3 * r + e] = T(v);
           else
-            y[3 * r + lr] = v;
+            y[3 * r + e] = v;
         }
+      }
     }
   }
 }
