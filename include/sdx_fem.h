@@ -232,6 +232,11 @@ sdx_status sdx_add_rigid_sphere(sdx_world* world, const double c[3], double r, i
                                 int32_t* out_collider);
 /* SetRootTransform translation: plane point or sphere centre. */
 sdx_status sdx_set_rigid_position(sdx_world* world, int32_t collider, const double pos[3]);
+/* The same for n colliders in one call: collider colliders[k] moves to
+ * positions[3 k .. 3 k + 2].  No counterpart in the reference (its rigid
+ * actors are moved one by one); a batch of environments sets every target of
+ * a step at once.  All indices are checked before any collider moves. */
+sdx_status sdx_set_rigid_positions(sdx_world* world, int32_t n, const int32_t* colliders, const double* positions);
 
 /* Scene::Step(dt): one backward-Euler stage per actor, each actor its own
  * island and Newton solve.  stats: n_actor entries (may be NULL). */

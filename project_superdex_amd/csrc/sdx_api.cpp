@@ -242,6 +242,18 @@ sdx_status sdx_set_rigid_position(sdx_world* world, int32_t collider, const doub
   });
 }
 
+sdx_status sdx_set_rigid_positions(sdx_world* world, int32_t n, const int32_t* colliders, const double* positions) {
+  SDX_REQUIRE(world && n >= 0 && (n == 0 || (colliders && positions)), "NULL argument");
+  return guarded([&]() -> sdx_status {
+    for (int32_t k = 0; k < n; ++k) world->impl->collider(colliders[k]);  // throws on a bad index
+    for (int32_t k = 0; k < n; ++k) {
+      sdx::HostCollider& c = world->impl->collider(colliders[k]);
+      for (int d = 0; d < 3; ++d) c.p[d] = positions[3 * size_t(k) + size_t(d)];
+    }
+    return SDX_OK;
+  });
+}
+
 static sdx_status validate_solver_params(const sdx_solver_params& p) {
   const sdx_solver_params* params = &p;
   SDX_REQUIRE(params->newton_max_iter >= 0 && params->linear.max_iter > 0, "invalid iteration limits");

@@ -108,6 +108,21 @@ class Collider:
         return tuple(self.point[i] + self.velocity[i] * dt * s for i in range(3))
 
 
+class ColliderPaths:
+    """Collider.position of a list of colliders, all at once: at(step, dt) is
+    the (n, 3) array of their positions."""
+
+    def __init__(self, colliders):
+        self.P = np.array([c.point for c in colliders], dtype=np.float64).reshape(-1, 3)
+        self.V = np.array([c.velocity for c in colliders], dtype=np.float64).reshape(-1, 3)
+        self.ramp = np.array([c.ramp_steps for c in colliders], dtype=np.int64)
+
+    def at(self, step: int, dt: float) -> np.ndarray:
+        s = np.full(len(self.ramp), step + 1)
+        s = np.where(self.ramp > 0, np.minimum(s, self.ramp), s)
+        return self.P + self.V * (dt * s)[:, None]
+
+
 @dataclass
 class SoftBody:
     X: np.ndarray

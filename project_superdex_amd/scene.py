@@ -142,10 +142,24 @@ class Scene:
         self.colliders.append(r)
         return r
 
-    def step(self, dt: float, params: _abi.SolverParams | None = None):
+    def set_rigid_positions(self, colliders, positions):
+        """Move colliders[k] (RigidActor or index) to positions[k] in one call."""
+        if isinstance(colliders, np.ndarray):
+            idx = np.ascontiguousarray(colliders, dtype=np.int32)
+        else:
+            idx = np.ascontiguousarray([c.index if isinstance(c, RigidActor) else int(c) for c in colliders],
+                                       dtype=np.int32)
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(len(idx), 3)
+        check(lib().sdx_set_rigid_positions(self._w, len(idx), idx.ctypes.data_as(_ip), pos.ctypes.data_as(_dp)))
+
+    def step(self, dt: float, params: _abi.SolverParams | None = None, as_array: bool = False):
+        """One step; per-actor statistics as dicts, or with as_array as one
+        numpy structured array (fields of sdx_step_stats, no per-actor copies)."""
         params = params or _abi.default_solver_params()
         stats = (_abi.StepStats * max(1, len(self.actors)))()
         check(lib().sdx_step(self._w, dt, C.byref(params), stats, len(self.actors)))
+        if as_array:
+            return np.ctypeslib.as_array(stats)[:len(self.actors)]
         return [s.as_dict() for s in stats[:len(self.actors)]]
 
     def info(self):

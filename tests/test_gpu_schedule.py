@@ -145,3 +145,33 @@ def test_autotune_on_first_step_via_params():
     ref, _ = run_gpu(spec, q, 2, precision=MIXED)
     for a, b in zip(ref.actors, sg.actors):
         assert np.abs(a.displacements() - b.displacements()).max() <= 1e-3 * np.abs(a.displacements()).max()
+
+
+def test_batched_collider_positions_match_one_by_one():
+    """sdx_set_rigid_positions moves every collider of a step in one call and
+    steps to the state of one sdx_set_rigid_position call per collider; a
+    batch with a bad index is rejected before any collider moves."""
+    spec = batched_spec(steps=4)
+    p = mixed_params()
+    s1, r1 = gs.build_scene(spec, precision=MIXED)
+    s2, r2 = gs.build_scene(spec, precision=MIXED)
+    paths = mesh.ColliderPaths(spec.colliders)
+    ids = np.array([r.index for r in r2], dtype=np.int32)
+    for s in range(spec.steps):
+        for r, c in zip(r1, spec.colliders):
+            r.set_position(c.position(s, spec.dt))
+        s2.set_rigid_positions(ids, paths.at(s, spec.dt))
+        a = s1.step(spec.dt, p)
+        b = s2.step(spec.dt, p, as_array=True)
+        for ea, eb in zip(a, b):
+            assert abs(ea["linear_iters"] - int(eb["linear_iters"])) <= 2
+    for (ua, _), (ub, _) in zip(state(s1), state(s2)):
+        assert np.abs(ua - ub).max() <= 1e-3 * np.abs(ua).max()
+    # a batch naming a collider that does not exist moves nothing
+    before = paths.at(spec.steps - 1, spec.dt)
+    with pytest.raises(RuntimeError):
+        s2.set_rigid_positions(np.array([ids[0], 10_000], dtype=np.int32), before[:2] + 1.0)
+    s1.step(spec.dt, p)
+    s2.step(spec.dt, p)
+    for (ua, _), (ub, _) in zip(state(s1), state(s2)):
+        assert np.abs(ua - ub).max() <= 1e-3 * np.abs(ua).max()
