@@ -762,7 +762,7 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
     // a lane's blocks of a chunk of U L blocks are loaded together: indices
     // past the row end are clamped to its last block (always valid, a row
     // holds its diagonal) and weighted by zero, so no load waits on a branch
-    constexpr int U = kSpmvChunk / L;
+    constexpr int U = kSpmvChunk / L > 0 ? kSpmvChunk / L : 1;
     for (int kb = k0 + lane; kb < k1; kb += U * L) {
       int kk[U], jc[U];
       T wv[U];
@@ -1047,7 +1047,8 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, WorkList wl
 // burst.
 __global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, const EnvCG* __restrict__ env,
                                                       int32_t* __restrict__ la, int32_t* __restrict__ lv,
-                                                      int32_t* counts, int par) {
+                                                      int32_t* counts, int par, const int32_t* num_active,
+                                                      int32_t* publish) {
   const int nba = (ta.n + 255) / 256;
   const bool isA = int(blockIdx.x) < nba;
   const Tiles& t = isA ? ta : tv;
@@ -1057,6 +1058,7 @@ __global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, const
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     counts[par ^ 1] = 0;
     counts[2 + (par ^ 1)] = 0;
+    if (publish) publish[par] = *num_active;  // pinned host memory, read after this kernel's event
   }
   const bool keep = i < t.n && env[t.env[i]].status == kCgActive;
   const unsigned long long m = __ballot(keep);
@@ -1362,10 +1364,11 @@ void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int gr
 }
 
 void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, const EnvCG* env, int32_t* la,
-                          int32_t* lv, int32_t* counts, int par) {
+                          int32_t* lv, int32_t* counts, int par, const int32_t* num_active, int32_t* publish) {
   const int nb = (ta.n + 255) / 256 + (tv.n + 255) / 256;
   if (nb == 0) return;
-  hipLaunchKernelGGL(k_compact_tiles, dim3(nb), dim3(256), 0, s, ta, tv, env, la, lv, counts, par);
+  hipLaunchKernelGGL(k_compact_tiles, dim3(nb), dim3(256), 0, s, ta, tv, env, la, lv, counts, par, num_active,
+                     publish);
 }
 
 template <class T, class X>

@@ -506,10 +506,9 @@ void World<T>::upload_colliders() {
   first[size_t(nEnv_)] = int32_t(cd.size() / 8);
   if (cd.empty()) cd.assign(8, 0.0);
   if (d_coll_.n < cd.size()) d_coll_.alloc(cd.size());
-  d_coll_.upload_async(cd, stream_);
+  put(d_coll_, cd);
   if (d_coll_first_.n < first.size()) d_coll_first_.alloc(first.size());
-  d_coll_first_.upload_async(first, stream_);
-  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  put(d_coll_first_, first);
   mesh_.coll_first = d_coll_first_.p;
   mesh_.coll = d_coll_.p;
   colliders_dirty_ = false;
@@ -517,7 +516,7 @@ void World<T>::upload_colliders() {
 
 template <class T>
 void World<T>::set_env_mask(const std::vector<int32_t>& mask) {
-  env_mask_.upload_async(mask, stream_);
+  put(env_mask_, mask);
 }
 
 // Assemble residual (and BSR + preconditioner if wantH) for the environments
@@ -548,9 +547,7 @@ void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH) {
 template <class T>
 std::vector<double> World<T>::read_env_sq() {
   std::vector<double> h(static_cast<size_t>(nEnv_));
-  hip_check(hipMemcpyAsync(h.data(), env_sq_.p, sizeof(double) * size_t(nEnv_), hipMemcpyDeviceToHost, stream_),
-            "hipMemcpyAsync");
-  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  get(env_sq_, h);
   flush_timers();
   return h;
 }
@@ -565,9 +562,8 @@ void World<T>::issue_cg(const CgConfig& cfg, int m, int k0, int n, bool timed) {
   T* accf = reinterpret_cast<T*>(acc_.p);
   WorkList wa{nullptr, nullptr}, wv{nullptr, nullptr};
   int ga = 0, gv = 0;
-  if (sched_.active_lists) {
+  if (sched_.active_lists) {  // the caller launched compact_tiles for this burst
     const int par = m & 1;
-    launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, env_cg_.p, list_a_.p, list_v_.p, list_count_.p, par);
     wa = WorkList{list_a_.p, list_count_.p + par};
     wv = WorkList{list_v_.p, list_count_.p + 2 + par};
     ga = std::max(8, std::min(dt_a_.view.n, kPersistentGridA));
@@ -637,9 +633,9 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
     n_warm += solve[size_t(e)] && warm[size_t(e)] ? 1 : 0;
   }
   if (active == 0) return h;
-  solve_mask_.upload_async(solve, stream_);
+  put(solve_mask_, solve);
   std::vector<int32_t> na(1, active);
-  num_active_.upload_async(na, stream_);
+  put(num_active_, na);
   const int32_t* wm = n_warm ? warm_mask_.p : nullptr;
   launch_fill<int32_t>(stream_, 4, 0, list_count_.p);
   // decide: keep an environment's guess only if |b - A x0| < |b| (on device)
@@ -647,7 +643,7 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
   if (n_warm && decide) launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
   if (precision_ != SDX_PREC_MODE_FP32) {
     if (n_warm) {
-      warm_mask_.upload_async(warm, stream_);
+      put(warm_mask_, warm);
       launch_residual64<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
                                    d_col_.p, vals_.p, nB_, xcg<double>(), res_.p, rref_.p, dacc);
       if (decide) launch_ws_decide(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p);
@@ -657,7 +653,7 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
                               dinv_.p, nV_, xcg<double>(), r_.p, z_.p, env_cg_.p, acc_.p);
   } else {
     if (n_warm) {
-      warm_mask_.upload_async(warm, stream_);
+      put(warm_mask_, warm);
       launch_residual64<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p, d_col_.p,
                               vals_.p, nB_, xcg<T>(), res_.p, rref_.p, dacc);
       if (decide) launch_ws_decide(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p);
@@ -669,28 +665,45 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
   }
   // Bursts of B iterations are issued one ahead of the host's check of the
   // active count, so the device never waits for the host between bursts; a
-  // burst issued after every environment finished exits at once.
+  // burst issued after every environment finished exits at once.  With work
+  // lists, burst m starts with the compaction, which also writes the count
+  // left by burst m - 1 into pinned host memory; the host waits for the
+  // compaction only.  Without lists the count is copied after each burst.
   const int B = std::max(1, sched_.cg_burst);
   int32_t* hna = pinned_active_;
+  const bool lists = sched_.active_lists != 0;
   for (int m = 0;; ++m) {
     const bool issue = int64_t(m) * B <= int64_t(cfg.max_iter);
     if (issue) {
       const bool timed = burst_timed(m);
+      if (lists) {
+        launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, env_cg_.p, list_a_.p, list_v_.p, list_count_.p, m & 1,
+                             num_active_.p, pinned_active_dev_);
+        hip_check(hipEventRecord(burst_done_[m & 1], stream_), "hipEventRecord");
+      }
       if (sched_.use_graphs && !timed)
         hip_check(hipGraphLaunch(cg_graph(cfg, m, B), stream_), "hipGraphLaunch");
       else
         issue_cg(cfg, m, m * B, B, timed);
-      hip_check(hipMemcpyAsync(hna + (m & 1), num_active_.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream_),
-                "hipMemcpyAsync");
-      hip_check(hipEventRecord(burst_done_[m & 1], stream_), "hipEventRecord");
+      if (!lists) {
+        hip_check(hipMemcpyAsync(hna + (m & 1), num_active_.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream_),
+                  "hipMemcpyAsync");
+        hip_check(hipEventRecord(burst_done_[m & 1], stream_), "hipEventRecord");
+      }
     }
     if (m > 0) {
-      hip_check(hipEventSynchronize(burst_done_[(m - 1) & 1]), "hipEventSynchronize");
-      if (hna[(m - 1) & 1] <= 0) break;
+      if (lists) {
+        if (!issue) break;  // burst m - 1 reached max_iter
+        hip_check(hipEventSynchronize(burst_done_[m & 1]), "hipEventSynchronize");
+        if (reinterpret_cast<volatile int32_t*>(hna)[m & 1] <= 0) break;  // left by burst m - 1
+      } else {
+        hip_check(hipEventSynchronize(burst_done_[(m - 1) & 1]), "hipEventSynchronize");
+        if (hna[(m - 1) & 1] <= 0) break;
+      }
     }
     if (!issue) break;
   }
-  env_cg_.download(h);
+  get(env_cg_, h);
   return h;
 }
 
@@ -698,7 +711,7 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
 template <class T>
 std::vector<double> World<T>::true_residual(const std::vector<int32_t>& mask) {
   std::vector<double> q(size_t(2 * nEnv_)), out(size_t(nEnv_), 0.0);
-  solve_mask_.upload_async(mask, stream_);
+  put(solve_mask_, mask);
   launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
   if (precision_ != SDX_PREC_MODE_FP32)
     launch_residual64<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, solve_mask_.p, d_row_ptr_.p,
@@ -707,7 +720,7 @@ std::vector<double> World<T>::true_residual(const std::vector<int32_t>& mask) {
     launch_residual64<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, solve_mask_.p, d_row_ptr_.p, d_col_.p,
                             vals_.p, nB_, xcg<T>(), res_.p, nullptr, acc_.p);
   launch_acc_finalize<double>(stream_, nEnv_, 2, acc_.p, norms2_.p);
-  norms2_.download(q);
+  get(norms2_, q);
   for (int e = 0; e < nEnv_; ++e)
     if (mask[size_t(e)] && q[size_t(2 * e + 1)] > 0) out[size_t(e)] = std::sqrt(q[size_t(2 * e)] / q[size_t(2 * e + 1)]);
   return out;
@@ -750,8 +763,8 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
   if (learned) {
     learned->accepted.assign(size_t(nEnv_), 0);
     learned->ratio.assign(size_t(nEnv_), 0.0);
-    warm_mask_.download(learned->accepted);
-    ws_ratio_.download(learned->ratio);
+    get(warm_mask_, learned->accepted);
+    get(ws_ratio_, learned->ratio);
   }
   std::vector<double> tr = true_residual(solve);
   const bool confirm = precision_ == SDX_PREC_MODE_MIXED && kp.norm_type == SDX_NORM_RESIDUAL_L2;
@@ -913,7 +926,7 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       launch_env_norms<T>(stream_, dt_v_.view, nEnv_, u_.p, dx_.p, acc_.p);
       launch_acc_finalize<double>(stream_, nEnv_, 2, acc_.p, norms2_.p);
       n2.resize(size_t(2 * nE));
-      norms2_.download(n2);
+      get(norms2_, n2);
     }
     int n_active = 0;
     for (int e = 0; e < nE; ++e) {
@@ -982,7 +995,7 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       launch_env_norms<T>(stream_, dt_v_.view, nEnv_, dx_.p, dx_.p, acc_.p);
       launch_acc_finalize<double>(stream_, nEnv_, 2, acc_.p, norms2_.p);
       std::vector<double> m2(size_t(2 * nE));
-      norms2_.download(m2);
+      get(norms2_, m2);
       for (int e = 0; e < nE; ++e)
         if (S[size_t(e)].active) S[size_t(e)].dx_norm = std::sqrt(m2[size_t(2 * e)]);
     }
@@ -998,8 +1011,8 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       if (it > 0)
         for (int e = 0; e < nE; ++e)
           if (searching[size_t(e)]) scale[size_t(e)] *= sp.line_search_alpha;
-      ls_scale_.upload_async(scale, stream_);
-      ls_mask_.upload_async(searching, stream_);
+      put(ls_scale_, scale);
+      put(ls_mask_, searching);
       launch_ls_update<T>(stream_, nV_, d_row_env_.p, ls_mask_.p, ls_scale_.p, sol0_.p, dx_.p, u_.p);
       // residual at the trial point (also the explosion-control residual and
       // the next iteration's residual: same u, same assembly)
@@ -1047,7 +1060,7 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
   // PostStage: v = (u - u0) / dt, reset on divergence (mochi_soft.cpp:135-163)
   std::vector<int32_t> reset(size_t(nE), 0);
   for (int e = 0; e < nE; ++e) reset[size_t(e)] = S[size_t(e)].conv == SDX_CONV_DIVERGED ? 1 : 0;
-  reset_.upload_async(reset, stream_);
+  put(reset_, reset);
   launch_post_step<T>(stream_, nV_, d_row_env_.p, reset_.p, dt, u0_.p, u_.p, v_.p, prev_dx_.p);
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   flush_timers();
