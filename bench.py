@@ -206,6 +206,9 @@ def main(argv=None) -> int:
     ap.add_argument("--warm-start", choices=["learned", "zero"], default="learned",
                     help="learned: the trained network of project_superdex_amd/weights (zero if absent)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--schedule", default=None,
+                    help="fixed launch schedule instead of the autotuner, e.g. "
+                         "spmv_lanes=4,cg_burst=4,use_graphs=0,element_block=128,active_lists=1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU PCG comparison")
     ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP64 position-error run")
@@ -248,6 +251,9 @@ def main(argv=None) -> int:
             learned = True
 
     # warm-up (the first step autotunes the launch schedule)
+    if args.schedule:
+        sc.set_schedule(**{k: int(v) for k, v in (kv.split("=") for kv in args.schedule.split(","))})
+        autotune = False
     run_steps(sc, rigid, spec, solver_params(spec, mode, autotune, learned), 0, 1)
     params = solver_params(spec, mode, learned=learned)
     run_steps(sc, rigid, spec, params, 1, warmup)
