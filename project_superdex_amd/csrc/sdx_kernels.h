@@ -177,7 +177,7 @@ void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int gr
                       EnvCG* env, A* acc, int32_t* num_active, const KernelEvents* ev = nullptr);
 // Work lists of the tiles of active environments (slot `par` of counts,
 // which must start at zero; slot par ^ 1 is cleared for the next burst).
-// publish (may be NULL; device-visible pinned host memory): publish[par] =
+// publish (may be NULL; device-visible pinned host memory): *publish =
 // *num_active, the active count left by the previous kernels.
 void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, const EnvCG* env, int32_t* la,
                           int32_t* lv, int32_t* counts, int par, const int32_t* num_active, int32_t* publish);

@@ -1,15 +1,18 @@
 // HIP kernels of the implicit soft-body FEM step for gfx950 (CDNA4).
 //
 // Data layout in HBM (V vertices, T tets, F boundary triangles, B BSR blocks):
-//   DoF vectors ............ [3V] node-major xyz, FP32 (FP64 in fp64 mode)
+//   DoF vectors ............ [3V] node-major xyz, FP32 (FP64 in fp64 mode);
+//                            z padded to [4V] for one-load neighbour gathers
 //   tet geometry ........... 14 SoA planes of T reals (coalesced per tet)
-//   element outputs ........ Re [T][12], He [T][80] (78 packed Hessian entries)
-//   BSR values ............. 9 SoA planes of B reals (vals[c*B + slot]); one
-//                            matvec lane per block reads each plane coalesced
-//   block-Jacobi inverse ... 9 SoA planes of V reals
-// Assembly is atomic-free: every BSR slot / vertex owns a precomputed,
-// sorted list of (element, local block) contributions and sums it in a
-// fixed order, so results are deterministic.
+//   records ................ element and contact contributions in destination
+//                            order: residual [4T + 3F][3] and diagonal blocks
+//                            [4T + 3F][6] vertex-sorted, off-diagonal blocks
+//                            [6T + 3F][9] sorted by (min, max) vertex pair
+//   BSR values ............. 9 SoA planes of B reals (vals[c*B + slot])
+//   block-Jacobi inverse ... 6 SoA planes of V reals (symmetric block)
+// Assembly is atomic-free: the element and contact kernels write every
+// contribution to its record, and every BSR slot / vertex sums one contiguous
+// run of records in a fixed order, so results are deterministic.
 //
 // CG (pcg.h:70-238, Polak-Ribiere) takes two kernels per iteration:
 //   cg_spmv:   w = A z (the only matvec), partial z.w
@@ -446,7 +449,7 @@ __global__ void __launch_bounds__(64 * kGatherWaves) k_gather_bsr(MeshView m, co
         o[0] = o[4] = o[8] = T(1);
       }
     }
-    dinv[(size_t)q * m.nV + (ib[t].x & ((1 << kItemRowBits) - 1))] = o[q];
+    if (r <= c) dinv[(size_t)sym_entry(r, c) * m.nV + (ib[t].x & ((1 << kItemRowBits) - 1))] = o[q];
   }
 }
 
@@ -473,8 +476,10 @@ __global__ void __launch_bounds__(256) k_block_inverse(int nV, const int32_t* __
       o[0] = o[4] = o[8] = T(1);
     }
   }
+  // symmetric storage, 6 planes 00 11 22 01 02 12 (upper triangle of the inverse)
+  const T sym[6] = {o[0], o[4], o[8], o[1], o[2], o[5]};
 #pragma unroll
-  for (int k = 0; k < 9; ++k) dinv[(size_t)k * nV + i] = o[k];
+  for (int k = 0; k < 6; ++k) dinv[(size_t)k * nV + i] = sym[k];
 }
 
 // One thread per vertex: residual = sum of its element and contact records,
@@ -612,19 +617,22 @@ __device__ __forceinline__ void store_z(T* z, int i, T a, T b, T c) {
   reinterpret_cast<Z4<T>*>(z)[i] = q;
 }
 
+// z = D^-1 r with the inverse of the (symmetric) diagonal block stored as 6
+// planes 00 11 22 01 02 12; Jacobi keeps the reciprocal diagonal in planes 0-2.
 template <class T>
 __device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int nV, int i, int prec, T r0, T r1, T r2,
                                            T& z0, T& z1, T& z2) {
+  const T* d = dinv + i;
+  const size_t s = nV;
   if (prec == 2) {
-    const T* d = dinv + i;
-    const size_t s = nV;
-    z0 = d[0] * r0 + d[1 * s] * r1 + d[2 * s] * r2;
-    z1 = d[3 * s] * r0 + d[4 * s] * r1 + d[5 * s] * r2;
-    z2 = d[6 * s] * r0 + d[7 * s] * r1 + d[8 * s] * r2;
+    const T d00 = d[0], d11 = d[s], d22 = d[2 * s], d01 = d[3 * s], d02 = d[4 * s], d12 = d[5 * s];
+    z0 = d00 * r0 + d01 * r1 + d02 * r2;
+    z1 = d01 * r0 + d11 * r1 + d12 * r2;
+    z2 = d02 * r0 + d12 * r1 + d22 * r2;
   } else if (prec == 1) {
-    z0 = dinv[i] * r0;
-    z1 = dinv[4 * (size_t)nV + i] * r1;
-    z2 = dinv[8 * (size_t)nV + i] * r2;
+    z0 = d[0] * r0;
+    z1 = d[s] * r1;
+    z2 = d[2 * s] * r2;
   } else {
     z0 = r0;
     z1 = r1;
@@ -1058,7 +1066,7 @@ __global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, const
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     counts[par ^ 1] = 0;
     counts[2 + (par ^ 1)] = 0;
-    if (publish) publish[par] = *num_active;  // pinned host memory, read after this kernel's event
+    if (publish) *publish = *num_active;  // pinned host memory, read after this kernel's event
   }
   const bool keep = i < t.n && env[t.env[i]].status == kCgActive;
   const unsigned long long m = __ballot(keep);

@@ -500,14 +500,45 @@ void launch_ws_env_scale(hipStream_t s, int nEnv, const double* acc, double* sca
   hipLaunchKernelGGL(k_ws_env_scale, dim3((nEnv + 63) / 64), dim3(64), 0, s, nEnv, acc, scale);
 }
 
+// Four lanes per vertex: the 1-ring loads of a row are split over the lanes
+// and issued together (a row holds about 13 neighbours); lane 0 writes the
+// feature row.  A 256-row vector tile takes four workgroups.
 template <class T>
-__global__ void __launch_bounds__(kVecThreads) k_ws_features(MeshView m, Tiles tv, const T* __restrict__ vals,
-                                                             const float* __restrict__ y, const T* __restrict__ dprev,
-                                                             const double* __restrict__ scale, float* __restrict__ F) {
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(256) k_ws_features(MeshView m, Tiles tv, const T* __restrict__ vals,
+                                                     const float* __restrict__ y, const T* __restrict__ dprev,
+                                                     const double* __restrict__ scale, float* __restrict__ F) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = b >> 2, quarter = b & 3;
   const int e = tv.env[tile];
-  const int i = tv.row0[tile] + threadIdx.x;
-  if (i >= tv.row1[tile]) return;
+  const int i = tv.row0[tile] + quarter * 64 + (int(threadIdx.x) >> 2);
+  const int lane = threadIdx.x & 3;
+  const bool valid = i < tv.row1[tile];
+  float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
+  int cnt = 0;
+  if (valid) {
+    const int k0 = m.row_ptr[i], k1 = m.row_ptr[i + 1];
+    for (int k = k0 + lane; k < k1; k += 16) {
+      int jj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) jj[u] = k + 4 * u < k1 ? m.col[k + 4 * u] : i;  // i: skipped
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (jj[u] == i) continue;
+        n0 += y[3 * jj[u]];
+        n1 += y[3 * jj[u] + 1];
+        n2 += y[3 * jj[u] + 2];
+        ++cnt;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 4; off <<= 1) {
+    n0 += __shfl_xor(n0, off, 4);
+    n1 += __shfl_xor(n1, off, 4);
+    n2 += __shfl_xor(n2, off, 4);
+    cnt += __shfl_xor(cnt, off, 4);
+  }
+  if (!valid || lane != 0) return;
   const float invs = float(1.0 / scale[e]);
   const int sl = m.diag_slot[i];
   const size_t nB = m.nB;
@@ -515,16 +546,6 @@ __global__ void __launch_bounds__(kVecThreads) k_ws_features(MeshView m, Tiles t
   const float d01 = float(vals[1 * nB + sl]), d02 = float(vals[2 * nB + sl]), d12 = float(vals[5 * nB + sl]);
   const float d = (d00 + d11 + d22) / 3.0f;
   const float invd = d > 0.0f ? 1.0f / d : 0.0f;
-  float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
-  int cnt = 0;
-  for (int k = m.row_ptr[i]; k < m.row_ptr[i + 1]; ++k) {
-    const int j = m.col[k];
-    if (j == i) continue;
-    n0 += y[3 * j];
-    n1 += y[3 * j + 1];
-    n2 += y[3 * j + 2];
-    ++cnt;
-  }
   const float invc = cnt ? invs / float(cnt) : 0.0f;
   float4* out = reinterpret_cast<float4*>(F + (size_t)i * kMlpInPad);
   out[0] = make_float4(y[3 * i] * invs, y[3 * i + 1] * invs, y[3 * i + 2] * invs, d00 * invd);
@@ -538,7 +559,8 @@ template <class T>
 void launch_ws_features(hipStream_t s, const MeshView& m, const Tiles& tv, const T* vals, const float* y,
                         const T* dprev, const double* scale, float* F) {
   if (tv.n == 0) return;
-  hipLaunchKernelGGL(k_ws_features<T>, dim3(tv.n), dim3(kVecThreads), 0, s, m, tv, vals, y, dprev, scale, F);
+  static_assert(kVecThreads == 256, "four workgroups of 64 rows per vector tile");
+  hipLaunchKernelGGL(k_ws_features<T>, dim3(unsigned(tv.n) * 4u), dim3(256), 0, s, m, tv, vals, y, dprev, scale, F);
 }
 
 template <class T>

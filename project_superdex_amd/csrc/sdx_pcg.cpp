@@ -87,7 +87,7 @@ sdx_status solve(int32_t device, int32_t nbr, int32_t nnzb, const int32_t* ptr, 
     hx[k] = A(x[k]);
   }
   Tmp<int32_t> dptr(size_t(nbr) + 1), dcol(static_cast<size_t>(nnzb)), ddiag(static_cast<size_t>(nbr));
-  Tmp<T> dvals(vals.size()), ddinv(size_t(9) * size_t(nbr)), db(n3), dr(n3), dz(size_t(4) * size_t(nbr)), dp(n3),
+  Tmp<T> dvals(vals.size()), ddinv(size_t(6) * size_t(nbr)), db(n3), dr(n3), dz(size_t(4) * size_t(nbr)), dp(n3),
       dsv(n3), dw(n3);
   Tmp<A> dx(n3);
   Tmp<double> drref(n3);

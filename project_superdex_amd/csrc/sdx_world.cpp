@@ -415,7 +415,7 @@ void World<T>::finalize() {
   cdiag_.alloc(6 * nrec_v + 4);
   coff_.alloc(9 * nrec_e + 4);
   vals_.alloc(size_t(9) * size_t(nB_));
-  dinv_.alloc(size_t(9) * size_t(nV_));
+  dinv_.alloc(size_t(6) * size_t(nV_));
   env_cg_.alloc(static_cast<size_t>(nEnv));
   acc_.alloc(size_t(kCgSets) * size_t(nEnv) * size_t(kSlots) * size_t(kSlotStride));
   solve_mask_.alloc(static_cast<size_t>(nEnv));
@@ -663,45 +663,47 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
     launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, decide ? ws_alpha_.p : nullptr, dinv_.p, nV_,
                          xcg<T>(), r_.p, z_.p, env_cg_.p, accf);
   }
-  // Bursts of B iterations are issued one ahead of the host's check of the
-  // active count, so the device never waits for the host between bursts; a
-  // burst issued after every environment finished exits at once.  With work
-  // lists, burst m starts with the compaction, which also writes the count
-  // left by burst m - 1 into pinned host memory; the host waits for the
-  // compaction only.  Without lists the count is copied after each burst.
+  // Bursts of B iterations are issued kBurstAhead bursts ahead of the host's
+  // check of the active count, so the device does not wait for the host
+  // between bursts even when a burst of the last few environments takes less
+  // time than the host needs to issue the next one; bursts issued after every
+  // environment finished exit at once.  With work lists, burst m starts with
+  // the compaction, which also writes the count left by burst m - 1 into
+  // pinned host memory (slot m mod kBurstSlots) and records event m; without
+  // lists the count is copied after burst m and event m follows the copy.
+  // After issuing burst m the host reads the count left by burst
+  // m - kBurstAhead.
   const int B = std::max(1, sched_.cg_burst);
-  int32_t* hna = pinned_active_;
+  volatile int32_t* hna = pinned_active_;
   const bool lists = sched_.active_lists != 0;
   for (int m = 0;; ++m) {
     const bool issue = int64_t(m) * B <= int64_t(cfg.max_iter);
+    const int slot = m % kBurstSlots;
     if (issue) {
       const bool timed = burst_timed(m);
       if (lists) {
         launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, env_cg_.p, list_a_.p, list_v_.p, list_count_.p, m & 1,
-                             num_active_.p, pinned_active_dev_);
-        hip_check(hipEventRecord(burst_done_[m & 1], stream_), "hipEventRecord");
+                             num_active_.p, pinned_active_dev_ + slot);
+        hip_check(hipEventRecord(burst_done_[slot], stream_), "hipEventRecord");
       }
       if (sched_.use_graphs && !timed)
         hip_check(hipGraphLaunch(cg_graph(cfg, m, B), stream_), "hipGraphLaunch");
       else
         issue_cg(cfg, m, m * B, B, timed);
       if (!lists) {
-        hip_check(hipMemcpyAsync(hna + (m & 1), num_active_.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream_),
+        hip_check(hipMemcpyAsync(pinned_active_ + slot, num_active_.p, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 stream_),
                   "hipMemcpyAsync");
-        hip_check(hipEventRecord(burst_done_[m & 1], stream_), "hipEventRecord");
+        hip_check(hipEventRecord(burst_done_[slot], stream_), "hipEventRecord");
       }
     }
-    if (m > 0) {
-      if (lists) {
-        if (!issue) break;  // burst m - 1 reached max_iter
-        hip_check(hipEventSynchronize(burst_done_[m & 1]), "hipEventSynchronize");
-        if (reinterpret_cast<volatile int32_t*>(hna)[m & 1] <= 0) break;  // left by burst m - 1
-      } else {
-        hip_check(hipEventSynchronize(burst_done_[(m - 1) & 1]), "hipEventSynchronize");
-        if (hna[(m - 1) & 1] <= 0) break;
-      }
+    if (!issue) break;  // every burst up to max_iter is queued
+    const int j = m - kBurstAhead;  // check the count left by burst j
+    if (j >= 0) {
+      const int s = (lists ? j + 1 : j) % kBurstSlots;  // compaction of burst j + 1, or the copy after burst j
+      hip_check(hipEventSynchronize(burst_done_[s]), "hipEventSynchronize");
+      if (hna[s] <= 0) break;
     }
-    if (!issue) break;
   }
   get(env_cg_, h);
   return h;

@@ -98,10 +98,10 @@ class WorldBase {
     hip_check(hipSetDevice(device), "hipSetDevice");
     hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
     // coherent and mapped: the compaction kernel writes the active count here
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&pinned_active_), 2 * sizeof(int32_t),
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&pinned_active_), kBurstSlots * sizeof(int32_t),
                             hipHostMallocMapped | hipHostMallocCoherent),
               "hipHostMalloc");
-    pinned_active_[0] = pinned_active_[1] = 0;
+    for (int k = 0; k < kBurstSlots; ++k) pinned_active_[k] = 0;
     hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&pinned_active_dev_), pinned_active_, 0),
               "hipHostGetDevicePointer");
     for (auto& ev : burst_done_) hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
@@ -277,7 +277,10 @@ class WorldBase {
   int device_;
   int precision_;
   hipStream_t stream_ = nullptr;
-  int32_t* pinned_active_ = nullptr;  // two slots: active count after bursts m - 1, m
+  // CG bursts in flight ahead of the host's check of the active count, and the
+  // ring of count slots / events they use (at least kBurstAhead + 2)
+  static constexpr int kBurstAhead = 2, kBurstSlots = 4;
+  int32_t* pinned_active_ = nullptr;  // active-count slots, one per burst mod kBurstSlots
   int32_t* pinned_active_dev_ = nullptr;  // the same slots as a device pointer
   char* stage_ = nullptr;  // pinned upload ring
   size_t stage_cap_ = 0, stage_off_ = 0;
@@ -299,7 +302,7 @@ class WorldBase {
     stage_off_ += need;
     return p;
   }
-  hipEvent_t burst_done_[2] = {nullptr, nullptr};
+  hipEvent_t burst_done_[kBurstSlots] = {};
   std::vector<HostActor> actors_;
   std::vector<HostCollider> colliders_;
   bool colliders_dirty_ = true;
