@@ -156,11 +156,13 @@ void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, 
 // Vectors of the CG are [3 nV] except z, stored [4 nV] (padded per vertex).
 // r0, z0, scaling and R_0.  warm (may be NULL): environments that keep x as
 // the initial guess and take r0 from r64 = b - A x (launch_residual64);
-// alpha (may be NULL): per-environment factor applied to the kept guess.
+// alpha (may be NULL): per-environment factor applied to the kept guess;
+// v, rv, beta (all NULL or none): a second guess v with rv = b - A v
+// (launch_residual64_pair) and its factor, x0 = alpha x + beta v.
 template <class T, class A>
 void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const int32_t* warm,
-                    const double* r64, const double* alpha, const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env,
-                    A* acc);
+                    const double* r64, const double* alpha, const T* v, const double* rv, const double* beta,
+                    const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env, A* acc);
 // CG iteration k: w = A z (+ FP64 true residual on refresh iterations), with
 // `lanes` (4, 8, 16) lanes per block row and kSpmvThreads / lanes rows per
 // tile; over all tiles, or persistent (`grid` >= 8 workgroups) over a list.
@@ -188,6 +190,13 @@ template <class T, class X>
 void launch_residual64(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
                        const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* x,
                        const T* b, double* r64, double* acc);
+// Both FP64 residuals ru = b - A u and rv = b - A v of the environments with
+// mask[e], and into set 0 of acc (zeroed by the caller): |b|^2, u.b, v.b,
+// u.Au, v.Av, u.Av, b.Au, b.Av, |Au|^2, |Av|^2, Au.Av.
+template <class T, class X>
+void launch_residual64_pair(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
+                            const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* u,
+                            const T* v, const T* b, double* ru, double* rv, double* acc);
 // dst[i] = D(src[i])
 template <class S, class D>
 void launch_convert(hipStream_t s, int64_t n, const S* src, D* dst);

@@ -680,7 +680,9 @@ template <class T, class A>
 __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg, const T* __restrict__ b,
                                                          const int32_t* __restrict__ warm,
                                                          const double* __restrict__ r64,
-                                                         const double* __restrict__ alpha, const T* __restrict__ dinv,
+                                                         const double* __restrict__ alpha,
+                                                         const T* __restrict__ v, const double* __restrict__ rv,
+                                                         const double* __restrict__ beta, const T* __restrict__ dinv,
                                                          int nV, A* __restrict__ x, T* __restrict__ r,
                                                          T* __restrict__ z, EnvCG* env, A* acc) {
   const int tile = blockIdx.x;
@@ -694,7 +696,15 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg,
     const T b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
     T r0 = b0, r1 = b1, r2 = b2;
     if (w) {
-      if (alpha) {  // x0 <- a x0, so r0 = b - a A x0 = a r64 + (1 - a) b
+      if (alpha && v) {  // x0 <- a x0 + c v: r0 = a (b - A x0) + c (b - A v) + (1 - a - c) b
+        const double a = alpha[e], c = beta[e], o = 1.0 - a - c;
+        r0 = T(a * r64[3 * i] + c * rv[3 * i] + o * double(b0));
+        r1 = T(a * r64[3 * i + 1] + c * rv[3 * i + 1] + o * double(b1));
+        r2 = T(a * r64[3 * i + 2] + c * rv[3 * i + 2] + o * double(b2));
+        x[3 * i] = A(a * double(x[3 * i]) + c * double(v[3 * i]));
+        x[3 * i + 1] = A(a * double(x[3 * i + 1]) + c * double(v[3 * i + 1]));
+        x[3 * i + 2] = A(a * double(x[3 * i + 2]) + c * double(v[3 * i + 2]));
+      } else if (alpha) {  // x0 <- a x0, so r0 = b - a A x0 = a r64 + (1 - a) b
         const double a = alpha[e];
         r0 = T(a * r64[3 * i] + (1.0 - a) * double(b0));
         r1 = T(a * r64[3 * i + 1] + (1.0 - a) * double(b1));
@@ -1136,6 +1146,77 @@ __global__ void __launch_bounds__(kSpmvThreads) k_residual64(Tiles ta, int nEnv,
   if (acc) acc_add<double, 4>(part, acc_ptr(acc, 0, nEnv, e), tile - ta.first[e]);
 }
 
+// The two FP64 residuals b - A u and b - A v of two initial guesses in one
+// pass over the matrix, for the environments with mask[e]: into ru, rv, and
+// eleven sums into set 0 of acc (zeroed by the caller): |b|^2; u.b, v.b,
+// u.Au, v.Av, u.Av (the Galerkin system of the combination a u + c v with the
+// least energy-norm error); b.Au, b.Av, |Au|^2, |Av|^2, Au.Av (its residual).
+template <class T, class X, int L>
+__global__ void __launch_bounds__(kSpmvThreads) k_residual64_pair(Tiles ta, int nEnv, const int32_t* __restrict__ mask,
+                                                                  const int32_t* __restrict__ row_ptr,
+                                                                  const int32_t* __restrict__ col,
+                                                                  const T* __restrict__ vals, int nB,
+                                                                  const X* __restrict__ u, const T* __restrict__ v,
+                                                                  const T* __restrict__ b, double* __restrict__ ru,
+                                                                  double* __restrict__ rv, double* acc) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int e = ta.env[tile];
+  if (!mask[e]) return;
+  const int i = ta.row0[tile] + (threadIdx.x / L);
+  const int lane = threadIdx.x % L;
+  const bool valid = i < ta.row1[tile];
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0, q0 = 0.0, q1 = 0.0, q2 = 0.0;
+  if (valid) {
+    for (int kk = row_ptr[i] + lane; kk < row_ptr[i + 1]; kk += L) {
+      const int jc = col[kk];
+      const double u0 = u[3 * jc], u1 = u[3 * jc + 1], u2 = u[3 * jc + 2];
+      const double v0 = v[3 * jc], v1 = v[3 * jc + 1], v2 = v[3 * jc + 2];
+      double a[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) a[c] = double(vals[(size_t)c * nB + kk]);
+      p0 += a[0] * u0 + a[1] * u1 + a[2] * u2;
+      p1 += a[3] * u0 + a[4] * u1 + a[5] * u2;
+      p2 += a[6] * u0 + a[7] * u1 + a[8] * u2;
+      q0 += a[0] * v0 + a[1] * v1 + a[2] * v2;
+      q1 += a[3] * v0 + a[4] * v1 + a[5] * v2;
+      q2 += a[6] * v0 + a[7] * v1 + a[8] * v2;
+    }
+  }
+#pragma unroll
+  for (int off = L / 2; off >= 1; off >>= 1) {
+    p0 += __shfl_xor(p0, off, L);
+    p1 += __shfl_xor(p1, off, L);
+    p2 += __shfl_xor(p2, off, L);
+    q0 += __shfl_xor(q0, off, L);
+    q1 += __shfl_xor(q1, off, L);
+    q2 += __shfl_xor(q2, off, L);
+  }
+  double part[11] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (valid && lane == 0) {
+    const double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
+    const double ui0 = u[3 * i], ui1 = u[3 * i + 1], ui2 = u[3 * i + 2];
+    const double vi0 = v[3 * i], vi1 = v[3 * i + 1], vi2 = v[3 * i + 2];
+    ru[3 * i] = b0 - p0;
+    ru[3 * i + 1] = b1 - p1;
+    ru[3 * i + 2] = b2 - p2;
+    rv[3 * i] = b0 - q0;
+    rv[3 * i + 1] = b1 - q1;
+    rv[3 * i + 2] = b2 - q2;
+    part[0] = b0 * b0 + b1 * b1 + b2 * b2;
+    part[1] = ui0 * b0 + ui1 * b1 + ui2 * b2;
+    part[2] = vi0 * b0 + vi1 * b1 + vi2 * b2;
+    part[3] = ui0 * p0 + ui1 * p1 + ui2 * p2;
+    part[4] = vi0 * q0 + vi1 * q1 + vi2 * q2;
+    part[5] = ui0 * q0 + ui1 * q1 + ui2 * q2;
+    part[6] = b0 * p0 + b1 * p1 + b2 * p2;
+    part[7] = b0 * q0 + b1 * q1 + b2 * q2;
+    part[8] = p0 * p0 + p1 * p1 + p2 * p2;
+    part[9] = q0 * q0 + q1 * q1 + q2 * q2;
+    part[10] = p0 * q0 + p1 * q1 + p2 * q2;
+  }
+  acc_add<double, 11>(part, acc_ptr(acc, 0, nEnv, e), tile - ta.first[e]);
+}
+
 template <class T, int L>
 __global__ void __launch_bounds__(kSpmvThreads) k_spmv_plain(Tiles ta, const int32_t* __restrict__ row_ptr,
                                                              const int32_t* __restrict__ col,
@@ -1323,11 +1404,11 @@ void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, 
 
 template <class T, class A>
 void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const int32_t* warm,
-                    const double* r64, const double* alpha, const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env,
-                    A* acc) {
+                    const double* r64, const double* alpha, const T* v, const double* rv, const double* beta,
+                    const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env, A* acc) {
   if (tv.n == 0) return;
-  hipLaunchKernelGGL((k_cg_init<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, b, warm, r64, alpha, dinv, nV,
-                     x, r, z, env, acc);
+  hipLaunchKernelGGL((k_cg_init<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, b, warm, r64, alpha, v, rv,
+                     beta, dinv, nV, x, r, z, env, acc);
 }
 
 template <class T, class A>
@@ -1389,6 +1470,18 @@ void launch_residual64(hipStream_t s, const Tiles& ta, int lanes, int nEnv, cons
                            : k_residual64<T, X, 16>;
   hipLaunchKernelGGL(kern, dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, nEnv, mask, row_ptr, col, vals, nB, x, b, r64,
                      acc);
+}
+
+template <class T, class X>
+void launch_residual64_pair(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
+                            const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* u,
+                            const T* v, const T* b, double* ru, double* rv, double* acc) {
+  if (ta.n == 0) return;
+  auto kern = lanes == 4   ? k_residual64_pair<T, X, 4>
+              : lanes == 8 ? k_residual64_pair<T, X, 8>
+                           : k_residual64_pair<T, X, 16>;
+  hipLaunchKernelGGL(kern, dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, nEnv, mask, row_ptr, col, vals, nB, u, v, b, ru,
+                     rv, acc);
 }
 
 template <class S, class D>
@@ -1484,7 +1577,11 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
 #define SDX_INST_TA(T, A)                                                                                         \
   template void launch_gather_res<T, A>(hipStream_t, const MeshView&, const Tiles&, const T*, T*, A*);          \
   template void launch_cg_init<T, A>(hipStream_t, const Tiles&, const CgConfig&, const T*, const int32_t*,       \
-                                     const double*, const double*, const T*, int32_t, A*, T*, T*, EnvCG*, A*);   \
+                                     const double*, const double*, const T*, const double*, const double*,      \
+                                     const T*, int32_t, A*, T*, T*, EnvCG*, A*);                                 \
+  template void launch_residual64_pair<T, A>(hipStream_t, const Tiles&, int, int, const int32_t*,              \
+                                             const int32_t*, const int32_t*, const T*, int32_t, const A*,       \
+                                             const T*, const T*, double*, double*, double*);                    \
   template void launch_residual64<T, A>(hipStream_t, const Tiles&, int, int, const int32_t*, const int32_t*,    \
                                         const int32_t*, const T*, int32_t, const A*, const T*, double*,         \
                                         double*);                                                               \

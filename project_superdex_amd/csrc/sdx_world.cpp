@@ -408,6 +408,7 @@ void World<T>::finalize() {
   sv_.alloc(n3);
   w_.alloc(n3);
   rref_.alloc(n3);
+  rref2_.alloc(n3);
   // padded by one 16-byte vector: the gathers stage these with vector loads
   const size_t nrec_v = size_t(4) * size_t(nT_) + size_t(3) * size_t(nF_);
   const size_t nrec_e = size_t(6) * size_t(nT_) + size_t(3) * size_t(nF_);
@@ -428,6 +429,7 @@ void World<T>::finalize() {
   ws_scale_.alloc(static_cast<size_t>(nEnv));
   ws_ratio_.alloc(static_cast<size_t>(nEnv));
   ws_alpha_.alloc(static_cast<size_t>(nEnv));
+  ws_beta_.alloc(static_cast<size_t>(nEnv));
   d_env_v0_.upload(env_v0_);
   num_active_.alloc(1);
   env_mask_.alloc(static_cast<size_t>(nEnv));
@@ -638,30 +640,49 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
   put(num_active_, na);
   const int32_t* wm = n_warm ? warm_mask_.p : nullptr;
   launch_fill<int32_t>(stream_, 4, 0, list_count_.p);
-  // decide: keep an environment's guess only if |b - A x0| < |b| (on device)
-  double* dacc = decide ? acc_.p : nullptr;
+  // decide (the learned guess u in x): the combination a u + c v with the
+  // previous Newton step v of least residual |b - A x0|, kept only where it
+  // removes most of the residual (on device)
   if (n_warm && decide) launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
+  const T* v = decide ? prev_dx_.p : nullptr;
+  const double* rv = decide ? rref2_.p : nullptr;
+  const double* wa = decide ? ws_alpha_.p : nullptr;
+  const double* wb = decide ? ws_beta_.p : nullptr;
   if (precision_ != SDX_PREC_MODE_FP32) {
     if (n_warm) {
       put(warm_mask_, warm);
-      launch_residual64<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
-                                   d_col_.p, vals_.p, nB_, xcg<double>(), res_.p, rref_.p, dacc);
-      if (decide) launch_ws_decide(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p);
+      if (decide) {
+        launch_residual64_pair<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
+                                          d_col_.p, vals_.p, nB_, xcg<double>(), prev_dx_.p, res_.p, rref_.p,
+                                          rref2_.p, acc_.p);
+        launch_ws_decide_pair(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p,
+                              ws_beta_.p);
+      } else {
+        launch_residual64<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
+                                     d_col_.p, vals_.p, nB_, xcg<double>(), res_.p, rref_.p, nullptr);
+      }
     }
     launch_cg_reset<double>(stream_, nEnv_, solve_mask_.p, env_cg_.p, acc_.p);
-    launch_cg_init<T, double>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, decide ? ws_alpha_.p : nullptr,
-                              dinv_.p, nV_, xcg<double>(), r_.p, z_.p, env_cg_.p, acc_.p);
+    launch_cg_init<T, double>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, wa, v, rv, wb, dinv_.p, nV_,
+                              xcg<double>(), r_.p, z_.p, env_cg_.p, acc_.p);
   } else {
     if (n_warm) {
       put(warm_mask_, warm);
-      launch_residual64<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p, d_col_.p,
-                              vals_.p, nB_, xcg<T>(), res_.p, rref_.p, dacc);
-      if (decide) launch_ws_decide(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p);
+      if (decide) {
+        launch_residual64_pair<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
+                                     d_col_.p, vals_.p, nB_, xcg<T>(), prev_dx_.p, res_.p, rref_.p, rref2_.p,
+                                     acc_.p);
+        launch_ws_decide_pair(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p,
+                              ws_beta_.p);
+      } else {
+        launch_residual64<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p, d_col_.p,
+                                vals_.p, nB_, xcg<T>(), res_.p, rref_.p, nullptr);
+      }
     }
     T* accf = reinterpret_cast<T*>(acc_.p);
     launch_cg_reset<T>(stream_, nEnv_, solve_mask_.p, env_cg_.p, accf);
-    launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, decide ? ws_alpha_.p : nullptr, dinv_.p, nV_,
-                         xcg<T>(), r_.p, z_.p, env_cg_.p, accf);
+    launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, wa, v, rv, wb, dinv_.p, nV_, xcg<T>(), r_.p,
+                         z_.p, env_cg_.p, accf);
   }
   // Bursts of B iterations are issued kBurstAhead bursts ahead of the host's
   // check of the active count, so the device does not wait for the host

@@ -417,6 +417,7 @@ class World final : public WorldBase {
   SDX_BUF(T, sv_)
   SDX_BUF(T, w_)
   SDX_BUF(double, rref_)
+  SDX_BUF(double, rref2_)  // b - A v of the previous Newton step v (learned warm start)
   SDX_BUF(T, cres_)   // element and contact residual records, vertex-sorted [4 nT + 3 nF][3]
   SDX_BUF(T, cdiag_)  // diagonal-block records, vertex-sorted [4 nT + 3 nF][6]
   SDX_BUF(T, coff_)   // off-diagonal-block records, pair-sorted [6 nT + 3 nF][9]
@@ -432,6 +433,7 @@ class World final : public WorldBase {
   SDX_BUF(double, ws_scale_)     // per-environment scale of features and output
   SDX_BUF(double, ws_ratio_)     // relative residual of the learned guess
   SDX_BUF(double, ws_alpha_)     // least-residual multiple of the learned guess
+  SDX_BUF(double, ws_beta_)      // and of the previous Newton step
   SDX_BUF(int32_t, d_env_v0_)    // first vertex of each environment [nEnv+1]
   SDX_BUF(int32_t, list_a_)      // work list of active matvec tiles
   SDX_BUF(int32_t, list_v_)      // work list of active vector tiles

@@ -8,12 +8,15 @@
 - the feature rows and targets the world collects equal the oracle's
   features of the oracle-assembled system and its exact Newton step;
 - on a held-out seed a trained model predicts the Newton step better than
-  the zero guess; the CG iterations with and without it are reported (the
-  gate keeps a guess only where it removes 80% of the residual, so the count
-  may not grow by more than 3%) and the result stays that of the plain solver
-  (positions within 1e-3 of the largest displacement); a bad model is
-  rejected environment by environment, leaving the plain result and
-  iteration counts.
+  the zero guess; the warm start (the least-residual combination of the
+  network's guess and the previous Newton step, kept where it removes 80% of
+  the residual) saves at least 5% of the CG iterations, the counts are
+  reported, and the result stays that of the plain solver (positions within
+  1e-3 of the largest displacement);
+- a network of noise cannot make the solve cost more than the plain method:
+  its iterations stay within 2% of the plain method's (the iterations of a
+  network that outputs zero, whose warm start is the previous step alone, are
+  reported beside them), and the result is the plain solver's.
 """
 import numpy as np
 import pytest
@@ -140,11 +143,10 @@ def trained_model():
 
 
 def test_learned_warm_start_on_held_out_seed(trained_model):
-    """What the trained network gains on a held-out seed: its guesses beat the
-    zero guess, and the CG iteration count with the learned warm start (kept
-    only where its best multiple removes 80% of the residual) is reported and
-    may not exceed the plain method's by more than 3%; the result is the plain
-    solver's."""
+    """What the warm start gains on a held-out seed: the network's guesses beat
+    the zero guess, and the CG iteration count with the learned warm start
+    (network guess combined with the previous Newton step) is reported and at
+    least 5% below the plain method's; the result is the plain solver's."""
     spec = batched(100, 16)  # held-out seed
     zero = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
     learned = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
@@ -162,8 +164,8 @@ def test_learned_warm_start_on_held_out_seed(trained_model):
     print(f"held-out: |net - dx|/|dx| = {err:.3f}; CG iterations zero init {it0}, learned {it1} "
           f"(x{it1 / it0:.3f}); guesses kept {kept}/{tried}, median residual ratio {ratio:.3f}")
     assert err < 0.9  # the network predicts the Newton step better than zero does
-    assert tried == 16 * 16 and ratio < 1.0
-    assert it1 <= 1.03 * it0, (it0, it1)
+    assert tried == 16 * 16 and ratio < 0.2 and kept > tried // 2
+    assert it1 <= 0.95 * it0, (it0, it1)
     for st in st1:
         for e in st:
             assert e["linear_convergence"] == _abi.CONV_CONVERGED and e["linear_true_rel_res"] <= 1e-6
@@ -172,21 +174,27 @@ def test_learned_warm_start_on_held_out_seed(trained_model):
         assert np.abs(ua - ub).max() <= 1e-3 * np.abs(ua).max()
 
 
-def test_bad_model_falls_back_to_zero_init():
+def test_noise_model_costs_no_more_than_plain():
     spec = batched(101, 6)
     zero = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
     learned = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
     learned.warm_start = _abi.WARM_START_LEARNED
     bad = WarmStartModel(seed=3)
-    bad.params = bad.params * 8.0  # guesses of vertex-to-vertex noise: no multiple of them helps
+    bad.params = bad.params * 8.0  # guesses of vertex-to-vertex noise
+    null = WarmStartModel(seed=3)
+    null.params = np.zeros_like(null.params)  # outputs zero: the previous step alone
 
     s0, st0 = run(spec, zero)
     s1, st1 = run(spec, learned, model=bad)
-    assert sum(e["warm_start_accepted"] for st in st1 for e in st) == 0
-    assert all(e["warm_start_rel_res"] >= 0.2 for st in st1 for e in st)
-    for a, b in zip(st0, st1):
-        for ea, eb in zip(a, b):
-            assert abs(ea["linear_iters"] - eb["linear_iters"]) <= 2
+    s2, st2 = run(spec, learned, model=null)
+    it0 = sum(e["linear_iters"] for st in st0 for e in st)
+    it1 = sum(e["linear_iters"] for st in st1 for e in st)
+    it2 = sum(e["linear_iters"] for st in st2 for e in st)
+    print(f"noise network: CG iterations {it1}, zero network {it2}, plain {it0}")
+    assert it1 <= 1.02 * it0, (it1, it0)
+    for st in st1:
+        for e in st:
+            assert e["linear_convergence"] == _abi.CONV_CONVERGED and e["linear_true_rel_res"] <= 1e-6
     for a, b in zip(s0.actors, s1.actors):
         ua, ub = a.displacements(), b.displacements()
-        assert np.abs(ua - ub).max() <= 1e-4 * np.abs(ua).max()
+        assert np.abs(ua - ub).max() <= 1e-3 * np.abs(ua).max()
