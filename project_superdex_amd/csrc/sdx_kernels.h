@@ -157,12 +157,14 @@ void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, 
 // r0, z0, scaling and R_0.  warm (may be NULL): environments that keep x as
 // the initial guess and take r0 from r64 = b - A x (launch_residual64);
 // alpha (may be NULL): per-environment factor applied to the kept guess;
-// v, rv, beta (all NULL or none): a second guess v with rv = b - A v
-// (launch_residual64_pair) and its factor, x0 = alpha x + beta v.
+// v, rv, beta, w, rw, gamma (all NULL or none): two more guesses with their
+// residuals (launch_residual64_multi) and factors, x0 = alpha x + beta v +
+// gamma w.
 template <class T, class A>
 void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const int32_t* warm,
                     const double* r64, const double* alpha, const T* v, const double* rv, const double* beta,
-                    const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env, A* acc);
+                    const T* w, const double* rw, const double* gamma, const T* dinv, int32_t nV, A* x, T* r, T* z,
+                    EnvCG* env, A* acc);
 // CG iteration k: w = A z (+ FP64 true residual on refresh iterations), with
 // `lanes` (4, 8, 16) lanes per block row and kSpmvThreads / lanes rows per
 // tile; over all tiles, or persistent (`grid` >= 8 workgroups) over a list.
@@ -190,13 +192,13 @@ template <class T, class X>
 void launch_residual64(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
                        const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* x,
                        const T* b, double* r64, double* acc);
-// Both FP64 residuals ru = b - A u and rv = b - A v of the environments with
-// mask[e], and into set 0 of acc (zeroed by the caller): |b|^2, u.b, v.b,
-// u.Au, v.Av, u.Av, b.Au, b.Av, |Au|^2, |Av|^2, Au.Av.
+// The FP64 residuals ru = b - A u, rv = b - A v, rw = b - A w of the
+// environments with mask[e], and into set 0 of acc (zeroed by the caller):
+// |b|^2, b.Au, b.Av, b.Aw, |Au|^2, |Av|^2, |Aw|^2, Au.Av, Au.Aw, Av.Aw.
 template <class T, class X>
-void launch_residual64_pair(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
-                            const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* u,
-                            const T* v, const T* b, double* ru, double* rv, double* acc);
+void launch_residual64_multi(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
+                             const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* u,
+                             const T* v, const T* w, const T* b, double* ru, double* rv, double* rw, double* acc);
 // dst[i] = D(src[i])
 template <class S, class D>
 void launch_convert(hipStream_t s, int64_t n, const S* src, D* dst);
@@ -222,7 +224,7 @@ void launch_ls_update(hipStream_t s, int32_t nV, const int32_t* row_env, const i
 // prev = 0 when given).
 template <class T>
 void launch_post_step(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* reset, double dt,
-                      const T* u0, T* u, T* v, T* prev = nullptr);
+                      const T* u0, T* u, T* v, T* prev = nullptr, T* prev2 = nullptr);
 // dst = src on the rows of environments with mask[env].
 template <class T>
 void launch_env_copy(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* mask, const T* src, T* dst);

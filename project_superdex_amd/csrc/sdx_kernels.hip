@@ -682,7 +682,9 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg,
                                                          const double* __restrict__ r64,
                                                          const double* __restrict__ alpha,
                                                          const T* __restrict__ v, const double* __restrict__ rv,
-                                                         const double* __restrict__ beta, const T* __restrict__ dinv,
+                                                         const double* __restrict__ beta, const T* __restrict__ w2,
+                                                         const double* __restrict__ rw,
+                                                         const double* __restrict__ gamma, const T* __restrict__ dinv,
                                                          int nV, A* __restrict__ x, T* __restrict__ r,
                                                          T* __restrict__ z, EnvCG* env, A* acc) {
   const int tile = blockIdx.x;
@@ -696,14 +698,14 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg,
     const T b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
     T r0 = b0, r1 = b1, r2 = b2;
     if (w) {
-      if (alpha && v) {  // x0 <- a x0 + c v: r0 = a (b - A x0) + c (b - A v) + (1 - a - c) b
-        const double a = alpha[e], c = beta[e], o = 1.0 - a - c;
-        r0 = T(a * r64[3 * i] + c * rv[3 * i] + o * double(b0));
-        r1 = T(a * r64[3 * i + 1] + c * rv[3 * i + 1] + o * double(b1));
-        r2 = T(a * r64[3 * i + 2] + c * rv[3 * i + 2] + o * double(b2));
-        x[3 * i] = A(a * double(x[3 * i]) + c * double(v[3 * i]));
-        x[3 * i + 1] = A(a * double(x[3 * i + 1]) + c * double(v[3 * i + 1]));
-        x[3 * i + 2] = A(a * double(x[3 * i + 2]) + c * double(v[3 * i + 2]));
+      if (alpha && v) {  // x0 <- a x0 + c v + d w: r0 = a (b - A x0) + c (b - A v) + d (b - A w) + (1 - a - c - d) b
+        const double a = alpha[e], c = beta[e], d = gamma[e], o = 1.0 - a - c - d;
+        r0 = T(a * r64[3 * i] + c * rv[3 * i] + d * rw[3 * i] + o * double(b0));
+        r1 = T(a * r64[3 * i + 1] + c * rv[3 * i + 1] + d * rw[3 * i + 1] + o * double(b1));
+        r2 = T(a * r64[3 * i + 2] + c * rv[3 * i + 2] + d * rw[3 * i + 2] + o * double(b2));
+        x[3 * i] = A(a * double(x[3 * i]) + c * double(v[3 * i]) + d * double(w2[3 * i]));
+        x[3 * i + 1] = A(a * double(x[3 * i + 1]) + c * double(v[3 * i + 1]) + d * double(w2[3 * i + 1]));
+        x[3 * i + 2] = A(a * double(x[3 * i + 2]) + c * double(v[3 * i + 2]) + d * double(w2[3 * i + 2]));
       } else if (alpha) {  // x0 <- a x0, so r0 = b - a A x0 = a r64 + (1 - a) b
         const double a = alpha[e];
         r0 = T(a * r64[3 * i] + (1.0 - a) * double(b0));
@@ -1146,75 +1148,69 @@ __global__ void __launch_bounds__(kSpmvThreads) k_residual64(Tiles ta, int nEnv,
   if (acc) acc_add<double, 4>(part, acc_ptr(acc, 0, nEnv, e), tile - ta.first[e]);
 }
 
-// The two FP64 residuals b - A u and b - A v of two initial guesses in one
-// pass over the matrix, for the environments with mask[e]: into ru, rv, and
-// eleven sums into set 0 of acc (zeroed by the caller): |b|^2; u.b, v.b,
-// u.Au, v.Av, u.Av (the Galerkin system of the combination a u + c v with the
-// least energy-norm error); b.Au, b.Av, |Au|^2, |Av|^2, Au.Av (its residual).
+// The FP64 residuals b - A u, b - A v, b - A w of three initial guesses in
+// one pass over the matrix, for the environments with mask[e]: into ru, rv,
+// rw, and the ten sums |b|^2, b.Au, b.Av, b.Aw, |Au|^2, |Av|^2, |Aw|^2,
+// Au.Av, Au.Aw, Av.Aw into set 0 of acc (zeroed by the caller): the normal
+// equations of the combination with the least residual.
 template <class T, class X, int L>
-__global__ void __launch_bounds__(kSpmvThreads) k_residual64_pair(Tiles ta, int nEnv, const int32_t* __restrict__ mask,
-                                                                  const int32_t* __restrict__ row_ptr,
-                                                                  const int32_t* __restrict__ col,
-                                                                  const T* __restrict__ vals, int nB,
-                                                                  const X* __restrict__ u, const T* __restrict__ v,
-                                                                  const T* __restrict__ b, double* __restrict__ ru,
-                                                                  double* __restrict__ rv, double* acc) {
+__global__ void __launch_bounds__(kSpmvThreads) k_residual64_multi(Tiles ta, int nEnv, const int32_t* __restrict__ mask,
+                                                                   const int32_t* __restrict__ row_ptr,
+                                                                   const int32_t* __restrict__ col,
+                                                                   const T* __restrict__ vals, int nB,
+                                                                   const X* __restrict__ u, const T* __restrict__ v,
+                                                                   const T* __restrict__ w, const T* __restrict__ b,
+                                                                   double* __restrict__ ru, double* __restrict__ rv,
+                                                                   double* __restrict__ rw, double* acc) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int e = ta.env[tile];
   if (!mask[e]) return;
   const int i = ta.row0[tile] + (threadIdx.x / L);
   const int lane = threadIdx.x % L;
   const bool valid = i < ta.row1[tile];
-  double p0 = 0.0, p1 = 0.0, p2 = 0.0, q0 = 0.0, q1 = 0.0, q2 = 0.0;
+  double p[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};  // A u, A v, A w rows
   if (valid) {
     for (int kk = row_ptr[i] + lane; kk < row_ptr[i + 1]; kk += L) {
       const int jc = col[kk];
-      const double u0 = u[3 * jc], u1 = u[3 * jc + 1], u2 = u[3 * jc + 2];
-      const double v0 = v[3 * jc], v1 = v[3 * jc + 1], v2 = v[3 * jc + 2];
       double a[9];
 #pragma unroll
       for (int c = 0; c < 9; ++c) a[c] = double(vals[(size_t)c * nB + kk]);
-      p0 += a[0] * u0 + a[1] * u1 + a[2] * u2;
-      p1 += a[3] * u0 + a[4] * u1 + a[5] * u2;
-      p2 += a[6] * u0 + a[7] * u1 + a[8] * u2;
-      q0 += a[0] * v0 + a[1] * v1 + a[2] * v2;
-      q1 += a[3] * v0 + a[4] * v1 + a[5] * v2;
-      q2 += a[6] * v0 + a[7] * v1 + a[8] * v2;
+      const double g[3][3] = {{double(u[3 * jc]), double(u[3 * jc + 1]), double(u[3 * jc + 2])},
+                              {double(v[3 * jc]), double(v[3 * jc + 1]), double(v[3 * jc + 2])},
+                              {double(w[3 * jc]), double(w[3 * jc + 1]), double(w[3 * jc + 2])}};
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) p[q][r] += a[3 * r] * g[q][0] + a[3 * r + 1] * g[q][1] + a[3 * r + 2] * g[q][2];
     }
   }
 #pragma unroll
-  for (int off = L / 2; off >= 1; off >>= 1) {
-    p0 += __shfl_xor(p0, off, L);
-    p1 += __shfl_xor(p1, off, L);
-    p2 += __shfl_xor(p2, off, L);
-    q0 += __shfl_xor(q0, off, L);
-    q1 += __shfl_xor(q1, off, L);
-    q2 += __shfl_xor(q2, off, L);
-  }
-  double part[11] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int off = L / 2; off >= 1; off >>= 1)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) p[q][r] += __shfl_xor(p[q][r], off, L);
+  double part[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (valid && lane == 0) {
-    const double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
-    const double ui0 = u[3 * i], ui1 = u[3 * i + 1], ui2 = u[3 * i + 2];
-    const double vi0 = v[3 * i], vi1 = v[3 * i + 1], vi2 = v[3 * i + 2];
-    ru[3 * i] = b0 - p0;
-    ru[3 * i + 1] = b1 - p1;
-    ru[3 * i + 2] = b2 - p2;
-    rv[3 * i] = b0 - q0;
-    rv[3 * i + 1] = b1 - q1;
-    rv[3 * i + 2] = b2 - q2;
-    part[0] = b0 * b0 + b1 * b1 + b2 * b2;
-    part[1] = ui0 * b0 + ui1 * b1 + ui2 * b2;
-    part[2] = vi0 * b0 + vi1 * b1 + vi2 * b2;
-    part[3] = ui0 * p0 + ui1 * p1 + ui2 * p2;
-    part[4] = vi0 * q0 + vi1 * q1 + vi2 * q2;
-    part[5] = ui0 * q0 + ui1 * q1 + ui2 * q2;
-    part[6] = b0 * p0 + b1 * p1 + b2 * p2;
-    part[7] = b0 * q0 + b1 * q1 + b2 * q2;
-    part[8] = p0 * p0 + p1 * p1 + p2 * p2;
-    part[9] = q0 * q0 + q1 * q1 + q2 * q2;
-    part[10] = p0 * q0 + p1 * q1 + p2 * q2;
+    const double bb[3] = {double(b[3 * i]), double(b[3 * i + 1]), double(b[3 * i + 2])};
+    double* out[3] = {ru, rv, rw};
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) out[q][3 * i + r] = bb[r] - p[q][r];
+    auto dot = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    part[0] = dot(bb, bb);
+    part[1] = dot(bb, p[0]);
+    part[2] = dot(bb, p[1]);
+    part[3] = dot(bb, p[2]);
+    part[4] = dot(p[0], p[0]);
+    part[5] = dot(p[1], p[1]);
+    part[6] = dot(p[2], p[2]);
+    part[7] = dot(p[0], p[1]);
+    part[8] = dot(p[0], p[2]);
+    part[9] = dot(p[1], p[2]);
   }
-  acc_add<double, 11>(part, acc_ptr(acc, 0, nEnv, e), tile - ta.first[e]);
+  acc_add<double, 10>(part, acc_ptr(acc, 0, nEnv, e), tile - ta.first[e]);
 }
 
 template <class T, int L>
@@ -1278,7 +1274,8 @@ __global__ void k_ls_update(int nV, const int32_t* __restrict__ row_env, const i
 
 template <class T>
 __global__ void k_post_step(int nV, const int32_t* __restrict__ row_env, const int32_t* __restrict__ reset, T idt,
-                            const T* __restrict__ u0, T* __restrict__ u, T* __restrict__ v, T* __restrict__ prev) {
+                            const T* __restrict__ u0, T* __restrict__ u, T* __restrict__ v, T* __restrict__ prev,
+                            T* __restrict__ prev2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nV) return;
   const bool rs = reset[row_env[i]] != 0;
@@ -1289,6 +1286,7 @@ __global__ void k_post_step(int nV, const int32_t* __restrict__ row_env, const i
     if (rs) {
       u[3 * i + d] = T(0);
       if (prev) prev[3 * i + d] = T(0);
+      if (prev2) prev2[3 * i + d] = T(0);
     }
   }
 }
@@ -1405,10 +1403,11 @@ void launch_cg_reset(hipStream_t s, int nEnv, const int32_t* solve, EnvCG* env, 
 template <class T, class A>
 void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T* b, const int32_t* warm,
                     const double* r64, const double* alpha, const T* v, const double* rv, const double* beta,
-                    const T* dinv, int32_t nV, A* x, T* r, T* z, EnvCG* env, A* acc) {
+                    const T* w, const double* rw, const double* gamma, const T* dinv, int32_t nV, A* x, T* r, T* z,
+                    EnvCG* env, A* acc) {
   if (tv.n == 0) return;
   hipLaunchKernelGGL((k_cg_init<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, tv, cfg, b, warm, r64, alpha, v, rv,
-                     beta, dinv, nV, x, r, z, env, acc);
+                     beta, w, rw, gamma, dinv, nV, x, r, z, env, acc);
 }
 
 template <class T, class A>
@@ -1473,15 +1472,15 @@ void launch_residual64(hipStream_t s, const Tiles& ta, int lanes, int nEnv, cons
 }
 
 template <class T, class X>
-void launch_residual64_pair(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
-                            const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* u,
-                            const T* v, const T* b, double* ru, double* rv, double* acc) {
+void launch_residual64_multi(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
+                             const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* u,
+                             const T* v, const T* w, const T* b, double* ru, double* rv, double* rw, double* acc) {
   if (ta.n == 0) return;
-  auto kern = lanes == 4   ? k_residual64_pair<T, X, 4>
-              : lanes == 8 ? k_residual64_pair<T, X, 8>
-                           : k_residual64_pair<T, X, 16>;
-  hipLaunchKernelGGL(kern, dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, nEnv, mask, row_ptr, col, vals, nB, u, v, b, ru,
-                     rv, acc);
+  auto kern = lanes == 4   ? k_residual64_multi<T, X, 4>
+              : lanes == 8 ? k_residual64_multi<T, X, 8>
+                           : k_residual64_multi<T, X, 16>;
+  hipLaunchKernelGGL(kern, dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, nEnv, mask, row_ptr, col, vals, nB, u, v, w, b,
+                     ru, rv, rw, acc);
 }
 
 template <class S, class D>
@@ -1534,10 +1533,10 @@ void launch_ls_update(hipStream_t s, int32_t nV, const int32_t* row_env, const i
 
 template <class T>
 void launch_post_step(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* reset, double dt,
-                      const T* u0, T* u, T* v, T* prev) {
+                      const T* u0, T* u, T* v, T* prev, T* prev2) {
   if (nV == 0) return;
   hipLaunchKernelGGL(k_post_step<T>, dim3(blocks_for(nV, 256)), dim3(256), 0, s, nV, row_env, reset,
-                     T(1.0 / dt), u0, u, v, prev);
+                     T(1.0 / dt), u0, u, v, prev, prev2);
 }
 
 template <class T>
@@ -1567,7 +1566,7 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
   template void launch_ls_update<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, const double*,        \
                                     const T*, const T*, T*);                                                    \
   template void launch_post_step<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, double, const T*, T*, \
-                                    T*, T*);                                                                    \
+                                    T*, T*, T*);                                                                \
   template void launch_env_copy<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, const T*, T*);        \
   template void launch_env_norms<T>(hipStream_t, const Tiles&, int, const T*, const T*, double*);              \
   template void launch_fill<T>(hipStream_t, int64_t, T, T*);                                                    \
@@ -1578,10 +1577,12 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
   template void launch_gather_res<T, A>(hipStream_t, const MeshView&, const Tiles&, const T*, T*, A*);          \
   template void launch_cg_init<T, A>(hipStream_t, const Tiles&, const CgConfig&, const T*, const int32_t*,       \
                                      const double*, const double*, const T*, const double*, const double*,      \
-                                     const T*, int32_t, A*, T*, T*, EnvCG*, A*);                                 \
-  template void launch_residual64_pair<T, A>(hipStream_t, const Tiles&, int, int, const int32_t*,              \
-                                             const int32_t*, const int32_t*, const T*, int32_t, const A*,       \
-                                             const T*, const T*, double*, double*, double*);                    \
+                                     const T*, const double*, const double*, const T*, int32_t, A*, T*, T*,     \
+                                     EnvCG*, A*);                                                               \
+  template void launch_residual64_multi<T, A>(hipStream_t, const Tiles&, int, int, const int32_t*,             \
+                                              const int32_t*, const int32_t*, const T*, int32_t, const A*,      \
+                                              const T*, const T*, const T*, double*, double*, double*,          \
+                                              double*);                                                         \
   template void launch_residual64<T, A>(hipStream_t, const Tiles&, int, int, const int32_t*, const int32_t*,    \
                                         const int32_t*, const T*, int32_t, const A*, const T*, double*,         \
                                         double*);                                                               \

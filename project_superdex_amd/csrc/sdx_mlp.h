@@ -108,11 +108,13 @@ void launch_ws_collect(hipStream_t s, int nEnv, const int32_t* env_v0, const int
 constexpr double kWarmKeep = 0.2;
 void launch_ws_decide(hipStream_t s, int nEnv, const int32_t* solve, const double* acc, int32_t* warm,
                       double* ratio, double* alpha);
-// The same for two guesses u (the network's) and v (the previous Newton
-// step), from the sums of launch_residual64_pair in set 0 of acc: the
-// combination alpha u + beta v with the least residual, kept when that
-// residual is below kWarmKeep |b|.
-void launch_ws_decide_pair(hipStream_t s, int nEnv, const int32_t* solve, const double* acc, int32_t* warm,
-                           double* ratio, double* alpha, double* beta);
+// The same for three guesses u (the network's), v and w (the two previous
+// Newton steps), from the sums of launch_residual64_multi in set 0 of acc:
+// the combination alpha u + beta v + gamma w with the least residual over
+// every nonempty subset of the three (normal equations; subsets whose
+// equations are nearly singular are skipped), kept when that residual is
+// below kWarmKeep |b|.
+void launch_ws_decide_multi(hipStream_t s, int nEnv, const int32_t* solve, const double* acc, int32_t* warm,
+                            double* ratio, double* alpha, double* beta, double* gamma);
 
 }  // namespace sdx

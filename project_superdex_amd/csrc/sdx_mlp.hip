@@ -630,55 +630,95 @@ void launch_ws_decide(hipStream_t s, int nEnv, const int32_t* solve, const doubl
   hipLaunchKernelGGL(k_ws_decide, dim3((nEnv + 63) / 64), dim3(64), 0, s, nEnv, solve, acc, warm, ratio, alpha);
 }
 
-__global__ void k_ws_decide_pair(int nEnv, const int32_t* __restrict__ solve, const double* __restrict__ acc,
-                                 int32_t* __restrict__ warm, double* __restrict__ ratio, double* __restrict__ alpha,
-                                 double* __restrict__ beta) {
+// Least |b - sum_i c_i A g_i|^2 = |b|^2 - 2 c.h + c^T G c over the guesses of
+// `mask` (G the Gram matrix of the A g_i, h_i = b.A g_i): solves the normal
+// equations of the subset; false when they are (nearly) singular.
+__device__ bool ws_fit(const double G[3][3], const double h[3], int mask, double c[3]) {
+  int idx[3], n = 0;
+  for (int i = 0; i < 3; ++i) {
+    c[i] = 0.0;
+    if (mask >> i & 1) idx[n++] = i;
+  }
+  double M[3][3], y[3];
+  for (int i = 0; i < n; ++i) {
+    y[i] = h[idx[i]];
+    for (int j = 0; j < n; ++j) M[i][j] = G[idx[i]][idx[j]];
+  }
+  double diag = 1.0;
+  for (int i = 0; i < n; ++i) {
+    if (!(M[i][i] > 0.0)) return false;
+    diag *= M[i][i];
+  }
+  if (n == 1) {
+    c[idx[0]] = y[0] / M[0][0];
+  } else if (n == 2) {
+    const double det = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+    if (!(det > 1e-10 * diag)) return false;
+    c[idx[0]] = (y[0] * M[1][1] - y[1] * M[0][1]) / det;
+    c[idx[1]] = (y[1] * M[0][0] - y[0] * M[1][0]) / det;
+  } else {  // symmetric 3 x 3: c = adj(M) y / det(M)
+    const double a00 = M[1][1] * M[2][2] - M[1][2] * M[2][1], a01 = M[0][2] * M[2][1] - M[0][1] * M[2][2],
+                 a02 = M[0][1] * M[1][2] - M[0][2] * M[1][1], a11 = M[0][0] * M[2][2] - M[0][2] * M[2][0],
+                 a12 = M[0][2] * M[1][0] - M[0][0] * M[1][2], a22 = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+    const double det = M[0][0] * a00 + M[0][1] * a01 + M[0][2] * a02;
+    if (!(det > 1e-10 * diag)) return false;
+    c[idx[0]] = (a00 * y[0] + a01 * y[1] + a02 * y[2]) / det;
+    c[idx[1]] = (a01 * y[0] + a11 * y[1] + a12 * y[2]) / det;
+    c[idx[2]] = (a02 * y[0] + a12 * y[1] + a22 * y[2]) / det;
+  }
+  return isfinite(c[0]) && isfinite(c[1]) && isfinite(c[2]);
+}
+
+__global__ void k_ws_decide_multi(int nEnv, const int32_t* __restrict__ solve, const double* __restrict__ acc,
+                                  int32_t* __restrict__ warm, double* __restrict__ ratio, double* __restrict__ alpha,
+                                  double* __restrict__ beta, double* __restrict__ gamma) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nEnv) return;
   alpha[e] = 1.0;
   beta[e] = 0.0;
+  gamma[e] = 0.0;
   if (!solve[e]) {
     warm[e] = 0;
     ratio[e] = 0.0;
     return;
   }
-  double q[11];  // |b|^2; u.b, v.b, u.Au, v.Av, u.Av; b.Au, b.Av, |Au|^2, |Av|^2, Au.Av
-  acc_read<double, 11>(acc_ptr(acc, 0, nEnv, e), q);
-  const double bb = q[0], bu = q[6], bv = q[7], uu = q[8], vv = q[9], uv = q[10];
-  // least residual |b - A(a u + c v)|, the norm the stop test measures: the
-  // 2 x 2 normal equations, or the better single guess when one vanishes or
-  // the two are (nearly) parallel.  (The energy-norm fit, from the Galerkin
-  // sums q[1..5], measured slightly worse on batched indentation: 12.4%
-  // instead of 13.4% fewer CG iterations on a held-out seed.)
-  double a = 0.0, c = 0.0;
-  const double det = uu * vv - uv * uv;
-  if (uu > 0.0 && vv > 0.0 && det > 1e-10 * uu * vv) {
-    a = (bu * vv - bv * uv) / det;
-    c = (bv * uu - bu * uv) / det;
-  } else {
-    const double au = uu > 0.0 ? bu / uu : 0.0, av = vv > 0.0 ? bv / vv : 0.0;
-    if (bu * au >= bv * av)  // the squared residual falls by (b.Au)^2 / |Au|^2
-      a = au;
-    else
-      c = av;
+  double q[10];  // |b|^2, b.Au, b.Av, b.Aw, |Au|^2, |Av|^2, |Aw|^2, Au.Av, Au.Aw, Av.Aw
+  acc_read<double, 10>(acc_ptr(acc, 0, nEnv, e), q);
+  const double bb = q[0];
+  const double h[3] = {q[1], q[2], q[3]};
+  const double G[3][3] = {{q[4], q[7], q[8]}, {q[7], q[5], q[9]}, {q[8], q[9], q[6]}};
+  // every nonempty subset of the three guesses; the least residual wins
+  double best = bb, bc[3] = {0.0, 0.0, 0.0};
+  for (int mask = 1; mask < 8; ++mask) {
+    double c[3];
+    if (!ws_fit(G, h, mask, c)) continue;
+    double r2 = bb;
+    for (int i = 0; i < 3; ++i) {
+      r2 -= 2.0 * c[i] * h[i];
+      for (int j = 0; j < 3; ++j) r2 += c[i] * c[j] * G[i][j];
+    }
+    if (r2 < best) {
+      best = r2;
+      for (int i = 0; i < 3; ++i) bc[i] = c[i];
+    }
   }
-  double r2 = bb - 2.0 * (a * bu + c * bv) + a * a * uu + 2.0 * a * c * uv + c * c * vv;
-  if (r2 < 0.0) r2 = 0.0;
-  const double r = bb > 0.0 ? sqrt(r2 / bb) : 1.0;
-  const bool keep = bb > 0.0 && (uu > 0.0 || vv > 0.0) && r < kWarmKeep && isfinite(a) && isfinite(c);
+  if (best < 0.0) best = 0.0;
+  const double r = bb > 0.0 ? sqrt(best / bb) : 1.0;
+  const bool keep = bb > 0.0 && r < kWarmKeep;
   warm[e] = keep ? 1 : 0;
   ratio[e] = r;
   if (keep) {
-    alpha[e] = a;
-    beta[e] = c;
+    alpha[e] = bc[0];
+    beta[e] = bc[1];
+    gamma[e] = bc[2];
   }
 }
 
-void launch_ws_decide_pair(hipStream_t s, int nEnv, const int32_t* solve, const double* acc, int32_t* warm,
-                           double* ratio, double* alpha, double* beta) {
+void launch_ws_decide_multi(hipStream_t s, int nEnv, const int32_t* solve, const double* acc, int32_t* warm,
+                            double* ratio, double* alpha, double* beta, double* gamma) {
   if (nEnv == 0) return;
-  hipLaunchKernelGGL(k_ws_decide_pair, dim3((nEnv + 63) / 64), dim3(64), 0, s, nEnv, solve, acc, warm, ratio, alpha,
-                     beta);
+  hipLaunchKernelGGL(k_ws_decide_multi, dim3((nEnv + 63) / 64), dim3(64), 0, s, nEnv, solve, acc, warm, ratio,
+                     alpha, beta, gamma);
 }
 
 #define SDX_MLP_INST(T)                                                                                          \

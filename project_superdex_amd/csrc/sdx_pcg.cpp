@@ -131,9 +131,9 @@ sdx_status solve(int32_t device, int32_t nbr, int32_t nnzb, const int32_t* ptr, 
     launch_residual64<T, A>(s, ta, kSpmvLanes, 1, dsolve.p, dptr.p, dcol.p, dvals.p, nnzb, dx.p, db.p, drref.p, nullptr);
   A* acc = reinterpret_cast<A*>(dacc.p);
   launch_cg_reset<A>(s, 1, dsolve.p, denv.p, acc);
-  launch_cg_init<T, A>(s, tv, cfg, db.p, warm ? dsolve.p : nullptr, drref.p, nullptr, nullptr, nullptr, nullptr, ddinv.p,
-                       nbr, dx.p, dr.p, dz.p,
-                       denv.p, acc);
+  launch_cg_init<T, A>(s, tv, cfg, db.p, warm ? dsolve.p : nullptr, drref.p, /*alpha*/ nullptr, /*v*/ nullptr,
+                       /*rv*/ nullptr, /*beta*/ nullptr, /*w*/ nullptr, /*rw*/ nullptr, /*gamma*/ nullptr, ddinv.p, nbr,
+                       dx.p, dr.p, dz.p, denv.p, acc);
   int j = 0;
   int32_t na = 1;
   while (true) {

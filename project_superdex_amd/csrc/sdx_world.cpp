@@ -409,6 +409,7 @@ void World<T>::finalize() {
   w_.alloc(n3);
   rref_.alloc(n3);
   rref2_.alloc(n3);
+  rref3_.alloc(n3);
   // padded by one 16-byte vector: the gathers stage these with vector loads
   const size_t nrec_v = size_t(4) * size_t(nT_) + size_t(3) * size_t(nF_);
   const size_t nrec_e = size_t(6) * size_t(nT_) + size_t(3) * size_t(nF_);
@@ -424,12 +425,15 @@ void World<T>::finalize() {
   list_count_.alloc(4);
   prev_dx_.alloc(n3);
   hip_check(hipMemsetAsync(prev_dx_.p, 0, prev_dx_.bytes(), stream_), "hipMemsetAsync");
+  prev2_dx_.alloc(n3);
+  hip_check(hipMemsetAsync(prev2_dx_.p, 0, prev2_dx_.bytes(), stream_), "hipMemsetAsync");
   ws_y_.alloc(n3);
   ws_F_.alloc(size_t(kMlpInPad) * size_t(nV_));
   ws_scale_.alloc(static_cast<size_t>(nEnv));
   ws_ratio_.alloc(static_cast<size_t>(nEnv));
   ws_alpha_.alloc(static_cast<size_t>(nEnv));
   ws_beta_.alloc(static_cast<size_t>(nEnv));
+  ws_gamma_.alloc(static_cast<size_t>(nEnv));
   d_env_v0_.upload(env_v0_);
   num_active_.alloc(1);
   env_mask_.alloc(static_cast<size_t>(nEnv));
@@ -640,40 +644,43 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
   put(num_active_, na);
   const int32_t* wm = n_warm ? warm_mask_.p : nullptr;
   launch_fill<int32_t>(stream_, 4, 0, list_count_.p);
-  // decide (the learned guess u in x): the combination a u + c v with the
-  // previous Newton step v of least residual |b - A x0|, kept only where it
-  // removes most of the residual (on device)
+  // decide (the learned guess u in x): the combination of u with the two
+  // previous Newton steps v, w of least residual |b - A x0|, kept only where
+  // it removes most of the residual (on device)
   if (n_warm && decide) launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
   const T* v = decide ? prev_dx_.p : nullptr;
+  const T* w = decide ? prev2_dx_.p : nullptr;
   const double* rv = decide ? rref2_.p : nullptr;
+  const double* rw = decide ? rref3_.p : nullptr;
   const double* wa = decide ? ws_alpha_.p : nullptr;
   const double* wb = decide ? ws_beta_.p : nullptr;
+  const double* wc = decide ? ws_gamma_.p : nullptr;
   if (precision_ != SDX_PREC_MODE_FP32) {
     if (n_warm) {
       put(warm_mask_, warm);
       if (decide) {
-        launch_residual64_pair<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
-                                          d_col_.p, vals_.p, nB_, xcg<double>(), prev_dx_.p, res_.p, rref_.p,
-                                          rref2_.p, acc_.p);
-        launch_ws_decide_pair(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p,
-                              ws_beta_.p);
+        launch_residual64_multi<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
+                                           d_col_.p, vals_.p, nB_, xcg<double>(), prev_dx_.p, prev2_dx_.p, res_.p,
+                                           rref_.p, rref2_.p, rref3_.p, acc_.p);
+        launch_ws_decide_multi(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p,
+                               ws_beta_.p, ws_gamma_.p);
       } else {
         launch_residual64<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
                                      d_col_.p, vals_.p, nB_, xcg<double>(), res_.p, rref_.p, nullptr);
       }
     }
     launch_cg_reset<double>(stream_, nEnv_, solve_mask_.p, env_cg_.p, acc_.p);
-    launch_cg_init<T, double>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, wa, v, rv, wb, dinv_.p, nV_,
+    launch_cg_init<T, double>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, wa, v, rv, wb, w, rw, wc, dinv_.p, nV_,
                               xcg<double>(), r_.p, z_.p, env_cg_.p, acc_.p);
   } else {
     if (n_warm) {
       put(warm_mask_, warm);
       if (decide) {
-        launch_residual64_pair<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
-                                     d_col_.p, vals_.p, nB_, xcg<T>(), prev_dx_.p, res_.p, rref_.p, rref2_.p,
-                                     acc_.p);
-        launch_ws_decide_pair(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p,
-                              ws_beta_.p);
+        launch_residual64_multi<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
+                                      d_col_.p, vals_.p, nB_, xcg<T>(), prev_dx_.p, prev2_dx_.p, res_.p, rref_.p,
+                                      rref2_.p, rref3_.p, acc_.p);
+        launch_ws_decide_multi(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p,
+                               ws_beta_.p, ws_gamma_.p);
       } else {
         launch_residual64<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p, d_col_.p,
                                 vals_.p, nB_, xcg<T>(), res_.p, rref_.p, nullptr);
@@ -681,8 +688,8 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
     }
     T* accf = reinterpret_cast<T*>(acc_.p);
     launch_cg_reset<T>(stream_, nEnv_, solve_mask_.p, env_cg_.p, accf);
-    launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, wa, v, rv, wb, dinv_.p, nV_, xcg<T>(), r_.p,
-                         z_.p, env_cg_.p, accf);
+    launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, wa, v, rv, wb, w, rw, wc, dinv_.p, nV_,
+                         xcg<T>(), r_.p, z_.p, env_cg_.p, accf);
   }
   // Bursts of B iterations are issued kBurstAhead bursts ahead of the host's
   // check of the active count, so the device does not wait for the host
@@ -992,6 +999,7 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
                            ws_scale_.p, coll_n_, coll_x_.p, coll_t_.p);
       coll_n_ += int64_t(nE) * coll_per_env_;
     }
+    launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, prev_dx_.p, prev2_dx_.p);
     launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, dx_.p, prev_dx_.p);
     std::vector<int32_t> lin_ok(size_t(nE), 0);
     for (int e = 0; e < nE; ++e) {
@@ -1084,7 +1092,7 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
   std::vector<int32_t> reset(size_t(nE), 0);
   for (int e = 0; e < nE; ++e) reset[size_t(e)] = S[size_t(e)].conv == SDX_CONV_DIVERGED ? 1 : 0;
   put(reset_, reset);
-  launch_post_step<T>(stream_, nV_, d_row_env_.p, reset_.p, dt, u0_.p, u_.p, v_.p, prev_dx_.p);
+  launch_post_step<T>(stream_, nV_, d_row_env_.p, reset_.p, dt, u0_.p, u_.p, v_.p, prev_dx_.p, prev2_dx_.p);
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   flush_timers();
   if (stats) {
@@ -1207,19 +1215,22 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
   sp.schedule = 0;
   const int64_t n3 = int64_t(3) * nV_;
   std::vector<DBufBase*> reg;
-  DBuf<T> su(reg, &stream_), sv(reg, &stream_), sp0(reg, &stream_);
+  DBuf<T> su(reg, &stream_), sv(reg, &stream_), sp0(reg, &stream_), sp1(reg, &stream_);
   su.alloc(size_t(n3));
   sv.alloc(size_t(n3));
   sp0.alloc(size_t(n3));
+  sp1.alloc(size_t(n3));
   launch_copy<T>(stream_, n3, u_.p, su.p);
   launch_copy<T>(stream_, n3, v_.p, sv.p);
   launch_copy<T>(stream_, n3, prev_dx_.p, sp0.p);
+  launch_copy<T>(stream_, n3, prev2_dx_.p, sp1.p);
   const int64_t coll_n0 = coll_n_;
   const uint32_t coll_salt0 = coll_salt_;
   auto restore = [&]() {
     launch_copy<T>(stream_, n3, su.p, u_.p);
     launch_copy<T>(stream_, n3, sv.p, v_.p);
     launch_copy<T>(stream_, n3, sp0.p, prev_dx_.p);
+    launch_copy<T>(stream_, n3, sp1.p, prev2_dx_.p);
     coll_n_ = coll_n0;  // trial steps add no training samples
     coll_salt_ = coll_salt0;
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");

@@ -417,7 +417,8 @@ class World final : public WorldBase {
   SDX_BUF(T, sv_)
   SDX_BUF(T, w_)
   SDX_BUF(double, rref_)
-  SDX_BUF(double, rref2_)  // b - A v of the previous Newton step v (learned warm start)
+  SDX_BUF(double, rref2_)  // b - A v, b - A w of the two previous Newton steps (learned warm start)
+  SDX_BUF(double, rref3_)
   SDX_BUF(T, cres_)   // element and contact residual records, vertex-sorted [4 nT + 3 nF][3]
   SDX_BUF(T, cdiag_)  // diagonal-block records, vertex-sorted [4 nT + 3 nF][6]
   SDX_BUF(T, coff_)   // off-diagonal-block records, pair-sorted [6 nT + 3 nF][9]
@@ -428,12 +429,14 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, solve_mask_)
   SDX_BUF(int32_t, warm_mask_)
   SDX_BUF(T, prev_dx_)           // last Newton step of each environment
+  SDX_BUF(T, prev2_dx_)  // the Newton step before it
   SDX_BUF(float, ws_y_)          // Jacobi-scaled residual
   SDX_BUF(float, ws_F_)          // feature rows [nV][16]
   SDX_BUF(double, ws_scale_)     // per-environment scale of features and output
   SDX_BUF(double, ws_ratio_)     // relative residual of the learned guess
   SDX_BUF(double, ws_alpha_)     // least-residual multiple of the learned guess
-  SDX_BUF(double, ws_beta_)      // and of the previous Newton step
+  SDX_BUF(double, ws_beta_)      // and of the two previous Newton steps
+  SDX_BUF(double, ws_gamma_)
   SDX_BUF(int32_t, d_env_v0_)    // first vertex of each environment [nEnv+1]
   SDX_BUF(int32_t, list_a_)      // work list of active matvec tiles
   SDX_BUF(int32_t, list_v_)      // work list of active vector tiles
