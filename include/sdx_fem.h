@@ -162,8 +162,8 @@ typedef struct sdx_step_stats {
   double linear_rel_res;          /* last linear solve ||r||/||b|| (as the stop test measured it) */
   double linear_true_rel_res;     /* last linear solve ||b - A x||/||b|| recomputed in FP64 */
   double warm_start_rel_res;      /* ||b - A x0||/||b|| of the last learned warm start x0 (the
-                                     combination of the network's guess and the previous Newton
-                                     step; 0: none tried) */
+                                     combination of the network's guess and the two previous
+                                     Newton steps; 0: none tried) */
 } sdx_step_stats;
 
 /* SoftActorParams subset (cpp_api/mochi_structs.h:174-197). */
@@ -307,11 +307,11 @@ sdx_status sdx_autotune(sdx_world* world, double dt, const sdx_solver_params* pa
  * maps features of the assembled Newton system to a guess u of the Newton
  * step; sdx_step uses it when sdx_solver_params.warm_start is
  * SDX_WARM_START_LEARNED and a model is set.  Per environment the initial
- * guess is the combination x0 = a u + c v of the network's guess and the
- * previous Newton step v with the least residual |b - A x0| (2 x 2 normal
- * equations; one of the two when the other vanishes or they are parallel),
- * kept only if that residual is below 0.2 |b|; else the environment starts
- * from zero, as the plain method does.  Inference runs in BF16 on the matrix
+ * guess is the combination x0 = a u + c v + d w of the network's guess and
+ * the environment's two previous Newton steps v, w with the least residual
+ * |b - A x0| (normal equations over every nonempty subset of the three, so a
+ * vanishing or parallel guess drops out), kept only if that residual is below
+ * 0.2 |b|; else the environment starts from zero, as the plain method does.  Inference runs in BF16 on the matrix
  * cores, training in FP32 on the device.
  *
  * Feature row (16 floats, the last zero) of vertex i of environment e, with

@@ -9,14 +9,14 @@
   features of the oracle-assembled system and its exact Newton step;
 - on a held-out seed a trained model predicts the Newton step better than
   the zero guess; the warm start (the least-residual combination of the
-  network's guess and the previous Newton step, kept where it removes 80% of
-  the residual) saves at least 5% of the CG iterations, the counts are
-  reported, and the result stays that of the plain solver (positions within
-  1e-3 of the largest displacement);
+  network's guess and the two previous Newton steps, kept where it removes
+  80% of the residual) saves at least 10% of the CG iterations, the counts
+  are reported, and the result stays that of the plain solver (positions
+  within 1e-3 of the largest displacement);
 - a network of noise cannot make the solve cost more than the plain method:
   its iterations stay within 2% of the plain method's (the iterations of a
-  network that outputs zero, whose warm start is the previous step alone, are
-  reported beside them), and the result is the plain solver's.
+  network that outputs zero, whose warm start is the previous steps alone,
+  are reported beside them), and the result is the plain solver's.
 """
 import numpy as np
 import pytest
@@ -145,8 +145,9 @@ def trained_model():
 def test_learned_warm_start_on_held_out_seed(trained_model):
     """What the warm start gains on a held-out seed: the network's guesses beat
     the zero guess, and the CG iteration count with the learned warm start
-    (network guess combined with the previous Newton step) is reported and at
-    least 5% below the plain method's; the result is the plain solver's."""
+    (network guess combined with the two previous Newton steps) is reported
+    and at least 10% below the plain method's; the result is the plain
+    solver's."""
     spec = batched(100, 16)  # held-out seed
     zero = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
     learned = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
@@ -165,7 +166,7 @@ def test_learned_warm_start_on_held_out_seed(trained_model):
           f"(x{it1 / it0:.3f}); guesses kept {kept}/{tried}, median residual ratio {ratio:.3f}")
     assert err < 0.9  # the network predicts the Newton step better than zero does
     assert tried == 16 * 16 and ratio < 0.2 and kept > tried // 2
-    assert it1 <= 0.95 * it0, (it0, it1)
+    assert it1 <= 0.90 * it0, (it0, it1)
     for st in st1:
         for e in st:
             assert e["linear_convergence"] == _abi.CONV_CONVERGED and e["linear_true_rel_res"] <= 1e-6
@@ -182,7 +183,7 @@ def test_noise_model_costs_no_more_than_plain():
     bad = WarmStartModel(seed=3)
     bad.params = bad.params * 8.0  # guesses of vertex-to-vertex noise
     null = WarmStartModel(seed=3)
-    null.params = np.zeros_like(null.params)  # outputs zero: the previous step alone
+    null.params = np.zeros_like(null.params)  # outputs zero: the previous steps alone
 
     s0, st0 = run(spec, zero)
     s1, st1 = run(spec, learned, model=bad)
