@@ -308,15 +308,17 @@ sdx_status sdx_autotune(sdx_world* world, double dt, const sdx_solver_params* pa
  * step; sdx_step uses it when sdx_solver_params.warm_start is
  * SDX_WARM_START_LEARNED and a model is set.  Per environment the initial
  * guess is the combination x0 = a u + c v + d w of the network's guess and
- * the environment's two previous Newton steps v, w with the least residual
- * |b - A x0| (normal equations over every nonempty subset of the three, so a
- * vanishing or parallel guess drops out), kept only if that residual is below
- * 0.2 |b|; else the environment starts from zero, as the plain method does.  Inference runs in BF16 on the matrix
- * cores, training in FP32 on the device.
+ * the environment's Newton steps v, w of the same Newton iteration one and two
+ * time steps back, with the least residual |b - A x0| (normal equations over
+ * every nonempty subset of the three, so a vanishing or parallel guess drops
+ * out), kept only if that residual is below 0.2 |b|; else the environment
+ * starts from zero, as the plain method does.  Inference runs in BF16 on the
+ * matrix cores, training in FP32 on the device.
  *
  * Feature row (16 floats, the last zero) of vertex i of environment e, with
  * b the residual, D_i the 3x3 diagonal block, d_i = trace(D_i) / 3,
- * y = b / d, dprev the previous Newton step, s_e = sqrt(max(|dprev|^2,
+ * y = b / d, dprev the Newton step of the same Newton iteration one time
+ * step back (zero before the first), s_e = sqrt(max(|dprev|^2,
  * |y|^2) / n_e) over the environment:
  *   y_i / s_e (3), (D00, D11, D22, D01, D02, D12) / d_i (6), dprev_i / s_e (3),
  *   mean of y_j / s_e over the 1-ring neighbours j (3).

@@ -220,11 +220,11 @@ void launch_apply_pins(hipStream_t s, int64_t n, const uint8_t* pinned, const T*
 template <class T>
 void launch_ls_update(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* ls_mask,
                       const double* scale, const T* sol0, const T* dx, T* x);
-// v = (u - u0) / dt; environments with reset[env] get u = v = 0 (and
-// prev = 0 when given).
+// v = (u - u0) / dt; environments with reset[env] get u = v = 0 and their
+// rows of the n_hist [3 nV] vectors at hist zeroed.
 template <class T>
 void launch_post_step(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* reset, double dt,
-                      const T* u0, T* u, T* v, T* prev = nullptr, T* prev2 = nullptr);
+                      const T* u0, T* u, T* v, T* hist = nullptr, int n_hist = 0);
 // dst = src on the rows of environments with mask[env].
 template <class T>
 void launch_env_copy(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* mask, const T* src, T* dst);

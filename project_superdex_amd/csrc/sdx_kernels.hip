@@ -1274,8 +1274,8 @@ __global__ void k_ls_update(int nV, const int32_t* __restrict__ row_env, const i
 
 template <class T>
 __global__ void k_post_step(int nV, const int32_t* __restrict__ row_env, const int32_t* __restrict__ reset, T idt,
-                            const T* __restrict__ u0, T* __restrict__ u, T* __restrict__ v, T* __restrict__ prev,
-                            T* __restrict__ prev2) {
+                            const T* __restrict__ u0, T* __restrict__ u, T* __restrict__ v, T* __restrict__ hist,
+                            int n_hist) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nV) return;
   const bool rs = reset[row_env[i]] != 0;
@@ -1285,8 +1285,7 @@ __global__ void k_post_step(int nV, const int32_t* __restrict__ row_env, const i
     v[3 * i + d] = rs ? T(0) : (uu - u0[3 * i + d]) * idt;
     if (rs) {
       u[3 * i + d] = T(0);
-      if (prev) prev[3 * i + d] = T(0);
-      if (prev2) prev2[3 * i + d] = T(0);
+      for (int h = 0; h < n_hist; ++h) hist[(size_t)h * 3 * nV + 3 * i + d] = T(0);
     }
   }
 }
@@ -1533,10 +1532,10 @@ void launch_ls_update(hipStream_t s, int32_t nV, const int32_t* row_env, const i
 
 template <class T>
 void launch_post_step(hipStream_t s, int32_t nV, const int32_t* row_env, const int32_t* reset, double dt,
-                      const T* u0, T* u, T* v, T* prev, T* prev2) {
+                      const T* u0, T* u, T* v, T* hist, int n_hist) {
   if (nV == 0) return;
   hipLaunchKernelGGL(k_post_step<T>, dim3(blocks_for(nV, 256)), dim3(256), 0, s, nV, row_env, reset,
-                     T(1.0 / dt), u0, u, v, prev, prev2);
+                     T(1.0 / dt), u0, u, v, hist, n_hist);
 }
 
 template <class T>
@@ -1566,7 +1565,7 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
   template void launch_ls_update<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, const double*,        \
                                     const T*, const T*, T*);                                                    \
   template void launch_post_step<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, double, const T*, T*, \
-                                    T*, T*, T*);                                                                \
+                                    T*, T*, int);                                                               \
   template void launch_env_copy<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, const T*, T*);        \
   template void launch_env_norms<T>(hipStream_t, const Tiles&, int, const T*, const T*, double*);              \
   template void launch_fill<T>(hipStream_t, int64_t, T, T*);                                                    \

@@ -423,10 +423,8 @@ void World<T>::finalize() {
   solve_mask_.alloc(static_cast<size_t>(nEnv));
   warm_mask_.alloc(static_cast<size_t>(nEnv));
   list_count_.alloc(4);
-  prev_dx_.alloc(n3);
-  hip_check(hipMemsetAsync(prev_dx_.p, 0, prev_dx_.bytes(), stream_), "hipMemsetAsync");
-  prev2_dx_.alloc(n3);
-  hip_check(hipMemsetAsync(prev2_dx_.p, 0, prev2_dx_.bytes(), stream_), "hipMemsetAsync");
+  hist_.alloc(size_t(2 * kNewtonHist) * n3);
+  hip_check(hipMemsetAsync(hist_.p, 0, hist_.bytes(), stream_), "hipMemsetAsync");
   ws_y_.alloc(n3);
   ws_F_.alloc(size_t(kMlpInPad) * size_t(nV_));
   ws_scale_.alloc(static_cast<size_t>(nEnv));
@@ -648,8 +646,8 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
   // previous Newton steps v, w of least residual |b - A x0|, kept only where
   // it removes most of the residual (on device)
   if (n_warm && decide) launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
-  const T* v = decide ? prev_dx_.p : nullptr;
-  const T* w = decide ? prev2_dx_.p : nullptr;
+  const T* v = decide ? hist1() : nullptr;
+  const T* w = decide ? hist2() : nullptr;
   const double* rv = decide ? rref2_.p : nullptr;
   const double* rw = decide ? rref3_.p : nullptr;
   const double* wa = decide ? ws_alpha_.p : nullptr;
@@ -660,7 +658,7 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
       put(warm_mask_, warm);
       if (decide) {
         launch_residual64_multi<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
-                                           d_col_.p, vals_.p, nB_, xcg<double>(), prev_dx_.p, prev2_dx_.p, res_.p,
+                                           d_col_.p, vals_.p, nB_, xcg<double>(), v, w, res_.p,
                                            rref_.p, rref2_.p, rref3_.p, acc_.p);
         launch_ws_decide_multi(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p,
                                ws_beta_.p, ws_gamma_.p);
@@ -677,7 +675,7 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
       put(warm_mask_, warm);
       if (decide) {
         launch_residual64_multi<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
-                                      d_col_.p, vals_.p, nB_, xcg<T>(), prev_dx_.p, prev2_dx_.p, res_.p, rref_.p,
+                                      d_col_.p, vals_.p, nB_, xcg<T>(), v, w, res_.p, rref_.p,
                                       rref2_.p, rref3_.p, acc_.p);
         launch_ws_decide_multi(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p,
                                ws_beta_.p, ws_gamma_.p);
@@ -836,9 +834,9 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
 template <class T>
 void World<T>::ws_features() {
   launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
-  launch_ws_scaled_residual<T>(stream_, mesh_, dt_v_.view, res_.p, vals_.p, prev_dx_.p, ws_y_.p, acc_.p);
+  launch_ws_scaled_residual<T>(stream_, mesh_, dt_v_.view, res_.p, vals_.p, hist1(), ws_y_.p, acc_.p);
   launch_ws_env_scale(stream_, nEnv_, acc_.p, ws_scale_.p);
-  launch_ws_features<T>(stream_, mesh_, dt_v_.view, vals_.p, ws_y_.p, prev_dx_.p, ws_scale_.p, ws_F_.p);
+  launch_ws_features<T>(stream_, mesh_, dt_v_.view, vals_.p, ws_y_.p, hist1(), ws_scale_.p, ws_F_.p);
 }
 
 template <class T>
@@ -982,8 +980,10 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
     for (int e = 0; e < nE; ++e) mask[size_t(e)] = S[size_t(e)].active ? 1 : 0;
     set_env_mask(mask);
     if (!h_valid) assemble(dt, sp, true);
+    int newton_it = 0;  // every active environment is in the same Newton iteration
     for (int e = 0; e < nE; ++e)
-      if (S[size_t(e)].active) S[size_t(e)].iters++;
+      if (S[size_t(e)].active) newton_it = S[size_t(e)].iters++;
+    hist_k_ = newton_it;  // the warm start draws on the steps of this Newton iteration
     // linear solve, solution in dx_: from zero, or from the learned warm
     // start where its guess lowers the residual
     const bool learned = sp.warm_start == SDX_WARM_START_LEARNED && ws_on_;
@@ -999,8 +999,8 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
                            ws_scale_.p, coll_n_, coll_x_.p, coll_t_.p);
       coll_n_ += int64_t(nE) * coll_per_env_;
     }
-    launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, prev_dx_.p, prev2_dx_.p);
-    launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, dx_.p, prev_dx_.p);
+    launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, hist1(), hist2());
+    launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, dx_.p, hist1());
     std::vector<int32_t> lin_ok(size_t(nE), 0);
     for (int e = 0; e < nE; ++e) {
       EnvState& s = S[size_t(e)];
@@ -1092,7 +1092,7 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
   std::vector<int32_t> reset(size_t(nE), 0);
   for (int e = 0; e < nE; ++e) reset[size_t(e)] = S[size_t(e)].conv == SDX_CONV_DIVERGED ? 1 : 0;
   put(reset_, reset);
-  launch_post_step<T>(stream_, nV_, d_row_env_.p, reset_.p, dt, u0_.p, u_.p, v_.p, prev_dx_.p, prev2_dx_.p);
+  launch_post_step<T>(stream_, nV_, d_row_env_.p, reset_.p, dt, u0_.p, u_.p, v_.p, hist_.p, 2 * kNewtonHist);
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   flush_timers();
   if (stats) {
@@ -1215,22 +1215,20 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
   sp.schedule = 0;
   const int64_t n3 = int64_t(3) * nV_;
   std::vector<DBufBase*> reg;
-  DBuf<T> su(reg, &stream_), sv(reg, &stream_), sp0(reg, &stream_), sp1(reg, &stream_);
+  DBuf<T> su(reg, &stream_), sv(reg, &stream_), sh(reg, &stream_);
+  const int64_t nh = int64_t(2 * kNewtonHist) * n3;
   su.alloc(size_t(n3));
   sv.alloc(size_t(n3));
-  sp0.alloc(size_t(n3));
-  sp1.alloc(size_t(n3));
+  sh.alloc(size_t(nh));
   launch_copy<T>(stream_, n3, u_.p, su.p);
   launch_copy<T>(stream_, n3, v_.p, sv.p);
-  launch_copy<T>(stream_, n3, prev_dx_.p, sp0.p);
-  launch_copy<T>(stream_, n3, prev2_dx_.p, sp1.p);
+  launch_copy<T>(stream_, nh, hist_.p, sh.p);
   const int64_t coll_n0 = coll_n_;
   const uint32_t coll_salt0 = coll_salt_;
   auto restore = [&]() {
     launch_copy<T>(stream_, n3, su.p, u_.p);
     launch_copy<T>(stream_, n3, sv.p, v_.p);
-    launch_copy<T>(stream_, n3, sp0.p, prev_dx_.p);
-    launch_copy<T>(stream_, n3, sp1.p, prev2_dx_.p);
+    launch_copy<T>(stream_, nh, sh.p, hist_.p);
     coll_n_ = coll_n0;  // trial steps add no training samples
     coll_salt_ = coll_salt0;
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");

@@ -344,6 +344,10 @@ class World final : public WorldBase {
   // workgroups of the persistent CG kernels: about 1.6 x the matvec workgroups
   // (5 waves per SIMD) and 2 x the update workgroups that 256 CUs hold at once;
   // on 512 batched cubes these measured 3% faster than grids of half the size
+  static constexpr int kNewtonHist = 4;
+  int hist_k_ = 0;  // Newton iteration whose history the warm start uses
+  T* hist1() { return hist_.p + size_t(2 * std::min(hist_k_, kNewtonHist - 1)) * size_t(3 * nV_); }
+  T* hist2() { return hist1() + size_t(3 * nV_); }
   static constexpr int kPersistentGridA = 8192, kPersistentGridV = 4096;
   void upload_colliders();
   void set_env_mask(const std::vector<int32_t>& mask);
@@ -428,8 +432,10 @@ class World final : public WorldBase {
   SDX_BUF(double, acc_)
   SDX_BUF(int32_t, solve_mask_)
   SDX_BUF(int32_t, warm_mask_)
-  SDX_BUF(T, prev_dx_)           // last Newton step of each environment
-  SDX_BUF(T, prev2_dx_)  // the Newton step before it
+  // Newton steps of earlier time steps, per Newton iteration index k (the
+  // last kNewtonHist - 1 share a slot): [2 kNewtonHist][3 nV], slot 2k the
+  // step of iteration k one time step back, slot 2k + 1 two time steps back
+  SDX_BUF(T, hist_)
   SDX_BUF(float, ws_y_)          // Jacobi-scaled residual
   SDX_BUF(float, ws_F_)          // feature rows [nV][16]
   SDX_BUF(double, ws_scale_)     // per-environment scale of features and output
