@@ -199,3 +199,30 @@ def test_noise_model_costs_no_more_than_plain():
     for a, b in zip(s0.actors, s1.actors):
         ua, ub = a.displacements(), b.displacements()
         assert np.abs(ua - ub).max() <= 1e-3 * np.abs(ua).max()
+
+
+def test_learned_warm_start_two_newton_iterations(trained_model):
+    """Two Newton iterations per step: each iteration's warm start draws on the previous time
+    steps' Newton steps of the same iteration. The solves converge, cost no more than the
+    plain method's, and land on the plain solver's result."""
+    spec = batched(102, 10)
+    zero = _abi.pinned_solver_params(2, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
+    learned = _abi.pinned_solver_params(2, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
+    learned.warm_start = _abi.WARM_START_LEARNED
+    s0, st0 = run(spec, zero)
+    s1, st1 = run(spec, learned, model=trained_model)
+    it0 = sum(e["linear_iters"] for st in st0 for e in st)
+    it1 = sum(e["linear_iters"] for st in st1 for e in st)
+    tried = sum(e["warm_start_tried"] for st in st1 for e in st)
+    kept = sum(e["warm_start_accepted"] for st in st1 for e in st)
+    print(f"two Newton iterations: CG iterations zero init {it0}, learned {it1} "
+          f"(x{it1 / it0:.3f}); guesses kept {kept}/{tried}")
+    assert tried == 16 * 10 * 2
+    assert it1 <= 1.02 * it0, (it0, it1)
+    for st in st1:
+        for e in st:
+            assert e["newton_iters"] == 2
+            assert e["linear_convergence"] == _abi.CONV_CONVERGED and e["linear_true_rel_res"] <= 1e-6
+    for a, b in zip(s0.actors, s1.actors):
+        ua, ub = a.displacements(), b.displacements()
+        assert np.abs(ua - ub).max() <= 1e-3 * np.abs(ua).max()

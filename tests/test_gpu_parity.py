@@ -171,6 +171,51 @@ def test_assembly_matches_oracle(oracle, mode, tol, contact, amp):
     np.testing.assert_allclose(vg, vo, rtol=tol, atol=tol * np.abs(vo).max())
 
 
+def fan_mesh(m, radius=0.05, height=0.02):
+    """m tets around one shared edge: hub (0,0,0) and apex (0,0,height) with a ring of m points
+    at z=0. The hub and apex each carry m tets and the hub-apex pair m records, so the gathers
+    stage each of those runs through LDS in several chunks."""
+    th = 2 * np.pi * np.arange(m) / m
+    ring = np.stack([radius * np.cos(th), radius * np.sin(th), np.zeros(m)], 1)
+    X = np.vstack([[0, 0, 0], [0, 0, height], ring])
+    k = np.arange(m)
+    tets = np.stack([np.zeros(m, int), np.ones(m, int), 2 + k, 2 + (k + 1) % m], 1).astype(np.int32)
+    return X, tets
+
+
+@pytest.mark.parametrize("mode,tol", [(_abi.PREC_MODE_FP64, 1e-9), (_abi.PREC_MODE_FP32, 2e-4)])
+@pytest.mark.parametrize("contact", [False, True])
+def test_assembly_long_runs_match_oracle(oracle, mode, tol, contact):
+    """2000 tets on one vertex and one edge: residual and BSR runs longer than one LDS chunk."""
+    X, tets = fan_mesh(2000)
+    assert np.all(mesh.tet_volumes(X, tets) > 0)
+    sp = _abi.default_soft_params(E=1e5, nu=0.4)
+    pins = np.arange(2, 2 + 2000, 97, dtype=np.int32)
+    g = gs.Scene(precision=mode)
+    a = g.create_soft_actor(X, tets, sp)
+    a.add_boundary_condition_nodes(pins)
+    o = oracle.OracleScene(fp64=mode == _abi.PREC_MODE_FP64)
+    o.pin_nodes(o.add_soft_tet_mesh(X, tets, sp), pins)
+    if contact:  # every vertex at z=0 sits 1e-5 inside the plane's half-space
+        g.create_rigid_plane((0, 0, 1e-5), (0, 0, 1))
+        o.add_plane((0, 0, 1e-5), (0, 0, 1))
+    # smooth displacement (a random per-vertex one would invert the thin wedges)
+    rng = np.random.default_rng(1)
+    B = 1e-2 * rng.standard_normal((3, 3))
+    u = X @ B.T + 2.0 * (X[:, :1] * X[:, 1:2]) * rng.standard_normal(3)
+    u[pins] = 0
+    a.set_displacements(u)
+    o.set_displacements(0, u)
+    p = _abi.pinned_solver_params()
+    rg, pg, cg, vg = a.assemble(1e-3, p)
+    ro, po, co, vo = o.assemble(0, 1e-3, p)
+    np.testing.assert_array_equal(pg, po)
+    np.testing.assert_array_equal(cg, co)
+    assert pg[1] - pg[0] == 2002  # the hub couples to every vertex
+    np.testing.assert_allclose(rg, ro, rtol=tol, atol=tol * np.abs(ro).max())
+    np.testing.assert_allclose(vg, vo, rtol=tol, atol=tol * np.abs(vo).max())
+
+
 def run_gpu(spec, params, steps, precision=_abi.PREC_MODE_FP32):
     sc, rigid = gs.build_scene(spec, precision=precision)
     stats = []
