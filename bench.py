@@ -278,7 +278,11 @@ def main(argv=None) -> int:
     ls = np.array([[s["line_search_iters"] for s in st] for st in stats], dtype=np.float64)
     value = nT * steps / elapsed
     s_bytes = 8 if mode == _abi.PREC_MODE_FP64 else 4
-    per_launch = matvec_bytes(nB, nV, s_bytes)
+    # with environment groups, the timed first iterations run group by group:
+    # one launch per group covers 1/groups of the rows (groups of nearly equal
+    # tile counts), so a launch's algorithmic bytes are the full pass's / groups
+    groups = int(sc.info()["cg_groups"])
+    per_launch = matvec_bytes(nB, nV, s_bytes) / groups
     avg_ms = ms / launches if launches else None
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms else None
     # reference-equivalent bytes per step, environment by environment
@@ -316,8 +320,10 @@ def main(argv=None) -> int:
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
             "traffic": read_traffic(args.workload), "bytes_per_launch": per_launch,
             "avg_launch_us": 1e3 * avg_ms if avg_ms else None, "timed_launches": int(launches),
-            "timed": ("every launch of the first CG burst (all environments active) of every "
-                      f"{args.time_every}-th solve") if args.time_every > 0 else None,
+            "env_groups": groups,
+            "timed": ("every launch of the first CG burst (all environments active; the first burst of "
+                      f"each environment group when grouped) of every {args.time_every}-th solve")
+            if args.time_every > 0 else None,
         },
         "step": {"cg_iters_mean_per_env": float(it.mean()),
                  "cg_iters_tet_weighted": float((it * sizes).sum(axis=1).mean() / sizes.sum()),

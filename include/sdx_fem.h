@@ -251,7 +251,9 @@ sdx_status sdx_get_velocities(sdx_world* world, int32_t actor, double* out, int3
 sdx_status sdx_set_velocities(sdx_world* world, int32_t actor, const double* in, int32_t n);
 sdx_status sdx_get_positions_world(sdx_world* world, int32_t actor, double* out, int32_t n);
 
-/* World totals: counts over all actors (vertices, tets, BSR blocks, boundary triangles). */
+/* World totals: out[0..4] counts over all actors (vertices, tets, BSR blocks, boundary
+ * triangles, environments); out[5], out[6] the CG matvec and vector tiles; out[7] the
+ * environment groups of the grouped CG iterations under the current schedule (1: none). */
 sdx_status sdx_world_info(sdx_world* world, int64_t out[8]);
 
 /* Assembly of one actor at its current displacement, copied to the host:
@@ -288,6 +290,14 @@ typedef struct sdx_schedule {
   int32_t active_lists;   /* 1: each CG burst first compacts the tiles of the environments still
                              active into work lists and runs persistent CG kernels over them
                              (default 1) */
+  int32_t cg_groups;      /* with active_lists: the environments are split into this many
+                             contiguous groups of about equal matrix size, and the first
+                             group_iters CG iterations run group after group, so a group's
+                             matrix and vectors are re-read from the GPU's last-level cache
+                             instead of HBM; later iterations run over all environments
+                             (1..256, default 1: no groups) */
+  int32_t group_iters;    /* CG iterations run group by group, rounded up to whole bursts
+                             (0..4096, default 16; ignored when cg_groups is 1) */
 } sdx_schedule;
 
 sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* schedule);
@@ -295,7 +305,8 @@ sdx_status sdx_get_schedule(sdx_world* world, sdx_schedule* out);
 
 /* Autotuner: times sdx_step of the world's current state under candidate
  * schedules (coordinate search over matvec lanes, element workgroup size,
- * work lists, CG burst length and graph replay; the best of `trials` steps each),
+ * work lists, CG burst length and graph replay, then environment groups; the
+ * best of `trials` steps each),
  * restores the state after every trial and keeps the fastest schedule.
  * out / out_ms (may be NULL): the chosen schedule and its step time. */
 sdx_status sdx_autotune(sdx_world* world, double dt, const sdx_solver_params* params, int32_t trials,

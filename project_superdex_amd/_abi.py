@@ -80,6 +80,8 @@ class Schedule(C.Structure):
         ("spmv_lanes", C.c_int32),
         ("element_block", C.c_int32),
         ("active_lists", C.c_int32),
+        ("cg_groups", C.c_int32),
+        ("group_iters", C.c_int32),
     ]
 
     def as_dict(self):

@@ -179,12 +179,14 @@ template <class T, class A>
 void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int grid, const CgConfig& cfg, int k,
                       const T* dinv, int32_t nV, const T* w, const double* rref, A* x, T* r, T* z, T* p, T* sv,
                       EnvCG* env, A* acc, int32_t* num_active, const KernelEvents* ev = nullptr);
-// Work lists of the tiles of active environments (slot `par` of counts,
-// which must start at zero; slot par ^ 1 is cleared for the next burst).
+// Work lists of the tiles of active environments among tiles [a0, a1) of ta
+// and [v0, v1) of tv (slot `par` of counts, which must start at zero; slot
+// par ^ 1 is cleared for the next burst).
 // publish (may be NULL; device-visible pinned host memory): *publish =
 // *num_active, the active count left by the previous kernels.
-void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, const EnvCG* env, int32_t* la,
-                          int32_t* lv, int32_t* counts, int par, const int32_t* num_active, int32_t* publish);
+void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, int a0, int a1, int v0, int v1,
+                          const EnvCG* env, int32_t* la, int32_t* lv, int32_t* counts, int par,
+                          const int32_t* num_active, int32_t* publish);
 // FP64 residual b - A x of the environments with mask[e]: into r64 when
 // given; |b - A x|^2, |b|^2, b.Ax, |Ax|^2 into set 0 of acc when given
 // (zeroed by the caller).  Matvec tiles of kSpmvThreads / lanes rows.

@@ -1065,14 +1065,18 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, WorkList wl
 // {matvec list, matvec list (other parity), vector list, vector list (other
 // parity)}; this launch fills slot `par` and clears slot par ^ 1 for the next
 // burst.
-__global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, const EnvCG* __restrict__ env,
-                                                      int32_t* __restrict__ la, int32_t* __restrict__ lv,
-                                                      int32_t* counts, int par, const int32_t* num_active,
-                                                      int32_t* publish) {
-  const int nba = (ta.n + 255) / 256;
+__global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, int a0, int na, int v0, int nv,
+                                                      const EnvCG* __restrict__ env, int32_t* __restrict__ la,
+                                                      int32_t* __restrict__ lv, int32_t* counts, int par,
+                                                      const int32_t* num_active, int32_t* publish) {
+  // tiles [a0, a0 + na) of ta and [v0, v0 + nv) of tv: one environment group
+  // (or all of them)
+  const int nba = (na + 255) / 256;
   const bool isA = int(blockIdx.x) < nba;
   const Tiles& t = isA ? ta : tv;
+  const int n = isA ? na : nv;
   const int i = (isA ? int(blockIdx.x) : int(blockIdx.x) - nba) * 256 + int(threadIdx.x);
+  const int tile = (isA ? a0 : v0) + i;
   int32_t* list = isA ? la : lv;
   int32_t* cnt = counts + (isA ? 0 : 2) + par;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1080,7 +1084,7 @@ __global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, const
     counts[2 + (par ^ 1)] = 0;
     if (publish) *publish = *num_active;  // pinned host memory, read after this kernel's event
   }
-  const bool keep = i < t.n && env[t.env[i]].status == kCgActive;
+  const bool keep = i < n && env[t.env[tile]].status == kCgActive;
   const unsigned long long m = __ballot(keep);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   __shared__ int ws[4];
@@ -1094,7 +1098,7 @@ __global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, const
   __syncthreads();
   int off = base;
   for (int q = 0; q < wv; ++q) off += ws[q];
-  if (keep) list[off + __popcll(m & ((1ull << lane) - 1ull))] = i;
+  if (keep) list[off + __popcll(m & ((1ull << lane) - 1ull))] = tile;
 }
 
 // FP64 residual b - A x of the environments with mask[e]: stored in r64
@@ -1450,12 +1454,16 @@ void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int gr
                r, z, p, sv, env, acc, num_active);
 }
 
-void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, const EnvCG* env, int32_t* la,
-                          int32_t* lv, int32_t* counts, int par, const int32_t* num_active, int32_t* publish) {
-  const int nb = (ta.n + 255) / 256 + (tv.n + 255) / 256;
-  if (nb == 0) return;
-  hipLaunchKernelGGL(k_compact_tiles, dim3(nb), dim3(256), 0, s, ta, tv, env, la, lv, counts, par, num_active,
-                     publish);
+void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, int a0, int a1, int v0, int v1,
+                          const EnvCG* env, int32_t* la, int32_t* lv, int32_t* counts, int par,
+                          const int32_t* num_active, int32_t* publish) {
+  if (a0 < 0 || a1 < a0 || a1 > ta.n || v0 < 0 || v1 < v0 || v1 > tv.n)
+    throw std::runtime_error("compact_tiles: tile range outside the tiles");
+  const int na = a1 - a0, nv = v1 - v0;
+  // at least one workgroup: it clears the other parity's counts and publishes
+  const int nb = std::max(1, (na + 255) / 256 + (nv + 255) / 256);
+  hipLaunchKernelGGL(k_compact_tiles, dim3(nb), dim3(256), 0, s, ta, tv, a0, na, v0, nv, env, la, lv, counts, par,
+                     num_active, publish);
 }
 
 template <class T, class X>

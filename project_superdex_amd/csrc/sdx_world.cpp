@@ -558,20 +558,22 @@ std::vector<double> World<T>::read_env_sq() {
 
 // Launch CG burst m, iterations [k0, k0 + n), on the stream; with `timed`,
 // every matvec and update launch records its own dispatch events.  With
-// work lists, the burst starts by compacting the tiles of the environments
-// still active and its kernels are persistent over those lists.
+// work lists, the caller compacted the tiles of the environments still
+// active into the lists of parity `par`, and the kernels are persistent over
+// them (grids sized for the largest group in the grouped bursts m <
+// group_bursts_).
 template <class T>
-void World<T>::issue_cg(const CgConfig& cfg, int m, int k0, int n, bool timed) {
+void World<T>::issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed) {
   const bool dd = precision_ != SDX_PREC_MODE_FP32;
   T* accf = reinterpret_cast<T*>(acc_.p);
   WorkList wa{nullptr, nullptr}, wv{nullptr, nullptr};
   int ga = 0, gv = 0;
-  if (sched_.active_lists) {  // the caller launched compact_tiles for this burst
-    const int par = m & 1;
+  if (sched_.active_lists) {
     wa = WorkList{list_a_.p, list_count_.p + par};
     wv = WorkList{list_v_.p, list_count_.p + 2 + par};
-    ga = std::max(8, std::min(dt_a_.view.n, kPersistentGridA));
-    gv = std::max(8, std::min(dt_v_.view.n, kPersistentGridV));
+    const bool grouped = m < group_bursts_;
+    ga = std::max(8, std::min(grouped ? group_max_a_ : dt_a_.view.n, kPersistentGridA));
+    gv = std::max(8, std::min(grouped ? group_max_v_ : dt_v_.view.n, kPersistentGridV));
   }
   for (int k = k0; k < k0 + n; ++k) {
     const KernelEvents* e0 = timed ? timer_events(0) : nullptr;
@@ -592,28 +594,55 @@ void World<T>::issue_cg(const CgConfig& cfg, int m, int k0, int n, bool timed) {
   }
 }
 
-// CG burst m (iterations [m B, m B + B)) as a replayable graph, captured on
-// first use for the current configuration.
+// CG burst m (iterations [m B, m B + B)) over the lists of parity par as a
+// replayable graph, captured on first use for the current configuration.
 template <class T>
-hipGraphExec_t World<T>::cg_graph(const CgConfig& cfg, int m, int B) {
+hipGraphExec_t World<T>::cg_graph(const CgConfig& cfg, int m, int par, int B) {
   if (std::memcmp(&graph_cfg_, &cfg, sizeof(CgConfig)) != 0 || graph_burst_ != B) {
     drop_graphs();
     graph_cfg_ = cfg;
     graph_burst_ = B;
   }
-  if (size_t(m) >= graphs_.size()) graphs_.resize(size_t(m) + 1, nullptr);
-  if (!graphs_[size_t(m)]) {
+  const size_t key = 2 * size_t(m) + size_t(par);
+  if (key >= graphs_.size()) graphs_.resize(key + 1, nullptr);
+  if (!graphs_[key]) {
     hipGraph_t g = nullptr;
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    issue_cg(cfg, m, m * B, B, false);
+    issue_cg(cfg, m, par, m * B, B, false);
     hip_check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
     hipGraphExec_t ex = nullptr;
     const hipError_t err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     hip_check(err, "hipGraphInstantiate");
-    graphs_[size_t(m)] = ex;
+    graphs_[key] = ex;
   }
-  return graphs_[size_t(m)];
+  return graphs_[key];
+}
+
+// Environment groups of the grouped CG bursts: `n` contiguous ranges of
+// about equal matvec tile counts, as tile ranges of the matvec and the
+// update tiles; also the largest group's tile counts.
+template <class T>
+void World<T>::plan_groups(int n) {
+  group_plan_n_ = n;
+  group_a_.assign(1, 0);
+  group_v_.assign(1, 0);
+  group_max_a_ = group_max_v_ = 0;
+  const std::vector<int32_t>& fa = tiles_a_.first;
+  const std::vector<int32_t>& fv = tiles_v_.first;
+  const int64_t total = fa[size_t(nEnv_)];
+  int e = 0;
+  for (int g = 1; g <= n && e < nEnv_; ++g) {
+    const int64_t goal = total * g / n;
+    int e1 = e + 1;  // at least one environment per group
+    while (e1 < nEnv_ && fa[size_t(e1)] < goal) ++e1;
+    if (g == n) e1 = nEnv_;
+    group_a_.push_back(fa[size_t(e1)]);
+    group_v_.push_back(fv[size_t(e1)]);
+    group_max_a_ = std::max(group_max_a_, fa[size_t(e1)] - fa[size_t(e)]);
+    group_max_v_ = std::max(group_max_v_, fv[size_t(e1)] - fv[size_t(e)]);
+    e = e1;
+  }
 }
 
 template <class T>
@@ -702,20 +731,44 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
   const int B = std::max(1, sched_.cg_burst);
   volatile int32_t* hna = pinned_active_;
   const bool lists = sched_.active_lists != 0;
-  for (int m = 0;; ++m) {
+  const bool timed0 = burst_timed(0);
+  int q = 0;  // compactions issued: list parity q & 1
+  // Grouped bursts: the first group_bursts_ bursts run group after group, so
+  // that one group's matrix and vectors stay in the last-level cache across
+  // the group's iterations.  They are queued without host checks.
+  const int groups = lists ? std::min(std::max(1, sched_.cg_groups), nEnv_) : 1;
+  group_bursts_ = 0;
+  if (groups > 1 && sched_.group_iters > 0) {
+    group_bursts_ = std::min((sched_.group_iters + B - 1) / B, cfg.max_iter / B + 1);
+    if (group_plan_n_ != groups) plan_groups(groups);
+    const int ng = int(group_a_.size()) - 1;
+    for (int g = 0; g < ng; ++g)
+      for (int m = 0; m < group_bursts_; ++m, ++q) {
+        const bool timed = m == 0 && timed0;
+        launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, group_a_[size_t(g)], group_a_[size_t(g) + 1],
+                             group_v_[size_t(g)], group_v_[size_t(g) + 1], env_cg_.p, list_a_.p, list_v_.p,
+                             list_count_.p, q & 1, num_active_.p, nullptr);
+        if (sched_.use_graphs && !timed)
+          hip_check(hipGraphLaunch(cg_graph(cfg, m, q & 1, B), stream_), "hipGraphLaunch");
+        else
+          issue_cg(cfg, m, q & 1, m * B, B, timed);
+      }
+  }
+  for (int m = group_bursts_;; ++m) {
     const bool issue = int64_t(m) * B <= int64_t(cfg.max_iter);
     const int slot = m % kBurstSlots;
     if (issue) {
-      const bool timed = burst_timed(m);
+      const bool timed = m == 0 && timed0;
       if (lists) {
-        launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, env_cg_.p, list_a_.p, list_v_.p, list_count_.p, m & 1,
-                             num_active_.p, pinned_active_dev_ + slot);
+        launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, 0, dt_a_.view.n, 0, dt_v_.view.n, env_cg_.p, list_a_.p,
+                             list_v_.p, list_count_.p, q & 1, num_active_.p, pinned_active_dev_ + slot);
         hip_check(hipEventRecord(burst_done_[slot], stream_), "hipEventRecord");
       }
       if (sched_.use_graphs && !timed)
-        hip_check(hipGraphLaunch(cg_graph(cfg, m, B), stream_), "hipGraphLaunch");
+        hip_check(hipGraphLaunch(cg_graph(cfg, m, q & 1, B), stream_), "hipGraphLaunch");
       else
-        issue_cg(cfg, m, m * B, B, timed);
+        issue_cg(cfg, m, q & 1, m * B, B, timed);
+      ++q;
       if (!lists) {
         hip_check(hipMemcpyAsync(pinned_active_ + slot, num_active_.p, sizeof(int32_t), hipMemcpyDeviceToHost,
                                  stream_),
@@ -725,8 +778,10 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
     }
     if (!issue) break;  // every burst up to max_iter is queued
     const int j = m - kBurstAhead;  // check the count left by burst j
-    if (j >= 0) {
-      const int s = (lists ? j + 1 : j) % kBurstSlots;  // compaction of burst j + 1, or the copy after burst j
+    // the compaction of burst j + 1 (a batch-wide one, which publishes the
+    // count), or the copy after burst j
+    if (j >= 0 && (!lists || j + 1 >= group_bursts_)) {
+      const int s = (lists ? j + 1 : j) % kBurstSlots;
       hip_check(hipEventSynchronize(burst_done_[s]), "hipEventSynchronize");
       if (hna[s] <= 0) break;
     }
@@ -1196,6 +1251,7 @@ void World<T>::set_schedule(const sdx_schedule& sc) {
   const bool lanes_changed = sc.spmv_lanes != sched_.spmv_lanes;
   sched_ = sc;
   drop_graphs();
+  group_plan_n_ = 0;  // the groups' tile ranges follow the tiles
   if (finalized_ && lanes_changed) {
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     BuildTiles(env_v0_, nEnv_, kSpmvThreads / sched_.spmv_lanes, tiles_a_);
@@ -1279,6 +1335,17 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
         c.cg_burst = B;
         consider(c);
       }
+  if (cur.active_lists && nEnv_ > 1) {
+    const sdx_schedule base = cur;
+    for (int ng : {2, 4, 8})
+      for (int gi : {8, 16, 32})
+        if (ng <= nEnv_) {
+          sdx_schedule c = base;
+          c.cg_groups = ng;
+          c.group_iters = gi;
+          consider(c);
+        }
+  }
   set_schedule(cur);
   restore();
   tuned_ = true;
@@ -1296,6 +1363,13 @@ void World<T>::info(int64_t out[8]) {
   out[4] = nEnv_;
   out[5] = int64_t(tiles_a_.env.size());
   out[6] = int64_t(tiles_v_.env.size());
+  const int groups = sched_.active_lists ? std::min(std::max(1, sched_.cg_groups), nEnv_) : 1;
+  if (groups > 1 && sched_.group_iters > 0) {
+    if (group_plan_n_ != groups) plan_groups(groups);
+    out[7] = int64_t(group_a_.size()) - 1;
+  } else {
+    out[7] = 1;
+  }
 }
 
 template class World<float>;

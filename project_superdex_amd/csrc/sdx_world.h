@@ -220,7 +220,7 @@ class WorldBase {
     flush_timers();
     return {double(timers_[size_t(k)].launches), timers_[size_t(k)].ms};
   }
-  sdx_schedule sched_{8, 1, 16, 128, 1};
+  sdx_schedule sched_{8, 1, 16, 128, 1, 1, 16};
   bool tuned_ = false;  // the schedule was autotuned for the current topology
 
  protected:
@@ -252,8 +252,8 @@ class WorldBase {
     return timer_events(k);
   }
   // Whether CG burst m of the current solve is launched eagerly with its
-  // launches timed: the first burst (every environment still active) of
-  // every `every`-th solve.
+  // launches timed: the first burst (every environment still active; with
+  // environment groups, each group's first burst) of every `every`-th solve.
   bool burst_timed(int m) {
     if (m != 0 || !(timers_[0].enabled || timers_[1].enabled)) return false;
     Timer& t = timers_[0];
@@ -364,13 +364,20 @@ class World final : public WorldBase {
                             const std::vector<int32_t>& warm, bool decide);
   void ws_features();
   std::vector<double> true_residual(const std::vector<int32_t>& mask);
-  void issue_cg(const CgConfig& cfg, int m, int k0, int n, bool timed);
-  hipGraphExec_t cg_graph(const CgConfig& cfg, int m, int B);
+  void issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed);
+  void plan_groups(int n);
+  hipGraphExec_t cg_graph(const CgConfig& cfg, int m, int par, int B);
   void drop_graphs();
   template <class A>
   A* xcg() { return reinterpret_cast<A*>(xcg_.p); }
 
   std::vector<hipGraphExec_t> graphs_;
+  // environment groups of the grouped CG bursts (cg_groups): tile ranges
+  // [group_a_[g], group_a_[g + 1]) of the matvec tiles and likewise of the
+  // update tiles, the largest group's tile counts, the number of groups they
+  // were planned for (0: none) and the grouped bursts of the current solve
+  std::vector<int32_t> group_a_, group_v_;
+  int group_max_a_ = 0, group_max_v_ = 0, group_plan_n_ = 0, group_bursts_ = 0;
   CgConfig graph_cfg_{};
   int graph_burst_ = 0;
 

@@ -166,7 +166,7 @@ class Scene:
         out = (C.c_int64 * 8)()
         check(lib().sdx_world_info(self._w, out))
         return {"n_verts": out[0], "n_tets": out[1], "nnzb": out[2], "n_tris": out[3], "n_envs": out[4],
-                "matvec_tiles": out[5], "vector_tiles": out[6]}
+                "matvec_tiles": out[5], "vector_tiles": out[6], "cg_groups": out[7]}
 
     def set_cg_burst(self, burst: int):
         check(lib().sdx_set_cg_burst(self._w, burst))

@@ -5,7 +5,8 @@
   at or below 1e-6, iteration counts stay within 15% of the oracle's
   (standard PCG) and positions match the oracle as in the parity tests;
 - every launch schedule (matvec lanes, element workgroup, burst length,
-  graph replay, persistent kernels over work lists of active tiles) steps to the same state: displacements within 1e-3 of the
+  graph replay, persistent kernels over work lists of active tiles, the
+  first iterations run environment group by group) steps to the same state: displacements within 1e-3 of the
   largest displacement (FP32 solves at rel. residual 1e-6 differ at the
   1e-4 level when only the summation order changes; a wrong result is off
   by O(1)), iteration counts within 2;
@@ -60,6 +61,9 @@ def state(sc):
     dict(element_block=256, use_graphs=0, cg_burst=1),
     dict(active_lists=0),
     dict(active_lists=0, use_graphs=0, spmv_lanes=8),
+    dict(cg_groups=3, group_iters=8, cg_burst=4),
+    dict(cg_groups=6, group_iters=5, cg_burst=2, use_graphs=0),  # 3 bursts per group: list parity flips
+    dict(cg_groups=4, group_iters=64, cg_burst=8, spmv_lanes=8),  # every iteration grouped
 ])
 def test_schedules_step_to_the_same_state(sched):
     # one Newton iteration per step: a second one starts from a residual at
@@ -71,6 +75,9 @@ def test_schedules_step_to_the_same_state(sched):
     sc, rigid = gs.build_scene(spec, precision=MIXED)
     sc.set_schedule(**sched)
     assert {k: sc.schedule()[k] for k in sched} == sched
+    # groups of about equal size: at most the requested number
+    g = sc.info()["cg_groups"]
+    assert g == 1 if sched.get("cg_groups", 1) == 1 else 1 < g <= sched["cg_groups"]
     st1 = []
     for s in range(5):
         for r, c in zip(rigid, spec.colliders):
