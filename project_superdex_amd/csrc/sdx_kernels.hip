@@ -294,31 +294,45 @@ struct Vec16<double> {
 template <class T>
 constexpr int kStage = 16384 / int(sizeof(T));
 
-// The workgroup copies src[c0, c1) (at most CH values) into lds with 16-byte
-// loads, all of a thread's loads issued before its first LDS store so the
-// copy waits on memory once; c0 is a multiple of the vector width and src is
-// padded by one vector, so the last vector may run past c1 but never past the
-// allocation.
+// The workgroup copies src[c0, c1) (at most CH values, c1 > c0) into lds
+// with 16-byte loads: stage_load issues a thread's loads into registers,
+// stage_store writes them to LDS.  The loads are unconditional (a thread
+// past the end reloads the last vector), so no branch separates them and a
+// thread's loads are all in flight before the first wait; c0 is a multiple
+// of the vector width and src is padded by one vector, so the last vector
+// may run past c1 but never past the allocation.
+template <class T, int NT, int CH>
+struct Stage {
+  using V = typename Vec16<T>::type;
+  static constexpr int kW = 16 / int(sizeof(T));
+  static constexpr int kU = CH / (kW * NT);  // vectors per thread
+  static_assert(kU * kW * NT == CH, "chunk must be a whole number of vectors per thread");
+  V v[kU];
+  int nv;
+  __device__ __forceinline__ void load(const T* __restrict__ src, int64_t c0, int64_t c1) {
+    const V* s = reinterpret_cast<const V*>(src + c0);
+    nv = int((c1 - c0 + kW - 1) / kW);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int k = int(threadIdx.x) + u * NT;
+      v[u] = s[k < nv ? k : nv - 1];
+    }
+  }
+  __device__ __forceinline__ void store(T* lds) const {
+    V* d = reinterpret_cast<V*>(lds);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int k = int(threadIdx.x) + u * NT;
+      if (k < nv) d[k] = v[u];
+    }
+  }
+};
+
 template <class T, int NT, int CH>
 __device__ __forceinline__ void stage_chunk(const T* __restrict__ src, int64_t c0, int64_t c1, T* lds) {
-  using V = typename Vec16<T>::type;
-  constexpr int kW = 16 / int(sizeof(T));
-  constexpr int kU = CH / (kW * NT);  // vectors per thread
-  static_assert(kU * kW * NT == CH, "chunk must be a whole number of vectors per thread");
-  const V* s = reinterpret_cast<const V*>(src + c0);
-  V* d = reinterpret_cast<V*>(lds);
-  const int nv = int((c1 - c0 + kW - 1) / kW);
-  V v[kU];
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    const int k = int(threadIdx.x) + u * NT;
-    if (k < nv) v[u] = s[k];
-  }
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    const int k = int(threadIdx.x) + u * NT;
-    if (k < nv) d[k] = v[u];
-  }
+  Stage<T, NT, CH> st;
+  st.load(src, c0, c1);
+  st.store(lds);
 }
 
 // Records k of [k0, k1) whose component e (of W per record) lies in the
@@ -339,35 +353,14 @@ __device__ __forceinline__ T sum_run(const T* lds, int k0, int k1, int e, int64_
   return s0 + s1;
 }
 
-// BSR gather workgroups: four waves of seven 9-lane items per round; the
-// records of a workgroup's items are staged in chunks of 16 KB
-// (four 16-byte loads per thread).  Measured on 512 batched cubes: one round
-// of 28 items per workgroup is as fast as workgroups of 1 or 2 waves; 2 or 4
-// rounds per workgroup, or eight loads per thread, are 1.2x to 4x slower.
-// Chunks are kStage<T> values.
+// BSR gather workgroups: four waves of seven 9-lane items; the records of a
+// workgroup's items are staged in chunks of 16 KB (four 16-byte loads per
+// thread).  Measured on 512 batched cubes: 28 items per workgroup is as fast
+// as 56 (eight waves, or two rounds of four waves) and as workgroups of 1 or
+// 2 waves; one lane per item summing its records straight from global memory
+// is 1.8x slower.
 constexpr int kGatherWaves = 4;
-constexpr int kGatherRounds = 1;
-constexpr int kGatherRoundItems = 7 * kGatherWaves;
-constexpr int kGatherItems = kGatherRoundItems * kGatherRounds;
-
-template <class T, int W>
-__device__ __forceinline__ void gather_rounds(const T* __restrict__ src, int rec0, int rec1,
-                                              const int4 (&ia)[kGatherRounds], const bool (&on)[kGatherRounds],
-                                              int e, T (&acc)[kGatherRounds], T* lds) {
-  constexpr int kW = 16 / int(sizeof(T));
-  const int64_t end = int64_t(rec1) * W;
-#pragma unroll
-  for (int t = 0; t < kGatherRounds; ++t) acc[t] = T(0);
-  for (int64_t c0 = (int64_t(rec0) * W) & ~int64_t(kW - 1); c0 < end; c0 += kStage<T>) {
-    const int64_t c1 = c0 + kStage<T> < end ? c0 + kStage<T> : end;
-    stage_chunk<T, 64 * kGatherWaves, kStage<T>>(src, c0, c1, lds);
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < kGatherRounds; ++t)
-      if (on[t]) acc[t] += sum_run<T, W>(lds, ia[t].z, ia[t].w, e, c0, c1);
-    __syncthreads();
-  }
-}
+constexpr int kGatherItems = 7 * kGatherWaves;
 
 template <class T>
 __global__ void __launch_bounds__(64 * kGatherWaves) k_gather_bsr(MeshView m, const T* __restrict__ Dv,
@@ -382,11 +375,12 @@ __global__ void __launch_bounds__(64 * kGatherWaves) k_gather_bsr(MeshView m, co
   // pair-sorted off-diagonal records, element and contact alike), so a
   // workgroup stages the records of its items through LDS with 16-byte
   // loads and the lanes sum from LDS.  Workgroups [0, n_diag_blocks) take
-  // diagonal items, the rest upper items.  The descriptions of all rounds
-  // (slot, mirror, record range; row with the Dirichlet flags of its row and
-  // column DoFs, environment) are loaded with the workgroup's record range
-  // before anything waits, so a workgroup waits on memory twice: for the
-  // descriptions, then for the records and the environment masks.
+  // diagonal items, the rest upper items.  Every load before the first wait
+  // is unconditional (lanes without an item load the workgroup's first
+  // item), so the item descriptions (slot, mirror, record range; row with the
+  // Dirichlet flags of its row and column DoFs, environment), the
+  // workgroup's record range and the records are in flight together, and the
+  // environment masks follow the descriptions.
   __shared__ typename Vec16<T>::type lds_v[kStage<T> * sizeof(T) / 16];
   T* lds = reinterpret_cast<T*>(lds_v);
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
@@ -394,63 +388,72 @@ __global__ void __launch_bounds__(64 * kGatherWaves) k_gather_bsr(MeshView m, co
   const int first = diag ? blk * kGatherItems : m.nV + (blk - n_diag_blocks) * kGatherItems;
   const int last = min(first + kGatherItems, diag ? m.nV : m.n_items);  // exclusive
   const int rec0 = m.item_a[4 * first + 2], rec1 = m.item_a[4 * (last - 1) + 3];
+  const int W = diag ? 6 : 9;
+  const T* src = diag ? Dv : Ev;
+  constexpr int kW = 16 / int(sizeof(T));
+  const int64_t start = (int64_t(rec0) * W) & ~int64_t(kW - 1), end = int64_t(rec1) * W;
+  // the first chunk's records, then the item descriptions, all in flight
+  // together (a workgroup of vertices in no element or triangle has no records)
+  const bool any = start < end;
+  Stage<T, 64 * kGatherWaves, kStage<T>> st;
+  int64_t c1 = start + kStage<T> < end ? start + kStage<T> : end;
+  if (any) st.load(src, start, c1);
   const int lane = int(threadIdx.x) % 64;
   const int g = lane / 9, q = lane % 9;
   const int item0 = first + (int(threadIdx.x) / 64) * 7 + g;
-  int4 ia[kGatherRounds];
-  int2 ib[kGatherRounds];
-  bool on[kGatherRounds];
-#pragma unroll
-  for (int t = 0; t < kGatherRounds; ++t) {
-    const int item = item0 + t * kGatherRoundItems;
-    const bool mine = g < 7 && item < last;
-    ia[t] = mine ? reinterpret_cast<const int4*>(m.item_a)[item] : make_int4(0, 0, 0, 0);
-    ib[t] = mine ? reinterpret_cast<const int2*>(m.item_b)[item] : make_int2(0, 0);
-    on[t] = mine;
-  }
-#pragma unroll
-  for (int t = 0; t < kGatherRounds; ++t) on[t] = on[t] && m.env_mask[ib[t].y] != 0;
+  const bool mine = g < 7 && item0 < last;
+  const int item = mine ? item0 : first;
+  const int4 ia = reinterpret_cast<const int4*>(m.item_a)[item];
+  const int2 ib = reinterpret_cast<const int2*>(m.item_b)[item];
+  const int msk = m.env_mask[ib.y];
   const int r = q / 3, c = q % 3;
-  T acc[kGatherRounds];
-  if (diag)
-    gather_rounds<T, 6>(Dv, rec0, rec1, ia, on, sym_entry(r, c), acc, lds);
-  else
-    gather_rounds<T, 9>(Ev, rec0, rec1, ia, on, q, acc, lds);
-  const int nB = m.nB;
-#pragma unroll
-  for (int t = 0; t < kGatherRounds; ++t) {
-    if (!on[t]) continue;
-    T v = acc[t];
-    // Dirichlet rows / columns (mochi_solve.cpp:337-396); the test is symmetric in (i r, j c)
-    const int pins = ib[t].x >> kItemRowBits;
-    if (((pins >> r) | (pins >> (3 + c))) & 1) v = (diag && r == c) ? T(1) : T(0);
-    vals[(size_t)q * nB + ia[t].x] = v;
-    if (!diag) {
-      vals[(size_t)(3 * c + r) * nB + ia[t].y] = v;
-      continue;
-    }
-    // diagonal block: every lane of the item gathers the 9 entries and
-    // writes its entry of the preconditioner block
-    const int base = g * 9;
-    T blk9[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) blk9[k] = __shfl(v, base + k, 64);
-    T o[9];
-    if (prec == 2) {
-      inv3(blk9, o);
-    } else {
-#pragma unroll
-      for (int k = 0; k < 9; ++k) o[k] = T(0);
-      if (prec == 1) {
-        o[0] = T(1) / blk9[0];
-        o[4] = T(1) / blk9[4];
-        o[8] = T(1) / blk9[8];
-      } else {
-        o[0] = o[4] = o[8] = T(1);
-      }
-    }
-    if (r <= c) dinv[(size_t)sym_entry(r, c) * m.nV + (ib[t].x & ((1 << kItemRowBits) - 1))] = o[q];
+  const int e = diag ? sym_entry(r, c) : q;
+  if (any) st.store(lds);
+  __syncthreads();
+  const bool on = mine && msk != 0;
+  T acc = T(0);
+  for (int64_t c0 = start; any;) {
+    if (on) acc += diag ? sum_run<T, 6>(lds, ia.z, ia.w, e, c0, c1) : sum_run<T, 9>(lds, ia.z, ia.w, e, c0, c1);
+    c0 += kStage<T>;
+    if (c0 >= end) break;  // uniform over the workgroup
+    __syncthreads();
+    c1 = c0 + kStage<T> < end ? c0 + kStage<T> : end;
+    stage_chunk<T, 64 * kGatherWaves, kStage<T>>(src, c0, c1, lds);
+    __syncthreads();
   }
+  if (!on) return;
+  const int nB = m.nB;
+  T v = acc;
+  // Dirichlet rows / columns (mochi_solve.cpp:337-396); the test is symmetric in (i r, j c)
+  const int pins = ib.x >> kItemRowBits;
+  if (((pins >> r) | (pins >> (3 + c))) & 1) v = (diag && r == c) ? T(1) : T(0);
+  vals[(size_t)q * nB + ia.x] = v;
+  if (!diag) {
+    vals[(size_t)(3 * c + r) * nB + ia.y] = v;
+    return;
+  }
+  // diagonal block: every lane of the item gathers the 9 entries and writes
+  // its entry of the preconditioner block (the lanes of one item are all on
+  // or all off, and a diagonal workgroup has no upper items)
+  const int base = g * 9;
+  T blk9[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) blk9[k] = __shfl(v, base + k, 64);
+  T o[9];
+  if (prec == 2) {
+    inv3(blk9, o);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[k] = T(0);
+    if (prec == 1) {
+      o[0] = T(1) / blk9[0];
+      o[4] = T(1) / blk9[4];
+      o[8] = T(1) / blk9[8];
+    } else {
+      o[0] = o[4] = o[8] = T(1);
+    }
+  }
+  if (r <= c) dinv[(size_t)sym_entry(r, c) * m.nV + (ib.x & ((1 << kItemRowBits) - 1))] = o[q];
 }
 
 template <class T>
@@ -495,22 +498,35 @@ __global__ void __launch_bounds__(kVecThreads) k_gather_res(MeshView m, Tiles tv
   constexpr int kW = 16 / int(sizeof(T));
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int env = tv.env[tile];
-  if (!m.env_mask[env]) return;  // uniform over the tile
   const int row0 = tv.row0[tile], row1 = tv.row1[tile];
+  const int msk = m.env_mask[env];
+  // the first chunk's records and the thread's record range are loaded
+  // before the (uniform) environment mask is tested
+  // (a tile of vertices in no element or triangle has no records)
+  const int64_t start = (int64_t(m.vtx_ptr[row0]) * 3) & ~int64_t(kW - 1), end = int64_t(m.vtx_ptr[row1]) * 3;
+  const bool any = start < end;
+  Stage<T, kVecThreads, kStage<T>> st;
+  int64_t c1 = start + kStage<T> < end ? start + kStage<T> : end;
+  if (any) st.load(Rv, start, c1);
   const int i = row0 + int(threadIdx.x);
   const bool mine = i < row1;
-  const int k0 = mine ? m.vtx_ptr[i] : 0, k1 = mine ? m.vtx_ptr[i + 1] : 0;
-  const int64_t end = int64_t(m.vtx_ptr[row1]) * 3;
+  const int ic = mine ? i : row0;
+  const int k0 = m.vtx_ptr[ic], k1 = m.vtx_ptr[ic + 1];
+  if (!msk) return;  // uniform over the tile
+  if (any) st.store(lds);
+  __syncthreads();
   T r0 = T(0), r1 = T(0), r2 = T(0);
-  for (int64_t c0 = (int64_t(m.vtx_ptr[row0]) * 3) & ~int64_t(kW - 1); c0 < end; c0 += kStage<T>) {
-    const int64_t c1 = c0 + kStage<T> < end ? c0 + kStage<T> : end;
-    stage_chunk<T, kVecThreads, kStage<T>>(Rv, c0, c1, lds);
-    __syncthreads();
+  for (int64_t c0 = start; any;) {
     if (mine) {
       r0 += sum_run<T, 3>(lds, k0, k1, 0, c0, c1);
       r1 += sum_run<T, 3>(lds, k0, k1, 1, c0, c1);
       r2 += sum_run<T, 3>(lds, k0, k1, 2, c0, c1);
     }
+    c0 += kStage<T>;
+    if (c0 >= end) break;  // uniform over the workgroup
+    __syncthreads();
+    c1 = c0 + kStage<T> < end ? c0 + kStage<T> : end;
+    stage_chunk<T, kVecThreads, kStage<T>>(Rv, c0, c1, lds);
     __syncthreads();
   }
   A part[1] = {A(0)};

@@ -186,9 +186,11 @@ def fan_mesh(m, radius=0.05, height=0.02):
 @pytest.mark.parametrize("mode,tol", [(_abi.PREC_MODE_FP64, 1e-9), (_abi.PREC_MODE_FP32, 2e-4)])
 @pytest.mark.parametrize("contact", [False, True])
 def test_assembly_long_runs_match_oracle(oracle, mode, tol, contact):
-    """2000 tets on one vertex and one edge: residual and BSR runs longer than one LDS chunk."""
+    """2000 tets on one vertex and one edge: residual and BSR runs longer than one LDS chunk;
+    400 vertices in no tet after them: whole gather workgroups without records."""
     X, tets = fan_mesh(2000)
     assert np.all(mesh.tet_volumes(X, tets) > 0)
+    X = np.vstack([X, np.random.default_rng(2).uniform(1.0, 2.0, (400, 3))])
     sp = _abi.default_soft_params(E=1e5, nu=0.4)
     pins = np.arange(2, 2 + 2000, 97, dtype=np.int32)
     g = gs.Scene(precision=mode)
