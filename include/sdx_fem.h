@@ -295,7 +295,8 @@ typedef struct sdx_schedule {
                              group_iters CG iterations run group after group, so a group's
                              matrix and vectors are re-read from the GPU's last-level cache
                              instead of HBM; later iterations run over all environments
-                             (1..256, default 1: no groups) */
+                             (1..256; 1: no groups; default 0: groups of about 200 MB of
+                             matrix values and column indices, sdx_world_info out[7]) */
   int32_t group_iters;    /* CG iterations run group by group, rounded up to whole bursts
                              (0..4096, default 16; ignored when cg_groups is 1) */
 } sdx_schedule;

@@ -376,7 +376,7 @@ sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* sc) {
   SDX_REQUIRE(sc->element_block == 64 || sc->element_block == 128 || sc->element_block == 256,
               "element_block must be 64, 128 or 256");
   SDX_REQUIRE(sc->active_lists == 0 || sc->active_lists == 1, "active_lists must be 0 or 1");
-  SDX_REQUIRE(sc->cg_groups >= 1 && sc->cg_groups <= 256, "cg_groups must be in 1..256");
+  SDX_REQUIRE(sc->cg_groups >= 0 && sc->cg_groups <= 256, "cg_groups must be in 0..256");
   SDX_REQUIRE(sc->group_iters >= 0 && sc->group_iters <= 4096, "group_iters must be in 0..4096");
   return guarded([&]() -> sdx_status {
     world->impl->set_schedule(*sc);

@@ -619,6 +619,19 @@ hipGraphExec_t World<T>::cg_graph(const CgConfig& cfg, int m, int par, int B) {
   return graphs_[key];
 }
 
+// Environment groups asked for by the schedule; cg_groups 0 sizes them to
+// about kGroupBytes of matrix values and column indices each, which with the
+// group's vectors stays within the 256 MB last-level cache.
+template <class T>
+int World<T>::cg_groups(const sdx_schedule& sc) const {
+  int n = sc.cg_groups;
+  if (n == 0) {
+    const double bytes = double(nB_) * double(9 * sizeof(T) + sizeof(int32_t));
+    n = int(std::lround(bytes / kGroupBytes));
+  }
+  return std::min(std::max(1, n), std::max(1, nEnv_));
+}
+
 // Environment groups of the grouped CG bursts: `n` contiguous ranges of
 // about equal matvec tile counts, as tile ranges of the matvec and the
 // update tiles; also the largest group's tile counts.
@@ -736,7 +749,7 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
   // Grouped bursts: the first group_bursts_ bursts run group after group, so
   // that one group's matrix and vectors stay in the last-level cache across
   // the group's iterations.  They are queued without host checks.
-  const int groups = lists ? std::min(std::max(1, sched_.cg_groups), nEnv_) : 1;
+  const int groups = lists ? cg_groups(sched_) : 1;
   group_bursts_ = 0;
   if (groups > 1 && sched_.group_iters > 0) {
     group_bursts_ = std::min((sched_.group_iters + B - 1) / B, cfg.max_iter / B + 1);
@@ -1337,14 +1350,21 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
       }
   if (cur.active_lists && nEnv_ > 1) {
     const sdx_schedule base = cur;
-    for (int ng : {2, 4, 8})
-      for (int gi : {8, 16, 32})
-        if (ng <= nEnv_) {
-          sdx_schedule c = base;
-          c.cg_groups = ng;
+    for (int ng : {1, 2, 3, 4, 6, 8})
+      if (ng <= nEnv_ && ng != cg_groups(base)) {
+        sdx_schedule c = base;
+        c.cg_groups = ng;
+        consider(c);
+      }
+    if (cg_groups(cur) > 1) {
+      const sdx_schedule chosen = cur;
+      for (int gi : {8, 24, 32})
+        if (gi != chosen.group_iters) {
+          sdx_schedule c = chosen;
           c.group_iters = gi;
           consider(c);
         }
+    }
   }
   set_schedule(cur);
   restore();
@@ -1363,7 +1383,7 @@ void World<T>::info(int64_t out[8]) {
   out[4] = nEnv_;
   out[5] = int64_t(tiles_a_.env.size());
   out[6] = int64_t(tiles_v_.env.size());
-  const int groups = sched_.active_lists ? std::min(std::max(1, sched_.cg_groups), nEnv_) : 1;
+  const int groups = sched_.active_lists ? cg_groups(sched_) : 1;
   if (groups > 1 && sched_.group_iters > 0) {
     if (group_plan_n_ != groups) plan_groups(groups);
     out[7] = int64_t(group_a_.size()) - 1;

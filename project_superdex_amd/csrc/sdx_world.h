@@ -220,7 +220,7 @@ class WorldBase {
     flush_timers();
     return {double(timers_[size_t(k)].launches), timers_[size_t(k)].ms};
   }
-  sdx_schedule sched_{8, 1, 16, 128, 1, 1, 16};
+  sdx_schedule sched_{8, 1, 16, 128, 1, 0, 16};
   bool tuned_ = false;  // the schedule was autotuned for the current topology
 
  protected:
@@ -366,6 +366,8 @@ class World final : public WorldBase {
   std::vector<double> true_residual(const std::vector<int32_t>& mask);
   void issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed);
   void plan_groups(int n);
+  int cg_groups(const sdx_schedule& sc) const;
+  static constexpr double kGroupBytes = 200e6;
   hipGraphExec_t cg_graph(const CgConfig& cfg, int m, int par, int B);
   void drop_graphs();
   template <class A>
