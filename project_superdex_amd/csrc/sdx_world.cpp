@@ -668,9 +668,9 @@ void World<T>::drop_graphs() {
 // One CG run for the environments with solve[e]; those with warm[e] keep
 // dx_ as the initial guess.  Returns the per-environment CG state.
 template <class T>
-std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve,
-                                    const std::vector<int32_t>& warm, bool decide) {
-  std::vector<EnvCG> h(static_cast<size_t>(nEnv_));
+bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, const std::vector<int32_t>& warm,
+                      bool decide, std::vector<EnvCG>& h) {
+  h.assign(static_cast<size_t>(nEnv_), EnvCG{});
   int active = 0, n_warm = 0;
   for (int e = 0; e < nEnv_; ++e) {
     std::memset(&h[size_t(e)], 0, sizeof(EnvCG));
@@ -678,7 +678,7 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
     active += solve[size_t(e)] ? 1 : 0;
     n_warm += solve[size_t(e)] && warm[size_t(e)] ? 1 : 0;
   }
-  if (active == 0) return h;
+  if (active == 0) return false;
   put(solve_mask_, solve);
   std::vector<int32_t> na(1, active);
   put(num_active_, na);
@@ -799,14 +799,13 @@ std::vector<EnvCG> World<T>::cg_run(const CgConfig& cfg, const std::vector<int32
       if (hna[s] <= 0) break;
     }
   }
-  get(env_cg_, h);
-  return h;
+  return true;
 }
 
-// FP64 ||b - A x|| / ||b|| of every environment with mask[e].
+// FP64 ||b - A x||^2 and ||b||^2 of every environment with mask[e] into
+// norms2_, queued on the stream.
 template <class T>
-std::vector<double> World<T>::true_residual(const std::vector<int32_t>& mask) {
-  std::vector<double> q(size_t(2 * nEnv_)), out(size_t(nEnv_), 0.0);
+void World<T>::issue_true_residual(const std::vector<int32_t>& mask) {
   put(solve_mask_, mask);
   launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
   if (precision_ != SDX_PREC_MODE_FP32)
@@ -816,7 +815,12 @@ std::vector<double> World<T>::true_residual(const std::vector<int32_t>& mask) {
     launch_residual64<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, solve_mask_.p, d_row_ptr_.p, d_col_.p,
                             vals_.p, nB_, xcg<T>(), res_.p, nullptr, acc_.p);
   launch_acc_finalize<double>(stream_, nEnv_, 2, acc_.p, norms2_.p);
-  get(norms2_, q);
+}
+
+// ||b - A x|| / ||b|| from the downloaded norms2_.
+template <class T>
+std::vector<double> World<T>::true_rel_from(const std::vector<int32_t>& mask, const std::vector<double>& q) const {
+  std::vector<double> out(size_t(nEnv_), 0.0);
   for (int e = 0; e < nEnv_; ++e)
     if (mask[size_t(e)] && q[size_t(2 * e + 1)] > 0) out[size_t(e)] = std::sqrt(q[size_t(2 * e)] / q[size_t(2 * e + 1)]);
   return out;
@@ -855,14 +859,21 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
     else
       launch_convert<T, T>(stream_, n3, dx_.p, xcg<T>());
   }
-  std::vector<EnvCG> h = cg_run(cfg, todo, warm, learned != nullptr);
+  // the solve, the true residuals and every download, then one wait
+  std::vector<EnvCG> h;
+  const bool ran = cg_run(cfg, todo, warm, learned != nullptr, h);
+  issue_true_residual(solve);
+  std::vector<double> q(size_t(2 * nEnv_));
+  if (ran) fetch(env_cg_, h);
   if (learned) {
     learned->accepted.assign(size_t(nEnv_), 0);
     learned->ratio.assign(size_t(nEnv_), 0.0);
-    get(warm_mask_, learned->accepted);
-    get(ws_ratio_, learned->ratio);
+    fetch(warm_mask_, learned->accepted);
+    fetch(ws_ratio_, learned->ratio);
   }
-  std::vector<double> tr = true_residual(solve);
+  fetch(norms2_, q);
+  fetch_all();
+  std::vector<double> tr = true_rel_from(solve, q);
   const bool confirm = precision_ == SDX_PREC_MODE_MIXED && kp.norm_type == SDX_NORM_RESIDUAL_L2;
   for (int round = 0; confirm && round < 2; ++round) {
     int n = 0;
@@ -876,8 +887,14 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
     }
     if (n == 0) break;
     CgConfig c2 = cfg;
-    std::vector<EnvCG> h2 = cg_run(c2, todo, warm, false);
-    std::vector<double> tr2 = true_residual(todo);
+    std::vector<EnvCG> h2;
+    const bool ran2 = cg_run(c2, todo, warm, false, h2);
+    issue_true_residual(todo);
+    std::vector<double> q2(size_t(2 * nEnv_));
+    if (ran2) fetch(env_cg_, h2);
+    fetch(norms2_, q2);
+    fetch_all();
+    std::vector<double> tr2 = true_rel_from(todo, q2);
     for (int e = 0; e < nEnv_; ++e) {
       if (!todo[size_t(e)]) continue;
       const int prev = h[size_t(e)].iter;

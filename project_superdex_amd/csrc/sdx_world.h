@@ -150,6 +150,38 @@ class WorldBase {
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     std::memcpy(h.data(), dl_, bytes);
   }
+  // Queued downloads: fetch() records a copy of h.size() values of d into h,
+  // fetch_all() issues every recorded copy through pinned memory and waits
+  // once.  h must not be resized before fetch_all().
+  template <class U>
+  void fetch(DBuf<U>& d, std::vector<U>& h) {
+    if (h.empty()) return;
+    if (!d.p || d.n < h.size()) throw std::logic_error("device buffer too small");
+    fetches_.push_back(Fetch{d.p, h.data(), h.size() * sizeof(U)});
+  }
+  void fetch_all() {
+    size_t total = 0;
+    for (const Fetch& f : fetches_) total += (f.bytes + 255) & ~size_t(255);
+    if (total == 0) return;
+    if (dl_cap_ < total) {  // no copy into dl_ is pending: every download waits
+      if (dl_) hip_check(hipHostFree(dl_), "hipHostFree");
+      dl_ = nullptr;
+      dl_cap_ = std::max<size_t>(total, size_t(1) << 16);
+      hip_check(hipHostMalloc(reinterpret_cast<void**>(&dl_), dl_cap_), "hipHostMalloc");
+    }
+    size_t off = 0;
+    for (const Fetch& f : fetches_) {
+      hip_check(hipMemcpyAsync(dl_ + off, f.src, f.bytes, hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+      off += (f.bytes + 255) & ~size_t(255);
+    }
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    off = 0;
+    for (const Fetch& f : fetches_) {
+      std::memcpy(f.dst, dl_ + off, f.bytes);
+      off += (f.bytes + 255) & ~size_t(255);
+    }
+    fetches_.clear();
+  }
   int device() const { return device_; }
   int precision() const { return precision_; }
 
@@ -286,6 +318,12 @@ class WorldBase {
   size_t stage_cap_ = 0, stage_off_ = 0;
   char* dl_ = nullptr;  // pinned download buffer
   size_t dl_cap_ = 0;
+  struct Fetch {
+    const void* src;
+    void* dst;
+    size_t bytes;
+  };
+  std::vector<Fetch> fetches_;
   void* stage_take(size_t bytes) {
     const size_t need = (bytes + 255) & ~size_t(255);
     if (stage_off_ + need > stage_cap_) {
@@ -360,10 +398,15 @@ class World final : public WorldBase {
   std::vector<EnvCG> cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve,
                               std::vector<double>* true_rel, const std::vector<int32_t>* warm,
                               WarmStats* learned = nullptr);
-  std::vector<EnvCG> cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve,
-                            const std::vector<int32_t>& warm, bool decide);
+  // Runs CG on the device; false if no environment had anything to solve (h
+  // then holds the final states), else the states are on the device (env_cg_).
+  bool cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, const std::vector<int32_t>& warm,
+              bool decide, std::vector<EnvCG>& h);
   void ws_features();
-  std::vector<double> true_residual(const std::vector<int32_t>& mask);
+  // FP64 ||b - A x||^2 and ||b||^2 of every environment with mask[e] into
+  // norms2_ (queued); true_rel_from turns them into relative residuals.
+  void issue_true_residual(const std::vector<int32_t>& mask);
+  std::vector<double> true_rel_from(const std::vector<int32_t>& mask, const std::vector<double>& q) const;
   void issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed);
   void plan_groups(int n);
   int cg_groups(const sdx_schedule& sc) const;
