@@ -493,18 +493,47 @@ void World<T>::upload_tiles(const HostTiles& h, DevTiles& d) {
   d.view.n = int32_t(h.env.size());
 }
 
+// The colliders of each environment (those of every actor and its own), in
+// collider order: own colliders bucketed by actor, then merged per
+// environment with the shared ones, in O(environments + colliders).
 template <class T>
 void World<T>::upload_colliders() {
+  const int nC = int(colliders_.size());
+  std::vector<int32_t> shared, own_first(static_cast<size_t>(nEnv_) + 1, 0), own(static_cast<size_t>(nC));
+  for (int i = 0; i < nC; ++i) {
+    const int a = colliders_[size_t(i)].actor;
+    if (a < 0)
+      shared.push_back(i);
+    else if (a < nEnv_)
+      ++own_first[size_t(a) + 1];
+  }
+  for (int e = 0; e < nEnv_; ++e) own_first[size_t(e) + 1] += own_first[size_t(e)];
+  {
+    std::vector<int32_t> fill(own_first.begin(), own_first.end() - 1);
+    for (int i = 0; i < nC; ++i) {
+      const int a = colliders_[size_t(i)].actor;
+      if (a >= 0 && a < nEnv_) own[size_t(fill[size_t(a)]++)] = i;
+    }
+  }
   std::vector<int32_t> first(size_t(nEnv_) + 1, 0);
   std::vector<double> cd;
+  cd.reserve(size_t(8) * (size_t(own_first[size_t(nEnv_)]) + shared.size() * size_t(nEnv_)));
+  auto emit = [&](int i) {
+    const HostCollider& c = colliders_[size_t(i)];
+    cd.push_back(double(c.kind));
+    cd.insert(cd.end(), c.p, c.p + 3);
+    cd.insert(cd.end(), c.n, c.n + 3);
+    cd.push_back(c.r);
+  };
   for (int e = 0; e < nEnv_; ++e) {
     first[size_t(e)] = int32_t(cd.size() / 8);
-    for (const auto& c : colliders_) {
-      if (c.actor >= 0 && c.actor != e) continue;
-      cd.push_back(double(c.kind));
-      cd.insert(cd.end(), c.p, c.p + 3);
-      cd.insert(cd.end(), c.n, c.n + 3);
-      cd.push_back(c.r);
+    size_t a = 0, b = size_t(own_first[size_t(e)]);
+    const size_t b1 = size_t(own_first[size_t(e) + 1]);
+    while (a < shared.size() || b < b1) {
+      if (b >= b1 || (a < shared.size() && shared[a] < own[b]))
+        emit(shared[a++]);
+      else
+        emit(own[b++]);
     }
   }
   first[size_t(nEnv_)] = int32_t(cd.size() / 8);

@@ -118,6 +118,8 @@ class ColliderPaths:
         self.ramp = np.array([c.ramp_steps for c in colliders], dtype=np.int64)
 
     def at(self, step: int, dt: float) -> np.ndarray:
+        if not self.ramp.any():
+            return self.P + self.V * (dt * (step + 1))
         s = np.full(len(self.ramp), step + 1)
         s = np.where(self.ramp > 0, np.minimum(s, self.ramp), s)
         return self.P + self.V * (dt * s)[:, None]

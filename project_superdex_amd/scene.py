@@ -156,11 +156,14 @@ class Scene:
         """One step; per-actor statistics as dicts, or with as_array as one
         numpy structured array (fields of sdx_step_stats, no per-actor copies)."""
         params = params or _abi.default_solver_params()
-        stats = (_abi.StepStats * max(1, len(self.actors)))()
-        check(lib().sdx_step(self._w, dt, C.byref(params), stats, len(self.actors)))
-        if as_array:
-            return np.ctypeslib.as_array(stats)[:len(self.actors)]
-        return [s.as_dict() for s in stats[:len(self.actors)]]
+        n = len(self.actors)
+        if as_array:  # written in place: no per-step ctypes conversion
+            out = np.zeros(max(1, n), dtype=_stats_dtype())
+            check(lib().sdx_step(self._w, dt, C.byref(params), out.ctypes.data_as(C.POINTER(_abi.StepStats)), n))
+            return out[:n]
+        stats = (_abi.StepStats * max(1, n))()
+        check(lib().sdx_step(self._w, dt, C.byref(params), stats, n))
+        return [s.as_dict() for s in stats[:n]]
 
     def info(self):
         out = (C.c_int64 * 8)()
@@ -216,6 +219,18 @@ class Scene:
         out = (C.c_double * 2)()
         check(lib().sdx_kernel_timing(self._w, kernel, enable, out))
         return out[0], out[1]
+
+
+_STATS_DTYPE = None
+
+
+def _stats_dtype() -> np.dtype:
+    """numpy layout of sdx_step_stats (built once: deriving it from the ctypes
+    structure costs tens of microseconds)."""
+    global _STATS_DTYPE
+    if _STATS_DTYPE is None:
+        _STATS_DTYPE = np.dtype(_abi.StepStats)
+    return _STATS_DTYPE
 
 
 def pcg_bsr3_solve(indptr, indices, blocks, b, x0=None, params: _abi.KrylovParams | None = None,
