@@ -110,28 +110,26 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, FieldPtrs<T> f, con
       for (int r = 0; r < 9; ++r) d[r] = o[r];
     }
   };
-  T res[12];
-  smith_element<T>(dN, ue, vol, mu, lam, wantH != 0, project != 0, res, emit);
-  // gravity (fem_gravity.h:108-118): res_f -= rho g vol / 4
-  const T gq = T(0.25) * mass;
-  T sw[3] = {w[0] + w[3] + w[6] + w[9], w[1] + w[4] + w[7] + w[10], w[2] + w[5] + w[8] + w[11]};
-  T sd[3] = {wd[0] + wd[3] + wd[6] + wd[9], wd[1] + wd[4] + wd[7] + wd[10], wd[2] + wd[5] + wd[8] + wd[11]};
-  const T gv[3] = {gx, gy, gz};
+  // the residual record of each vertex: stress, inertia of the consistent
+  // mass and gravity (fem_gravity.h:108-118: res_f -= rho g vol / 4),
+  // written before the Hessian is formed
+  auto on_res = [&](const T* res) {
+    const T gq = T(0.25) * mass;
+    const T sw[3] = {w[0] + w[3] + w[6] + w[9], w[1] + w[4] + w[7] + w[10], w[2] + w[5] + w[8] + w[11]};
+    const T sd[3] = {wd[0] + wd[3] + wd[6] + wd[9], wd[1] + wd[4] + wd[7] + wd[10], wd[2] + wd[5] + wd[8] + wd[11]};
+    const T gv[3] = {gx, gy, gz};
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a) {
+      T* re = Rv + (size_t)pv[a] * 3;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      T r = res[3 * a + d] - gv[d] * gq + idt2 * m20 * (w[3 * a + d] + sw[d]);
-      if (mdamp > T(0)) r += mdamp * m20 * (wd[3 * a + d] + sd[d]);
-      res[3 * a + d] = r;
+      for (int d = 0; d < 3; ++d) {
+        T r = res[3 * a + d] - gv[d] * gq + idt2 * m20 * (w[3 * a + d] + sw[d]);
+        if (mdamp > T(0)) r += mdamp * m20 * (wd[3 * a + d] + sd[d]);
+        re[d] = r;
+      }
     }
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    T* re = Rv + (size_t)pv[a] * 3;
-    re[0] = res[3 * a];
-    re[1] = res[3 * a + 1];
-    re[2] = res[3 * a + 2];
-  }
+  };
+  smith_element<T>(dN, ue, vol, mu, lam, wantH != 0, project != 0, on_res, emit);
 }
 
 // ------------------------------------------------------------------ contact

@@ -307,9 +307,9 @@ __device__ __forceinline__ void smith_mode(const T la[3], const V3<T> ae[3], con
 //     with a_f = F dN_f, b_f = cof(F) dN_f, skew(v)_rc = eps_rcm v_m
 //   projected tangent: H_fg = vol sum_k max(l_k, eps) (Q_k dN_f)(Q_k dN_g)^T
 //     over Smith's 9 eigenpairs (3 scaling, 3 twist, 3 flip).
-template <class T, class Emit>
+template <class T, class OnRes, class Emit>
 __device__ __forceinline__ void smith_element(const V3<T> dN[4], const T u[12], T vol, T mu, T lambda, bool wantH,
-                                              bool projectPsd, T res[12], Emit&& emit) {
+                                              bool projectPsd, OnRes&& on_res, Emit&& emit) {
   M33<T> F;
   F.m[0] = T(1);
   F.m[1] = T(0);
@@ -342,14 +342,19 @@ __device__ __forceinline__ void smith_element(const V3<T> dN[4], const T u[12], 
   const M33<T> cof = cofactor(F);
   const T c0 = muHat * (one - IcPlus1Inv);
   const T c1 = lambdaHat * Jma;
-  // res_f = vol * (c0 F + c1 cof) dN_f
+  // res_f = vol * (c0 F + c1 cof) dN_f, handed to on_res before the Hessian
+  // so that nothing of the residual stays live through it
+  {
+    T res[12];
 #pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    const V3<T> a = mul(F, dN[f]);
-    const V3<T> b = mul(cof, dN[f]);
-    res[3 * f + 0] = vol * (c0 * a.x + c1 * b.x);
-    res[3 * f + 1] = vol * (c0 * a.y + c1 * b.y);
-    res[3 * f + 2] = vol * (c0 * a.z + c1 * b.z);
+    for (int f = 0; f < 4; ++f) {
+      const V3<T> a = mul(F, dN[f]);
+      const V3<T> b = mul(cof, dN[f]);
+      res[3 * f + 0] = vol * (c0 * a.x + c1 * b.x);
+      res[3 * f + 1] = vol * (c0 * a.y + c1 * b.y);
+      res[3 * f + 2] = vol * (c0 * a.z + c1 * b.z);
+    }
+    on_res(res);
   }
   if (!wantH) return;
   // Conservative indefiniteness test (Ic bounds max sigma^2); a tangent that
