@@ -365,6 +365,8 @@ void World<T>::finalize() {
       }
   BuildTiles(env_v0_, nEnv, kVecThreads, tiles_v_);
   BuildTiles(env_v0_, nEnv, kSpmvThreads / sched_.spmv_lanes, tiles_a_);
+  max_env_a_ = 0;
+  group_plan_n_ = 0;
 
   // upload
   d_tets_.upload(tets);
@@ -589,10 +591,12 @@ std::vector<double> World<T>::read_env_sq() {
 // every matvec and update launch records its own dispatch events.  With
 // work lists, the caller compacted the tiles of the environments still
 // active into the lists of parity `par`, and the kernels are persistent over
-// them (grids sized for the largest group in the grouped bursts m <
-// group_bursts_).
+// them: grids sized for the largest group in the grouped bursts m <
+// group_bursts_, else for all tiles, divided by 4^cls (grid class cls: the
+// caller's bound on the active tiles; any grid is correct, a persistent
+// kernel strides over its list).
 template <class T>
-void World<T>::issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed) {
+void World<T>::issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed, int cls) {
   const bool dd = precision_ != SDX_PREC_MODE_FP32;
   T* accf = reinterpret_cast<T*>(acc_.p);
   WorkList wa{nullptr, nullptr}, wv{nullptr, nullptr};
@@ -601,8 +605,8 @@ void World<T>::issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool
     wa = WorkList{list_a_.p, list_count_.p + par};
     wv = WorkList{list_v_.p, list_count_.p + 2 + par};
     const bool grouped = m < group_bursts_;
-    ga = std::max(8, std::min(grouped ? group_max_a_ : dt_a_.view.n, kPersistentGridA));
-    gv = std::max(8, std::min(grouped ? group_max_v_ : dt_v_.view.n, kPersistentGridV));
+    ga = std::max(8, std::min(grouped ? group_max_a_ : dt_a_.view.n, kPersistentGridA >> (2 * cls)));
+    gv = std::max(8, std::min(grouped ? group_max_v_ : dt_v_.view.n, kPersistentGridV >> (2 * cls)));
   }
   for (int k = k0; k < k0 + n; ++k) {
     const KernelEvents* e0 = timed ? timer_events(0) : nullptr;
@@ -626,18 +630,18 @@ void World<T>::issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool
 // CG burst m (iterations [m B, m B + B)) over the lists of parity par as a
 // replayable graph, captured on first use for the current configuration.
 template <class T>
-hipGraphExec_t World<T>::cg_graph(const CgConfig& cfg, int m, int par, int B) {
+hipGraphExec_t World<T>::cg_graph(const CgConfig& cfg, int m, int par, int B, int cls) {
   if (std::memcmp(&graph_cfg_, &cfg, sizeof(CgConfig)) != 0 || graph_burst_ != B) {
     drop_graphs();
     graph_cfg_ = cfg;
     graph_burst_ = B;
   }
-  const size_t key = 2 * size_t(m) + size_t(par);
+  const size_t key = (size_t(m) * kGridClasses + size_t(cls)) * 2 + size_t(par);
   if (key >= graphs_.size()) graphs_.resize(key + 1, nullptr);
   if (!graphs_[key]) {
     hipGraph_t g = nullptr;
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    issue_cg(cfg, m, par, m * B, B, false);
+    issue_cg(cfg, m, par, m * B, B, false, cls);
     hip_check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
     hipGraphExec_t ex = nullptr;
     const hipError_t err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -775,6 +779,9 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
   const bool lists = sched_.active_lists != 0;
   const bool timed0 = burst_timed(0);
   int q = 0;  // compactions issued: list parity q & 1
+  if (max_env_a_ == 0)
+    for (int e = 0; e < nEnv_; ++e)
+      max_env_a_ = std::max(max_env_a_, tiles_a_.first[size_t(e) + 1] - tiles_a_.first[size_t(e)]);
   // Grouped bursts: the first group_bursts_ bursts run group after group, so
   // that one group's matrix and vectors stay in the last-level cache across
   // the group's iterations.  They are queued without host checks.
@@ -791,11 +798,15 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
                              group_v_[size_t(g)], group_v_[size_t(g) + 1], env_cg_.p, list_a_.p, list_v_.p,
                              list_count_.p, q & 1, num_active_.p, nullptr);
         if (sched_.use_graphs && !timed)
-          hip_check(hipGraphLaunch(cg_graph(cfg, m, q & 1, B), stream_), "hipGraphLaunch");
+          hip_check(hipGraphLaunch(cg_graph(cfg, m, q & 1, B, 0), stream_), "hipGraphLaunch");
         else
-          issue_cg(cfg, m, q & 1, m * B, B, timed);
+          issue_cg(cfg, m, q & 1, m * B, B, timed, 0);
       }
   }
+  // grid class of the batch-wide bursts from the last published active
+  // count: at most that many environments, each with at most max_env_a_
+  // matvec tiles, are left
+  int known_active = active;
   for (int m = group_bursts_;; ++m) {
     const bool issue = int64_t(m) * B <= int64_t(cfg.max_iter);
     const int slot = m % kBurstSlots;
@@ -806,10 +817,15 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
                              list_v_.p, list_count_.p, q & 1, num_active_.p, pinned_active_dev_ + slot);
         hip_check(hipEventRecord(burst_done_[slot], stream_), "hipEventRecord");
       }
+      int cls = 0;
+      if (lists) {
+        const int64_t bound = int64_t(known_active) * max_env_a_;
+        while (cls + 1 < kGridClasses && bound <= int64_t(kPersistentGridA >> (2 * (cls + 1)))) ++cls;
+      }
       if (sched_.use_graphs && !timed)
-        hip_check(hipGraphLaunch(cg_graph(cfg, m, q & 1, B), stream_), "hipGraphLaunch");
+        hip_check(hipGraphLaunch(cg_graph(cfg, m, q & 1, B, cls), stream_), "hipGraphLaunch");
       else
-        issue_cg(cfg, m, q & 1, m * B, B, timed);
+        issue_cg(cfg, m, q & 1, m * B, B, timed, cls);
       ++q;
       if (!lists) {
         hip_check(hipMemcpyAsync(pinned_active_ + slot, num_active_.p, sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -826,6 +842,7 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
       const int s = (lists ? j + 1 : j) % kBurstSlots;
       hip_check(hipEventSynchronize(burst_done_[s]), "hipEventSynchronize");
       if (hna[s] <= 0) break;
+      known_active = std::min(known_active, int(hna[s]));
     }
   }
   return true;
@@ -1311,6 +1328,7 @@ void World<T>::set_schedule(const sdx_schedule& sc) {
   sched_ = sc;
   drop_graphs();
   group_plan_n_ = 0;  // the groups' tile ranges follow the tiles
+  max_env_a_ = 0;
   if (finalized_ && lanes_changed) {
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     BuildTiles(env_v0_, nEnv_, kSpmvThreads / sched_.spmv_lanes, tiles_a_);

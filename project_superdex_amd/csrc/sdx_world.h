@@ -407,11 +407,11 @@ class World final : public WorldBase {
   // norms2_ (queued); true_rel_from turns them into relative residuals.
   void issue_true_residual(const std::vector<int32_t>& mask);
   std::vector<double> true_rel_from(const std::vector<int32_t>& mask, const std::vector<double>& q) const;
-  void issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed);
+  void issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed, int cls);
   void plan_groups(int n);
   int cg_groups(const sdx_schedule& sc) const;
   static constexpr double kGroupBytes = 200e6;
-  hipGraphExec_t cg_graph(const CgConfig& cfg, int m, int par, int B);
+  hipGraphExec_t cg_graph(const CgConfig& cfg, int m, int par, int B, int cls);
   void drop_graphs();
   template <class A>
   A* xcg() { return reinterpret_cast<A*>(xcg_.p); }
@@ -423,6 +423,10 @@ class World final : public WorldBase {
   // were planned for (0: none) and the grouped bursts of the current solve
   std::vector<int32_t> group_a_, group_v_;
   int group_max_a_ = 0, group_max_v_ = 0, group_plan_n_ = 0, group_bursts_ = 0;
+  // matvec tiles of the largest environment (0: not yet counted), and the
+  // persistent grid classes (grids of 1, 1/4, 1/16, 1/64 of the full size)
+  int max_env_a_ = 0;
+  static constexpr int kGridClasses = 4;
   CgConfig graph_cfg_{};
   int graph_burst_ = 0;
 
