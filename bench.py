@@ -43,11 +43,21 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "tet-steps/sec at rel. residual 1e-6, learned+batched vs plain PCG, 1 GPU"
 
 
-def matvec_bytes(nB: int, nV: int, s: int = 4) -> int:
+def matvec_bytes(nB: int, nV: int, s: int = 4, col_bytes: int = 4) -> int:
     """Algorithmic HBM bytes of one CG matvec launch (k_cg_spmv, w = A z) over
-    every row: block values and column indices once ((9s+4) B), row pointers
+    every row: block values and column indices once ((9s + c) B; c = 2 when
+    the columns are read as 16-bit offsets from their rows), row pointers
     (4(V+1)), z read once (3sV), w written once (3sV)."""
-    return (9 * s + 4) * nB + 4 * (nV + 1) + 6 * s * nV
+    return (9 * s + col_bytes) * nB + 4 * (nV + 1) + 6 * s * nV
+
+
+def column_bytes(spec) -> int:
+    """Bytes per column index the matvec reads: 2 when every block's column
+    lies within 32767 of its row (the library then reads 16-bit offsets).
+    Blocks couple the vertices of one tet of one body, numbered contiguously
+    per body, so the widest tet decides."""
+    widest = max(int((b.tets.max(axis=1) - b.tets.min(axis=1)).max()) for b in spec.bodies)
+    return 2 if widest <= 32767 else 4
 
 
 def step_bytes(nT, nB, nV, cg_iters, ls_trials, newton=1, s=4):
@@ -282,7 +292,7 @@ def main(argv=None) -> int:
     # one launch per group covers 1/groups of the rows (groups of nearly equal
     # tile counts), so a launch's algorithmic bytes are the full pass's / groups
     groups = int(sc.info()["cg_groups"])
-    per_launch = matvec_bytes(nB, nV, s_bytes) / groups
+    per_launch = matvec_bytes(nB, nV, s_bytes, column_bytes(spec)) / groups
     avg_ms = ms / launches if launches else None
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms else None
     # reference-equivalent bytes per step, environment by environment

@@ -168,11 +168,13 @@ void launch_cg_init(hipStream_t s, const Tiles& tv, const CgConfig& cfg, const T
 // CG iteration k: w = A z (+ FP64 true residual on refresh iterations), with
 // `lanes` (4, 8, 16) lanes per block row and kSpmvThreads / lanes rows per
 // tile; over all tiles, or persistent (`grid` >= 8 workgroups) over a list.
+// col16 (may be NULL): the columns as signed 16-bit offsets from their rows,
+// read instead of col.
 template <class T, class A>
 void launch_cg_spmv(hipStream_t s, const Tiles& ta, const WorkList& wl, int grid, const CgConfig& cfg, int k,
                     int lanes, const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const T* z,
                     const A* x, const T* b, T* w, double* rref, EnvCG* env, A* acc, int32_t* num_active,
-                    const KernelEvents* ev = nullptr);
+                    const KernelEvents* ev = nullptr, const int16_t* col16 = nullptr);
 // CG iteration k: p, s = A p, x, r, z updates.  The iterate x is kept in the
 // accumulator type A (FP64 in mixed precision).
 template <class T, class A>

@@ -613,11 +613,12 @@ struct SpmvBufs {
   __amdgpu_buffer_rsrc_t col, z;
 };
 template <class T>
-__device__ __forceinline__ SpmvBufs spmv_bufs(const int32_t* col, const T* vals, int nB, const T* z, int nV) {
+__device__ __forceinline__ SpmvBufs spmv_bufs(const void* col, uint32_t col_bytes, const T* vals, int nB, const T* z,
+                                              int nV) {
   SpmvBufs b;
 #pragma unroll
   for (int c = 0; c < 9; ++c) b.v[c] = make_rsrc(vals + (size_t)c * nB, uint32_t(nB) * uint32_t(sizeof(T)));
-  b.col = make_rsrc(col, uint32_t(nB) * 4u);
+  b.col = make_rsrc(col, col_bytes);
   b.z = make_rsrc(z, uint32_t(nV) * uint32_t(4 * sizeof(T)));
   return b;
 }
@@ -769,7 +770,7 @@ __device__ __forceinline__ double slot_reduce(double v) {
 // r_k from R_k; the first tile of a finished environment records it.  Then
 // w_k = A z_k is stored and z_k.w_k goes into D_k.  On refresh iterations the
 // true residual rref = b - A x_k is also formed in FP64 (residual replacement).
-template <class T, class A, bool kRefresh, int L>
+template <class T, class A, bool kRefresh, int L, bool kC16>
 __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const CgConfig& cfg, int k,
                                              const int32_t* __restrict__ row_ptr, const SpmvBufs& bf,
                                              const T* __restrict__ z, const A* __restrict__ x,
@@ -805,7 +806,10 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
         const int kr = kb + u * L;
         kk[u] = kr < k1 ? kr : k1 - 1;
         wv[u] = kr < k1 ? T(1) : T(0);
-        jc[u] = buf_ld<int>(bf.col, uint32_t(kk[u]) * 4u);
+        if (kC16)  // column as a signed 16-bit offset from the row
+          jc[u] = i + int(short(__builtin_amdgcn_raw_buffer_load_b16(bf.col, uint32_t(kk[u]) * 2u, 0, 0)));
+        else
+          jc[u] = buf_ld<int>(bf.col, uint32_t(kk[u]) * 4u);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -1042,17 +1046,18 @@ __device__ __forceinline__ void for_each_tile(const WorkList& wl, int n_tiles, F
   }
 }
 
-template <class T, class A, bool kRefresh, int L>
+// kC16: col holds each block's column as a signed 16-bit offset from its row.
+template <class T, class A, bool kRefresh, int L, bool kC16>
 __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(Tiles ta, WorkList wl, CgConfig cfg, int k,
                                                           const int32_t* __restrict__ row_ptr,
-                                                          const int32_t* __restrict__ col, const T* __restrict__ vals,
+                                                          const void* __restrict__ col, const T* __restrict__ vals,
                                                           int nB, const T* __restrict__ z, const A* __restrict__ x,
                                                           const T* __restrict__ b, T* __restrict__ w,
                                                           double* __restrict__ rref, EnvCG* env, A* acc,
                                                           int32_t* num_active) {
-  const SpmvBufs bf = spmv_bufs<T>(col, vals, nB, z, cfg.n_rows);
+  const SpmvBufs bf = spmv_bufs<T>(col, uint32_t(nB) * (kC16 ? 2u : 4u), vals, nB, z, cfg.n_rows);
   for_each_tile(wl, wl.list ? 0 : int(gridDim.x), [&](int tile) {
-    cg_spmv_tile<T, A, kRefresh, L>(tile, ta, cfg, k, row_ptr, bf, z, x, b, w, rref, env, acc, num_active);
+    cg_spmv_tile<T, A, kRefresh, L, kC16>(tile, ta, cfg, k, row_ptr, bf, z, x, b, w, rref, env, acc, num_active);
   });
 }
 
@@ -1431,15 +1436,22 @@ template <class T, class A>
 void launch_cg_spmv(hipStream_t s, const Tiles& ta, const WorkList& wl, int grid, const CgConfig& cfg, int k,
                     int lanes, const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const T* z,
                     const A* x, const T* b, T* w, double* rref, EnvCG* env, A* acc, int32_t* num_active,
-                    const KernelEvents* ev) {
+                    const KernelEvents* ev, const int16_t* col16) {
   if (ta.n == 0) return;
   if (uint64_t(nB) * sizeof(T) >= (uint64_t(1) << 32) || uint64_t(cfg.n_rows) * 4 * sizeof(T) >= (uint64_t(1) << 32))
     throw std::invalid_argument("matrix too large for 32-bit buffer offsets of the CG matvec");
   const int g = wl.list ? grid : ta.n;
   const bool refresh = cg_refresh(cfg, k);
-#define SDX_SPMV(R, L)                                                                                           \
-  SDX_LAUNCH(ev, (k_cg_spmv<T, A, R, L>), dim3(g), dim3(kSpmvThreads), s, ta, wl, cfg, k, row_ptr, col, vals, nB, \
-             z, x, b, w, rref, env, acc, num_active)
+  const void* cp = col16 ? static_cast<const void*>(col16) : static_cast<const void*>(col);
+#define SDX_SPMV(R, L)                                                                                                \
+  do {                                                                                                                \
+    if (col16)                                                                                                        \
+      SDX_LAUNCH(ev, (k_cg_spmv<T, A, R, L, true>), dim3(g), dim3(kSpmvThreads), s, ta, wl, cfg, k, row_ptr, cp, vals, \
+                 nB, z, x, b, w, rref, env, acc, num_active);                                                         \
+    else                                                                                                              \
+      SDX_LAUNCH(ev, (k_cg_spmv<T, A, R, L, false>), dim3(g), dim3(kSpmvThreads), s, ta, wl, cfg, k, row_ptr, cp,      \
+                 vals, nB, z, x, b, w, rref, env, acc, num_active);                                                   \
+  } while (0)
   switch (lanes) {
     case 4:
       if (refresh) SDX_SPMV(true, 4); else SDX_SPMV(false, 4);
@@ -1609,7 +1621,8 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
                                         double*);                                                               \
   template void launch_cg_spmv<T, A>(hipStream_t, const Tiles&, const WorkList&, int, const CgConfig&, int, int, \
                                      const int32_t*, const int32_t*, const T*, int32_t, const T*, const A*,      \
-                                     const T*, T*, double*, EnvCG*, A*, int32_t*, const KernelEvents*);          \
+                                     const T*, T*, double*, EnvCG*, A*, int32_t*, const KernelEvents*,           \
+                                     const int16_t*);                                                            \
   template void launch_cg_update<T, A>(hipStream_t, const Tiles&, const WorkList&, int, const CgConfig&, int,    \
                                        const T*, int32_t, const T*, const double*, A*, T*, T*, T*, T*, EnvCG*,   \
                                        A*, int32_t*, const KernelEvents*);

@@ -376,6 +376,23 @@ void World<T>::finalize() {
   d_row_env_.upload(row_env);
   d_row_ptr_.upload(row_ptr);
   d_col_.upload(col);
+  {  // the CG matvec reads 16-bit column offsets when every block's fits
+    std::vector<int16_t> c16(col.size());
+    bool fits = true;
+    for (size_t i = 0; i + 1 < row_ptr.size() && fits; ++i)
+      for (int32_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+        const int64_t d = int64_t(col[size_t(k)]) - int64_t(i);
+        if (d < -32768 || d > 32767) {
+          fits = false;
+          break;
+        }
+        c16[size_t(k)] = int16_t(d);
+      }
+    if (fits && !c16.empty())
+      d_col16_.upload(c16);
+    else
+      d_col16_.free();
+  }
   d_slot_row_.upload(slot_row);
   d_diag_slot_.upload(diag_slot);
   d_eptr_.upload(eptr);
@@ -613,10 +630,11 @@ void World<T>::issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool
     if (dd)
       launch_cg_spmv<T, double>(stream_, dt_a_.view, wa, ga, cfg, k, sched_.spmv_lanes, d_row_ptr_.p, d_col_.p,
                                 vals_.p, nB_, z_.p, xcg<double>(), res_.p, w_.p, rref_.p, env_cg_.p, acc_.p,
-                                num_active_.p, e0);
+                                num_active_.p, e0, d_col16_.p);
     else
       launch_cg_spmv<T, T>(stream_, dt_a_.view, wa, ga, cfg, k, sched_.spmv_lanes, d_row_ptr_.p, d_col_.p, vals_.p,
-                           nB_, z_.p, xcg<T>(), res_.p, w_.p, rref_.p, env_cg_.p, accf, num_active_.p, e0);
+                           nB_, z_.p, xcg<T>(), res_.p, w_.p, rref_.p, env_cg_.p, accf, num_active_.p, e0,
+                           d_col16_.p);
     const KernelEvents* e1 = timed ? timer_events(1) : nullptr;
     if (dd)
       launch_cg_update<T, double>(stream_, dt_v_.view, wv, gv, cfg, k, dinv_.p, nV_, w_.p, rref_.p, xcg<double>(),

@@ -446,6 +446,7 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, d_row_env_)
   SDX_BUF(int32_t, d_row_ptr_)
   SDX_BUF(int32_t, d_col_)
+  SDX_BUF(int16_t, d_col16_)  // columns as 16-bit offsets from their rows (empty if one does not fit)
   SDX_BUF(int32_t, d_slot_row_)
   SDX_BUF(int32_t, d_diag_slot_)
   SDX_BUF(int32_t, d_eptr_)

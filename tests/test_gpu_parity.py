@@ -309,3 +309,32 @@ def test_diverged_step_resets_state():
     st = sc.step(1e-3, p)[0]
     assert st["convergence"] == _abi.CONV_DIVERGED
     assert np.all(a.displacements() == 0) and np.all(a.velocities() == 0)
+
+
+def test_wide_column_offsets_take_the_32bit_path():
+    """The CG matvec reads columns as 16-bit offsets from their rows when every block's fits.
+    The same cube with 40000 pinned, unattached vertices numbered between its two halves has
+    blocks 40000 columns off the diagonal (32-bit columns): it steps to the same state."""
+    X, tets = mesh.hex_grid_tets(4, 0.02)
+    n, gap, half = X.shape[0], 40000, X.shape[0] // 2
+    new_index = np.where(np.arange(n) < half, np.arange(n), np.arange(n) + gap)
+    Xw = np.zeros((n + gap, 3))
+    Xw[new_index] = X
+    Xw[half:half + gap] = np.random.default_rng(3).uniform(1.0, 2.0, (gap, 3))
+    tets_w = new_index[tets].astype(np.int32)
+    assert (tets_w.max(axis=1) - tets_w.min(axis=1)).max() > 32767
+    pins = mesh.nodes_on_plane(X, 1, 0.0)
+    p = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
+    out = []
+    for Xs, ts, pn in ((X, tets, pins), (Xw, tets_w, np.concatenate([new_index[pins], half + np.arange(gap)]))):
+        sc = gs.Scene(precision=_abi.PREC_MODE_MIXED)
+        a = sc.create_soft_actor(Xs, ts, _abi.default_soft_params(E=1e5, nu=0.4))
+        a.add_boundary_condition_nodes(pn.astype(np.int32))
+        sc.create_rigid_sphere((0.04, 0.08 + 0.02 - 0.002, 0.04), 0.02)
+        for _ in range(4):
+            st = sc.step(1e-3, p)[0]
+            assert st["linear_convergence"] == _abi.CONV_CONVERGED and st["linear_true_rel_res"] <= 1e-6
+        out.append(a.displacements())
+    u0, u1 = out[0], out[1][new_index]
+    assert np.abs(out[1][half:half + gap]).max() == 0.0
+    assert np.abs(u0 - u1).max() <= 1e-3 * np.abs(u0).max()
