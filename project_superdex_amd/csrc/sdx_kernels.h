@@ -99,6 +99,8 @@ struct MeshView {
   const uint32_t* pos_e;    // [6 nT] pair-sorted record of pair k of a tet; top bit: transposed
   const int32_t* pos_vc;    // [3 nF] vertex-sorted record of contact contribution (triangle, a)
   const int32_t* pos_ec;    // [3 nF] pair-sorted record of pair k of a boundary triangle
+  uint8_t* coll_flags;      // [nF] bit 0 / bit 1: the triangle's residual / Hessian contact records
+                            // hold nonzero values (or were never written)
   const int32_t* vtx_ptr;   // [nV+1] records (element and contact) of each vertex
   const uint8_t* pinned;    // [3 nV]
   const int32_t* coll_first; // [nEnv+1] into coll

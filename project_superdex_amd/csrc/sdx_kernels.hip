@@ -153,6 +153,7 @@ __global__ void __launch_bounds__(256) k_contact(MeshView m, FieldPtrs<T> f, con
   if (!m.env_mask[env]) return;
   T rc[9];
   T hc[36];
+  bool touched = false;  // some quadrature point is in contact
 #pragma unroll
   for (int k = 0; k < 9; ++k) rc[k] = T(0);
   if (wantH)
@@ -199,6 +200,7 @@ __global__ void __launch_bounds__(256) k_contact(MeshView m, FieldPtrs<T> f, con
         }
         if (!(dist <= thr)) continue;
         if (!(dot3(gr, fnv) <= maxAlign)) continue;
+        touched = true;
         T pen, dpen, ddpen;
         poly_relu<T>(-dist, hp, hp - thr, pen, dpen, ddpen);
         const T fn = kp * pen * dpen;
@@ -223,31 +225,41 @@ __global__ void __launch_bounds__(256) k_contact(MeshView m, FieldPtrs<T> f, con
       }
     }
   }
-  int pv[3];
+  // A triangle out of contact has zero records: they are written only if
+  // it is in contact now or its records still hold values from an earlier
+  // write (flag bits), so most triangles write nothing.
+  const uint8_t fl = m.coll_flags[tr];
+  uint8_t nf = uint8_t((fl & 2) | (touched ? 1 : 0));
+  if (touched || (fl & 1)) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    pv[a] = m.pos_vc[3 * tr + a];
-    T* re = Rv + (size_t)pv[a] * 3;
-    re[0] = rc[3 * a];
-    re[1] = rc[3 * a + 1];
-    re[2] = rc[3 * a + 2];
+    for (int a = 0; a < 3; ++a) {
+      T* re = Rv + (size_t)m.pos_vc[3 * tr + a] * 3;
+      re[0] = rc[3 * a];
+      re[1] = rc[3 * a + 1];
+      re[2] = rc[3 * a + 2];
+    }
   }
-  if (!wantH) return;
+  if (wantH) {
+    nf = uint8_t((nf & 1) | (touched ? 2 : 0));
+    if (touched || (fl & 2)) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    T* d = Dv + (size_t)pv[a] * 6;
+      for (int a = 0; a < 3; ++a) {
+        T* d = Dv + (size_t)m.pos_vc[3 * tr + a] * 6;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) d[k] = hc[6 * hc_block(a, a) + k];
+        for (int k = 0; k < 6; ++k) d[k] = hc[6 * hc_block(a, a) + k];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const T* q = hc + 6 * hc_block(kTriPairA[k], kTriPairB[k]);
+        T* d = Ev + (size_t)m.pos_ec[3 * tr + k] * 9;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) d[3 * r + c] = q[sym_entry(r, c)];
+      }
+    }
   }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const T* q = hc + 6 * hc_block(kTriPairA[k], kTriPairB[k]);
-    T* d = Ev + (size_t)m.pos_ec[3 * tr + k] * 9;
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) d[3 * r + c] = q[sym_entry(r, c)];
-  }
+  if (nf != fl) m.coll_flags[tr] = nf;
 }
 
 // ------------------------------------------------------------------ gathers

@@ -403,6 +403,8 @@ void World<T>::finalize() {
   d_pos_e_.upload(pos_e);
   d_pos_vc_.upload(pos_vc);
   d_pos_ec_.upload(pos_ec);
+  // every triangle's contact records start out unwritten (both flags set)
+  d_coll_flags_.upload(std::vector<uint8_t>(std::max<size_t>(pos_vc.size() / 3, 1), uint8_t(3)));
   d_vtx_ptr_.upload(vtx_ptr);
   d_pinned_.upload(pinned);
   d_pin_value_.upload(pin_value);
@@ -484,6 +486,7 @@ void World<T>::finalize() {
   mesh_.pos_e = d_pos_e_.p;
   mesh_.pos_vc = d_pos_vc_.p;
   mesh_.pos_ec = d_pos_ec_.p;
+  mesh_.coll_flags = d_coll_flags_.p;
   mesh_.vtx_ptr = d_vtx_ptr_.p;
   mesh_.pinned = d_pinned_.p;
   mesh_.env_mask = env_mask_.p;

@@ -458,6 +458,7 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, d_pos_ec_)
   SDX_BUF(int32_t, d_vtx_ptr_)
   SDX_BUF(uint8_t, d_pinned_)
+  SDX_BUF(uint8_t, d_coll_flags_)
   SDX_BUF(T, d_pin_value_)
   SDX_BUF(double, d_contact_)
   SDX_BUF(double, d_coll_)

@@ -338,3 +338,33 @@ def test_wide_column_offsets_take_the_32bit_path():
     u0, u1 = out[0], out[1][new_index]
     assert np.abs(out[1][half:half + gap]).max() == 0.0
     assert np.abs(u0 - u1).max() <= 1e-3 * np.abs(u0).max()
+
+
+@pytest.mark.parametrize("mode,tol", [(_abi.PREC_MODE_FP64, 1e-9), (_abi.PREC_MODE_FP32, 2e-4)])
+def test_contact_records_cleared_when_contact_ends(oracle, mode, tol):
+    """Triangles out of contact skip writing their (zero) contact records unless those still
+    hold values: after the sphere moves away, the assembly equals the oracle's without it."""
+    p = _abi.pinned_solver_params()
+    g, u = small_world(True, contact="sphere", precision=mode, amp=4e-4)
+    o, _ = small_world(False, contact="sphere", precision=mode, amp=4e-4)
+    g.actors[0].set_displacements(u)
+    o.set_displacements(0, u)
+    for step in range(3):  # in contact, moved away (residual-only and full assemblies), back
+        if step == 1:
+            g.colliders[0].set_position((0.5, 0.5, 0.5))
+            o.set_collider_position(0, (0.5, 0.5, 0.5))
+        if step == 2:
+            g.colliders[0].set_position((0.02, 0.05 + 0.02 - 0.002, 0.03))
+            o.set_collider_position(0, (0.02, 0.05 + 0.02 - 0.002, 0.03))
+        rg, pg, cg, vg = g.actors[0].assemble(1e-3, p)
+        ro, po, co, vo = o.assemble(0, 1e-3, p)
+        np.testing.assert_array_equal(cg, co)
+        np.testing.assert_allclose(rg, ro, rtol=tol, atol=tol * np.abs(ro).max())
+        np.testing.assert_allclose(vg, vo, rtol=tol, atol=tol * np.abs(vo).max())
+    # a step after the contact ended: residual-only assemblies (line search) run too
+    g.colliders[0].set_position((0.5, 0.5, 0.5))
+    o.set_collider_position(0, (0.5, 0.5, 0.5))
+    g.step(1e-3, p)
+    o.step(1e-3, p)
+    ug, uo = g.actors[0].displacements(), o.displacements(0)
+    assert np.abs(ug - uo).max() <= 2e-3 * np.abs(uo).max()
