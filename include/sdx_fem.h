@@ -324,8 +324,11 @@ sdx_status sdx_autotune(sdx_world* world, double dt, const sdx_solver_params* pa
  * time steps back, with the least residual |b - A x0| (normal equations over
  * every nonempty subset of the three, so a vanishing or parallel guess drops
  * out), kept only if that residual is below 0.2 |b|; else the environment
- * starts from zero, as the plain method does.  Inference runs in BF16 on the
- * matrix cores, training in FP32 on the device.
+ * starts from zero, as the plain method does.  A Newton iteration (by index
+ * within the step) whose guesses fewer than half of the environments kept in
+ * its last warm-started solve (its third or later) starts from zero for the
+ * next 8 steps, then tries again.  Inference runs in BF16 on the matrix cores, training in FP32
+ * on the device.
  *
  * Feature row (16 floats, the last zero) of vertex i of environment e, with
  * b the residual, D_i the 3x3 diagonal block, d_i = trace(D_i) / 3,

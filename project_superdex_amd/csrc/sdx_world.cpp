@@ -1013,6 +1013,8 @@ void World<T>::set_ws_params(const float* params) {
   ws_w_ = MlpBf16{w, w + kBf16W1, w + kBf16W1 + kBf16W2, w + kBf16W1 + 2 * kBf16W2, b, b + kMlpHid, b + 2 * kMlpHid,
                   b + 3 * kMlpHid};
   ws_on_ = true;
+  for (int& k : ws_skip_) k = 0;
+  for (int& k : ws_seen_) k = 0;
 }
 
 template <class T>
@@ -1137,8 +1139,16 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       if (S[size_t(e)].active) newton_it = S[size_t(e)].iters++;
     hist_k_ = newton_it;  // the warm start draws on the steps of this Newton iteration
     // linear solve, solution in dx_: from zero, or from the learned warm
-    // start where its guess lowers the residual
-    const bool learned = sp.warm_start == SDX_WARM_START_LEARNED && ws_on_;
+    // start where its guess lowers the residual.  A Newton iteration whose
+    // guesses most environments rejected in its last warm-started solve
+    // starts from zero for the next kWarmSkipSteps steps (the network,
+    // features and guess residuals cost more than the few kept guesses save)
+    const int hk = std::min(newton_it, kNewtonHist - 1);
+    bool learned = sp.warm_start == SDX_WARM_START_LEARNED && ws_on_;
+    if (learned && ws_skip_[hk] > 0) {
+      --ws_skip_[hk];
+      learned = false;
+    }
     const bool collecting = coll_per_env_ > 0 && coll_n_ + int64_t(nE) * coll_per_env_ <= coll_cap_;
     if (learned || collecting) ws_features();
     if (learned)
@@ -1153,6 +1163,17 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
     }
     launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, hist1(), hist2());
     launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, dx_.p, hist1());
+    if (learned) {
+      int tried = 0, kept = 0;
+      for (int e = 0; e < nE; ++e)
+        if (S[size_t(e)].active) {
+          ++tried;
+          kept += ws.accepted[size_t(e)] ? 1 : 0;
+        }
+      // the first two warm-started solves of an iteration index run before
+      // its history of previous Newton steps is filled, and never skip
+      if (++ws_seen_[hk] > 2 && double(kept) < kWarmMinKept * double(tried)) ws_skip_[hk] = kWarmSkipSteps;
+    }
     std::vector<int32_t> lin_ok(size_t(nE), 0);
     for (int e = 0; e < nE; ++e) {
       EnvState& s = S[size_t(e)];
@@ -1379,7 +1400,12 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
   launch_copy<T>(stream_, nh, hist_.p, sh.p);
   const int64_t coll_n0 = coll_n_;
   const uint32_t coll_salt0 = coll_salt_;
+  int ws_skip0[kNewtonHist], ws_seen0[kNewtonHist];
+  std::memcpy(ws_skip0, ws_skip_, sizeof(ws_skip_));
+  std::memcpy(ws_seen0, ws_seen_, sizeof(ws_seen_));
   auto restore = [&]() {
+    std::memcpy(ws_skip_, ws_skip0, sizeof(ws_skip_));
+    std::memcpy(ws_seen_, ws_seen0, sizeof(ws_seen_));
     launch_copy<T>(stream_, n3, su.p, u_.p);
     launch_copy<T>(stream_, n3, sv.p, v_.p);
     launch_copy<T>(stream_, nh, sh.p, hist_.p);

@@ -522,6 +522,13 @@ class World final : public WorldBase {
   DBuf<float> ws_bpack_{keep_, &stream_};
   MlpBf16 ws_w_{};
   bool ws_on_ = false;
+  // per Newton iteration index: steps left that start from zero because most
+  // guesses of its last warm-started solve were rejected (fewer than
+  // kWarmMinKept of the environments kept theirs)
+  int ws_skip_[kNewtonHist] = {0, 0, 0, 0};
+  int ws_seen_[kNewtonHist] = {0, 0, 0, 0};  // warm-started solves per index (capped use: > 2)
+  static constexpr double kWarmMinKept = 0.5;
+  static constexpr int kWarmSkipSteps = 8;
   DBuf<float> coll_x_{keep_, &stream_};
   DBuf<float> coll_t_{keep_, &stream_};
   int64_t coll_cap_ = 0, coll_n_ = 0;

@@ -203,8 +203,9 @@ def test_noise_model_costs_no_more_than_plain():
 
 def test_learned_warm_start_two_newton_iterations(trained_model):
     """Two Newton iterations per step: each iteration's warm start draws on the previous time
-    steps' Newton steps of the same iteration. The solves converge, cost no more than the
-    plain method's, and land on the plain solver's result."""
+    steps' Newton steps of the same iteration, and a Newton iteration whose guesses most
+    environments rejected starts from zero for the next 8 steps. The solves converge, cost no
+    more than the plain method's, and land on the plain solver's result."""
     spec = batched(102, 10)
     zero = _abi.pinned_solver_params(2, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
     learned = _abi.pinned_solver_params(2, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
@@ -217,7 +218,7 @@ def test_learned_warm_start_two_newton_iterations(trained_model):
     kept = sum(e["warm_start_accepted"] for st in st1 for e in st)
     print(f"two Newton iterations: CG iterations zero init {it0}, learned {it1} "
           f"(x{it1 / it0:.3f}); guesses kept {kept}/{tried}")
-    assert tried == 16 * 10 * 2
+    assert 16 * 10 <= tried <= 16 * 10 * 2
     assert it1 <= 1.02 * it0, (it0, it1)
     for st in st1:
         for e in st:
