@@ -272,9 +272,11 @@ sdx_status sdx_assemble(sdx_world* world, int32_t actor, double dt, const sdx_so
  * CG matvec (w = A z + z.w), 1 = CG vector update, 2 = element Hessian, 3 =
  * BSR gather.  enable N > 0 clears the accumulators and times every N-th
  * launch of an assembly class, or every launch of the first CG burst (all
- * environments active) of every N-th linear solve (that burst is launched
- * eagerly, the others replay as HIP graphs); 0 stops timing, -1 leaves the
- * setting unchanged.  out (may be NULL): {timed launches, total milliseconds}. */
+ * environments active; with environment groups, the first burst of each
+ * group, whose launches cover that group's rows) of every N-th linear solve
+ * (those bursts are launched eagerly, the others as the schedule says); 0
+ * stops timing, -1 leaves the setting unchanged.  out (may be NULL): {timed
+ * launches, total milliseconds}. */
 sdx_status sdx_kernel_timing(sdx_world* world, int32_t kernel, int32_t enable, double out[2]);
 
 /* CG iterations launched between host convergence checks (default 8). */
