@@ -301,6 +301,11 @@ typedef struct sdx_schedule {
                              matrix values and column indices, sdx_world_info out[7]) */
   int32_t group_iters;    /* CG iterations run group by group, rounded up to whole bursts
                              (0..4096, default 16; ignored when cg_groups is 1) */
+  int32_t fused_tail;     /* with active_lists: 1 runs each CG burst whose active tiles fit the
+                             smallest persistent grid (the tail of a batched solve) as one
+                             cooperative launch with grid-wide barriers between the matvec and
+                             update phases instead of two launches per iteration (default 0:
+                             the barriers' device-scope fences cost more than the launches) */
 } sdx_schedule;
 
 sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* schedule);

@@ -82,6 +82,7 @@ class Schedule(C.Structure):
         ("active_lists", C.c_int32),
         ("cg_groups", C.c_int32),
         ("group_iters", C.c_int32),
+        ("fused_tail", C.c_int32),
     ]
 
     def as_dict(self):

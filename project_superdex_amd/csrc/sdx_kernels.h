@@ -190,7 +190,18 @@ void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int gr
 // *num_active, the active count left by the previous kernels.
 void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, int a0, int a1, int v0, int v1,
                           const EnvCG* env, int32_t* la, int32_t* lv, int32_t* counts, int par,
-                          const int32_t* num_active, int32_t* publish);
+                          const int32_t* num_active, int32_t* publish, uint32_t* bar = nullptr);
+// CG iterations [k0, k0 + n) (matvec and update of each) in one cooperative
+// launch of `grid` workgroups over the work lists wa / wv, with grid-wide
+// barriers between the phases; bar (zeroed by the compaction before the
+// launch) counts barrier arrivals, and *abort is set if a barrier wait ran
+// out (the solve is then invalid).
+template <class T, class A>
+void launch_cg_tail(hipStream_t s, const Tiles& ta, const Tiles& tv, const WorkList& wa, const WorkList& wv, int grid,
+                    const CgConfig& cfg, int k0, int n, int lanes, const int32_t* row_ptr, const int32_t* col,
+                    const int16_t* col16, const T* vals, int32_t nB, const T* dinv, int32_t nV, A* x, const T* b,
+                    T* w, double* rref, T* r, T* z, T* p, T* sv, EnvCG* env, A* acc, int32_t* num_active,
+                    uint32_t* bar, int32_t* abort);
 // FP64 residual b - A x of the environments with mask[e]: into r64 when
 // given; |b - A x|^2, |b|^2, b.Ax, |Ax|^2 into set 0 of acc when given
 // (zeroed by the caller).  Matvec tiles of kSpmvThreads / lanes rows.

@@ -25,6 +25,7 @@
 #include <hip/hip_ext.h>
 
 #include <stdexcept>
+#include <string>
 
 #include "sdx_device.h"
 #include "sdx_kernels.h"
@@ -667,6 +668,16 @@ __device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int nV, i
   }
 }
 
+// Per-environment CG state that other workgroups of the same launch write
+// (k_cg_tail runs both CG phases in one launch): read with vector loads at
+// device scope, never through the scalar cache.
+__device__ __forceinline__ int env_status(const EnvCG* env, int e) {
+  return __hip_atomic_load(&env[e].status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double env_ld(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // StopTest (stopping_criterion.h:53-98): scaling from b, status of `latest`.
 // Returns kCgActive, kCgConverged or kCgDiverged.
 __device__ __forceinline__ int stop_test(const CgConfig& cfg, double start, double latest) {
@@ -789,7 +800,7 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
                                              const T* __restrict__ b, T* __restrict__ w,
                                              double* __restrict__ rref, EnvCG* env, A* acc, int32_t* num_active) {
   const int e = ta.env[tile];
-  const int status = env[e].status;  // branched on after the row loads are issued
+  const int status = env_status(env, e);  // branched on after the row loads are issued
   const int nEnv = cfg.n_env;
   // raw accumulator slots (one per lane of each 16-lane group), issued first
   const int sl = (threadIdx.x & (kSlots - 1)) * kSlotStride;
@@ -907,7 +918,7 @@ __device__ __forceinline__ void cg_update_tile(int tile, const Tiles& tv, const 
                                                T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
                                                T* __restrict__ s, EnvCG* env, A* acc, int32_t* num_active) {
   const int e = tv.env[tile];
-  const int status = env[e].status;  // branched on after the row loads are issued
+  const int status = env_status(env, e);  // branched on after the row loads are issued
   const int nEnv = cfg.n_env;
   const int sl = (threadIdx.x & (kSlots - 1)) * kSlotStride;
   const int nsel = cfg.norm_type == 0 ? 0 : (cfg.norm_type == 1 ? 2 : 1);
@@ -918,7 +929,7 @@ __device__ __forceinline__ void cg_update_tile(int tile, const Tiles& tv, const 
   double w_rho = double(s_r[3]);
   double w_delta = double(acc_ptr(acc, kSetD + (k & 1), nEnv, e)[sl]);
   const int pk = (k + 1) & 1;  // slot of the previous iteration's scalars
-  const double g_prev = env[e].gamma[pk], a_prev = env[e].alpha[pk], m_prev = env[e].mu[pk];
+  const double g_prev = env_ld(&env[e].gamma[pk]), a_prev = env_ld(&env[e].alpha[pk]), m_prev = env_ld(&env[e].mu[pk]);
   const int i = tv.row0[tile] + threadIdx.x;
   const bool valid = i < tv.row1[tile];
   A x0 = A(0), x1 = A(0), x2 = A(0);
@@ -1089,6 +1100,70 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, WorkList wl
   });
 }
 
+// Grid-wide barrier of a cooperative launch (every workgroup resident):
+// arrivals are counted on *bar, zeroed before the launch, and barrier j of
+// the launch waits for j * gridDim.x of them.  The wait is bounded: past
+// kBarrierSpins polls, or once another workgroup gave up, it sets *abort and
+// returns false, and the caller leaves the kernel (the host then reports the
+// solve as failed).  The fences before arriving and after leaving make every
+// workgroup's writes of the phase visible to every other one.
+constexpr uint32_t kBarrierSpins = 4000000u;
+__device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t target, int32_t* abort) {
+  __shared__ int s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(bar, 1u);
+    int ok = 1;
+    for (uint32_t spins = 0; __hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target; ++spins) {
+      if (spins > kBarrierSpins || __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        atomicExch(abort, 1);
+        ok = 0;
+        break;
+      }
+    }
+    __threadfence();
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+// A whole CG burst, iterations [k0, k0 + n), in one cooperative launch of
+// a few workgroups: the matvec and update phases of every iteration stride
+// over the work lists as k_cg_spmv and k_cg_update do, separated by grid
+// barriers instead of kernel boundaries.  For the tail of a batched solve,
+// when a few environments are left and a launch costs more than its work.
+template <class T, class A, int L, bool kC16>
+__global__ void __launch_bounds__(kSpmvThreads) k_cg_tail(Tiles ta, Tiles tv, WorkList wa, WorkList wv, CgConfig cfg,
+                                                          int k0, int n, const int32_t* __restrict__ row_ptr,
+                                                          const void* __restrict__ col, const T* __restrict__ vals,
+                                                          int nB, const T* __restrict__ dinv, int nV, A* x,
+                                                          const T* __restrict__ b, T* w, double* rref, T* r, T* z,
+                                                          T* p, T* sv, EnvCG* env, A* acc, int32_t* num_active,
+                                                          uint32_t* bar, int32_t* abort) {
+  static_assert(kSpmvThreads == kVecThreads, "one workgroup size for both phases");
+  const SpmvBufs bf = spmv_bufs<T>(col, uint32_t(nB) * (kC16 ? 2u : 4u), vals, nB, z, cfg.n_rows);
+  uint32_t j = 0;
+  for (int k = k0; k < k0 + n; ++k) {
+    const bool refresh = cg_refresh(cfg, k);
+    for_each_tile(wa, 0, [&](int tile) {
+      if (refresh)
+        cg_spmv_tile<T, A, true, L, kC16>(tile, ta, cfg, k, row_ptr, bf, z, x, b, w, rref, env, acc, num_active);
+      else
+        cg_spmv_tile<T, A, false, L, kC16>(tile, ta, cfg, k, row_ptr, bf, z, x, b, w, rref, env, acc, num_active);
+    });
+    if (!grid_barrier(bar, ++j * gridDim.x, abort)) return;
+    for_each_tile(wv, 0, [&](int tile) {
+      if (refresh)
+        cg_update_tile<T, A, true>(tile, tv, cfg, k, dinv, nV, w, rref, x, r, z, p, sv, env, acc, num_active);
+      else
+        cg_update_tile<T, A, false>(tile, tv, cfg, k, dinv, nV, w, rref, x, r, z, p, sv, env, acc, num_active);
+    });
+    if (!grid_barrier(bar, ++j * gridDim.x, abort)) return;
+  }
+}
+
 // Work lists of the matvec and vector tiles whose environment is still
 // active, for the bursts' persistent CG kernels.  Blocks [0, nba) scan the
 // matvec tiles, the rest the vector tiles; each reserves a contiguous run of
@@ -1099,7 +1174,8 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, WorkList wl
 __global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, int a0, int na, int v0, int nv,
                                                       const EnvCG* __restrict__ env, int32_t* __restrict__ la,
                                                       int32_t* __restrict__ lv, int32_t* counts, int par,
-                                                      const int32_t* num_active, int32_t* publish) {
+                                                      const int32_t* num_active, int32_t* publish,
+                                                      uint32_t* bar) {
   // tiles [a0, a0 + na) of ta and [v0, v0 + nv) of tv: one environment group
   // (or all of them)
   const int nba = (na + 255) / 256;
@@ -1114,6 +1190,7 @@ __global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, int a
     counts[par ^ 1] = 0;
     counts[2 + (par ^ 1)] = 0;
     if (publish) *publish = *num_active;  // pinned host memory, read after this kernel's event
+    if (bar) *bar = 0u;                     // the barrier counter of a k_cg_tail burst
   }
   const bool keep = i < n && env[t.env[tile]].status == kCgActive;
   const unsigned long long m = __ballot(keep);
@@ -1494,14 +1571,45 @@ void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int gr
 
 void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, int a0, int a1, int v0, int v1,
                           const EnvCG* env, int32_t* la, int32_t* lv, int32_t* counts, int par,
-                          const int32_t* num_active, int32_t* publish) {
+                          const int32_t* num_active, int32_t* publish, uint32_t* bar) {
   if (a0 < 0 || a1 < a0 || a1 > ta.n || v0 < 0 || v1 < v0 || v1 > tv.n)
     throw std::runtime_error("compact_tiles: tile range outside the tiles");
   const int na = a1 - a0, nv = v1 - v0;
   // at least one workgroup: it clears the other parity's counts and publishes
   const int nb = std::max(1, (na + 255) / 256 + (nv + 255) / 256);
   hipLaunchKernelGGL(k_compact_tiles, dim3(nb), dim3(256), 0, s, ta, tv, a0, na, v0, nv, env, la, lv, counts, par,
-                     num_active, publish);
+                     num_active, publish, bar);
+}
+
+template <class T, class A>
+void launch_cg_tail(hipStream_t s, const Tiles& ta, const Tiles& tv, const WorkList& wa, const WorkList& wv, int grid,
+                    const CgConfig& cfg, int k0, int n, int lanes, const int32_t* row_ptr, const int32_t* col,
+                    const int16_t* col16, const T* vals, int32_t nB, const T* dinv, int32_t nV, A* x, const T* b,
+                    T* w, double* rref, T* r, T* z, T* p, T* sv, EnvCG* env, A* acc, int32_t* num_active,
+                    uint32_t* bar, int32_t* abort) {
+  if (ta.n == 0 || tv.n == 0 || n <= 0) return;
+  if (!wa.list || !wv.list || grid < 8) throw std::invalid_argument("cg_tail: work lists and >= 8 workgroups required");
+  if (uint64_t(nB) * sizeof(T) >= (uint64_t(1) << 32) || uint64_t(cfg.n_rows) * 4 * sizeof(T) >= (uint64_t(1) << 32))
+    throw std::invalid_argument("matrix too large for 32-bit buffer offsets of the CG matvec");
+  const void* cp = col16 ? static_cast<const void*>(col16) : static_cast<const void*>(col);
+  const void* kern = nullptr;
+#define SDX_TAIL(L)                                                                                     \
+  kern = col16 ? reinterpret_cast<const void*>(&k_cg_tail<T, A, L, true>)                              \
+               : reinterpret_cast<const void*>(&k_cg_tail<T, A, L, false>)
+  switch (lanes) {
+    case 4: SDX_TAIL(4); break;
+    case 8: SDX_TAIL(8); break;
+    default: SDX_TAIL(16); break;
+  }
+#undef SDX_TAIL
+  Tiles a = ta, v = tv;
+  WorkList la = wa, lv = wv;
+  CgConfig c = cfg;
+  // one pointer per kernel parameter, in the kernel's order
+  void* args[] = {&a, &v,    &la,   &lv, &c, &k0, &n,  &row_ptr, &cp,  &vals,       &nB,  &dinv, &nV,
+                  &x, &b,    &w,    &rref, &r, &z,  &p, &sv,      &env, &acc, &num_active, &bar, &abort};
+  const hipError_t e = hipLaunchCooperativeKernel(kern, dim3(unsigned(grid)), dim3(kSpmvThreads), args, 0, s);
+  if (e != hipSuccess) throw std::runtime_error(std::string("hipLaunchCooperativeKernel: ") + hipGetErrorString(e));
 }
 
 template <class T, class X>
@@ -1637,7 +1745,11 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
                                      const int16_t*);                                                            \
   template void launch_cg_update<T, A>(hipStream_t, const Tiles&, const WorkList&, int, const CgConfig&, int,    \
                                        const T*, int32_t, const T*, const double*, A*, T*, T*, T*, T*, EnvCG*,   \
-                                       A*, int32_t*, const KernelEvents*);
+                                       A*, int32_t*, const KernelEvents*);                                       \
+  template void launch_cg_tail<T, A>(hipStream_t, const Tiles&, const Tiles&, const WorkList&, const WorkList&,  \
+                                     int, const CgConfig&, int, int, int, const int32_t*, const int32_t*,         \
+                                     const int16_t*, const T*, int32_t, const T*, int32_t, A*, const T*, T*,      \
+                                     double*, T*, T*, T*, T*, EnvCG*, A*, int32_t*, uint32_t*, int32_t*);
 
 SDX_INST_T(float)
 SDX_INST_T(double)

@@ -444,6 +444,8 @@ void World<T>::finalize() {
   solve_mask_.alloc(static_cast<size_t>(nEnv));
   warm_mask_.alloc(static_cast<size_t>(nEnv));
   list_count_.alloc(4);
+  tail_bar_.alloc(1);
+  tail_abort_.alloc(1);
   hist_.alloc(size_t(2 * kNewtonHist) * n3);
   hip_check(hipMemsetAsync(hist_.p, 0, hist_.bytes(), stream_), "hipMemsetAsync");
   ws_y_.alloc(n3);
@@ -648,6 +650,23 @@ void World<T>::issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool
   }
 }
 
+// CG iterations [k0, k0 + n) over the lists of parity par as one cooperative
+// launch of kTailGrid workgroups (k_cg_tail).
+template <class T>
+void World<T>::issue_cg_tail(const CgConfig& cfg, int par, int k0, int n) {
+  const WorkList wa{list_a_.p, list_count_.p + par}, wv{list_v_.p, list_count_.p + 2 + par};
+  if (precision_ != SDX_PREC_MODE_FP32)
+    launch_cg_tail<T, double>(stream_, dt_a_.view, dt_v_.view, wa, wv, kTailGrid, cfg, k0, n, sched_.spmv_lanes,
+                              d_row_ptr_.p, d_col_.p, d_col16_.p, vals_.p, nB_, dinv_.p, nV_, xcg<double>(), res_.p,
+                              w_.p, rref_.p, r_.p, z_.p, p_.p, sv_.p, env_cg_.p, acc_.p, num_active_.p, tail_bar_.p,
+                              tail_abort_.p);
+  else
+    launch_cg_tail<T, T>(stream_, dt_a_.view, dt_v_.view, wa, wv, kTailGrid, cfg, k0, n, sched_.spmv_lanes,
+                         d_row_ptr_.p, d_col_.p, d_col16_.p, vals_.p, nB_, dinv_.p, nV_, xcg<T>(), res_.p, w_.p,
+                         rref_.p, r_.p, z_.p, p_.p, sv_.p, env_cg_.p, reinterpret_cast<T*>(acc_.p), num_active_.p,
+                         tail_bar_.p, tail_abort_.p);
+}
+
 // CG burst m (iterations [m B, m B + B)) over the lists of parity par as a
 // replayable graph, captured on first use for the current configuration.
 template <class T>
@@ -800,6 +819,7 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
   const bool lists = sched_.active_lists != 0;
   const bool timed0 = burst_timed(0);
   int q = 0;  // compactions issued: list parity q & 1
+  tail_used_ = false;
   if (max_env_a_ == 0)
     for (int e = 0; e < nEnv_; ++e)
       max_env_a_ = std::max(max_env_a_, tiles_a_.first[size_t(e) + 1] - tiles_a_.first[size_t(e)]);
@@ -833,17 +853,27 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
     const int slot = m % kBurstSlots;
     if (issue) {
       const bool timed = m == 0 && timed0;
-      if (lists) {
-        launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, 0, dt_a_.view.n, 0, dt_v_.view.n, env_cg_.p, list_a_.p,
-                             list_v_.p, list_count_.p, q & 1, num_active_.p, pinned_active_dev_ + slot);
-        hip_check(hipEventRecord(burst_done_[slot], stream_), "hipEventRecord");
-      }
       int cls = 0;
       if (lists) {
         const int64_t bound = int64_t(known_active) * max_env_a_;
         while (cls + 1 < kGridClasses && bound <= int64_t(kPersistentGridA >> (2 * (cls + 1)))) ++cls;
       }
-      if (sched_.use_graphs && !timed)
+      // the tail: few enough tiles for the smallest grid, one cooperative
+      // launch per burst (always eager: not captured into graphs)
+      const bool tail = lists && sched_.fused_tail && cls == kGridClasses - 1 && !timed;
+      if (tail && !tail_used_) {
+        launch_fill<int32_t>(stream_, 1, 0, tail_abort_.p);
+        tail_used_ = true;
+      }
+      if (lists) {
+        launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, 0, dt_a_.view.n, 0, dt_v_.view.n, env_cg_.p, list_a_.p,
+                             list_v_.p, list_count_.p, q & 1, num_active_.p, pinned_active_dev_ + slot,
+                             tail ? tail_bar_.p : nullptr);
+        hip_check(hipEventRecord(burst_done_[slot], stream_), "hipEventRecord");
+      }
+      if (tail)
+        issue_cg_tail(cfg, q & 1, m * B, B);
+      else if (sched_.use_graphs && !timed)
         hip_check(hipGraphLaunch(cg_graph(cfg, m, q & 1, B, cls), stream_), "hipGraphLaunch");
       else
         issue_cg(cfg, m, q & 1, m * B, B, timed, cls);
@@ -939,7 +969,10 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
     fetch(ws_ratio_, learned->ratio);
   }
   fetch(norms2_, q);
+  std::vector<int32_t> tail_abort(1, 0);
+  if (tail_used_) fetch(tail_abort_, tail_abort);
   fetch_all();
+  if (tail_abort[0]) throw std::runtime_error("CG tail burst: a grid barrier wait ran out; the solve is invalid");
   std::vector<double> tr = true_rel_from(solve, q);
   const bool confirm = precision_ == SDX_PREC_MODE_MIXED && kp.norm_type == SDX_NORM_RESIDUAL_L2;
   for (int round = 0; confirm && round < 2; ++round) {
@@ -960,7 +993,10 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
     std::vector<double> q2(size_t(2 * nEnv_));
     if (ran2) fetch(env_cg_, h2);
     fetch(norms2_, q2);
+    std::vector<int32_t> tail_abort2(1, 0);
+    if (tail_used_) fetch(tail_abort_, tail_abort2);
     fetch_all();
+    if (tail_abort2[0]) throw std::runtime_error("CG tail burst: a grid barrier wait ran out; the solve is invalid");
     std::vector<double> tr2 = true_rel_from(todo, q2);
     for (int e = 0; e < nEnv_; ++e) {
       if (!todo[size_t(e)]) continue;
@@ -1467,6 +1503,11 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
         c.cg_groups = ng;
         consider(c);
       }
+    {
+      sdx_schedule c = cur;
+      c.fused_tail = 1 - cur.fused_tail;
+      consider(c);
+    }
     if (cg_groups(cur) > 1) {
       const sdx_schedule chosen = cur;
       for (int gi : {8, 24, 32})

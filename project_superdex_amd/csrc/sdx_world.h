@@ -252,7 +252,7 @@ class WorldBase {
     flush_timers();
     return {double(timers_[size_t(k)].launches), timers_[size_t(k)].ms};
   }
-  sdx_schedule sched_{8, 1, 16, 128, 1, 0, 16};
+  sdx_schedule sched_{8, 1, 16, 128, 1, 0, 16, 0};
   bool tuned_ = false;  // the schedule was autotuned for the current topology
 
  protected:
@@ -408,6 +408,7 @@ class World final : public WorldBase {
   void issue_true_residual(const std::vector<int32_t>& mask);
   std::vector<double> true_rel_from(const std::vector<int32_t>& mask, const std::vector<double>& q) const;
   void issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed, int cls);
+  void issue_cg_tail(const CgConfig& cfg, int par, int k0, int n);
   void plan_groups(int n);
   int cg_groups(const sdx_schedule& sc) const;
   static constexpr double kGroupBytes = 200e6;
@@ -427,6 +428,11 @@ class World final : public WorldBase {
   // persistent grid classes (grids of 1, 1/4, 1/16, 1/64 of the full size)
   int max_env_a_ = 0;
   static constexpr int kGridClasses = 4;
+  // fused tail bursts (k_cg_tail): workgroups per launch, the barrier counter,
+  // the flag a barrier sets when its wait runs out, and whether the current
+  // solve used them
+  static constexpr int kTailGrid = 64;
+  bool tail_used_ = false;
   CgConfig graph_cfg_{};
   int graph_burst_ = 0;
 
@@ -447,6 +453,8 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, d_row_ptr_)
   SDX_BUF(int32_t, d_col_)
   SDX_BUF(int16_t, d_col16_)  // columns as 16-bit offsets from their rows (empty if one does not fit)
+  SDX_BUF(uint32_t, tail_bar_)
+  SDX_BUF(int32_t, tail_abort_)
   SDX_BUF(int32_t, d_slot_row_)
   SDX_BUF(int32_t, d_diag_slot_)
   SDX_BUF(int32_t, d_eptr_)

@@ -6,7 +6,8 @@
   (standard PCG) and positions match the oracle as in the parity tests;
 - every launch schedule (matvec lanes, element workgroup, burst length,
   graph replay, persistent kernels over work lists of active tiles, the
-  first iterations run environment group by group) steps to the same state: displacements within 1e-3 of the
+  first iterations run environment group by group, tail bursts as one
+  cooperative launch) steps to the same state: displacements within 1e-3 of the
   largest displacement (FP32 solves at rel. residual 1e-6 differ at the
   1e-4 level when only the summation order changes; a wrong result is off
   by O(1)), iteration counts within 2;
@@ -64,6 +65,8 @@ def state(sc):
     dict(cg_groups=3, group_iters=8, cg_burst=4),
     dict(cg_groups=6, group_iters=5, cg_burst=2, use_graphs=0),  # 3 bursts per group: list parity flips
     dict(cg_groups=4, group_iters=64, cg_burst=8, spmv_lanes=8),  # every iteration grouped
+    dict(fused_tail=1),  # tail bursts as one cooperative launch each
+    dict(fused_tail=1, cg_burst=16, spmv_lanes=16),
 ])
 def test_schedules_step_to_the_same_state(sched):
     # one Newton iteration per step: a second one starts from a residual at
