@@ -368,3 +368,21 @@ def test_contact_records_cleared_when_contact_ends(oracle, mode, tol):
     o.step(1e-3, p)
     ug, uo = g.actors[0].displacements(), o.displacements(0)
     assert np.abs(ug - uo).max() <= 2e-3 * np.abs(uo).max()
+
+
+def test_line_search_backtracking_matches_oracle(oracle):
+    """Fast indenters (1 m/s) make full Newton steps overshoot in the first steps, so the
+    residual-norm line search (line_search.h:300-335) backtracks; positions still match the
+    oracle's, and both sides backtrack."""
+    spec = mesh.batched_indent_scene(num_envs=3, seed=5, n_min=3, n_max=5, steps=8, speed=1.0)
+    p = _abi.pinned_solver_params(2)
+    sg, stg = run_gpu(spec, p, 8)
+    so, sto = oracle.run_scene(spec, p, 8, fp64=False)
+    compare(spec, sg, so, 5e-3)
+    ls_g = sum(e["line_search_iters"] for st in stg for e in st)
+    ls_o = sum(e["line_search_iters"] for st in sto for e in st)
+    print(f"line-search iterations: GPU {ls_g}, oracle {ls_o}")
+    assert ls_g > 0 and ls_o > 0
+    for st in stg:
+        for e in st:
+            assert e["convergence"] != _abi.CONV_DIVERGED
