@@ -219,10 +219,6 @@ void launch_residual64_multi(hipStream_t s, const Tiles& ta, int lanes, int nEnv
 // dst[i] = D(src[i])
 template <class S, class D>
 void launch_convert(hipStream_t s, int64_t n, const S* src, D* dst);
-// y = A x over all rows (plain matvec), `lanes` lanes per block row.
-template <class T>
-void launch_spmv_plain(hipStream_t s, const Tiles& ta, int lanes, const int32_t* row_ptr, const int32_t* col,
-                       const T* vals, int32_t nB, const T* x, T* y);
 // Diagonal-block inverse of a BSR matrix (L1 entry; assembly fuses it).
 template <class T>
 void launch_block_inverse(hipStream_t s, int32_t nV, const int32_t* diag_slot, const T* vals, int32_t nB,

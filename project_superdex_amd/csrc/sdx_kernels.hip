@@ -1325,38 +1325,6 @@ __global__ void __launch_bounds__(kSpmvThreads) k_residual64_multi(Tiles ta, int
   acc_add<double, 10>(part, acc_ptr(acc, 0, nEnv, e), tile - ta.first[e]);
 }
 
-template <class T, int L>
-__global__ void __launch_bounds__(kSpmvThreads) k_spmv_plain(Tiles ta, const int32_t* __restrict__ row_ptr,
-                                                             const int32_t* __restrict__ col,
-                                                             const T* __restrict__ vals, int nB,
-                                                             const T* __restrict__ x, T* __restrict__ y) {
-  const int tile = blockIdx.x;
-  const int i = ta.row0[tile] + (threadIdx.x / L);
-  const int lane = threadIdx.x % L;
-  T a0 = T(0), a1 = T(0), a2 = T(0);
-  const bool valid = i < ta.row1[tile];
-  if (valid) {
-    for (int k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += L) {
-      const int j = col[k];
-      const T x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
-      a0 += vals[k] * x0 + vals[(size_t)nB + k] * x1 + vals[2 * (size_t)nB + k] * x2;
-      a1 += vals[3 * (size_t)nB + k] * x0 + vals[4 * (size_t)nB + k] * x1 + vals[5 * (size_t)nB + k] * x2;
-      a2 += vals[6 * (size_t)nB + k] * x0 + vals[7 * (size_t)nB + k] * x1 + vals[8 * (size_t)nB + k] * x2;
-    }
-  }
-#pragma unroll
-  for (int off = L / 2; off >= 1; off >>= 1) {
-    a0 += __shfl_xor(a0, off, L);
-    a1 += __shfl_xor(a1, off, L);
-    a2 += __shfl_xor(a2, off, L);
-  }
-  if (valid && lane == 0) {
-    y[3 * i] = a0;
-    y[3 * i + 1] = a1;
-    y[3 * i + 2] = a2;
-  }
-}
-
 // ------------------------------------------------------------------ vectors
 
 template <class T>
@@ -1649,14 +1617,6 @@ void launch_convert(hipStream_t s, int64_t n, const S* src, D* dst) {
 }
 
 template <class T>
-void launch_spmv_plain(hipStream_t s, const Tiles& ta, int lanes, const int32_t* row_ptr, const int32_t* col,
-                       const T* vals, int32_t nB, const T* x, T* y) {
-  if (ta.n == 0) return;
-  auto kern = lanes == 4 ? k_spmv_plain<T, 4> : lanes == 8 ? k_spmv_plain<T, 8> : k_spmv_plain<T, 16>;
-  hipLaunchKernelGGL(kern, dim3(ta.n), dim3(kSpmvThreads), 0, s, ta, row_ptr, col, vals, nB, x, y);
-}
-
-template <class T>
 void launch_block_inverse(hipStream_t s, int32_t nV, const int32_t* diag_slot, const T* vals, int32_t nB, int prec,
                           T* dinv) {
   if (nV == 0) return;
@@ -1711,8 +1671,6 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
   template void launch_contact<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, int, T*, T*, T*); \
   template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const T*, const T*, int, T*, T*,             \
                                      const KernelEvents*);                                                      \
-  template void launch_spmv_plain<T>(hipStream_t, const Tiles&, int, const int32_t*, const int32_t*, const T*,  \
-                                     int32_t, const T*, T*);                                                    \
   template void launch_block_inverse<T>(hipStream_t, int32_t, const int32_t*, const T*, int32_t, int, T*);     \
   template void launch_copy<T>(hipStream_t, int64_t, const T*, T*);                                             \
   template void launch_apply_pins<T>(hipStream_t, int64_t, const uint8_t*, const T*, T*);                       \

@@ -97,23 +97,19 @@ template <class T>
 void launch_ws_collect(hipStream_t s, int nEnv, const int32_t* env_v0, const int32_t* mask, int per_env,
                        uint32_t salt, const float* F, const T* dx, const double* scale, int64_t base, float* XF,
                        float* XT);
-// Per environment with solve[e], from set 0 of acc (|b - A x0|^2, |b|^2,
-// b.Ax0, |Ax0|^2): alpha[e] = b.Ax0 / |Ax0|^2, the multiple of the guess with
-// the least residual, ratio[e] = |b - alpha A x0| / |b| (0 when not tried);
-// warm[e] = 1 when ratio[e] < kWarmKeep, else the environment starts from
-// zero (alpha = 1).  Measured on batched indentation and plate presses, guesses
-// that leave 40% of the residual cost CG as many iterations as they save (the
+// A warm-start guess is kept only if it leaves less than kWarmKeep |b| of the
+// residual.  Measured on batched indentation and plate presses, guesses that
+// leave 40% of the residual cost CG as many iterations as they save (the
 // per-vertex error they carry is rough), so only guesses that remove 80% of it
 // are kept.
 constexpr double kWarmKeep = 0.2;
-void launch_ws_decide(hipStream_t s, int nEnv, const int32_t* solve, const double* acc, int32_t* warm,
-                      double* ratio, double* alpha);
-// The same for three guesses u (the network's), v and w (the two previous
-// Newton steps), from the sums of launch_residual64_multi in set 0 of acc:
-// the combination alpha u + beta v + gamma w with the least residual over
-// every nonempty subset of the three (normal equations; subsets whose
-// equations are nearly singular are skipped), kept when that residual is
-// below kWarmKeep |b|.
+// Per environment with solve[e], for three guesses u (the network's), v and w
+// (the two previous Newton steps), from the sums of launch_residual64_multi in
+// set 0 of acc: the combination alpha u + beta v + gamma w with the least
+// residual over every nonempty subset of the three (normal equations; subsets
+// whose equations are nearly singular are skipped); ratio[e] = that residual /
+// |b| (0 when not tried), warm[e] = 1 when ratio[e] < kWarmKeep, else the
+// environment starts from zero.
 void launch_ws_decide_multi(hipStream_t s, int nEnv, const int32_t* solve, const double* acc, int32_t* warm,
                             double* ratio, double* alpha, double* beta, double* gamma);
 

@@ -595,41 +595,6 @@ void launch_ws_collect(hipStream_t s, int nEnv, const int32_t* env_v0, const int
                      salt, F, dx, scale, base, XF, XT);
 }
 
-__global__ void k_ws_decide(int nEnv, const int32_t* __restrict__ solve, const double* __restrict__ acc,
-                            int32_t* __restrict__ warm, double* __restrict__ ratio, double* __restrict__ alpha) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nEnv) return;
-  alpha[e] = 1.0;
-  if (!solve[e]) {
-    warm[e] = 0;
-    ratio[e] = 0.0;
-    return;
-  }
-  double q[4];  // |b - A x|^2, |b|^2, b.Ax, |Ax|^2
-  acc_read<double, 4>(acc_ptr(acc, 0, nEnv, e), q);
-  // the multiple a x of the guess with the least residual: a = b.Ax / |Ax|^2,
-  // |b - a A x|^2 = |b|^2 - (b.Ax)^2 / |Ax|^2
-  double r2 = q[0], a = 1.0;
-  if (q[3] > 0.0) {
-    a = q[2] / q[3];
-    r2 = q[1] - q[2] * a;
-    if (r2 < 0.0) r2 = 0.0;
-  }
-  const double r = q[1] > 0.0 ? sqrt(r2 / q[1]) : 1.0;
-  // keep the guess only if it removes most of the residual: a guess of
-  // little use still costs CG the rough error it carries
-  const bool keep = q[1] > 0.0 && q[3] > 0.0 && r < kWarmKeep && isfinite(a);
-  warm[e] = keep ? 1 : 0;
-  ratio[e] = r;
-  alpha[e] = keep ? a : 1.0;
-}
-
-void launch_ws_decide(hipStream_t s, int nEnv, const int32_t* solve, const double* acc, int32_t* warm,
-                      double* ratio, double* alpha) {
-  if (nEnv == 0) return;
-  hipLaunchKernelGGL(k_ws_decide, dim3((nEnv + 63) / 64), dim3(64), 0, s, nEnv, solve, acc, warm, ratio, alpha);
-}
-
 // Least |b - sum_i c_i A g_i|^2 = |b|^2 - 2 c.h + c^T G c over the guesses of
 // `mask` (G the Gram matrix of the A g_i, h_i = b.A g_i): solves the normal
 // equations of the subset; false when they are (nearly) singular.
