@@ -34,6 +34,11 @@ struct CgConfig {
   int32_t n_rows;         // block rows of the matrix (size of z)
   int32_t pad1;
   double abs_tol, rel_tol, rel_div_tol;
+  // mixed precision (T = float, accumulator double): the iterate is x + xd,
+  // xd [3 n_rows] T holding the increments since the last refresh iteration,
+  // which folds them into x (x is read and written in FP64 only then); NULL:
+  // every update adds into x
+  void* xd;
 };
 
 // Iteration k of the CG replaces the recursive residual by the FP64 true
@@ -178,7 +183,7 @@ void launch_cg_spmv(hipStream_t s, const Tiles& ta, const WorkList& wl, int grid
                     const A* x, const T* b, T* w, double* rref, EnvCG* env, A* acc, int32_t* num_active,
                     const KernelEvents* ev = nullptr, const int16_t* col16 = nullptr);
 // CG iteration k: p, s = A p, x, r, z updates.  The iterate x is kept in the
-// accumulator type A (FP64 in mixed precision).
+// accumulator type A (FP64 in mixed precision; with cfg.xd, x + xd).
 template <class T, class A>
 void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int grid, const CgConfig& cfg, int k,
                       const T* dinv, int32_t nV, const T* w, const double* rref, A* x, T* r, T* z, T* p, T* sv,
@@ -216,6 +221,10 @@ template <class T, class X>
 void launch_residual64_multi(hipStream_t s, const Tiles& ta, int lanes, int nEnv, const int32_t* mask,
                              const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* u,
                              const T* v, const T* w, const T* b, double* ru, double* rv, double* rw, double* acc);
+// x[i] += xd[i], xd[i] = 0 over n values: the iterate of a mixed-precision CG
+// (CgConfig.xd) made whole in x.
+template <class T, class A>
+void launch_cg_fold(hipStream_t s, int64_t n, A* x, T* xd);
 // dst[i] = D(src[i])
 template <class S, class D>
 void launch_convert(hipStream_t s, int64_t n, const S* src, D* dst);

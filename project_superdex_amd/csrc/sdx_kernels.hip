@@ -762,6 +762,12 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg,
       x[3 * i + 1] = A(0);
       x[3 * i + 2] = A(0);
     }
+    if (cfg.xd) {
+      T* xd = static_cast<T*>(cfg.xd);
+      xd[3 * i] = T(0);
+      xd[3 * i + 1] = T(0);
+      xd[3 * i + 2] = T(0);
+    }
     T zb0, zb1, zb2, z0, z1, z2;
     apply_dinv(dinv, nV, i, cfg.prec, b0, b1, b2, zb0, zb1, zb2);
     apply_dinv(dinv, nV, i, cfg.prec, r0, r1, r2, z0, z1, z2);
@@ -848,7 +854,13 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
         a1 += v3 * zj0 + v4 * zj1 + v5 * zj2;
         a2 += v6 * zj0 + v7 * zj1 + v8 * zj2;
         if (kRefresh) {
-          const double x0 = x[3 * jc[u]], x1 = x[3 * jc[u] + 1], x2 = x[3 * jc[u] + 2];
+          double x0 = x[3 * jc[u]], x1 = x[3 * jc[u] + 1], x2 = x[3 * jc[u] + 2];
+          if (cfg.xd) {  // the increments not yet folded into x
+            const T* xd = static_cast<const T*>(cfg.xd);
+            x0 += double(xd[3 * jc[u]]);
+            x1 += double(xd[3 * jc[u] + 1]);
+            x2 += double(xd[3 * jc[u] + 2]);
+          }
           q0 += double(v0) * x0 + double(v1) * x1 + double(v2) * x2;
           q1 += double(v3) * x0 + double(v4) * x1 + double(v5) * x2;
           q2 += double(v6) * x0 + double(v7) * x1 + double(v8) * x2;
@@ -936,10 +948,19 @@ __device__ __forceinline__ void cg_update_tile(int tile, const Tiles& tv, const 
   T z0 = T(0), z1 = T(0), z2 = T(0), w0 = T(0), w1 = T(0), w2 = T(0);
   T p0 = T(0), p1 = T(0), p2 = T(0), s0 = T(0), s1 = T(0), s2 = T(0);
   double ro0 = 0.0, ro1 = 0.0, ro2 = 0.0;
+  T* xd = static_cast<T*>(cfg.xd);
+  T d0 = T(0), d1 = T(0), d2 = T(0);
   if (valid) {
-    x0 = x[3 * i];
-    x1 = x[3 * i + 1];
-    x2 = x[3 * i + 2];
+    if (xd) {  // increments since the last refresh; x itself only on refresh iterations
+      d0 = xd[3 * i];
+      d1 = xd[3 * i + 1];
+      d2 = xd[3 * i + 2];
+    }
+    if (!xd || kRefresh) {
+      x0 = x[3 * i];
+      x1 = x[3 * i + 1];
+      x2 = x[3 * i + 2];
+    }
     z0 = z[4 * i];
     z1 = z[4 * i + 1];
     z2 = z[4 * i + 2];
@@ -1015,10 +1036,22 @@ __device__ __forceinline__ void cg_update_tile(int tile, const Tiles& tv, const 
     s[3 * i] = s0;
     s[3 * i + 1] = s1;
     s[3 * i + 2] = s2;
-    // the iterate accumulates in the dot-product type (FP64 in mixed precision)
-    x[3 * i] = x0 + A(alpha) * A(p0);
-    x[3 * i + 1] = x1 + A(alpha) * A(p1);
-    x[3 * i + 2] = x2 + A(alpha) * A(p2);
+    // the iterate accumulates in the dot-product type (FP64 in mixed
+    // precision); with xd, in FP32 increments that refresh iterations fold in
+    if (xd && !kRefresh) {
+      xd[3 * i] = d0 + al * p0;
+      xd[3 * i + 1] = d1 + al * p1;
+      xd[3 * i + 2] = d2 + al * p2;
+    } else {
+      x[3 * i] = x0 + A(d0) + A(alpha) * A(p0);
+      x[3 * i + 1] = x1 + A(d1) + A(alpha) * A(p1);
+      x[3 * i + 2] = x2 + A(d2) + A(alpha) * A(p2);
+      if (xd) {
+        xd[3 * i] = T(0);
+        xd[3 * i + 1] = T(0);
+        xd[3 * i + 2] = T(0);
+      }
+    }
     T r0, r1, r2;
     if (kRefresh) {
       r0 = T(ro0 - alpha * double(s0));
@@ -1616,6 +1649,20 @@ void launch_convert(hipStream_t s, int64_t n, const S* src, D* dst) {
   hipLaunchKernelGGL((k_convert<S, D>), dim3(blocks_for(n, 256)), dim3(256), 0, s, n, src, dst);
 }
 
+template <class T, class A>
+__global__ void k_cg_fold(int64_t n, A* __restrict__ x, T* __restrict__ xd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] += A(xd[i]);
+  xd[i] = T(0);
+}
+
+template <class T, class A>
+void launch_cg_fold(hipStream_t s, int64_t n, A* x, T* xd) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_cg_fold<T, A>), dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, xd);
+}
+
 template <class T>
 void launch_block_inverse(hipStream_t s, int32_t nV, const int32_t* diag_slot, const T* vals, int32_t nB, int prec,
                           T* dinv) {
@@ -1719,5 +1766,7 @@ template void launch_fill<int32_t>(hipStream_t, int64_t, int32_t, int32_t*);
 template void launch_convert<float, double>(hipStream_t, int64_t, const float*, double*);
 template void launch_convert<double, float>(hipStream_t, int64_t, const double*, float*);
 template void launch_convert<double, double>(hipStream_t, int64_t, const double*, double*);
+template void launch_cg_fold<float, double>(hipStream_t, int64_t, double*, float*);
+template void launch_cg_fold<double, double>(hipStream_t, int64_t, double*, double*);
 
 }  // namespace sdx

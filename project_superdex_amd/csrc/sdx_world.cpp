@@ -25,6 +25,8 @@
 
 #include "../../include/sdx_fem.h"
 #include "sdx_kernels.h"
+#include <type_traits>
+
 #include "sdx_world.h"
 
 namespace sdx {
@@ -422,6 +424,8 @@ void World<T>::finalize() {
   res_.alloc(n3);
   dx_.alloc(n3);
   xcg_.alloc(n3);
+  xd_.alloc(n3);
+  hip_check(hipMemsetAsync(xd_.p, 0, xd_.bytes(), stream_), "hipMemsetAsync");
   sol0_.alloc(n3);
   r_.alloc(n3);
   z_.alloc(size_t(4) * size_t(nV_));  // padded to 4 components per vertex
@@ -896,6 +900,8 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
       known_active = std::min(known_active, int(hna[s]));
     }
   }
+  if (cfg.xd)  // the increments since the last refresh, into x
+    launch_cg_fold<T, double>(stream_, int64_t(3) * nV_, xcg<double>(), static_cast<T*>(cfg.xd));
   return true;
 }
 
@@ -946,6 +952,8 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
   cfg.abs_tol = kp.abs_tol;
   cfg.rel_tol = kp.rel_tol;
   cfg.rel_div_tol = kp.rel_div_tol;
+  // mixed precision: FP32 increments of the FP64 iterate between refreshes
+  cfg.xd = (precision_ == SDX_PREC_MODE_MIXED && std::is_same<T, float>::value) ? static_cast<void*>(xd_.p) : nullptr;
   std::vector<int32_t> warm = warm_in ? *warm_in : std::vector<int32_t>(size_t(nEnv_), 0);
   std::vector<int32_t> todo = solve;
   const int64_t n3 = int64_t(3) * nV_;

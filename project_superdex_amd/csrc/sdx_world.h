@@ -480,6 +480,7 @@ class World final : public WorldBase {
   SDX_BUF(T, res_)
   SDX_BUF(T, dx_)
   SDX_BUF(double, xcg_)  // CG iterate in the accumulator type (float in FP32 mode)
+  SDX_BUF(T, xd_)        // mixed precision: FP32 increments of the iterate since the last refresh
   SDX_BUF(T, sol0_)
   SDX_BUF(T, r_)
   SDX_BUF(T, z_)
