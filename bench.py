@@ -160,15 +160,16 @@ def position_error_vs_fp64(spec, total, sc32):
 
 def cpu_baseline(spec, seconds_budget: float):
     """The oracle (C++ restatement, OpenMP) on the same workload, bounded to
-    about `seconds_budget` of stepping: the first 16 environments of a batch
-    (the whole mesh for press40), stepped from rest."""
+    about `seconds_budget` of stepping: the first 64 environments of a batch
+    (the whole mesh for press40), stepped from rest through the workload's
+    steps (10-20 s of stepping on 16 cores)."""
     from oracle import oracle_api
     from project_superdex_amd import _abi
 
     threads = min(16, os.cpu_count() or 1)
     oracle_api.set_threads(threads)
     n_env = len(spec.bodies)
-    envs = list(range(min(n_env, 16)))
+    envs = list(range(min(n_env, 64)))
     sub = spec.subset(envs) if n_env > 1 else spec
     params = _abi.pinned_solver_params(spec.newton_iters, lin_rel_tol=1e-6)
     sc, ids = oracle_api.build_scene(sub, fp64=False)
