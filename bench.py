@@ -187,6 +187,32 @@ def cpu_baseline(spec, seconds_budget: float):
                       f"OpenMP threads, {t_used:.1f} s of stepping"}
 
 
+def state_snapshot_costs(sc, reps: int = 5) -> dict:
+    """Capture / restore of the world's state after the timed steps (outside
+    the timed region): device handles and the host byte form, best of `reps`
+    (the byte form: of 2).  The world is left in the state it was captured in."""
+    def best(fn, n):
+        t = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            fn()
+            t.append(time.perf_counter() - t0)
+        return min(t)
+
+    handles = []
+    cap = best(lambda: handles.append(sc.capture_state()), reps)
+    h = handles[-1]
+    res = best(lambda: sc.restore_state(h), reps)
+    blob = []
+    to_b = best(lambda: blob.append(sc.capture_state_to_bytes()), 2)
+    from_b = best(lambda: sc.restore_state_from_bytes(blob[-1]), 2)
+    sc.release_all_states()
+    nbytes = len(blob[-1])
+    return {"bytes": nbytes, "capture_ms": 1e3 * cap, "restore_ms": 1e3 * res,
+            "to_bytes_ms": 1e3 * to_b, "from_bytes_ms": 1e3 * from_b,
+            "capture_GBps": nbytes / cap / 1e9, "restore_GBps": nbytes / res / 1e9}
+
+
 def copy_probe(mib: int) -> int:
     """Three launches of one elementwise kernel that reads `mib` MiB and writes
     `mib` MiB: the known-size reference for FETCH_SIZE / WRITE_SIZE."""
@@ -278,6 +304,7 @@ def main(argv=None) -> int:
     launches, ms = sc.kernel_timing(0) if args.time_every > 0 else (0, 0.0)
     sc.kernel_timing(0, enable=0)
     print(f"[bench] timed {steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    snapshot = state_snapshot_costs(sc)
 
     nT, nV, nB = info["n_tets"], info["n_verts"], info["nnzb"]
     per_env = [s for st in stats for s in st]
@@ -341,6 +368,7 @@ def main(argv=None) -> int:
                  "cg_iters_max": int(it.max()), "ls_iters_mean": float(ls.mean()),
                  "all_converged": ok, "max_true_rel_res": true_rel,
                  "reference_equivalent_GBps": step_b / (elapsed / steps) / 1e9},
+        "state_snapshot": snapshot,
     }
     if learned:  # the same steps from a zero initial guess, same schedule
         z, zr = gs.build_scene(spec, device=0, precision=mode)

@@ -251,6 +251,23 @@ sdx_status sdx_get_velocities(sdx_world* world, int32_t actor, double* out, int3
 sdx_status sdx_set_velocities(sdx_world* world, int32_t actor, const double* in, int32_t n);
 sdx_status sdx_get_positions_world(sdx_world* world, int32_t actor, double* out, int32_t n);
 
+/* State snapshots (Scene::CaptureState / RestoreState / ReleaseState / ReleaseAllStates /
+ * IsEqualState / CaptureStateToBytes / RestoreStateFromBytes,
+ * mochi_physics/include/mochi_physics/cpp_api/mochi_scene.h:79-91).  A state is the
+ * displacements and velocities of every actor, the collider targets and the Newton-step
+ * history of the learned warm start, so steps after a restore repeat the steps after the
+ * capture.  A handle keeps its state in device memory until it is released, or until actors,
+ * pins or materials change (then every handle is dropped).  The byte form (sdx_state_size
+ * bytes) starts with a header naming the world it came from; a world of another precision or
+ * topology rejects it with SDX_ERR_STATE. */
+sdx_status sdx_capture_state(sdx_world* world, int32_t* handle);
+sdx_status sdx_restore_state(sdx_world* world, int32_t handle, int32_t release_immediately);
+sdx_status sdx_release_state(sdx_world* world, int32_t handle);  /* 0: every handle */
+sdx_status sdx_is_equal_state(sdx_world* world, int32_t a, int32_t b, int32_t* equal);
+sdx_status sdx_state_size(sdx_world* world, int64_t* bytes);
+sdx_status sdx_capture_state_to_bytes(sdx_world* world, uint8_t* out, int64_t capacity);
+sdx_status sdx_restore_state_from_bytes(sdx_world* world, const uint8_t* data, int64_t size);
+
 /* World totals: out[0..4] counts over all actors (vertices, tets, BSR blocks, boundary
  * triangles, environments); out[5], out[6] the CG matvec and vector tiles; out[7] the
  * environment groups of the grouped CG iterations under the current schedule (1: none). */

@@ -43,6 +43,13 @@ SIGNATURES = {
     "sdx_get_velocities": (C.c_int32, [_vp, C.c_int32, _dp, C.c_int32]),
     "sdx_set_velocities": (C.c_int32, [_vp, C.c_int32, _dp, C.c_int32]),
     "sdx_get_positions_world": (C.c_int32, [_vp, C.c_int32, _dp, C.c_int32]),
+    "sdx_capture_state": (C.c_int32, [_vp, _ip]),
+    "sdx_restore_state": (C.c_int32, [_vp, C.c_int32, C.c_int32]),
+    "sdx_release_state": (C.c_int32, [_vp, C.c_int32]),
+    "sdx_is_equal_state": (C.c_int32, [_vp, C.c_int32, C.c_int32, _ip]),
+    "sdx_state_size": (C.c_int32, [_vp, _i64p]),
+    "sdx_capture_state_to_bytes": (C.c_int32, [_vp, C.c_void_p, C.c_int64]),
+    "sdx_restore_state_from_bytes": (C.c_int32, [_vp, C.c_void_p, C.c_int64]),
     "sdx_world_info": (C.c_int32, [_vp, _i64p]),
     "sdx_actor_bsr_size": (C.c_int32, [_vp, C.c_int32, _ip, _ip]),
     "sdx_assemble": (C.c_int32, [_vp, C.c_int32, C.c_double, C.POINTER(_abi.SolverParams), _dp, _ip, _ip, _dp]),

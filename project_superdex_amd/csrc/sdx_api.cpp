@@ -318,6 +318,66 @@ sdx_status sdx_get_velocities(sdx_world* w, int32_t a, double* out, int32_t n) {
 sdx_status sdx_set_velocities(sdx_world* w, int32_t a, const double* in, int32_t n) { return set_state(w, a, 1, in, n); }
 sdx_status sdx_get_positions_world(sdx_world* w, int32_t a, double* out, int32_t n) { return get_state(w, a, 2, out, n); }
 
+sdx_status sdx_capture_state(sdx_world* world, int32_t* handle) {
+  SDX_REQUIRE(world && handle, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    *handle = world->impl->capture_state();
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_restore_state(sdx_world* world, int32_t handle, int32_t release_immediately) {
+  SDX_REQUIRE(world, "NULL argument");
+  SDX_REQUIRE(handle > 0, "state handles are positive");
+  return guarded([&]() -> sdx_status {
+    world->impl->restore_state(handle, release_immediately != 0);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_release_state(sdx_world* world, int32_t handle) {
+  SDX_REQUIRE(world, "NULL argument");
+  SDX_REQUIRE(handle >= 0, "state handles are positive (0: every handle)");
+  return guarded([&]() -> sdx_status {
+    world->impl->release_state(handle);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_is_equal_state(sdx_world* world, int32_t a, int32_t b, int32_t* equal) {
+  SDX_REQUIRE(world && equal, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    *equal = world->impl->equal_states(a, b) ? 1 : 0;
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_state_size(sdx_world* world, int64_t* bytes) {
+  SDX_REQUIRE(world && bytes, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    *bytes = int64_t(world->impl->state_size());
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_capture_state_to_bytes(sdx_world* world, uint8_t* out, int64_t capacity) {
+  SDX_REQUIRE(world && out, "NULL argument");
+  SDX_REQUIRE(capacity >= 0, "capacity must not be negative");
+  return guarded([&]() -> sdx_status {
+    world->impl->capture_bytes(out, size_t(capacity));
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_restore_state_from_bytes(sdx_world* world, const uint8_t* data, int64_t size) {
+  SDX_REQUIRE(world && data, "NULL argument");
+  SDX_REQUIRE(size >= 0, "size must not be negative");
+  return guarded([&]() -> sdx_status {
+    world->impl->restore_bytes(data, size_t(size));
+    return SDX_OK;
+  });
+}
+
 sdx_status sdx_world_info(sdx_world* world, int64_t out[8]) {
   SDX_REQUIRE(world && out, "NULL argument");
   return guarded([&]() -> sdx_status {

@@ -158,6 +158,8 @@ World<T>::~World() {
 
 template <class T>
 void World<T>::release_device() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  snaps_.clear();  // snapshots hold the old topology's sizes
   drop_graphs();
   tuned_ = false;
   for (auto* b : bufs_) b->free();
@@ -1550,6 +1552,217 @@ void World<T>::info(int64_t out[8]) {
   } else {
     out[7] = 1;
   }
+}
+
+// ---- state snapshots ----------------------------------------------------
+
+namespace {
+// Header of the byte form of a state: the world it was taken from (real
+// size, precision, vertex, actor and collider counts, a hash of the actors'
+// sizes) and the warm-start counters, followed by the collider targets
+// (double), then u, v and the Newton-step history (the world's real type).
+struct StateHeader {
+  char magic[8];
+  uint32_t version;
+  uint32_t real_bytes;
+  int32_t precision;
+  int32_t n_actors;
+  int64_t n_verts;
+  int32_t n_colliders;
+  int32_t n_hist;
+  uint64_t topology;
+  int32_t ws_skip[4];
+  int32_t ws_seen[4];
+};
+constexpr char kStateMagic[8] = {'S', 'D', 'X', 'S', 'T', 'A', 'T', 0};
+constexpr uint32_t kStateVersion = 1;
+}  // namespace
+
+template <class T>
+typename World<T>::Snapshot& World<T>::snapshot(int32_t handle) {
+  auto it = snaps_.find(handle);
+  if (it == snaps_.end()) throw std::out_of_range("no such state handle");
+  return *it->second;
+}
+
+template <class T>
+std::unique_ptr<typename World<T>::Snapshot> World<T>::take_snapshot() {
+  finalize();
+  const int64_t n3 = int64_t(3) * nV_, nh = int64_t(2 * kNewtonHist) * n3;
+  auto s = std::make_unique<Snapshot>(&stream_);
+  s->u.alloc(size_t(n3));
+  s->v.alloc(size_t(n3));
+  s->hist.alloc(size_t(nh));
+  launch_copy<T>(stream_, n3, u_.p, s->u.p);
+  launch_copy<T>(stream_, n3, v_.p, s->v.p);
+  launch_copy<T>(stream_, nh, hist_.p, s->hist.p);
+  s->coll_p.resize(3 * colliders_.size());
+  for (size_t c = 0; c < colliders_.size(); ++c)
+    for (int d = 0; d < 3; ++d) s->coll_p[3 * c + size_t(d)] = colliders_[c].p[d];
+  std::memcpy(s->ws_skip, ws_skip_, sizeof(ws_skip_));
+  std::memcpy(s->ws_seen, ws_seen_, sizeof(ws_seen_));
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  return s;
+}
+
+template <class T>
+void World<T>::put_snapshot(const Snapshot& s) {
+  finalize();
+  const int64_t n3 = int64_t(3) * nV_, nh = int64_t(2 * kNewtonHist) * n3;
+  if (s.u.n != size_t(n3) || s.hist.n != size_t(nh) || s.coll_p.size() != 3 * colliders_.size())
+    throw std::runtime_error("state was captured from a world of another size");
+  launch_copy<T>(stream_, n3, s.u.p, u_.p);
+  launch_copy<T>(stream_, n3, s.v.p, v_.p);
+  launch_copy<T>(stream_, nh, s.hist.p, hist_.p);
+  for (size_t c = 0; c < colliders_.size(); ++c)
+    for (int d = 0; d < 3; ++d) colliders_[c].p[d] = s.coll_p[3 * c + size_t(d)];
+  colliders_dirty_ = true;
+  std::memcpy(ws_skip_, s.ws_skip, sizeof(ws_skip_));
+  std::memcpy(ws_seen_, s.ws_seen, sizeof(ws_seen_));
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+template <class T>
+int32_t World<T>::capture_state() {
+  auto s = take_snapshot();
+  if (next_snap_ == std::numeric_limits<int32_t>::max()) throw std::runtime_error("state handles exhausted");
+  const int32_t h = next_snap_++;
+  snaps_[h] = std::move(s);
+  return h;
+}
+
+template <class T>
+void World<T>::restore_state(int32_t handle, bool release) {
+  put_snapshot(snapshot(handle));
+  if (release) snaps_.erase(handle);
+}
+
+template <class T>
+void World<T>::release_state(int32_t handle) {
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  if (handle == 0) {
+    snaps_.clear();
+    return;
+  }
+  if (snaps_.erase(handle) == 0) throw std::out_of_range("no such state handle");
+}
+
+// Equal values (NaN equal to NaN) of every captured field.
+template <class T>
+bool World<T>::equal_states(int32_t a, int32_t b) {
+  const Snapshot& A = snapshot(a);
+  const Snapshot& B = snapshot(b);
+  if (A.u.n != B.u.n || A.hist.n != B.hist.n || A.coll_p != B.coll_p) return false;
+  if (std::memcmp(A.ws_skip, B.ws_skip, sizeof(A.ws_skip)) != 0 ||
+      std::memcmp(A.ws_seen, B.ws_seen, sizeof(A.ws_seen)) != 0)
+    return false;
+  auto same = [](const DBuf<T>& x, const DBuf<T>& y) {
+    std::vector<T> hx(x.n), hy(y.n);
+    const_cast<DBuf<T>&>(x).download(hx);
+    const_cast<DBuf<T>&>(y).download(hy);
+    for (size_t i = 0; i < hx.size(); ++i)
+      if (!(hx[i] == hy[i] || (hx[i] != hx[i] && hy[i] != hy[i]))) return false;
+    return true;
+  };
+  return same(A.u, B.u) && same(A.v, B.v) && same(A.hist, B.hist);
+}
+
+template <class T>
+size_t World<T>::state_size() {
+  finalize();
+  const size_t n3 = size_t(3) * size_t(nV_);
+  return sizeof(StateHeader) + 3 * colliders_.size() * sizeof(double) + (2 + 2 * kNewtonHist) * n3 * sizeof(T);
+}
+
+namespace {
+template <class A>
+uint64_t topology_hash(const std::vector<A>& actors) {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the actors' vertex and tet counts
+  auto mix = [&](uint64_t v) {
+    for (int k = 0; k < 8; ++k) {
+      h ^= (v >> (8 * k)) & 0xffu;
+      h *= 1099511628211ull;
+    }
+  };
+  for (const auto& a : actors) {
+    mix(uint64_t(uint32_t(a.nV)));
+    mix(uint64_t(uint32_t(a.nT)));
+  }
+  return h;
+}
+}  // namespace
+
+template <class T>
+void World<T>::capture_bytes(uint8_t* out, size_t cap) {
+  const size_t need = state_size();
+  if (cap < need) throw std::invalid_argument("state buffer too small (sdx_state_size gives the size)");
+  static_assert(kNewtonHist == 4, "the byte form holds four warm-start counters");
+  StateHeader hd{};
+  std::memcpy(hd.magic, kStateMagic, sizeof(hd.magic));
+  hd.version = kStateVersion;
+  hd.real_bytes = uint32_t(sizeof(T));
+  hd.precision = precision_;
+  hd.n_actors = int32_t(actors_.size());
+  hd.n_verts = nV_;
+  hd.n_colliders = int32_t(colliders_.size());
+  hd.n_hist = kNewtonHist;
+  hd.topology = topology_hash(actors_);
+  std::memcpy(hd.ws_skip, ws_skip_, sizeof(hd.ws_skip));
+  std::memcpy(hd.ws_seen, ws_seen_, sizeof(hd.ws_seen));
+  uint8_t* o = out;
+  std::memcpy(o, &hd, sizeof(hd));
+  o += sizeof(hd);
+  for (const HostCollider& c : colliders_) {
+    std::memcpy(o, c.p, 3 * sizeof(double));
+    o += 3 * sizeof(double);
+  }
+  const size_t n3 = size_t(3) * size_t(nV_);
+  std::vector<T> h(n3);
+  u_.download(h);
+  std::memcpy(o, h.data(), n3 * sizeof(T));
+  o += n3 * sizeof(T);
+  v_.download(h);
+  std::memcpy(o, h.data(), n3 * sizeof(T));
+  o += n3 * sizeof(T);
+  h.resize(size_t(2 * kNewtonHist) * n3);
+  hist_.download(h);
+  std::memcpy(o, h.data(), h.size() * sizeof(T));
+}
+
+template <class T>
+void World<T>::restore_bytes(const uint8_t* in, size_t n) {
+  const size_t need = state_size();
+  StateHeader hd{};
+  if (n < sizeof(hd)) throw std::invalid_argument("not a state: too short");
+  std::memcpy(&hd, in, sizeof(hd));
+  if (std::memcmp(hd.magic, kStateMagic, sizeof(hd.magic)) != 0 || hd.version != kStateVersion)
+    throw std::invalid_argument("not a state of this library version");
+  if (hd.real_bytes != sizeof(T) || hd.precision != precision_ || hd.n_actors != int32_t(actors_.size()) ||
+      hd.n_verts != nV_ || hd.n_colliders != int32_t(colliders_.size()) || hd.n_hist != kNewtonHist ||
+      hd.topology != topology_hash(actors_))
+    throw std::runtime_error("state was captured from a world of another precision or topology");
+  if (n != need) throw std::invalid_argument("state size does not match its header");
+  const uint8_t* q = in + sizeof(hd);
+  for (HostCollider& c : colliders_) {
+    std::memcpy(c.p, q, 3 * sizeof(double));
+    q += 3 * sizeof(double);
+  }
+  colliders_dirty_ = true;
+  const size_t n3 = size_t(3) * size_t(nV_);
+  std::vector<T> h(n3);
+  std::memcpy(h.data(), q, n3 * sizeof(T));
+  q += n3 * sizeof(T);
+  put(u_, h);
+  std::vector<T> h2(n3);
+  std::memcpy(h2.data(), q, n3 * sizeof(T));
+  q += n3 * sizeof(T);
+  put(v_, h2);
+  std::vector<T> h3(size_t(2 * kNewtonHist) * n3);
+  std::memcpy(h3.data(), q, h3.size() * sizeof(T));
+  put(hist_, h3);
+  std::memcpy(ws_skip_, hd.ws_skip, sizeof(ws_skip_));
+  std::memcpy(ws_seen_, hd.ws_seen, sizeof(ws_seen_));
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
 template class World<float>;

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -233,6 +235,21 @@ class WorldBase {
   // Device rows of the collected samples (after a stream synchronisation).
   virtual void collected_device(const float** x, const float** t, int64_t* n) = 0;
   virtual double autotune(double dt, const sdx_solver_params& sp, int trials) = 0;
+  // State snapshots (Scene::CaptureState / RestoreState / ReleaseState /
+  // IsEqualState / CaptureStateToBytes / RestoreStateFromBytes,
+  // mochi_physics/include/mochi_physics/cpp_api/mochi_scene.h:79-93): the
+  // displacements and velocities of every actor, the collider targets and
+  // the Newton-step history of the learned warm start.  Handles keep a copy
+  // on the device and are dropped with the device buffers when the topology
+  // changes; the byte form is self-describing and checked against the world
+  // on restore.
+  virtual int32_t capture_state() = 0;
+  virtual void restore_state(int32_t handle, bool release) = 0;
+  virtual void release_state(int32_t handle) = 0;  // 0: every handle
+  virtual bool equal_states(int32_t a, int32_t b) = 0;
+  virtual size_t state_size() = 0;
+  virtual void capture_bytes(uint8_t* out, size_t cap) = 0;
+  virtual void restore_bytes(const uint8_t* in, size_t n) = 0;
   const sdx_schedule& schedule() const { return sched_; }
 
   // kernel timing: class 0 CG matvec, 1 CG update, 2 element, 3 BSR gather.
@@ -365,6 +382,13 @@ class World final : public WorldBase {
   void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) override;
   void set_schedule(const sdx_schedule& s) override;
   double autotune(double dt, const sdx_solver_params& sp, int trials) override;
+  int32_t capture_state() override;
+  void restore_state(int32_t handle, bool release) override;
+  void release_state(int32_t handle) override;
+  bool equal_states(int32_t a, int32_t b) override;
+  size_t state_size() override;
+  void capture_bytes(uint8_t* out, size_t cap) override;
+  void restore_bytes(const uint8_t* in, size_t n) override;
   void set_ws_params(const float* params) override;
   void collect(int64_t capacity, int per_env) override;
   int64_t collected(float* x, float* t, int64_t cap_rows) override;
@@ -538,6 +562,19 @@ class World final : public WorldBase {
   int ws_seen_[kNewtonHist] = {0, 0, 0, 0};  // warm-started solves per index (capped use: > 2)
   static constexpr double kWarmMinKept = 0.5;
   static constexpr int kWarmSkipSteps = 8;
+  // state snapshots under their handles
+  struct Snapshot {
+    std::vector<DBufBase*> reg;
+    DBuf<T> u, v, hist;
+    std::vector<double> coll_p;  // collider targets, 3 per collider
+    int ws_skip[kNewtonHist], ws_seen[kNewtonHist];
+    explicit Snapshot(hipStream_t* s) : u(reg, s), v(reg, s), hist(reg, s) {}
+  };
+  std::map<int32_t, std::unique_ptr<Snapshot>> snaps_;
+  int32_t next_snap_ = 1;
+  Snapshot& snapshot(int32_t handle);
+  std::unique_ptr<Snapshot> take_snapshot();
+  void put_snapshot(const Snapshot& s);
   DBuf<float> coll_x_{keep_, &stream_};
   DBuf<float> coll_t_{keep_, &stream_};
   int64_t coll_cap_ = 0, coll_n_ = 0;

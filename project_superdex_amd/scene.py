@@ -171,6 +171,40 @@ class Scene:
         return {"n_verts": out[0], "n_tets": out[1], "nnzb": out[2], "n_tris": out[3], "n_envs": out[4],
                 "matvec_tiles": out[5], "vector_tiles": out[6], "cg_groups": out[7]}
 
+    # State snapshots (Scene::CaptureState / RestoreState / ReleaseState /
+    # IsEqualState / CaptureStateToBytes / RestoreStateFromBytes,
+    # mochi_physics/include/mochi_physics/cpp_api/mochi_scene.h:79-91)
+    def capture_state(self) -> int:
+        """Snapshot of the dynamic state kept in device memory; returns its handle."""
+        h = C.c_int32()
+        check(lib().sdx_capture_state(self._w, C.byref(h)))
+        return h.value
+
+    def restore_state(self, handle: int, release_immediately: bool = False):
+        check(lib().sdx_restore_state(self._w, int(handle), int(bool(release_immediately))))
+
+    def release_state(self, handle: int):
+        check(lib().sdx_release_state(self._w, int(handle)))
+
+    def release_all_states(self):
+        check(lib().sdx_release_state(self._w, 0))
+
+    def is_equal_state(self, a: int, b: int) -> bool:
+        eq = C.c_int32()
+        check(lib().sdx_is_equal_state(self._w, int(a), int(b), C.byref(eq)))
+        return bool(eq.value)
+
+    def capture_state_to_bytes(self) -> bytes:
+        n = C.c_int64()
+        check(lib().sdx_state_size(self._w, C.byref(n)))
+        buf = (C.c_uint8 * n.value)()
+        check(lib().sdx_capture_state_to_bytes(self._w, buf, n.value))
+        return bytes(buf)
+
+    def restore_state_from_bytes(self, data: bytes):
+        buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+        check(lib().sdx_restore_state_from_bytes(self._w, buf, len(data)))
+
     def set_cg_burst(self, burst: int):
         check(lib().sdx_set_cg_burst(self._w, burst))
 
