@@ -1658,8 +1658,8 @@ bool World<T>::equal_states(int32_t a, int32_t b) {
     return false;
   auto same = [](const DBuf<T>& x, const DBuf<T>& y) {
     std::vector<T> hx(x.n), hy(y.n);
-    const_cast<DBuf<T>&>(x).download(hx);
-    const_cast<DBuf<T>&>(y).download(hy);
+    x.download(hx);
+    y.download(hy);
     for (size_t i = 0; i < hx.size(); ++i)
       if (!(hx[i] == hy[i] || (hx[i] != hx[i] && hy[i] != hy[i]))) return false;
     return true;
@@ -1749,17 +1749,17 @@ void World<T>::restore_bytes(const uint8_t* in, size_t n) {
   }
   colliders_dirty_ = true;
   const size_t n3 = size_t(3) * size_t(nV_);
+  // one-off transfers: plain uploads, so the pinned staging ring keeps its size
   std::vector<T> h(n3);
   std::memcpy(h.data(), q, n3 * sizeof(T));
   q += n3 * sizeof(T);
-  put(u_, h);
-  std::vector<T> h2(n3);
-  std::memcpy(h2.data(), q, n3 * sizeof(T));
+  u_.upload(h);
+  std::memcpy(h.data(), q, n3 * sizeof(T));
   q += n3 * sizeof(T);
-  put(v_, h2);
-  std::vector<T> h3(size_t(2 * kNewtonHist) * n3);
-  std::memcpy(h3.data(), q, h3.size() * sizeof(T));
-  put(hist_, h3);
+  v_.upload(h);
+  h.resize(size_t(2 * kNewtonHist) * n3);
+  std::memcpy(h.data(), q, h.size() * sizeof(T));
+  hist_.upload(h);
   std::memcpy(ws_skip_, hd.ws_skip, sizeof(ws_skip_));
   std::memcpy(ws_seen_, hd.ws_seen, sizeof(ws_seen_));
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");

@@ -66,7 +66,7 @@ struct DBuf : DBufBase {
     if (!h.empty())
       hip_check(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
   }
-  void download(std::vector<T>& h) {
+  void download(std::vector<T>& h) const {
     if (!h.empty())
       hip_check(hipMemcpyAsync(h.data(), p, h.size() * sizeof(T), hipMemcpyDeviceToHost, *stream), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(*stream), "hipStreamSynchronize");
