@@ -261,7 +261,7 @@ def main(argv=None) -> int:
     if args.copy_probe_mib:
         return copy_probe(args.copy_probe_mib)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: every GPU box here has one GPU; run with --gpus 1",
+        print("bench.py measures one GPU only (the path is built and timed on a single GPU); run with --gpus 1",
               file=sys.stderr)
         return 2
 

@@ -1,4 +1,4 @@
-"""GPU: the learned CG warm start (no reference counterpart; the plain method is the judge).
+"""GPU: the learned CG warm start (no reference counterpart; the plain method decides what counts).
 
 - gradients of the device training kernels against the FP64 oracle network
   (loss within 1e-5 relative, gradient within 1e-3 of its norm: FP32 sums);

@@ -1,6 +1,6 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle.
 
-Tolerances (north star: positions within 1e-4 relative in FP32; CG at rel.
+Tolerances (target: positions within 1e-4 relative in FP32; CG at rel.
 residual 1e-6):
 - L1 PCG on the reference's 100x100 golden system: converged, solution
   error < 2e-2 |x| (KrylovSolver.Pcg's bound, pcg_test.cpp:81-82) and,
