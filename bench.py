@@ -2,16 +2,17 @@
 
 Workloads (all synthetic, seeded, resident in HBM before the timed region):
 
-  batch512 (default)  512 independent soft cubes, n ~ U{6..16} hexes per side
-                      (h = 1 cm, 6 tets/hex), E log-uniform in [1e4, 1e6] Pa,
-                      nu ~ U(0.3, 0.45), bottom face pinned, each pressed by a
-                      sphere indenter (r = 2 cm, 5 cm/s down, penalty 1e6),
-                      solved together: one set of launches per CG iteration for
-                      all environments, CG bursts replayed as HIP graphs, launch
+  full4096 (default)  4096 independent soft cubes, n ~ U{4..12} hexes per side
+                      (h = 1 cm, 6 tets/hex, 16.5 M tets), E log-uniform in
+                      [1e4, 1e6] Pa, nu ~ U(0.3, 0.45), bottom face pinned,
+                      1-3 sphere indenters each (r = 2 cm, random velocities
+                      |v| <= 5 cm/s, penalty 1e6), 2 Newton iterations per
+                      step, learned CG warm start, solved together: one set of
+                      launches per CG iteration for all environments, launch
                       schedule chosen by the autotuner on the first step.
+  batch512            512 cubes, n ~ U{6..16}, one indenter pressing down,
+                      1 Newton iteration.
   press40             one 40^3-hex cube (384,000 tets) under a 2 mm plate press.
-  full4096            4096 cubes, n ~ U{4..12}, 1-3 indenters with random
-                      velocities |v| <= 5 cm/s, 2 Newton iterations per step.
 
 Common: backward Euler dt = 1e-3, block-Jacobi PCG to relative residual 1e-6
 (ResidualL2, confirmed by an FP64 recompute of |b - A x| at exit), FP32 matvec
@@ -19,10 +20,11 @@ with FP64 dots, FP64 iterate and an FP64 residual refresh every 10 iterations.
 
 A "step" is one Scene::Step of the workload.  The JSON line reports
 tet-steps/s, the roofline of the dominant kernel (the CG matvec, timed with
-HIP events at its own dispatch during the timed region), the plain GPU PCG
-path on a sample of the same environments (one environment per launch
-sequence, zero initial guess, eager launches), the position error against an
-FP64 run of the same steps, and the CPU oracle on a bounded sample.
+HIP events at its own dispatch during the timed region), the same steps from
+a zero initial guess, the plain GPU PCG path on a sample of the environments
+(each environment stepped on its own, FP32 PCG from a zero initial guess,
+eager launches), the position error against the CPU FP64 oracle on 32
+sampled environments, and the CPU oracle's throughput on a bounded sample.
 """
 from __future__ import annotations
 
@@ -121,11 +123,15 @@ def run_steps(sc, rigid, spec, params, s0, s1, stats=None):
             stats.append(st)
 
 
-def plain_gpu(spec, mode, steps, warmup, sample):
+def plain_gpu(spec, steps, warmup, sample):
     """The plain GPU PCG path: every environment of `sample` stepped on its own
-    (one environment per launch sequence), zero initial guess, eager launches."""
+    (one environment per launch sequence), FP32 block-Jacobi PCG from a zero
+    initial guess, eager launches; no batching, no mixed precision, no warm
+    start."""
+    from project_superdex_amd import _abi
     from project_superdex_amd import scene as gs
 
+    mode = _abi.PREC_MODE_FP32
     tets, secs = 0, 0.0
     for e in sample:
         sub = spec.subset([e])
@@ -141,8 +147,37 @@ def plain_gpu(spec, mode, steps, warmup, sample):
     return tets / secs, secs
 
 
-def position_error_vs_fp64(spec, total, sc32):
-    """Max over vertices of |x - x_fp64| / |x_fp64| after the same steps run in FP64."""
+def sampled_envs(spec, n: int, seed: int):
+    """n environments drawn without replacement (seeded), sorted."""
+    rng = np.random.default_rng(seed)
+    return sorted(int(e) for e in rng.choice(len(spec.bodies), size=min(n, len(spec.bodies)), replace=False))
+
+
+def position_error_vs_oracle(spec, total, sc32, envs, fp64=True):
+    """Max over vertices of the environments `envs` of |x - x_o| / |x_o|, with
+    x_o the CPU oracle's positions after the same `total` steps (the oracle
+    steps the sampled environments on their own, FP64 by default); positions
+    in each environment's own frame (origin at its rest corner), |x_o| floored
+    at 1 mm."""
+    from oracle import oracle_api
+    from project_superdex_amd import _abi
+
+    oracle_api.set_threads(host_threads())
+    sub = spec.subset(envs)
+    params = _abi.pinned_solver_params(spec.newton_iters, lin_rel_tol=1e-6)
+    so, _ = oracle_api.run_scene(sub, params, total, fp64=fp64)
+    worst = 0.0
+    for k, e in enumerate(envs):
+        X = spec.bodies[e].X - spec.bodies[e].X.min(axis=0)  # the environment's own frame
+        xg = X + sc32.actors[e].displacements()
+        xo = X + so.displacements(k)
+        worst = max(worst, float(np.max(np.linalg.norm(xg - xo, axis=1) /
+                                        np.maximum(np.linalg.norm(xo, axis=1), 1e-3))))
+    return worst
+
+
+def position_error_vs_gpu_fp64(spec, total, sc32):
+    """Max over vertices of |x - x_fp64| / |x_fp64| after the same steps run in FP64 on the GPU."""
     from project_superdex_amd import _abi
     from project_superdex_amd import scene as gs
 
@@ -158,16 +193,25 @@ def position_error_vs_fp64(spec, total, sc32):
     return worst
 
 
+def host_threads() -> int:
+    """Host threads this process may use: the CPUs of its affinity mask, capped by
+    OMP_NUM_THREADS when it is set (the GPU box gives a process a share of its host's cores)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
 def cpu_baseline(spec, seconds_budget: float):
     """The oracle (C++ restatement, OpenMP) on the same workload, bounded to
     about `seconds_budget` of stepping: the first 64 environments of a batch
     (the whole mesh for press40), stepped from rest through the workload's
-    steps (10-20 s of stepping on 16 cores)."""
+    steps, on every host thread this process may use."""
     from oracle import oracle_api
     from project_superdex_amd import _abi
 
-    threads = min(16, os.cpu_count() or 1)
-    oracle_api.set_threads(threads)
+    threads = oracle_api.set_threads(host_threads())
     n_env = len(spec.bodies)
     envs = list(range(min(n_env, 64)))
     sub = spec.subset(envs) if n_env > 1 else spec
@@ -182,9 +226,12 @@ def cpu_baseline(spec, seconds_budget: float):
         t_used += time.perf_counter() - t0
         steps += 1
     what = f"{len(envs)} environments" if n_env > 1 else "the whole mesh"
+    nproc = os.cpu_count() or 1
     return {"value": sub.num_tets * steps / t_used, "unit": "tet-steps/s", "cores": threads, "kind": "port",
+            "nproc": nproc,
             "sample": f"first {steps} steps of {what} ({sub.num_tets} tets), FP32 oracle with {threads} "
-                      f"OpenMP threads, {t_used:.1f} s of stepping"}
+                      f"OpenMP threads (host nproc {nproc}; the process may use {threads}), "
+                      f"{t_used:.1f} s of stepping"}
 
 
 def state_snapshot_costs(sc, reps: int = 5) -> dict:
@@ -235,7 +282,7 @@ DEFAULTS = {"press40": (50, 3), "batch512": (20, 3), "full4096": (20, 2)}
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(DEFAULTS), default="batch512")
+    ap.add_argument("--workload", choices=sorted(DEFAULTS), default="full4096")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--seed", type=int, default=0)
@@ -248,7 +295,11 @@ def main(argv=None) -> int:
                          "spmv_lanes=4,cg_burst=4,use_graphs=0,element_block=128,active_lists=1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU PCG comparison")
-    ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP64 position-error run")
+    ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP64 position-error runs")
+    ap.add_argument("--fp64-envs", type=int, default=32,
+                    help="environments the CPU FP64 oracle steps for the position error")
+    ap.add_argument("--gpu-fp64-check", action="store_true",
+                    help="also compare every environment with the same steps run in FP64 on the GPU")
     ap.add_argument("--plain-envs", type=int, default=32, help="environments timed on the plain path")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--time-every", type=int, default=1,
@@ -391,19 +442,24 @@ def main(argv=None) -> int:
         z.close()
         print(f"[bench] zero-init comparison done", file=sys.stderr, flush=True)
     if batched and not args.no_plain:
-        rng = np.random.default_rng(args.seed + 1)
-        sample = sorted(int(e) for e in rng.choice(len(spec.bodies), size=min(args.plain_envs, len(spec.bodies)),
-                                                   replace=False))
-        rate, secs = plain_gpu(spec, mode, steps, warmup, sample)
+        sample = sampled_envs(spec, args.plain_envs, args.seed + 1)
+        rate, secs = plain_gpu(spec, steps, warmup, sample)
         out["plain_gpu"] = {"value": rate, "unit": "tet-steps/s",
                             "sample": f"{len(sample)} of the {len(spec.bodies)} environments, each stepped on its "
-                                      f"own ({steps} timed steps after {warmup}), zero initial guess, eager "
-                                      f"launches; {secs:.1f} s",
+                                      f"own ({steps} timed steps after {warmup}), FP32 block-Jacobi PCG from a "
+                                      f"zero initial guess, eager launches; {secs:.1f} s",
                             "speedup": value / rate}
         print(f"[bench] plain path done", file=sys.stderr, flush=True)
     if not args.no_fp64_check and mode != _abi.PREC_MODE_FP64:
-        out["position_error_vs_fp64"] = position_error_vs_fp64(spec, total, sc)
-        print(f"[bench] fp64 check done", file=sys.stderr, flush=True)
+        envs = sampled_envs(spec, args.fp64_envs, args.seed + 2) if batched else [0]
+        t0 = time.perf_counter()
+        out["position_error_vs_fp64"] = position_error_vs_oracle(spec, total, sc, envs, fp64=True)
+        out["position_error_vs_fp64_sample"] = (
+            f"{len(envs)} sampled environments ({sum(spec.bodies[e].tets.shape[0] for e in envs)} tets) stepped "
+            f"{total} steps by the CPU FP64 oracle; {time.perf_counter() - t0:.1f} s")
+        print(f"[bench] fp64 oracle check done", file=sys.stderr, flush=True)
+        if args.gpu_fp64_check:
+            out["position_error_vs_gpu_fp64"] = position_error_vs_gpu_fp64(spec, total, sc)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(make_spec(args.workload, total, args.seed), args.cpu_seconds)
     else:

@@ -203,6 +203,18 @@ sdx_status sdx_pcg_bsr3_solve(int32_t device, int32_t n_block_rows, int32_t nnzb
                               const double* values, int32_t layout, const double* b, double* x,
                               const sdx_krylov_params* params, sdx_linear_status* out);
 
+/* The same solve on float arrays: the reference's default build (real = float,
+ * mochi_core/include/mochi_core/mochi_config.h:106-116) hands
+ * CudaIterativeSolve<3> a BlockSparseMatrixView<float const, 3> and float b, x
+ * (linear_solver.h:615-662), which bind here without a conversion pass.  The
+ * arithmetic is still params->precision_mode (FP64 mode widens on the device
+ * side of the copy). */
+sdx_status sdx_pcg_bsr3_solve_f32(int32_t device, int32_t n_block_rows, int32_t nnzb,
+                                  const int32_t* block_row_ptr, const int32_t* block_col_idx,
+                                  const float* values, int32_t layout, const float* b, float* x,
+                                  const sdx_krylov_params* params, sdx_linear_status* out);
+
+
 /* ---------------- L2: world / step (Scene::Step seam) ---------------- */
 
 /* Context/Scene creation (cpp_api/mochi_context.h, mochi_scene.h).

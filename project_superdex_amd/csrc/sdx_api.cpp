@@ -27,22 +27,7 @@ struct sdx_mlp {
 
 namespace {
 
-template <class F>
-sdx_status guarded(F&& fn) {
-  try {
-    return fn();
-  } catch (const sdx::DeviceError& e) {
-    return set_error(SDX_ERR_DEVICE, e.what());
-  } catch (const std::bad_alloc&) {
-    return set_error(SDX_ERR_OUT_OF_MEMORY, "out of device or host memory");
-  } catch (const std::out_of_range& e) {
-    return set_error(SDX_ERR_INVALID_ARGUMENT, e.what());
-  } catch (const std::invalid_argument& e) {
-    return set_error(SDX_ERR_INVALID_ARGUMENT, e.what());
-  } catch (const std::exception& e) {
-    return set_error(SDX_ERR_STATE, e.what());
-  }
-}
+using sdx::guarded;
 
 #define SDX_REQUIRE(cond, msg) \
   do {                         \

@@ -954,8 +954,11 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
   cfg.abs_tol = kp.abs_tol;
   cfg.rel_tol = kp.rel_tol;
   cfg.rel_div_tol = kp.rel_div_tol;
-  // mixed precision: FP32 increments of the FP64 iterate between refreshes
-  cfg.xd = (precision_ == SDX_PREC_MODE_MIXED && std::is_same<T, float>::value) ? static_cast<void*>(xd_.p) : nullptr;
+  // mixed precision: FP32 increments of the FP64 iterate between refreshes;
+  // without refreshes every update adds into the FP64 iterate itself
+  cfg.xd = (precision_ == SDX_PREC_MODE_MIXED && std::is_same<T, float>::value && cfg.refresh_every > 0)
+               ? static_cast<void*>(xd_.p)
+               : nullptr;
   std::vector<int32_t> warm = warm_in ? *warm_in : std::vector<int32_t>(size_t(nEnv_), 0);
   std::vector<int32_t> todo = solve;
   const int64_t n3 = int64_t(3) * nV_;
@@ -1558,8 +1561,8 @@ void World<T>::info(int64_t out[8]) {
 
 namespace {
 // Header of the byte form of a state: the world it was taken from (real
-// size, precision, vertex, actor and collider counts, a hash of the actors'
-// sizes) and the warm-start counters, followed by the collider targets
+// size, precision, vertex, actor and collider counts, a hash of the meshes,
+// materials, pins and colliders) and the warm-start counters, followed by the collider targets
 // (double), then u, v and the Newton-step history (the world's real type).
 struct StateHeader {
   char magic[8];
@@ -1575,7 +1578,7 @@ struct StateHeader {
   int32_t ws_seen[4];
 };
 constexpr char kStateMagic[8] = {'S', 'D', 'X', 'S', 'T', 'A', 'T', 0};
-constexpr uint32_t kStateVersion = 1;
+constexpr uint32_t kStateVersion = 2;
 }  // namespace
 
 template <class T>
@@ -1675,20 +1678,55 @@ size_t World<T>::state_size() {
 }
 
 namespace {
-template <class A>
-uint64_t topology_hash(const std::vector<A>& actors) {
-  uint64_t h = 1469598103934665603ull;  // FNV-1a over the actors' vertex and tet counts
-  auto mix = [&](uint64_t v) {
-    for (int k = 0; k < 8; ++k) {
-      h ^= (v >> (8 * k)) & 0xffu;
-      h *= 1099511628211ull;
+// FNV-1a over what decides how a state steps: every actor's counts, rest
+// positions, tets, material (homogeneous parameters and per-element fields),
+// pinned DoFs with their values, and every collider's kind, attachment,
+// normal and radius (collider positions are part of the state itself).
+class TopologyHash {
+ public:
+  void bytes(const void* p, size_t n) {
+    const auto* b = static_cast<const uint8_t*>(p);
+    for (size_t k = 0; k < n; ++k) {
+      h_ ^= b[k];
+      h_ *= 1099511628211ull;
     }
-  };
-  for (const auto& a : actors) {
-    mix(uint64_t(uint32_t(a.nV)));
-    mix(uint64_t(uint32_t(a.nT)));
   }
-  return h;
+  template <class V>
+  void vec(const std::vector<V>& v) {
+    const uint64_t n = v.size();
+    bytes(&n, sizeof(n));
+    if (n) bytes(v.data(), v.size() * sizeof(V));
+  }
+  template <class V>
+  void pod(const V& v) {
+    bytes(&v, sizeof(V));
+  }
+  uint64_t value() const { return h_; }
+
+ private:
+  uint64_t h_ = 1469598103934665603ull;
+};
+
+uint64_t topology_hash(const std::vector<HostActor>& actors, const std::vector<HostCollider>& colliders) {
+  TopologyHash h;
+  for (const HostActor& a : actors) {
+    h.pod(a.nV);
+    h.pod(a.nT);
+    h.vec(a.X);
+    h.vec(a.tets);
+    h.pod(a.params);
+    h.vec(a.E);
+    h.vec(a.nu);
+    h.vec(a.pinned);
+    h.vec(a.pin_value);
+  }
+  for (const HostCollider& c : colliders) {
+    h.pod(c.kind);
+    h.pod(c.actor);
+    h.bytes(c.n, sizeof(c.n));
+    h.pod(c.r);
+  }
+  return h.value();
 }
 }  // namespace
 
@@ -1706,7 +1744,7 @@ void World<T>::capture_bytes(uint8_t* out, size_t cap) {
   hd.n_verts = nV_;
   hd.n_colliders = int32_t(colliders_.size());
   hd.n_hist = kNewtonHist;
-  hd.topology = topology_hash(actors_);
+  hd.topology = topology_hash(actors_, colliders_);
   std::memcpy(hd.ws_skip, ws_skip_, sizeof(hd.ws_skip));
   std::memcpy(hd.ws_seen, ws_seen_, sizeof(hd.ws_seen));
   uint8_t* o = out;
@@ -1739,7 +1777,7 @@ void World<T>::restore_bytes(const uint8_t* in, size_t n) {
     throw std::invalid_argument("not a state of this library version");
   if (hd.real_bytes != sizeof(T) || hd.precision != precision_ || hd.n_actors != int32_t(actors_.size()) ||
       hd.n_verts != nV_ || hd.n_colliders != int32_t(colliders_.size()) || hd.n_hist != kNewtonHist ||
-      hd.topology != topology_hash(actors_))
+      hd.topology != topology_hash(actors_, colliders_))
     throw std::runtime_error("state was captured from a world of another precision or topology");
   if (n != need) throw std::invalid_argument("state size does not match its header");
   const uint8_t* q = in + sizeof(hd);

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -26,6 +27,27 @@ struct DeviceError : std::runtime_error {
 extern thread_local std::string g_last_error;
 sdx_status set_error(sdx_status code, const std::string& msg);
 void hip_check(hipError_t e, const char* what);
+
+// Runs fn() and maps every exception to an sdx_status with its message, so
+// nothing thrown crosses the extern "C" boundary.
+template <class F>
+sdx_status guarded(F&& fn) {
+  try {
+    return fn();
+  } catch (const DeviceError& e) {
+    return set_error(SDX_ERR_DEVICE, e.what());
+  } catch (const std::bad_alloc&) {
+    return set_error(SDX_ERR_OUT_OF_MEMORY, "out of device or host memory");
+  } catch (const std::out_of_range& e) {
+    return set_error(SDX_ERR_INVALID_ARGUMENT, e.what());
+  } catch (const std::invalid_argument& e) {
+    return set_error(SDX_ERR_INVALID_ARGUMENT, e.what());
+  } catch (const std::exception& e) {
+    return set_error(SDX_ERR_STATE, e.what());
+  } catch (...) {
+    return set_error(SDX_ERR_STATE, "unknown exception");
+  }
+}
 
 struct DBufBase {
   virtual ~DBufBase() = default;

@@ -268,18 +268,26 @@ def _stats_dtype() -> np.dtype:
 
 
 def pcg_bsr3_solve(indptr, indices, blocks, b, x0=None, params: _abi.KrylovParams | None = None,
-                   layout: int = _abi.BSR_BLOCK_ROWMAJOR, device: int = 0):
-    """L1: solve A x = b on the GPU; blocks (nnzb, 3, 3) row-major (or strip layout values)."""
+                   layout: int = _abi.BSR_BLOCK_ROWMAJOR, device: int = 0, dtype=np.float64):
+    """L1: solve A x = b on the GPU; blocks (nnzb, 3, 3) row-major (or strip layout values).
+    dtype float64 calls sdx_pcg_bsr3_solve, float32 sdx_pcg_bsr3_solve_f32 (the arrays of the
+    reference's default real = float build); x comes back in that type."""
+    dtype = np.dtype(dtype)
+    if dtype == np.float64:
+        fn, ptr_t = lib().sdx_pcg_bsr3_solve, _dp
+    elif dtype == np.float32:
+        fn, ptr_t = lib().sdx_pcg_bsr3_solve_f32, C.POINTER(C.c_float)
+    else:
+        raise ValueError("dtype must be float64 or float32")
     indptr = np.ascontiguousarray(indptr, dtype=np.int32)
     indices = np.ascontiguousarray(indices, dtype=np.int32)
-    vals = np.ascontiguousarray(blocks, dtype=np.float64).reshape(-1)
-    b = np.ascontiguousarray(b, dtype=np.float64)
-    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
+    vals = np.ascontiguousarray(blocks, dtype=dtype).reshape(-1)
+    b = np.ascontiguousarray(b, dtype=dtype)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=dtype)
     params = params or _abi.default_krylov_params()
     st = _abi.LinearStatus()
-    check(lib().sdx_pcg_bsr3_solve(device, len(indptr) - 1, len(indices), _i(indptr), _i(indices),
-                                   vals.ctypes.data_as(_dp), layout, b.ctypes.data_as(_dp), x.ctypes.data_as(_dp),
-                                   C.byref(params), C.byref(st)))
+    check(fn(device, len(indptr) - 1, len(indices), _i(indptr), _i(indices), vals.ctypes.data_as(ptr_t), layout,
+             b.ctypes.data_as(ptr_t), x.ctypes.data_as(ptr_t), C.byref(params), C.byref(st)))
     return x, {"num_iter_done": st.num_iter_done, "convergence": st.convergence,
                "residual_norm": st.residual_norm, "relative_residual_norm": st.relative_residual_norm}
 

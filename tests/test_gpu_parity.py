@@ -426,3 +426,64 @@ def test_mass_damping_velocity_decay_on_gpu():
     for n in range(1, 6):
         sc.step(dt, p)
         np.testing.assert_allclose(a.velocities(), v0 / (1 + alpha * dt) ** n, rtol=1e-8)
+
+
+@pytest.mark.parametrize("mode", [_abi.PREC_MODE_FP32, _abi.PREC_MODE_MIXED, _abi.PREC_MODE_FP64])
+def test_l1_float_arrays_entry(mode):
+    """sdx_pcg_bsr3_solve_f32 (the reference's default real = float arrays): in FP32 and mixed
+    mode the device sees the same values as through the double entry (both round to float), so
+    the solutions agree within 1e-5 |x| and the iteration counts within 1; in FP64 mode the
+    float-rounded system is solved to its own rel. residual 1e-10."""
+    indptr, indices, blocks, b, xref, A = padded_psd_bsr()
+    kp = kparams(mode, 1e-10 if mode == _abi.PREC_MODE_FP64 else 1e-6)
+    x64, s64 = gs.pcg_bsr3_solve(indptr, indices, blocks, b, params=kp)
+    x32, s32 = gs.pcg_bsr3_solve(indptr, indices, blocks, b, params=kp, dtype=np.float32)
+    assert x32.dtype == np.float32
+    assert s32["convergence"] == s64["convergence"] == _abi.CONV_CONVERGED
+    if mode == _abi.PREC_MODE_FP64:
+        A32 = np.eye(102)
+        A32[:100, :100] = A.astype(np.float32).astype(np.float64)
+        b32 = b.astype(np.float32).astype(np.float64)
+        r = b32 - A32 @ x32.astype(np.float64)
+        # x itself is rounded to float on the way out
+        assert np.linalg.norm(r) <= 1e-5 * np.linalg.norm(b32)
+        assert np.linalg.norm(x32[:100] - xref) < 2e-2 * np.linalg.norm(xref)
+    else:
+        assert abs(s32["num_iter_done"] - s64["num_iter_done"]) <= 1
+        assert np.linalg.norm(x32 - x64) <= 1e-5 * np.linalg.norm(x64)
+
+
+def test_l1_errors_come_back_as_codes():
+    """Every failure of the L1 entry returns an sdx_status (no exception crosses the C ABI)."""
+    from project_superdex_amd._lib import SdxError
+    indptr, indices, blocks, b, xref, A = padded_psd_bsr()
+    kp = kparams(_abi.PREC_MODE_FP32, 1e-6)
+    kp.precision_mode = 9
+    with pytest.raises(SdxError, match="precision mode"):
+        gs.pcg_bsr3_solve(indptr, indices, blocks, b, params=kp)
+    bad = indices.copy()
+    bad[0] = 1000
+    with pytest.raises(SdxError, match="out of range"):
+        gs.pcg_bsr3_solve(indptr, bad, blocks, b, params=kparams(_abi.PREC_MODE_FP32, 1e-6))
+
+
+def test_mixed_without_refresh_keeps_an_fp64_iterate(oracle):
+    """Mixed precision with refresh_every = 0 (the library default) and the default
+    preconditioned-residual norm: the iterate is updated in FP64 every iteration (no FP32
+    increment vector accumulates the whole solve), so the steps match the FP64 oracle within
+    1e-4 relative positions and the FP64 GPU run within 2e-3 of the largest displacement, and
+    with the ResidualL2 norm every solve ends at an FP64 true residual <= 1e-6."""
+    spec = mesh.cube_drop_scene(10, steps=10)
+    p = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=0)
+    p.linear.norm_type = _abi.NORM_PRECONDITIONED_RESIDUAL_L2
+    sg, _ = run_gpu(spec, p, 10, precision=_abi.PREC_MODE_MIXED)
+    p64 = _abi.pinned_solver_params(1, lin_rel_tol=1e-10, precision_mode=_abi.PREC_MODE_FP64)
+    so, _ = oracle.run_scene(spec, p64, 10, fp64=True)
+    compare(spec, sg, so, 2e-3)
+    pr = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=0)
+    sr, st = run_gpu(spec, pr, 10, precision=_abi.PREC_MODE_MIXED)
+    for s in st:
+        assert s[0]["linear_convergence"] == _abi.CONV_CONVERGED
+        assert s[0]["linear_true_rel_res"] <= 1e-6, s[0]
+        assert s[0]["fp64_refreshes"] == 0
+    compare(spec, sr, so, 2e-3)

@@ -166,3 +166,39 @@ def test_byte_form_round_trips_and_is_checked():
     other, _ = gs.build_scene(spec_of(seed=12), precision=MIXED)
     with pytest.raises(SdxError):
         other.restore_state_from_bytes(b1)
+
+
+def test_byte_form_rejects_same_size_worlds_of_another_scene():
+    """The header's hash covers rest positions, tets, materials, pins and collider shapes, not only
+    the counts: worlds with the same vertex, tet, actor and collider counts but another mesh, other
+    pins, another material or another collider radius reject the bytes."""
+    spec = spec_of(steps=4)
+    p = params()
+    sc, rigid = gs.build_scene(spec, precision=MIXED)
+    run(sc, rigid, spec, p, 0, 2)
+    b1 = sc.capture_state_to_bytes()
+
+    def variant(edit):
+        s = spec_of(steps=4)
+        edit(s)
+        return gs.build_scene(s, precision=MIXED)[0]
+
+    def moved_mesh(s):
+        s.bodies[0].X = s.bodies[0].X * 1.01
+
+    def other_pins(s):
+        s.bodies[1].pinned = s.bodies[1].pinned[:-1]
+
+    def other_material(s):
+        s.bodies[2].E = s.bodies[2].E * 2.0
+
+    def other_radius(s):
+        s.colliders[0].radius = s.colliders[0].radius * 1.5
+
+    for edit in (moved_mesh, other_pins, other_material, other_radius):
+        w = variant(edit)
+        with pytest.raises(SdxError, match="another precision or topology"):
+            w.restore_state_from_bytes(b1)
+    same = variant(lambda s: None)
+    same.restore_state_from_bytes(b1)
+    assert same.capture_state_to_bytes() == b1
