@@ -214,6 +214,24 @@ sdx_status sdx_pcg_bsr3_solve_f32(int32_t device, int32_t n_block_rows, int32_t 
                                   const float* values, int32_t layout, const float* b, float* x,
                                   const sdx_krylov_params* params, sdx_linear_status* out);
 
+/* Allocation-free variant of sdx_pcg_bsr3_solve.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, checks, results and status as sdx_pcg_bsr3_solve, but it
+ * calls no hipMalloc / hipFree / stream creation: every device buffer lives in
+ * a fixed 64 MiB per-device array that the library holds from its first call
+ * on a device until the process ends (8 MiB of it stage host arrays; the rest
+ * holds the system: about 9 s + 4 bytes per block plus 160 bytes per block row
+ * at s = 4 (FP32 and MIXED), so a 40^3-hex cube's system of 993,961 blocks
+ * fits; FP64 takes about twice as much).  A larger system is rejected with
+ * SDX_ERR_INVALID_ARGUMENT.  It runs on the null stream and returns when x is
+ * written.  Calls on one device share the array: calls from threads of one
+ * process are serialised by the library, and each call writes every value it
+ * reads before reading it. */
+sdx_status sdx_pcg_bsr3_solve_noalloc(int32_t device, int32_t n_block_rows, int32_t nnzb,
+                                      const int32_t* block_row_ptr, const int32_t* block_col_idx,
+                                      const double* values, int32_t layout, const double* b, double* x,
+                                      const sdx_krylov_params* params, sdx_linear_status* out);
 
 /* ---------------- L2: world / step (Scene::Step seam) ---------------- */
 

@@ -30,6 +30,8 @@ SIGNATURES = {
                                        C.POINTER(_abi.KrylovParams), C.POINTER(_abi.LinearStatus)]),
     "sdx_pcg_bsr3_solve_f32": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _fp, C.c_int32, _fp, _fp,
                                            C.POINTER(_abi.KrylovParams), C.POINTER(_abi.LinearStatus)]),
+    "sdx_pcg_bsr3_solve_noalloc": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _dp, C.c_int32, _dp,
+                                               _dp, C.POINTER(_abi.KrylovParams), C.POINTER(_abi.LinearStatus)]),
     "sdx_world_create": (C.c_int32, [C.c_int32, C.c_int32, C.POINTER(_vp)]),
     "sdx_world_destroy": (C.c_int32, [_vp]),
     "sdx_add_soft_tet_mesh": (C.c_int32, [_vp, _dp, C.c_int32, _ip, C.c_int32, C.POINTER(_abi.SoftParams), _ip]),
