@@ -295,6 +295,8 @@ def main(argv=None) -> int:
                          "spmv_lanes=4,cg_burst=4,use_graphs=0,element_block=128,active_lists=1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU PCG comparison")
+    ap.add_argument("--no-zero-compare", action="store_true",
+                    help="skip the same steps from a zero initial guess (learned warm start only)")
     ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP64 position-error runs")
     ap.add_argument("--fp64-envs", type=int, default=32,
                     help="environments the CPU FP64 oracle steps for the position error")
@@ -421,7 +423,7 @@ def main(argv=None) -> int:
                  "reference_equivalent_GBps": step_b / (elapsed / steps) / 1e9},
         "state_snapshot": snapshot,
     }
-    if learned:  # the same steps from a zero initial guess, same schedule
+    if learned and not args.no_zero_compare:  # the same steps from a zero initial guess, same schedule
         z, zr = gs.build_scene(spec, device=0, precision=mode)
         z.set_schedule(**sc.schedule())
         zp = solver_params(spec, mode)

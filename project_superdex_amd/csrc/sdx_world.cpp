@@ -748,7 +748,7 @@ void World<T>::drop_graphs() {
 // dx_ as the initial guess.  Returns the per-environment CG state.
 template <class T>
 bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, const std::vector<int32_t>& warm,
-                      bool decide, std::vector<EnvCG>& h) {
+                      bool decide, std::vector<EnvCG>& h, bool timing_ok) {
   h.assign(static_cast<size_t>(nEnv_), EnvCG{});
   int active = 0, n_warm = 0;
   for (int e = 0; e < nEnv_; ++e) {
@@ -823,7 +823,9 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
   const int B = std::max(1, sched_.cg_burst);
   volatile int32_t* hna = pinned_active_;
   const bool lists = sched_.active_lists != 0;
-  const bool timed0 = burst_timed(0);
+  // kernel timing samples whole solves only: a continuation of a few
+  // environments would not cover the rows its launches are counted for
+  const bool timed0 = timing_ok && burst_timed(0);
   int q = 0;  // compactions issued: list parity q & 1
   tail_used_ = false;
   if (max_env_a_ == 0)
@@ -971,7 +973,7 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
   }
   // the solve, the true residuals and every download, then one wait
   std::vector<EnvCG> h;
-  const bool ran = cg_run(cfg, todo, warm, learned != nullptr, h);
+  const bool ran = cg_run(cfg, todo, warm, learned != nullptr, h, true);
   issue_true_residual(solve);
   std::vector<double> q(size_t(2 * nEnv_));
   if (ran) fetch(env_cg_, h);
@@ -1001,7 +1003,7 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
     if (n == 0) break;
     CgConfig c2 = cfg;
     std::vector<EnvCG> h2;
-    const bool ran2 = cg_run(c2, todo, warm, false, h2);
+    const bool ran2 = cg_run(c2, todo, warm, false, h2, false);
     issue_true_residual(todo);
     std::vector<double> q2(size_t(2 * nEnv_));
     if (ran2) fetch(env_cg_, h2);

@@ -446,8 +446,10 @@ class World final : public WorldBase {
                               WarmStats* learned = nullptr);
   // Runs CG on the device; false if no environment had anything to solve (h
   // then holds the final states), else the states are on the device (env_cg_).
+  // timing_ok: its first burst may be a timed sample (a whole solve, not a
+  // continuation of a few environments).
   bool cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, const std::vector<int32_t>& warm,
-              bool decide, std::vector<EnvCG>& h);
+              bool decide, std::vector<EnvCG>& h, bool timing_ok);
   void ws_features();
   // FP64 ||b - A x||^2 and ||b||^2 of every environment with mask[e] into
   // norms2_ (queued); true_rel_from turns them into relative residuals.

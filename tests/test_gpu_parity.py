@@ -487,3 +487,57 @@ def test_mixed_without_refresh_keeps_an_fp64_iterate(oracle):
         assert s[0]["linear_true_rel_res"] <= 1e-6, s[0]
         assert s[0]["fp64_refreshes"] == 0
     compare(spec, sr, so, 2e-3)
+
+
+def rotation(axis, angle):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * K @ K
+
+
+SVD_CASES = {
+    "inverted": np.diag([1.1, 0.9, -0.3]),
+    "inverted_rotated": rotation([1, 2, 3], 0.7) @ np.diag([1.05, 0.95, -0.6]) @ rotation([3, -1, 1], 0.4).T,
+    "repeated_pair": rotation([0, 1, 1], 0.5) @ np.diag([1.2, 1.2, 0.8]),
+    "repeated_all": 1.1 * rotation([1, 1, 0], 0.3),
+    "pure_rotation": rotation([1, -2, 0.5], 0.9),
+    "near_flat": np.diag([1.0, 1.0, 1e-3]),
+}
+
+
+@pytest.mark.parametrize("mode,tol", [(_abi.PREC_MODE_FP64, 1e-8), (_abi.PREC_MODE_FP32, 2e-4)])
+@pytest.mark.parametrize("case", sorted(SVD_CASES))
+def test_one_tet_svd_edge_cases_match_oracle(oracle, mode, tol, case):
+    """One tet deformed affinely to F: inverted (sigma_3 < 0), repeated singular values (a pair,
+    all three), a pure rotation and a nearly flat tet.  These are the degenerate cases of the
+    rotation-variant SVD and the symmetric eigen-solver behind the projected Smith tangent
+    (decomposition_utils.h:168-228); the GPU's analytic eigensystem and the oracle's 9x9
+    eigen-decomposition give the same residual and projected Hessian.  dt = 1 s keeps the
+    inertia term small beside the stress."""
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=float) * 0.05
+    X[3] += [0.004, 0.007, 0.0]
+    tets = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    F = SVD_CASES[case]
+    u = (X - X[0]) @ (F - np.eye(3)).T
+    sp = _abi.default_soft_params(E=1e5, nu=0.4)
+    p = _abi.pinned_solver_params()
+    g = gs.Scene(precision=mode)
+    a = g.create_soft_actor(X, tets, sp)
+    a.set_displacements(u)
+    o = oracle.OracleScene(fp64=mode == _abi.PREC_MODE_FP64)
+    o.add_soft_tet_mesh(X, tets, sp)
+    o.set_displacements(0, u)
+    rg, pg, cg, vg = a.assemble(1.0, p)
+    ro, po, co, vo = o.assemble(0, 1.0, p)
+    np.testing.assert_array_equal(cg, co)
+    # a pure rotation is stress-free: its residual is gravity plus the roundoff of cancelling
+    # stresses, so residual errors are measured against the force of a 1% strain at least
+    force_1pct = 1e-2 * np.abs(vo).max() * 0.05
+    np.testing.assert_allclose(rg, ro, rtol=tol, atol=tol * max(np.abs(ro).max(), force_1pct))
+    np.testing.assert_allclose(vg, vo, rtol=tol, atol=tol * np.abs(vo).max())
+    # the projected tangent is positive semi-definite on both sides
+    K = np.zeros((12, 12))
+    for i in range(4):
+        for k in range(pg[i], pg[i + 1]):
+            K[3 * i:3 * i + 3, 3 * cg[k]:3 * cg[k] + 3] = vg[k]
+    assert np.linalg.eigvalsh(0.5 * (K + K.T)).min() >= -1e-6 * np.abs(K).max()
