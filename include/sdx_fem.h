@@ -353,6 +353,14 @@ typedef struct sdx_schedule {
                              cooperative launch with grid-wide barriers between the matvec and
                              update phases instead of two launches per iteration (default 0:
                              the barriers' device-scope fences cost more than the launches) */
+  int32_t asm_groups;     /* assembly groups: the environments are assembled in this many
+                             contiguous groups, each writing its element and contact records and
+                             gathering them back before the next group reuses the same record
+                             buffers (sized for the largest group), so a group's records can stay
+                             in the GPU's last-level cache (0: groups of about 96 MB of records;
+                             default 1: one group, the records of every environment kept, which
+                             measured fastest: the element and gather kernels are bound by their
+                             scattered accesses, not by HBM bandwidth) */
 } sdx_schedule;
 
 sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* schedule);

@@ -83,6 +83,7 @@ class Schedule(C.Structure):
         ("cg_groups", C.c_int32),
         ("group_iters", C.c_int32),
         ("fused_tail", C.c_int32),
+        ("asm_groups", C.c_int32),
     ]
 
     def as_dict(self):

@@ -136,21 +136,37 @@ struct FieldPtrs {
 constexpr int kItemRowBits = 26;  // rows of the gather items below 2^26
 constexpr int kTriPairA[3] = {0, 0, 1};
 constexpr int kTriPairB[3] = {1, 2, 2};
+//
+// An assembly runs over a contiguous range of environments (an assembly
+// group): its tets [t0, t1), boundary triangles [f0, f1), diagonal gather
+// items [d0, d1) (= its vertices), upper gather items [u0, u1) and vector
+// tiles [tile0, tile1).  Record indices stay global; the record pointers a
+// launch takes are the group's buffers shifted by the group's first record,
+// so a group writes and reads only its own records.  contact_always: every
+// boundary triangle writes its contact records (zeros out of contact),
+// because the records of the previous assembly of these triangles may have
+// been overwritten by another group; 0: a triangle skips writing records
+// that are known to hold zeros (its flag bits).
+struct AsmRange {
+  int32_t t0, t1, f0, f1, d0, d1, u0, u1, tile0, tile1;
+  int32_t contact_always;
+};
 template <class T>
-void launch_element(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0,
-                    const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv, T* Ev,
-                    int block = 128, const KernelEvents* ev = nullptr);
+void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
+                    const T* u0, const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv,
+                    T* Ev, int block = 128, const KernelEvents* ev = nullptr);
 template <class T>
-void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, int wantH, T* Rv, T* Dv,
-                    T* Ev);
+void launch_contact(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
+                    int wantH, T* Rv, T* Dv, T* Ev);
 // BSR values (and the preconditioner blocks) from the records, over the work
-// items of the mesh (diagonal and upper slots).
+// items of the range (diagonal and upper slots).
 template <class T>
-void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* Dv, const T* Ev, int prec, T* vals, T* dinv,
-                       const KernelEvents* ev = nullptr);
+void launch_gather_bsr(hipStream_t s, const MeshView& m, const AsmRange& r, const T* Dv, const T* Ev, int prec,
+                       T* vals, T* dinv, const KernelEvents* ev = nullptr);
 // acc: set 0 of the accumulators (zeroed by the caller); per-env ||res||^2.
 template <class T, class A>
-void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Rv, T* res, A* acc);
+void launch_gather_res(hipStream_t s, const MeshView& m, const AsmRange& r, const Tiles& tv, const T* Rv, T* res,
+                       A* acc);
 template <class A>
 void launch_fill(hipStream_t s, int64_t n, A v, A* d);
 // out[e * nv + k] = sum over slots of set 0, k < nv.

@@ -36,12 +36,12 @@ namespace sdx {
 // ------------------------------------------------------------------ element
 
 template <class T>
-__global__ void __launch_bounds__(256) k_element(MeshView m, FieldPtrs<T> f, const T* __restrict__ u,
+__global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldPtrs<T> f, const T* __restrict__ u,
                                                  const T* __restrict__ u0, const T* __restrict__ v0, T dt,
                                                  T gx, T gy, T gz, int wantH, int project, T* __restrict__ Rv,
                                                  T* __restrict__ Dv, T* __restrict__ Ev) {
-  const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= m.nT) return;
+  const int t = ar.t0 + xcd_tile(blockIdx.x, gridDim.x) * int(blockDim.x) + int(threadIdx.x);
+  if (t >= ar.t1) return;
   const int env = m.tet_env[t];
   if (!m.env_mask[env]) return;
   const int nT = m.nT;
@@ -146,10 +146,11 @@ __device__ __forceinline__ int sym_entry(int r, int c) { return r == c ? r : 2 +
 // (zeros when nothing touches): 3 residual rows, 3 diagonal blocks and the 3
 // blocks of its vertex pairs (symmetric, so no transposition).
 template <class T>
-__global__ void __launch_bounds__(256) k_contact(MeshView m, FieldPtrs<T> f, const T* __restrict__ u, int wantH,
-                                                 T* __restrict__ Rv, T* __restrict__ Dv, T* __restrict__ Ev) {
-  const int tr = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tr >= m.nF) return;
+__global__ void __launch_bounds__(256) k_contact(MeshView m, AsmRange ar, FieldPtrs<T> f, const T* __restrict__ u,
+                                                 int wantH, T* __restrict__ Rv, T* __restrict__ Dv,
+                                                 T* __restrict__ Ev) {
+  const int tr = ar.f0 + int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (tr >= ar.f1) return;
   const int env = m.tri_env[tr];
   if (!m.env_mask[env]) return;
   T rc[9];
@@ -228,8 +229,10 @@ __global__ void __launch_bounds__(256) k_contact(MeshView m, FieldPtrs<T> f, con
   }
   // A triangle out of contact has zero records: they are written only if
   // it is in contact now or its records still hold values from an earlier
-  // write (flag bits), so most triangles write nothing.
-  const uint8_t fl = m.coll_flags[tr];
+  // write (flag bits), so most triangles write nothing; with contact_always
+  // (records shared between assembly groups) every triangle writes them and
+  // the flags stay set.
+  const uint8_t fl = ar.contact_always ? uint8_t(3) : m.coll_flags[tr];
   uint8_t nf = uint8_t((fl & 2) | (touched ? 1 : 0));
   if (touched || (fl & 1)) {
 #pragma unroll
@@ -260,7 +263,7 @@ __global__ void __launch_bounds__(256) k_contact(MeshView m, FieldPtrs<T> f, con
       }
     }
   }
-  if (nf != fl) m.coll_flags[tr] = nf;
+  if (nf != fl && !ar.contact_always) m.coll_flags[tr] = nf;
 }
 
 // ------------------------------------------------------------------ gathers
@@ -374,13 +377,14 @@ constexpr int kGatherWaves = 4;
 constexpr int kGatherItems = 7 * kGatherWaves;
 
 template <class T>
-__global__ void __launch_bounds__(64 * kGatherWaves) k_gather_bsr(MeshView m, const T* __restrict__ Dv,
+__global__ void __launch_bounds__(64 * kGatherWaves) k_gather_bsr(MeshView m, AsmRange ar, const T* __restrict__ Dv,
                                                                   const T* __restrict__ Ev, int prec,
                                                                   int n_diag_blocks, T* __restrict__ vals,
                                                                   T* __restrict__ dinv) {
   // Nine lanes per work item, lane q owning entry q = 3 r + c of the 3x3
-  // block.  Items: the diagonal slots (the first nV items, one per vertex)
-  // and the upper slots (i < j); an upper slot also writes its mirror (j, i)
+  // block.  Items: the diagonal slots (the first nV items, one per vertex;
+  // those of the range are [d0, d1)) and the upper slots (i < j; [u0, u1));
+  // an upper slot also writes its mirror (j, i)
   // transposed, so each pair's records are read once.  The records of
   // consecutive items are contiguous (vertex-sorted diagonal records,
   // pair-sorted off-diagonal records, element and contact alike), so a
@@ -396,8 +400,8 @@ __global__ void __launch_bounds__(64 * kGatherWaves) k_gather_bsr(MeshView m, co
   T* lds = reinterpret_cast<T*>(lds_v);
   const int blk = xcd_tile(blockIdx.x, gridDim.x);
   const bool diag = blk < n_diag_blocks;
-  const int first = diag ? blk * kGatherItems : m.nV + (blk - n_diag_blocks) * kGatherItems;
-  const int last = min(first + kGatherItems, diag ? m.nV : m.n_items);  // exclusive
+  const int first = diag ? ar.d0 + blk * kGatherItems : ar.u0 + (blk - n_diag_blocks) * kGatherItems;
+  const int last = min(first + kGatherItems, diag ? ar.d1 : ar.u1);  // exclusive
   const int rec0 = m.item_a[4 * first + 2], rec1 = m.item_a[4 * (last - 1) + 3];
   const int W = diag ? 6 : 9;
   const T* src = diag ? Dv : Ev;
@@ -502,12 +506,12 @@ __global__ void __launch_bounds__(256) k_block_inverse(int nV, const int32_t* __
 // contiguous (vertex-sorted), so the workgroup stages them through LDS with
 // 16-byte loads and each thread sums its run from LDS.
 template <class T, class A>
-__global__ void __launch_bounds__(kVecThreads) k_gather_res(MeshView m, Tiles tv, const T* __restrict__ Rv,
-                                                            T* __restrict__ res, A* acc) {
+__global__ void __launch_bounds__(kVecThreads) k_gather_res(MeshView m, AsmRange ar, Tiles tv,
+                                                            const T* __restrict__ Rv, T* __restrict__ res, A* acc) {
   __shared__ typename Vec16<T>::type lds_v[kStage<T> * sizeof(T) / 16];
   T* lds = reinterpret_cast<T*>(lds_v);
   constexpr int kW = 16 / int(sizeof(T));
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = ar.tile0 + xcd_tile(blockIdx.x, gridDim.x);
   const int env = tv.env[tile];
   const int row0 = tv.row0[tile], row1 = tv.row1[tile];
   const int msk = m.env_mask[env];
@@ -1464,34 +1468,38 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs
   } while (0)
 
 template <class T>
-void launch_element(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0, const T* v0,
-                    double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv, T* Ev, int block,
-                    const KernelEvents* ev) {
-  if (m.nT == 0) return;
-  SDX_LAUNCH(ev, k_element<T>, dim3(blocks_for(m.nT, block)), dim3(block), s, m, f, u, u0, v0, T(dt), T(grav[0]),
-             T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev);
+void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
+                    const T* u0, const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv,
+                    T* Ev, int block, const KernelEvents* ev) {
+  if (r.t1 <= r.t0) return;
+  SDX_LAUNCH(ev, k_element<T>, dim3(blocks_for(r.t1 - r.t0, block)), dim3(block), s, m, r, f, u, u0, v0, T(dt),
+             T(grav[0]), T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev);
 }
 
 template <class T>
-void launch_contact(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, int wantH, T* Rv, T* Dv,
-                    T* Ev) {
-  if (m.nF == 0) return;
-  hipLaunchKernelGGL(k_contact<T>, dim3(blocks_for(m.nF, 256)), dim3(256), 0, s, m, f, u, wantH, Rv, Dv, Ev);
+void launch_contact(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
+                    int wantH, T* Rv, T* Dv, T* Ev) {
+  if (r.f1 <= r.f0) return;
+  hipLaunchKernelGGL(k_contact<T>, dim3(blocks_for(r.f1 - r.f0, 256)), dim3(256), 0, s, m, r, f, u, wantH, Rv, Dv,
+                     Ev);
 }
 
 template <class T>
-void launch_gather_bsr(hipStream_t s, const MeshView& m, const T* Dv, const T* Ev, int prec, T* vals, T* dinv,
-                       const KernelEvents* ev) {
-  if (m.n_items == 0) return;
-  const int nd = (m.nV + kGatherItems - 1) / kGatherItems;
-  const int nu = (m.n_items - m.nV + kGatherItems - 1) / kGatherItems;
-  SDX_LAUNCH(ev, k_gather_bsr<T>, dim3(unsigned(nd + nu)), dim3(64 * kGatherWaves), s, m, Dv, Ev, prec, nd, vals, dinv);
+void launch_gather_bsr(hipStream_t s, const MeshView& m, const AsmRange& r, const T* Dv, const T* Ev, int prec,
+                       T* vals, T* dinv, const KernelEvents* ev) {
+  const int nd = (r.d1 - r.d0 + kGatherItems - 1) / kGatherItems;
+  const int nu = (r.u1 - r.u0 + kGatherItems - 1) / kGatherItems;
+  if (nd + nu == 0) return;
+  SDX_LAUNCH(ev, k_gather_bsr<T>, dim3(unsigned(nd + nu)), dim3(64 * kGatherWaves), s, m, r, Dv, Ev, prec, nd, vals,
+             dinv);
 }
 
 template <class T, class A>
-void launch_gather_res(hipStream_t s, const MeshView& m, const Tiles& tv, const T* Rv, T* res, A* acc) {
-  if (tv.n == 0) return;
-  hipLaunchKernelGGL((k_gather_res<T, A>), dim3(tv.n), dim3(kVecThreads), 0, s, m, tv, Rv, res, acc);
+void launch_gather_res(hipStream_t s, const MeshView& m, const AsmRange& r, const Tiles& tv, const T* Rv, T* res,
+                       A* acc) {
+  if (r.tile1 <= r.tile0) return;
+  hipLaunchKernelGGL((k_gather_res<T, A>), dim3(unsigned(r.tile1 - r.tile0)), dim3(kVecThreads), 0, s, m, r, tv, Rv,
+                     res, acc);
 }
 
 template <class A>
@@ -1712,12 +1720,13 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
 }
 
 #define SDX_INST_T(T)                                                                                            \
-  template void launch_element<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, const T*,        \
-                                  const T*, double, const double*, int, int, T*, T*, T*, int,                   \
+  template void launch_element<T>(hipStream_t, const MeshView&, const AsmRange&, const FieldPtrs<T>&, const T*, \
+                                  const T*, const T*, double, const double*, int, int, T*, T*, T*, int,         \
                                   const KernelEvents*);                                                         \
-  template void launch_contact<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, int, T*, T*, T*); \
-  template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const T*, const T*, int, T*, T*,             \
-                                     const KernelEvents*);                                                      \
+  template void launch_contact<T>(hipStream_t, const MeshView&, const AsmRange&, const FieldPtrs<T>&, const T*, \
+                                  int, T*, T*, T*);                                                             \
+  template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const AsmRange&, const T*, const T*, int, T*, \
+                                     T*, const KernelEvents*);                                                  \
   template void launch_block_inverse<T>(hipStream_t, int32_t, const int32_t*, const T*, int32_t, int, T*);     \
   template void launch_copy<T>(hipStream_t, int64_t, const T*, T*);                                             \
   template void launch_apply_pins<T>(hipStream_t, int64_t, const uint8_t*, const T*, T*);                       \
@@ -1732,7 +1741,8 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
   template void launch_cg_reset<T>(hipStream_t, int, const int32_t*, EnvCG*, T*);
 
 #define SDX_INST_TA(T, A)                                                                                         \
-  template void launch_gather_res<T, A>(hipStream_t, const MeshView&, const Tiles&, const T*, T*, A*);          \
+  template void launch_gather_res<T, A>(hipStream_t, const MeshView&, const AsmRange&, const Tiles&, const T*,  \
+                                        T*, A*);                                                                \
   template void launch_cg_init<T, A>(hipStream_t, const Tiles&, const CgConfig&, const T*, const int32_t*,       \
                                      const double*, const double*, const T*, const double*, const double*,      \
                                      const T*, const double*, const double*, const T*, int32_t, A*, T*, T*,     \

@@ -324,6 +324,17 @@ void World<T>::finalize() {
   std::vector<uint32_t> ecode, vtx_code;
   CountingCsr(nB_, edge_keys, edge_codes, eptr, ecode);
   CountingCsr(nV_, vtx_keys, vtx_codes, vtx_ptr, vtx_code);
+  // per-environment starts of tets, triangles and records (assembly groups)
+  env_t0_ = env_t0;
+  env_f0_.assign(size_t(nEnv) + 1, nF_);
+  for (int f = nF_ - 1; f >= 0; --f) env_f0_[size_t(tri_env[size_t(f)])] = f;
+  for (int e = nEnv - 1; e >= 0; --e) env_f0_[size_t(e)] = std::min(env_f0_[size_t(e)], env_f0_[size_t(e) + 1]);
+  env_rv0_.assign(size_t(nEnv) + 1, 0);
+  env_re0_.assign(size_t(nEnv) + 1, 0);
+  for (int e = 0; e <= nEnv; ++e) {
+    env_rv0_[size_t(e)] = vtx_ptr[size_t(env_v0_[size_t(e)])];
+    env_re0_[size_t(e)] = eptr[size_t(row_ptr[size_t(env_v0_[size_t(e)])])];
+  }
   // record of every element and contact contribution: the vertex-sorted
   // position of (tet, a) / (triangle, a) for residuals and diagonal blocks,
   // the pair-sorted position of (tet, pair) / (triangle, pair) for
@@ -352,8 +363,11 @@ void World<T>::finalize() {
   std::vector<int32_t> item_a, item_b;
   item_a.reserve(size_t(4) * (size_t(nV_) + size_t(nB_ - nV_) / 2));
   item_b.reserve(item_a.capacity() / 2);
+  env_u0_.assign(size_t(nEnv) + 1, 0);
   for (int pass = 0; pass < 2; ++pass)
-    for (int i = 0; i < nV_; ++i)
+    for (int i = 0; i < nV_; ++i) {
+      if (pass == 1 && i == env_v0_[size_t(row_env[size_t(i)])])  // first upper item of the environment
+        env_u0_[size_t(row_env[size_t(i)])] = int32_t(item_b.size() / 2);
       for (int k = row_ptr[size_t(i)]; k < row_ptr[size_t(i) + 1]; ++k) {
         const int j = col[size_t(k)];
         if (pass == 0 ? j != i : j <= i) continue;
@@ -367,6 +381,8 @@ void World<T>::finalize() {
         item_a.insert(item_a.end(), {k, mirror, r0, r1});
         item_b.insert(item_b.end(), {i | (pins << kItemRowBits), row_env[size_t(i)]});
       }
+    }
+  env_u0_[size_t(nEnv)] = int32_t(item_b.size() / 2);
   BuildTiles(env_v0_, nEnv, kVecThreads, tiles_v_);
   BuildTiles(env_v0_, nEnv, kSpmvThreads / sched_.spmv_lanes, tiles_a_);
   max_env_a_ = 0;
@@ -407,8 +423,6 @@ void World<T>::finalize() {
   d_pos_e_.upload(pos_e);
   d_pos_vc_.upload(pos_vc);
   d_pos_ec_.upload(pos_ec);
-  // every triangle's contact records start out unwritten (both flags set)
-  d_coll_flags_.upload(std::vector<uint8_t>(std::max<size_t>(pos_vc.size() / 3, 1), uint8_t(3)));
   d_vtx_ptr_.upload(vtx_ptr);
   d_pinned_.upload(pinned);
   d_pin_value_.upload(pin_value);
@@ -437,12 +451,8 @@ void World<T>::finalize() {
   rref_.alloc(n3);
   rref2_.alloc(n3);
   rref3_.alloc(n3);
-  // padded by one 16-byte vector: the gathers stage these with vector loads
-  const size_t nrec_v = size_t(4) * size_t(nT_) + size_t(3) * size_t(nF_);
-  const size_t nrec_e = size_t(6) * size_t(nT_) + size_t(3) * size_t(nF_);
-  cres_.alloc(3 * nrec_v + 4);
-  cdiag_.alloc(6 * nrec_v + 4);
-  coff_.alloc(9 * nrec_e + 4);
+  asm_plan_n_ = 0;
+  plan_assembly(asm_groups(sched_));  // record buffers and assembly group ranges
   vals_.alloc(size_t(9) * size_t(nB_));
   dinv_.alloc(size_t(6) * size_t(nV_));
   env_cg_.alloc(static_cast<size_t>(nEnv));
@@ -583,28 +593,118 @@ void World<T>::set_env_mask(const std::vector<int32_t>& mask) {
 }
 
 // Assemble residual (and BSR + preconditioner if wantH) for the environments
-// in the current mask; per-environment ||res||^2 lands in env_sq_.
+// in the current mask; per-environment ||res||^2 lands in env_sq_.  Assembly
+// group by assembly group: a group's element and contact kernels write its
+// records, its gathers read them back at once, so with several groups the
+// records of one group (sized to stay in the last-level cache) are all that
+// is live, and the next group reuses the same buffers.
 template <class T>
 void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH) {
   const int project = sp.psd_proj_mode == SDX_PSD_ALWAYS ? 1 : 0;
-  launch_element<T>(stream_, mesh_, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, cres_.p,
-                    cdiag_.p, coff_.p, sched_.element_block, sampled_events(2));
-  launch_contact<T>(stream_, mesh_, fields_, u_.p, wantH ? 1 : 0, cres_.p, cdiag_.p, coff_.p);
-  if (wantH) {
-    launch_gather_bsr<T>(stream_, mesh_, cdiag_.p, coff_.p, sp.linear.preconditioner, vals_.p, dinv_.p,
-                         sampled_events(3));
-  }
   const int64_t set0 = int64_t(nEnv_) * kSlots * kSlotStride;
-  if (precision_ == SDX_PREC_MODE_FP32) {
-    T* acc = reinterpret_cast<T*>(acc_.p);
-    launch_fill<T>(stream_, set0, T(0), acc);
-    launch_gather_res<T, T>(stream_, mesh_, dt_v_.view, cres_.p, res_.p, acc);
-    launch_acc_finalize<T>(stream_, nEnv_, 1, acc, env_sq_.p);
-  } else {
+  const bool dd = precision_ != SDX_PREC_MODE_FP32;
+  if (dd)
     launch_fill<double>(stream_, set0, 0.0, acc_.p);
-    launch_gather_res<T, double>(stream_, mesh_, dt_v_.view, cres_.p, res_.p, acc_.p);
-    launch_acc_finalize<double>(stream_, nEnv_, 1, acc_.p, env_sq_.p);
+  else
+    launch_fill<T>(stream_, set0, T(0), reinterpret_cast<T*>(acc_.p));
+  const bool shared = asm_plan_.size() > 1;
+  for (const AsmGroup& g : asm_plan_) {
+    AsmRange r = g.r;
+    r.contact_always = shared ? 1 : 0;
+    T* Rv = group_records(cres_.p, 3, g.rv0);
+    T* Dv = group_records(cdiag_.p, 6, g.rv0);
+    T* Ev = group_records(coff_.p, 9, g.re0);
+    launch_element<T>(stream_, mesh_, r, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, Rv, Dv,
+                      Ev, sched_.element_block, sampled_events(2));
+    launch_contact<T>(stream_, mesh_, r, fields_, u_.p, wantH ? 1 : 0, Rv, Dv, Ev);
+    if (wantH)
+      launch_gather_bsr<T>(stream_, mesh_, r, Dv, Ev, sp.linear.preconditioner, vals_.p, dinv_.p, sampled_events(3));
+    if (dd)
+      launch_gather_res<T, double>(stream_, mesh_, r, dt_v_.view, Rv, res_.p, acc_.p);
+    else
+      launch_gather_res<T, T>(stream_, mesh_, r, dt_v_.view, Rv, res_.p, reinterpret_cast<T*>(acc_.p));
   }
+  if (dd)
+    launch_acc_finalize<double>(stream_, nEnv_, 1, acc_.p, env_sq_.p);
+  else
+    launch_acc_finalize<T>(stream_, nEnv_, 1, reinterpret_cast<T*>(acc_.p), env_sq_.p);
+}
+
+// Record pointer of a group whose first record is rec0 (W values each) for
+// the buffer `buf`: shifted so that record rec0 lands at buf + off, with off
+// (< one 16-byte vector) chosen so that the gathers' 16-byte-aligned loads
+// of global record positions stay aligned.  Kernels index it with global
+// record positions of the group only, so every access lies in [buf, buf +
+// off + records of the group + one vector).
+template <class T>
+T* World<T>::group_records(T* buf, int W, int64_t rec0) const {
+  constexpr int64_t kW = 16 / int64_t(sizeof(T));
+  const int64_t off = (rec0 * W) % kW;
+  return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(buf) +
+                              uintptr_t(off - rec0 * W) * uintptr_t(sizeof(T)));
+}
+
+// Assembly groups asked for by the schedule; asm_groups 0 sizes them to
+// about kAsmGroupBytes of element and contact records each.
+template <class T>
+int World<T>::asm_groups(const sdx_schedule& sc) const {
+  int n = sc.asm_groups;
+  if (n == 0) {
+    const double bytes = double(env_rv0_.back()) * 9.0 * sizeof(T) + double(env_re0_.back()) * 9.0 * sizeof(T);
+    n = int(std::ceil(bytes / kAsmGroupBytes));
+  }
+  return std::min(std::max(1, n), std::max(1, nEnv_));
+}
+
+// n contiguous environment ranges of about equal record counts, their tet,
+// triangle, item and tile ranges, and record buffers for the largest.
+template <class T>
+void World<T>::plan_assembly(int n) {
+  if (asm_plan_n_ == n && !asm_plan_.empty()) return;
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  asm_plan_n_ = n;
+  asm_plan_.clear();
+  const int64_t total = int64_t(env_rv0_.back()) * 9 + int64_t(env_re0_.back()) * 9;
+  int64_t max_rv = 0, max_re = 0;
+  int e = 0;
+  for (int g = 1; g <= n && e < nEnv_; ++g) {
+    const int64_t goal = total * g / n;
+    int e1 = e + 1;  // at least one environment per group
+    auto recs = [&](int k) { return int64_t(env_rv0_[size_t(k)]) * 9 + int64_t(env_re0_[size_t(k)]) * 9; };
+    while (e1 < nEnv_ && recs(e1 + 1) <= goal) ++e1;
+    if (g == n) e1 = nEnv_;
+    AsmGroup G{};
+    G.r.t0 = env_t0_[size_t(e)];
+    G.r.t1 = env_t0_[size_t(e1)];
+    G.r.f0 = env_f0_[size_t(e)];
+    G.r.f1 = env_f0_[size_t(e1)];
+    G.r.d0 = env_v0_[size_t(e)];
+    G.r.d1 = env_v0_[size_t(e1)];
+    G.r.u0 = env_u0_[size_t(e)];  // item indices: the nV diagonal items come first
+    G.r.u1 = env_u0_[size_t(e1)];
+    G.r.tile0 = tiles_v_.first[size_t(e)];
+    G.r.tile1 = tiles_v_.first[size_t(e1)];
+    G.rv0 = env_rv0_[size_t(e)];
+    G.re0 = env_re0_[size_t(e)];
+    // every range within its array: the kernels index with these unchecked
+    if (G.r.t0 < 0 || G.r.t1 > nT_ || G.r.t0 > G.r.t1 || G.r.f0 < 0 || G.r.f1 > nF_ || G.r.f0 > G.r.f1 ||
+        G.r.d0 < 0 || G.r.d1 > nV_ || G.r.d0 > G.r.d1 || G.r.u0 < nV_ || G.r.u1 > n_items_ || G.r.u0 > G.r.u1 ||
+        G.r.tile0 < 0 || G.r.tile1 > int(tiles_v_.env.size()) || G.r.tile0 > G.r.tile1)
+      throw std::logic_error("assembly group range out of bounds");
+    max_rv = std::max<int64_t>(max_rv, int64_t(env_rv0_[size_t(e1)]) - G.rv0);
+    max_re = std::max<int64_t>(max_re, int64_t(env_re0_[size_t(e1)]) - G.re0);
+    asm_plan_.push_back(G);
+    e = e1;
+  }
+  // the group's records, its alignment offset and one 16-byte vector past
+  // them (the gathers' vector loads)
+  constexpr size_t kW = 16 / sizeof(T);
+  cres_.alloc(size_t(3 * max_rv) + 2 * kW);
+  cdiag_.alloc(size_t(6 * max_rv) + 2 * kW);
+  coff_.alloc(size_t(9 * max_re) + 2 * kW);
+  // every triangle's contact records start out unwritten (both flags set)
+  d_coll_flags_.upload(std::vector<uint8_t>(std::max<size_t>(size_t(nF_), 1), uint8_t(3)));
+  mesh_.coll_flags = d_coll_flags_.p;
 }
 
 template <class T>
@@ -1428,6 +1528,7 @@ void World<T>::set_schedule(const sdx_schedule& sc) {
     upload_tiles(tiles_a_, dt_a_);
     alloc_lists();
   }
+  if (finalized_) plan_assembly(asm_groups(sched_));
 }
 
 // Coordinate search over the schedule: each candidate runs `trials` timed
@@ -1532,6 +1633,16 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
           consider(c);
         }
     }
+  }
+  if (nEnv_ > 1) {  // assembly groups: one (every record kept), the automatic count, half and twice it
+    const int a = asm_groups(cur);
+    std::vector<int> cand = {1, std::max(1, a / 2), asm_groups(sdx_schedule{0, 0, 0, 0, 0, 0, 0, 0, 0}), 2 * a};
+    for (int ag : cand)
+      if (ag <= nEnv_ && ag != asm_groups(cur)) {
+        sdx_schedule c = cur;
+        c.asm_groups = ag;
+        consider(c);
+      }
   }
   set_schedule(cur);
   restore();

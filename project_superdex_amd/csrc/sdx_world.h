@@ -291,7 +291,7 @@ class WorldBase {
     flush_timers();
     return {double(timers_[size_t(k)].launches), timers_[size_t(k)].ms};
   }
-  sdx_schedule sched_{8, 1, 16, 128, 1, 0, 16, 0};
+  sdx_schedule sched_{8, 1, 16, 128, 1, 0, 16, 0, 1};
   bool tuned_ = false;  // the schedule was autotuned for the current topology
 
  protected:
@@ -455,6 +455,20 @@ class World final : public WorldBase {
   // norms2_ (queued); true_rel_from turns them into relative residuals.
   void issue_true_residual(const std::vector<int32_t>& mask);
   std::vector<double> true_rel_from(const std::vector<int32_t>& mask, const std::vector<double>& q) const;
+  // assembly groups (sdx_schedule.asm_groups): contiguous environment ranges
+  // assembled one after the other through record buffers sized for the
+  // largest; one group keeps the records of every environment
+  struct AsmGroup {
+    AsmRange r;
+    int64_t rv0, re0;  // first vertex-sorted and pair-sorted record of the group
+  };
+  std::vector<AsmGroup> asm_plan_;
+  int asm_plan_n_ = 0;
+  static constexpr double kAsmGroupBytes = 96e6;
+  int asm_groups(const sdx_schedule& sc) const;
+  void plan_assembly(int n);
+  T* group_records(T* buf, int W, int64_t rec0) const;
+  std::vector<int32_t> env_t0_, env_f0_, env_u0_, env_rv0_, env_re0_;
   void issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed, int cls);
   void issue_cg_tail(const CgConfig& cfg, int par, int k0, int n);
   void plan_groups(int n);
