@@ -177,6 +177,16 @@ typedef struct sdx_soft_params {
   double penalty_smoothing_half_distance; /* default 0.005 m */
   double penalty_threshold;       /* default 0.001 m */
   double max_alignment_normals;   /* default 0 */
+  /* Contact dissipation (contact_params.h:170-225), evaluated as the reference's default
+   * ExperimentalEvalParams do (eval_params.h:73-140): C1-regularised Coulomb friction along the
+   * collider normal's tangent plane, viscous friction and normal damping, all proportional to the
+   * penalty force at the sample's stage-start distance and faded by normal alignment, with the
+   * fitted (PSD) friction Hessian.  The colliders carry no contact parameters of their own: these
+   * are the pair's values (the reference's geometric mean of equal coefficients). */
+  double coulomb_friction_coefficient;       /* default 0.5 */
+  double viscous_friction_coefficient;       /* [s/m], default 0 */
+  double friction_falloff_vel;               /* [m/s], default 0.01 */
+  double normal_viscous_damping_coefficient; /* [s/m], default 0 */
 } sdx_soft_params;
 
 typedef struct sdx_world sdx_world;

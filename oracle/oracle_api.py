@@ -208,10 +208,7 @@ def build_scene(spec: SceneSpec, fp64: bool = False) -> tuple[OracleScene, list]
     """Create the oracle scene of a SceneSpec; returns (scene, collider ids)."""
     sc = OracleScene(fp64=fp64)
     for b in spec.bodies:
-        E0 = float(np.mean(b.E)) if np.ndim(b.E) else float(b.E)
-        nu0 = float(np.mean(b.nu)) if np.ndim(b.nu) else float(b.nu)
-        a = sc.add_soft_tet_mesh(b.X, b.tets, _abi.default_soft_params(E0, nu0, b.density, b.penalty,
-                                                                       b.mass_damping))
+        a = sc.add_soft_tet_mesh(b.X, b.tets, b.soft_params(_abi))
         if np.ndim(b.E):
             sc.set_element_materials(a, b.E, b.nu)
         if len(b.pinned):

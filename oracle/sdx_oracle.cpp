@@ -209,6 +209,26 @@ void PolyReLU(T x, T t, T shift, T& f, T& df, T& ddf) {
   ddf = std::max(T(0), mask ? T(0) : T(0.75) * (T(1) - x2t2) / t);
 }
 
+// IPCstepC1, activations.h:131-160: the C1 interpolant of the unit step of
+// Incremental Potential Contact (Li et al. 2020) and its antiderivative, for
+// x >= 0 and transition width t >= 0: df = 1 for x >= t, else 2x/t - x^2/t^2;
+// dfx = df / x (0 at x = 0).
+template <class T>
+void IPCStepC1(T x, T t, T& f, T& df, T& ddf, T& dfx) {
+  if (x >= t) {
+    f = x - t / T(3);
+    ddf = T(0);
+    dfx = x == T(0) ? T(0) : T(1) / x;
+    df = T(1);
+  } else {
+    const T xt = x / t;
+    f = x * xt * (T(1) - xt / T(3));
+    ddf = T(2) * (T(1) - xt) / t;
+    dfx = (T(2) - xt) / t;
+    df = x * dfx;
+  }
+}
+
 // ------------------------------------------------------------------ element geometry
 
 template <class T>
@@ -527,6 +547,7 @@ struct Collider {
   int kind;  // 0 plane, 1 sphere
   double p[3], n[3], r;
   int actor;
+  double p0[3];  // position at stage start (the previous step's target)
 };
 
 template <class T>
@@ -538,6 +559,8 @@ struct Actor {
   std::vector<T> mu, lambda;  // per tet
   T density = 1000, massDamping = 0;
   T kPen = 1e6, hPen = 0.005, thrPen = 0.001, maxAlign = 0;
+  // dissipation (ContactParams, contact_params.h:170-225)
+  T muCoulomb = 0, cViscous = 0, falloffVel = 0.01, cNormalDamping = 0;
   std::vector<uint8_t> pinned;  // per DoF
   std::vector<T> pinValue;      // per DoF
   std::vector<T> u, v;
@@ -573,6 +596,10 @@ struct Scene {
     a.hPen = T(sp.penalty_smoothing_half_distance);
     a.thrPen = T(sp.penalty_threshold);
     a.maxAlign = T(sp.max_alignment_normals);
+    a.muCoulomb = T(sp.coulomb_friction_coefficient);
+    a.cViscous = T(sp.viscous_friction_coefficient);
+    a.falloffVel = T(sp.friction_falloff_vel);
+    a.cNormalDamping = T(sp.normal_viscous_damping_coefficient);
     a.pinned.assign(3 * size_t(nV), 0);
     a.pinValue.assign(3 * size_t(nV), T(0));
     a.u.assign(3 * size_t(nV), T(0));
@@ -753,23 +780,43 @@ struct Scene {
           }
       }
     }
-    AssembleContact(a, ai, u, wantH, res);
+    AssembleContact(a, ai, u, u0, dt, wantH, res);
     ApplyBc(a, wantH, res);
   }
 
-  void AssembleContact(Actor<T>& a, int ai, const std::vector<T>& u, bool wantH, std::vector<T>& res) {
+  // Penalty contact and its dissipation (friction, viscous friction, normal
+  // damping) at the trace-Q3 samples of the boundary triangles, for the
+  // reference's default evaluation settings (ExperimentalEvalParams,
+  // mochi_core/include/mochi_core/utils/eval_params.h:73-140: implicit
+  // normals, friction faded by normal alignment, explicit (stage-start)
+  // normal force for dissipation, fitted friction Hessian, C1-regularised
+  // Coulomb friction; frictionWithColliderNormal).
+  // ComputeBatchCollisionForceDForce (contact_utils.h:1010-1200) with
+  // ComputeBatchContactPenaltyForceDForce (:498-570),
+  // ComputeBatchContactPenaltyForceNormDForceNorm (:575-622) and
+  // ComputeBatchContactDissipationForceDForce (:630-935).  The relative
+  // displacement of a sample is taken in the collider's frame (translation
+  // only here): pRel = (x - x0) - (P - P0), with x0 the sample at stage start
+  // and P0 the collider's stage-start position
+  // (mochi_physics/src/mochi_contact.cpp:1200-1260).
+  void AssembleContact(Actor<T>& a, int ai, const std::vector<T>& u, const std::vector<T>& u0, T dt, bool wantH,
+                       std::vector<T>& res) {
     static const T kBary[3][3] = {{T(1.0 / 6.0), T(2.0 / 3.0), T(1.0 / 6.0)},
                                   {T(1.0 / 6.0), T(1.0 / 6.0), T(2.0 / 3.0)},
                                   {T(2.0 / 3.0), T(1.0 / 6.0), T(1.0 / 6.0)}};
     const int nTri = int(a.triArea.size());
+    const bool dissip = a.muCoulomb > T(0) || a.cViscous > T(0) || a.cNormalDamping > T(0);
     for (size_t c = 0; c < colliders.size(); ++c) {
       const Collider& col = colliders[c];
       if (col.actor >= 0 && col.actor != ai) continue;
       for (int tr = 0; tr < nTri; ++tr) {
         const int32_t* nd = &a.tris[3 * size_t(tr)];
-        T xs[3][3];
+        T xs[3][3], x0s[3][3];
         for (int i = 0; i < 3; ++i)
-          for (int d = 0; d < 3; ++d) xs[i][d] = a.X[3 * size_t(nd[i]) + d] + u[3 * size_t(nd[i]) + d];
+          for (int d = 0; d < 3; ++d) {
+            xs[i][d] = a.X[3 * size_t(nd[i]) + d] + u[3 * size_t(nd[i]) + d];
+            x0s[i][d] = a.X[3 * size_t(nd[i]) + d] + u0[3 * size_t(nd[i]) + d];
+          }
         // current outward face normal (SetupActiveCollisionNormals, mochi_deformable.cpp:160-176)
         T e1[3], e2[3], fn[3];
         for (int d = 0; d < 3; ++d) {
@@ -783,9 +830,12 @@ struct Scene {
         for (int d = 0; d < 3; ++d) fn[d] /= fl;
         const T w = a.triArea[size_t(tr)] / T(3);
         for (int q = 0; q < 3; ++q) {
-          T x[3] = {0, 0, 0};
+          T x[3] = {0, 0, 0}, x0[3] = {0, 0, 0};
           for (int i = 0; i < 3; ++i)
-            for (int d = 0; d < 3; ++d) x[d] += kBary[q][i] * xs[i][d];
+            for (int d = 0; d < 3; ++d) {
+              x[d] += kBary[q][i] * xs[i][d];
+              x0[d] += kBary[q][i] * x0s[i][d];
+            }
           T dist, grad[3];
           if (col.kind == 0) {
             dist = 0;
@@ -811,9 +861,49 @@ struct Scene {
           PolyReLU<T>(-dist, a.hPen, a.hPen - a.thrPen, pen, dpen, ddpen);
           const T fnorm = a.kPen * pen * dpen;
           const T dfn = -a.kPen * (dpen * dpen + pen * ddpen);
+          // force on the sample (collider normal times the penalty force) and its
+          // derivative dF = cI I + cN n n^T
+          T F[3];
+          for (int d = 0; d < 3; ++d) F[d] = fnorm * grad[d];
+          T cI = 0, cN = dfn;
+          if (dissip) {
+            // alignment fading (eval_params.h:90-103), maxAlignmentNormals > -1
+            const T fade = a.maxAlign > T(-1) ? (a.maxAlign - align) / (a.maxAlign + T(1)) : T(1);
+            T pRel[3];
+            for (int d = 0; d < 3; ++d) pRel[d] = (x[d] - x0[d]) - T(col.p[d] - col.p0[d]);
+            // explicit normal force at the estimated stage-start distance
+            const T dss = dist - (grad[0] * pRel[0] + grad[1] * pRel[1] + grad[2] * pRel[2]);
+            T p0, dp0, ddp0;
+            PolyReLU<T>(-dss, a.hPen, a.hPen - a.thrPen, p0, dp0, ddp0);
+            const T fN = fade * a.kPen * p0 * dp0;
+            const T pn = grad[0] * pRel[0] + grad[1] * pRel[1] + grad[2] * pRel[2];
+            T pT[3];
+            for (int d = 0; d < 3; ++d) pT[d] = pRel[d] - pn * grad[d];
+            const T pTn = std::sqrt(pT[0] * pT[0] + pT[1] * pT[1] + pT[2] * pT[2]);
+            T eigP = 0, eigN = 0;
+            if (a.muCoulomb > T(0)) {
+              T f, df, ddf, dfx;
+              IPCStepC1<T>(pTn, a.falloffVel * dt, f, df, ddf, dfx);
+              const T tmp = a.muCoulomb * fN * dfx;
+              for (int d = 0; d < 3; ++d) F[d] -= tmp * pT[d];
+              eigP = -tmp;  // fitted Hessian: the same eigenvalue along the slip direction
+            }
+            if (a.cViscous > T(0)) {
+              const T tmp = (a.cViscous / dt) * fN;
+              for (int d = 0; d < 3; ++d) F[d] -= tmp * pT[d];
+              eigP -= tmp;
+            }
+            if (a.cNormalDamping > T(0)) {
+              const T tmp = (a.cNormalDamping / dt) * fN;
+              for (int d = 0; d < 3; ++d) F[d] -= tmp * pn * grad[d];
+              eigN = -tmp;
+            }
+            cI += eigP;
+            cN += eigN - eigP;
+          }
           for (int i = 0; i < 3; ++i) {
             const T bs = kBary[q][i] * w;
-            for (int d = 0; d < 3; ++d) res[3 * size_t(nd[i]) + d] -= fnorm * grad[d] * bs;
+            for (int d = 0; d < 3; ++d) res[3 * size_t(nd[i]) + d] -= F[d] * bs;
           }
           if (!wantH) continue;
           for (int i = 0; i < 3; ++i)
@@ -821,7 +911,8 @@ struct Scene {
               T* blk = &a.K.val[9 * size_t(a.K.Slot(nd[i], nd[j]))];
               const T bb = kBary[q][i] * w * kBary[q][j];
               for (int r = 0; r < 3; ++r)
-                for (int cc = 0; cc < 3; ++cc) blk[r * 3 + cc] -= dfn * grad[r] * grad[cc] * bb;
+                for (int cc = 0; cc < 3; ++cc)
+                  blk[r * 3 + cc] -= (cN * grad[r] * grad[cc] + (r == cc ? cI : T(0))) * bb;
             }
         }
       }
@@ -968,6 +1059,9 @@ struct Scene {
       }
       if (stats && int(ai) < nStats) stats[ai] = st;
     }
+    // the colliders' targets of this step are the next step's stage start
+    for (Collider& c : colliders)
+      for (int d = 0; d < 3; ++d) c.p0[d] = c.p[d];
   }
 };
 
@@ -1057,6 +1151,7 @@ int sdxo_add_collider(void* sv, int kind, const double* p, const double* nrm, do
     }
     c.r = r;
     c.actor = actor;
+    for (int d = 0; d < 3; ++d) c.p0[d] = c.p[d];
     sc.colliders.push_back(c);
     return int(sc.colliders.size()) - 1;
   });

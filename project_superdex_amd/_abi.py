@@ -136,6 +136,10 @@ class SoftParams(C.Structure):
         ("penalty_smoothing_half_distance", C.c_double),
         ("penalty_threshold", C.c_double),
         ("max_alignment_normals", C.c_double),
+        ("coulomb_friction_coefficient", C.c_double),
+        ("viscous_friction_coefficient", C.c_double),
+        ("friction_falloff_vel", C.c_double),
+        ("normal_viscous_damping_coefficient", C.c_double),
     ]
 
 
@@ -188,8 +192,14 @@ def pinned_solver_params(newton_iters: int = 1, lin_rel_tol: float = 1e-6, lin_m
 
 
 def default_soft_params(E: float = 1e5, nu: float = 0.4, density: float = 1000.0,
-                        penalty: float = 1e6, mass_damping: float = 0.0) -> SoftParams:
-    """Soft material + ContactParams defaults (material_params.h:77-105, contact_params.h:96-160)."""
+                        penalty: float = 1e6, mass_damping: float = 0.0, coulomb: float = 0.0,
+                        viscous: float = 0.0, falloff_vel: float = 0.01,
+                        normal_damping: float = 0.0) -> SoftParams:
+    """Soft material + ContactParams (material_params.h:77-105, contact_params.h:96-225) of the
+    pinned workloads: the reference's smoothing, threshold and alignment defaults, with penalty
+    1e6 and no friction unless asked for (the reference defaults to Coulomb 0.5)."""
     return SoftParams(youngs_modulus=E, poisson_ratio=nu, density=density, mass_damping=mass_damping,
                       penalty_coefficient=penalty, penalty_smoothing_half_distance=0.005,
-                      penalty_threshold=0.001, max_alignment_normals=0.0)
+                      penalty_threshold=0.001, max_alignment_normals=0.0,
+                      coulomb_friction_coefficient=coulomb, viscous_friction_coefficient=viscous,
+                      friction_falloff_vel=falloff_vel, normal_viscous_damping_coefficient=normal_damping)

@@ -85,6 +85,10 @@ void sdx_default_soft_params(sdx_soft_params* s) {
   s->penalty_smoothing_half_distance = 0.005;
   s->penalty_threshold = 0.001;
   s->max_alignment_normals = 0.0;
+  s->coulomb_friction_coefficient = 0.5;
+  s->viscous_friction_coefficient = 0.0;
+  s->friction_falloff_vel = 0.01;
+  s->normal_viscous_damping_coefficient = 0.0;
 }
 
 int32_t sdx_abi_struct_size(int32_t which) {
@@ -131,6 +135,11 @@ sdx_status sdx_add_soft_tet_mesh(sdx_world* world, const double* X, int32_t n_ve
   SDX_REQUIRE(params->density > 0, "density must be positive");
   SDX_REQUIRE(params->penalty_coefficient > 0 && params->penalty_smoothing_half_distance >= 0,
               "invalid contact parameters");
+  SDX_REQUIRE(params->max_alignment_normals >= -1 && params->max_alignment_normals <= 1,
+              "max_alignment_normals must be in [-1, 1]");
+  SDX_REQUIRE(params->coulomb_friction_coefficient >= 0 && params->viscous_friction_coefficient >= 0 &&
+                  params->friction_falloff_vel >= 0 && params->normal_viscous_damping_coefficient >= 0,
+              "friction coefficients and falloff velocity must not be negative");
   for (int64_t k = 0; k < 4 * int64_t(n_tets); ++k)
     SDX_REQUIRE(tets[k] >= 0 && tets[k] < n_verts, "tet node index out of range");
   return guarded([&]() -> sdx_status {
@@ -192,6 +201,7 @@ sdx_status sdx_add_rigid_plane(sdx_world* world, const double p[3], const double
     c.kind = 0;
     for (int d = 0; d < 3; ++d) {
       c.p[d] = p[d];
+      c.p0[d] = p[d];  // stage start of the first step: where it was created
       c.n[d] = n[d] / l;
     }
     c.actor = actor;
@@ -209,7 +219,7 @@ sdx_status sdx_add_rigid_sphere(sdx_world* world, const double c[3], double r, i
     SDX_REQUIRE(actor >= -1 && actor < world->impl->num_actors(), "no such actor");
     sdx::HostCollider s;
     s.kind = 1;
-    for (int d = 0; d < 3; ++d) s.p[d] = c[d];
+    for (int d = 0; d < 3; ++d) s.p[d] = s.p0[d] = c[d];  // stage start of the first step: where it was created
     s.r = r;
     s.actor = actor;
     const int id = world->impl->add_collider(s);

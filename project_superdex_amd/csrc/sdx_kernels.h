@@ -109,7 +109,7 @@ struct MeshView {
   const int32_t* vtx_ptr;   // [nV+1] records (element and contact) of each vertex
   const uint8_t* pinned;    // [3 nV]
   const int32_t* coll_first; // [nEnv+1] into coll
-  const double* coll;        // 8 doubles per collider: kind, p(3), n(3), r
+  const double* coll;        // kCollStride doubles per collider: kind, p(3), n(3), r, p0(3), 0
   const int32_t* env_mask;   // [nEnv] 1: assemble/solve this environment
 };
 
@@ -117,11 +117,19 @@ struct ContactParamsDev {
   double k, h, thr, max_align;
 };
 
+// Doubles per collider: kind, position p (the step's target), plane normal n,
+// sphere radius r, stage-start position p0, padding.
+constexpr int kCollStride = 12;
+// Contact parameters per environment: penalty k, smoothing half-width h,
+// threshold, max normal alignment, Coulomb mu, viscous friction, friction
+// falloff velocity, normal damping.
+constexpr int kContactParams = 8;
+
 template <class T>
 struct FieldPtrs {
   const T* X;   // rest positions [3nV]
   const T* geom;  // [kGeomPlanes * nT]
-  const double* contact;  // [4 nEnv] k, h, thr, maxAlign
+  const double* contact;  // [kContactParams nEnv]
 };
 
 // ---- assembly ----
@@ -155,9 +163,10 @@ template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
                     const T* u0, const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv,
                     T* Ev, int block = 128, const KernelEvents* ev = nullptr);
+// u0: the stage-start displacement (relative sliding of the samples).
 template <class T>
 void launch_contact(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
-                    int wantH, T* Rv, T* Dv, T* Ev);
+                    const T* u0, T dt, int wantH, T* Rv, T* Dv, T* Ev);
 // BSR values (and the preconditioner blocks) from the records, over the work
 // items of the range (diagonal and upper slots).
 template <class T>

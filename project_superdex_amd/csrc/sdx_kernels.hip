@@ -142,13 +142,21 @@ __device__ __forceinline__ int sym_entry(int r, int c) { return r == c ? r : 2 +
 
 // Penalty contact of the 3 trace-Q3 samples of a boundary triangle against
 // the analytic colliders of its environment (contact_utils.h:498-570,
-// fem_traction.h:58-170).  Outputs per triangle, written to their records
-// (zeros when nothing touches): 3 residual rows, 3 diagonal blocks and the 3
-// blocks of its vertex pairs (symmetric, so no transposition).
+// fem_traction.h:58-170) and its dissipation (contact_utils.h:630-935):
+// C1-regularised Coulomb friction (activations.h:131-160), viscous friction
+// and normal damping, each proportional to the penalty force at the sample's
+// estimated stage-start distance d - n.pRel and faded by the alignment of the
+// collider and face normals, with pRel = (x - x0) - (P - P0) the sample's
+// displacement over the step relative to the collider, and the fitted
+// friction Hessian (the tangent-plane eigenvalue -mu fN df/|pT| in every
+// tangent direction), so every sample's Hessian is c_I I + c_N n n^T.
+// Outputs per triangle, written to their records (zeros when nothing
+// touches): 3 residual rows, 3 diagonal blocks and the 3 blocks of its vertex
+// pairs (symmetric, so no transposition).
 template <class T>
 __global__ void __launch_bounds__(256) k_contact(MeshView m, AsmRange ar, FieldPtrs<T> f, const T* __restrict__ u,
-                                                 int wantH, T* __restrict__ Rv, T* __restrict__ Dv,
-                                                 T* __restrict__ Ev) {
+                                                 const T* __restrict__ u0, T dt, int wantH, T* __restrict__ Rv,
+                                                 T* __restrict__ Dv, T* __restrict__ Ev) {
   const int tr = ar.f0 + int(blockIdx.x * blockDim.x + threadIdx.x);
   if (tr >= ar.f1) return;
   const int env = m.tri_env[tr];
@@ -164,24 +172,29 @@ __global__ void __launch_bounds__(256) k_contact(MeshView m, AsmRange ar, FieldP
   const int c0 = m.coll_first[env], c1 = m.coll_first[env + 1];
   if (c1 > c0) {
     const int nd[3] = {m.tris[3 * tr], m.tris[3 * tr + 1], m.tris[3 * tr + 2]};
-    V3<T> xs[3], Xr[3];
+    V3<T> xs[3], Xr[3], dxs[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       Xr[i] = mk3(f.X[3 * nd[i]], f.X[3 * nd[i] + 1], f.X[3 * nd[i] + 2]);
-      xs[i] = add3(Xr[i], mk3(u[3 * nd[i]], u[3 * nd[i] + 1], u[3 * nd[i] + 2]));
+      const V3<T> ui = mk3(u[3 * nd[i]], u[3 * nd[i] + 1], u[3 * nd[i] + 2]);
+      xs[i] = add3(Xr[i], ui);
+      dxs[i] = sub3(ui, mk3(u0[3 * nd[i]], u0[3 * nd[i] + 1], u0[3 * nd[i] + 2]));  // x - x0
     }
     // rest area -> quad weight A0/3 ; current outward normal for the alignment mask
     const V3<T> n0 = cross3(sub3(Xr[1], Xr[0]), sub3(Xr[2], Xr[0]));
     const T w = T(0.5) * dsqrt(dot3(n0, n0)) / T(3);
     V3<T> fnv = cross3(sub3(xs[1], xs[0]), sub3(xs[2], xs[0]));
     fnv = scl3(T(1) / dsqrt(dot3(fnv, fnv)), fnv);
-    const double* cp = f.contact + 4 * env;
+    const double* cp = f.contact + kContactParams * env;
     const T kp = T(cp[0]), hp = T(cp[1]), thr = T(cp[2]), maxAlign = T(cp[3]);
+    const T mu = T(cp[4]), cvd = T(cp[5]) / dt, falloff = T(cp[6]) * dt, cnd = T(cp[7]) / dt;
+    const bool dissip = mu > T(0) || cvd > T(0) || cnd > T(0);
     const T kSix = T(1.0 / 6.0), kTwoThirds = T(2.0 / 3.0);
     for (int c = c0; c < c1; ++c) {
-      const double* cd = m.coll + 8 * c;
+      const double* cd = m.coll + kCollStride * c;
       const int kind = (int)cd[0];
       const V3<T> P = mk3(T(cd[1]), T(cd[2]), T(cd[3]));
+      const V3<T> dP = mk3(T(cd[1] - cd[8]), T(cd[2] - cd[9]), T(cd[3] - cd[10]));  // P - P0
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         T N[3];
@@ -201,24 +214,61 @@ __global__ void __launch_bounds__(256) k_contact(MeshView m, AsmRange ar, FieldP
           dist = l - T(cd[7]);
         }
         if (!(dist <= thr)) continue;
-        if (!(dot3(gr, fnv) <= maxAlign)) continue;
+        const T align = dot3(gr, fnv);
+        if (!(align <= maxAlign)) continue;
         touched = true;
         T pen, dpen, ddpen;
         poly_relu<T>(-dist, hp, hp - thr, pen, dpen, ddpen);
         const T fn = kp * pen * dpen;
-        const T dfn = -kp * (dpen * dpen + pen * ddpen);
-        const T gv[3] = {gr.x, gr.y, gr.z};
+        // force on the sample F, its derivative cI I + cN n n^T
+        V3<T> F = scl3(fn, gr);
+        T cI = T(0), cN = -kp * (dpen * dpen + pen * ddpen);
+        if (dissip) {
+          const T fade = maxAlign > T(-1) ? (maxAlign - align) / (maxAlign + T(1)) : T(1);
+          const V3<T> pRel =
+              sub3(add3(add3(scl3(N[0], dxs[0]), scl3(N[1], dxs[1])), scl3(N[2], dxs[2])), dP);
+          const T pn = dot3(gr, pRel);
+          T p0, dp0, ddp0;
+          poly_relu<T>(-(dist - pn), hp, hp - thr, p0, dp0, ddp0);
+          const T fN = fade * kp * p0 * dp0;
+          const V3<T> pT = sub3(pRel, scl3(pn, gr));
+          const T pTn = dsqrt(dot3(pT, pT));
+          T eigP = T(0), eigN = T(0);
+          if (mu > T(0)) {
+            // IPCstepC1(|pT|, falloff): df / |pT|
+            const T dfx = pTn >= falloff ? (pTn == T(0) ? T(0) : T(1) / pTn) : (T(2) - pTn / falloff) / falloff;
+            const T tmp = mu * fN * dfx;
+            F = sub3(F, scl3(tmp, pT));
+            eigP = -tmp;
+          }
+          if (cvd > T(0)) {
+            const T tmp = cvd * fN;
+            F = sub3(F, scl3(tmp, pT));
+            eigP -= tmp;
+          }
+          if (cnd > T(0)) {
+            const T tmp = cnd * fN;
+            F = sub3(F, scl3(tmp * pn, gr));
+            eigN = -tmp;
+          }
+          cI = eigP;
+          cN += eigN - eigP;
+        }
+        const T fv[3] = {F.x, F.y, F.z};
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int d = 0; d < 3; ++d) rc[3 * i + d] -= fn * gv[d] * (N[i] * w);
+          for (int d = 0; d < 3; ++d) rc[3 * i + d] -= fv[d] * (N[i] * w);
         if (wantH) {
-          const T gg[6] = {gv[0] * gv[0], gv[1] * gv[1], gv[2] * gv[2], gv[0] * gv[1], gv[0] * gv[2], gv[1] * gv[2]};
+          const T gv[3] = {gr.x, gr.y, gr.z};
+          // -(cN n n^T + cI I) in 00 11 22 01 02 12 order
+          const T gg[6] = {-(cN * gv[0] * gv[0] + cI), -(cN * gv[1] * gv[1] + cI), -(cN * gv[2] * gv[2] + cI),
+                           -cN * gv[0] * gv[1], -cN * gv[0] * gv[2], -cN * gv[1] * gv[2]};
 #pragma unroll
           for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = a; b < 3; ++b) {
-              const T s = -dfn * N[a] * N[b] * w;
+              const T s = N[a] * N[b] * w;
               T* blk = hc + 6 * hc_block(a, b);
 #pragma unroll
               for (int k = 0; k < 6; ++k) blk[k] += s * gg[k];
@@ -1478,10 +1528,10 @@ void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const F
 
 template <class T>
 void launch_contact(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
-                    int wantH, T* Rv, T* Dv, T* Ev) {
+                    const T* u0, T dt, int wantH, T* Rv, T* Dv, T* Ev) {
   if (r.f1 <= r.f0) return;
-  hipLaunchKernelGGL(k_contact<T>, dim3(blocks_for(r.f1 - r.f0, 256)), dim3(256), 0, s, m, r, f, u, wantH, Rv, Dv,
-                     Ev);
+  hipLaunchKernelGGL(k_contact<T>, dim3(blocks_for(r.f1 - r.f0, 256)), dim3(256), 0, s, m, r, f, u, u0, dt, wantH,
+                     Rv, Dv, Ev);
 }
 
 template <class T>
@@ -1724,7 +1774,7 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
                                   const T*, const T*, double, const double*, int, int, T*, T*, T*, int,         \
                                   const KernelEvents*);                                                         \
   template void launch_contact<T>(hipStream_t, const MeshView&, const AsmRange&, const FieldPtrs<T>&, const T*, \
-                                  int, T*, T*, T*);                                                             \
+                                  const T*, T, int, T*, T*, T*);                                                \
   template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const AsmRange&, const T*, const T*, int, T*, \
                                      T*, const KernelEvents*);                                                  \
   template void launch_block_inverse<T>(hipStream_t, int32_t, const int32_t*, const T*, int32_t, int, T*);     \

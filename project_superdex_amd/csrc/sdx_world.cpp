@@ -223,7 +223,7 @@ void World<T>::finalize() {
   std::vector<uint32_t> edge_codes;
   std::vector<int32_t> vtx_keys;
   std::vector<uint32_t> vtx_codes;
-  std::vector<double> contact(size_t(4) * size_t(nEnv));
+  std::vector<double> contact(size_t(kContactParams) * size_t(nEnv));
   for (int e = 0; e < nEnv; ++e) {
     auto& A = actors_[size_t(e)];
     const int v0 = env_v0_[size_t(e)], t0 = env_t0[size_t(e)];
@@ -309,10 +309,15 @@ void World<T>::finalize() {
         edge_codes.push_back((uint32_t(gf) * 3u + uint32_t(k)) | 0x40000000u);
       }
     }
-    contact[size_t(4 * e + 0)] = A.params.penalty_coefficient;
-    contact[size_t(4 * e + 1)] = A.params.penalty_smoothing_half_distance;
-    contact[size_t(4 * e + 2)] = A.params.penalty_threshold;
-    contact[size_t(4 * e + 3)] = A.params.max_alignment_normals;
+    double* cp = &contact[size_t(kContactParams) * size_t(e)];
+    cp[0] = A.params.penalty_coefficient;
+    cp[1] = A.params.penalty_smoothing_half_distance;
+    cp[2] = A.params.penalty_threshold;
+    cp[3] = A.params.max_alignment_normals;
+    cp[4] = A.params.coulomb_friction_coefficient;
+    cp[5] = A.params.viscous_friction_coefficient;
+    cp[6] = A.params.friction_falloff_vel;
+    cp[7] = A.params.normal_viscous_damping_coefficient;
   }
   nB_ = int(col.size());
   nF_ = int(tri_env.size());
@@ -557,16 +562,18 @@ void World<T>::upload_colliders() {
   }
   std::vector<int32_t> first(size_t(nEnv_) + 1, 0);
   std::vector<double> cd;
-  cd.reserve(size_t(8) * (size_t(own_first[size_t(nEnv_)]) + shared.size() * size_t(nEnv_)));
+  cd.reserve(size_t(kCollStride) * (size_t(own_first[size_t(nEnv_)]) + shared.size() * size_t(nEnv_)));
   auto emit = [&](int i) {
     const HostCollider& c = colliders_[size_t(i)];
     cd.push_back(double(c.kind));
     cd.insert(cd.end(), c.p, c.p + 3);
     cd.insert(cd.end(), c.n, c.n + 3);
     cd.push_back(c.r);
+    cd.insert(cd.end(), c.p0, c.p0 + 3);
+    cd.push_back(0.0);
   };
   for (int e = 0; e < nEnv_; ++e) {
-    first[size_t(e)] = int32_t(cd.size() / 8);
+    first[size_t(e)] = int32_t(cd.size() / kCollStride);
     size_t a = 0, b = size_t(own_first[size_t(e)]);
     const size_t b1 = size_t(own_first[size_t(e) + 1]);
     while (a < shared.size() || b < b1) {
@@ -576,8 +583,8 @@ void World<T>::upload_colliders() {
         emit(own[b++]);
     }
   }
-  first[size_t(nEnv_)] = int32_t(cd.size() / 8);
-  if (cd.empty()) cd.assign(8, 0.0);
+  first[size_t(nEnv_)] = int32_t(cd.size() / kCollStride);
+  if (cd.empty()) cd.assign(kCollStride, 0.0);
   if (d_coll_.n < cd.size()) d_coll_.alloc(cd.size());
   put(d_coll_, cd);
   if (d_coll_first_.n < first.size()) d_coll_first_.alloc(first.size());
@@ -616,7 +623,7 @@ void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH) {
     T* Ev = group_records(coff_.p, 9, g.re0);
     launch_element<T>(stream_, mesh_, r, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, Rv, Dv,
                       Ev, sched_.element_block, sampled_events(2));
-    launch_contact<T>(stream_, mesh_, r, fields_, u_.p, wantH ? 1 : 0, Rv, Dv, Ev);
+    launch_contact<T>(stream_, mesh_, r, fields_, u_.p, u0_.p, T(dt), wantH ? 1 : 0, Rv, Dv, Ev);
     if (wantH)
       launch_gather_bsr<T>(stream_, mesh_, r, Dv, Ev, sp.linear.preconditioner, vals_.p, dinv_.p, sampled_events(3));
     if (dd)
@@ -1418,6 +1425,12 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
   put(reset_, reset);
   launch_post_step<T>(stream_, nV_, d_row_env_.p, reset_.p, dt, u0_.p, u_.p, v_.p, hist_.p, 2 * kNewtonHist);
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  // the colliders' targets of this step are the next step's stage start
+  for (HostCollider& c : colliders_)
+    if (c.p0[0] != c.p[0] || c.p0[1] != c.p[1] || c.p0[2] != c.p[2]) {
+      for (int d = 0; d < 3; ++d) c.p0[d] = c.p[d];
+      colliders_dirty_ = true;
+    }
   flush_timers();
   if (stats) {
     for (int e = 0; e < std::min(nE, n_stats); ++e) {
@@ -1555,7 +1568,10 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
   int ws_skip0[kNewtonHist], ws_seen0[kNewtonHist];
   std::memcpy(ws_skip0, ws_skip_, sizeof(ws_skip_));
   std::memcpy(ws_seen0, ws_seen_, sizeof(ws_seen_));
+  const std::vector<HostCollider> colliders0 = colliders_;  // a trial step advances their stage start
   auto restore = [&]() {
+    colliders_ = colliders0;
+    colliders_dirty_ = true;
     std::memcpy(ws_skip_, ws_skip0, sizeof(ws_skip_));
     std::memcpy(ws_seen_, ws_seen0, sizeof(ws_seen_));
     launch_copy<T>(stream_, n3, su.p, u_.p);
@@ -1676,7 +1692,8 @@ namespace {
 // Header of the byte form of a state: the world it was taken from (real
 // size, precision, vertex, actor and collider counts, a hash of the meshes,
 // materials, pins and colliders) and the warm-start counters, followed by the collider targets
-// (double), then u, v and the Newton-step history (the world's real type).
+// and stage-start positions (double), then u, v and the Newton-step history (the world's real
+// type).
 struct StateHeader {
   char magic[8];
   uint32_t version;
@@ -1691,7 +1708,7 @@ struct StateHeader {
   int32_t ws_seen[4];
 };
 constexpr char kStateMagic[8] = {'S', 'D', 'X', 'S', 'T', 'A', 'T', 0};
-constexpr uint32_t kStateVersion = 2;
+constexpr uint32_t kStateVersion = 3;
 }  // namespace
 
 template <class T>
@@ -1712,9 +1729,12 @@ std::unique_ptr<typename World<T>::Snapshot> World<T>::take_snapshot() {
   launch_copy<T>(stream_, n3, u_.p, s->u.p);
   launch_copy<T>(stream_, n3, v_.p, s->v.p);
   launch_copy<T>(stream_, nh, hist_.p, s->hist.p);
-  s->coll_p.resize(3 * colliders_.size());
+  s->coll_p.resize(6 * colliders_.size());
   for (size_t c = 0; c < colliders_.size(); ++c)
-    for (int d = 0; d < 3; ++d) s->coll_p[3 * c + size_t(d)] = colliders_[c].p[d];
+    for (int d = 0; d < 3; ++d) {
+      s->coll_p[6 * c + size_t(d)] = colliders_[c].p[d];
+      s->coll_p[6 * c + 3 + size_t(d)] = colliders_[c].p0[d];
+    }
   std::memcpy(s->ws_skip, ws_skip_, sizeof(ws_skip_));
   std::memcpy(s->ws_seen, ws_seen_, sizeof(ws_seen_));
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
@@ -1725,13 +1745,16 @@ template <class T>
 void World<T>::put_snapshot(const Snapshot& s) {
   finalize();
   const int64_t n3 = int64_t(3) * nV_, nh = int64_t(2 * kNewtonHist) * n3;
-  if (s.u.n != size_t(n3) || s.hist.n != size_t(nh) || s.coll_p.size() != 3 * colliders_.size())
+  if (s.u.n != size_t(n3) || s.hist.n != size_t(nh) || s.coll_p.size() != 6 * colliders_.size())
     throw std::runtime_error("state was captured from a world of another size");
   launch_copy<T>(stream_, n3, s.u.p, u_.p);
   launch_copy<T>(stream_, n3, s.v.p, v_.p);
   launch_copy<T>(stream_, nh, s.hist.p, hist_.p);
   for (size_t c = 0; c < colliders_.size(); ++c)
-    for (int d = 0; d < 3; ++d) colliders_[c].p[d] = s.coll_p[3 * c + size_t(d)];
+    for (int d = 0; d < 3; ++d) {
+      colliders_[c].p[d] = s.coll_p[6 * c + size_t(d)];
+      colliders_[c].p0[d] = s.coll_p[6 * c + 3 + size_t(d)];
+    }
   colliders_dirty_ = true;
   std::memcpy(ws_skip_, s.ws_skip, sizeof(ws_skip_));
   std::memcpy(ws_seen_, s.ws_seen, sizeof(ws_seen_));
@@ -1787,7 +1810,7 @@ template <class T>
 size_t World<T>::state_size() {
   finalize();
   const size_t n3 = size_t(3) * size_t(nV_);
-  return sizeof(StateHeader) + 3 * colliders_.size() * sizeof(double) + (2 + 2 * kNewtonHist) * n3 * sizeof(T);
+  return sizeof(StateHeader) + 6 * colliders_.size() * sizeof(double) + (2 + 2 * kNewtonHist) * n3 * sizeof(T);
 }
 
 namespace {
@@ -1865,7 +1888,8 @@ void World<T>::capture_bytes(uint8_t* out, size_t cap) {
   o += sizeof(hd);
   for (const HostCollider& c : colliders_) {
     std::memcpy(o, c.p, 3 * sizeof(double));
-    o += 3 * sizeof(double);
+    std::memcpy(o + 3 * sizeof(double), c.p0, 3 * sizeof(double));
+    o += 6 * sizeof(double);
   }
   const size_t n3 = size_t(3) * size_t(nV_);
   std::vector<T> h(n3);
@@ -1896,7 +1920,8 @@ void World<T>::restore_bytes(const uint8_t* in, size_t n) {
   const uint8_t* q = in + sizeof(hd);
   for (HostCollider& c : colliders_) {
     std::memcpy(c.p, q, 3 * sizeof(double));
-    q += 3 * sizeof(double);
+    std::memcpy(c.p0, q + 3 * sizeof(double), 3 * sizeof(double));
+    q += 6 * sizeof(double);
   }
   colliders_dirty_ = true;
   const size_t n3 = size_t(3) * size_t(nV_);

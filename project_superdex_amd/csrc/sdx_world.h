@@ -110,6 +110,7 @@ struct HostCollider {
   int kind = 0;  // 0 plane, 1 sphere
   double p[3] = {0, 0, 0}, n[3] = {0, 1, 0}, r = 0;
   int actor = -1;
+  double p0[3] = {0, 0, 0};  // position at stage start: the previous step's target
 };
 
 struct HostTiles {
@@ -604,7 +605,7 @@ class World final : public WorldBase {
   struct Snapshot {
     std::vector<DBufBase*> reg;
     DBuf<T> u, v, hist;
-    std::vector<double> coll_p;  // collider targets, 3 per collider
+    std::vector<double> coll_p;  // collider targets and stage-start positions, 6 per collider
     int ws_skip[kNewtonHist], ws_seen[kNewtonHist];
     explicit Snapshot(hipStream_t* s) : u(reg, s), v(reg, s), hist(reg, s) {}
   };

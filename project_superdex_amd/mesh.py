@@ -135,6 +135,17 @@ class SoftBody:
     pinned: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
     penalty: float = 1e6
     mass_damping: float = 0.0
+    coulomb: float = 0.0
+    viscous: float = 0.0
+    falloff_vel: float = 0.01
+    normal_damping: float = 0.0
+
+    def soft_params(self, abi):
+        """The sdx_soft_params of this body (homogeneous part; per-element fields apart)."""
+        E0 = float(np.mean(self.E)) if np.ndim(self.E) else float(self.E)
+        nu0 = float(np.mean(self.nu)) if np.ndim(self.nu) else float(self.nu)
+        return abi.default_soft_params(E0, nu0, self.density, self.penalty, self.mass_damping, self.coulomb,
+                                       self.viscous, self.falloff_vel, self.normal_damping)
 
 
 @dataclass

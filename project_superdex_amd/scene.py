@@ -296,10 +296,7 @@ def build_scene(spec, device: int = 0, precision: int = _abi.PREC_MODE_FP32):
     """Scene of a mesh.SceneSpec; returns (scene, rigid actors in spec order)."""
     sc = Scene(device, precision)
     for b in spec.bodies:
-        E0 = float(np.mean(b.E)) if np.ndim(b.E) else float(b.E)
-        nu0 = float(np.mean(b.nu)) if np.ndim(b.nu) else float(b.nu)
-        a = sc.create_soft_actor(b.X, b.tets, _abi.default_soft_params(E0, nu0, b.density, b.penalty,
-                                                                      b.mass_damping))
+        a = sc.create_soft_actor(b.X, b.tets, b.soft_params(_abi))
         if np.ndim(b.E):
             a.set_element_materials(b.E, b.nu)
         if len(b.pinned):

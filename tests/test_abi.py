@@ -63,6 +63,9 @@ def test_defaults_are_the_reference_defaults(L):
     assert s.youngs_modulus == 1e5 and s.poisson_ratio == 0.45 and s.density == 1000.0
     assert s.penalty_coefficient == 1e9 and s.penalty_smoothing_half_distance == 0.005
     assert s.penalty_threshold == 0.001 and s.max_alignment_normals == 0.0
+    # contact_params.h:170-225: Coulomb friction on by default
+    assert s.coulomb_friction_coefficient == 0.5 and s.viscous_friction_coefficient == 0.0
+    assert s.friction_falloff_vel == 0.01 and s.normal_viscous_damping_coefficient == 0.0
 
 
 def test_invalid_arguments_are_rejected_before_any_device_call(L):

@@ -574,3 +574,87 @@ def test_assembly_groups_match_oracle(oracle, asm_groups):
         np.testing.assert_array_equal(cg, co)
         np.testing.assert_allclose(rg, ro, rtol=2e-4, atol=2e-4 * np.abs(ro).max())
         np.testing.assert_allclose(vg, vo, rtol=2e-4, atol=2e-4 * np.abs(vo).max())
+
+
+def friction_world(gpu, precision=None, amp=4e-4, seed=0, dissipation=True):
+    """A 3^3-hex cube (bottom pinned) pressed by a sphere that has moved sideways since its
+    creation (its stage start), displaced with a seeded tangential slip: Coulomb friction above
+    and below the falloff, viscous friction and normal damping all active."""
+    X, tets = mesh.hex_grid_tets(3, 0.05 / 3)
+    sp = _abi.default_soft_params(E=1e5, nu=0.4, coulomb=0.5, viscous=0.2, falloff_vel=0.05,
+                                  normal_damping=0.1)
+    if not dissipation:
+        sp.coulomb_friction_coefficient = sp.viscous_friction_coefficient = 0.0
+        sp.normal_viscous_damping_coefficient = 0.0
+    pins = mesh.nodes_on_plane(X, 1, 0.0)
+    c0, c1 = (0.02, 0.05 + 0.02 - 0.002, 0.03), (0.0203, 0.05 + 0.02 - 0.0022, 0.0299)
+    if gpu:
+        sc = gs.Scene(precision=precision)
+        a = sc.create_soft_actor(X, tets, sp)
+        a.add_boundary_condition_nodes(pins)
+        r = sc.create_rigid_sphere(c0, 0.02)
+        r.set_position(c1)
+    else:
+        from oracle import oracle_api as O
+        sc = O.OracleScene(fp64=precision == _abi.PREC_MODE_FP64)
+        a = sc.add_soft_tet_mesh(X, tets, sp)
+        sc.pin_nodes(a, pins)
+        cid = sc.add_sphere(c0, 0.02)
+        sc.set_collider_position(cid, c1)
+    rng = np.random.default_rng(seed)
+    u = amp * rng.standard_normal((X.shape[0], 3))
+    u[:, 0] += 3e-4  # a common slip along x
+    u[pins] = 0
+    return sc, u
+
+
+@pytest.mark.parametrize("mode,tol,tol_part", [(_abi.PREC_MODE_FP64, 1e-9, 1e-7), (_abi.PREC_MODE_FP32, 2e-4, 3e-3)])
+def test_friction_assembly_matches_oracle(oracle, mode, tol, tol_part):
+    """Contact dissipation (contact_utils.h:630-935) in the contact kernel: the assembled residual
+    and BSR values with Coulomb and viscous friction and normal damping, the sphere moved since
+    its stage start, equal the oracle's; so does the dissipation part alone (the assembly minus
+    the same assembly with the coefficients zero), to tol_part of its largest entry (in FP32 it
+    is a difference of two assemblies of about 1e3 times its size)."""
+    p = _abi.pinned_solver_params()
+    out = {}
+    for dis in (True, False):
+        g, u = friction_world(True, precision=mode, amp=1e-4, dissipation=dis)
+        o, _ = friction_world(False, precision=mode, amp=1e-4, dissipation=dis)
+        g.actors[0].set_displacements(u)
+        o.set_displacements(0, u)
+        out[dis] = (g.actors[0].assemble(1e-3, p), o.assemble(0, 1e-3, p))
+    (rg, pg, cg, vg), (ro, po, co, vo) = out[True]
+    np.testing.assert_array_equal(cg, co)
+    np.testing.assert_allclose(rg, ro, rtol=tol, atol=tol * np.abs(ro).max())
+    np.testing.assert_allclose(vg, vo, rtol=tol, atol=tol * np.abs(vo).max())
+    (rg0, _, _, vg0), (ro0, _, _, vo0) = out[False]
+    dro, dvo = ro - ro0, vo - vo0
+    assert np.abs(dro).max() > 1e-5 * np.abs(ro).max()  # the dissipation is there
+    np.testing.assert_allclose(rg - rg0, dro, rtol=tol_part, atol=tol_part * np.abs(dro).max())
+    np.testing.assert_allclose(vg - vg0, dvo, rtol=tol_part, atol=tol_part * np.abs(dvo).max())
+
+
+@pytest.mark.parametrize("mode", [_abi.PREC_MODE_FP32, _abi.PREC_MODE_MIXED])
+def test_friction_steps_match_oracle(oracle, mode):
+    """Batched cubes under 1-3 sphere indenters moving with tangential velocities (penalty 1e7),
+    Coulomb 1.0, viscous friction and normal damping, 2 Newton iterations: the GPU steps match the
+    oracle's (the colliders' stage start advances with every step on both sides), and friction
+    changes the result (the same steps without it differ by over 5% of the largest
+    displacement)."""
+    spec = mesh.batched_indent_scene(num_envs=4, seed=7, n_min=3, n_max=5, steps=8, speed=0.5,
+                                     indenters=(1, 3), newton_iters=2)
+    for b in spec.bodies:
+        b.penalty = 1e7
+        b.coulomb, b.viscous, b.falloff_vel, b.normal_damping = 1.0, 0.1, 0.01, 0.05
+    p = _abi.pinned_solver_params(2, precision_mode=mode, refresh_every=10)
+    sg, stg = run_gpu(spec, p, 8, precision=mode)
+    so, _ = oracle.run_scene(spec, _abi.pinned_solver_params(2), 8, fp64=False)
+    compare(spec, sg, so, 5e-3)
+    for st in stg:
+        for e in st:
+            assert e["linear_convergence"] == _abi.CONV_CONVERGED and e["convergence"] != _abi.CONV_DIVERGED
+    for b in spec.bodies:
+        b.coulomb, b.viscous, b.normal_damping = 0.0, 0.0, 0.0
+    s0, _ = run_gpu(spec, p, 8, precision=mode)
+    diff = max(np.abs(a.displacements() - b.displacements()).max() for a, b in zip(sg.actors, s0.actors))
+    assert diff > 5e-2 * max(np.abs(a.displacements()).max() for a in sg.actors)
