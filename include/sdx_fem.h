@@ -63,16 +63,39 @@ enum {
   SDX_PREC_BLOCK_JACOBI = 2,
 };
 
-/* LineSearchType subset (nonlinear_solver_params.h:137-194): None = 0, ResidualNorm = 5. */
+/* LineSearchType (nonlinear_solver_params.h:137-194; line_search.h).  The merit of
+ * Simple, Armijo, WolfeWeak, WolfeStrong and ArmijoOrResidualNorm is the objective of the
+ * backward-Euler step (stress, gravity, inertia, mass damping and contact energies). */
 enum {
   SDX_LS_NONE = 0,
+  SDX_LS_SIMPLE = 1,
+  SDX_LS_ARMIJO = 2,
+  SDX_LS_WOLFE_WEAK = 3,
+  SDX_LS_WOLFE_STRONG = 4,
   SDX_LS_RESIDUAL_NORM = 5,
+  SDX_LS_ARMIJO_OR_RESIDUAL_NORM = 6,
 };
 
-/* PsdProjectionMode (nonlinear_solver_params.h:69-75); IfFail* are not implemented. */
+/* PsdProjectionMode (nonlinear_solver_params.h:69-75).  A Newton iteration whose linear
+ * solve does not converge or whose line search finds no improvement is retried once with
+ * the projected DResidual in every mode (newton_solver.cpp:736-860); IfFailAlways then
+ * keeps projecting for the rest of the solve. */
 enum {
   SDX_PSD_NEVER = 0,
   SDX_PSD_ALWAYS = 1,
+  SDX_PSD_IF_FAIL_RETRY = 2,
+  SDX_PSD_IF_FAIL_ALWAYS = 3,
+};
+
+/* LinearToleranceStrategy (nonlinear_solver_params.h:196-255): the relative tolerance of
+ * each Newton iteration's linear solve (newton_solver.cpp:426-501).  Constant keeps
+ * linear.rel_tol; Eisenstat-Walker choices 1 and 2 start at 0.01 and stay in
+ * [100 eps, 0.01]; choice 3 is max(min(1/(2 + k), sqrt(|r|)), linear.rel_tol). */
+enum {
+  SDX_LINTOL_CONSTANT = 0,
+  SDX_LINTOL_EW1 = 1,
+  SDX_LINTOL_EW2 = 2,
+  SDX_LINTOL_EW3 = 3,
 };
 
 /* Arithmetic of the CG (extension: FP32 matvec with FP64 dots + refresh). */
@@ -134,8 +157,13 @@ typedef struct sdx_solver_params {
   double abs_div_tol;             /* default 1e9 */
   double rel_div_tol;             /* default 1e4 */
   int32_t line_search_max_iter;   /* default 4 */
-  int32_t linear_tolerance_constant; /* 1: Constant (the only strategy implemented) */
+  int32_t linear_tolerance_strategy; /* SDX_LINTOL_* (default EisenstatWalker2) */
   double line_search_alpha;       /* default 0.5 */
+  double line_search_wolfe1;      /* default 1e-4: Armijo and Wolfe sufficient decrease */
+  double line_search_wolfe2;      /* default 0.9: Wolfe curvature, in (wolfe1, 1) */
+  double line_search_max_rel_increase; /* default 0: Simple accepts merit <= m0 + |m0| x this */
+  int32_t gradient_descent_fallback; /* default 0: after a failed last try, step along -residual */
+  int32_t reserved0;
   /* LinearSolverParams */
   sdx_krylov_params linear;
   /* scene */
@@ -323,6 +351,13 @@ sdx_status sdx_actor_bsr_size(sdx_world* world, int32_t actor, int32_t* n_block_
 sdx_status sdx_assemble(sdx_world* world, int32_t actor, double dt, const sdx_solver_params* params,
                         double* out_residual, int32_t* out_row_ptr, int32_t* out_col_idx,
                         double* out_values);
+/* Objective (merit) of one actor at its current displacement with the same stage start as
+ * sdx_assemble (SnleProblem::UpdateObjective, snle_problem.cpp; the energies of
+ * fem_stress.h, fem_gravity.h:93-111, fem_inertia.h:85-130 and contact_utils.h:560, :935):
+ * the function whose gradient sdx_assemble's residual is, as the objective-based line
+ * searches evaluate it. */
+sdx_status sdx_objective(sdx_world* world, int32_t actor, double dt, const sdx_solver_params* params,
+                         double* out_objective);
 
 /* Timing of one kernel class inside sdx_step, measured with HIP events
  * recorded at the kernel's own dispatch on the world's stream.  kernel: 0 =

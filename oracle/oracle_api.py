@@ -53,6 +53,13 @@ def lib():
         L.sdxo_assemble.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(_abi.SolverParams),
                                     _dp, _dp, _dp, _ip, _ip, _dp]
         L.sdxo_boundary.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+        L.sdxo_objective.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(_abi.SolverParams),
+                                     _dp, _dp, _dp]
+        L.sdxo_newton_synthetic.argtypes = [C.c_int, C.c_int, _dp, C.POINTER(_abi.SolverParams), _dp,
+                                            C.POINTER(_abi.StepStats), _dp]
+        L.sdxo_forcing_term.restype = C.c_double
+        L.sdxo_forcing_term.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double]
         L.sdxo_smith_nh.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, C.c_int]
         L.sdxo_polyrelu.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
         L.sdxo_stress_element.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp]
@@ -98,6 +105,23 @@ def stress_element(X, u, mu, lam, project=True):
     lib().sdxo_stress_element(_d(X), _d(u), mu, lam, int(project), r.ctypes.data_as(_dp),
                               H.ctypes.data_as(_dp))
     return r, H.reshape(12, 12)
+
+
+def newton_synthetic(kind: int, x0, params: _abi.SolverParams):
+    """The Newton driver on a synthetic problem of the reference's Newton tests:
+    kind 0 the scalar problem, 1 the tridiagonal Broyden problem; returns (x, stats, merit)."""
+    x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)
+    x = np.zeros_like(x0)
+    st = _abi.StepStats()
+    m = C.c_double()
+    rc = lib().sdxo_newton_synthetic(kind, x0.size, _d(x0), C.byref(params), x.ctypes.data_as(_dp), C.byref(st),
+                                     C.byref(m))
+    assert rc == 0, rc
+    return x, st, m.value
+
+
+def forcing_term(strategy, k, res_norm, prev_res_norm, lin_res_norm, prev_eta, rel_tol):
+    return lib().sdxo_forcing_term(strategy, k, res_norm, prev_res_norm, lin_res_norm, prev_eta, rel_tol)
 
 
 def pcg_csr(indptr, indices, data, b, x0=None, bs=3, params=None, fp64=True):
@@ -195,6 +219,12 @@ class OracleScene:
                             res.ctypes.data_as(_dp), ptr.ctypes.data_as(_ip), col.ctypes.data_as(_ip),
                             val.ctypes.data_as(_dp))
         return res, ptr, col, val.reshape(nnzb, 3, 3)
+
+    def objective(self, actor, dt, params, u0=None, v0=None) -> float:
+        out = C.c_double()
+        lib().sdxo_objective(self._h, actor, dt, C.byref(params), None if u0 is None else _d(u0),
+                             None if v0 is None else _d(v0), C.byref(out))
+        return out.value
 
     def boundary(self, actor):
         n = self.info(actor)["n_tris"]

@@ -541,6 +541,355 @@ LinStatus PCG(const LinOp<T>& A, const std::vector<T>& b, std::vector<T>& x, con
   return out;
 }
 
+// ------------------------------------------------------------------ Newton driver
+
+// Forcing term: the relative tolerance of the linear solve of Newton
+// iteration k (the number of iterations done before it),
+// NewtonSolver::SetLinearSolverRelativeTolerance (newton_solver.cpp:426-501).
+// Eisenstat & Walker (1996), choices 1 and 2 (gamma 0.9, alpha the golden
+// ratio), with eta_max 0.01 at k = 0, their safeguard taken only when it
+// exceeds 0.01, and the result clamped to [100 eps(T), 0.01]; choice 3
+// max(min(1/(2 + k), sqrt(|r|)), relTol).  Constant keeps the previous value.
+template <class T>
+T ForcingTerm(int strategy, int k, T resNorm, T prevResNorm, T linResNorm, T prevEta, T relTol) {
+  const T kMaxEta = T(0.01), kSafeguardEtaTh = T(0.01);
+  const T kMinEta = T(100) * std::numeric_limits<T>::epsilon();
+  const T alpha = T(0.5) + T(0.5) * std::sqrt(T(5));
+  auto clamp = [&](T eta) { return std::min(std::max(eta, kMinEta), kMaxEta); };
+  switch (strategy) {
+    case SDX_LINTOL_EW1: {
+      if (k == 0) return kMaxEta;
+      const T safeguard = std::pow(prevEta, alpha);
+      T eta = std::fabs(resNorm - linResNorm) / prevResNorm;
+      if (safeguard > kSafeguardEtaTh) eta = std::max(eta, safeguard);
+      return clamp(eta);
+    }
+    case SDX_LINTOL_EW2: {
+      if (k == 0) return kMaxEta;
+      const T gamma = T(0.9);
+      const T safeguard = gamma * std::pow(prevEta, alpha);
+      T eta = gamma * std::pow(resNorm / prevResNorm, alpha);
+      if (safeguard > kSafeguardEtaTh) eta = std::max(eta, safeguard);
+      return clamp(eta);
+    }
+    case SDX_LINTOL_EW3:
+      return std::max(std::min(T(1) / (T(2) + T(k)), std::sqrt(resNorm)), relTol);
+    default:
+      return prevEta;
+  }
+}
+
+// A system of nonlinear equations res(x) = 0 with an objective whose gradient
+// is res (SnleProblem, snle_problem.h), as the Newton driver sees it.
+template <class T>
+struct NewtonProblem {
+  virtual ~NewtonProblem() = default;
+  virtual std::vector<T>& X() = 0;
+  // Residual at X() and, with obj, the objective; with wantH also the
+  // DResidual (projected to PSD with projectPsd) used by the next Solve.
+  virtual void Eval(bool obj, bool wantH, bool projectPsd, std::vector<T>& res, double& objective) = 0;
+  // DResidual dx = res with zero initial guess.
+  virtual LinStatus Solve(const std::vector<T>& res, std::vector<T>& dx, const sdx_krylov_params& kp) = 0;
+};
+
+// Line searches along the increment inc from sol0 (line_search.h:76-390).
+// On entry X() = sol0, res is the residual there and merit its objective.  The
+// increment is scaled in place; on return X() is the last trial point, res the
+// residual there when the search evaluated it, merit the accepted objective,
+// and it the number of rejected trials (maxIter when none was accepted).
+template <class T>
+bool LineSearchStep(NewtonProblem<T>& P, const sdx_solver_params& sp, int type, std::vector<T>& inc,
+                    std::vector<T>& res, double& merit, int& it) {
+  std::vector<T>& x = P.X();
+  const std::vector<T> sol0 = x;
+  const size_t n = x.size();
+  const int maxIter = sp.line_search_max_iter;
+  const T alpha = T(sp.line_search_alpha);
+  auto update = [&]() {
+    for (size_t i = 0; i < n; ++i) x[i] = sol0[i] + inc[i];
+  };
+  auto scaleInc = [&](T f) {
+    for (size_t i = 0; i < n; ++i) inc[i] *= f;
+  };
+  auto dotRes = [&]() {
+    double d = 0;
+    for (size_t i = 0; i < n; ++i) d += double(res[i]) * double(inc[i]);
+    return d;
+  };
+  auto norm = [&]() {
+    double d = 0;
+    for (size_t i = 0; i < n; ++i) d += double(res[i]) * double(res[i]);
+    return T(std::sqrt(d));
+  };
+  double obj = 0;
+  switch (type) {
+    case SDX_LS_NONE:  // (:76-87) the iteration count is left as it was
+      update();
+      return true;
+    case SDX_LS_SIMPLE:  // (:89-124)
+      for (it = 0; it < maxIter; ++it) {
+        if (it > 0) scaleInc(alpha);
+        update();
+        P.Eval(true, false, false, res, obj);
+        if (obj <= merit + std::fabs(merit * sp.line_search_max_rel_increase)) {
+          merit = obj;
+          return true;
+        }
+      }
+      return false;
+    case SDX_LS_ARMIJO: {  // (:126-167)
+      const double factor0 = dotRes() * sp.line_search_wolfe1;
+      T scale = T(1);
+      for (it = 0; it < maxIter; ++it) {
+        if (it > 0) {
+          scale *= alpha;
+          scaleInc(alpha);
+        }
+        update();
+        P.Eval(true, false, false, res, obj);
+        if (obj <= merit + double(scale) * factor0) {
+          merit = obj;
+          return true;
+        }
+      }
+      return false;
+    }
+    case SDX_LS_WOLFE_WEAK:
+    case SDX_LS_WOLFE_STRONG: {  // (:169-289)
+      const double dot0 = dotRes();
+      const double t1 = sp.line_search_wolfe1 * dot0, t2 = sp.line_search_wolfe2 * dot0;
+      const bool strong = type == SDX_LS_WOLFE_STRONG;
+      T scale = T(1), scaleMin = T(0), scaleMax = std::numeric_limits<T>::infinity();
+      for (it = 0; it < maxIter; ++it) {
+        update();
+        P.Eval(true, false, false, res, obj);
+        const double d = dotRes();
+        const bool bad = !std::isfinite(d) || !std::isfinite(obj) || obj > merit + double(scale) * t1;
+        const bool shrink = bad || (strong && d > std::fabs(t2 * double(scale)));
+        const bool grow = !shrink && (strong ? d < -std::fabs(t2 * double(scale)) : d < t2 * double(scale));
+        if (shrink) {
+          scaleMax = scale;
+          const T scaleNew = scaleMin + alpha * (scale - scaleMin);
+          scaleInc(scaleNew / scale);
+          scale = scaleNew;
+        } else if (grow) {
+          scaleMin = scale;
+          const T scaleNew = std::isfinite(double(scaleMax)) ? T(0.5) * (scale + scaleMax) : scale / alpha;
+          scaleInc(scaleNew / scale);
+          scale = scaleNew;
+        } else {
+          merit = obj;
+          return true;
+        }
+      }
+      return false;
+    }
+    case SDX_LS_ARMIJO_OR_RESIDUAL_NORM: {  // (:338-390)
+      const double dot0 = dotRes();
+      if (!(dot0 > 0)) {
+        const double factor0 = dot0 * sp.line_search_wolfe1;
+        const T resNorm0 = norm();
+        T scale = T(1);
+        for (it = 0; it < maxIter; ++it) {
+          if (it > 0) {
+            scale *= alpha;
+            scaleInc(alpha);
+          }
+          update();
+          P.Eval(true, false, false, res, obj);
+          if (obj <= merit + double(scale) * factor0 || norm() < resNorm0) {
+            merit = obj;
+            return true;
+          }
+        }
+        return false;
+      }
+      // not a descent direction: the residual-norm search below
+    }
+      [[fallthrough]];
+    case SDX_LS_RESIDUAL_NORM:
+    default: {  // (:291-336)
+      const T resNorm0 = norm();
+      for (it = 0; it < maxIter; ++it) {
+        if (it > 0) scaleInc(alpha);
+        update();
+        P.Eval(false, false, false, res, obj);
+        if (norm() <= resNorm0) return true;
+      }
+      return false;
+    }
+  }
+}
+
+// NewtonSolver<T>::Solve (newton_solver.cpp:603-876) with the stop criteria
+// ResNorm (Global), MaxIters and RelStepNorm (:120-200), TakeStep (:357-397)
+// and the default fitted-Hessian settings (SaturationHessianParams: joint
+// friction off, so the first try of an iteration is never its last): a try
+// whose linear solve does not converge, or whose line search finds no
+// improvement, is undone and retried once with the PSD-projected DResidual;
+// the retry keeps its final line-search iterate (or, with the gradient-descent
+// fallback, searches along -res from the iterate it started at).
+template <class T>
+void NewtonDriver(NewtonProblem<T>& P, const sdx_solver_params& sp, sdx_step_stats& stats, double* meritOut) {
+  std::vector<T>& x = P.X();
+  const size_t n = x.size();
+  std::vector<T> res(n), res0(n), dx(n, T(0)), inc(n);
+  const int lsType = sp.line_search_type;
+  const bool needObj = lsType != SDX_LS_NONE && lsType != SDX_LS_RESIDUAL_NORM;
+  int psdMode = sp.psd_proj_mode;
+  auto normOf = [](const std::vector<T>& v) {
+    double s = 0;
+    for (T e : v) s += double(e) * double(e);
+    return T(std::sqrt(s));
+  };
+  T eta = T(sp.linear.rel_tol), linResNorm = T(0), resNorm = T(0), resNorm0 = T(0), prevResNorm = T(0);
+  double merit = 0.0, dummy = 0.0;
+  T dxNorm = std::numeric_limits<T>::infinity();
+  int numIterDone = 0;
+  stats = sdx_step_stats{};
+  // TakeStep: the line search from the current iterate, undone without
+  // improvement unless takeFinal.
+  auto takeStep = [&](bool takeFinal) {
+    const std::vector<T> sol0 = x;
+    int it = 0;
+    const bool improved = LineSearchStep<T>(P, sp, lsType, inc, res, merit, it);
+    stats.line_search_iters += it;
+    if (!improved && !takeFinal) {
+      x = sol0;
+      P.Eval(needObj, false, false, res, merit);
+    }
+    return improved;
+  };
+  while (true) {
+    const bool first = numIterDone == 0;
+    bool projectPsd = psdMode == SDX_PSD_ALWAYS;
+    P.Eval(needObj, first, projectPsd, res, merit);
+    resNorm = normOf(res);
+    res0 = res;
+    if (first) resNorm0 = resNorm;
+    const bool absConv = resNorm <= T(sp.newton_abs_tol);
+    const bool relConv = numIterDone > 0 && resNorm <= T(sp.newton_rel_tol) * resNorm0;
+    if (absConv || relConv) {
+      stats.convergence = SDX_CONV_CONVERGED;
+      break;
+    }
+    if (numIterDone >= sp.newton_max_iter) {
+      stats.convergence = SDX_CONV_STOPPED;
+      break;
+    }
+    if (numIterDone > 0 && dxNorm <= T(sp.rel_step_tol) * normOf(x)) {
+      stats.convergence = SDX_CONV_STOPPED;
+      break;
+    }
+    eta = ForcingTerm<T>(sp.linear_tolerance_strategy, numIterDone, resNorm, prevResNorm, linResNorm, eta,
+                         T(sp.linear.rel_tol));
+    prevResNorm = resNorm;
+    numIterDone++;
+    bool retry = false, fittedAll = false;
+    do {
+      if (!first || retry) P.Eval(false, true, projectPsd, res, dummy);
+      sdx_krylov_params kp = sp.linear;
+      kp.rel_tol = double(eta);
+      kp.initial_guess_zero = 1;
+      std::fill(dx.begin(), dx.end(), T(0));
+      const LinStatus ls = P.Solve(res, dx, kp);
+      linResNorm = T(ls.resNorm);
+      stats.linear_iters += ls.iters;
+      stats.linear_convergence = ls.conv;
+      stats.linear_rel_res = ls.relResNorm;
+      dxNorm = normOf(dx);
+      const bool linConv = ls.conv == SDX_CONV_CONVERGED;
+      retry = !linConv;
+      const bool isLastTry = fittedAll && (projectPsd || psdMode == SDX_PSD_NEVER);
+      if (isLastTry || linConv) {
+        for (size_t i = 0; i < n; ++i) inc[i] = -dx[i];
+        retry = !takeStep(isLastTry && !sp.gradient_descent_fallback);
+      }
+      if (retry) {
+        if (isLastTry) {
+          if (sp.gradient_descent_fallback) {
+            for (size_t i = 0; i < n; ++i) inc[i] = -res0[i];
+            takeStep(true);
+          }
+          break;
+        }
+        fittedAll = true;
+        if (psdMode == SDX_PSD_IF_FAIL_ALWAYS) psdMode = SDX_PSD_ALWAYS;
+        projectPsd = true;
+      }
+    } while (retry);
+    if (sp.explosion_control) {
+      P.Eval(false, false, false, res, dummy);
+      resNorm = normOf(res);
+      if (resNorm > T(sp.rel_div_tol) * resNorm0 || resNorm > T(sp.abs_div_tol) ||
+          !std::isfinite(double(resNorm))) {
+        stats.convergence = SDX_CONV_DIVERGED;
+        break;
+      }
+    }
+  }
+  stats.newton_iters = numIterDone;
+  stats.res_norm0 = double(resNorm0);
+  stats.res_norm = double(resNorm);
+  if (meritOut) *meritOut = merit;
+}
+
+// The reference's synthetic Newton test problems (newton_solver_test.cpp):
+// 0: the scalar problem with objective -x^3/30 - x sin x - cos x (:34-80),
+// 1: the tridiagonal Broyden problem (:82-150).  DResidual solves by CG
+// without preconditioner.
+template <class T>
+struct SyntheticProblem : NewtonProblem<T> {
+  int kind;
+  std::vector<T> x;
+  CsrOp<T> H;
+  std::vector<int32_t> ptr, col;
+  SyntheticProblem(int k, const double* x0, int n) : kind(k), x(x0, x0 + n) {
+    ptr.assign(1, 0);
+    for (int i = 0; i < n; ++i) {
+      for (int j = std::max(0, i - 1); j <= std::min(n - 1, i + 1); ++j)
+        if (kind != 0 || j == i) col.push_back(j);
+      ptr.push_back(int32_t(col.size()));
+    }
+    H.n = n;
+    H.ptr = ptr.data();
+    H.col = col.data();
+    H.val.assign(col.size(), T(0));
+  }
+  std::vector<T>& X() override { return x; }
+  void Eval(bool obj, bool wantH, bool, std::vector<T>& res, double& objective) override {
+    const int n = int(x.size());
+    res.assign(size_t(n), T(0));
+    if (kind == 0) {
+      const T v = x[0];
+      res[0] = -(v * v) / T(10) - v * std::cos(v);
+      if (obj) objective = double(-(v * v * v) / T(30) - v * std::sin(v) - std::cos(v));
+      if (wantH) H.val[0] = -v / T(5) - std::cos(v) + v * std::sin(v);
+      return;
+    }
+    if (obj) {
+      double o = 0.0;
+      for (int i = 0; i < n; ++i) o += double(x[size_t(i)] * (T(1) + T(1.5) * x[size_t(i)] - x[size_t(i)] * x[size_t(i)] / T(6)));
+      for (int i = 0; i + 1 < n; ++i) o -= double(T(2) * x[size_t(i)] * x[size_t(i) + 1]);
+      objective = o;
+    }
+    for (int i = 0; i < n; ++i) {
+      T r = T(1) + (T(3) - T(0.5) * x[size_t(i)]) * x[size_t(i)];
+      if (i > 0) r -= T(2) * x[size_t(i) - 1];
+      if (i + 1 < n) r -= T(2) * x[size_t(i) + 1];
+      res[size_t(i)] = r;
+    }
+    if (wantH)
+      for (int i = 0; i < n; ++i)
+        for (int k = ptr[size_t(i)]; k < ptr[size_t(i) + 1]; ++k)
+          H.val[size_t(k)] = col[size_t(k)] == i ? T(3) - x[size_t(i)] : T(-2);
+  }
+  LinStatus Solve(const std::vector<T>& res, std::vector<T>& dx, const sdx_krylov_params& kp) override {
+    Prec<T> M;
+    return PCG<T>(H, res, dx, M, kp);
+  }
+};
+
 // ------------------------------------------------------------------ scene
 
 struct Collider {
@@ -799,8 +1148,10 @@ struct Scene {
   // only here): pRel = (x - x0) - (P - P0), with x0 the sample at stage start
   // and P0 the collider's stage-start position
   // (mochi_physics/src/mochi_contact.cpp:1200-1260).
+  // With energy set, the contact energy sum over samples w (1/2 k p^2 + fN (mu f(|pT|) +
+  // 1/2 cv/dt |pT|^2 + 1/2 cn/dt pn^2)) (contact_utils.h:560, :935) is added to it.
   void AssembleContact(Actor<T>& a, int ai, const std::vector<T>& u, const std::vector<T>& u0, T dt, bool wantH,
-                       std::vector<T>& res) {
+                       std::vector<T>& res, double* energy = nullptr) {
     static const T kBary[3][3] = {{T(1.0 / 6.0), T(2.0 / 3.0), T(1.0 / 6.0)},
                                   {T(1.0 / 6.0), T(1.0 / 6.0), T(2.0 / 3.0)},
                                   {T(2.0 / 3.0), T(1.0 / 6.0), T(1.0 / 6.0)}};
@@ -861,6 +1212,7 @@ struct Scene {
           PolyReLU<T>(-dist, a.hPen, a.hPen - a.thrPen, pen, dpen, ddpen);
           const T fnorm = a.kPen * pen * dpen;
           const T dfn = -a.kPen * (dpen * dpen + pen * ddpen);
+          double eSample = 0.5 * double(a.kPen) * double(pen) * double(pen);
           // force on the sample (collider normal times the penalty force) and its
           // derivative dF = cI I + cN n n^T
           T F[3];
@@ -881,26 +1233,34 @@ struct Scene {
             for (int d = 0; d < 3; ++d) pT[d] = pRel[d] - pn * grad[d];
             const T pTn = std::sqrt(pT[0] * pT[0] + pT[1] * pT[1] + pT[2] * pT[2]);
             T eigP = 0, eigN = 0;
+            double ef = 0.0;  // energy factor: the dissipation energy is fN ef
             if (a.muCoulomb > T(0)) {
               T f, df, ddf, dfx;
               IPCStepC1<T>(pTn, a.falloffVel * dt, f, df, ddf, dfx);
               const T tmp = a.muCoulomb * fN * dfx;
               for (int d = 0; d < 3; ++d) F[d] -= tmp * pT[d];
               eigP = -tmp;  // fitted Hessian: the same eigenvalue along the slip direction
+              ef += double(a.muCoulomb) * double(f);
             }
             if (a.cViscous > T(0)) {
-              const T tmp = (a.cViscous / dt) * fN;
+              const T cvd = a.cViscous / dt;
+              const T tmp = cvd * fN;
               for (int d = 0; d < 3; ++d) F[d] -= tmp * pT[d];
               eigP -= tmp;
+              ef += 0.5 * double(cvd) * double(pTn) * double(pTn);
             }
             if (a.cNormalDamping > T(0)) {
-              const T tmp = (a.cNormalDamping / dt) * fN;
+              const T cnd = a.cNormalDamping / dt;
+              const T tmp = cnd * fN;
               for (int d = 0; d < 3; ++d) F[d] -= tmp * pn * grad[d];
               eigN = -tmp;
+              ef += 0.5 * double(cnd) * double(pn) * double(pn);
             }
             cI += eigP;
             cN += eigN - eigP;
+            eSample += double(fN) * ef;
           }
+          if (energy) *energy += double(w) * eSample;
           for (int i = 0; i < 3; ++i) {
             const T bs = kBary[q][i] * w;
             for (int d = 0; d < 3; ++d) res[3 * size_t(nd[i]) + d] -= F[d] * bs;
@@ -938,109 +1298,99 @@ struct Scene {
 
   static T Norm(const std::vector<T>& v) { return std::sqrt(Dot(v, v)); }
 
+  // Objective (merit) of one actor at displacement u: the stress energy
+  // vol psi(F) per tet (Q1, fem_stress.h), gravity -rho g.(X + u) at the
+  // centroid (fem_gravity.h:93-111), inertia rho/(2 dt^2)|u - (u0 + dt v0)|^2
+  // and mass damping (alpha/dt) rho/2 |u - u0|^2 at the Q4 points
+  // (fem_inertia.h:85-130), and the contact energies of AssembleContact.
+  double Objective(Actor<T>& a, int ai, const std::vector<T>& u, const std::vector<T>& u0,
+                   const std::vector<T>& v0, T dt, const T grav[3]) {
+    const double dtfi2 = 1.0 / (double(dt) * double(dt));
+    const double massDampingScale = a.massDamping > T(0) ? double(a.massDamping) / double(dt) : 0.0;
+    double e = 0.0;
+    for (int t = 0; t < a.nT; ++t) {
+      const auto& g = a.geom[size_t(t)];
+      const int32_t* tv = &a.tets[4 * size_t(t)];
+      T F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+      for (int f = 0; f < 4; ++f)
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) F[r * 3 + c] += u[3 * size_t(tv[f]) + r] * g.dN[f][c];
+      double psi = 0.0;
+      SmithNH<T>(a.mu[size_t(t)], a.lambda[size_t(t)], F, &psi, nullptr, nullptr, false);
+      e += double(g.vol) * psi;
+      double xc[3] = {0, 0, 0};
+      for (int f = 0; f < 4; ++f)
+        for (int d = 0; d < 3; ++d)
+          xc[d] += 0.25 * (double(a.X[3 * size_t(tv[f]) + d]) + double(u[3 * size_t(tv[f]) + d]));
+      e -= double(a.density) * double(g.vol) * (grav[0] * xc[0] + grav[1] * xc[1] + grav[2] * xc[2]);
+      for (int q = 0; q < 4; ++q) {
+        T N[4];
+        Q4Basis<T>(q, N);
+        const double w = double(g.vol) * 0.25;
+        double qv[3] = {0, 0, 0}, qd[3] = {0, 0, 0};
+        for (int f = 0; f < 4; ++f)
+          for (int d = 0; d < 3; ++d) {
+            const size_t k = 3 * size_t(tv[f]) + d;
+            qv[d] += double(N[f]) * (double(u[k]) - (double(u0[k]) + double(dt) * double(v0[k])));
+            qd[d] += double(N[f]) * (double(u[k]) - double(u0[k]));
+          }
+        e += 0.5 * dtfi2 * double(a.density) * (qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2]) * w;
+        if (massDampingScale > 0.0)
+          e += 0.5 * massDampingScale * double(a.density) * (qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2]) * w;
+      }
+    }
+    std::vector<T> scratch(3 * size_t(a.nV), T(0));
+    AssembleContact(a, ai, u, u0, dt, false, scratch, &e);
+    return e;
+  }
+
+  // One actor's backward-Euler step as a NewtonProblem: the assembly above
+  // and a PCG solve of the global BSR with the configured preconditioner.
+  struct ActorProblem : NewtonProblem<T> {
+    Scene* sc;
+    Actor<T>* a;
+    int ai;
+    const std::vector<T>* u0;
+    const std::vector<T>* v0;
+    T dt;
+    const T* grav;
+    std::vector<T>& X() override { return a->u; }
+    void Eval(bool obj, bool wantH, bool projectPsd, std::vector<T>& res, double& objective) override {
+      sc->Assemble(*a, ai, a->u, *u0, *v0, dt, grav, wantH, projectPsd, res);
+      if (obj) objective = sc->Objective(*a, ai, a->u, *u0, *v0, dt, grav);
+    }
+    LinStatus Solve(const std::vector<T>& res, std::vector<T>& dx, const sdx_krylov_params& kp) override {
+      Prec<T> M;
+      if (kp.preconditioner == SDX_PREC_BLOCK_JACOBI) {
+        M.bs = 3;
+        M.inv.resize(9 * size_t(a->nV));
+        for (int i = 0; i < a->nV; ++i)
+          Inverse3(&a->K.val[9 * size_t(a->K.diag[size_t(i)])], &M.inv[9 * size_t(i)]);
+      } else if (kp.preconditioner == SDX_PREC_JACOBI) {
+        M.bs = 1;
+        M.inv.resize(3 * size_t(a->nV));
+        for (int i = 0; i < a->nV; ++i)
+          for (int d = 0; d < 3; ++d)
+            M.inv[3 * size_t(i) + d] = T(1) / a->K.val[9 * size_t(a->K.diag[size_t(i)]) + 4 * d];
+      }
+      BsrOp<T> op(&a->K);
+      return PCG<T>(op, res, dx, M, kp);
+    }
+  };
+
   // NewtonSolver<T>::Solve (newton_solver.cpp:603-876) for one actor's island.
   void NewtonSolve(Actor<T>& a, int ai, const std::vector<T>& u0, const std::vector<T>& v0, T dt,
                    const sdx_solver_params& sp, sdx_step_stats& stats) {
     const T grav[3] = {T(sp.gravity[0]), T(sp.gravity[1]), T(sp.gravity[2])};
-    const size_t n = 3 * size_t(a.nV);
-    std::vector<T> res(n), dx(n, T(0)), sol0;
-    std::vector<T>& x = a.u;
-    const bool projectPsd = sp.psd_proj_mode == SDX_PSD_ALWAYS;
-    int numIterDone = 0;
-    T resNorm = 0, resNorm0 = 0;
-    double dxNorm = std::numeric_limits<double>::infinity();
-    bool haveDx = false;
-    stats = sdx_step_stats{};
-    while (true) {
-      const bool first = numIterDone == 0;
-      Assemble(a, ai, x, u0, v0, dt, grav, first, projectPsd, res);
-      resNorm = Norm(res);
-      if (first) resNorm0 = resNorm;
-      // stop criteria: ResNorm, MaxIters, (MaxElapsedTime off), RelStepNorm
-      const bool absConv = resNorm <= T(sp.newton_abs_tol);
-      const bool relConv = numIterDone > 0 && resNorm <= T(sp.newton_rel_tol) * resNorm0;
-      if (absConv || relConv) {
-        stats.convergence = SDX_CONV_CONVERGED;
-        break;
-      }
-      if (numIterDone >= sp.newton_max_iter) {
-        stats.convergence = SDX_CONV_STOPPED;
-        break;
-      }
-      if (numIterDone > 0 && haveDx && dxNorm <= double(T(sp.rel_step_tol)) * double(Norm(x))) {
-        stats.convergence = SDX_CONV_STOPPED;
-        break;
-      }
-      numIterDone++;
-      if (!first) Assemble(a, ai, x, u0, v0, dt, grav, true, projectPsd, res);
-      // linear solve with the global BSR and block Jacobi
-      Prec<T> M;
-      if (sp.linear.preconditioner == SDX_PREC_BLOCK_JACOBI) {
-        M.bs = 3;
-        M.inv.resize(9 * size_t(a.nV));
-        for (int i = 0; i < a.nV; ++i)
-          Inverse3(&a.K.val[9 * size_t(a.K.diag[size_t(i)])], &M.inv[9 * size_t(i)]);
-      } else if (sp.linear.preconditioner == SDX_PREC_JACOBI) {
-        M.bs = 1;
-        M.inv.resize(n);
-        for (int i = 0; i < a.nV; ++i)
-          for (int d = 0; d < 3; ++d) M.inv[3 * size_t(i) + d] = T(1) / a.K.val[9 * size_t(a.K.diag[size_t(i)]) + 4 * d];
-      }
-      sdx_krylov_params kp = sp.linear;
-      kp.initial_guess_zero = 1;
-      std::fill(dx.begin(), dx.end(), T(0));
-      BsrOp<T> op(&a.K);
-      LinStatus ls = PCG<T>(op, res, dx, M, kp);
-      stats.linear_iters += ls.iters;
-      stats.linear_convergence = ls.conv;
-      stats.linear_rel_res = ls.relResNorm;
-      dxNorm = double(Norm(dx));
-      haveDx = true;
-      // Line search along -dx.  A converged solve takes the step; a failed line
-      // search or an unconverged solve retries once with an identical
-      // operator (fitted Hessian already has no effect without friction),
-      // which reproduces the same step, and then keeps the final iterate
-      // (newton_solver.cpp:783-854).
-      const int lsMax = sp.line_search_type == SDX_LS_RESIDUAL_NORM ? sp.line_search_max_iter : 1;
-      sol0 = x;
-      const T lsRes0 = resNorm;
-      bool improved = false;
-      int it = 0;
-      T scale = T(1);
-      for (it = 0; it < lsMax; ++it) {
-        if (it > 0) scale *= T(sp.line_search_alpha);
-        for (size_t i = 0; i < n; ++i) x[i] = sol0[i] - scale * dx[i];
-        if (sp.line_search_type != SDX_LS_RESIDUAL_NORM) {
-          improved = true;
-          break;
-        }
-        Assemble(a, ai, x, u0, v0, dt, grav, false, projectPsd, res);
-        if (Norm(res) <= lsRes0) {
-          improved = true;
-          break;
-        }
-      }
-      const int lsIters = (it < lsMax) ? it : lsMax;
-      stats.line_search_iters += lsIters;
-      const bool linConv = ls.conv == SDX_CONV_CONVERGED;
-      if (!improved || !linConv) {
-        // the retry repeats the same assembly, solve and line search
-        if (linConv) stats.line_search_iters += lsIters;
-        stats.linear_iters += ls.iters;
-      }
-      if (sp.explosion_control) {
-        Assemble(a, ai, x, u0, v0, dt, grav, false, projectPsd, res);
-        resNorm = Norm(res);
-        if (resNorm > T(sp.rel_div_tol) * resNorm0 || resNorm > T(sp.abs_div_tol) ||
-            !std::isfinite(double(resNorm))) {
-          stats.convergence = SDX_CONV_DIVERGED;
-          break;
-        }
-      }
-    }
-    stats.newton_iters = numIterDone;
-    stats.res_norm0 = double(resNorm0);
-    stats.res_norm = double(resNorm);
+    ActorProblem P;
+    P.sc = this;
+    P.a = &a;
+    P.ai = ai;
+    P.u0 = &u0;
+    P.v0 = &v0;
+    P.dt = dt;
+    P.grav = grav;
+    NewtonDriver<T>(P, sp, stats, nullptr);
   }
 
   // Scene::Step for every actor: EntityIncrementStep, PreStage, Newton, PostStage.
@@ -1226,6 +1576,47 @@ int sdxo_assemble(void* sv, int actor, double dt, const sdx_solver_params* sp, c
       for (size_t i = 0; i < a.K.val.size(); ++i) val[i] = double(a.K.val[i]);
     return int(SDX_OK);
   });
+}
+
+// Objective (merit) of one actor at its current displacement with stage start
+// (u0, v0) given explicitly (NULL: zeros).
+int sdxo_objective(void* sv, int actor, double dt, const sdx_solver_params* sp, const double* u0in,
+                   const double* v0in, double* out) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    using T = typename std::remove_reference_t<decltype(sc.actors[0].X)>::value_type;
+    auto& a = sc.actors[size_t(actor)];
+    const size_t n = 3 * size_t(a.nV);
+    std::vector<T> u0(n, T(0)), v0(n, T(0));
+    if (u0in)
+      for (size_t i = 0; i < n; ++i) u0[i] = T(u0in[i]);
+    if (v0in)
+      for (size_t i = 0; i < n; ++i) v0[i] = T(v0in[i]);
+    const T grav[3] = {T(sp->gravity[0]), T(sp->gravity[1]), T(sp->gravity[2])};
+    *out = sc.Objective(a, actor, a.u, u0, v0, T(dt), grav);
+    return int(SDX_OK);
+  });
+}
+
+// The Newton driver on one of the reference's synthetic test problems
+// (0: scalar, 1: tridiagonal Broyden of size n) from x0, in double; x holds
+// the solution and merit the last accepted objective.
+int sdxo_newton_synthetic(int kind, int n, const double* x0, const sdx_solver_params* sp, double* x,
+                          sdx_step_stats* stats, double* merit) {
+  if ((kind != 0 && kind != 1) || n < 1 || (kind == 0 && n != 1)) return SDX_ERR_INVALID_ARGUMENT;
+  SyntheticProblem<double> P(kind, x0, n);
+  sdx_step_stats st{};
+  double m = 0.0;
+  NewtonDriver<double>(P, *sp, st, &m);
+  std::copy(P.x.begin(), P.x.end(), x);
+  if (stats) *stats = st;
+  if (merit) *merit = m;
+  return SDX_OK;
+}
+
+// Forcing term of the linear solve (double arithmetic).
+double sdxo_forcing_term(int strategy, int k, double resNorm, double prevResNorm, double linResNorm,
+                         double prevEta, double relTol) {
+  return ForcingTerm<double>(strategy, k, resNorm, prevResNorm, linResNorm, prevEta, relTol);
 }
 
 // Boundary triangles (3 per row) and rest areas of one actor.

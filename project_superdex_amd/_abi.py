@@ -18,8 +18,9 @@ SDX_ERR_UNSUPPORTED = -5
 CONV_NONE, CONV_CONVERGED, CONV_STOPPED, CONV_DIVERGED = 0, 1, 2, 3
 NORM_RESIDUAL_L2, NORM_PRECONDITIONED_RESIDUAL_L2, NORM_RESIDUAL_PRECONDITIONER_INDUCED = 0, 1, 2
 PREC_NONE, PREC_JACOBI, PREC_BLOCK_JACOBI = 0, 1, 2
-LS_NONE, LS_RESIDUAL_NORM = 0, 5
-PSD_NEVER, PSD_ALWAYS = 0, 1
+LS_NONE, LS_SIMPLE, LS_ARMIJO, LS_WOLFE_WEAK, LS_WOLFE_STRONG, LS_RESIDUAL_NORM, LS_ARMIJO_OR_RESIDUAL_NORM = range(7)
+PSD_NEVER, PSD_ALWAYS, PSD_IF_FAIL_RETRY, PSD_IF_FAIL_ALWAYS = 0, 1, 2, 3
+LINTOL_CONSTANT, LINTOL_EW1, LINTOL_EW2, LINTOL_EW3 = 0, 1, 2, 3
 PREC_MODE_FP32, PREC_MODE_MIXED, PREC_MODE_FP64 = 0, 1, 2
 WARM_START_ZERO, WARM_START_LEARNED = 0, 1
 BSR_BLOCK_ROWMAJOR, BSR_REF_ROW_STRIP = 0, 1
@@ -64,8 +65,13 @@ class SolverParams(C.Structure):
         ("abs_div_tol", C.c_double),
         ("rel_div_tol", C.c_double),
         ("line_search_max_iter", C.c_int32),
-        ("linear_tolerance_constant", C.c_int32),
+        ("linear_tolerance_strategy", C.c_int32),
         ("line_search_alpha", C.c_double),
+        ("line_search_wolfe1", C.c_double),
+        ("line_search_wolfe2", C.c_double),
+        ("line_search_max_rel_increase", C.c_double),
+        ("gradient_descent_fallback", C.c_int32),
+        ("reserved0", C.c_int32),
         ("linear", KrylovParams),
         ("gravity", C.c_double * 3),
         ("warm_start", C.c_int32),
@@ -164,8 +170,12 @@ def default_solver_params() -> SolverParams:
     p.abs_div_tol = 1e9
     p.rel_div_tol = 1e4
     p.line_search_max_iter = 4
-    p.linear_tolerance_constant = 1
+    p.linear_tolerance_strategy = LINTOL_EW2
     p.line_search_alpha = 0.5
+    p.line_search_wolfe1 = 1e-4
+    p.line_search_wolfe2 = 0.9
+    p.line_search_max_rel_increase = 0.0
+    p.gradient_descent_fallback = 0
     p.linear = default_krylov_params()
     p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -9.8, 0.0
     p.warm_start = WARM_START_ZERO
@@ -179,6 +189,7 @@ def pinned_solver_params(newton_iters: int = 1, lin_rel_tol: float = 1e-6, lin_m
     constant linear tolerance, fixed Newton iteration count, abs/rel Newton
     tolerances 0, PSD projection always, explosion control on)."""
     p = default_solver_params()
+    p.linear_tolerance_strategy = LINTOL_CONSTANT
     p.newton_max_iter = newton_iters
     p.newton_abs_tol = 0.0
     p.newton_rel_tol = 0.0

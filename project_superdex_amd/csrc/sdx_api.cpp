@@ -66,8 +66,12 @@ void sdx_default_solver_params(sdx_solver_params* p) {
   p->abs_div_tol = 1e9;
   p->rel_div_tol = 1e4;
   p->line_search_max_iter = 4;
-  p->linear_tolerance_constant = 1;
+  p->linear_tolerance_strategy = SDX_LINTOL_EW2;
   p->line_search_alpha = 0.5;
+  p->line_search_wolfe1 = 1e-4;
+  p->line_search_wolfe2 = 0.9;
+  p->line_search_max_rel_increase = 0.0;
+  p->gradient_descent_fallback = 0;
   sdx_default_krylov(&p->linear);
   p->gravity[0] = 0.0;
   p->gravity[1] = -9.8;
@@ -252,10 +256,24 @@ sdx_status sdx_set_rigid_positions(sdx_world* world, int32_t n, const int32_t* c
 static sdx_status validate_solver_params(const sdx_solver_params& p) {
   const sdx_solver_params* params = &p;
   SDX_REQUIRE(params->newton_max_iter >= 0 && params->linear.max_iter > 0, "invalid iteration limits");
-  SDX_REQUIRE(params->line_search_type == SDX_LS_NONE || params->line_search_type == SDX_LS_RESIDUAL_NORM,
-              "only the None and ResidualNorm line searches are implemented");
-  SDX_REQUIRE(params->psd_proj_mode == SDX_PSD_ALWAYS || params->psd_proj_mode == SDX_PSD_NEVER,
-              "only PSD projection modes Always and Never are implemented");
+  SDX_REQUIRE(params->line_search_type >= SDX_LS_NONE && params->line_search_type <= SDX_LS_ARMIJO_OR_RESIDUAL_NORM,
+              "unknown line search type");
+  SDX_REQUIRE(params->line_search_type == SDX_LS_NONE || params->line_search_max_iter >= 1,
+              "a line search needs line_search_max_iter >= 1 (use SDX_LS_NONE for none)");
+  // NewtonSolver::SetParams (newton_solver.cpp:262-280) only warns about these; a step
+  // factor, or Wolfe coefficients, outside these ranges never end a line search well
+  SDX_REQUIRE(params->line_search_alpha > 0.0 && params->line_search_alpha < 1.0, "line_search_alpha must be in (0, 1)");
+  SDX_REQUIRE(params->line_search_wolfe1 > 0.0 && params->line_search_wolfe1 < 1.0,
+              "line_search_wolfe1 must be in (0, 1)");
+  SDX_REQUIRE(params->line_search_wolfe2 > params->line_search_wolfe1 && params->line_search_wolfe2 < 1.0,
+              "line_search_wolfe2 must be in (line_search_wolfe1, 1)");
+  SDX_REQUIRE(params->line_search_max_rel_increase >= 0.0 && std::isfinite(params->line_search_max_rel_increase),
+              "line_search_max_rel_increase must be finite and >= 0");
+  SDX_REQUIRE(params->psd_proj_mode >= SDX_PSD_NEVER && params->psd_proj_mode <= SDX_PSD_IF_FAIL_ALWAYS,
+              "unknown PSD projection mode");
+  SDX_REQUIRE(params->linear_tolerance_strategy >= SDX_LINTOL_CONSTANT &&
+                  params->linear_tolerance_strategy <= SDX_LINTOL_EW3,
+              "unknown linear tolerance strategy");
   SDX_REQUIRE(params->linear.preconditioner >= SDX_PREC_NONE && params->linear.preconditioner <= SDX_PREC_BLOCK_JACOBI,
               "unsupported preconditioner");
   SDX_REQUIRE(params->linear.norm_type >= 0 && params->linear.norm_type <= 2, "unsupported norm type");
@@ -273,16 +291,6 @@ sdx_status sdx_step(sdx_world* world, double dt, const sdx_solver_params* params
     const sdx_status v = validate_solver_params(*params);
     if (v != SDX_OK) return v;
   }
-  SDX_REQUIRE(params->newton_max_iter >= 0 && params->linear.max_iter > 0, "invalid iteration limits");
-  SDX_REQUIRE(params->line_search_type == SDX_LS_NONE || params->line_search_type == SDX_LS_RESIDUAL_NORM,
-              "only the None and ResidualNorm line searches are implemented");
-  SDX_REQUIRE(params->psd_proj_mode == SDX_PSD_ALWAYS || params->psd_proj_mode == SDX_PSD_NEVER,
-              "only PSD projection modes Always and Never are implemented");
-  SDX_REQUIRE(params->linear.preconditioner >= SDX_PREC_NONE && params->linear.preconditioner <= SDX_PREC_BLOCK_JACOBI,
-              "unsupported preconditioner");
-  SDX_REQUIRE(params->linear.norm_type >= 0 && params->linear.norm_type <= 2, "unsupported norm type");
-  SDX_REQUIRE(params->warm_start == SDX_WARM_START_ZERO || params->warm_start == SDX_WARM_START_LEARNED,
-              "unknown warm start");
   return guarded([&]() -> sdx_status {
     world->impl->step(dt, *params, stats, stats ? n_stats : 0);
     return SDX_OK;
@@ -395,6 +403,16 @@ sdx_status sdx_assemble(sdx_world* world, int32_t actor, double dt, const sdx_so
   SDX_REQUIRE(dt > 0, "dt must be positive");
   return guarded([&]() -> sdx_status {
     world->impl->assemble_actor(actor, dt, *params, out_residual, out_row_ptr, out_col_idx, out_values);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_objective(sdx_world* world, int32_t actor, double dt, const sdx_solver_params* params,
+                         double* out_objective) {
+  SDX_REQUIRE(world && params && out_objective, "NULL argument");
+  SDX_REQUIRE(dt > 0, "dt must be positive");
+  return guarded([&]() -> sdx_status {
+    *out_objective = world->impl->objective_actor(actor, dt, *params);
     return SDX_OK;
   });
 }

@@ -39,6 +39,9 @@ struct CgConfig {
   // which folds them into x (x is read and written in FP64 only then); NULL:
   // every update adds into x
   void* xd;
+  // per-environment relative tolerance [n_env] (the Newton forcing terms);
+  // NULL: rel_tol for every environment
+  const double* env_rel_tol;
 };
 
 // Iteration k of the CG replaces the recursive residual by the FP64 true
@@ -278,5 +281,14 @@ void launch_env_copy(hipStream_t s, int32_t nV, const int32_t* row_env, const in
 // Per-environment |a|^2, |b|^2 added into set 0 of acc (zeroed by the caller).
 template <class T>
 void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, const T* b, double* acc);
+// Per-environment a.b added into set 0 of acc (zeroed by the caller).
+template <class T>
+void launch_env_dot(hipStream_t s, const Tiles& tv, int nEnv, const T* a, const T* b, double* acc);
+// Objective (merit) of the environments with env_mask set at displacement u,
+// stage start u0, v0: stress, gravity, inertia, mass damping and contact
+// energies, per environment in FP64 into set 0 of acc (zeroed by the caller).
+template <class T>
+void launch_energy(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0, const T* v0,
+                   double dt, const double grav[3], double* acc);
 
 }  // namespace sdx

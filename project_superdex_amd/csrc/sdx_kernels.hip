@@ -140,16 +140,114 @@ __host__ __device__ constexpr int hc_block(int a, int b) { return a == 0 ? b : (
 // Entry (r, c) of a symmetric 3x3 block stored as 00 11 22 01 02 12.
 __device__ __forceinline__ int sym_entry(int r, int c) { return r == c ? r : 2 + r + c; }
 
-// Penalty contact of the 3 trace-Q3 samples of a boundary triangle against
-// the analytic colliders of its environment (contact_utils.h:498-570,
-// fem_traction.h:58-170) and its dissipation (contact_utils.h:630-935):
+// One trace-Q3 sample x of a boundary triangle (face normal fnv, displacement
+// dxq = x - x0 over the step) against one analytic collider cd: the penalty
+// (contact_utils.h:498-570) and its dissipation (contact_utils.h:630-935):
 // C1-regularised Coulomb friction (activations.h:131-160), viscous friction
 // and normal damping, each proportional to the penalty force at the sample's
 // estimated stage-start distance d - n.pRel and faded by the alignment of the
 // collider and face normals, with pRel = (x - x0) - (P - P0) the sample's
-// displacement over the step relative to the collider, and the fitted
-// friction Hessian (the tangent-plane eigenvalue -mu fN df/|pT| in every
-// tangent direction), so every sample's Hessian is c_I I + c_N n n^T.
+// displacement relative to the collider, and the fitted friction Hessian (the
+// tangent-plane eigenvalue -mu fN df/|pT| in every tangent direction).
+// Returns whether the sample touches; F is the force on the sample, its
+// derivative cI I + cN n n^T (n = gr), and with want_e the energy density
+// 1/2 k p^2 + fN (mu f(|pT|) + 1/2 cv/dt |pT|^2 + 1/2 cn/dt pn^2).
+template <class T>
+struct ContactSample {
+  V3<T> F, gr;
+  T cI, cN;
+  double e;
+};
+
+template <class T>
+__device__ __forceinline__ bool contact_sample(const V3<T>& x, const V3<T>& dxq, const V3<T>& fnv,
+                                               const double* __restrict__ cd, const double* __restrict__ cp, T dt,
+                                               bool want_e, ContactSample<T>& out) {
+  const T kp = T(cp[0]), hp = T(cp[1]), thr = T(cp[2]), maxAlign = T(cp[3]);
+  const T mu = T(cp[4]), cvd = T(cp[5]) / dt, falloff = T(cp[6]) * dt, cnd = T(cp[7]) / dt;
+  const int kind = (int)cd[0];
+  const V3<T> P = mk3(T(cd[1]), T(cd[2]), T(cd[3]));
+  T dist;
+  V3<T> gr;
+  if (kind == 0) {
+    gr = mk3(T(cd[4]), T(cd[5]), T(cd[6]));
+    dist = dot3(gr, sub3(x, P));
+  } else {
+    const V3<T> dv = sub3(x, P);
+    const T l = dsqrt(dot3(dv, dv));
+    gr = scl3(T(1) / l, dv);
+    dist = l - T(cd[7]);
+  }
+  if (!(dist <= thr)) return false;
+  const T align = dot3(gr, fnv);
+  if (!(align <= maxAlign)) return false;
+  T pen, dpen, ddpen;
+  poly_relu<T>(-dist, hp, hp - thr, pen, dpen, ddpen);
+  out.gr = gr;
+  out.F = scl3(kp * pen * dpen, gr);
+  out.cI = T(0);
+  out.cN = -kp * (dpen * dpen + pen * ddpen);
+  out.e = want_e ? 0.5 * double(kp) * double(pen) * double(pen) : 0.0;
+  if (mu > T(0) || cvd > T(0) || cnd > T(0)) {
+    const T fade = maxAlign > T(-1) ? (maxAlign - align) / (maxAlign + T(1)) : T(1);
+    const V3<T> dP = mk3(T(cd[1] - cd[8]), T(cd[2] - cd[9]), T(cd[3] - cd[10]));  // P - P0
+    const V3<T> pRel = sub3(dxq, dP);
+    const T pn = dot3(gr, pRel);
+    T p0, dp0, ddp0;
+    poly_relu<T>(-(dist - pn), hp, hp - thr, p0, dp0, ddp0);
+    const T fN = fade * kp * p0 * dp0;
+    const V3<T> pT = sub3(pRel, scl3(pn, gr));
+    const T pTn = dsqrt(dot3(pT, pT));
+    T eigP = T(0), eigN = T(0);
+    double ef = 0.0;
+    if (mu > T(0)) {
+      // IPCstepC1(|pT|, falloff): df / |pT| and the antiderivative f
+      T dfx, fs;
+      if (pTn >= falloff) {
+        dfx = pTn == T(0) ? T(0) : T(1) / pTn;
+        fs = pTn - falloff / T(3);
+      } else {
+        const T xt = pTn / falloff;
+        dfx = (T(2) - xt) / falloff;
+        fs = pTn * xt * (T(1) - xt / T(3));
+      }
+      const T tmp = mu * fN * dfx;
+      out.F = sub3(out.F, scl3(tmp, pT));
+      eigP = -tmp;
+      ef += double(mu) * double(fs);
+    }
+    if (cvd > T(0)) {
+      const T tmp = cvd * fN;
+      out.F = sub3(out.F, scl3(tmp, pT));
+      eigP -= tmp;
+      ef += 0.5 * double(cvd) * double(pTn) * double(pTn);
+    }
+    if (cnd > T(0)) {
+      const T tmp = cnd * fN;
+      out.F = sub3(out.F, scl3(tmp * pn, gr));
+      eigN = -tmp;
+      ef += 0.5 * double(cnd) * double(pn) * double(pn);
+    }
+    out.cI = eigP;
+    out.cN += eigN - eigP;
+    if (want_e) out.e += double(fN) * ef;
+  }
+  return true;
+}
+
+// Trace-Q3 sample q of a triangle: barycentric weights (2/3 on vertex (q + 1) % 3).
+template <class T>
+__device__ __forceinline__ void tri_q3(int q, T N[3]) {
+  const T kSix = T(1.0 / 6.0), kTwoThirds = T(2.0 / 3.0);
+  N[0] = q == 2 ? kTwoThirds : kSix;
+  N[1] = q == 0 ? kTwoThirds : kSix;
+  N[2] = q == 1 ? kTwoThirds : kSix;
+}
+
+// Penalty contact and its dissipation (contact_sample) at the 3 trace-Q3
+// samples of a boundary triangle against the analytic colliders of its
+// environment (fem_traction.h:58-170): each sample adds -F N_i w to the
+// residual of vertex i and -(cN n n^T + cI I) N_i N_j w to block (i, j).
 // Outputs per triangle, written to their records (zeros when nothing
 // touches): 3 residual rows, 3 diagonal blocks and the 3 blocks of its vertex
 // pairs (symmetric, so no transposition).
@@ -186,84 +284,28 @@ __global__ void __launch_bounds__(256) k_contact(MeshView m, AsmRange ar, FieldP
     V3<T> fnv = cross3(sub3(xs[1], xs[0]), sub3(xs[2], xs[0]));
     fnv = scl3(T(1) / dsqrt(dot3(fnv, fnv)), fnv);
     const double* cp = f.contact + kContactParams * env;
-    const T kp = T(cp[0]), hp = T(cp[1]), thr = T(cp[2]), maxAlign = T(cp[3]);
-    const T mu = T(cp[4]), cvd = T(cp[5]) / dt, falloff = T(cp[6]) * dt, cnd = T(cp[7]) / dt;
-    const bool dissip = mu > T(0) || cvd > T(0) || cnd > T(0);
-    const T kSix = T(1.0 / 6.0), kTwoThirds = T(2.0 / 3.0);
     for (int c = c0; c < c1; ++c) {
       const double* cd = m.coll + kCollStride * c;
-      const int kind = (int)cd[0];
-      const V3<T> P = mk3(T(cd[1]), T(cd[2]), T(cd[3]));
-      const V3<T> dP = mk3(T(cd[1] - cd[8]), T(cd[2] - cd[9]), T(cd[3] - cd[10]));  // P - P0
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         T N[3];
-        N[0] = q == 2 ? kTwoThirds : kSix;
-        N[1] = q == 0 ? kTwoThirds : kSix;
-        N[2] = q == 1 ? kTwoThirds : kSix;
+        tri_q3<T>(q, N);
         const V3<T> x = add3(add3(scl3(N[0], xs[0]), scl3(N[1], xs[1])), scl3(N[2], xs[2]));
-        T dist;
-        V3<T> gr;
-        if (kind == 0) {
-          gr = mk3(T(cd[4]), T(cd[5]), T(cd[6]));
-          dist = dot3(gr, sub3(x, P));
-        } else {
-          const V3<T> dv = sub3(x, P);
-          const T l = dsqrt(dot3(dv, dv));
-          gr = scl3(T(1) / l, dv);
-          dist = l - T(cd[7]);
-        }
-        if (!(dist <= thr)) continue;
-        const T align = dot3(gr, fnv);
-        if (!(align <= maxAlign)) continue;
+        const V3<T> dxq = add3(add3(scl3(N[0], dxs[0]), scl3(N[1], dxs[1])), scl3(N[2], dxs[2]));
+        ContactSample<T> cs;
+        if (!contact_sample<T>(x, dxq, fnv, cd, cp, dt, false, cs)) continue;
         touched = true;
-        T pen, dpen, ddpen;
-        poly_relu<T>(-dist, hp, hp - thr, pen, dpen, ddpen);
-        const T fn = kp * pen * dpen;
-        // force on the sample F, its derivative cI I + cN n n^T
-        V3<T> F = scl3(fn, gr);
-        T cI = T(0), cN = -kp * (dpen * dpen + pen * ddpen);
-        if (dissip) {
-          const T fade = maxAlign > T(-1) ? (maxAlign - align) / (maxAlign + T(1)) : T(1);
-          const V3<T> pRel =
-              sub3(add3(add3(scl3(N[0], dxs[0]), scl3(N[1], dxs[1])), scl3(N[2], dxs[2])), dP);
-          const T pn = dot3(gr, pRel);
-          T p0, dp0, ddp0;
-          poly_relu<T>(-(dist - pn), hp, hp - thr, p0, dp0, ddp0);
-          const T fN = fade * kp * p0 * dp0;
-          const V3<T> pT = sub3(pRel, scl3(pn, gr));
-          const T pTn = dsqrt(dot3(pT, pT));
-          T eigP = T(0), eigN = T(0);
-          if (mu > T(0)) {
-            // IPCstepC1(|pT|, falloff): df / |pT|
-            const T dfx = pTn >= falloff ? (pTn == T(0) ? T(0) : T(1) / pTn) : (T(2) - pTn / falloff) / falloff;
-            const T tmp = mu * fN * dfx;
-            F = sub3(F, scl3(tmp, pT));
-            eigP = -tmp;
-          }
-          if (cvd > T(0)) {
-            const T tmp = cvd * fN;
-            F = sub3(F, scl3(tmp, pT));
-            eigP -= tmp;
-          }
-          if (cnd > T(0)) {
-            const T tmp = cnd * fN;
-            F = sub3(F, scl3(tmp * pn, gr));
-            eigN = -tmp;
-          }
-          cI = eigP;
-          cN += eigN - eigP;
-        }
-        const T fv[3] = {F.x, F.y, F.z};
+        const T fv[3] = {cs.F.x, cs.F.y, cs.F.z};
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int d = 0; d < 3; ++d) rc[3 * i + d] -= fv[d] * (N[i] * w);
         if (wantH) {
-          const T gv[3] = {gr.x, gr.y, gr.z};
+          const T gv[3] = {cs.gr.x, cs.gr.y, cs.gr.z};
           // -(cN n n^T + cI I) in 00 11 22 01 02 12 order
-          const T gg[6] = {-(cN * gv[0] * gv[0] + cI), -(cN * gv[1] * gv[1] + cI), -(cN * gv[2] * gv[2] + cI),
-                           -cN * gv[0] * gv[1], -cN * gv[0] * gv[2], -cN * gv[1] * gv[2]};
+          const T gg[6] = {-(cs.cN * gv[0] * gv[0] + cs.cI), -(cs.cN * gv[1] * gv[1] + cs.cI),
+                           -(cs.cN * gv[2] * gv[2] + cs.cI), -cs.cN * gv[0] * gv[1], -cs.cN * gv[0] * gv[2],
+                           -cs.cN * gv[1] * gv[2]};
 #pragma unroll
           for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -734,9 +776,10 @@ __device__ __forceinline__ double env_ld(const double* p) {
 
 // StopTest (stopping_criterion.h:53-98): scaling from b, status of `latest`.
 // Returns kCgActive, kCgConverged or kCgDiverged.
-__device__ __forceinline__ int stop_test(const CgConfig& cfg, double start, double latest) {
+__device__ __forceinline__ int stop_test(const CgConfig& cfg, int e, double start, double latest) {
   const double div = (start * cfg.rel_div_tol) * cfg.rel_div_tol;
-  const double c1 = cfg.abs_tol * cfg.abs_tol, c2 = (start * cfg.rel_tol) * cfg.rel_tol;
+  const double rel = cfg.env_rel_tol ? cfg.env_rel_tol[e] : cfg.rel_tol;
+  const double c1 = cfg.abs_tol * cfg.abs_tol, c2 = (start * rel) * rel;
   const double conv = c1 > c2 ? c1 : c2;
   if (!(latest == latest) || isinf(latest) || latest > div || latest < -conv) return kCgDiverged;
   if ((latest < 0.0 ? 0.0 : latest) <= conv) return kCgConverged;
@@ -745,8 +788,9 @@ __device__ __forceinline__ int stop_test(const CgConfig& cfg, double start, doub
 
 // Status of an environment before update k (pcg.h:186-222): stop test on
 // r_k, a vanishing r.z, then the iteration budget.
-__device__ __forceinline__ int cg_decide(const CgConfig& cfg, int k, double start, double latest, double gamma) {
-  int st = stop_test(cfg, start, latest);
+__device__ __forceinline__ int cg_decide(const CgConfig& cfg, int e, int k, double start, double latest,
+                                         double gamma) {
+  int st = stop_test(cfg, e, start, latest);
   if (st == kCgActive && k >= 1 && gamma == 0.0) st = kCgDiverged;
   if (st == kCgActive && k >= cfg.max_iter) st = kCgStopped;
   return st;
@@ -931,7 +975,7 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
   const double start = slot_reduce(w_start);
   const double latest = slot_reduce(w_latest);
   const double gamma = slot_reduce(w_gamma);
-  const int st = cg_decide(cfg, k, start, latest, gamma);
+  const int st = cg_decide(cfg, e, k, start, latest, gamma);
   const bool lead = tile == ta.first[e];
   if (st != kCgActive) {
     if (lead && threadIdx.x == 0) {
@@ -1043,7 +1087,7 @@ __device__ __forceinline__ void cg_update_tile(int tile, const Tiles& tv, const 
   const double start = slot_reduce(w_start);
   const double latest = slot_reduce(w_latest);
   const double gamma = slot_reduce(w_gamma);
-  if (cg_decide(cfg, k, start, latest, gamma) != kCgActive) return;  // recorded by cg_spmv(k)
+  if (cg_decide(cfg, e, k, start, latest, gamma) != kCgActive) return;  // recorded by cg_spmv(k)
   const double rho = slot_reduce(w_rho);
   const double delta = slot_reduce(w_delta);
   double beta = 0.0, mu = delta;
@@ -1484,6 +1528,147 @@ __global__ void __launch_bounds__(kVecThreads) k_env_norms(Tiles tv, int nEnv, c
   acc_add<double, 2>(part, acc_ptr(acc, 0, nEnv, e), tile - tv.first[e]);
 }
 
+// Per-environment a.b (FP64) added into set 0 of acc (zeroed by the caller).
+template <class T>
+__global__ void __launch_bounds__(kVecThreads) k_env_dot(Tiles tv, int nEnv, const T* __restrict__ a,
+                                                         const T* __restrict__ b, double* acc) {
+  const int tile = blockIdx.x;
+  const int e = tv.env[tile];
+  const int i = tv.row0[tile] + threadIdx.x;
+  double part[1] = {0.0};
+  if (i < tv.row1[tile])
+#pragma unroll
+    for (int d = 0; d < 3; ++d) part[0] += double(a[3 * i + d]) * double(b[3 * i + d]);
+  acc_add<double, 1>(part, acc_ptr(acc, 0, nEnv, e), tile - tv.first[e]);
+}
+
+// v of every lane added into environment env's accumulator of set 0: with
+// one atomic per wave when every lane of the wave holds the same environment
+// (lanes past the end hold v = 0 and lane 0's environment), else per lane.
+// Valid lanes are a prefix of the wave.
+__device__ __forceinline__ void env_add_wave(double* acc, int nEnv, int env, bool valid, double v) {
+  const int lane = int(threadIdx.x) & 63;
+  const int e0 = __shfl(env, 0, 64);
+  if (!valid) {
+    env = e0;
+    v = 0.0;
+  }
+  if (e0 < 0) return;  // no valid lane in this wave
+  const int slot = (int(blockIdx.x) * int(blockDim.x >> 6) + int(threadIdx.x >> 6)) & (kSlots - 1);
+  if (__all(env == e0)) {
+    v = wave_sum(v);
+    if (lane == 0) atomicAdd(acc_ptr(acc, 0, nEnv, e0) + slot * kSlotStride, v);
+  } else if (v != 0.0) {
+    atomicAdd(acc_ptr(acc, 0, nEnv, env) + slot * kSlotStride, v);
+  }
+}
+
+// Objective of a backward-Euler step (the merit of the objective-based line
+// searches, newton_solver.cpp:664-686; line_search.h:125-290, 338-390) of the
+// environments with env_mask[e], summed per environment in FP64 into set 0 of
+// acc (zeroed by the caller).  Per tet: the stress energy vol psi(F)
+// (fem_stress.h), gravity -rho vol g.(X_c + u_c) at the centroid (the Q1
+// rule of fem_gravity.h:93-111), inertia 1/(2 dt^2) sum_ab m_ab
+// (u - x~)_a.(u - x~)_b with x~ = u0 + dt v0, and mass damping alpha/(2 dt)
+// sum_ab m_ab (u - u0)_a.(u - u0)_b, with the consistent mass
+// m_ab = rho vol / 20 (1 + delta_ab) (fem_inertia.h:55-130, exact for the Q4
+// rule).  Per boundary-triangle sample: w times the contact energy density of
+// contact_sample (contact_utils.h:560, :935; fem_traction.h:130-135).
+// Blocks [0, n_tet_blocks) take tets, the rest triangles.
+template <class T>
+__global__ void __launch_bounds__(256) k_energy(MeshView m, FieldPtrs<T> f, const T* __restrict__ u,
+                                                const T* __restrict__ u0, const T* __restrict__ v0, T dt, double gx,
+                                                double gy, double gz, int n_tet_blocks, double* acc) {
+  if (int(blockIdx.x) < n_tet_blocks) {
+    const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const bool in = t < m.nT;
+    const int env = in ? m.tet_env[t] : -1;
+    const bool valid = in && m.env_mask[env];
+    double e = 0.0;
+    if (valid) {
+      const int nT = m.nT;
+      const int4 tv = reinterpret_cast<const int4*>(m.tets)[t];
+      const int nd[4] = {tv.x, tv.y, tv.z, tv.w};
+      const T* g = f.geom;
+      V3<T> dN[4];
+      dN[1] = mk3(g[0 * nT + t], g[1 * nT + t], g[2 * nT + t]);
+      dN[2] = mk3(g[3 * nT + t], g[4 * nT + t], g[5 * nT + t]);
+      dN[3] = mk3(g[6 * nT + t], g[7 * nT + t], g[8 * nT + t]);
+      dN[0] = scl3(T(-1), add3(add3(dN[1], dN[2]), dN[3]));
+      const double vol = double(g[kGVol * nT + t]), mu = double(g[kGMu * nT + t]);
+      const double lam = double(g[kGLambda * nT + t]), mass = double(g[kGMass * nT + t]);
+      const double alpha = double(g[kGAlpha * nT + t]);
+      M33<T> F;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) F.m[k] = (k % 4 == 0) ? T(1) : T(0);
+      double s1 = 0.0, sd1 = 0.0, sum[3] = {0.0, 0.0, 0.0}, sumd[3] = {0.0, 0.0, 0.0}, xc[3] = {0.0, 0.0, 0.0};
+      const double dtd = double(dt);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const T d[3] = {dN[a].x, dN[a].y, dN[a].z};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const T ua = u[3 * nd[a] + r];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) F.m[r * 3 + c] += ua * d[c];
+          const double us = double(u0[3 * nd[a] + r]);
+          const double del = double(ua) - (us + dtd * double(v0[3 * nd[a] + r]));  // u - x~
+          const double dd = double(ua) - us;                                          // u - u0
+          s1 += del * del;
+          sd1 += dd * dd;
+          sum[r] += del;
+          sumd[r] += dd;
+          xc[r] += 0.25 * (double(f.X[3 * nd[a] + r]) + double(ua));
+        }
+      }
+      const double m20 = mass / 20.0;
+      e = vol * smith_energy<T>(F, mu, lam);
+      e -= mass * (gx * xc[0] + gy * xc[1] + gz * xc[2]);
+      e += 0.5 / (dtd * dtd) * m20 * (s1 + sum[0] * sum[0] + sum[1] * sum[1] + sum[2] * sum[2]);
+      if (alpha > 0.0)
+        e += 0.5 * alpha / dtd * m20 * (sd1 + sumd[0] * sumd[0] + sumd[1] * sumd[1] + sumd[2] * sumd[2]);
+    }
+    env_add_wave(acc, m.nEnv, env, valid, e);
+    return;
+  }
+  const int tr = int((blockIdx.x - unsigned(n_tet_blocks)) * blockDim.x + threadIdx.x);
+  const bool in = tr < m.nF;
+  const int env = in ? m.tri_env[tr] : -1;
+  const bool valid = in && m.env_mask[env];
+  double e = 0.0;
+  if (valid) {
+    const int c0 = m.coll_first[env], c1 = m.coll_first[env + 1];
+    if (c1 > c0) {
+      const int nd[3] = {m.tris[3 * tr], m.tris[3 * tr + 1], m.tris[3 * tr + 2]};
+      V3<T> xs[3], Xr[3], dxs[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        Xr[i] = mk3(f.X[3 * nd[i]], f.X[3 * nd[i] + 1], f.X[3 * nd[i] + 2]);
+        const V3<T> ui = mk3(u[3 * nd[i]], u[3 * nd[i] + 1], u[3 * nd[i] + 2]);
+        xs[i] = add3(Xr[i], ui);
+        dxs[i] = sub3(ui, mk3(u0[3 * nd[i]], u0[3 * nd[i] + 1], u0[3 * nd[i] + 2]));
+      }
+      const V3<T> n0 = cross3(sub3(Xr[1], Xr[0]), sub3(Xr[2], Xr[0]));
+      const double w = 0.5 * double(dsqrt(dot3(n0, n0))) / 3.0;
+      V3<T> fnv = cross3(sub3(xs[1], xs[0]), sub3(xs[2], xs[0]));
+      fnv = scl3(T(1) / dsqrt(dot3(fnv, fnv)), fnv);
+      const double* cp = f.contact + kContactParams * env;
+      for (int c = c0; c < c1; ++c) {
+        const double* cd = m.coll + kCollStride * c;
+        for (int q = 0; q < 3; ++q) {
+          T N[3];
+          tri_q3<T>(q, N);
+          const V3<T> x = add3(add3(scl3(N[0], xs[0]), scl3(N[1], xs[1])), scl3(N[2], xs[2]));
+          const V3<T> dxq = add3(add3(scl3(N[0], dxs[0]), scl3(N[1], dxs[1])), scl3(N[2], dxs[2]));
+          ContactSample<T> cs;
+          if (contact_sample<T>(x, dxq, fnv, cd, cp, dt, true, cs)) e += w * cs.e;
+        }
+      }
+    }
+  }
+  env_add_wave(acc, m.nEnv, env, valid, e);
+}
+
 // Sum the kSlots lines of set 0 of each environment: out[e * nv + k].
 template <class A>
 __global__ void k_acc_finalize(int nEnv, int nv, const A* __restrict__ acc, double* __restrict__ out) {
@@ -1769,6 +1954,21 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
   hipLaunchKernelGGL(k_env_norms<T>, dim3(tv.n), dim3(kVecThreads), 0, s, tv, nEnv, a, b, acc);
 }
 
+template <class T>
+void launch_env_dot(hipStream_t s, const Tiles& tv, int nEnv, const T* a, const T* b, double* acc) {
+  if (tv.n == 0) return;
+  hipLaunchKernelGGL(k_env_dot<T>, dim3(tv.n), dim3(kVecThreads), 0, s, tv, nEnv, a, b, acc);
+}
+
+template <class T>
+void launch_energy(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0, const T* v0,
+                   double dt, const double grav[3], double* acc) {
+  const int nbt = int(blocks_for(m.nT, 256)), nbf = int(blocks_for(m.nF, 256));
+  if (nbt + nbf == 0) return;
+  hipLaunchKernelGGL(k_energy<T>, dim3(unsigned(nbt + nbf)), dim3(256), 0, s, m, f, u, u0, v0, T(dt), grav[0],
+                     grav[1], grav[2], nbt, acc);
+}
+
 #define SDX_INST_T(T)                                                                                            \
   template void launch_element<T>(hipStream_t, const MeshView&, const AsmRange&, const FieldPtrs<T>&, const T*, \
                                   const T*, const T*, double, const double*, int, int, T*, T*, T*, int,         \
@@ -1786,6 +1986,9 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
                                     T*, T*, int);                                                               \
   template void launch_env_copy<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, const T*, T*);        \
   template void launch_env_norms<T>(hipStream_t, const Tiles&, int, const T*, const T*, double*);              \
+  template void launch_env_dot<T>(hipStream_t, const Tiles&, int, const T*, const T*, double*);                \
+  template void launch_energy<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, const T*,         \
+                                 const T*, double, const double*, double*);                                     \
   template void launch_fill<T>(hipStream_t, int64_t, T, T*);                                                    \
   template void launch_acc_finalize<T>(hipStream_t, int, int, const T*, double*);                               \
   template void launch_cg_reset<T>(hipStream_t, int, const int32_t*, EnvCG*, T*);

@@ -295,6 +295,26 @@ __device__ __forceinline__ void smith_mode(const T la[3], const V3<T> ae[3], con
     }
 }
 
+// Smith stable Neo-Hookean energy density (batched_smith_neo_hookean.cpp:51-68)
+//   psi = mu^/2 (Ic - 3) + lambda^/2 (J - alpha)^2 - mu^/2 ln(Ic + 1),
+//   mu^ = 4/3 mu, lambda^ = lambda + 5/6 mu, alpha = 1 + 3/4 mu^/lambda^,
+// evaluated in FP64 from F.
+template <class T>
+__device__ __forceinline__ double smith_energy(const M33<T>& F, double mu, double lambda) {
+  double f[9], Ic = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    f[i] = double(F.m[i]);
+    Ic += f[i] * f[i];
+  }
+  const double J = f[0] * (f[4] * f[8] - f[5] * f[7]) - f[1] * (f[3] * f[8] - f[5] * f[6]) +
+                   f[2] * (f[3] * f[7] - f[4] * f[6]);
+  const double muHat = mu * (4.0 / 3.0), lambdaHat = lambda + mu * (5.0 / 6.0);
+  const double alpha = 1.0 + 0.75 * muHat / lambdaHat;
+  const double Jma = J - alpha;
+  return 0.5 * muHat * (Ic - 3.0) + 0.5 * lambdaHat * Jma * Jma - 0.5 * muHat * log(Ic + 1.0);
+}
+
 // Element kernel core: stress residual (12) and, if wantH, the 10 upper 3x3
 // blocks of the Hessian of one tet, both scaled by the Q1 weight `vol`; each
 // block goes to emit(f, g, blk) as soon as it is formed (f == g: 6 entries

@@ -484,6 +484,10 @@ void World<T>::finalize() {
   reset_.alloc(static_cast<size_t>(nEnv));
   env_sq_.alloc(static_cast<size_t>(nEnv));
   norms2_.alloc(size_t(2) * size_t(nEnv));
+  env_tol_.alloc(static_cast<size_t>(nEnv));
+  env_obj_.alloc(static_cast<size_t>(nEnv));
+  env_dot_.alloc(static_cast<size_t>(nEnv));
+  res0_.alloc(n3);
   finalized_ = true;
   push_state();
   colliders_dirty_ = true;
@@ -606,8 +610,7 @@ void World<T>::set_env_mask(const std::vector<int32_t>& mask) {
 // records of one group (sized to stay in the last-level cache) are all that
 // is live, and the next group reuses the same buffers.
 template <class T>
-void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH) {
-  const int project = sp.psd_proj_mode == SDX_PSD_ALWAYS ? 1 : 0;
+void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH, int project) {
   const int64_t set0 = int64_t(nEnv_) * kSlots * kSlotStride;
   const bool dd = precision_ != SDX_PREC_MODE_FP32;
   if (dd)
@@ -635,6 +638,40 @@ void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH) {
     launch_acc_finalize<double>(stream_, nEnv_, 1, acc_.p, env_sq_.p);
   else
     launch_acc_finalize<T>(stream_, nEnv_, 1, reinterpret_cast<T*>(acc_.p), env_sq_.p);
+}
+
+// DResidual of the environments in mask, in up to two assemblies: the
+// projected ones (proj[e]) and the others.  Leaves env_mask_ at the last
+// subset assembled.
+template <class T>
+void World<T>::assemble_h(double dt, const sdx_solver_params& sp, const std::vector<int32_t>& mask,
+                          const std::vector<uint8_t>& proj) {
+  for (int pr = 1; pr >= 0; --pr) {
+    std::vector<int32_t> m(size_t(nEnv_), 0);
+    bool any = false;
+    for (int e = 0; e < nEnv_; ++e)
+      if (mask[size_t(e)] && int(proj[size_t(e)] != 0) == pr) {
+        m[size_t(e)] = 1;
+        any = true;
+      }
+    if (!any) continue;
+    set_env_mask(m);
+    assemble(dt, sp, true, pr);
+  }
+}
+
+template <class T>
+void World<T>::queue_energy(double dt, const sdx_solver_params& sp) {
+  launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
+  launch_energy<T>(stream_, mesh_, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, acc_.p);
+  launch_acc_finalize<double>(stream_, nEnv_, 1, acc_.p, env_obj_.p);
+}
+
+template <class T>
+void World<T>::queue_env_dot(const T* a, const T* b) {
+  launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
+  launch_env_dot<T>(stream_, dt_v_.view, nEnv_, a, b, acc_.p);
+  launch_acc_finalize<double>(stream_, nEnv_, 1, acc_.p, env_dot_.p);
 }
 
 // Record pointer of a group whose first record is rec0 (W values each) for
@@ -1051,7 +1088,7 @@ std::vector<double> World<T>::true_rel_from(const std::vector<int32_t>& mask, co
 template <class T>
 std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve,
                                       std::vector<double>* true_rel, const std::vector<int32_t>* warm_in,
-                                      WarmStats* learned) {
+                                      WarmStats* learned, const std::vector<double>* env_tol) {
   CgConfig cfg{};
   cfg.n_env = nEnv_;
   cfg.n_rows = nV_;
@@ -1063,6 +1100,11 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
   cfg.abs_tol = kp.abs_tol;
   cfg.rel_tol = kp.rel_tol;
   cfg.rel_div_tol = kp.rel_div_tol;
+  if (env_tol) {  // the values live on the device: the captured graphs stay valid across Newton iterations
+    put(env_tol_, *env_tol);
+    cfg.env_rel_tol = env_tol_.p;
+  }
+  auto rel_tol_of = [&](int e) { return env_tol ? (*env_tol)[size_t(e)] : kp.rel_tol; };
   // mixed precision: FP32 increments of the FP64 iterate between refreshes;
   // without refreshes every update adds into the FP64 iterate itself
   cfg.xd = (precision_ == SDX_PREC_MODE_MIXED && std::is_same<T, float>::value && cfg.refresh_every > 0)
@@ -1101,7 +1143,7 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
     int n = 0;
     for (int e = 0; e < nEnv_; ++e) {
       const EnvCG& c = h[size_t(e)];
-      const bool redo = todo[size_t(e)] && c.status == kCgConverged && tr[size_t(e)] > kp.rel_tol &&
+      const bool redo = todo[size_t(e)] && c.status == kCgConverged && tr[size_t(e)] > rel_tol_of(e) &&
                         tr[size_t(e)] * std::sqrt(c.start) > kp.abs_tol && c.iter < kp.max_iter;
       todo[size_t(e)] = redo ? 1 : 0;
       warm[size_t(e)] = todo[size_t(e)];
@@ -1212,6 +1254,53 @@ void World<T>::collected_device(const float** x, const float** t, int64_t* n) {
   *n = coll_n_;
 }
 
+// Forcing term of the linear solve of Newton iteration k
+// (NewtonSolver::SetLinearSolverRelativeTolerance, newton_solver.cpp:426-501):
+// Eisenstat-Walker choices 1 and 2 from 0.01, their safeguard only above
+// 0.01, clamped to [100 eps(T), 0.01]; choice 3 max(min(1/(2 + k),
+// sqrt(|r|)), relTol); Constant keeps the previous value.
+template <class T>
+static double forcing_term(int strategy, int k, double res, double prev_res, double lin_res, double prev_eta,
+                           double rel_tol) {
+  const double kMaxEta = 0.01, kSafeguardEtaTh = 0.01;
+  const double kMinEta = 100.0 * double(std::numeric_limits<T>::epsilon());
+  const double alpha = 0.5 + 0.5 * std::sqrt(5.0);
+  auto clamp = [&](double eta) { return std::min(std::max(eta, kMinEta), kMaxEta); };
+  switch (strategy) {
+    case SDX_LINTOL_EW1: {
+      if (k == 0) return kMaxEta;
+      const double safeguard = std::pow(prev_eta, alpha);
+      double eta = std::fabs(res - lin_res) / prev_res;
+      if (safeguard > kSafeguardEtaTh) eta = std::max(eta, safeguard);
+      return clamp(eta);
+    }
+    case SDX_LINTOL_EW2: {
+      if (k == 0) return kMaxEta;
+      const double safeguard = 0.9 * std::pow(prev_eta, alpha);
+      double eta = 0.9 * std::pow(res / prev_res, alpha);
+      if (safeguard > kSafeguardEtaTh) eta = std::max(eta, safeguard);
+      return clamp(eta);
+    }
+    case SDX_LINTOL_EW3:
+      return std::max(std::min(1.0 / (2.0 + double(k)), std::sqrt(res)), rel_tol);
+    default:
+      return prev_eta;
+  }
+}
+
+// One backward-Euler step of every environment: NewtonSolver<T>::Solve
+// (newton_solver.cpp:603-876) run in lockstep over the environments, each
+// with its own stop decisions, forcing terms, tries and line search.
+//
+// Tries (:736-860), with the default fitted-Hessian settings (joint friction
+// off: the first try of an iteration is never its last): a try whose linear
+// solve does not converge, or whose line search finds no improvement, is
+// undone and retried once with the PSD-projected DResidual; the retry keeps
+// its last line-search iterate or, with the gradient-descent fallback,
+// searches along -res from where the iteration started.  Where the first try
+// was already projected (PSD mode Always), the retry would repeat the same
+// solve and search exactly: it is not run again, its iterations are counted
+// and its outcome taken from the first.
 template <class T>
 void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stats, int n_stats) {
   if (!(dt > 0)) throw std::invalid_argument("dt must be positive");
@@ -1225,35 +1314,44 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
   launch_copy<T>(stream_, int64_t(n3), v_.p, v0_.p);
   launch_apply_pins<T>(stream_, int64_t(n3), d_pinned_.p, d_pin_value_.p, u_.p);
 
-  struct EnvState {
-    bool active = true;
-    int iters = 0;
-    double res0 = 0, res = 0;
-    int conv = SDX_CONV_NONE;
-    int lin_iters = 0, ls_iters = 0, lin_conv = SDX_CONV_NONE, refreshes = 0;
-    double lin_rel = 0, lin_true_rel = 0;
-    int ws_tried = 0, ws_accepted = 0;
-    double ws_ratio = 0;
-    double dx_norm = std::numeric_limits<double>::infinity();
-    double ls_res0 = 0;
-    bool have_dx = false;
-  };
+  const int ls_type = sp.line_search_type;
+  const bool need_obj = ls_type != SDX_LS_NONE && ls_type != SDX_LS_RESIDUAL_NORM;
+  const bool gd = sp.gradient_descent_fallback != 0;
+  const bool adaptive_tol = sp.linear_tolerance_strategy != SDX_LINTOL_CONSTANT;
   std::vector<EnvState> S(static_cast<size_t>(nE));
+  for (EnvState& s : S) {
+    s.psd = sp.psd_proj_mode;
+    s.eta = sp.linear.rel_tol;
+  }
   std::vector<int32_t> mask(size_t(nE), 1);
-  bool res_valid = false;  // res_ and S.res hold the residual at the current u
+  std::vector<uint8_t> proj(size_t(nE), 0);
   // |dx| is needed by the RelStepNorm test of the next Newton iteration only
   const bool need_dx = sp.newton_max_iter > 1 && sp.rel_step_tol > 0;
+  bool first_iter = true;
   while (true) {
     bool h_valid = false;
     for (int e = 0; e < nE; ++e) mask[size_t(e)] = S[size_t(e)].active ? 1 : 0;
-    if (!res_valid) {
-      // first Newton iteration: residual + dresidual (newton_solver.cpp:667-686)
+    if (first_iter) {
+      // residual + dresidual (newton_solver.cpp:667-686); every environment
+      // starts in the same PSD mode
       set_env_mask(mask);
-      assemble(dt, sp, true);
+      assemble(dt, sp, true, sp.psd_proj_mode == SDX_PSD_ALWAYS ? 1 : 0);
       h_valid = true;
       auto sq = read_env_sq();
+      for (int e = 0; e < nE; ++e) {
+        S[size_t(e)].res = std::sqrt(sq[size_t(e)]);
+        S[size_t(e)].res_known = true;
+      }
+      first_iter = false;
+    }
+    if (need_obj) {  // the merit at the current iterate
+      set_env_mask(mask);
+      queue_energy(dt, sp);
+      std::vector<double> ob(static_cast<size_t>(nE));
+      fetch(env_obj_, ob);
+      fetch_all();
       for (int e = 0; e < nE; ++e)
-        if (S[size_t(e)].active) S[size_t(e)].res = std::sqrt(sq[size_t(e)]);
+        if (S[size_t(e)].active) S[size_t(e)].merit = ob[size_t(e)];
     }
     // stop criteria (newton_solver.cpp:304-314): ResNorm, MaxIters, RelStepNorm
     bool need_norms = false;
@@ -1290,134 +1388,186 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
     }
     if (n_active == 0) break;
     for (int e = 0; e < nE; ++e) mask[size_t(e)] = S[size_t(e)].active ? 1 : 0;
-    set_env_mask(mask);
-    if (!h_valid) assemble(dt, sp, true);
     int newton_it = 0;  // every active environment is in the same Newton iteration
-    for (int e = 0; e < nE; ++e)
-      if (S[size_t(e)].active) newton_it = S[size_t(e)].iters++;
+    std::vector<double> env_tol(size_t(nE), sp.linear.rel_tol);
+    for (int e = 0; e < nE; ++e) {
+      EnvState& s = S[size_t(e)];
+      if (!s.active) continue;
+      s.eta = forcing_term<T>(sp.linear_tolerance_strategy, s.iters, s.res, s.prev_res, s.lin_res, s.eta,
+                              sp.linear.rel_tol);
+      env_tol[size_t(e)] = s.eta;
+      s.prev_res = s.res;
+      s.res_start = s.res;
+      s.merit0 = s.merit;
+      newton_it = s.iters++;
+    }
     hist_k_ = newton_it;  // the warm start draws on the steps of this Newton iteration
-    // linear solve, solution in dx_: from zero, or from the learned warm
-    // start where its guess lowers the residual.  A Newton iteration whose
-    // guesses most environments rejected in its last warm-started solve
-    // starts from zero for the next kWarmSkipSteps steps (the network,
-    // features and guess residuals cost more than the few kept guesses save)
-    const int hk = std::min(newton_it, kNewtonHist - 1);
-    bool learned = sp.warm_start == SDX_WARM_START_LEARNED && ws_on_;
-    if (learned && ws_skip_[hk] > 0) {
-      --ws_skip_[hk];
-      learned = false;
-    }
-    const bool collecting = coll_per_env_ > 0 && coll_n_ + int64_t(nE) * coll_per_env_ <= coll_cap_;
-    if (learned || collecting) ws_features();
-    if (learned)
-      launch_mlp_infer<T>(stream_, ws_w_, ws_F_.p, nV_, d_row_env_.p, ws_scale_.p, d_pinned_.p, nullptr, dx_.p);
-    std::vector<double> true_rel;
-    WarmStats ws;
-    auto cg = cg_solve(sp.linear, mask, &true_rel, learned ? &mask : nullptr, learned ? &ws : nullptr);
-    if (collecting) {  // (features, Newton step / scale) pairs of this solve
-      launch_ws_collect<T>(stream_, nE, d_env_v0_.p, env_mask_.p, coll_per_env_, coll_salt_++, ws_F_.p, dx_.p,
-                           ws_scale_.p, coll_n_, coll_x_.p, coll_t_.p);
-      coll_n_ += int64_t(nE) * coll_per_env_;
-    }
-    launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, hist1(), hist2());
-    launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, dx_.p, hist1());
-    if (learned) {
-      int tried = 0, kept = 0;
-      for (int e = 0; e < nE; ++e)
-        if (S[size_t(e)].active) {
-          ++tried;
-          kept += ws.accepted[size_t(e)] ? 1 : 0;
-        }
-      // the first two warm-started solves of an iteration index run before
-      // its history of previous Newton steps is filled, and never skip
-      if (++ws_seen_[hk] > 2 && double(kept) < kWarmMinKept * double(tried)) ws_skip_[hk] = kWarmSkipSteps;
-    }
-    std::vector<int32_t> lin_ok(size_t(nE), 0);
-    for (int e = 0; e < nE; ++e) {
-      EnvState& s = S[size_t(e)];
-      if (!s.active) continue;
-      const EnvCG& c = cg[size_t(e)];
-      if (learned) {
-        s.ws_tried++;
-        s.ws_accepted += ws.accepted[size_t(e)] ? 1 : 0;
-        s.ws_ratio = ws.ratio[size_t(e)];
-      }
-      s.lin_iters += c.iter;
-      s.lin_conv = c.status == kCgConverged ? SDX_CONV_CONVERGED
-                   : c.status == kCgDiverged ? SDX_CONV_DIVERGED
-                                             : SDX_CONV_STOPPED;
-      s.lin_rel = c.start > 0 ? std::sqrt(c.latest / c.start) : 0.0;
-      s.lin_true_rel = true_rel[size_t(e)];
-      s.refreshes += c.refreshes;
-      s.have_dx = true;
-      lin_ok[size_t(e)] = s.lin_conv == SDX_CONV_CONVERGED;
-    }
-    if (need_dx) {
-      launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
-      launch_env_norms<T>(stream_, dt_v_.view, nEnv_, dx_.p, dx_.p, acc_.p);
-      launch_acc_finalize<double>(stream_, nEnv_, 2, acc_.p, norms2_.p);
-      std::vector<double> m2(size_t(2 * nE));
-      get(norms2_, m2);
-      for (int e = 0; e < nE; ++e)
-        if (S[size_t(e)].active) S[size_t(e)].dx_norm = std::sqrt(m2[size_t(2 * e)]);
-    }
-    // line search along -dx (line_search.h:300-335)
-    for (int e = 0; e < nE; ++e) S[size_t(e)].ls_res0 = S[size_t(e)].res;
     launch_copy<T>(stream_, int64_t(n3), u_.p, sol0_.p);
-    std::vector<double> scale(size_t(nE), 1.0);
-    std::vector<int32_t> searching(mask);
-    std::vector<int> accepted_it(size_t(nE), -1);
-    const bool ls_on = sp.line_search_type == SDX_LS_RESIDUAL_NORM;
-    const int ls_max = ls_on ? std::max(1, sp.line_search_max_iter) : 1;
-    for (int it = 0; it < ls_max; ++it) {
-      if (it > 0)
+    if (gd) launch_copy<T>(stream_, int64_t(n3), res_.p, res0_.p);
+    for (int e = 0; e < nE; ++e) proj[size_t(e)] = S[size_t(e)].psd == SDX_PSD_ALWAYS ? 1 : 0;
+    std::vector<int32_t> trying(mask);
+    std::vector<uint8_t> retried(size_t(nE), 0);
+    bool first_try = true;
+    while (true) {
+      int n_try = 0;
+      for (int e = 0; e < nE; ++e) n_try += trying[size_t(e)] ? 1 : 0;
+      if (n_try == 0) break;
+      // the DResidual at the iterate the iteration started from (assembled with the residual on
+      // the first iteration)
+      if (!h_valid) assemble_h(dt, sp, trying, proj);
+      h_valid = false;
+      set_env_mask(trying);
+      // linear solve, solution in dx_: from zero, or on the first try from the learned warm start
+      // where its guess lowers the residual.  A Newton iteration whose guesses most environments
+      // rejected in its last warm-started solve starts from zero for the next kWarmSkipSteps steps
+      // (the network, features and guess residuals cost more than the few kept guesses save)
+      const int hk = std::min(newton_it, kNewtonHist - 1);
+      bool learned = first_try && sp.warm_start == SDX_WARM_START_LEARNED && ws_on_;
+      if (learned && ws_skip_[hk] > 0) {
+        --ws_skip_[hk];
+        learned = false;
+      }
+      const bool collecting = first_try && coll_per_env_ > 0 && coll_n_ + int64_t(nE) * coll_per_env_ <= coll_cap_;
+      if (learned || collecting) ws_features();
+      if (learned)
+        launch_mlp_infer<T>(stream_, ws_w_, ws_F_.p, nV_, d_row_env_.p, ws_scale_.p, d_pinned_.p, nullptr, dx_.p);
+      std::vector<double> true_rel;
+      WarmStats ws;
+      auto cg = cg_solve(sp.linear, trying, &true_rel, learned ? &trying : nullptr, learned ? &ws : nullptr,
+                         adaptive_tol ? &env_tol : nullptr);
+      if (collecting) {  // (features, Newton step / scale) pairs of this solve
+        launch_ws_collect<T>(stream_, nE, d_env_v0_.p, env_mask_.p, coll_per_env_, coll_salt_++, ws_F_.p, dx_.p,
+                             ws_scale_.p, coll_n_, coll_x_.p, coll_t_.p);
+        coll_n_ += int64_t(nE) * coll_per_env_;
+      }
+      if (first_try) {
+        launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, hist1(), hist2());
+        launch_env_copy<T>(stream_, nV_, d_row_env_.p, env_mask_.p, dx_.p, hist1());
+      }
+      if (learned) {
+        int tried = 0, kept = 0;
         for (int e = 0; e < nE; ++e)
-          if (searching[size_t(e)]) scale[size_t(e)] *= sp.line_search_alpha;
-      put(ls_scale_, scale);
-      put(ls_mask_, searching);
-      launch_ls_update<T>(stream_, nV_, d_row_env_.p, ls_mask_.p, ls_scale_.p, sol0_.p, dx_.p, u_.p);
-      // residual at the trial point (also the explosion-control residual and
-      // the next iteration's residual: same u, same assembly)
-      set_env_mask(searching);
-      assemble(dt, sp, false);
-      auto sq = read_env_sq();
-      int still = 0;
+          if (trying[size_t(e)]) {
+            ++tried;
+            kept += ws.accepted[size_t(e)] ? 1 : 0;
+          }
+        // the first two warm-started solves of an iteration index run before
+        // its history of previous Newton steps is filled, and never skip
+        if (++ws_seen_[hk] > 2 && double(kept) < kWarmMinKept * double(tried)) ws_skip_[hk] = kWarmSkipSteps;
+      }
+      std::vector<uint8_t> lin_ok(size_t(nE), 0);
       for (int e = 0; e < nE; ++e) {
-        if (!searching[size_t(e)]) continue;
-        const double rn = std::sqrt(sq[size_t(e)]);
-        S[size_t(e)].res = rn;  // residual at the current trial
-        if (!ls_on || rn <= S[size_t(e)].ls_res0) {
-          accepted_it[size_t(e)] = it;
-          searching[size_t(e)] = 0;
-        } else {
-          ++still;
+        EnvState& s = S[size_t(e)];
+        if (!trying[size_t(e)]) continue;
+        const EnvCG& c = cg[size_t(e)];
+        if (learned) {
+          s.ws_tried++;
+          s.ws_accepted += ws.accepted[size_t(e)] ? 1 : 0;
+          s.ws_ratio = ws.ratio[size_t(e)];
+        }
+        s.lin_iters += c.iter;
+        s.lin_conv = c.status == kCgConverged ? SDX_CONV_CONVERGED
+                     : c.status == kCgDiverged ? SDX_CONV_DIVERGED
+                                               : SDX_CONV_STOPPED;
+        s.lin_rel = c.start > 0 ? std::sqrt(c.latest / c.start) : 0.0;
+        s.lin_res = std::sqrt(std::max(c.latest, 0.0));
+        s.lin_true_rel = true_rel[size_t(e)];
+        s.refreshes += c.refreshes;
+        s.have_dx = true;
+        lin_ok[size_t(e)] = s.lin_conv == SDX_CONV_CONVERGED;
+      }
+      if (need_dx) {
+        launch_fill<double>(stream_, int64_t(nEnv_) * kSlots * kSlotStride, 0.0, acc_.p);
+        launch_env_norms<T>(stream_, dt_v_.view, nEnv_, dx_.p, dx_.p, acc_.p);
+        launch_acc_finalize<double>(stream_, nEnv_, 2, acc_.p, norms2_.p);
+        std::vector<double> m2(size_t(2 * nE));
+        get(norms2_, m2);
+        for (int e = 0; e < nE; ++e)
+          if (trying[size_t(e)]) S[size_t(e)].dx_norm = std::sqrt(m2[size_t(2 * e)]);
+      }
+      // who steps: a converged solve, a last try, or a first try already projected (its retry
+      // would be the same solve); a projected first try whose solve failed counts that solve twice
+      std::vector<int32_t> take(size_t(nE), 0), keep_final(size_t(nE), 0);
+      for (int e = 0; e < nE; ++e) {
+        if (!trying[size_t(e)]) continue;
+        const bool last = retried[size_t(e)] != 0;
+        const bool same_retry = !last && proj[size_t(e)];
+        if (lin_ok[size_t(e)] || last || same_retry) {
+          take[size_t(e)] = 1;
+          keep_final[size_t(e)] = (last || same_retry) && !gd ? 1 : 0;
+          if (same_retry && !lin_ok[size_t(e)]) S[size_t(e)].lin_iters += cg[size_t(e)].iter;
         }
       }
-      if (still == 0) break;
-    }
-    for (int e = 0; e < nE; ++e) {
-      EnvState& s = S[size_t(e)];
-      if (!s.active) continue;
-      const bool improved = accepted_it[size_t(e)] >= 0;
-      const int lsIters = improved ? accepted_it[size_t(e)] : ls_max;
-      s.ls_iters += lsIters;
-      // a failed line search or an unconverged solve is retried once with an
-      // identical operator, which repeats the same solve and line search
-      // (newton_solver.cpp:783-854); count that work, keep the final iterate
-      if (!improved || !lin_ok[size_t(e)]) {
-        if (lin_ok[size_t(e)]) s.ls_iters += lsIters;
-        s.lin_iters += cg[size_t(e)].iter;
+      std::vector<uint8_t> improved;
+      std::vector<int> ls_it;
+      line_search(dt, sp, take, keep_final, S, improved, ls_it);
+      std::vector<int32_t> next(size_t(nE), 0), gd_set(size_t(nE), 0);
+      bool any_gd = false;
+      for (int e = 0; e < nE; ++e) {
+        if (!trying[size_t(e)]) continue;
+        EnvState& s = S[size_t(e)];
+        const bool last = retried[size_t(e)] != 0;
+        const bool same_retry = !last && proj[size_t(e)];
+        if (take[size_t(e)]) {
+          s.ls_iters += ls_it[size_t(e)];
+          if (improved[size_t(e)]) continue;
+          if (same_retry && lin_ok[size_t(e)]) {  // the identical retry: same solve, same search
+            s.lin_iters += cg[size_t(e)].iter;
+            s.ls_iters += ls_it[size_t(e)];
+          }
+          if (last || same_retry) {
+            if (gd) {
+              gd_set[size_t(e)] = 1;
+              any_gd = true;
+            }
+            continue;
+          }
+        }
+        // retry with the projected DResidual from where the iteration started
+        next[size_t(e)] = 1;
+        retried[size_t(e)] = 1;
+        proj[size_t(e)] = 1;
+        if (s.psd == SDX_PSD_IF_FAIL_ALWAYS) s.psd = SDX_PSD_ALWAYS;
       }
-      // explosion control (newton_solver.cpp:856-874)
-      if (sp.explosion_control) {
-        if (s.res > sp.rel_div_tol * s.res0 || s.res > sp.abs_div_tol || !std::isfinite(s.res)) {
+      if (any_gd) {  // gradient-descent fallback along -res from the iteration's start
+        put(ls_mask_, gd_set);
+        launch_env_copy<T>(stream_, nV_, d_row_env_.p, ls_mask_.p, res0_.p, dx_.p);
+        launch_env_copy<T>(stream_, nV_, d_row_env_.p, ls_mask_.p, res0_.p, res_.p);
+        std::vector<int32_t> all_final(size_t(nE), 1);
+        line_search(dt, sp, gd_set, all_final, S, improved, ls_it);
+        for (int e = 0; e < nE; ++e)
+          if (gd_set[size_t(e)]) S[size_t(e)].ls_iters += ls_it[size_t(e)];
+      }
+      trying = next;
+      first_try = false;
+    }
+    // the residual at the new iterate where the line search did not leave it
+    // (explosion control, newton_solver.cpp:856-874, and the next iteration)
+    std::vector<int32_t> need_res(size_t(nE), 0);
+    bool any_res = false;
+    for (int e = 0; e < nE; ++e)
+      if (S[size_t(e)].active && !S[size_t(e)].res_known) {
+        need_res[size_t(e)] = 1;
+        any_res = true;
+      }
+    if (any_res) {
+      set_env_mask(need_res);
+      assemble(dt, sp, false, 0);
+      auto sq = read_env_sq();
+      for (int e = 0; e < nE; ++e)
+        if (need_res[size_t(e)]) {
+          S[size_t(e)].res = std::sqrt(sq[size_t(e)]);
+          S[size_t(e)].res_known = true;
+        }
+    }
+    if (sp.explosion_control)
+      for (int e = 0; e < nE; ++e) {
+        EnvState& s = S[size_t(e)];
+        if (s.active && (s.res > sp.rel_div_tol * s.res0 || s.res > sp.abs_div_tol || !std::isfinite(s.res))) {
           s.conv = SDX_CONV_DIVERGED;
           s.active = false;
         }
       }
-    }
-    // the residual of each env's last line-search trial is its residual at u
-    res_valid = true;
   }
   // PostStage: v = (u - u0) / dt, reset on divergence (mochi_soft.cpp:135-163)
   std::vector<int32_t> reset(size_t(nE), 0);
@@ -1451,6 +1601,161 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       o.warm_start_accepted = s.ws_accepted;
       o.warm_start_rel_res = s.ws_ratio;
     }
+  }
+}
+
+// Line search (line_search.h:76-390) along -dx_ from sol0_ (= u_ of these
+// environments on entry, res_ the residual there) for the environments in
+// take, all rules in lockstep: each trial moves the searching environments
+// to u = sol0 - scale dx and evaluates what their rules read (residual norm,
+// objective, res.dx) in one pass.  improved[e] tells whether a trial was
+// accepted, ls_it[e] the trials rejected before it (max_iter when none).
+// Without improvement an environment keeps its last trial if keep_final[e],
+// else returns to sol0.  S[e].res, res_known and merit follow u.
+template <class T>
+void World<T>::line_search(double dt, const sdx_solver_params& sp, const std::vector<int32_t>& take,
+                           const std::vector<int32_t>& keep_final, std::vector<EnvState>& S,
+                           std::vector<uint8_t>& improved, std::vector<int>& ls_it) {
+  const int nE = nEnv_;
+  improved.assign(size_t(nE), 0);
+  ls_it.assign(size_t(nE), 0);
+  int n = 0;
+  for (int e = 0; e < nE; ++e) n += take[size_t(e)] ? 1 : 0;
+  if (n == 0) return;
+  const int type = sp.line_search_type;
+  if (type == SDX_LS_NONE) {  // (:76-87) take the step; the trial count is left as it was
+    put(ls_scale_, std::vector<double>(size_t(nE), 1.0));
+    put(ls_mask_, take);
+    launch_ls_update<T>(stream_, nV_, d_row_env_.p, ls_mask_.p, ls_scale_.p, sol0_.p, dx_.p, u_.p);
+    for (int e = 0; e < nE; ++e)
+      if (take[size_t(e)]) {
+        improved[size_t(e)] = 1;
+        S[size_t(e)].res_known = false;
+      }
+    return;
+  }
+  // each environment's rule, and its slope res.inc at the start (inc = -dx)
+  std::vector<int> rule(size_t(nE), type);
+  std::vector<double> slope(size_t(nE), 0.0);
+  if (type == SDX_LS_ARMIJO || type == SDX_LS_WOLFE_WEAK || type == SDX_LS_WOLFE_STRONG ||
+      type == SDX_LS_ARMIJO_OR_RESIDUAL_NORM) {
+    queue_env_dot(res_.p, dx_.p);
+    std::vector<double> d(static_cast<size_t>(nE));
+    fetch(env_dot_, d);
+    fetch_all();
+    for (int e = 0; e < nE; ++e) {
+      slope[size_t(e)] = -d[size_t(e)];
+      // not a descent direction: the residual-norm search (:343-348)
+      if (type == SDX_LS_ARMIJO_OR_RESIDUAL_NORM && slope[size_t(e)] > 0) rule[size_t(e)] = SDX_LS_RESIDUAL_NORM;
+    }
+  }
+  const double alpha = sp.line_search_alpha, w1 = sp.line_search_wolfe1, w2 = sp.line_search_wolfe2;
+  const double inf = std::numeric_limits<double>::infinity();
+  std::vector<double> scale(size_t(nE), 1.0), lo(size_t(nE), 0.0), hi(size_t(nE), inf);
+  std::vector<int32_t> searching(take);
+  std::vector<uint8_t> res_at_last(size_t(nE), 0);
+  const int max_it = sp.line_search_max_iter;
+  for (int it = 0; it < max_it; ++it) {
+    bool want_res = false, want_obj = false, want_dot = false, any = false;
+    for (int e = 0; e < nE; ++e) {
+      if (!searching[size_t(e)]) continue;
+      any = true;
+      const int r = rule[size_t(e)];
+      const bool wolfe = r == SDX_LS_WOLFE_WEAK || r == SDX_LS_WOLFE_STRONG;
+      want_res |= r == SDX_LS_RESIDUAL_NORM || wolfe || r == SDX_LS_ARMIJO_OR_RESIDUAL_NORM;
+      want_obj |= r != SDX_LS_RESIDUAL_NORM;
+      want_dot |= wolfe;
+    }
+    if (!any) break;
+    put(ls_scale_, scale);
+    put(ls_mask_, searching);
+    launch_ls_update<T>(stream_, nV_, d_row_env_.p, ls_mask_.p, ls_scale_.p, sol0_.p, dx_.p, u_.p);
+    set_env_mask(searching);
+    std::vector<double> sq(static_cast<size_t>(nE)), ob(static_cast<size_t>(nE)), dd(static_cast<size_t>(nE));
+    if (want_res) {
+      assemble(dt, sp, false, 0);
+      fetch(env_sq_, sq);
+    }
+    if (want_obj) {
+      queue_energy(dt, sp);
+      fetch(env_obj_, ob);
+    }
+    if (want_dot) {
+      queue_env_dot(res_.p, dx_.p);
+      fetch(env_dot_, dd);
+    }
+    fetch_all();
+    flush_timers();
+    for (int e = 0; e < nE; ++e) {
+      if (!searching[size_t(e)]) continue;
+      EnvState& s = S[size_t(e)];
+      const int r = rule[size_t(e)];
+      const double a = scale[size_t(e)], m = ob[size_t(e)];
+      if (want_res) s.res = std::sqrt(sq[size_t(e)]);
+      res_at_last[size_t(e)] = want_res ? 1 : 0;
+      bool ok = false, backtrack = true;
+      switch (r) {
+        case SDX_LS_SIMPLE:  // (:89-124)
+          ok = m <= s.merit0 + std::fabs(s.merit0 * sp.line_search_max_rel_increase);
+          break;
+        case SDX_LS_ARMIJO:  // (:126-167)
+          ok = m <= s.merit0 + a * w1 * slope[size_t(e)];
+          break;
+        case SDX_LS_WOLFE_WEAK:
+        case SDX_LS_WOLFE_STRONG: {  // (:169-289)
+          backtrack = false;
+          const double g = -a * dd[size_t(e)];  // res(x).inc with inc = -a dx
+          const double t1 = w1 * slope[size_t(e)], t2 = w2 * slope[size_t(e)];
+          const bool strong = r == SDX_LS_WOLFE_STRONG;
+          const bool bad = !std::isfinite(g) || !std::isfinite(m) || m > s.merit0 + a * t1;
+          const bool shrink = bad || (strong && g > std::fabs(t2 * a));
+          const bool grow = !shrink && (strong ? g < -std::fabs(t2 * a) : g < t2 * a);
+          if (shrink) {
+            hi[size_t(e)] = a;
+            scale[size_t(e)] = lo[size_t(e)] + alpha * (a - lo[size_t(e)]);
+          } else if (grow) {
+            lo[size_t(e)] = a;
+            scale[size_t(e)] = std::isfinite(hi[size_t(e)]) ? 0.5 * (a + hi[size_t(e)]) : a / alpha;
+          } else {
+            ok = true;
+          }
+          break;
+        }
+        case SDX_LS_ARMIJO_OR_RESIDUAL_NORM:  // (:338-390)
+          ok = m <= s.merit0 + a * w1 * slope[size_t(e)] || s.res < s.res_start;
+          break;
+        default:  // residual norm (:291-336)
+          ok = s.res <= s.res_start;
+          break;
+      }
+      if (ok) {
+        improved[size_t(e)] = 1;
+        ls_it[size_t(e)] = it;
+        searching[size_t(e)] = 0;
+        if (r != SDX_LS_RESIDUAL_NORM) s.merit = m;
+      } else {
+        if (backtrack) scale[size_t(e)] *= alpha;
+        ls_it[size_t(e)] = it + 1;
+      }
+    }
+  }
+  std::vector<int32_t> back(size_t(nE), 0);
+  bool any_back = false;
+  for (int e = 0; e < nE; ++e) {
+    if (!take[size_t(e)]) continue;
+    EnvState& s = S[size_t(e)];
+    s.res_known = res_at_last[size_t(e)] != 0;
+    if (!improved[size_t(e)] && !keep_final[size_t(e)]) {
+      back[size_t(e)] = 1;
+      any_back = true;
+      s.res = s.res_start;
+      s.res_known = false;
+      s.merit = s.merit0;
+    }
+  }
+  if (any_back) {
+    put(ls_mask_, back);
+    launch_env_copy<T>(stream_, nV_, d_row_env_.p, ls_mask_.p, sol0_.p, u_.p);
   }
 }
 
@@ -1503,7 +1808,7 @@ void World<T>::assemble_actor(int a, double dt, const sdx_solver_params& sp, dou
   std::vector<int32_t> mask(size_t(nEnv_), 0);
   mask[size_t(a)] = 1;
   set_env_mask(mask);
-  assemble(dt, sp, true);
+  assemble(dt, sp, true, sp.psd_proj_mode == SDX_PSD_ALWAYS ? 1 : 0);
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   const int v0 = env_v0_[size_t(a)], v1 = env_v0_[size_t(a) + 1];
   const int b0 = h_row_ptr_[size_t(v0)], b1 = h_row_ptr_[size_t(v1)];
@@ -1526,6 +1831,25 @@ void World<T>::assemble_actor(int a, double dt, const sdx_solver_params& sp, dou
       for (size_t k = 0; k < h.size(); ++k) vals[9 * k + size_t(c)] = double(h[k]);
     }
   }
+}
+
+// Objective of one actor at its current displacement with stage start
+// u0 = v0 = 0, as the assembly entry above.
+template <class T>
+double World<T>::objective_actor(int a, double dt, const sdx_solver_params& sp) {
+  finalize();
+  actor(a);
+  if (colliders_dirty_) upload_colliders();
+  hip_check(hipMemsetAsync(u0_.p, 0, u0_.bytes(), stream_), "hipMemsetAsync");
+  hip_check(hipMemsetAsync(v0_.p, 0, v0_.bytes(), stream_), "hipMemsetAsync");
+  std::vector<int32_t> mask(size_t(nEnv_), 0);
+  mask[size_t(a)] = 1;
+  set_env_mask(mask);
+  queue_energy(dt, sp);
+  std::vector<double> ob(static_cast<size_t>(nEnv_));
+  fetch(env_obj_, ob);
+  fetch_all();
+  return ob[size_t(a)];
 }
 
 template <class T>

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <new>
@@ -245,6 +246,7 @@ class WorldBase {
   virtual void set_state(int actor, int which, const double* in) = 0;
   virtual void assemble_actor(int actor, double dt, const sdx_solver_params& sp, double* res, int32_t* ptr,
                               int32_t* col, double* vals) = 0;
+  virtual double objective_actor(int actor, double dt, const sdx_solver_params& sp) = 0;
   virtual void info(int64_t out[8]) = 0;
   virtual void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) = 0;
   virtual void set_schedule(const sdx_schedule& s) = 0;
@@ -401,6 +403,7 @@ class World final : public WorldBase {
   void set_state(int actor, int which, const double* in) override;
   void assemble_actor(int actor, double dt, const sdx_solver_params& sp, double* res, int32_t* ptr, int32_t* col,
                       double* vals) override;
+  double objective_actor(int actor, double dt, const sdx_solver_params& sp) override;
   void info(int64_t out[8]) override;
   void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) override;
   void set_schedule(const sdx_schedule& s) override;
@@ -436,15 +439,47 @@ class World final : public WorldBase {
   static constexpr int kPersistentGridA = 8192, kPersistentGridV = 4096;
   void upload_colliders();
   void set_env_mask(const std::vector<int32_t>& mask);
-  void assemble(double dt, const sdx_solver_params& sp, bool wantH);
+  // Newton state of one environment during a step
+  struct EnvState {
+    bool active = true;
+    int iters = 0;
+    double res0 = 0, res = 0;
+    int conv = SDX_CONV_NONE;
+    int lin_iters = 0, ls_iters = 0, lin_conv = SDX_CONV_NONE, refreshes = 0;
+    double lin_rel = 0, lin_true_rel = 0;
+    int ws_tried = 0, ws_accepted = 0;
+    double ws_ratio = 0;
+    double dx_norm = std::numeric_limits<double>::infinity();
+    bool have_dx = false;
+    int psd = SDX_PSD_ALWAYS;  // PSD mode (IfFailAlways turns Always after a failed try)
+    double merit = 0;          // objective at u (objective-based line searches)
+    double merit0 = 0;         // and where the Newton iteration started
+    double prev_res = 0;       // |res| at the start of the previous Newton iteration
+    double lin_res = 0;        // final residual norm of the last linear solve (its monitored norm)
+    double eta = 0;            // relative tolerance of the current linear solve
+    double res_start = 0;      // |res| where the Newton iteration started
+    bool res_known = false;    // res_ and res hold the residual at the current u
+  };
+  void line_search(double dt, const sdx_solver_params& sp, const std::vector<int32_t>& take,
+                   const std::vector<int32_t>& keep_final, std::vector<EnvState>& S, std::vector<uint8_t>& improved,
+                   std::vector<int>& ls_it);
+  // project: PSD-project the element Hessians of this assembly
+  void assemble(double dt, const sdx_solver_params& sp, bool wantH, int project);
+  // the DResidual (and residual) of the environments in mask, projected where proj[e]
+  void assemble_h(double dt, const sdx_solver_params& sp, const std::vector<int32_t>& mask,
+                  const std::vector<uint8_t>& proj);
   std::vector<double> read_env_sq();
+  // queue the objective of the environments in the current mask into env_obj_
+  void queue_energy(double dt, const sdx_solver_params& sp);
+  // queue the per-environment dot a.b into env_dot_
+  void queue_env_dot(const T* a, const T* b);
   struct WarmStats {
     std::vector<int32_t> accepted;  // learned warm start kept (per environment)
     std::vector<double> ratio;      // |b - A x0| / |b| of the learned guess
   };
   std::vector<EnvCG> cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve,
                               std::vector<double>* true_rel, const std::vector<int32_t>* warm,
-                              WarmStats* learned = nullptr);
+                              WarmStats* learned = nullptr, const std::vector<double>* env_tol = nullptr);
   // Runs CG on the device; false if no environment had anything to solve (h
   // then holds the final states), else the states are on the device (env_cg_).
   // timing_ok: its first burst may be a timed sample (a whole solve, not a
@@ -584,6 +619,10 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, reset_)
   SDX_BUF(double, env_sq_)
   SDX_BUF(double, norms2_)
+  SDX_BUF(double, env_tol_)  // per-environment relative tolerance of the linear solve
+  SDX_BUF(double, env_obj_)  // per-environment objective (merit)
+  SDX_BUF(double, env_dot_)  // per-environment dot product
+  SDX_BUF(T, res0_)          // residual at the start of a Newton iteration (gradient-descent fallback)
 #undef SDX_BUF
   DevTiles dt_v_{bufs_, &stream_};
   DevTiles dt_a_{bufs_, &stream_};

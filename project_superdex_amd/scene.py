@@ -84,6 +84,12 @@ class SoftActor:
                                  ptr.ctypes.data_as(_ip), col.ctypes.data_as(_ip), val.ctypes.data_as(_dp)))
         return res, ptr, col, val.reshape(nb, 3, 3)
 
+    def objective(self, dt: float, params: _abi.SolverParams) -> float:
+        """Objective (merit) at the current displacement, stage start as in assemble()."""
+        out = C.c_double()
+        check(lib().sdx_objective(self.scene._w, self.index, dt, C.byref(params), C.byref(out)))
+        return out.value
+
 
 class RigidActor:
     def __init__(self, scene: "Scene", index: int):

@@ -48,6 +48,8 @@ def test_defaults_are_the_reference_defaults(L):
     assert p.line_search_type == _abi.LS_RESIDUAL_NORM and p.line_search_max_iter == 4
     assert p.line_search_alpha == 0.5 and p.explosion_control == 1
     assert p.abs_div_tol == 1e9 and p.rel_div_tol == 1e4 and p.psd_proj_mode == _abi.PSD_ALWAYS
+    assert p.linear_tolerance_strategy == _abi.LINTOL_EW2 and p.gradient_descent_fallback == 0
+    assert p.line_search_wolfe1 == 1e-4 and p.line_search_wolfe2 == 0.9 and p.line_search_max_rel_increase == 0.0
     # linear_solver_params.h:318-345, krylov_solver.h:42
     assert p.linear.abs_tol == 1e-9 and p.linear.rel_tol == 1e-5 and p.linear.rel_div_tol == 1e10
     assert p.linear.max_iter == 500 and p.linear.norm_type == _abi.NORM_PRECONDITIONED_RESIDUAL_L2
