@@ -200,6 +200,9 @@ typedef struct sdx_soft_params {
   double poisson_ratio;
   double density;
   double mass_damping;            /* massDampingCoefficient [1/s] */
+  double stiffness_damping;       /* stiffnessDampingCoefficient beta [s] (material_params.h:95-105):
+                                     viscous PK2 S = (beta/dt) C0 : (E(F) - E(F_stage_start)), C0
+                                     the material's rest stiffness (fem_stress_damping.h); default 0 */
   /* ContactParams (contact/contact_params.h:96-160) */
   double penalty_coefficient;     /* [Pa/m] */
   double penalty_smoothing_half_distance; /* default 0.005 m */
@@ -215,6 +218,11 @@ typedef struct sdx_soft_params {
   double viscous_friction_coefficient;       /* [s/m], default 0 */
   double friction_falloff_vel;               /* [m/s], default 0.01 */
   double normal_viscous_damping_coefficient; /* [s/m], default 0 */
+  int32_t stiffness_damping_geometric_term;  /* stiffnessDampingIncludeGeometricTerm (material_params.h:
+                                                107-120): 1 adds the geometric block of the viscous
+                                                tangent (exact Newton), 0 (default) keeps the
+                                                material block only */
+  int32_t reserved;
 } sdx_soft_params;
 
 typedef struct sdx_world sdx_world;

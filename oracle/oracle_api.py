@@ -63,6 +63,9 @@ def lib():
         L.sdxo_smith_nh.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, C.c_int]
         L.sdxo_polyrelu.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
         L.sdxo_stress_element.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.sdxo_stress_damping_element.restype = C.c_double
+        L.sdxo_stress_damping_element.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                                  C.c_int, _dp, _dp]
         L.sdxo_pcg_csr.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int,
                                    C.POINTER(_abi.KrylovParams), _ip, _ip, _dp]
         _lib = L
@@ -122,6 +125,18 @@ def newton_synthetic(kind: int, x0, params: _abi.SolverParams):
 
 def forcing_term(strategy, k, res_norm, prev_res_norm, lin_res_norm, prev_eta, rel_tol):
     return lib().sdxo_forcing_term(strategy, k, res_norm, prev_res_norm, lin_res_norm, prev_eta, rel_tol)
+
+
+def stress_damping_element(X, u, us, kappa, mu, lam, geo=False, project=False):
+    """Stiffness damping of one tet: (energy, residual[12], tangent[12, 12])."""
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(12)
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(12)
+    us = np.ascontiguousarray(us, dtype=np.float64).reshape(12)
+    r = np.zeros(12)
+    H = np.zeros(144)
+    e = lib().sdxo_stress_damping_element(_d(X), _d(u), _d(us), kappa, mu, lam, int(geo), int(project),
+                                          r.ctypes.data_as(_dp), H.ctypes.data_as(_dp))
+    return e, r, H.reshape(12, 12)
 
 
 def pcg_csr(indptr, indices, data, b, x0=None, bs=3, params=None, fp64=True):

@@ -308,6 +308,94 @@ static void StressElement(const TetGeom<T>& g, const T u[12], T mu, T lambda, bo
         for (int c = 0; c < 3; ++c) H[(gg * 3 + c) * 12 + f * 3 + r] = H[(f * 3 + r) * 12 + gg * 3 + c];
 }
 
+// Stiffness (viscous) damping of one tet, StressDampingWork
+// (fem_stress_damping.h:102-501) on the isotropic path: C0, the rest
+// stiffness dS/dE of the Smith material at F = I, is the Lame tensor
+// lambda I(x)I + 2 mu I_sym and its rest stress vanishes
+// (reference_material_stiffness.h:67-90), so with kappa = beta / dt
+//   dE = E(F) - E(Fs), E = (F^T F - I)/2, S = kappa (lambda tr(dE) I + 2 mu dE)
+//   energy vol dE:S / 2, res_f = vol F S dN_f,
+//   material tangent vol kappa [lambda a_f a_g^T + mu ((F F^T)(dN_f.dN_g) + a_g a_f^T)], a = F dN,
+//   and with geo the geometric block vol (dN_f . S~ dN_g) I, S~ = S projected
+//   to PSD (eigenvalues clamped at eps) when projectPsd and S is not positive
+//   definite (BatchedProjectSymPsd, decomposition_utils_inl.h:1558-1600).
+// Returns the energy.
+template <class T>
+static double StressDampingElement(const TetGeom<T>& g, const T u[12], const T us[12], T kappa, T mu, T lambda,
+                                   bool geo, bool projectPsd, bool wantH, T res[12], T H[144]) {
+  T F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Fs[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int f = 0; f < 4; ++f)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        F[r * 3 + c] += u[f * 3 + r] * g.dN[f][c];
+        Fs[r * 3 + c] += us[f * 3 + r] * g.dN[f][c];
+      }
+  T dE[9], S[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      T c = 0, cs = 0;
+      for (int p = 0; p < 3; ++p) {
+        c += F[p * 3 + i] * F[p * 3 + j];
+        cs += Fs[p * 3 + i] * Fs[p * 3 + j];
+      }
+      dE[i * 3 + j] = T(0.5) * (c - cs);
+    }
+  const T tr = dE[0] + dE[4] + dE[8];
+  double psi = 0.0;
+  for (int i = 0; i < 9; ++i) {
+    S[i] = kappa * (T(2) * mu * dE[i] + ((i % 4 == 0) ? lambda * tr : T(0)));
+    psi += 0.5 * double(S[i]) * double(dE[i]);
+  }
+  T P[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      T v = 0;
+      for (int k = 0; k < 3; ++k) v += F[i * 3 + k] * S[k * 3 + j];
+      P[i * 3 + j] = v;
+    }
+  for (int f = 0; f < 4; ++f)
+    for (int i = 0; i < 3; ++i)
+      res[f * 3 + i] += g.vol * (P[i * 3 + 0] * g.dN[f][0] + P[i * 3 + 1] * g.dN[f][1] + P[i * 3 + 2] * g.dN[f][2]);
+  if (!wantH) return double(g.vol) * psi;
+  T Sg[9];
+  for (int i = 0; i < 9; ++i) Sg[i] = geo ? S[i] : T(0);
+  if (geo && projectPsd) {
+    const T m1 = Sg[0], m2 = Sg[0] * Sg[4] - Sg[1] * Sg[3], m3 = Det3(Sg);
+    if (!(m1 > T(0) && m2 > T(0) && m3 > T(0))) {
+      double A[9], Q[9], l[3];
+      for (int i = 0; i < 9; ++i) A[i] = double(Sg[i]);
+      JacobiEigen(3, A, Q, l);
+      const double eps = double(std::numeric_limits<T>::epsilon());
+      for (int k = 0; k < 3; ++k) l[k] = std::max(l[k], eps);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double v = 0;
+          for (int k = 0; k < 3; ++k) v += Q[i * 3 + k] * l[k] * Q[j * 3 + k];
+          Sg[i * 3 + j] = T(v);
+        }
+    }
+  }
+  T a[4][3], FFt[9];
+  for (int f = 0; f < 4; ++f)
+    for (int i = 0; i < 3; ++i) a[f][i] = F[i * 3 + 0] * g.dN[f][0] + F[i * 3 + 1] * g.dN[f][1] + F[i * 3 + 2] * g.dN[f][2];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) FFt[i * 3 + j] = F[i * 3 + 0] * F[j * 3 + 0] + F[i * 3 + 1] * F[j * 3 + 1] + F[i * 3 + 2] * F[j * 3 + 2];
+  for (int f = 0; f < 4; ++f)
+    for (int gg = 0; gg < 4; ++gg) {
+      const T dd = g.dN[f][0] * g.dN[gg][0] + g.dN[f][1] * g.dN[gg][1] + g.dN[f][2] * g.dN[gg][2];
+      T gs = 0;
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) gs += g.dN[f][k] * Sg[k * 3 + l] * g.dN[gg][l];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          T v = kappa * (lambda * a[f][i] * a[gg][j] + mu * (FFt[i * 3 + j] * dd + a[gg][i] * a[f][j]));
+          if (i == j) v += gs;
+          H[(f * 3 + i) * 12 + gg * 3 + j] += g.vol * v;
+        }
+    }
+  return double(g.vol) * psi;
+}
+
 // ------------------------------------------------------------------ BSR
 
 template <class T>
@@ -907,6 +995,8 @@ struct Actor {
   std::vector<TetGeom<T>> geom;
   std::vector<T> mu, lambda;  // per tet
   T density = 1000, massDamping = 0;
+  T stiffDamping = 0;       // stiffnessDampingCoefficient beta [s]
+  bool stiffDampingGeo = false;
   T kPen = 1e6, hPen = 0.005, thrPen = 0.001, maxAlign = 0;
   // dissipation (ContactParams, contact_params.h:170-225)
   T muCoulomb = 0, cViscous = 0, falloffVel = 0.01, cNormalDamping = 0;
@@ -941,6 +1031,8 @@ struct Scene {
     a.lambda.assign(size_t(nT), T(lam));
     a.density = T(sp.density);
     a.massDamping = T(sp.mass_damping);
+    a.stiffDamping = T(sp.stiffness_damping);
+    a.stiffDampingGeo = sp.stiffness_damping_geometric_term != 0;
     a.kPen = T(sp.penalty_coefficient);
     a.hPen = T(sp.penalty_smoothing_half_distance);
     a.thrPen = T(sp.penalty_threshold);
@@ -1086,6 +1178,9 @@ struct Scene {
       if (H)
         for (int i = 0; i < 144; ++i) H[i] = 0;
       StressElement<T>(g, ue, a.mu[size_t(t)], a.lambda[size_t(t)], wantH, projectPsd, r, H);
+      if (a.stiffDamping > T(0))  // StressDampingWork with kappa = beta / dt (mochi_soft.h:259-292)
+        StressDampingElement<T>(g, ue, st, a.stiffDamping / dt, a.mu[size_t(t)], a.lambda[size_t(t)],
+                                a.stiffDampingGeo, projectPsd, wantH, r, H);
       // GravityWork, Q1: res_f -= rho g N_f(centroid) w
       for (int f = 0; f < 4; ++f)
         for (int d = 0; d < 3; ++d) r[3 * f + d] -= (grav[d] * a.density) * (T(0.25) * g.vol);
@@ -1318,6 +1413,16 @@ struct Scene {
       double psi = 0.0;
       SmithNH<T>(a.mu[size_t(t)], a.lambda[size_t(t)], F, &psi, nullptr, nullptr, false);
       e += double(g.vol) * psi;
+      if (a.stiffDamping > T(0)) {
+        T ue[12], us[12], rr[12] = {};
+        for (int f = 0; f < 4; ++f)
+          for (int d = 0; d < 3; ++d) {
+            ue[3 * f + d] = u[3 * size_t(tv[f]) + d];
+            us[3 * f + d] = u0[3 * size_t(tv[f]) + d];
+          }
+        e += StressDampingElement<T>(g, ue, us, a.stiffDamping / dt, a.mu[size_t(t)], a.lambda[size_t(t)], false,
+                                     false, false, rr, nullptr);
+      }
       double xc[3] = {0, 0, 0};
       for (int f = 0; f < 4; ++f)
         for (int d = 0; d < 3; ++d)
@@ -1663,6 +1768,19 @@ void sdxo_stress_element(const double* X, const double* u, double mu, double lam
   StressElement<double>(g, u, mu, lambda, true, project != 0, r, h);
   std::copy(r, r + 12, res);
   std::copy(h, h + 144, H);
+}
+
+// Stiffness damping of one tet (rest positions X[12]) at u with stage start us;
+// residual and full 12x12 tangent; returns the energy.
+double sdxo_stress_damping_element(const double* X, const double* u, const double* us, double kappa, double mu,
+                                   double lambda, int geo, int project, double* res, double* H) {
+  const int32_t tv[4] = {0, 1, 2, 3};
+  TetGeom<double> g = MakeTetGeom<double>(X, tv);
+  double r[12] = {}, h[144] = {};
+  const double e = StressDampingElement<double>(g, u, us, kappa, mu, lambda, geo != 0, project != 0, true, r, h);
+  std::copy(r, r + 12, res);
+  std::copy(h, h + 144, H);
+  return e;
 }
 
 // PCG on a CSR matrix (scalar rows) with preconditioner block size bs in {0,1,3}.

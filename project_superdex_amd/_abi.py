@@ -138,6 +138,7 @@ class SoftParams(C.Structure):
         ("poisson_ratio", C.c_double),
         ("density", C.c_double),
         ("mass_damping", C.c_double),
+        ("stiffness_damping", C.c_double),
         ("penalty_coefficient", C.c_double),
         ("penalty_smoothing_half_distance", C.c_double),
         ("penalty_threshold", C.c_double),
@@ -146,6 +147,8 @@ class SoftParams(C.Structure):
         ("viscous_friction_coefficient", C.c_double),
         ("friction_falloff_vel", C.c_double),
         ("normal_viscous_damping_coefficient", C.c_double),
+        ("stiffness_damping_geometric_term", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -205,12 +208,14 @@ def pinned_solver_params(newton_iters: int = 1, lin_rel_tol: float = 1e-6, lin_m
 def default_soft_params(E: float = 1e5, nu: float = 0.4, density: float = 1000.0,
                         penalty: float = 1e6, mass_damping: float = 0.0, coulomb: float = 0.0,
                         viscous: float = 0.0, falloff_vel: float = 0.01,
-                        normal_damping: float = 0.0) -> SoftParams:
+                        normal_damping: float = 0.0, stiffness_damping: float = 0.0,
+                        stiffness_damping_geometric: int = 0) -> SoftParams:
     """Soft material + ContactParams (material_params.h:77-105, contact_params.h:96-225) of the
     pinned workloads: the reference's smoothing, threshold and alignment defaults, with penalty
     1e6 and no friction unless asked for (the reference defaults to Coulomb 0.5)."""
     return SoftParams(youngs_modulus=E, poisson_ratio=nu, density=density, mass_damping=mass_damping,
-                      penalty_coefficient=penalty, penalty_smoothing_half_distance=0.005,
+                      stiffness_damping=stiffness_damping, stiffness_damping_geometric_term=stiffness_damping_geometric,
+                      reserved=0, penalty_coefficient=penalty, penalty_smoothing_half_distance=0.005,
                       penalty_threshold=0.001, max_alignment_normals=0.0,
                       coulomb_friction_coefficient=coulomb, viscous_friction_coefficient=viscous,
                       friction_falloff_vel=falloff_vel, normal_viscous_damping_coefficient=normal_damping)

@@ -85,6 +85,7 @@ void sdx_default_soft_params(sdx_soft_params* s) {
   s->poisson_ratio = 0.45;
   s->density = 1000.0;
   s->mass_damping = 0.0;
+  s->stiffness_damping = 0.0;
   s->penalty_coefficient = 1e9;
   s->penalty_smoothing_half_distance = 0.005;
   s->penalty_threshold = 0.001;
@@ -93,6 +94,8 @@ void sdx_default_soft_params(sdx_soft_params* s) {
   s->viscous_friction_coefficient = 0.0;
   s->friction_falloff_vel = 0.01;
   s->normal_viscous_damping_coefficient = 0.0;
+  s->stiffness_damping_geometric_term = 0;
+  s->reserved = 0;
 }
 
 int32_t sdx_abi_struct_size(int32_t which) {
@@ -144,6 +147,12 @@ sdx_status sdx_add_soft_tet_mesh(sdx_world* world, const double* X, int32_t n_ve
   SDX_REQUIRE(params->coulomb_friction_coefficient >= 0 && params->viscous_friction_coefficient >= 0 &&
                   params->friction_falloff_vel >= 0 && params->normal_viscous_damping_coefficient >= 0,
               "friction coefficients and falloff velocity must not be negative");
+  // material_params_utils.cpp:107-109
+  SDX_REQUIRE(std::isfinite(params->mass_damping) && params->mass_damping >= 0 &&
+                  std::isfinite(params->stiffness_damping) && params->stiffness_damping >= 0,
+              "damping coefficients must be finite and non-negative");
+  SDX_REQUIRE(params->stiffness_damping_geometric_term == 0 || params->stiffness_damping_geometric_term == 1,
+              "stiffness_damping_geometric_term must be 0 or 1");
   for (int64_t k = 0; k < 4 * int64_t(n_tets); ++k)
     SDX_REQUIRE(tets[k] >= 0 && tets[k] < n_verts, "tet node index out of range");
   return guarded([&]() -> sdx_status {

@@ -86,7 +86,19 @@ constexpr int kSlotStride = 16;
 constexpr int kCgSets = 6;
 
 // Geometry planes of a tet, stored SoA with stride nT.
-enum : int { kGdN = 0, kGVol = 9, kGMu = 10, kGLambda = 11, kGMass = 12, kGAlpha = 13, kGeomPlanes = 14 };
+// kGBeta: stiffness damping coefficient beta [s]; kGBetaGeo: 1 where its tangent
+// takes the geometric block.
+enum : int {
+  kGdN = 0,
+  kGVol = 9,
+  kGMu = 10,
+  kGLambda = 11,
+  kGMass = 12,
+  kGAlpha = 13,
+  kGBeta = 14,
+  kGBetaGeo = 15,
+  kGeomPlanes = 16
+};
 
 struct MeshView {
   int32_t nV, nT, nF, nB, nEnv;
@@ -165,7 +177,7 @@ struct AsmRange {
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
                     const T* u0, const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv,
-                    T* Ev, int block = 128, const KernelEvents* ev = nullptr);
+                    T* Ev, int block = 128, const KernelEvents* ev = nullptr, bool damp = false);
 // u0: the stage-start displacement (relative sliding of the samples).
 template <class T>
 void launch_contact(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,

@@ -35,7 +35,7 @@ namespace sdx {
 
 // ------------------------------------------------------------------ element
 
-template <class T>
+template <class T, bool kDamp>
 __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldPtrs<T> f, const T* __restrict__ u,
                                                  const T* __restrict__ u0, const T* __restrict__ v0, T dt,
                                                  T gx, T gy, T gz, int wantH, int project, T* __restrict__ Rv,
@@ -65,6 +65,19 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldP
       w[3 * a + d] = uu - (s0 + dt * v0[3 * nd[a] + d]);  // u - predicted target (batched_element_utils.h:90-120)
       wd[3 * a + d] = uu - s0;
     }
+  // stiffness damping (fem_stress_damping.h), kappa = beta / dt (mochi_soft.h:259-292)
+  Viscous<T> vis;
+  bool damp = false;
+  if constexpr (kDamp) {
+    const T beta = g[kGBeta * nT + t];
+    damp = beta > T(0);
+    if (damp) {
+      T us[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) us[k] = ue[k] - wd[k];
+      viscous_setup<T>(dN, ue, us, beta / dt, mu, lam, g[kGBetaGeo * nT + t] != T(0), project != 0, vis);
+    }
+  }
   // destination records of the tet, loaded before the element math
   int pv[4];
   uint32_t pe[6];
@@ -84,19 +97,24 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldP
   // where the pair runs backwards
   auto emit = [&](int fa, int fb, const T* q) {
     if (fa == fb) {
+      T dq[6];
+#pragma unroll
+      for (int r = 0; r < 6; ++r) dq[r] = q[r];
+      if (kDamp && damp) viscous_block<T>(vis, dN[fa], dN[fb], true, vol, dq);
       T* d = Dv + (size_t)pv[fa] * 6;
-      d[0] = q[0] + T(2) * ms;
-      d[1] = q[1] + T(2) * ms;
-      d[2] = q[2] + T(2) * ms;
-      d[3] = q[3];
-      d[4] = q[4];
-      d[5] = q[5];
+      d[0] = dq[0] + T(2) * ms;
+      d[1] = dq[1] + T(2) * ms;
+      d[2] = dq[2] + T(2) * ms;
+      d[3] = dq[3];
+      d[4] = dq[4];
+      d[5] = dq[5];
       return;
     }
     const uint32_t p = pe[fa == 0 ? fb - 1 : (fa == 1 ? fb + 1 : 5)];  // pair (kPairA, kPairB) index
     T o[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) o[r] = q[r];
+    if (kDamp && damp) viscous_block<T>(vis, dN[fa], dN[fb], false, vol, o);
     o[0] += ms;
     o[4] += ms;
     o[8] += ms;
@@ -114,7 +132,11 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldP
   // the residual record of each vertex: stress, inertia of the consistent
   // mass and gravity (fem_gravity.h:108-118: res_f -= rho g vol / 4),
   // written before the Hessian is formed
-  auto on_res = [&](const T* res) {
+  auto on_res = [&](const T* res_in) {
+    T res[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) res[k] = res_in[k];
+    if (kDamp && damp) viscous_res<T>(vis, dN, vol, res);
     const T gq = T(0.25) * mass;
     const T sw[3] = {w[0] + w[3] + w[6] + w[9], w[1] + w[4] + w[7] + w[10], w[2] + w[5] + w[8] + w[11]};
     const T sd[3] = {wd[0] + wd[3] + wd[6] + wd[9], wd[1] + wd[4] + wd[7] + wd[10], wd[2] + wd[5] + wd[8] + wd[11]};
@@ -1623,6 +1645,21 @@ __global__ void __launch_bounds__(256) k_energy(MeshView m, FieldPtrs<T> f, cons
       }
       const double m20 = mass / 20.0;
       e = vol * smith_energy<T>(F, mu, lam);
+      const T beta = g[kGBeta * nT + t];
+      if (beta > T(0)) {  // stiffness damping energy vol dE:S/2
+        T ue[12], us[12];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            ue[3 * a + r] = u[3 * nd[a] + r];
+            us[3 * a + r] = u0[3 * nd[a] + r];
+          }
+        Viscous<T> vis;
+        double psi = 0.0;
+        viscous_setup<T>(dN, ue, us, beta / dt, T(mu), T(lam), false, false, vis, &psi);
+        e += vol * psi;
+      }
       e -= mass * (gx * xc[0] + gy * xc[1] + gz * xc[2]);
       e += 0.5 / (dtd * dtd) * m20 * (s1 + sum[0] * sum[0] + sum[1] * sum[1] + sum[2] * sum[2]);
       if (alpha > 0.0)
@@ -1705,10 +1742,14 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
                     const T* u0, const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv,
-                    T* Ev, int block, const KernelEvents* ev) {
+                    T* Ev, int block, const KernelEvents* ev, bool damp) {
   if (r.t1 <= r.t0) return;
-  SDX_LAUNCH(ev, k_element<T>, dim3(blocks_for(r.t1 - r.t0, block)), dim3(block), s, m, r, f, u, u0, v0, T(dt),
-             T(grav[0]), T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev);
+  if (damp)
+    SDX_LAUNCH(ev, (k_element<T, true>), dim3(blocks_for(r.t1 - r.t0, block)), dim3(block), s, m, r, f, u, u0, v0,
+               T(dt), T(grav[0]), T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev);
+  else
+    SDX_LAUNCH(ev, (k_element<T, false>), dim3(blocks_for(r.t1 - r.t0, block)), dim3(block), s, m, r, f, u, u0, v0,
+               T(dt), T(grav[0]), T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev);
 }
 
 template <class T>
@@ -1972,7 +2013,7 @@ void launch_energy(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, cons
 #define SDX_INST_T(T)                                                                                            \
   template void launch_element<T>(hipStream_t, const MeshView&, const AsmRange&, const FieldPtrs<T>&, const T*, \
                                   const T*, const T*, double, const double*, int, int, T*, T*, T*, int,         \
-                                  const KernelEvents*);                                                         \
+                                  const KernelEvents*, bool);                                                   \
   template void launch_contact<T>(hipStream_t, const MeshView&, const AsmRange&, const FieldPtrs<T>&, const T*, \
                                   const T*, T, int, T*, T*, T*);                                                \
   template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const AsmRange&, const T*, const T*, int, T*, \

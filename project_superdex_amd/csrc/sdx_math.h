@@ -315,6 +315,137 @@ __device__ __forceinline__ double smith_energy(const M33<T>& F, double mu, doubl
   return 0.5 * muHat * (Ic - 3.0) + 0.5 * lambdaHat * Jma * Jma - 0.5 * muHat * log(Ic + 1.0);
 }
 
+// Stiffness (viscous) damping of one tet (fem_stress_damping.h:102-501),
+// isotropic rest stiffness C0 = Lame(lambda, mu) of the Smith material:
+//   dE = E(F) - E(Fs), S = kappa (lambda tr(dE) I + 2 mu dE), kappa = beta/dt
+//   residual  vol F S dN_f
+//   tangent   vol kappa [lambda a_f a_g^T + mu ((F F^T)(dN_f.dN_g) + a_g a_f^T)], a = F dN,
+//             plus (geometric term on) vol (dN_f . Sg dN_g) I, Sg = S projected to
+//             PSD (eigenvalues clamped at eps) where it is not positive definite and
+//             the tangent is projected.
+// Symmetric 3x3 values in the order 00 11 22 01 02 12.
+template <class T>
+struct Viscous {
+  M33<T> F;
+  T S[6], Sg[6], FFt[6];
+  T kl, km;
+};
+
+// psi (if not NULL) receives the energy density dE:S/2 in FP64.
+template <class T>
+__device__ __forceinline__ void viscous_setup(const V3<T> dN[4], const T u[12], const T us[12], T kappa, T mu,
+                                              T lambda, bool geo, bool project, Viscous<T>& v,
+                                              double* psi = nullptr) {
+  M33<T> Fs;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) v.F.m[k] = Fs.m[k] = (k % 4 == 0) ? T(1) : T(0);
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const T d[3] = {dN[f].x, dN[f].y, dN[f].z};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        v.F.m[r * 3 + c] += u[f * 3 + r] * d[c];
+        Fs.m[r * 3 + c] += us[f * 3 + r] * d[c];
+      }
+  }
+  const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
+  T dE[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    T c = T(0);
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+      c += v.F.m[p * 3 + ii[k]] * v.F.m[p * 3 + jj[k]] - Fs.m[p * 3 + ii[k]] * Fs.m[p * 3 + jj[k]];
+    dE[k] = T(0.5) * c;
+    v.FFt[k] = v.F.m[ii[k] * 3] * v.F.m[jj[k] * 3] + v.F.m[ii[k] * 3 + 1] * v.F.m[jj[k] * 3 + 1] +
+               v.F.m[ii[k] * 3 + 2] * v.F.m[jj[k] * 3 + 2];
+  }
+  const T tr = dE[0] + dE[1] + dE[2];
+  v.kl = kappa * lambda;
+  v.km = kappa * mu;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    v.S[k] = T(2) * v.km * dE[k] + (k < 3 ? v.kl * tr : T(0));
+    v.Sg[k] = geo ? v.S[k] : T(0);
+  }
+  if (psi) {
+    double e = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) e += (k < 3 ? 0.5 : 1.0) * double(v.S[k]) * double(dE[k]);
+    *psi = e;
+  }
+  if (geo && project) {
+    const T* q = v.S;
+    const T m2 = q[0] * q[1] - q[3] * q[3];
+    const T m3 = q[0] * (q[1] * q[2] - q[5] * q[5]) - q[3] * (q[3] * q[2] - q[5] * q[4]) +
+                 q[4] * (q[3] * q[5] - q[1] * q[4]);
+    if (!(q[0] > T(0) && m2 > T(0) && m3 > T(0))) {
+      T la[3];
+      V3<T> ev[3];
+      sym_eig3(v.S, la, ev);
+      const T eps = sizeof(T) == 4 ? T(1.1920928955078125e-07) : T(2.220446049250313e-16);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) la[k] = la[k] > eps ? la[k] : eps;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const V3<T> a = ev[0], b = ev[1], c = ev[2];
+        const T ai[3] = {a.x, a.y, a.z}, bi[3] = {b.x, b.y, b.z}, ci[3] = {c.x, c.y, c.z};
+        v.Sg[k] = la[0] * ai[ii[k]] * ai[jj[k]] + la[1] * bi[ii[k]] * bi[jj[k]] + la[2] * ci[ii[k]] * ci[jj[k]];
+      }
+    }
+  }
+}
+
+// Residual res_f += vol F S dN_f.
+template <class T>
+__device__ __forceinline__ void viscous_res(const Viscous<T>& v, const V3<T> dN[4], T vol, T res[12]) {
+  const T* S = v.S;
+  const M33<T> Sm = {{S[0], S[3], S[4], S[3], S[1], S[5], S[4], S[5], S[2]}};
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const V3<T> sd = mul(Sm, dN[f]);
+    const V3<T> p = mul(v.F, sd);
+    res[3 * f + 0] += vol * p.x;
+    res[3 * f + 1] += vol * p.y;
+    res[3 * f + 2] += vol * p.z;
+  }
+}
+
+// Block (f, g) of the viscous tangent added to blk: f == g the 6 entries
+// 00 11 22 01 02 12, else 9 row-major.
+template <class T>
+__device__ __forceinline__ void viscous_block(const Viscous<T>& v, const V3<T>& dNf, const V3<T>& dNg, bool diag,
+                                              T vol, T* blk) {
+  const V3<T> af = mul(v.F, dNf), ag = mul(v.F, dNg);
+  const T dd = dot3(dNf, dNg);
+  const T* Sg = v.Sg;
+  const M33<T> Sm = {{Sg[0], Sg[3], Sg[4], Sg[3], Sg[1], Sg[5], Sg[4], Sg[5], Sg[2]}};
+  const T gs = dot3(dNf, mul(Sm, dNg));
+  const T a1[3] = {af.x, af.y, af.z}, a2[3] = {ag.x, ag.y, ag.z};
+  const T* Q = v.FFt;
+  const T FF[9] = {Q[0], Q[3], Q[4], Q[3], Q[1], Q[5], Q[4], Q[5], Q[2]};
+  auto entry = [&](int i, int j) {
+    T x = v.kl * a1[i] * a2[j] + v.km * (FF[i * 3 + j] * dd + a2[i] * a1[j]);
+    if (i == j) x += gs;
+    return vol * x;
+  };
+  if (diag) {
+    blk[0] += entry(0, 0);
+    blk[1] += entry(1, 1);
+    blk[2] += entry(2, 2);
+    blk[3] += entry(0, 1);
+    blk[4] += entry(0, 2);
+    blk[5] += entry(1, 2);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) blk[i * 3 + j] += entry(i, j);
+  }
+}
+
 // Element kernel core: stress residual (12) and, if wantH, the 10 upper 3x3
 // blocks of the Hessian of one tet, both scaled by the Q1 weight `vol`; each
 // block goes to emit(f, g, blk) as soon as it is formed (f == g: 6 entries

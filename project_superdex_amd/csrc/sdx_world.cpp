@@ -203,6 +203,7 @@ void World<T>::finalize() {
   hip_check(hipSetDevice(device_), "hipSetDevice");
   const int nEnv = int(actors_.size());
   nEnv_ = nEnv;
+  stiff_damp_ = false;
   env_v0_.assign(size_t(nEnv) + 1, 0);
   std::vector<int32_t> env_t0(size_t(nEnv) + 1, 0);
   for (int e = 0; e < nEnv; ++e) {
@@ -277,6 +278,9 @@ void World<T>::finalize() {
       geom[size_t(kGLambda) * size_t(nT_) + size_t(gt)] = T(lam);
       geom[size_t(kGMass) * size_t(nT_) + size_t(gt)] = T(A.params.density * vol);
       geom[size_t(kGAlpha) * size_t(nT_) + size_t(gt)] = T(A.params.mass_damping);
+      geom[size_t(kGBeta) * size_t(nT_) + size_t(gt)] = T(A.params.stiffness_damping);
+      geom[size_t(kGBetaGeo) * size_t(nT_) + size_t(gt)] = T(A.params.stiffness_damping_geometric_term ? 1 : 0);
+      if (A.params.stiffness_damping > 0) stiff_damp_ = true;
       for (int a = 0; a < 4; ++a) {
         vtx_keys.push_back(v0 + tv[a]);
         vtx_codes.push_back(uint32_t(gt) * 4u + uint32_t(a));
@@ -625,7 +629,7 @@ void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH, int 
     T* Dv = group_records(cdiag_.p, 6, g.rv0);
     T* Ev = group_records(coff_.p, 9, g.re0);
     launch_element<T>(stream_, mesh_, r, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, Rv, Dv,
-                      Ev, sched_.element_block, sampled_events(2));
+                      Ev, sched_.element_block, sampled_events(2), stiff_damp_);
     launch_contact<T>(stream_, mesh_, r, fields_, u_.p, u0_.p, T(dt), wantH ? 1 : 0, Rv, Dv, Ev);
     if (wantH)
       launch_gather_bsr<T>(stream_, mesh_, r, Dv, Ev, sp.linear.preconditioner, vals_.p, dinv_.p, sampled_events(3));

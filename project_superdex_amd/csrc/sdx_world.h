@@ -434,6 +434,7 @@ class World final : public WorldBase {
   // on 512 batched cubes these measured 3% faster than grids of half the size
   static constexpr int kNewtonHist = 4;
   int hist_k_ = 0;  // Newton iteration whose history the warm start uses
+  bool stiff_damp_ = false;  // some actor has stiffness damping (the element kernel's damping variant)
   T* hist1() { return hist_.p + size_t(2 * std::min(hist_k_, kNewtonHist - 1)) * size_t(3 * nV_); }
   T* hist2() { return hist1() + size_t(3 * nV_); }
   static constexpr int kPersistentGridA = 8192, kPersistentGridV = 4096;

@@ -139,13 +139,16 @@ class SoftBody:
     viscous: float = 0.0
     falloff_vel: float = 0.01
     normal_damping: float = 0.0
+    stiffness_damping: float = 0.0
+    stiffness_damping_geometric: int = 0
 
     def soft_params(self, abi):
         """The sdx_soft_params of this body (homogeneous part; per-element fields apart)."""
         E0 = float(np.mean(self.E)) if np.ndim(self.E) else float(self.E)
         nu0 = float(np.mean(self.nu)) if np.ndim(self.nu) else float(self.nu)
         return abi.default_soft_params(E0, nu0, self.density, self.penalty, self.mass_damping, self.coulomb,
-                                       self.viscous, self.falloff_vel, self.normal_damping)
+                                       self.viscous, self.falloff_vel, self.normal_damping,
+                                       self.stiffness_damping, self.stiffness_damping_geometric)
 
 
 @dataclass

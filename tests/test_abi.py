@@ -68,6 +68,8 @@ def test_defaults_are_the_reference_defaults(L):
     # contact_params.h:170-225: Coulomb friction on by default
     assert s.coulomb_friction_coefficient == 0.5 and s.viscous_friction_coefficient == 0.0
     assert s.friction_falloff_vel == 0.01 and s.normal_viscous_damping_coefficient == 0.0
+    # material_params.h:95-120: no stiffness damping, material tangent only
+    assert s.stiffness_damping == 0.0 and s.stiffness_damping_geometric_term == 0 and s.mass_damping == 0.0
 
 
 def test_invalid_arguments_are_rejected_before_any_device_call(L):
