@@ -98,13 +98,14 @@ def make_spec(workload: str, steps: int, seed: int):
     raise ValueError(workload)
 
 
-def solver_params(spec, mode, autotune=False, learned=False):
+def solver_params(spec, mode, autotune=False, learned=False, history=False):
     from project_superdex_amd import _abi
 
     p = _abi.pinned_solver_params(spec.newton_iters, lin_rel_tol=1e-6, precision_mode=mode,
                                   refresh_every=10 if mode == _abi.PREC_MODE_MIXED else 0)
     p.schedule = -1 if autotune else 0
-    p.warm_start = _abi.WARM_START_LEARNED if learned else _abi.WARM_START_ZERO
+    p.warm_start = (_abi.WARM_START_LEARNED if learned else
+                    _abi.WARM_START_HISTORY if history else _abi.WARM_START_ZERO)
     return p
 
 
@@ -287,8 +288,9 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "mixed", "fp64"], default="mixed")
-    ap.add_argument("--warm-start", choices=["learned", "zero"], default="learned",
-                    help="learned: the trained network of project_superdex_amd/weights (zero if absent)")
+    ap.add_argument("--warm-start", choices=["learned", "history", "zero"], default="learned",
+                    help="learned: the trained network of project_superdex_amd/weights (zero if absent); "
+                         "history: the two previous Newton steps alone")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--schedule", default=None,
                     help="fixed launch schedule instead of the autotuner, e.g. "
@@ -333,6 +335,7 @@ def main(argv=None) -> int:
     sc, rigid = gs.build_scene(spec, device=0, precision=mode)
     info = sc.info()
     learned = False
+    history = args.warm_start == "history"
     if args.warm_start == "learned":
         from project_superdex_amd.learned import DEFAULT_WEIGHTS, WarmStartModel
 
@@ -344,8 +347,8 @@ def main(argv=None) -> int:
     if args.schedule:
         sc.set_schedule(**{k: int(v) for k, v in (kv.split("=") for kv in args.schedule.split(","))})
         autotune = False
-    run_steps(sc, rigid, spec, solver_params(spec, mode, autotune, learned), 0, 1)
-    params = solver_params(spec, mode, learned=learned)
+    run_steps(sc, rigid, spec, solver_params(spec, mode, autotune, learned, history), 0, 1)
+    params = solver_params(spec, mode, learned=learned, history=history)
     run_steps(sc, rigid, spec, params, 1, warmup)
     print(f"[bench] warm, schedule {sc.schedule()}", file=sys.stderr, flush=True)
     if args.time_every > 0:
@@ -403,7 +406,7 @@ def main(argv=None) -> int:
                             "full4096": "4096 soft cubes n~U{4..12}, 1-3 indenters, 2 Newton iters"}[args.workload],
             "envs": len(spec.bodies), "tets": nT, "verts": nV, "nnzb": nB, "dt": spec.dt,
             "newton_iters": spec.newton_iters, "lin_rel_tol": 1e-6, "precision": args.precision,
-            "warm_start": "learned" if learned else "zero", "schedule": sc.schedule(),
+            "warm_start": "learned" if learned else ("history" if history else "zero"), "schedule": sc.schedule(),
         },
         "roofline": {
             "bound": "hbm", "kernel": "k_cg_spmv (CG matvec w = A z and z.w)",
@@ -423,26 +426,35 @@ def main(argv=None) -> int:
                  "reference_equivalent_GBps": step_b / (elapsed / steps) / 1e9},
         "state_snapshot": snapshot,
     }
-    if learned and not args.no_zero_compare:  # the same steps from a zero initial guess, same schedule
-        z, zr = gs.build_scene(spec, device=0, precision=mode)
-        z.set_schedule(**sc.schedule())
-        zp = solver_params(spec, mode)
-        run_steps(z, zr, spec, zp, 0, warmup)
-        zst = []
-        t0 = time.perf_counter()
-        run_steps(z, zr, spec, zp, warmup, total, zst)
-        zel = time.perf_counter() - t0
-        zit = np.array([[s["linear_iters"] for s in st] for st in zst], dtype=np.float64)
+    if learned and not args.no_zero_compare:
+        # the same steps from a zero initial guess and from the two previous Newton steps alone
+        # (the warm start without the network), same schedule
+        def leg(hist):
+            z, zr = gs.build_scene(spec, device=0, precision=mode)
+            z.set_schedule(**sc.schedule())
+            zp = solver_params(spec, mode, history=hist)
+            run_steps(z, zr, spec, zp, 0, warmup)
+            zst = []
+            t0 = time.perf_counter()
+            run_steps(z, zr, spec, zp, warmup, total, zst)
+            zel = time.perf_counter() - t0
+            z.close()
+            zit = np.array([[s["linear_iters"] for s in st] for st in zst], dtype=np.float64)
+            return nT * steps / zel, float(zit.sum(axis=1).mean())
+
+        zero_value, zero_it = leg(False)
+        hist_value, hist_it = leg(True)
         kept = sum(s["warm_start_accepted"] for s in per_env)
         tried = sum(s["warm_start_tried"] for s in per_env)
         out["learned_vs_zero"] = {
-            "zero_init_value": nT * steps / zel, "learned_value": value, "speedup": (nT * steps / elapsed) / (nT * steps / zel),
-            "cg_iters_zero": float(zit.sum(axis=1).mean()), "cg_iters_learned": float(it.sum(axis=1).mean()),
+            "zero_init_value": zero_value, "learned_value": value, "speedup": value / zero_value,
+            "cg_iters_zero": zero_it, "cg_iters_learned": float(it.sum(axis=1).mean()),
+            "history_only_value": hist_value, "cg_iters_history_only": hist_it,
+            "network_vs_history_speedup": value / hist_value,
             "warm_starts_kept": int(kept), "warm_starts_tried": int(tried),
             "median_guess_residual": float(np.median([s["warm_start_rel_res"] for s in per_env])),
         }
-        z.close()
-        print(f"[bench] zero-init comparison done", file=sys.stderr, flush=True)
+        print(f"[bench] zero-init and history-only comparisons done", file=sys.stderr, flush=True)
     if batched and not args.no_plain:
         sample = sampled_envs(spec, args.plain_envs, args.seed + 1)
         rate, secs = plain_gpu(spec, steps, warmup, sample)

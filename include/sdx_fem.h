@@ -105,10 +105,14 @@ enum {
   SDX_PREC_MODE_FP64 = 2,     /* everything FP64 (reference SUPERDEX_PRECISION=fp64) */
 };
 
-/* CG initial guess (extension). */
+/* CG initial guess (extension).  HISTORY: the least-residual combination of the
+ * environment's two previous Newton steps of the same Newton iteration (the learned
+ * warm start without the network's guess); LEARNED adds the network's guess to the
+ * combination (needs a model, sdx_world_set_warm_start_model). */
 enum {
   SDX_WARM_START_ZERO = 0,
   SDX_WARM_START_LEARNED = 1,
+  SDX_WARM_START_HISTORY = 2,
 };
 
 /* BSR value layout accepted by sdx_pcg_bsr3_solve. */

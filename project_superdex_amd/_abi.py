@@ -22,7 +22,7 @@ LS_NONE, LS_SIMPLE, LS_ARMIJO, LS_WOLFE_WEAK, LS_WOLFE_STRONG, LS_RESIDUAL_NORM,
 PSD_NEVER, PSD_ALWAYS, PSD_IF_FAIL_RETRY, PSD_IF_FAIL_ALWAYS = 0, 1, 2, 3
 LINTOL_CONSTANT, LINTOL_EW1, LINTOL_EW2, LINTOL_EW3 = 0, 1, 2, 3
 PREC_MODE_FP32, PREC_MODE_MIXED, PREC_MODE_FP64 = 0, 1, 2
-WARM_START_ZERO, WARM_START_LEARNED = 0, 1
+WARM_START_ZERO, WARM_START_LEARNED, WARM_START_HISTORY = 0, 1, 2
 BSR_BLOCK_ROWMAJOR, BSR_REF_ROW_STRIP = 0, 1
 
 FLOAT32_EPS = 1.1920928955078125e-07

@@ -286,7 +286,8 @@ static sdx_status validate_solver_params(const sdx_solver_params& p) {
   SDX_REQUIRE(params->linear.preconditioner >= SDX_PREC_NONE && params->linear.preconditioner <= SDX_PREC_BLOCK_JACOBI,
               "unsupported preconditioner");
   SDX_REQUIRE(params->linear.norm_type >= 0 && params->linear.norm_type <= 2, "unsupported norm type");
-  SDX_REQUIRE(params->warm_start == SDX_WARM_START_ZERO || params->warm_start == SDX_WARM_START_LEARNED,
+  SDX_REQUIRE(params->warm_start == SDX_WARM_START_ZERO || params->warm_start == SDX_WARM_START_LEARNED ||
+                  params->warm_start == SDX_WARM_START_HISTORY,
               "unknown warm start");
   SDX_REQUIRE(params->schedule == 0 || params->schedule == -1, "schedule must be 0 or -1");
   return SDX_OK;

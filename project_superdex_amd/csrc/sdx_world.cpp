@@ -1426,14 +1426,19 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       // rejected in its last warm-started solve starts from zero for the next kWarmSkipSteps steps
       // (the network, features and guess residuals cost more than the few kept guesses save)
       const int hk = std::min(newton_it, kNewtonHist - 1);
-      bool learned = first_try && sp.warm_start == SDX_WARM_START_LEARNED && ws_on_;
+      // the history-only start is the learned one with a zero network guess, which the
+      // least-residual combination drops (its Gram rows vanish)
+      const bool history_only = sp.warm_start == SDX_WARM_START_HISTORY;
+      bool learned = first_try && ((sp.warm_start == SDX_WARM_START_LEARNED && ws_on_) || history_only);
       if (learned && ws_skip_[hk] > 0) {
         --ws_skip_[hk];
         learned = false;
       }
       const bool collecting = first_try && coll_per_env_ > 0 && coll_n_ + int64_t(nE) * coll_per_env_ <= coll_cap_;
-      if (learned || collecting) ws_features();
-      if (learned)
+      if ((learned && !history_only) || collecting) ws_features();
+      if (learned && history_only)
+        launch_fill<T>(stream_, int64_t(n3), T(0), dx_.p);
+      else if (learned)
         launch_mlp_infer<T>(stream_, ws_w_, ws_F_.p, nV_, d_row_env_.p, ws_scale_.p, d_pinned_.p, nullptr, dx_.p);
       std::vector<double> true_rel;
       WarmStats ws;
