@@ -304,6 +304,28 @@ sdx_status sdx_set_element_materials(sdx_world* world, int32_t actor, const doub
  * DoFs of the nodes at their current displacement. */
 sdx_status sdx_pin_nodes(sdx_world* world, int32_t actor, const int32_t* nodes, int32_t n);
 
+/* AddBoundaryConditionDofsWorld / ClearBoundaryConditions (mochi_actor.h:187-211): hold
+ * DoFs (3 node + axis) at world coordinates from the next step on; with clear != 0 the
+ * earlier clearable conditions are dropped first (the permanent ones of sdx_pin_nodes
+ * stay).  A new set of held DoFs rebuilds the assembly maps at the next step; new values
+ * for the same set only move the targets.  DoF indices must be unique and in range, values
+ * finite. */
+sdx_status sdx_set_boundary_conditions(sdx_world* world, int32_t actor, const int32_t* dofs,
+                                       const double* values_world, int32_t n, int32_t clear);
+
+/* Contact query (CQueryActorContactForces, mochi_actor.cpp:2320-2345: registered, its
+ * results are available after the next step): with enable != 0 every step ends by
+ * evaluating the contact of each soft actor with each collider acting on it at the step's
+ * result.  sdx_get_contact_wrench returns for one (actor, collider) pair: out[0..2] the
+ * contact force on the actor (penalty and dissipation, GetContactForceWorld), out[3..5] the
+ * torque of the reaction on the collider about its position, out[6] the least signed
+ * distance of the actor's contact points (samples within the detection threshold,
+ * GetContactPointsWorld) or +inf when there are none.  SDX_ERR_STATE without a registered
+ * query or before its first step; SDX_ERR_INVALID_ARGUMENT for a collider that does not act
+ * on the actor. */
+sdx_status sdx_register_contact_query(sdx_world* world, int32_t enable);
+sdx_status sdx_get_contact_wrench(sdx_world* world, int32_t actor, int32_t collider, double out[7]);
+
 /* CreateRigidActor with a plane / sphere shape (kinematic).  actor = -1: the
  * collider touches every soft actor, else only that one.  plane: point p,
  * unit normal n pointing to the free side.  sphere: centre c, radius r. */

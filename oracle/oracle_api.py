@@ -53,6 +53,9 @@ def lib():
         L.sdxo_assemble.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(_abi.SolverParams),
                                     _dp, _dp, _dp, _ip, _ip, _dp]
         L.sdxo_boundary.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+        L.sdxo_set_contact_query.argtypes = [C.c_void_p, C.c_int]
+        L.sdxo_get_contact_wrench.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.sdxo_set_boundary_conditions.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_int, C.c_int]
         L.sdxo_objective.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(_abi.SolverParams),
                                      _dp, _dp, _dp]
         L.sdxo_newton_synthetic.argtypes = [C.c_int, C.c_int, _dp, C.POINTER(_abi.SolverParams), _dp,
@@ -234,6 +237,21 @@ class OracleScene:
                             res.ctypes.data_as(_dp), ptr.ctypes.data_as(_ip), col.ctypes.data_as(_ip),
                             val.ctypes.data_as(_dp))
         return res, ptr, col, val.reshape(nnzb, 3, 3)
+
+    def set_contact_query(self, on=True):
+        lib().sdxo_set_contact_query(self._h, int(on))
+
+    def contact_wrench(self, actor, collider):
+        out = np.zeros(7)
+        rc = lib().sdxo_get_contact_wrench(self._h, actor, collider, out.ctypes.data_as(_dp))
+        assert rc == 0, rc
+        return out
+
+    def set_boundary_conditions(self, actor, dofs, values, clear=True):
+        dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        lib().sdxo_set_boundary_conditions(self._h, actor, dofs.ctypes.data_as(_ip), values.ctypes.data_as(_dp),
+                                           dofs.size, int(clear))
 
     def objective(self, actor, dt, params, u0=None, v0=None) -> float:
         out = C.c_double()

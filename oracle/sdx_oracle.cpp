@@ -1002,6 +1002,8 @@ struct Actor {
   T muCoulomb = 0, cViscous = 0, falloffVel = 0.01, cNormalDamping = 0;
   std::vector<uint8_t> pinned;  // per DoF
   std::vector<T> pinValue;      // per DoF
+  std::vector<uint8_t> perm;    // permanent pins (sdxo_pin_nodes)
+  std::vector<T> permValue;
   std::vector<T> u, v;
   Bsr<T> M;       // consistent mass (Q4), same pattern as the stiffness
   Bsr<T> K;       // dres
@@ -1043,6 +1045,8 @@ struct Scene {
     a.cNormalDamping = T(sp.normal_viscous_damping_coefficient);
     a.pinned.assign(3 * size_t(nV), 0);
     a.pinValue.assign(3 * size_t(nV), T(0));
+    a.perm.assign(3 * size_t(nV), 0);
+    a.permValue.assign(3 * size_t(nV), T(0));
     a.u.assign(3 * size_t(nV), T(0));
     a.v.assign(3 * size_t(nV), T(0));
     BuildPattern(a);
@@ -1245,8 +1249,12 @@ struct Scene {
   // (mochi_physics/src/mochi_contact.cpp:1200-1260).
   // With energy set, the contact energy sum over samples w (1/2 k p^2 + fN (mu f(|pT|) +
   // 1/2 cv/dt |pT|^2 + 1/2 cn/dt pn^2)) (contact_utils.h:560, :935) is added to it.
+  // With wrench set (7 per collider), each collider's contact force on the body
+  // sum w F, the torque of the reaction about the collider's position
+  // sum (x - P) x (-w F) and the least signed distance of the samples in contact
+  // are accumulated there (distance slots start at +inf).
   void AssembleContact(Actor<T>& a, int ai, const std::vector<T>& u, const std::vector<T>& u0, T dt, bool wantH,
-                       std::vector<T>& res, double* energy = nullptr) {
+                       std::vector<T>& res, double* energy = nullptr, std::vector<double>* wrench = nullptr) {
     static const T kBary[3][3] = {{T(1.0 / 6.0), T(2.0 / 3.0), T(1.0 / 6.0)},
                                   {T(1.0 / 6.0), T(1.0 / 6.0), T(2.0 / 3.0)},
                                   {T(2.0 / 3.0), T(1.0 / 6.0), T(1.0 / 6.0)}};
@@ -1356,6 +1364,19 @@ struct Scene {
             eSample += double(fN) * ef;
           }
           if (energy) *energy += double(w) * eSample;
+          if (wrench) {
+            double* o = &(*wrench)[7 * c];
+            double Fw[3], r[3];
+            for (int d = 0; d < 3; ++d) {
+              Fw[d] = double(w) * double(F[d]);
+              r[d] = double(x[d]) - col.p[d];
+              o[d] += Fw[d];
+            }
+            o[3] -= r[1] * Fw[2] - r[2] * Fw[1];
+            o[4] -= r[2] * Fw[0] - r[0] * Fw[2];
+            o[5] -= r[0] * Fw[1] - r[1] * Fw[0];
+            o[6] = std::min(o[6], double(dist));
+          }
           for (int i = 0; i < 3; ++i) {
             const T bs = kBary[q][i] * w;
             for (int d = 0; d < 3; ++d) res[3 * size_t(nd[i]) + d] -= F[d] * bs;
@@ -1498,8 +1519,13 @@ struct Scene {
     NewtonDriver<T>(P, sp, stats, nullptr);
   }
 
+  // Contact query: per actor, 7 values per collider from the last step.
+  bool contactQuery = false;
+  std::vector<std::vector<double>> wrenches;
+
   // Scene::Step for every actor: EntityIncrementStep, PreStage, Newton, PostStage.
   void Step(T dt, const sdx_solver_params& sp, sdx_step_stats* stats, int nStats) {
+    if (contactQuery) wrenches.assign(actors.size(), {});
     for (size_t ai = 0; ai < actors.size(); ++ai) {
       Actor<T>& a = actors[ai];
       const std::vector<T> u0 = a.u, v0 = a.v;
@@ -1507,6 +1533,13 @@ struct Scene {
         if (a.pinned[i]) a.u[i] = a.pinValue[i];
       sdx_step_stats st{};
       NewtonSolve(a, int(ai), u0, v0, dt, sp, st);
+      if (contactQuery) {  // at the step's result, before a diverged actor is reset
+        std::vector<double> w(7 * colliders.size(), 0.0);
+        for (size_t c = 0; c < colliders.size(); ++c) w[7 * c + 6] = std::numeric_limits<double>::infinity();
+        std::vector<T> scratch(3 * size_t(a.nV), T(0));
+        AssembleContact(a, int(ai), a.u, u0, dt, false, scratch, nullptr, &w);
+        wrenches[ai] = std::move(w);
+      }
       for (size_t i = 0; i < a.u.size(); ++i) a.v[i] = (a.u[i] - u0[i]) * (T(1) / dt);
       if (st.convergence == SDX_CONV_DIVERGED) {
         std::fill(a.u.begin(), a.u.end(), T(0));
@@ -1589,8 +1622,9 @@ int sdxo_pin_nodes(void* sv, int actor, const int32_t* nodes, int n) {
     auto& a = sc.actors[size_t(actor)];
     for (int i = 0; i < n; ++i)
       for (int d = 0; d < 3; ++d) {
-        a.pinned[3 * size_t(nodes[i]) + d] = 1;
-        a.pinValue[3 * size_t(nodes[i]) + d] = a.u[3 * size_t(nodes[i]) + d];
+        const size_t k = 3 * size_t(nodes[i]) + d;
+        a.pinned[k] = a.perm[k] = 1;
+        a.pinValue[k] = a.permValue[k] = a.u[k];
       }
     return int(SDX_OK);
   });
@@ -1722,6 +1756,47 @@ int sdxo_newton_synthetic(int kind, int n, const double* x0, const sdx_solver_pa
 double sdxo_forcing_term(int strategy, int k, double resNorm, double prevResNorm, double linResNorm,
                          double prevEta, double relTol) {
   return ForcingTerm<double>(strategy, k, resNorm, prevResNorm, linResNorm, prevEta, relTol);
+}
+
+int sdxo_set_contact_query(void* sv, int on) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    sc.contactQuery = on != 0;
+    sc.wrenches.clear();
+    return int(SDX_OK);
+  });
+}
+
+// Contact force on the actor from the collider, torque on the collider, least distance.
+int sdxo_get_contact_wrench(void* sv, int actor, int collider, double* out) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    if (size_t(actor) >= sc.wrenches.size() || sc.wrenches[size_t(actor)].empty()) return int(SDX_ERR_STATE);
+    for (int k = 0; k < 7; ++k) out[k] = sc.wrenches[size_t(actor)][7 * size_t(collider) + size_t(k)];
+    return int(SDX_OK);
+  });
+}
+
+// Clearable boundary conditions: DoFs held at world coordinates (clear: drop
+// the earlier clearable ones; the permanent pins stay).
+int sdxo_set_boundary_conditions(void* sv, int actor, const int32_t* dofs, const double* values, int n, int clear) {
+  return Dispatch(static_cast<sdxo_scene*>(sv), [&](auto& sc) {
+    using T = typename std::remove_reference_t<decltype(sc.actors[0].X)>::value_type;
+    auto& a = sc.actors[size_t(actor)];
+    if (clear) {
+      for (size_t k = 0; k < a.pinned.size(); ++k) {
+        if (!a.perm[k]) {
+          a.pinned[k] = 0;
+          a.pinValue[k] = T(0);
+        } else {
+          a.pinValue[k] = a.permValue[k];
+        }
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      a.pinned[size_t(dofs[i])] = 1;
+      a.pinValue[size_t(dofs[i])] = T(values[i]) - a.X[size_t(dofs[i])];
+    }
+    return int(SDX_OK);
+  });
 }
 
 // Boundary triangles (3 per row) and rest areas of one actor.

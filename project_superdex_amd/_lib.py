@@ -58,6 +58,9 @@ SIGNATURES = {
     "sdx_actor_bsr_size": (C.c_int32, [_vp, C.c_int32, _ip, _ip]),
     "sdx_assemble": (C.c_int32, [_vp, C.c_int32, C.c_double, C.POINTER(_abi.SolverParams), _dp, _ip, _ip, _dp]),
     "sdx_objective": (C.c_int32, [_vp, C.c_int32, C.c_double, C.POINTER(_abi.SolverParams), _dp]),
+    "sdx_set_boundary_conditions": (C.c_int32, [_vp, C.c_int32, _ip, _dp, C.c_int32, C.c_int32]),
+    "sdx_register_contact_query": (C.c_int32, [_vp, C.c_int32]),
+    "sdx_get_contact_wrench": (C.c_int32, [_vp, C.c_int32, C.c_int32, _dp]),
     "sdx_kernel_timing": (C.c_int32, [_vp, C.c_int32, C.c_int32, _dp]),
     "sdx_set_cg_burst": (C.c_int32, [_vp, C.c_int32]),
     "sdx_set_schedule": (C.c_int32, [_vp, C.POINTER(_abi.Schedule)]),

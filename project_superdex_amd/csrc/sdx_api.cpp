@@ -2,6 +2,7 @@
 // to World<T> / the standalone PCG solve.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -164,6 +165,8 @@ sdx_status sdx_add_soft_tet_mesh(sdx_world* world, const double* X, int32_t n_ve
     a.params = *params;
     a.pinned.assign(3 * size_t(n_verts), 0);
     a.pin_value.assign(3 * size_t(n_verts), 0.0);
+    a.perm.assign(3 * size_t(n_verts), 0);
+    a.perm_value.assign(3 * size_t(n_verts), 0.0);
     a.u.assign(3 * size_t(n_verts), 0.0);
     a.v.assign(3 * size_t(n_verts), 0.0);
     const int id = world->impl->add_actor(std::move(a));
@@ -196,9 +199,39 @@ sdx_status sdx_pin_nodes(sdx_world* world, int32_t actor, const int32_t* nodes, 
     for (int i = 0; i < n; ++i) SDX_REQUIRE(nodes[i] >= 0 && nodes[i] < a.nV, "node index out of range");
     for (int i = 0; i < n; ++i)
       for (int d = 0; d < 3; ++d) {
-        a.pinned[3 * size_t(nodes[i]) + size_t(d)] = 1;
-        a.pin_value[3 * size_t(nodes[i]) + size_t(d)] = a.u[3 * size_t(nodes[i]) + size_t(d)];
+        const size_t k = 3 * size_t(nodes[i]) + size_t(d);
+        a.pinned[k] = a.perm[k] = 1;
+        a.pin_value[k] = a.perm_value[k] = a.u[k];
       }
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_set_boundary_conditions(sdx_world* world, int32_t actor, const int32_t* dofs,
+                                       const double* values_world, int32_t n, int32_t clear) {
+  SDX_REQUIRE(world && n >= 0 && ((dofs && values_world) || n == 0), "NULL argument");
+  return guarded([&]() -> sdx_status {
+    std::vector<int32_t> sorted(dofs, dofs + n);
+    std::sort(sorted.begin(), sorted.end());
+    SDX_REQUIRE(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(),
+                "boundary-condition DoF indices must be unique");
+    world->impl->set_boundary_conditions(actor, dofs, values_world, n, clear != 0);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_register_contact_query(sdx_world* world, int32_t enable) {
+  SDX_REQUIRE(world, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    world->impl->set_contact_query(enable != 0);
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_get_contact_wrench(sdx_world* world, int32_t actor, int32_t collider, double out[7]) {
+  SDX_REQUIRE(world && out, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    world->impl->contact_wrench(actor, collider, out);
     return SDX_OK;
   });
 }

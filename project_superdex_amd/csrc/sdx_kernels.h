@@ -296,6 +296,15 @@ void launch_env_norms(hipStream_t s, const Tiles& tv, int nEnv, const T* a, cons
 // Per-environment a.b added into set 0 of acc (zeroed by the caller).
 template <class T>
 void launch_env_dot(hipStream_t s, const Tiles& tv, int nEnv, const T* a, const T* b, double* acc);
+// Contact wrench of every collider (kWrench doubles each in out, zeroed by the
+// caller: force on the soft samples, torque of the reaction on the collider
+// about its position) and the least signed distance of its samples in contact
+// in dmin (order-preserving int64 of the double, set to INT64_MAX by the caller),
+// over the environments with env_mask set, at displacement u, stage start u0.
+constexpr int kWrench = 6;
+template <class T>
+void launch_contact_wrench(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0, double dt,
+                           double* out, long long* dmin);
 // Objective (merit) of the environments with env_mask set at displacement u,
 // stage start u0, v0: stress, gravity, inertia, mass damping and contact
 // energies, per environment in FP64 into set 0 of acc (zeroed by the caller).

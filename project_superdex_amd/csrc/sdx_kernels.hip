@@ -177,7 +177,7 @@ __device__ __forceinline__ int sym_entry(int r, int c) { return r == c ? r : 2 +
 template <class T>
 struct ContactSample {
   V3<T> F, gr;
-  T cI, cN;
+  T cI, cN, dist;
   double e;
 };
 
@@ -206,6 +206,7 @@ __device__ __forceinline__ bool contact_sample(const V3<T>& x, const V3<T>& dxq,
   T pen, dpen, ddpen;
   poly_relu<T>(-dist, hp, hp - thr, pen, dpen, ddpen);
   out.gr = gr;
+  out.dist = dist;
   out.F = scl3(kp * pen * dpen, gr);
   out.cI = T(0);
   out.cN = -kp * (dpen * dpen + pen * ddpen);
@@ -1706,6 +1707,76 @@ __global__ void __launch_bounds__(256) k_energy(MeshView m, FieldPtrs<T> f, cons
   env_add_wave(acc, m.nEnv, env, valid, e);
 }
 
+// Order-preserving map of a double onto int64 (for atomicMin on distances).
+__device__ __forceinline__ long long ordered_bits(double x) {
+  const long long b = __double_as_longlong(x);
+  return b >= 0 ? b : (b ^ 0x7fffffffffffffffLL);
+}
+
+// Contact wrench of every collider at displacement u with stage start u0,
+// over the boundary-triangle samples of the environments with env_mask[e]:
+// out[kWrench c + 0..2] += the contact force on the samples from collider c
+// (penalty and dissipation, weighted), [3..5] += the torque of the reaction
+// on the collider about its position p, and dmin[c] the least signed
+// distance of the samples in contact (those within the detection threshold;
+// ordered_bits encoding).  One thread per triangle.
+template <class T>
+__global__ void __launch_bounds__(256) k_contact_wrench(MeshView m, FieldPtrs<T> f, const T* __restrict__ u,
+                                                        const T* __restrict__ u0, T dt, double* out,
+                                                        long long* dmin) {
+  const int tr = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (tr >= m.nF) return;
+  const int env = m.tri_env[tr];
+  if (!m.env_mask[env]) return;
+  const int c0 = m.coll_first[env], c1 = m.coll_first[env + 1];
+  if (c1 <= c0) return;
+  const int nd[3] = {m.tris[3 * tr], m.tris[3 * tr + 1], m.tris[3 * tr + 2]};
+  V3<T> xs[3], Xr[3], dxs[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Xr[i] = mk3(f.X[3 * nd[i]], f.X[3 * nd[i] + 1], f.X[3 * nd[i] + 2]);
+    const V3<T> ui = mk3(u[3 * nd[i]], u[3 * nd[i] + 1], u[3 * nd[i] + 2]);
+    xs[i] = add3(Xr[i], ui);
+    dxs[i] = sub3(ui, mk3(u0[3 * nd[i]], u0[3 * nd[i] + 1], u0[3 * nd[i] + 2]));
+  }
+  const V3<T> n0 = cross3(sub3(Xr[1], Xr[0]), sub3(Xr[2], Xr[0]));
+  const T w = T(0.5) * dsqrt(dot3(n0, n0)) / T(3);
+  V3<T> fnv = cross3(sub3(xs[1], xs[0]), sub3(xs[2], xs[0]));
+  fnv = scl3(T(1) / dsqrt(dot3(fnv, fnv)), fnv);
+  const double* cp = f.contact + kContactParams * env;
+  for (int c = c0; c < c1; ++c) {
+    const double* cd = m.coll + kCollStride * c;
+    const V3<T> P = mk3(T(cd[1]), T(cd[2]), T(cd[3]));
+    double F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0}, dm = 0;
+    bool any = false;
+    for (int q = 0; q < 3; ++q) {
+      T N[3];
+      tri_q3<T>(q, N);
+      const V3<T> x = add3(add3(scl3(N[0], xs[0]), scl3(N[1], xs[1])), scl3(N[2], xs[2]));
+      const V3<T> dxq = add3(add3(scl3(N[0], dxs[0]), scl3(N[1], dxs[1])), scl3(N[2], dxs[2]));
+      ContactSample<T> cs;
+      if (!contact_sample<T>(x, dxq, fnv, cd, cp, dt, false, cs)) continue;
+      const V3<T> Fw = scl3(w, cs.F);
+      const V3<T> tq = cross3(sub3(x, P), scl3(T(-1), Fw));
+      F[0] += double(Fw.x);
+      F[1] += double(Fw.y);
+      F[2] += double(Fw.z);
+      Tq[0] += double(tq.x);
+      Tq[1] += double(tq.y);
+      Tq[2] += double(tq.z);
+      dm = any ? fmin(dm, double(cs.dist)) : double(cs.dist);
+      any = true;
+    }
+    if (!any) continue;
+    double* o = out + kWrench * c;
+    for (int k = 0; k < 3; ++k) {
+      atomicAdd(o + k, F[k]);
+      atomicAdd(o + 3 + k, Tq[k]);
+    }
+    atomicMin(dmin + c, ordered_bits(dm));
+  }
+}
+
 // Sum the kSlots lines of set 0 of each environment: out[e * nv + k].
 template <class A>
 __global__ void k_acc_finalize(int nEnv, int nv, const A* __restrict__ acc, double* __restrict__ out) {
@@ -1776,6 +1847,14 @@ void launch_gather_res(hipStream_t s, const MeshView& m, const AsmRange& r, cons
   if (r.tile1 <= r.tile0) return;
   hipLaunchKernelGGL((k_gather_res<T, A>), dim3(unsigned(r.tile1 - r.tile0)), dim3(kVecThreads), 0, s, m, r, tv, Rv,
                      res, acc);
+}
+
+template <class T>
+void launch_contact_wrench(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, const T* u, const T* u0, double dt,
+                           double* out, long long* dmin) {
+  if (m.nF == 0) return;
+  hipLaunchKernelGGL(k_contact_wrench<T>, dim3(blocks_for(m.nF, 256)), dim3(256), 0, s, m, f, u, u0, T(dt), out,
+                     dmin);
 }
 
 template <class A>
@@ -2028,6 +2107,8 @@ void launch_energy(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, cons
   template void launch_env_copy<T>(hipStream_t, int32_t, const int32_t*, const int32_t*, const T*, T*);        \
   template void launch_env_norms<T>(hipStream_t, const Tiles&, int, const T*, const T*, double*);              \
   template void launch_env_dot<T>(hipStream_t, const Tiles&, int, const T*, const T*, double*);                \
+  template void launch_contact_wrench<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, const T*, \
+                                         double, double*, long long*);                                         \
   template void launch_energy<T>(hipStream_t, const MeshView&, const FieldPtrs<T>&, const T*, const T*,         \
                                  const T*, double, const double*, double*);                                     \
   template void launch_fill<T>(hipStream_t, int64_t, T, T*);                                                    \
@@ -2067,6 +2148,7 @@ SDX_INST_TA(float, double)
 SDX_INST_TA(double, double)
 template void launch_convert<float, float>(hipStream_t, int64_t, const float*, float*);
 template void launch_fill<int32_t>(hipStream_t, int64_t, int32_t, int32_t*);
+template void launch_fill<long long>(hipStream_t, int64_t, long long, long long*);
 template void launch_convert<float, double>(hipStream_t, int64_t, const float*, double*);
 template void launch_convert<double, float>(hipStream_t, int64_t, const double*, float*);
 template void launch_convert<double, double>(hipStream_t, int64_t, const double*, double*);

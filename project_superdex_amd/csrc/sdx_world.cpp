@@ -162,6 +162,7 @@ void World<T>::release_device() {
   snaps_.clear();  // snapshots hold the old topology's sizes
   drop_graphs();
   tuned_ = false;
+  contact_ready_ = false;
   for (auto* b : bufs_) b->free();
   finalized_ = false;
 }
@@ -571,8 +572,10 @@ void World<T>::upload_colliders() {
   std::vector<int32_t> first(size_t(nEnv_) + 1, 0);
   std::vector<double> cd;
   cd.reserve(size_t(kCollStride) * (size_t(own_first[size_t(nEnv_)]) + shared.size() * size_t(nEnv_)));
+  coll_entry_id_.clear();
   auto emit = [&](int i) {
     const HostCollider& c = colliders_[size_t(i)];
+    coll_entry_id_.push_back(i);
     cd.push_back(double(c.kind));
     cd.insert(cd.end(), c.p, c.p + 3);
     cd.insert(cd.end(), c.n, c.n + 3);
@@ -592,6 +595,7 @@ void World<T>::upload_colliders() {
     }
   }
   first[size_t(nEnv_)] = int32_t(cd.size() / kCollStride);
+  coll_first_h_ = first;
   if (cd.empty()) cd.assign(kCollStride, 0.0);
   if (d_coll_.n < cd.size()) d_coll_.alloc(cd.size());
   put(d_coll_, cd);
@@ -1578,6 +1582,24 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
         }
       }
   }
+  // contact query at the step's result, before PostStage may reset diverged environments
+  if (contact_query_) {
+    const size_t ne = coll_entry_id_.size();
+    if (ne) {
+      if (wrench_.n < ne * kWrench) wrench_.alloc(ne * kWrench);
+      if (wrench_dmin_.n < ne) wrench_dmin_.alloc(ne);
+      launch_fill<double>(stream_, int64_t(ne * kWrench), 0.0, wrench_.p);
+      launch_fill<long long>(stream_, int64_t(ne), std::numeric_limits<long long>::max(), wrench_dmin_.p);
+      set_env_mask(std::vector<int32_t>(size_t(nE), 1));
+      launch_contact_wrench<T>(stream_, mesh_, fields_, u_.p, u0_.p, dt, wrench_.p, wrench_dmin_.p);
+      wrench_h_.assign(ne * kWrench, 0.0);
+      wrench_dmin_h_.assign(ne, 0);
+      fetch(wrench_, wrench_h_);
+      fetch(wrench_dmin_, wrench_dmin_h_);
+      fetch_all();
+    }
+    contact_ready_ = true;
+  }
   // PostStage: v = (u - u0) / dt, reset on divergence (mochi_soft.cpp:135-163)
   std::vector<int32_t> reset(size_t(nE), 0);
   for (int e = 0; e < nE; ++e) reset[size_t(e)] = S[size_t(e)].conv == SDX_CONV_DIVERGED ? 1 : 0;
@@ -1859,6 +1881,69 @@ double World<T>::objective_actor(int a, double dt, const sdx_solver_params& sp) 
   fetch(env_obj_, ob);
   fetch_all();
   return ob[size_t(a)];
+}
+
+// AddBoundaryConditionDofsWorld (mochi_actor.h:187-190) and ClearBoundaryConditions
+// (:211): a changed set of pinned DoFs rebuilds the gather items at the next step;
+// new values for the same set only replace the targets.
+template <class T>
+void World<T>::set_boundary_conditions(int a, const int32_t* dofs, const double* values, int n, bool clear) {
+  HostActor& A = actor(a);
+  const int nd = 3 * A.nV;
+  for (int i = 0; i < n; ++i) {
+    if (dofs[i] < 0 || dofs[i] >= nd) throw std::invalid_argument("boundary-condition DoF out of range");
+    if (!std::isfinite(values[i])) throw std::invalid_argument("boundary-condition values must be finite");
+  }
+  std::vector<uint8_t> pinned = clear ? A.perm : A.pinned;
+  std::vector<double> pv = A.pin_value;
+  if (clear)
+    for (int k = 0; k < nd; ++k) pv[size_t(k)] = A.perm[size_t(k)] ? A.perm_value[size_t(k)] : 0.0;
+  for (int i = 0; i < n; ++i) {
+    pinned[size_t(dofs[i])] = 1;
+    pv[size_t(dofs[i])] = values[i] - A.X[size_t(dofs[i])];
+  }
+  const bool same = pinned == A.pinned;
+  if (!same) {
+    invalidate();  // device state back to the host mirrors first
+    A.pinned = pinned;
+    A.pin_value = pv;
+    return;
+  }
+  A.pin_value = pv;
+  if (!finalized_) return;
+  std::vector<T> h(pv.size());
+  for (size_t k = 0; k < pv.size(); ++k) h[k] = T(pv[k]);
+  T* d = d_pin_value_.p + size_t(3) * size_t(env_v0_[size_t(a)]);
+  hip_check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream_), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+template <class T>
+void World<T>::set_contact_query(bool on) {
+  contact_query_ = on;
+  contact_ready_ = false;
+}
+
+template <class T>
+void World<T>::contact_wrench(int a, int c, double out[7]) {
+  actor(a);
+  if (c < 0 || c >= int(colliders_.size())) throw std::out_of_range("no such collider");
+  if (!contact_query_) throw std::logic_error("the contact query is not registered");
+  if (!contact_ready_ || !finalized_)
+    throw std::logic_error("contact query results are available after the next simulation step");
+  for (int k = coll_first_h_[size_t(a)]; k < coll_first_h_[size_t(a) + 1]; ++k) {
+    if (coll_entry_id_[size_t(k)] != c) continue;
+    for (int j = 0; j < kWrench; ++j) out[j] = wrench_h_[size_t(k) * kWrench + size_t(j)];
+    long long b = wrench_dmin_h_[size_t(k)];
+    if (b == std::numeric_limits<long long>::max()) {
+      out[6] = std::numeric_limits<double>::infinity();
+    } else {
+      if (b < 0) b ^= 0x7fffffffffffffffLL;
+      std::memcpy(&out[6], &b, sizeof(double));
+    }
+    return;
+  }
+  throw std::invalid_argument("the collider does not act on this actor");
 }
 
 template <class T>
@@ -2149,7 +2234,8 @@ size_t World<T>::state_size() {
 namespace {
 // FNV-1a over what decides how a state steps: every actor's counts, rest
 // positions, tets, material (homogeneous parameters and per-element fields),
-// pinned DoFs with their values, and every collider's kind, attachment,
+// permanently pinned DoFs with their values (clearable boundary conditions
+// are targets that change from step to step), and every collider's kind, attachment,
 // normal and radius (collider positions are part of the state itself).
 class TopologyHash {
  public:
@@ -2186,8 +2272,8 @@ uint64_t topology_hash(const std::vector<HostActor>& actors, const std::vector<H
     h.pod(a.params);
     h.vec(a.E);
     h.vec(a.nu);
-    h.vec(a.pinned);
-    h.vec(a.pin_value);
+    h.vec(a.perm);
+    h.vec(a.perm_value);
   }
   for (const HostCollider& c : colliders) {
     h.pod(c.kind);

@@ -104,6 +104,10 @@ struct HostActor {
   std::vector<double> E, nu;  // per-element fields (empty: homogeneous)
   std::vector<uint8_t> pinned;
   std::vector<double> pin_value;
+  // permanent boundary conditions (sdx_pin_nodes); pinned/pin_value also hold the
+  // clearable ones of sdx_set_boundary_conditions
+  std::vector<uint8_t> perm;
+  std::vector<double> perm_value;
   std::vector<double> u, v;
 };
 
@@ -247,6 +251,17 @@ class WorldBase {
   virtual void assemble_actor(int actor, double dt, const sdx_solver_params& sp, double* res, int32_t* ptr,
                               int32_t* col, double* vals) = 0;
   virtual double objective_actor(int actor, double dt, const sdx_solver_params& sp) = 0;
+  // Clearable boundary conditions (AddBoundaryConditionDofsWorld / ClearBoundaryConditions):
+  // DoF 3 node + axis held at a world coordinate; with clear, the earlier clearable ones go.
+  virtual void set_boundary_conditions(int actor, const int32_t* dofs, const double* values, int n,
+                                       bool clear) = 0;
+  // Contact query (CQueryActorContactForces): with it on, every step ends by
+  // evaluating each (actor, collider) pair's contact at the step's result.
+  virtual void set_contact_query(bool on) = 0;
+  // Force on the actor's samples from the collider (3), torque of the reaction on
+  // the collider about its position (3), least signed distance of the samples in
+  // contact (+inf: none), as of the last step.
+  virtual void contact_wrench(int actor, int collider, double out[7]) = 0;
   virtual void info(int64_t out[8]) = 0;
   virtual void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) = 0;
   virtual void set_schedule(const sdx_schedule& s) = 0;
@@ -404,6 +419,9 @@ class World final : public WorldBase {
   void assemble_actor(int actor, double dt, const sdx_solver_params& sp, double* res, int32_t* ptr, int32_t* col,
                       double* vals) override;
   double objective_actor(int actor, double dt, const sdx_solver_params& sp) override;
+  void set_boundary_conditions(int actor, const int32_t* dofs, const double* values, int n, bool clear) override;
+  void set_contact_query(bool on) override;
+  void contact_wrench(int actor, int collider, double out[7]) override;
   void info(int64_t out[8]) override;
   void actor_bsr_size(int actor, int32_t* nrows, int32_t* nnzb) override;
   void set_schedule(const sdx_schedule& s) override;
@@ -435,6 +453,13 @@ class World final : public WorldBase {
   static constexpr int kNewtonHist = 4;
   int hist_k_ = 0;  // Newton iteration whose history the warm start uses
   bool stiff_damp_ = false;  // some actor has stiffness damping (the element kernel's damping variant)
+  // contact query: per device collider entry (an environment's copy of a collider)
+  // its host collider id, the first entry of each environment, and the last step's
+  // results (kWrench values and the least distance per entry)
+  bool contact_query_ = false, contact_ready_ = false;
+  std::vector<int32_t> coll_entry_id_, coll_first_h_;
+  std::vector<double> wrench_h_;
+  std::vector<long long> wrench_dmin_h_;
   T* hist1() { return hist_.p + size_t(2 * std::min(hist_k_, kNewtonHist - 1)) * size_t(3 * nV_); }
   T* hist2() { return hist1() + size_t(3 * nV_); }
   static constexpr int kPersistentGridA = 8192, kPersistentGridV = 4096;
@@ -624,6 +649,8 @@ class World final : public WorldBase {
   SDX_BUF(double, env_obj_)  // per-environment objective (merit)
   SDX_BUF(double, env_dot_)  // per-environment dot product
   SDX_BUF(T, res0_)          // residual at the start of a Newton iteration (gradient-descent fallback)
+  SDX_BUF(double, wrench_)     // contact query: kWrench values per collider entry
+  SDX_BUF(long long, wrench_dmin_)
 #undef SDX_BUF
   DevTiles dt_v_{bufs_, &stream_};
   DevTiles dt_a_{bufs_, &stream_};

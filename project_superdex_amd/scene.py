@@ -68,6 +68,16 @@ class SoftActor:
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
         check(lib().sdx_set_velocities(self.scene._w, self.index, v.ctypes.data_as(_dp), v.size))
 
+    def set_boundary_conditions(self, dofs, values_world, clear: bool = True):
+        """AddBoundaryConditionDofsWorld (after ClearBoundaryConditions with clear): hold
+        DoFs 3 node + axis at world coordinates from the next step on."""
+        dofs = np.ascontiguousarray(dofs, dtype=np.int32).reshape(-1)
+        vals = np.ascontiguousarray(values_world, dtype=np.float64).reshape(-1)
+        if dofs.size != vals.size:
+            raise ValueError("dofs and values must have the same length")
+        check(lib().sdx_set_boundary_conditions(self.scene._w, self.index, _i(dofs), _d(vals), dofs.size,
+                                                int(clear)))
+
     def bsr_size(self):
         nr, nb = C.c_int32(), C.c_int32()
         check(lib().sdx_actor_bsr_size(self.scene._w, self.index, C.byref(nr), C.byref(nb)))
@@ -92,11 +102,35 @@ class SoftActor:
 
 
 class RigidActor:
+    def __init__(self, scene: "Scene", index: int, kind: str = "plane", position=(0.0, 0.0, 0.0), actor: int = -1):
+        self.scene, self.index, self.kind, self.actor = scene, index, kind, actor
+        self.position = np.array(position, dtype=np.float64)  # the current target
+
+    def set_position(self, pos):
+        pos = np.asarray(pos, float)
+        check(lib().sdx_set_rigid_position(self.scene._w, self.index, _d(pos)))
+        self.position = pos.copy()
+
+
+class EnvScene:
+    """One environment of a batched Scene: its soft actor and the colliders attached to it."""
+
     def __init__(self, scene: "Scene", index: int):
         self.scene, self.index = scene, index
 
-    def set_position(self, pos):
-        check(lib().sdx_set_rigid_position(self.scene._w, self.index, _d(np.asarray(pos, float))))
+    @property
+    def soft_actor(self) -> SoftActor:
+        return self.scene.actors[self.index]
+
+    @property
+    def colliders(self) -> list:
+        return [c for c in self.scene.colliders if c.actor == self.index]
+
+    def __eq__(self, other):
+        return isinstance(other, EnvScene) and other.scene is self.scene and other.index == self.index
+
+    def __hash__(self):
+        return hash((id(self.scene), self.index))
 
 
 class Scene:
@@ -109,6 +143,7 @@ class Scene:
         self.precision = precision
         self.actors: list[SoftActor] = []
         self.colliders: list[RigidActor] = []
+        self.solver_params: _abi.SolverParams | None = None  # what SceneBatchExecutorV3 steps with
 
     def close(self):
         if getattr(self, "_w", None):
@@ -136,7 +171,7 @@ class Scene:
         idx = C.c_int32()
         check(lib().sdx_add_rigid_plane(self._w, _d(np.asarray(point, float)), _d(np.asarray(normal, float)),
                                         actor, C.byref(idx)))
-        r = RigidActor(self, idx.value)
+        r = RigidActor(self, idx.value, "plane", point, actor)
         self.colliders.append(r)
         return r
 
@@ -144,7 +179,7 @@ class Scene:
         idx = C.c_int32()
         check(lib().sdx_add_rigid_sphere(self._w, _d(np.asarray(center, float)), float(radius), actor,
                                          C.byref(idx)))
-        r = RigidActor(self, idx.value)
+        r = RigidActor(self, idx.value, "sphere", center, actor)
         self.colliders.append(r)
         return r
 
@@ -157,6 +192,27 @@ class Scene:
                                        dtype=np.int32)
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(len(idx), 3)
         check(lib().sdx_set_rigid_positions(self._w, len(idx), idx.ctypes.data_as(_ip), pos.ctypes.data_as(_dp)))
+        for k, i in enumerate(idx):
+            if 0 <= i < len(self.colliders):
+                self.colliders[i].position = pos[k].copy()
+
+    def env(self, index: int) -> "EnvScene":
+        """Environment `index` (soft actor `index` and the colliders attached to it) as a scene
+        of its own, the unit of SceneBatchExecutorV3."""
+        if not 0 <= index < len(self.actors):
+            raise IndexError("no such environment")
+        return EnvScene(self, index)
+
+    def register_contact_query(self, enable: bool = True):
+        """CQueryActorContactForces: contact wrenches available after the next step."""
+        check(lib().sdx_register_contact_query(self._w, int(enable)))
+
+    def contact_wrench(self, actor: int, collider: int) -> np.ndarray:
+        """Force on the actor from the collider (3), torque of the reaction on the collider about
+        its position (3), least signed distance of the contact points (inf: none), last step."""
+        out = np.zeros(7)
+        check(lib().sdx_get_contact_wrench(self._w, int(actor), int(collider), out.ctypes.data_as(_dp)))
+        return out
 
     def step(self, dt: float, params: _abi.SolverParams | None = None, as_array: bool = False):
         """One step; per-actor statistics as dicts, or with as_array as one

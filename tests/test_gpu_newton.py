@@ -77,6 +77,10 @@ def check_same(sg, so, stg, sto, n_env, tol=1e-7, lin_rel=0.15):
 
 def params(ls=_abi.LS_RESIDUAL_NORM, iters=3, psd=_abi.PSD_ALWAYS, lintol=_abi.LINTOL_CONSTANT):
     p = _abi.pinned_solver_params(iters, lin_rel_tol=1e-12, precision_mode=FP64)
+    # converged environments stop: an objective decrease of a step from a residual near the
+    # solve's noise floor is below the roundoff of the objective itself, where neither side's
+    # acceptance decision is defined
+    p.newton_abs_tol = 1e-6
     p.line_search_type = ls
     p.psd_proj_mode = psd
     p.linear_tolerance_strategy = lintol

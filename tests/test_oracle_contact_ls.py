@@ -203,3 +203,29 @@ def test_line_search_backtracking_brute_force(oracle):
         if checked == 2:
             break
     assert checked >= 1
+
+
+def test_contact_query_wrench_closed_form(oracle):
+    """The contact query (CQueryActorContactForces) of the uniformly pressed face: the force on
+    the body is k p(d) p'(d) A along the plane normal, the torque of the reaction about the
+    plane's point is that of the resultant through the face centroid (the trace-Q3 rule
+    integrates the linear moment arm exactly), and the least distance of the contact points is
+    -d; a collider out of reach reports no contact point."""
+    n, edge, d = 4, 0.04, 5e-4
+    nrm = np.array([0.0, -1.0, 0.0])
+    P = np.array([0.0, edge - d, 0.0])
+    sc, X = cube_scene(oracle, n, edge, ("plane", tuple(P), tuple(nrm)))
+    far = sc.add_sphere((5.0, 5.0, 5.0), 0.1)
+    sc.set_contact_query(True)
+    p = _abi.pinned_solver_params(0)  # no Newton iteration: the wrench of the initial state
+    p.gravity[1] = 0.0
+    sc.step(1e-3, p)
+    w = sc.contact_wrench(0, 0)
+    pp, dpp, _ = polyrelu_from_conditions(d, H_PEN, H_PEN - THR)
+    f = K_PEN * pp * dpp * edge * edge
+    np.testing.assert_allclose(w[:3], f * nrm, rtol=1e-10, atol=1e-12 * f)
+    r = np.array([edge / 2, edge, edge / 2]) - P
+    np.testing.assert_allclose(w[3:6], np.cross(r, -f * nrm), rtol=1e-9, atol=1e-12 * f * edge)
+    assert abs(w[6] + d) <= 1e-12
+    wf = sc.contact_wrench(0, far)
+    assert np.all(wf[:6] == 0) and np.isinf(wf[6])
