@@ -454,6 +454,13 @@ def main(argv=None) -> int:
             "warm_starts_kept": int(kept), "warm_starts_tried": int(tried),
             "median_guess_residual": float(np.median([s["warm_start_rel_res"] for s in per_env])),
         }
+        # the network's own gate: in how many of the last warm starts it ran, and what its guess
+        # left of the residual the previous steps alone leave (geometric mean)
+        net = [(s["warm_start_rel_res"], s["warm_start_history_rel_res"]) for s in per_env
+               if s["warm_start_network"] and s["warm_start_history_rel_res"] > 0]
+        out["learned_vs_zero"]["network_ran_fraction"] = len(net) / max(1, len(per_env))
+        out["learned_vs_zero"]["network_residual_vs_history"] = (
+            float(np.exp(np.mean([np.log(max(r, 1e-300) / rh) for r, rh in net]))) if net else None)
         print(f"[bench] zero-init and history-only comparisons done", file=sys.stderr, flush=True)
     if batched and not args.no_plain:
         sample = sampled_envs(spec, args.plain_envs, args.seed + 1)

@@ -189,6 +189,12 @@ typedef struct sdx_step_stats {
   int32_t warm_start_accepted;    /* learned warm starts kept / tried */
   int32_t warm_start_tried;
   int32_t fp64_refreshes;
+  int32_t warm_start_network;     /* 1 when the network ran for the last warm start (its gate runs
+                                     the previous Newton steps alone for 8 steps of a Newton
+                                     iteration index when the network's guess left more than half
+                                     of the residual those leave, geometric mean over the
+                                     environments); 0 otherwise */
+  int32_t reserved0;
   double res_norm0;
   double res_norm;
   double linear_rel_res;          /* last linear solve ||r||/||b|| (as the stop test measured it) */
@@ -196,6 +202,8 @@ typedef struct sdx_step_stats {
   double warm_start_rel_res;      /* ||b - A x0||/||b|| of the last learned warm start x0 (the
                                      combination of the network's guess and the two previous
                                      Newton steps; 0: none tried) */
+  double warm_start_history_rel_res; /* the same for the previous Newton steps alone (what the
+                                     start leaves without the network; 0: none tried) */
 } sdx_step_stats;
 
 /* SoftActorParams subset (cpp_api/mochi_structs.h:174-197). */

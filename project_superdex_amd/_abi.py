@@ -121,11 +121,14 @@ class StepStats(C.Structure):
         ("warm_start_accepted", C.c_int32),
         ("warm_start_tried", C.c_int32),
         ("fp64_refreshes", C.c_int32),
+        ("warm_start_network", C.c_int32),
+        ("reserved0", C.c_int32),
         ("res_norm0", C.c_double),
         ("res_norm", C.c_double),
         ("linear_rel_res", C.c_double),
         ("linear_true_rel_res", C.c_double),
         ("warm_start_rel_res", C.c_double),
+        ("warm_start_history_rel_res", C.c_double),
     ]
 
     def as_dict(self):

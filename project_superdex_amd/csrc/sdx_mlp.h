@@ -108,9 +108,10 @@ constexpr double kWarmKeep = 0.2;
 // set 0 of acc: the combination alpha u + beta v + gamma w with the least
 // residual over every nonempty subset of the three (normal equations; subsets
 // whose equations are nearly singular are skipped); ratio[e] = that residual /
-// |b| (0 when not tried), warm[e] = 1 when ratio[e] < kWarmKeep, else the
-// environment starts from zero.
+// |b| (0 when not tried), ratio_hist[e] the same over the subsets without u
+// (what the previous steps alone leave), warm[e] = 1 when ratio[e] <
+// kWarmKeep, else the environment starts from zero.
 void launch_ws_decide_multi(hipStream_t s, int nEnv, const int32_t* solve, const double* acc, int32_t* warm,
-                            double* ratio, double* alpha, double* beta, double* gamma);
+                            double* ratio, double* ratio_hist, double* alpha, double* beta, double* gamma);
 
 }  // namespace sdx

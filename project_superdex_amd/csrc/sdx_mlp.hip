@@ -635,7 +635,8 @@ __device__ bool ws_fit(const double G[3][3], const double h[3], int mask, double
 }
 
 __global__ void k_ws_decide_multi(int nEnv, const int32_t* __restrict__ solve, const double* __restrict__ acc,
-                                  int32_t* __restrict__ warm, double* __restrict__ ratio, double* __restrict__ alpha,
+                                  int32_t* __restrict__ warm, double* __restrict__ ratio,
+                                  double* __restrict__ ratio_hist, double* __restrict__ alpha,
                                   double* __restrict__ beta, double* __restrict__ gamma) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nEnv) return;
@@ -645,6 +646,7 @@ __global__ void k_ws_decide_multi(int nEnv, const int32_t* __restrict__ solve, c
   if (!solve[e]) {
     warm[e] = 0;
     ratio[e] = 0.0;
+    ratio_hist[e] = 0.0;
     return;
   }
   double q[10];  // |b|^2, b.Au, b.Av, b.Aw, |Au|^2, |Av|^2, |Aw|^2, Au.Av, Au.Aw, Av.Aw
@@ -653,7 +655,8 @@ __global__ void k_ws_decide_multi(int nEnv, const int32_t* __restrict__ solve, c
   const double h[3] = {q[1], q[2], q[3]};
   const double G[3][3] = {{q[4], q[7], q[8]}, {q[7], q[5], q[9]}, {q[8], q[9], q[6]}};
   // every nonempty subset of the three guesses; the least residual wins
-  double best = bb, bc[3] = {0.0, 0.0, 0.0};
+  // (best_h: the same over the subsets without u, the previous steps alone)
+  double best = bb, best_h = bb, bc[3] = {0.0, 0.0, 0.0};
   for (int mask = 1; mask < 8; ++mask) {
     double c[3];
     if (!ws_fit(G, h, mask, c)) continue;
@@ -662,16 +665,19 @@ __global__ void k_ws_decide_multi(int nEnv, const int32_t* __restrict__ solve, c
       r2 -= 2.0 * c[i] * h[i];
       for (int j = 0; j < 3; ++j) r2 += c[i] * c[j] * G[i][j];
     }
+    if (!(mask & 1) && r2 < best_h) best_h = r2;
     if (r2 < best) {
       best = r2;
       for (int i = 0; i < 3; ++i) bc[i] = c[i];
     }
   }
   if (best < 0.0) best = 0.0;
+  if (best_h < 0.0) best_h = 0.0;
   const double r = bb > 0.0 ? sqrt(best / bb) : 1.0;
   const bool keep = bb > 0.0 && r < kWarmKeep;
   warm[e] = keep ? 1 : 0;
   ratio[e] = r;
+  ratio_hist[e] = bb > 0.0 ? sqrt(best_h / bb) : 1.0;
   if (keep) {
     alpha[e] = bc[0];
     beta[e] = bc[1];
@@ -680,10 +686,10 @@ __global__ void k_ws_decide_multi(int nEnv, const int32_t* __restrict__ solve, c
 }
 
 void launch_ws_decide_multi(hipStream_t s, int nEnv, const int32_t* solve, const double* acc, int32_t* warm,
-                            double* ratio, double* alpha, double* beta, double* gamma) {
+                            double* ratio, double* ratio_hist, double* alpha, double* beta, double* gamma) {
   if (nEnv == 0) return;
   hipLaunchKernelGGL(k_ws_decide_multi, dim3((nEnv + 63) / 64), dim3(64), 0, s, nEnv, solve, acc, warm, ratio,
-                     alpha, beta, gamma);
+                     ratio_hist, alpha, beta, gamma);
 }
 
 #define SDX_MLP_INST(T)                                                                                          \

@@ -478,6 +478,7 @@ void World<T>::finalize() {
   ws_F_.alloc(size_t(kMlpInPad) * size_t(nV_));
   ws_scale_.alloc(static_cast<size_t>(nEnv));
   ws_ratio_.alloc(static_cast<size_t>(nEnv));
+  ws_ratio_hist_.alloc(static_cast<size_t>(nEnv));
   ws_alpha_.alloc(static_cast<size_t>(nEnv));
   ws_beta_.alloc(static_cast<size_t>(nEnv));
   ws_gamma_.alloc(static_cast<size_t>(nEnv));
@@ -933,7 +934,8 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
         launch_residual64_multi<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
                                            d_col_.p, vals_.p, nB_, xcg<double>(), v, w, res_.p,
                                            rref_.p, rref2_.p, rref3_.p, acc_.p);
-        launch_ws_decide_multi(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p,
+        launch_ws_decide_multi(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_ratio_hist_.p,
+                               ws_alpha_.p,
                                ws_beta_.p, ws_gamma_.p);
       } else {
         launch_residual64<T, double>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
@@ -950,7 +952,8 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
         launch_residual64_multi<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p,
                                       d_col_.p, vals_.p, nB_, xcg<T>(), v, w, res_.p, rref_.p,
                                       rref2_.p, rref3_.p, acc_.p);
-        launch_ws_decide_multi(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_alpha_.p,
+        launch_ws_decide_multi(stream_, nEnv_, warm_mask_.p, acc_.p, warm_mask_.p, ws_ratio_.p, ws_ratio_hist_.p,
+                               ws_alpha_.p,
                                ws_beta_.p, ws_gamma_.p);
       } else {
         launch_residual64<T, T>(stream_, dt_a_.view, sched_.spmv_lanes, nEnv_, warm_mask_.p, d_row_ptr_.p, d_col_.p,
@@ -1138,7 +1141,9 @@ std::vector<EnvCG> World<T>::cg_solve(const sdx_krylov_params& kp, const std::ve
     learned->accepted.assign(size_t(nEnv_), 0);
     learned->ratio.assign(size_t(nEnv_), 0.0);
     fetch(warm_mask_, learned->accepted);
+    learned->ratio_hist.assign(size_t(nEnv_), 0.0);
     fetch(ws_ratio_, learned->ratio);
+    fetch(ws_ratio_hist_, learned->ratio_hist);
   }
   fetch(norms2_, q);
   std::vector<int32_t> tail_abort(1, 0);
@@ -1221,8 +1226,7 @@ void World<T>::set_ws_params(const float* params) {
   ws_w_ = MlpBf16{w, w + kBf16W1, w + kBf16W1 + kBf16W2, w + kBf16W1 + 2 * kBf16W2, b, b + kMlpHid, b + 2 * kMlpHid,
                   b + 3 * kMlpHid};
   ws_on_ = true;
-  for (int& k : ws_skip_) k = 0;
-  for (int& k : ws_seen_) k = 0;
+  wg_ = WarmGate{};
 }
 
 template <class T>
@@ -1434,13 +1438,19 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       // least-residual combination drops (its Gram rows vanish)
       const bool history_only = sp.warm_start == SDX_WARM_START_HISTORY;
       bool learned = first_try && ((sp.warm_start == SDX_WARM_START_LEARNED && ws_on_) || history_only);
-      if (learned && ws_skip_[hk] > 0) {
-        --ws_skip_[hk];
+      if (learned && wg_.skip[hk] > 0) {
+        --wg_.skip[hk];
         learned = false;
       }
+      // the network runs unless its gate holds this index on the previous steps alone
+      bool network = learned && !history_only;
+      if (network && wg_.net_skip[hk] > 0) {
+        --wg_.net_skip[hk];
+        network = false;
+      }
       const bool collecting = first_try && coll_per_env_ > 0 && coll_n_ + int64_t(nE) * coll_per_env_ <= coll_cap_;
-      if ((learned && !history_only) || collecting) ws_features();
-      if (learned && history_only)
+      if (network || collecting) ws_features();
+      if (learned && !network)
         launch_fill<T>(stream_, int64_t(n3), T(0), dx_.p);
       else if (learned)
         launch_mlp_infer<T>(stream_, ws_w_, ws_F_.p, nV_, d_row_env_.p, ws_scale_.p, d_pinned_.p, nullptr, dx_.p);
@@ -1459,14 +1469,21 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       }
       if (learned) {
         int tried = 0, kept = 0;
+        double log_gain = 0.0;  // sum of log(ratio / ratio_hist): what the network's guess adds
         for (int e = 0; e < nE; ++e)
           if (trying[size_t(e)]) {
             ++tried;
             kept += ws.accepted[size_t(e)] ? 1 : 0;
+            if (network)
+              log_gain += std::log(std::max(ws.ratio[size_t(e)], 1e-300) / std::max(ws.ratio_hist[size_t(e)], 1e-300));
           }
         // the first two warm-started solves of an iteration index run before
         // its history of previous Newton steps is filled, and never skip
-        if (++ws_seen_[hk] > 2 && double(kept) < kWarmMinKept * double(tried)) ws_skip_[hk] = kWarmSkipSteps;
+        if (++wg_.seen[hk] > 2) {
+          if (double(kept) < kWarmMinKept * double(tried)) wg_.skip[hk] = kWarmSkipSteps;
+          if (network && tried > 0 && log_gain > double(tried) * std::log(kNetMinGain))
+            wg_.net_skip[hk] = kWarmSkipSteps;
+        }
       }
       std::vector<uint8_t> lin_ok(size_t(nE), 0);
       for (int e = 0; e < nE; ++e) {
@@ -1477,6 +1494,8 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
           s.ws_tried++;
           s.ws_accepted += ws.accepted[size_t(e)] ? 1 : 0;
           s.ws_ratio = ws.ratio[size_t(e)];
+          s.ws_ratio_hist = ws.ratio_hist[size_t(e)];
+          s.ws_network = network ? 1 : 0;
         }
         s.lin_iters += c.iter;
         s.lin_conv = c.status == kCgConverged ? SDX_CONV_CONVERGED
@@ -1631,6 +1650,8 @@ void World<T>::step(double dt, const sdx_solver_params& sp, sdx_step_stats* stat
       o.warm_start_tried = s.ws_tried;
       o.warm_start_accepted = s.ws_accepted;
       o.warm_start_rel_res = s.ws_ratio;
+      o.warm_start_history_rel_res = s.ws_ratio_hist;
+      o.warm_start_network = s.ws_network;
     }
   }
 }
@@ -1983,15 +2004,12 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
   launch_copy<T>(stream_, nh, hist_.p, sh.p);
   const int64_t coll_n0 = coll_n_;
   const uint32_t coll_salt0 = coll_salt_;
-  int ws_skip0[kNewtonHist], ws_seen0[kNewtonHist];
-  std::memcpy(ws_skip0, ws_skip_, sizeof(ws_skip_));
-  std::memcpy(ws_seen0, ws_seen_, sizeof(ws_seen_));
+  const WarmGate wg0 = wg_;
   const std::vector<HostCollider> colliders0 = colliders_;  // a trial step advances their stage start
   auto restore = [&]() {
     colliders_ = colliders0;
     colliders_dirty_ = true;
-    std::memcpy(ws_skip_, ws_skip0, sizeof(ws_skip_));
-    std::memcpy(ws_seen_, ws_seen0, sizeof(ws_seen_));
+    wg_ = wg0;
     launch_copy<T>(stream_, n3, su.p, u_.p);
     launch_copy<T>(stream_, n3, sv.p, v_.p);
     launch_copy<T>(stream_, nh, sh.p, hist_.p);
@@ -2124,9 +2142,11 @@ struct StateHeader {
   uint64_t topology;
   int32_t ws_skip[4];
   int32_t ws_seen[4];
+  int32_t ws_net_skip[4];
+  int32_t reserved;
 };
 constexpr char kStateMagic[8] = {'S', 'D', 'X', 'S', 'T', 'A', 'T', 0};
-constexpr uint32_t kStateVersion = 3;
+constexpr uint32_t kStateVersion = 4;
 }  // namespace
 
 template <class T>
@@ -2153,8 +2173,7 @@ std::unique_ptr<typename World<T>::Snapshot> World<T>::take_snapshot() {
       s->coll_p[6 * c + size_t(d)] = colliders_[c].p[d];
       s->coll_p[6 * c + 3 + size_t(d)] = colliders_[c].p0[d];
     }
-  std::memcpy(s->ws_skip, ws_skip_, sizeof(ws_skip_));
-  std::memcpy(s->ws_seen, ws_seen_, sizeof(ws_seen_));
+  s->wg = wg_;
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   return s;
 }
@@ -2174,8 +2193,7 @@ void World<T>::put_snapshot(const Snapshot& s) {
       colliders_[c].p0[d] = s.coll_p[6 * c + 3 + size_t(d)];
     }
   colliders_dirty_ = true;
-  std::memcpy(ws_skip_, s.ws_skip, sizeof(ws_skip_));
-  std::memcpy(ws_seen_, s.ws_seen, sizeof(ws_seen_));
+  wg_ = s.wg;
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
@@ -2210,9 +2228,7 @@ bool World<T>::equal_states(int32_t a, int32_t b) {
   const Snapshot& A = snapshot(a);
   const Snapshot& B = snapshot(b);
   if (A.u.n != B.u.n || A.hist.n != B.hist.n || A.coll_p != B.coll_p) return false;
-  if (std::memcmp(A.ws_skip, B.ws_skip, sizeof(A.ws_skip)) != 0 ||
-      std::memcmp(A.ws_seen, B.ws_seen, sizeof(A.ws_seen)) != 0)
-    return false;
+  if (std::memcmp(&A.wg, &B.wg, sizeof(WarmGate)) != 0) return false;
   auto same = [](const DBuf<T>& x, const DBuf<T>& y) {
     std::vector<T> hx(x.n), hy(y.n);
     x.download(hx);
@@ -2300,8 +2316,9 @@ void World<T>::capture_bytes(uint8_t* out, size_t cap) {
   hd.n_colliders = int32_t(colliders_.size());
   hd.n_hist = kNewtonHist;
   hd.topology = topology_hash(actors_, colliders_);
-  std::memcpy(hd.ws_skip, ws_skip_, sizeof(hd.ws_skip));
-  std::memcpy(hd.ws_seen, ws_seen_, sizeof(hd.ws_seen));
+  std::memcpy(hd.ws_skip, wg_.skip, sizeof(hd.ws_skip));
+  std::memcpy(hd.ws_seen, wg_.seen, sizeof(hd.ws_seen));
+  std::memcpy(hd.ws_net_skip, wg_.net_skip, sizeof(hd.ws_net_skip));
   uint8_t* o = out;
   std::memcpy(o, &hd, sizeof(hd));
   o += sizeof(hd);
@@ -2355,8 +2372,9 @@ void World<T>::restore_bytes(const uint8_t* in, size_t n) {
   h.resize(size_t(2 * kNewtonHist) * n3);
   std::memcpy(h.data(), q, h.size() * sizeof(T));
   hist_.upload(h);
-  std::memcpy(ws_skip_, hd.ws_skip, sizeof(ws_skip_));
-  std::memcpy(ws_seen_, hd.ws_seen, sizeof(ws_seen_));
+  std::memcpy(wg_.skip, hd.ws_skip, sizeof(wg_.skip));
+  std::memcpy(wg_.seen, hd.ws_seen, sizeof(wg_.seen));
+  std::memcpy(wg_.net_skip, hd.ws_net_skip, sizeof(wg_.net_skip));
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 

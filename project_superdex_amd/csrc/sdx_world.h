@@ -473,8 +473,8 @@ class World final : public WorldBase {
     int conv = SDX_CONV_NONE;
     int lin_iters = 0, ls_iters = 0, lin_conv = SDX_CONV_NONE, refreshes = 0;
     double lin_rel = 0, lin_true_rel = 0;
-    int ws_tried = 0, ws_accepted = 0;
-    double ws_ratio = 0;
+    int ws_tried = 0, ws_accepted = 0, ws_network = 0;
+    double ws_ratio = 0, ws_ratio_hist = 0;
     double dx_norm = std::numeric_limits<double>::infinity();
     bool have_dx = false;
     int psd = SDX_PSD_ALWAYS;  // PSD mode (IfFailAlways turns Always after a failed try)
@@ -502,6 +502,7 @@ class World final : public WorldBase {
   struct WarmStats {
     std::vector<int32_t> accepted;  // learned warm start kept (per environment)
     std::vector<double> ratio;      // |b - A x0| / |b| of the learned guess
+    std::vector<double> ratio_hist; // the same for the previous Newton steps alone
   };
   std::vector<EnvCG> cg_solve(const sdx_krylov_params& kp, const std::vector<int32_t>& solve,
                               std::vector<double>* true_rel, const std::vector<int32_t>* warm,
@@ -631,6 +632,7 @@ class World final : public WorldBase {
   SDX_BUF(float, ws_F_)          // feature rows [nV][16]
   SDX_BUF(double, ws_scale_)     // per-environment scale of features and output
   SDX_BUF(double, ws_ratio_)     // relative residual of the learned guess
+  SDX_BUF(double, ws_ratio_hist_)  // and of the previous Newton steps alone
   SDX_BUF(double, ws_alpha_)     // least-residual multiple of the learned guess
   SDX_BUF(double, ws_beta_)      // and of the two previous Newton steps
   SDX_BUF(double, ws_gamma_)
@@ -661,19 +663,32 @@ class World final : public WorldBase {
   DBuf<float> ws_bpack_{keep_, &stream_};
   MlpBf16 ws_w_{};
   bool ws_on_ = false;
-  // per Newton iteration index: steps left that start from zero because most
-  // guesses of its last warm-started solve were rejected (fewer than
-  // kWarmMinKept of the environments kept theirs)
-  int ws_skip_[kNewtonHist] = {0, 0, 0, 0};
-  int ws_seen_[kNewtonHist] = {0, 0, 0, 0};  // warm-started solves per index (capped use: > 2)
+  // Gates of the warm start, per Newton iteration index (the last
+  // kNewtonHist - 1 share one); both act from the index's third warm-started
+  // solve on (the first two run before its history of previous steps is filled):
+  // - skip: steps left that start from zero because most guesses of its last
+  //   warm-started solve were rejected (fewer than kWarmMinKept of the
+  //   environments kept theirs);
+  // - net_skip: steps left that combine the previous Newton steps alone,
+  //   without running the network, because in its last solve with the network
+  //   the network's guess left more than kNetMinGain of the residual that the
+  //   previous steps alone leave (geometric mean over the environments);
+  // - seen: warm-started solves per index.
+  struct WarmGate {
+    int32_t skip[kNewtonHist] = {0, 0, 0, 0};
+    int32_t seen[kNewtonHist] = {0, 0, 0, 0};
+    int32_t net_skip[kNewtonHist] = {0, 0, 0, 0};
+  };
+  WarmGate wg_;
   static constexpr double kWarmMinKept = 0.5;
+  static constexpr double kNetMinGain = 0.5;
   static constexpr int kWarmSkipSteps = 8;
   // state snapshots under their handles
   struct Snapshot {
     std::vector<DBufBase*> reg;
     DBuf<T> u, v, hist;
     std::vector<double> coll_p;  // collider targets and stage-start positions, 6 per collider
-    int ws_skip[kNewtonHist], ws_seen[kNewtonHist];
+    WarmGate wg;
     explicit Snapshot(hipStream_t* s) : u(reg, s), v(reg, s), hist(reg, s) {}
   };
   std::map<int32_t, std::unique_ptr<Snapshot>> snaps_;

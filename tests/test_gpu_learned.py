@@ -168,9 +168,13 @@ def test_learned_warm_start_on_held_out_seed(trained_model):
     tried = sum(e["warm_start_tried"] for st in st1 for e in st)
     kept = sum(e["warm_start_accepted"] for st in st1 for e in st)
     ratio = np.median([e["warm_start_rel_res"] for st in st1 for e in st])
+    net = [(e["warm_start_rel_res"], e["warm_start_history_rel_res"]) for st in st1 for e in st
+           if e["warm_start_network"]]
+    gain = float(np.exp(np.mean([np.log(r / rh) for r, rh in net]))) if net else 1.0
     print(f"held-out: |net - dx|/|dx| = {err:.3f}; CG iterations zero init {it0}, learned {it1} "
           f"(x{it1 / it0:.3f}), previous steps alone {it2} (x{it2 / it0:.3f}); guesses kept {kept}/{tried}, "
-          f"median residual ratio {ratio:.3f}")
+          f"median residual ratio {ratio:.3f}; network ran in {len(net)} solves, leaving x{gain:.3f} of "
+          f"the residual the previous steps alone leave")
     assert err < 0.9  # the network predicts the Newton step better than zero does
     assert tried == 16 * 16 and ratio < 0.2 and kept > tried // 2
     assert it1 <= 0.90 * it0, (it0, it1)
@@ -247,5 +251,43 @@ def test_learned_warm_start_two_newton_iterations(trained_model):
             assert e["newton_iters"] == 2
             assert e["linear_convergence"] == _abi.CONV_CONVERGED and e["linear_true_rel_res"] <= 1e-6
     for a, b in zip(s0.actors, s1.actors):
+        ua, ub = a.displacements(), b.displacements()
+        assert np.abs(ua - ub).max() <= 1e-3 * np.abs(ua).max()
+
+
+def test_network_gate_runs_the_previous_steps_alone_when_the_network_adds_nothing():
+    """The network's own gate: from an iteration index's third warm-started solve on, a solve
+    whose network guess left more than half of the residual the previous Newton steps alone
+    leave (geometric mean over the environments) makes the next 8 steps of that index combine
+    the previous steps without running the network; then the network is tried again.  A
+    network of noise adds nothing: it runs on steps 1-3, is held off on steps 4-11, runs on
+    step 12 and is held off again.  On every step the combination with the network's guess
+    leaves at most what the previous steps alone leave, and exactly that where the network did
+    not run; the results are those of the history-only start."""
+    spec = batched(103, 14)
+    learned = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
+    learned.warm_start = _abi.WARM_START_LEARNED
+    history = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_MIXED, refresh_every=10)
+    history.warm_start = _abi.WARM_START_HISTORY
+    bad = WarmStartModel(seed=3)
+    bad.params = bad.params * 8.0
+    s1, st1 = run(spec, learned, model=bad)
+    s2, st2 = run(spec, history)
+    ran = [max(e["warm_start_network"] for e in st) for st in st1]
+    print("network ran per step:", ran)
+    assert ran == [1, 1, 1] + [0] * 8 + [1, 0, 0], ran
+    for st, on in zip(st1, ran):
+        for e in st:
+            assert e["warm_start_tried"] == 1
+            assert e["warm_start_network"] == on
+            r, rh = e["warm_start_rel_res"], e["warm_start_history_rel_res"]
+            assert 0.0 < r <= rh
+            if not on:
+                assert r == rh
+    for st in st2:
+        for e in st:
+            assert e["warm_start_network"] == 0
+            assert e["warm_start_rel_res"] == e["warm_start_history_rel_res"]
+    for a, b in zip(s1.actors, s2.actors):
         ua, ub = a.displacements(), b.displacements()
         assert np.abs(ua - ub).max() <= 1e-3 * np.abs(ua).max()
