@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "sdx_layout.h"
+
 namespace sdx {
 
 // Per-environment CG state, one per island, in device memory.
@@ -76,9 +78,6 @@ constexpr int kSpmvThreads = 256;
 constexpr int kSpmvChunk = 16;    // blocks of a row whose loads the CG matvec issues together
 constexpr int kSpmvRows = kSpmvThreads / kSpmvLanes;  // 16 rows per plain matvec tile
 constexpr int kVecThreads = 256;  // rows per vector tile
-// Off-diagonal local pairs (a, b), a < b, of a tet in the order of their index k.
-constexpr int kPairA[6] = {0, 0, 0, 1, 1, 2};
-constexpr int kPairB[6] = {1, 2, 3, 2, 3, 3};
 // Cross-block accumulators: per environment and set, kSlots lines of
 // kSlotStride values (one 128-byte line per slot for doubles).
 constexpr int kSlots = 16;
@@ -115,8 +114,10 @@ struct MeshView {
   const int32_t* item_a;    // [4 n_items] slot, mirror slot, first and end record
   const int32_t* item_b;    // [2 n_items] row | pin bits << kItemRowBits, environment
   const int32_t* eptr;      // [nB+1] off-diagonal records of each upper slot (pair-sorted)
-  const int32_t* pos_v;     // [4 nT] vertex-sorted record of element contribution (tet, a)
-  const uint32_t* pos_e;    // [6 nT] pair-sorted record of pair k of a tet; top bit: transposed
+  int32_t nP;               // element patches
+  const int32_t* patch_t0;  // [nP+1] tets of patch p: [patch_t0[p], patch_t0[p+1])
+  const int32_t* patch_meta_ptr; // [nP+1] description of patch p: words [ptr[p], ptr[p+1]) of patch_meta
+  const uint32_t* patch_meta;    // patch descriptions (layout at kPatchMetaWords)
   const int32_t* pos_vc;    // [3 nF] vertex-sorted record of contact contribution (triangle, a)
   const int32_t* pos_ec;    // [3 nF] pair-sorted record of pair k of a boundary triangle
   uint8_t* coll_flags;      // [nF] bit 0 / bit 1: the triangle's residual / Hessian contact records
@@ -148,20 +149,22 @@ struct FieldPtrs {
 };
 
 // ---- assembly ----
-// Element and contact residuals and Hessians are written straight to their
-// destination order (records): the residual and diagonal block of (tet, a) at
-// pos_v[4 tet + a] of Rv ([.][3]) and Dv ([.][6]: 00 11 22 01 02 12), its
-// off-diagonal block k at pos_e[6 tet + k] of Ev ([.][9], row-major for the
-// (min, max) vertex pair); those of boundary triangle f at pos_vc[3 f + a] and
-// pos_ec[3 f + k].  Every vertex / upper slot owns one contiguous run of
-// records.  Record counts: 4 nT + 3 nF (Rv, Dv), 6 nT + 3 nF (Ev); each array
-// is padded by one 16-byte vector for the gathers' vector loads.
+// Element and contact residuals and Hessians are written to their
+// destination order (records).  The tets of an environment form patches of
+// up to kPatchTets consecutive tets; a patch sums its tets' contributions to
+// each of its vertices and vertex pairs in LDS and writes one record per
+// local vertex (residual in Rv [.][3], diagonal block in Dv [.][6]: 00 11 22
+// 01 02 12) and one per local pair (off-diagonal block in Ev [.][9],
+// row-major for the (min, max) vertex pair).  Boundary triangle f writes its
+// own records at pos_vc[3 f + a] and pos_ec[3 f + k].  Every vertex / upper
+// slot owns one contiguous run of records (one per patch that touches it,
+// then the contact records).  Each array is padded by one 16-byte vector for
+// the gathers' vector loads.
+// (patch descriptions and the local pair order: sdx_layout.h)
 constexpr int kItemRowBits = 26;  // rows of the gather items below 2^26
-constexpr int kTriPairA[3] = {0, 0, 1};
-constexpr int kTriPairB[3] = {1, 2, 2};
 //
 // An assembly runs over a contiguous range of environments (an assembly
-// group): its tets [t0, t1), boundary triangles [f0, f1), diagonal gather
+// group): its tets [t0, t1) in patches [p0, p1), boundary triangles [f0, f1), diagonal gather
 // items [d0, d1) (= its vertices), upper gather items [u0, u1) and vector
 // tiles [tile0, tile1).  Record indices stay global; the record pointers a
 // launch takes are the group's buffers shifted by the group's first record,
@@ -173,6 +176,7 @@ constexpr int kTriPairB[3] = {1, 2, 2};
 struct AsmRange {
   int32_t t0, t1, f0, f1, d0, d1, u0, u1, tile0, tile1;
   int32_t contact_always;
+  int32_t p0, p1;
 };
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,

@@ -35,16 +35,55 @@ namespace sdx {
 
 // ------------------------------------------------------------------ element
 
+// Element patches, one wave each: lane i computes tet i of the patch (F, Smith
+// P, 12-vector residual with stress, gravity, inertia and damping, 12 x 12
+// Hessian with the consistent mass) and stages its contributions in the
+// wave's LDS; the wave then sums them per local vertex and local vertex pair
+// in the fixed order of the patch's lists and writes one record per local
+// vertex (residual, diagonal block) and per local pair (off-diagonal block in
+// (min, max) orientation).  Two phases share one stage region of kStageE
+// values per lane: first the residual and the diagonal blocks (kStageV per
+// tet), then the off-diagonal blocks, which wait in registers meanwhile.  The
+// patch description is copied into LDS with 16-byte loads issued before the
+// element's own loads.  Every lane of a workgroup passes every barrier (the
+// control flow around them depends only on wantH, uniform over the launch).
+constexpr int kStageV = 37;  // residual 12 + diagonal blocks 24, odd lane stride
+constexpr int kStageE = 55;  // off-diagonal blocks 54, odd lane stride
+constexpr int kMetaVecs = (kPatchMetaWords / 4 + 63) / 64;  // 16-byte loads per lane
+static_assert(kPatchTets == 64, "one tet per lane of a wave");
+
+extern __shared__ __align__(16) unsigned char sdx_dyn_lds[];
+
+template <class T>
+constexpr size_t patch_lds_bytes() {
+  return size_t(kStageE) * 64 * sizeof(T) + size_t(kPatchMetaWords) * 4;
+}
+
 template <class T, bool kDamp>
 __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldPtrs<T> f, const T* __restrict__ u,
                                                  const T* __restrict__ u0, const T* __restrict__ v0, T dt,
                                                  T gx, T gy, T gz, int wantH, int project, T* __restrict__ Rv,
                                                  T* __restrict__ Dv, T* __restrict__ Ev) {
-  const int t = ar.t0 + xcd_tile(blockIdx.x, gridDim.x) * int(blockDim.x) + int(threadIdx.x);
-  if (t >= ar.t1) return;
-  const int env = m.tet_env[t];
-  if (!m.env_mask[env]) return;
+  const int wpb = int(blockDim.x) / 64, wave = int(threadIdx.x) / 64, lane = int(threadIdx.x) % 64;
+  const int p = ar.p0 + xcd_tile(blockIdx.x, gridDim.x) * wpb + wave;
+  unsigned char* lds_w = sdx_dyn_lds + size_t(wave) * patch_lds_bytes<T>();
+  T* stage = reinterpret_cast<T*>(lds_w);
+  uint32_t* meta = reinterpret_cast<uint32_t*>(lds_w + size_t(kStageE) * 64 * sizeof(T));
+  // a wave past the range loads the first patch's data and computes nothing
+  const bool havep = p < ar.p1;
+  const int pc = havep ? p : ar.p0;
+  const int mw0 = m.patch_meta_ptr[pc], nv4 = (m.patch_meta_ptr[pc + 1] - mw0) / 4;
+  uint4 mv[kMetaVecs];
+  const uint4* msrc = reinterpret_cast<const uint4*>(m.patch_meta + mw0);
+#pragma unroll
+  for (int k = 0; k < kMetaVecs; ++k) {
+    const int i = lane + 64 * k;
+    mv[k] = msrc[i < nv4 ? i : 0];
+  }
+  const int t0 = m.patch_t0[pc], t1 = m.patch_t0[pc + 1];
+  const int env = m.tet_env[t0];
   const int nT = m.nT;
+  const int t = t0 + lane < t1 ? t0 + lane : t0;
   const int4 tv = reinterpret_cast<const int4*>(m.tets)[t];
   const int nd[4] = {tv.x, tv.y, tv.z, tv.w};
   const T* g = f.geom;
@@ -65,94 +104,150 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldP
       w[3 * a + d] = uu - (s0 + dt * v0[3 * nd[a] + d]);  // u - predicted target (batched_element_utils.h:90-120)
       wd[3 * a + d] = uu - s0;
     }
-  // stiffness damping (fem_stress_damping.h), kappa = beta / dt (mochi_soft.h:259-292)
-  Viscous<T> vis;
-  bool damp = false;
-  if constexpr (kDamp) {
-    const T beta = g[kGBeta * nT + t];
-    damp = beta > T(0);
-    if (damp) {
-      T us[12];
+  uint4* mdst = reinterpret_cast<uint4*>(meta);
 #pragma unroll
-      for (int k = 0; k < 12; ++k) us[k] = ue[k] - wd[k];
-      viscous_setup<T>(dN, ue, us, beta / dt, mu, lam, g[kGBetaGeo * nT + t] != T(0), project != 0, vis);
-    }
+  for (int k = 0; k < kMetaVecs; ++k) {
+    const int i = lane + 64 * k;
+    if (i < nv4) mdst[i] = mv[k];
   }
-  // destination records of the tet, loaded before the element math
-  int pv[4];
-  uint32_t pe[6];
+  const bool on = havep && m.env_mask[env] != 0;  // uniform over the wave
+  const bool mine = on && t0 + lane < t1;
+  T off[54];  // the off-diagonal blocks, staged in the second phase
+  if (mine) {
+    // stiffness damping (fem_stress_damping.h), kappa = beta / dt (mochi_soft.h:259-292)
+    Viscous<T> vis;
+    bool damp = false;
+    if constexpr (kDamp) {
+      const T beta = g[kGBeta * nT + t];
+      damp = beta > T(0);
+      if (damp) {
+        T us[12];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) pv[a] = m.pos_v[4 * t + a];
-  if (wantH)
-#pragma unroll
-    for (int k = 0; k < 6; ++k) pe[k] = m.pos_e[6 * t + k];
-  // consistent mass M_ab = m/20 (1 + delta_ab) (Q4 integrates P1 x P1 exactly)
-  const T idt2 = T(1) / (dt * dt);
-  const T mdamp = alpha > T(0) ? alpha / dt : T(0);
-  const T m20 = mass * T(0.05);
-  const T ms = (idt2 + mdamp) * m20;
-  // Hessian blocks go to their records as soon as they are formed, with the
-  // mass term added: diagonal blocks (6 entries) to the vertex-sorted
-  // records, off-diagonal blocks to the pair-sorted records, transposed
-  // where the pair runs backwards
-  auto emit = [&](int fa, int fb, const T* q) {
-    if (fa == fb) {
-      T dq[6];
-#pragma unroll
-      for (int r = 0; r < 6; ++r) dq[r] = q[r];
-      if (kDamp && damp) viscous_block<T>(vis, dN[fa], dN[fb], true, vol, dq);
-      T* d = Dv + (size_t)pv[fa] * 6;
-      d[0] = dq[0] + T(2) * ms;
-      d[1] = dq[1] + T(2) * ms;
-      d[2] = dq[2] + T(2) * ms;
-      d[3] = dq[3];
-      d[4] = dq[4];
-      d[5] = dq[5];
-      return;
-    }
-    const uint32_t p = pe[fa == 0 ? fb - 1 : (fa == 1 ? fb + 1 : 5)];  // pair (kPairA, kPairB) index
-    T o[9];
-#pragma unroll
-    for (int r = 0; r < 9; ++r) o[r] = q[r];
-    if (kDamp && damp) viscous_block<T>(vis, dN[fa], dN[fb], false, vol, o);
-    o[0] += ms;
-    o[4] += ms;
-    o[8] += ms;
-    T* d = Ev + (size_t)(p & 0x7fffffffu) * 9;
-    if (p >> 31) {
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) d[3 * r + c] = o[3 * c + r];
-    } else {
-#pragma unroll
-      for (int r = 0; r < 9; ++r) d[r] = o[r];
-    }
-  };
-  // the residual record of each vertex: stress, inertia of the consistent
-  // mass and gravity (fem_gravity.h:108-118: res_f -= rho g vol / 4),
-  // written before the Hessian is formed
-  auto on_res = [&](const T* res_in) {
-    T res[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) res[k] = res_in[k];
-    if (kDamp && damp) viscous_res<T>(vis, dN, vol, res);
-    const T gq = T(0.25) * mass;
-    const T sw[3] = {w[0] + w[3] + w[6] + w[9], w[1] + w[4] + w[7] + w[10], w[2] + w[5] + w[8] + w[11]};
-    const T sd[3] = {wd[0] + wd[3] + wd[6] + wd[9], wd[1] + wd[4] + wd[7] + wd[10], wd[2] + wd[5] + wd[8] + wd[11]};
-    const T gv[3] = {gx, gy, gz};
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      T* re = Rv + (size_t)pv[a] * 3;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        T r = res[3 * a + d] - gv[d] * gq + idt2 * m20 * (w[3 * a + d] + sw[d]);
-        if (mdamp > T(0)) r += mdamp * m20 * (wd[3 * a + d] + sd[d]);
-        re[d] = r;
+        for (int k = 0; k < 12; ++k) us[k] = ue[k] - wd[k];
+        viscous_setup<T>(dN, ue, us, beta / dt, mu, lam, g[kGBetaGeo * nT + t] != T(0), project != 0, vis);
       }
     }
-  };
-  smith_element<T>(dN, ue, vol, mu, lam, wantH != 0, project != 0, on_res, emit);
+    // consistent mass M_ab = m/20 (1 + delta_ab) (Q4 integrates P1 x P1 exactly)
+    const T idt2 = T(1) / (dt * dt);
+    const T mdamp = alpha > T(0) ? alpha / dt : T(0);
+    const T m20 = mass * T(0.05);
+    const T ms = (idt2 + mdamp) * m20;
+    T* sv = stage + lane * kStageV;
+    // Hessian blocks with the mass term added: diagonal blocks (6 entries)
+    // to the stage, off-diagonal blocks (pair index k of (kPairA, kPairB))
+    // to registers
+    auto emit = [&](int fa, int fb, const T* q) {
+      if (fa == fb) {
+        T dq[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) dq[r] = q[r];
+        if (kDamp && damp) viscous_block<T>(vis, dN[fa], dN[fb], true, vol, dq);
+        T* d = sv + 12 + 6 * fa;
+        d[0] = dq[0] + T(2) * ms;
+        d[1] = dq[1] + T(2) * ms;
+        d[2] = dq[2] + T(2) * ms;
+        d[3] = dq[3];
+        d[4] = dq[4];
+        d[5] = dq[5];
+        return;
+      }
+      const int pk = fa == 0 ? fb - 1 : (fa == 1 ? fb + 1 : 5);
+      T o[9];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) o[r] = q[r];
+      if (kDamp && damp) viscous_block<T>(vis, dN[fa], dN[fb], false, vol, o);
+      o[0] += ms;
+      o[4] += ms;
+      o[8] += ms;
+#pragma unroll
+      for (int r = 0; r < 9; ++r) off[9 * pk + r] = o[r];
+    };
+    // the residual of each vertex: stress, inertia of the consistent mass and
+    // gravity (fem_gravity.h:108-118: res_f -= rho g vol / 4), staged before
+    // the Hessian is formed
+    auto on_res = [&](const T* res_in) {
+      T res[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) res[k] = res_in[k];
+      if (kDamp && damp) viscous_res<T>(vis, dN, vol, res);
+      const T gq = T(0.25) * mass;
+      const T sw[3] = {w[0] + w[3] + w[6] + w[9], w[1] + w[4] + w[7] + w[10], w[2] + w[5] + w[8] + w[11]};
+      const T sd[3] = {wd[0] + wd[3] + wd[6] + wd[9], wd[1] + wd[4] + wd[7] + wd[10], wd[2] + wd[5] + wd[8] + wd[11]};
+      const T gv[3] = {gx, gy, gz};
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          T r = res[3 * a + d] - gv[d] * gq + idt2 * m20 * (w[3 * a + d] + sw[d]);
+          if (mdamp > T(0)) r += mdamp * m20 * (wd[3 * a + d] + sd[d]);
+          sv[3 * a + d] = r;
+        }
+    };
+    smith_element<T>(dN, ue, vol, mu, lam, wantH != 0, project != 0, on_res, emit);
+  }
+  __syncthreads();
+  const uint32_t hd = meta[0];
+  const int nLV = int(hd & 1023u), nLE = int((hd >> 10) & 1023u), nS = nLV + nLE;
+  const uint32_t* rec = meta + 1;
+  const uint16_t* ls = reinterpret_cast<const uint16_t*>(meta + 1 + nS);
+  const uint16_t* ent = reinterpret_cast<const uint16_t*>(meta + 1 + nS + (nS + 2) / 2);
+  // phase 1: residual and diagonal block of every local vertex
+  if (on)
+    for (int j = lane; j < nLV; j += 64) {
+      T r0 = T(0), r1 = T(0), r2 = T(0), d[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+      const int q1 = ls[j + 1];
+      for (int q = ls[j]; q < q1; ++q) {
+        const int c = ent[q];
+        const T* sp = stage + (c >> 2) * kStageV;
+        const int a = c & 3;
+        r0 += sp[3 * a];
+        r1 += sp[3 * a + 1];
+        r2 += sp[3 * a + 2];
+        if (wantH)
+#pragma unroll
+          for (int k = 0; k < 6; ++k) d[k] += sp[12 + 6 * a + k];
+      }
+      const size_t rp = rec[j];
+      Rv[3 * rp] = r0;
+      Rv[3 * rp + 1] = r1;
+      Rv[3 * rp + 2] = r2;
+      if (wantH)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Dv[6 * rp + size_t(k)] = d[k];
+    }
+  if (!wantH) return;
+  __syncthreads();
+  if (mine) {
+    T* sv = stage + lane * kStageE;
+#pragma unroll
+    for (int k = 0; k < 54; ++k) sv[k] = off[k];
+  }
+  __syncthreads();
+  // phase 2: off-diagonal block of every local pair, (min, max) orientation
+  if (on)
+    for (int j = lane; j < nLE; j += 64) {
+      T o[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) o[k] = T(0);
+      const int q1 = ls[nLV + j + 1];
+      for (int q = ls[nLV + j]; q < q1; ++q) {
+        const int c = ent[q];
+        const int id = c & 0x7fff;
+        const T* sp = stage + (id / 6) * kStageE + 9 * (id % 6);
+        if (c >> 15) {
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) o[3 * r + cc] += sp[3 * cc + r];
+        } else {
+#pragma unroll
+          for (int k = 0; k < 9; ++k) o[k] += sp[k];
+        }
+      }
+      const size_t rp = rec[nLV + j];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Ev[9 * rp + size_t(k)] = o[k];
+    }
 }
 
 // ------------------------------------------------------------------ contact
@@ -1810,17 +1905,30 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs
       hipLaunchKernelGGL(kern, grid, block, 0, s, __VA_ARGS__);                                        \
   } while (0)
 
+// One wave per patch, block / 64 patches per workgroup (fewer where their
+// LDS would pass 64 KB).
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
                     const T* u0, const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv,
                     T* Ev, int block, const KernelEvents* ev, bool damp) {
-  if (r.t1 <= r.t0) return;
+  if (r.p1 <= r.p0) return;
+  int wpb = block / 64;
+  while (wpb > 1 && size_t(wpb) * patch_lds_bytes<T>() > size_t(65536)) --wpb;
+  if (wpb < 1) wpb = 1;
+  const size_t lds = size_t(wpb) * patch_lds_bytes<T>();
+  const dim3 grid(blocks_for(r.p1 - r.p0, wpb)), blk(unsigned(64 * wpb));
+  auto go = [&](auto kern) {
+    if (ev)
+      hipExtLaunchKernelGGL(kern, grid, blk, lds, s, ev->start, ev->stop, 0, m, r, f, u, u0, v0, T(dt), T(grav[0]),
+                            T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev);
+    else
+      hipLaunchKernelGGL(kern, grid, blk, lds, s, m, r, f, u, u0, v0, T(dt), T(grav[0]), T(grav[1]), T(grav[2]),
+                         wantH, project, Rv, Dv, Ev);
+  };
   if (damp)
-    SDX_LAUNCH(ev, (k_element<T, true>), dim3(blocks_for(r.t1 - r.t0, block)), dim3(block), s, m, r, f, u, u0, v0,
-               T(dt), T(grav[0]), T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev);
+    go(k_element<T, true>);
   else
-    SDX_LAUNCH(ev, (k_element<T, false>), dim3(blocks_for(r.t1 - r.t0, block)), dim3(block), s, m, r, f, u, u0, v0,
-               T(dt), T(grav[0]), T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev);
+    go(k_element<T, false>);
 }
 
 template <class T>

@@ -25,6 +25,7 @@
 
 #include "../../include/sdx_fem.h"
 #include "sdx_kernels.h"
+#include "sdx_maps.h"
 #include <type_traits>
 
 #include "sdx_world.h"
@@ -45,38 +46,6 @@ void hip_check(hipError_t e, const char* what) {
 // ------------------------------------------------------------------ set-up helpers
 
 namespace {
-
-struct Pattern {
-  std::vector<int32_t> ptr, col;
-};
-
-// BSR pattern of one actor: node-to-node adjacency through tets, sorted.
-Pattern BuildPattern(int nV, const std::vector<int32_t>& tets) {
-  const int nT = int(tets.size() / 4);
-  std::vector<std::vector<int32_t>> adj(static_cast<size_t>(nV));
-  for (int i = 0; i < nV; ++i) adj[size_t(i)].push_back(i);
-  for (int t = 0; t < nT; ++t)
-    for (int a = 0; a < 4; ++a)
-      for (int b = 0; b < 4; ++b) adj[size_t(tets[4 * t + a])].push_back(tets[4 * t + b]);
-  Pattern p;
-  p.ptr.assign(size_t(nV) + 1, 0);
-  for (int i = 0; i < nV; ++i) {
-    auto& v = adj[size_t(i)];
-    std::sort(v.begin(), v.end());
-    v.erase(std::unique(v.begin(), v.end()), v.end());
-    p.ptr[size_t(i) + 1] = p.ptr[size_t(i)] + int32_t(v.size());
-  }
-  p.col.reserve(size_t(p.ptr.back()));
-  for (int i = 0; i < nV; ++i) p.col.insert(p.col.end(), adj[size_t(i)].begin(), adj[size_t(i)].end());
-  return p;
-}
-
-int FindSlot(const Pattern& p, int i, int j) {
-  auto b = p.col.begin() + p.ptr[size_t(i)], e = p.col.begin() + p.ptr[size_t(i) + 1];
-  auto it = std::lower_bound(b, e, j);
-  if (it == e || *it != j) throw std::runtime_error("BSR slot not in pattern");
-  return int(it - p.col.begin());
-}
 
 // Boundary triangles: tet faces used once, oriented with the normal away from
 // the opposite vertex.
@@ -115,17 +84,6 @@ void BuildBoundary(const std::vector<double>& X, const std::vector<int32_t>& tet
     }
     i = j;
   }
-}
-
-// Stable counting sort of (key, code) pairs into CSR form.
-void CountingCsr(int nKeys, const std::vector<int32_t>& keys, const std::vector<uint32_t>& codes,
-                 std::vector<int32_t>& ptr, std::vector<uint32_t>& out) {
-  ptr.assign(size_t(nKeys) + 1, 0);
-  for (int32_t k : keys) ptr[size_t(k) + 1]++;
-  for (int i = 0; i < nKeys; ++i) ptr[size_t(i) + 1] += ptr[size_t(i)];
-  out.resize(codes.size());
-  std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
-  for (size_t i = 0; i < keys.size(); ++i) out[size_t(pos[size_t(keys[i])]++)] = codes[i];
 }
 
 void BuildTiles(const std::vector<int32_t>& env_v0, int nEnv, int rows_per_tile, HostTiles& t) {
@@ -226,7 +184,10 @@ void World<T>::finalize() {
   std::vector<int32_t> vtx_keys;
   std::vector<uint32_t> vtx_codes;
   std::vector<double> contact(size_t(kContactParams) * size_t(nEnv));
+  PatchBuilder patches;
+  env_p0_.assign(size_t(nEnv) + 1, 0);
   for (int e = 0; e < nEnv; ++e) {
+    env_p0_[size_t(e)] = patches.count();
     auto& A = actors_[size_t(e)];
     const int v0 = env_v0_[size_t(e)], t0 = env_t0[size_t(e)];
     Pattern P = BuildPattern(A.nV, A.tets);
@@ -282,19 +243,12 @@ void World<T>::finalize() {
       geom[size_t(kGBeta) * size_t(nT_) + size_t(gt)] = T(A.params.stiffness_damping);
       geom[size_t(kGBetaGeo) * size_t(nT_) + size_t(gt)] = T(A.params.stiffness_damping_geometric_term ? 1 : 0);
       if (A.params.stiffness_damping > 0) stiff_damp_ = true;
-      for (int a = 0; a < 4; ++a) {
-        vtx_keys.push_back(v0 + tv[a]);
-        vtx_codes.push_back(uint32_t(gt) * 4u + uint32_t(a));
-      }
-      // off-diagonal pairs (a < b) go to the upper slot (min, max) of their
-      // vertex pair; the flag bit marks blocks stored transposed
-      for (int k = 0; k < 6; ++k) {
-        const int a = kPairA[k], b = kPairB[k];
-        const int va = tv[a], vb = tv[b];
-        edge_keys.push_back(b0 + FindSlot(P, std::min(va, vb), std::max(va, vb)));
-        edge_codes.push_back((uint32_t(gt) * 6u + uint32_t(k)) | (va > vb ? 0x80000000u : 0u));
-      }
     }
+    // element patches: runs of kPatchTets consecutive tets, each with its
+    // local vertices and vertex pairs (upper slots), sorted, one record each
+    for (int pt0 = 0; pt0 < A.nT; pt0 += kPatchTets)
+      patches.add(P, b0, v0, t0 + pt0, &A.tets[size_t(4 * pt0)], std::min(kPatchTets, A.nT - pt0), vtx_keys,
+                  vtx_codes, edge_keys, edge_codes);
     std::vector<int32_t> atris;
     BuildBoundary(A.X, A.tets, atris);
     const int f0 = int(tri_env.size());
@@ -324,6 +278,7 @@ void World<T>::finalize() {
     cp[6] = A.params.friction_falloff_vel;
     cp[7] = A.params.normal_viscous_damping_coefficient;
   }
+  env_p0_[size_t(nEnv)] = patches.count();
   nB_ = int(col.size());
   nF_ = int(tri_env.size());
   h_row_ptr_ = row_ptr;
@@ -345,27 +300,30 @@ void World<T>::finalize() {
     env_rv0_[size_t(e)] = vtx_ptr[size_t(env_v0_[size_t(e)])];
     env_re0_[size_t(e)] = eptr[size_t(row_ptr[size_t(env_v0_[size_t(e)])])];
   }
-  // record of every element and contact contribution: the vertex-sorted
-  // position of (tet, a) / (triangle, a) for residuals and diagonal blocks,
-  // the pair-sorted position of (tet, pair) / (triangle, pair) for
-  // off-diagonal blocks (flag bit 31 of a tet pair: stored transposed)
-  std::vector<int32_t> pos_v(size_t(4) * size_t(nT_)), pos_vc(size_t(3) * size_t(nF_));
+  // record of every patch vertex / pair and contact contribution: the
+  // vertex-sorted position of (patch, local vertex) / (triangle, a) for
+  // residuals and diagonal blocks, the pair-sorted position of (patch, local
+  // pair) / (triangle, pair) for off-diagonal blocks
+  std::vector<int32_t> pos_lv(patches.n_local_vertices()), pos_vc(size_t(3) * size_t(nF_));
   for (size_t q = 0; q < vtx_code.size(); ++q) {
     const uint32_t code = vtx_code[q];
     if (code >> 31)
       pos_vc[size_t(code & 0x7fffffffu)] = int32_t(q);
     else
-      pos_v[size_t(code)] = int32_t(q);
+      pos_lv[size_t(code)] = int32_t(q);
   }
-  std::vector<uint32_t> pos_e(size_t(6) * size_t(nT_));
-  std::vector<int32_t> pos_ec(size_t(3) * size_t(nF_));
+  std::vector<int32_t> pos_le(patches.n_local_pairs()), pos_ec(size_t(3) * size_t(nF_));
   for (size_t q = 0; q < ecode.size(); ++q) {
     const uint32_t code = ecode[q];
     if (code & 0x40000000u)
       pos_ec[size_t(code & 0x3fffffffu)] = int32_t(q);
     else
-      pos_e[size_t(code & 0x7fffffffu)] = uint32_t(q) | (code & 0x80000000u);
+      pos_le[size_t(code)] = int32_t(q);
   }
+  std::vector<int32_t> patch_t0, patch_meta_ptr;
+  std::vector<uint32_t> patch_meta;
+  patches.finish(pos_lv, pos_le, patch_t0, patch_meta_ptr, patch_meta);
+  nP_ = patches.count();
   // gather work items: the diagonal slots, then the upper slots (i < j),
   // each with its mirror (j, i), record range, row with the Dirichlet flags of
   // the row and column DoFs, and environment; lower slots are written by the
@@ -429,8 +387,9 @@ void World<T>::finalize() {
   d_item_a_.upload(item_a);
   d_item_b_.upload(item_b);
   n_items_ = int(item_b.size() / 2);
-  d_pos_v_.upload(pos_v);
-  d_pos_e_.upload(pos_e);
+  d_patch_t0_.upload(patch_t0);
+  d_patch_meta_ptr_.upload(patch_meta_ptr);
+  d_patch_meta_.upload(patch_meta);
   d_pos_vc_.upload(pos_vc);
   d_pos_ec_.upload(pos_ec);
   d_vtx_ptr_.upload(vtx_ptr);
@@ -515,8 +474,10 @@ void World<T>::finalize() {
   mesh_.item_a = d_item_a_.p;
   mesh_.item_b = d_item_b_.p;
   mesh_.n_items = n_items_;
-  mesh_.pos_v = d_pos_v_.p;
-  mesh_.pos_e = d_pos_e_.p;
+  mesh_.nP = nP_;
+  mesh_.patch_t0 = d_patch_t0_.p;
+  mesh_.patch_meta_ptr = d_patch_meta_ptr_.p;
+  mesh_.patch_meta = d_patch_meta_.p;
   mesh_.pos_vc = d_pos_vc_.p;
   mesh_.pos_ec = d_pos_ec_.p;
   mesh_.coll_flags = d_coll_flags_.p;
@@ -729,6 +690,8 @@ void World<T>::plan_assembly(int n) {
     AsmGroup G{};
     G.r.t0 = env_t0_[size_t(e)];
     G.r.t1 = env_t0_[size_t(e1)];
+    G.r.p0 = env_p0_[size_t(e)];
+    G.r.p1 = env_p0_[size_t(e1)];
     G.r.f0 = env_f0_[size_t(e)];
     G.r.f1 = env_f0_[size_t(e1)];
     G.r.d0 = env_v0_[size_t(e)];
@@ -740,7 +703,8 @@ void World<T>::plan_assembly(int n) {
     G.rv0 = env_rv0_[size_t(e)];
     G.re0 = env_re0_[size_t(e)];
     // every range within its array: the kernels index with these unchecked
-    if (G.r.t0 < 0 || G.r.t1 > nT_ || G.r.t0 > G.r.t1 || G.r.f0 < 0 || G.r.f1 > nF_ || G.r.f0 > G.r.f1 ||
+    if (G.r.t0 < 0 || G.r.t1 > nT_ || G.r.t0 > G.r.t1 || G.r.p0 < 0 || G.r.p1 > nP_ || G.r.p0 > G.r.p1 ||
+        G.r.f0 < 0 || G.r.f1 > nF_ || G.r.f0 > G.r.f1 ||
         G.r.d0 < 0 || G.r.d1 > nV_ || G.r.d0 > G.r.d1 || G.r.u0 < nV_ || G.r.u1 > n_items_ || G.r.u0 > G.r.u1 ||
         G.r.tile0 < 0 || G.r.tile1 > int(tiles_v_.env.size()) || G.r.tile0 > G.r.tile1)
       throw std::logic_error("assembly group range out of bounds");

@@ -531,7 +531,8 @@ class World final : public WorldBase {
   int asm_groups(const sdx_schedule& sc) const;
   void plan_assembly(int n);
   T* group_records(T* buf, int W, int64_t rec0) const;
-  std::vector<int32_t> env_t0_, env_f0_, env_u0_, env_rv0_, env_re0_;
+  std::vector<int32_t> env_t0_, env_f0_, env_u0_, env_rv0_, env_re0_, env_p0_;
+  int nP_ = 0;  // element patches
   void issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed, int cls);
   void issue_cg_tail(const CgConfig& cfg, int par, int k0, int n);
   void plan_groups(int n);
@@ -585,8 +586,9 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, d_eptr_)
   SDX_BUF(int32_t, d_item_a_)
   SDX_BUF(int32_t, d_item_b_)
-  SDX_BUF(int32_t, d_pos_v_)
-  SDX_BUF(uint32_t, d_pos_e_)
+  SDX_BUF(int32_t, d_patch_t0_)
+  SDX_BUF(int32_t, d_patch_meta_ptr_)
+  SDX_BUF(uint32_t, d_patch_meta_)
   SDX_BUF(int32_t, d_pos_vc_)
   SDX_BUF(int32_t, d_pos_ec_)
   SDX_BUF(int32_t, d_vtx_ptr_)

@@ -104,12 +104,17 @@ class SoftActor:
 class RigidActor:
     def __init__(self, scene: "Scene", index: int, kind: str = "plane", position=(0.0, 0.0, 0.0), actor: int = -1):
         self.scene, self.index, self.kind, self.actor = scene, index, kind, actor
-        self.position = np.array(position, dtype=np.float64)  # the current target
+        scene._put_collider_position(index, position)
+
+    @property
+    def position(self) -> np.ndarray:
+        """The current target (a copy)."""
+        return self.scene._coll_pos[self.index].copy()
 
     def set_position(self, pos):
         pos = np.asarray(pos, float)
         check(lib().sdx_set_rigid_position(self.scene._w, self.index, _d(pos)))
-        self.position = pos.copy()
+        self.scene._coll_pos[self.index] = pos
 
 
 class EnvScene:
@@ -143,7 +148,20 @@ class Scene:
         self.precision = precision
         self.actors: list[SoftActor] = []
         self.colliders: list[RigidActor] = []
+        # collider targets, row k for collider k (capacity doubles as colliders are added)
+        self._coll_pos_buf = np.zeros((0, 3))
         self.solver_params: _abi.SolverParams | None = None  # what SceneBatchExecutorV3 steps with
+
+    @property
+    def _coll_pos(self) -> np.ndarray:
+        return self._coll_pos_buf[:len(self.colliders)]
+
+    def _put_collider_position(self, index: int, position):
+        if index >= len(self._coll_pos_buf):
+            grown = np.zeros((max(16, 2 * len(self._coll_pos_buf), index + 1), 3))
+            grown[:len(self._coll_pos_buf)] = self._coll_pos_buf
+            self._coll_pos_buf = grown
+        self._coll_pos_buf[index] = np.asarray(position, dtype=np.float64)
 
     def close(self):
         if getattr(self, "_w", None):
@@ -192,9 +210,7 @@ class Scene:
                                        dtype=np.int32)
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(len(idx), 3)
         check(lib().sdx_set_rigid_positions(self._w, len(idx), idx.ctypes.data_as(_ip), pos.ctypes.data_as(_dp)))
-        for k, i in enumerate(idx):
-            if 0 <= i < len(self.colliders):
-                self.colliders[i].position = pos[k].copy()
+        self._coll_pos_buf[idx] = pos  # (the library checked every index)
 
     def env(self, index: int) -> "EnvScene":
         """Environment `index` (soft actor `index` and the colliders attached to it) as a scene
