@@ -117,6 +117,7 @@ struct MeshView {
   int32_t nP;               // element patches
   const int32_t* patch_t0;  // [nP+1] tets of patch p: [patch_t0[p], patch_t0[p+1])
   const int32_t* patch_meta_ptr; // [nP+1] description of patch p: words [ptr[p], ptr[p+1]) of patch_meta
+  const int32_t* patch_meta_mid; // [nP] first word of its pair part
   const uint32_t* patch_meta;    // patch descriptions (layout at kPatchMetaWords)
   const int32_t* pos_vc;    // [3 nF] vertex-sorted record of contact contribution (triangle, a)
   const int32_t* pos_ec;    // [3 nF] pair-sorted record of pair k of a boundary triangle
@@ -181,7 +182,8 @@ struct AsmRange {
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
                     const T* u0, const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv,
-                    T* Ev, int block = 128, const KernelEvents* ev = nullptr, bool damp = false);
+                    T* Ev, int block = 128, const KernelEvents* ev = nullptr, bool damp = false,
+                    const int32_t* plist = nullptr);
 // u0: the stage-start displacement (relative sliding of the samples).
 template <class T>
 void launch_contact(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,

@@ -63,16 +63,20 @@ template <class T, bool kDamp>
 __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldPtrs<T> f, const T* __restrict__ u,
                                                  const T* __restrict__ u0, const T* __restrict__ v0, T dt,
                                                  T gx, T gy, T gz, int wantH, int project, T* __restrict__ Rv,
-                                                 T* __restrict__ Dv, T* __restrict__ Ev) {
+                                                 T* __restrict__ Dv, T* __restrict__ Ev,
+                                                 const int32_t* __restrict__ plist) {
   const int wpb = int(blockDim.x) / 64, wave = int(threadIdx.x) / 64, lane = int(threadIdx.x) % 64;
-  const int p = ar.p0 + xcd_tile(blockIdx.x, gridDim.x) * wpb + wave;
+  // patches [p0, p1), or entries [p0, p1) of the patch list plist
+  const int q = ar.p0 + xcd_tile(blockIdx.x, gridDim.x) * wpb + wave;
   unsigned char* lds_w = sdx_dyn_lds + size_t(wave) * patch_lds_bytes<T>();
   T* stage = reinterpret_cast<T*>(lds_w);
   uint32_t* meta = reinterpret_cast<uint32_t*>(lds_w + size_t(kStageE) * 64 * sizeof(T));
   // a wave past the range loads the first patch's data and computes nothing
-  const bool havep = p < ar.p1;
-  const int pc = havep ? p : ar.p0;
-  const int mw0 = m.patch_meta_ptr[pc], nv4 = (m.patch_meta_ptr[pc + 1] - mw0) / 4;
+  const bool havep = q < ar.p1;
+  const int qc = havep ? q : ar.p0;
+  const int pc = plist ? plist[qc] : qc;
+  // the whole description, or its vertex part for a residual
+  const int mw0 = m.patch_meta_ptr[pc], nv4 = ((wantH ? m.patch_meta_ptr[pc + 1] : m.patch_meta_mid[pc]) - mw0) / 4;
   uint4 mv[kMetaVecs];
   const uint4* msrc = reinterpret_cast<const uint4*>(m.patch_meta + mw0);
 #pragma unroll
@@ -187,10 +191,10 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldP
   }
   __syncthreads();
   const uint32_t hd = meta[0];
-  const int nLV = int(hd & 1023u), nLE = int((hd >> 10) & 1023u), nS = nLV + nLE;
+  const int nLV = int(hd & 1023u), nLE = int((hd >> 10) & 1023u);
   const uint32_t* rec = meta + 1;
-  const uint16_t* ls = reinterpret_cast<const uint16_t*>(meta + 1 + nS);
-  const uint16_t* ent = reinterpret_cast<const uint16_t*>(meta + 1 + nS + (nS + 2) / 2);
+  const uint16_t* ls = reinterpret_cast<const uint16_t*>(meta + 1 + nLV);
+  const uint16_t* ent = ls + nLV + 1;
   // phase 1: residual and diagonal block of every local vertex
   if (on)
     for (int j = lane; j < nLV; j += 64) {
@@ -224,14 +228,17 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldP
   }
   __syncthreads();
   // phase 2: off-diagonal block of every local pair, (min, max) orientation
+  const uint32_t* prec = meta + PatchVertexWords(nLV, int(hd >> 20));
+  const uint16_t* pls = reinterpret_cast<const uint16_t*>(prec + nLE);
+  const uint16_t* pent = pls + nLE + 1;
   if (on)
     for (int j = lane; j < nLE; j += 64) {
       T o[9];
 #pragma unroll
       for (int k = 0; k < 9; ++k) o[k] = T(0);
-      const int q1 = ls[nLV + j + 1];
-      for (int q = ls[nLV + j]; q < q1; ++q) {
-        const int c = ent[q];
+      const int q1 = pls[j + 1];
+      for (int q = pls[j]; q < q1; ++q) {
+        const int c = pent[q];
         const int id = c & 0x7fff;
         const T* sp = stage + (id / 6) * kStageE + 9 * (id % 6);
         if (c >> 15) {
@@ -244,7 +251,7 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldP
           for (int k = 0; k < 9; ++k) o[k] += sp[k];
         }
       }
-      const size_t rp = rec[nLV + j];
+      const size_t rp = prec[j];
 #pragma unroll
       for (int k = 0; k < 9; ++k) Ev[9 * rp + size_t(k)] = o[k];
     }
@@ -1910,7 +1917,7 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs
 template <class T>
 void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
                     const T* u0, const T* v0, double dt, const double grav[3], int wantH, int project, T* Rv, T* Dv,
-                    T* Ev, int block, const KernelEvents* ev, bool damp) {
+                    T* Ev, int block, const KernelEvents* ev, bool damp, const int32_t* plist) {
   if (r.p1 <= r.p0) return;
   int wpb = block / 64;
   while (wpb > 1 && size_t(wpb) * patch_lds_bytes<T>() > size_t(65536)) --wpb;
@@ -1920,10 +1927,10 @@ void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const F
   auto go = [&](auto kern) {
     if (ev)
       hipExtLaunchKernelGGL(kern, grid, blk, lds, s, ev->start, ev->stop, 0, m, r, f, u, u0, v0, T(dt), T(grav[0]),
-                            T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev);
+                            T(grav[1]), T(grav[2]), wantH, project, Rv, Dv, Ev, plist);
     else
       hipLaunchKernelGGL(kern, grid, blk, lds, s, m, r, f, u, u0, v0, T(dt), T(grav[0]), T(grav[1]), T(grav[2]),
-                         wantH, project, Rv, Dv, Ev);
+                         wantH, project, Rv, Dv, Ev, plist);
   };
   if (damp)
     go(k_element<T, true>);
@@ -2200,7 +2207,7 @@ void launch_energy(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, cons
 #define SDX_INST_T(T)                                                                                            \
   template void launch_element<T>(hipStream_t, const MeshView&, const AsmRange&, const FieldPtrs<T>&, const T*, \
                                   const T*, const T*, double, const double*, int, int, T*, T*, T*, int,         \
-                                  const KernelEvents*, bool);                                                   \
+                                  const KernelEvents*, bool, const int32_t*);                                   \
   template void launch_contact<T>(hipStream_t, const MeshView&, const AsmRange&, const FieldPtrs<T>&, const T*, \
                                   const T*, T, int, T*, T*, T*);                                                \
   template void launch_gather_bsr<T>(hipStream_t, const MeshView&, const AsmRange&, const T*, const T*, int, T*, \

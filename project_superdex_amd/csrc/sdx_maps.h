@@ -126,28 +126,48 @@ class PatchBuilder {
     le_total_ += size_t(nLE);
   }
 
+  // Descriptions of every patch, in order: patch p occupies words
+  // [meta_ptr[p], meta_ptr[p + 1]) of meta, its pair part starts at word
+  // meta_mid[p].
   void finish(const std::vector<int32_t>& pos_lv, const std::vector<int32_t>& pos_le, std::vector<int32_t>& t0,
-              std::vector<int32_t>& meta_ptr, std::vector<uint32_t>& meta) const {
+              std::vector<int32_t>& meta_ptr, std::vector<int32_t>& meta_mid, std::vector<uint32_t>& meta) const {
     const int n = count();
     t0.assign(t0_.begin(), t0_.end());
     t0.push_back(n ? t0_.back() + nt_.back() : 0);
     meta_ptr.assign(1, 0);
+    meta_mid.clear();
     meta.clear();
     size_t lvo = 0, leo = 0;
+    std::vector<uint16_t> h;
+    auto put16 = [&]() {
+      for (size_t k = 0; k < h.size(); k += 2)
+        meta.push_back(uint32_t(h[k]) | (k + 1 < h.size() ? uint32_t(h[k + 1]) << 16 : 0u));
+    };
+    auto pad4 = [&](size_t w0) {
+      while ((meta.size() - w0) % 4) meta.push_back(0);
+    };
     for (int p = 0; p < n; ++p) {
       const int nt = nt_[size_t(p)], nLV = n_lv_[size_t(p)], nLE = n_le_[size_t(p)], nS = nLV + nLE;
+      const uint16_t* starts = &lists_[list0_[size_t(p)]];  // nS + 1 starts into the entries
+      const uint16_t* ent = starts + nS + 1;                  // 4 nt vertex entries, then 6 nt pair entries
       const size_t w0 = meta.size();
+      // vertex part: header, records, starts and entries
       meta.push_back(uint32_t(nLV) | uint32_t(nLE) << 10 | uint32_t(nt) << 20);
       for (int j = 0; j < nLV; ++j) meta.push_back(uint32_t(pos_lv[lvo + size_t(j)]));
+      h.assign(starts, starts + nLV + 1);
+      h.insert(h.end(), ent, ent + 4 * nt);
+      put16();
+      pad4(w0);
+      if (meta.size() - w0 != size_t(PatchVertexWords(nLV, nt))) throw std::logic_error("patch vertex part size");
+      meta_mid.push_back(int32_t(meta.size()));
+      // pair part: records, starts (from the first pair entry) and entries
+      const size_t w1 = meta.size();
       for (int j = 0; j < nLE; ++j) meta.push_back(uint32_t(pos_le[leo + size_t(j)]));
-      // list starts, then entries, 16 bits each; the entries start on a word
-      const uint16_t* l = &lists_[list0_[size_t(p)]];
-      auto put16 = [&](const uint16_t* v, int cnt) {
-        for (int k = 0; k < cnt; k += 2) meta.push_back(uint32_t(v[k]) | (k + 1 < cnt ? uint32_t(v[k + 1]) << 16 : 0u));
-      };
-      put16(l, nS + 1);
-      put16(l + nS + 1, 10 * nt);
-      while ((meta.size() - w0) % 4) meta.push_back(0);
+      h.clear();
+      for (int j = 0; j <= nLE; ++j) h.push_back(uint16_t(starts[nLV + j] - 4 * nt));
+      h.insert(h.end(), ent + 4 * nt, ent + 10 * nt);
+      put16();
+      pad4(w1);
       if (meta.size() - w0 > size_t(kPatchMetaWords)) throw std::logic_error("element patch description too long");
       if (meta.size() > size_t(INT32_MAX)) throw std::invalid_argument("mesh too large for its patch descriptions");
       meta_ptr.push_back(int32_t(meta.size()));

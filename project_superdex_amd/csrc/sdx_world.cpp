@@ -320,9 +320,9 @@ void World<T>::finalize() {
     else
       pos_le[size_t(code)] = int32_t(q);
   }
-  std::vector<int32_t> patch_t0, patch_meta_ptr;
+  std::vector<int32_t> patch_t0, patch_meta_ptr, patch_meta_mid;
   std::vector<uint32_t> patch_meta;
-  patches.finish(pos_lv, pos_le, patch_t0, patch_meta_ptr, patch_meta);
+  patches.finish(pos_lv, pos_le, patch_t0, patch_meta_ptr, patch_meta_mid, patch_meta);
   nP_ = patches.count();
   // gather work items: the diagonal slots, then the upper slots (i < j),
   // each with its mirror (j, i), record range, row with the Dirichlet flags of
@@ -389,7 +389,9 @@ void World<T>::finalize() {
   n_items_ = int(item_b.size() / 2);
   d_patch_t0_.upload(patch_t0);
   d_patch_meta_ptr_.upload(patch_meta_ptr);
+  d_patch_meta_mid_.upload(patch_meta_mid);
   d_patch_meta_.upload(patch_meta);
+  patch_list_.alloc(std::max<size_t>(size_t(nP_), 1));
   d_pos_vc_.upload(pos_vc);
   d_pos_ec_.upload(pos_ec);
   d_vtx_ptr_.upload(vtx_ptr);
@@ -477,6 +479,7 @@ void World<T>::finalize() {
   mesh_.nP = nP_;
   mesh_.patch_t0 = d_patch_t0_.p;
   mesh_.patch_meta_ptr = d_patch_meta_ptr_.p;
+  mesh_.patch_meta_mid = d_patch_meta_mid_.p;
   mesh_.patch_meta = d_patch_meta_.p;
   mesh_.pos_vc = d_pos_vc_.p;
   mesh_.pos_ec = d_pos_ec_.p;
@@ -571,6 +574,7 @@ void World<T>::upload_colliders() {
 template <class T>
 void World<T>::set_env_mask(const std::vector<int32_t>& mask) {
   put(env_mask_, mask);
+  env_mask_h_ = mask;
 }
 
 // Assemble residual (and BSR + preconditioner if wantH) for the environments
@@ -594,8 +598,28 @@ void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH, int 
     T* Rv = group_records(cres_.p, 3, g.rv0);
     T* Dv = group_records(cdiag_.p, 6, g.rv0);
     T* Ev = group_records(coff_.p, 9, g.re0);
-    launch_element<T>(stream_, mesh_, r, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, Rv, Dv,
-                      Ev, sched_.element_block, sampled_events(2), stiff_damp_);
+    // where the mask leaves out most of the group's patches, the element
+    // kernel runs over a list of the others (a masked-out patch still loads
+    // its description and tets before it finds its environment masked)
+    AsmRange re = r;
+    const int32_t* plist = nullptr;
+    int64_t on = 0;
+    if (env_mask_h_.size() == size_t(nEnv_))
+      for (int e = g.e0; e < g.e1; ++e)
+        if (env_mask_h_[size_t(e)]) on += env_p0_[size_t(e) + 1] - env_p0_[size_t(e)];
+    if (on * 4 < int64_t(r.p1 - r.p0) * 3) {
+      std::vector<int32_t> list;
+      list.reserve(size_t(on));
+      for (int e = g.e0; e < g.e1; ++e)
+        if (env_mask_h_[size_t(e)])
+          for (int q = env_p0_[size_t(e)]; q < env_p0_[size_t(e) + 1]; ++q) list.push_back(q);
+      put(patch_list_, list);
+      plist = patch_list_.p;
+      re.p0 = 0;
+      re.p1 = int32_t(list.size());
+    }
+    launch_element<T>(stream_, mesh_, re, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, wantH ? 1 : 0, project, Rv,
+                      Dv, Ev, sched_.element_block, sampled_events(2), stiff_damp_, plist);
     launch_contact<T>(stream_, mesh_, r, fields_, u_.p, u0_.p, T(dt), wantH ? 1 : 0, Rv, Dv, Ev);
     if (wantH)
       launch_gather_bsr<T>(stream_, mesh_, r, Dv, Ev, sp.linear.preconditioner, vals_.p, dinv_.p, sampled_events(3));
@@ -702,6 +726,8 @@ void World<T>::plan_assembly(int n) {
     G.r.tile1 = tiles_v_.first[size_t(e1)];
     G.rv0 = env_rv0_[size_t(e)];
     G.re0 = env_re0_[size_t(e)];
+    G.e0 = e;
+    G.e1 = e1;
     // every range within its array: the kernels index with these unchecked
     if (G.r.t0 < 0 || G.r.t1 > nT_ || G.r.t0 > G.r.t1 || G.r.p0 < 0 || G.r.p1 > nP_ || G.r.p0 > G.r.p1 ||
         G.r.f0 < 0 || G.r.f1 > nF_ || G.r.f0 > G.r.f1 ||

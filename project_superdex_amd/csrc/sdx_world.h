@@ -524,6 +524,7 @@ class World final : public WorldBase {
   struct AsmGroup {
     AsmRange r;
     int64_t rv0, re0;  // first vertex-sorted and pair-sorted record of the group
+    int e0, e1;        // its environments
   };
   std::vector<AsmGroup> asm_plan_;
   int asm_plan_n_ = 0;
@@ -588,6 +589,7 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, d_item_b_)
   SDX_BUF(int32_t, d_patch_t0_)
   SDX_BUF(int32_t, d_patch_meta_ptr_)
+  SDX_BUF(int32_t, d_patch_meta_mid_)
   SDX_BUF(uint32_t, d_patch_meta_)
   SDX_BUF(int32_t, d_pos_vc_)
   SDX_BUF(int32_t, d_pos_ec_)
@@ -644,6 +646,8 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, list_count_)  // list lengths, two parities each
   SDX_BUF(int32_t, num_active_)
   SDX_BUF(int32_t, env_mask_)
+  std::vector<int32_t> env_mask_h_;  // host copy of env_mask_
+  SDX_BUF(int32_t, patch_list_)      // element patches of the masked environments (partial assemblies)
   SDX_BUF(int32_t, ls_mask_)
   SDX_BUF(double, ls_scale_)
   SDX_BUF(int32_t, reset_)

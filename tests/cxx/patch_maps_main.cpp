@@ -72,9 +72,9 @@ int main(int argc, char** argv) {
   std::vector<int32_t> pos_lv(patches.n_local_vertices()), pos_le(patches.n_local_pairs());
   for (size_t q = 0; q < vtx_code.size(); ++q) pos_lv[vtx_code[q]] = int32_t(q);
   for (size_t q = 0; q < ecode.size(); ++q) pos_le[ecode[q]] = int32_t(q);
-  std::vector<int32_t> patch_t0, meta_ptr;
+  std::vector<int32_t> patch_t0, meta_ptr, meta_mid;
   std::vector<uint32_t> meta;
-  patches.finish(pos_lv, pos_le, patch_t0, meta_ptr, meta);
+  patches.finish(pos_lv, pos_le, patch_t0, meta_ptr, meta_mid, meta);
   if (patch_t0.back() != NT) return fail("patch tets", patch_t0.back(), NT);
 
   // contributions: residual 12, diagonal blocks 4 x 6, off-diagonal 6 x 9 per tet
@@ -94,13 +94,22 @@ int main(int argc, char** argv) {
     if (words % 4 || words > kPatchMetaWords) return fail("description size", p, words);
     if (meta_ptr[size_t(p)] % 4) return fail("description alignment", p, meta_ptr[size_t(p)]);
     const int pt0 = patch_t0[size_t(p)], nt = patch_t0[size_t(p) + 1] - pt0;
-    const int nLV = int(m[0] & 1023u), nLE = int((m[0] >> 10) & 1023u), nS = nLV + nLE;
+    const int nLV = int(m[0] & 1023u), nLE = int((m[0] >> 10) & 1023u);
     if (int(m[0] >> 20) != nt) return fail("patch tets in header", p, int(m[0] >> 20));
+    // vertex part
+    const int vw = PatchVertexWords(nLV, nt);
+    if (meta_mid[size_t(p)] - meta_ptr[size_t(p)] != vw) return fail("vertex part size", p, vw);
     const uint32_t* rec = m + 1;
-    const uint16_t* ls = reinterpret_cast<const uint16_t*>(m + 1 + nS);
-    const uint16_t* ent = reinterpret_cast<const uint16_t*>(m + 1 + nS + (nS + 2) / 2);
-    if (1 + nS + (nS + 2) / 2 + (10 * nt + 1) / 2 > words) return fail("description overrun", p, words);
-    if (ls[0] != 0 || ls[nLV] != 4 * nt || ls[nS] != 10 * nt) return fail("list starts", p, ls[nS]);
+    const uint16_t* ls = reinterpret_cast<const uint16_t*>(m + 1 + nLV);
+    const uint16_t* ent = ls + nLV + 1;
+    if (1 + nLV + (nLV + 1 + 4 * nt + 1) / 2 > vw) return fail("vertex part overrun", p, vw);
+    if (ls[0] != 0 || ls[nLV] != 4 * nt) return fail("vertex list starts", p, ls[nLV]);
+    // pair part
+    const uint32_t* prec = m + vw;
+    const uint16_t* pls = reinterpret_cast<const uint16_t*>(prec + nLE);
+    const uint16_t* pent = pls + nLE + 1;
+    if (vw + nLE + (nLE + 1 + 6 * nt + 1) / 2 > words) return fail("pair part overrun", p, words);
+    if (pls[0] != 0 || pls[nLE] != 6 * nt) return fail("pair list starts", p, pls[nLE]);
     std::vector<int> seen(size_t(10 * nt), 0);
     for (int j = 0; j < nLV; ++j) {
       double r[3] = {0, 0, 0}, d[6] = {0, 0, 0, 0, 0, 0};
@@ -118,15 +127,15 @@ int main(int argc, char** argv) {
     }
     for (int j = 0; j < nLE; ++j) {
       double o[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      for (int q = ls[nLV + j]; q < ls[nLV + j + 1]; ++q) {
-        const int c = ent[q], id = c & 0x7fff, tl = id / 6, k = id % 6;
+      for (int q = pls[j]; q < pls[j + 1]; ++q) {
+        const int c = pent[q], id = c & 0x7fff, tl = id / 6, k = id % 6;
         if (id >= 6 * nt) return fail("pair entry", p, c);
         seen[size_t(4 * nt + id)]++;
         const double* s = &coff[size_t(54) * (pt0 + tl) + size_t(9 * k)];
         for (int r = 0; r < 3; ++r)
           for (int cc = 0; cc < 3; ++cc) o[3 * r + cc] += (c >> 15) ? s[3 * cc + r] : s[3 * r + cc];
       }
-      const size_t rp = rec[nLV + j];
+      const size_t rp = prec[j];
       if (rp >= ecode.size()) return fail("pair record", p, long(rp));
       for (int k = 0; k < 9; ++k) Ev[9 * rp + size_t(k)] = o[k];
     }
