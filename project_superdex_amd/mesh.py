@@ -209,6 +209,32 @@ def plate_press_scene(n: int = 40, edge: float = 0.1, steps: int = 50, press: fl
     return SceneSpec(bodies=[body], colliders=[plate], dt=dt, steps=steps, name=f"press{n}")
 
 
+def batched_press_scene(num_envs: int, seed: int, n_min: int, n_max: int, h: float = 0.01, steps: int = 50,
+                        press=(1e-3, 3e-3), ramp=(10, 30), newton_iters: int = 1, dt: float = 1e-3) -> SceneSpec:
+    """Many independent plate presses of uneven size (the plate-press case, batched).
+
+    Per env: a cube of n ~ U{n_min..n_max} hexes per side of spacing h, E
+    log-uniform in [1e4, 1e6] Pa, nu ~ U(0.3, 0.45), bottom face pinned, under
+    a rigid plate (plane, normal -Y) that starts on the top face and moves
+    down a depth ~ U(press) over ~ U{ramp} steps, then holds.
+    """
+    rng = np.random.default_rng(seed)
+    bodies, colliders = [], []
+    for e in range(num_envs):
+        n = int(rng.integers(n_min, n_max + 1))
+        E = float(10.0 ** rng.uniform(4.0, 6.0))
+        nu = float(rng.uniform(0.3, 0.45))
+        edge = n * h
+        origin = (e * 0.5, 0.0, 0.0)
+        bodies.append(soft_cube(n, edge, E=E, nu=nu, origin=origin))
+        depth = float(rng.uniform(*press))
+        ramp_steps = int(rng.integers(ramp[0], ramp[1] + 1))
+        colliders.append(Collider(kind="plane", point=(origin[0], edge, 0.0), normal=(0.0, -1.0, 0.0),
+                                  velocity=(0.0, -depth / (ramp_steps * dt), 0.0), ramp_steps=ramp_steps, actor=e))
+    return SceneSpec(bodies=bodies, colliders=colliders, dt=dt, steps=steps, newton_iters=newton_iters,
+                     name=f"press_batch{num_envs}")
+
+
 def batched_indent_scene(num_envs: int, seed: int, n_min: int, n_max: int, h: float = 0.01,
                          radius: float = 0.02, speed: float = 0.05, steps: int = 20,
                          indenters=(1, 1), newton_iters: int = 1, dt: float = 1e-3) -> SceneSpec:

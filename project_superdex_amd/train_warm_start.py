@@ -16,9 +16,15 @@ steps): zero 83,948, previous step alone 70,809, with the network 71,526
 step; the network's guesses add none.  The history-only start
 (SDX_WARM_START_HISTORY) is that combination without the network: on the
 4096-environment workload it runs 3.8% faster than the learned start (the
-inference it skips) at the same CG iteration count.
+inference it skips) at the same CG iteration count.  With --generator press
+(8 seeds x 512 batched plate presses x 50 steps = 204,800 solves; weights in
+weights/warm_start_mlp_press.npz) the held-out presses (128 environments x 40
+steps) took 158,397 CG iterations from zero, 119,323 from the previous steps
+alone and 119,473 with the network (step target; 119,513 with the change
+target): again the previous steps carry the whole gain.
 
-    python -m project_superdex_amd.train_warm_start [--out weights/warm_start_mlp.npz]
+    python -m project_superdex_amd.train_warm_start [--generator indent|press] [--target step|change]
+                                                    [--out weights/warm_start_mlp.npz]
 """
 from __future__ import annotations
 
@@ -35,7 +41,10 @@ from . import scene as gs
 from .learned import DEFAULT_WEIGHTS, WarmStartModel
 
 
-def generator(seed: int, envs: int, steps: int):
+def generator(seed: int, envs: int, steps: int, kind: str = "indent"):
+    """The batched indentation case (sphere indenters), or the plate-press case batched."""
+    if kind == "press":
+        return mesh.batched_press_scene(envs, seed, 6, 16, h=0.01, steps=steps)
     return mesh.batched_indent_scene(envs, seed, 6, 16, h=0.01, radius=0.02, speed=0.05, steps=steps,
                                      indenters=(1, 1), newton_iters=1)
 
@@ -75,6 +84,8 @@ def main(argv=None) -> int:
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--target", choices=["change", "step"], default="step",
                     help="learn the change of the Newton step since the previous one, or the step itself")
+    ap.add_argument("--generator", choices=["indent", "press"], default="indent",
+                    help="training and held-out scenes: batched sphere indentation or batched plate presses")
     args = ap.parse_args(argv)
 
     t0 = time.time()
@@ -83,7 +94,7 @@ def main(argv=None) -> int:
     X, T = [], []
     solves = 0
     for seed in range(1, args.seeds + 1):
-        spec = generator(seed, args.envs, args.steps)
+        spec = generator(seed, args.envs, args.steps, args.generator)
         sc, _ = run(spec, params(), collect=(args.envs * args.steps * args.per_env, args.per_env))
         x, t = sc.collected()
         X.append(x)
@@ -104,7 +115,8 @@ def main(argv=None) -> int:
           file=sys.stderr, flush=True)
 
     # held-out seed: prediction error and CG iterations with / without the warm start
-    spec = generator(args.holdout_seed, min(args.envs, 128), 20)
+    spec = generator(args.holdout_seed, min(args.envs, 128), 20 if args.generator == "indent" else 40,
+                     args.generator)
     s0, st0 = run(spec, params(), collect=(spec.steps * len(spec.bodies) * 64, 64))
     Xh, Th = s0.collected()
     if args.target == "change":
@@ -121,7 +133,7 @@ def main(argv=None) -> int:
     tried = sum(e["warm_start_tried"] for st in st1 for e in st)
     print(json.dumps({"solves": solves, "rows": int(X.shape[0]), "epochs": args.epochs, "loss_first": losses[0],
                       "loss_last": losses[-1], "data_seconds": t_data, "train_seconds": t_train,
-                      "target": args.target, "holdout_rel_error": err, "holdout_cg_iters_zero": it0,
+                      "target": args.target, "generator": args.generator, "holdout_rel_error": err, "holdout_cg_iters_zero": it0,
                       "holdout_cg_iters_previous_step": it2, "holdout_cg_iters_learned": it1,
                       "holdout_kept": kept, "holdout_tried": tried, "weights": os.path.relpath(args.out)}))
     return 0
