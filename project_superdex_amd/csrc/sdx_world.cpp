@@ -819,12 +819,17 @@ void World<T>::issue_cg_tail(const CgConfig& cfg, int par, int k0, int n) {
 // CG burst m (iterations [m B, m B + B)) over the lists of parity par as a
 // replayable graph, captured on first use for the current configuration.
 template <class T>
-hipGraphExec_t World<T>::cg_graph(const CgConfig& cfg, int m, int par, int B, int cls) {
+void World<T>::graphs_for(const CgConfig& cfg, int B) {
   if (std::memcmp(&graph_cfg_, &cfg, sizeof(CgConfig)) != 0 || graph_burst_ != B) {
     drop_graphs();
     graph_cfg_ = cfg;
     graph_burst_ = B;
   }
+}
+
+template <class T>
+hipGraphExec_t World<T>::cg_graph(const CgConfig& cfg, int m, int par, int B, int cls) {
+  graphs_for(cfg, B);
   const size_t key = (size_t(m) * kGridClasses + size_t(cls)) * 2 + size_t(par);
   if (key >= graphs_.size()) graphs_.resize(key + 1, nullptr);
   if (!graphs_[key]) {
@@ -859,6 +864,7 @@ int World<T>::cg_groups(const sdx_schedule& sc) const {
 // update tiles; also the largest group's tile counts.
 template <class T>
 void World<T>::plan_groups(int n) {
+  drop_graphs();  // their grids and ranges follow the groups
   group_plan_n_ = n;
   group_a_.assign(1, 0);
   group_v_.assign(1, 0);
@@ -885,6 +891,8 @@ void World<T>::drop_graphs() {
   for (auto& g : graphs_)
     if (g) (void)hipGraphExecDestroy(g);
   graphs_.clear();
+  if (grouped_graph_) (void)hipGraphExecDestroy(grouped_graph_);
+  grouped_graph_ = nullptr;
 }
 
 // One CG run for the environments with solve[e]; those with warm[e] keep
@@ -985,17 +993,42 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
     group_bursts_ = std::min((sched_.group_iters + B - 1) / B, cfg.max_iter / B + 1);
     if (group_plan_n_ != groups) plan_groups(groups);
     const int ng = int(group_a_.size()) - 1;
-    for (int g = 0; g < ng; ++g)
-      for (int m = 0; m < group_bursts_; ++m, ++q) {
-        const bool timed = m == 0 && timed0;
-        launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, group_a_[size_t(g)], group_a_[size_t(g) + 1],
-                             group_v_[size_t(g)], group_v_[size_t(g) + 1], env_cg_.p, list_a_.p, list_v_.p,
-                             list_count_.p, q & 1, num_active_.p, nullptr);
-        if (sched_.use_graphs && !timed)
-          hip_check(hipGraphLaunch(cg_graph(cfg, m, q & 1, B, 0), stream_), "hipGraphLaunch");
-        else
-          issue_cg(cfg, m, q & 1, m * B, B, timed, 0);
+    // every group's compactions and bursts, in order (eager, or into a capture)
+    auto issue_grouped = [&](bool timed_ok) {
+      int qq = 0;
+      for (int g = 0; g < ng; ++g)
+        for (int m = 0; m < group_bursts_; ++m, ++qq) {
+          const bool timed = m == 0 && timed_ok;
+          launch_compact_tiles(stream_, dt_a_.view, dt_v_.view, group_a_[size_t(g)], group_a_[size_t(g) + 1],
+                               group_v_[size_t(g)], group_v_[size_t(g) + 1], env_cg_.p, list_a_.p, list_v_.p,
+                               list_count_.p, qq & 1, num_active_.p, nullptr);
+          issue_cg(cfg, m, qq & 1, m * B, B, timed, 0);
+        }
+    };
+    if (sched_.use_graphs && !timed0) {
+      // the whole grouped phase as one graph: no launch boundary between bursts
+      graphs_for(cfg, B);
+      if (grouped_graph_ && (grouped_graph_groups_ != ng || grouped_graph_bursts_ != group_bursts_)) {
+        (void)hipGraphExecDestroy(grouped_graph_);
+        grouped_graph_ = nullptr;
       }
+      if (!grouped_graph_) {
+        hipGraph_t gr = nullptr;
+        hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        issue_grouped(false);
+        hip_check(hipStreamEndCapture(stream_, &gr), "hipStreamEndCapture");
+        const hipError_t err = hipGraphInstantiate(&grouped_graph_, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (err != hipSuccess) grouped_graph_ = nullptr;
+        hip_check(err, "hipGraphInstantiate");
+        grouped_graph_groups_ = ng;
+        grouped_graph_bursts_ = group_bursts_;
+      }
+      hip_check(hipGraphLaunch(grouped_graph_, stream_), "hipGraphLaunch");
+    } else {
+      issue_grouped(timed0);
+    }
+    q = ng * group_bursts_;
   }
   // grid class of the batch-wide bursts from the last published active
   // count: at most that many environments, each with at most max_env_a_

@@ -562,6 +562,11 @@ class World final : public WorldBase {
   bool tail_used_ = false;
   CgConfig graph_cfg_{};
   int graph_burst_ = 0;
+  // the grouped bursts of a solve (every environment group) as one graph
+  hipGraphExec_t grouped_graph_ = nullptr;
+  int grouped_graph_groups_ = 0, grouped_graph_bursts_ = 0;
+  // drops the captured graphs when the CG configuration or burst length changed
+  void graphs_for(const CgConfig& cfg, int B);
 
   std::vector<DBufBase*> bufs_;
   int nEnv_ = 0, nV_ = 0, nT_ = 0, nB_ = 0, nF_ = 0, n_items_ = 0;
