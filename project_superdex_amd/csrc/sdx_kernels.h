@@ -110,9 +110,8 @@ struct MeshView {
   const int32_t* col;       // [nB]
   const int32_t* slot_row;  // [nB]
   const int32_t* diag_slot; // [nV]
-  int32_t n_items;          // diagonal slots (one per vertex, first), then upper slots
-  const int32_t* item_a;    // [4 n_items] slot, mirror slot, first and end record
-  const int32_t* item_b;    // [2 n_items] row | pin bits << kItemRowBits, environment
+  const int4* slot_desc;    // [nB] per BSR slot: first and end record, row | pin bits << kItemRowBits,
+                            // environment | kind << 30 (kSlotUpper / kSlotLower / kSlotDiag)
   const int32_t* eptr;      // [nB+1] off-diagonal records of each upper slot (pair-sorted)
   int32_t nP;               // element patches
   const int32_t* patch_t0;  // [nP+1] tets of patch p: [patch_t0[p], patch_t0[p+1])
@@ -162,12 +161,15 @@ struct FieldPtrs {
 // then the contact records).  Each array is padded by one 16-byte vector for
 // the gathers' vector loads.
 // (patch descriptions and the local pair order: sdx_layout.h)
-constexpr int kItemRowBits = 26;  // rows of the gather items below 2^26
+constexpr int kItemRowBits = 26;  // rows of the BSR slots below 2^26
+// Kinds of a BSR slot for the gather: an upper slot (i < j) sums its pair's
+// records, a lower slot their transposes, a diagonal slot its vertex's
+// diagonal-block records.
+enum : int32_t { kSlotUpper = 0, kSlotLower = 1, kSlotDiag = 2 };
 //
 // An assembly runs over a contiguous range of environments (an assembly
-// group): its tets [t0, t1) in patches [p0, p1), boundary triangles [f0, f1), diagonal gather
-// items [d0, d1) (= its vertices), upper gather items [u0, u1) and vector
-// tiles [tile0, tile1).  Record indices stay global; the record pointers a
+// group): its tets [t0, t1) in patches [p0, p1), boundary triangles [f0, f1),
+// BSR slots [s0, s1) and vector tiles [tile0, tile1).  Record indices stay global; the record pointers a
 // launch takes are the group's buffers shifted by the group's first record,
 // so a group writes and reads only its own records.  contact_always: every
 // boundary triangle writes its contact records (zeros out of contact),
@@ -175,7 +177,7 @@ constexpr int kItemRowBits = 26;  // rows of the gather items below 2^26
 // been overwritten by another group; 0: a triangle skips writing records
 // that are known to hold zeros (its flag bits).
 struct AsmRange {
-  int32_t t0, t1, f0, f1, d0, d1, u0, u1, tile0, tile1;
+  int32_t t0, t1, f0, f1, s0, s1, tile0, tile1;
   int32_t contact_always;
   int32_t p0, p1;
 };
@@ -188,8 +190,8 @@ void launch_element(hipStream_t s, const MeshView& m, const AsmRange& r, const F
 template <class T>
 void launch_contact(hipStream_t s, const MeshView& m, const AsmRange& r, const FieldPtrs<T>& f, const T* u,
                     const T* u0, T dt, int wantH, T* Rv, T* Dv, T* Ev);
-// BSR values (and the preconditioner blocks) from the records, over the work
-// items of the range (diagonal and upper slots).
+// BSR values (and the preconditioner blocks) from the records, over the
+// slots of the range.
 template <class T>
 void launch_gather_bsr(hipStream_t s, const MeshView& m, const AsmRange& r, const T* Dv, const T* Ev, int prec,
                        T* vals, T* dinv, const KernelEvents* ev = nullptr);

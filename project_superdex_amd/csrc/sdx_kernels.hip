@@ -584,108 +584,78 @@ __device__ __forceinline__ T sum_run(const T* lds, int k0, int k1, int e, int64_
   return s0 + s1;
 }
 
-// BSR gather workgroups: four waves of seven 9-lane items; the records of a
-// workgroup's items are staged in chunks of 16 KB (four 16-byte loads per
-// thread).  Measured on 512 batched cubes: 28 items per workgroup is as fast
-// as 56 (eight waves, or two rounds of four waves) and as workgroups of 1 or
-// 2 waves; one lane per item summing its records straight from global memory
-// is 1.8x slower.
-constexpr int kGatherWaves = 4;
-constexpr int kGatherItems = 7 * kGatherWaves;
+// BSR gather: one lane per BSR slot, in slot order, so that the lanes of a
+// wave write consecutive slots of each value plane.  A slot sums its records
+// straight from global memory: a diagonal slot its vertex's diagonal-block
+// records (00 11 22 01 02 12), an upper slot (i < j) its pair's off-diagonal
+// records, a lower slot the same records transposed (each pair's records are
+// read twice, every value written once, coalesced).  Then the Dirichlet
+// rows / columns, and on a diagonal slot the preconditioner block.
+constexpr int kGatherThreads = 256;
 
 template <class T>
-__global__ void __launch_bounds__(64 * kGatherWaves) k_gather_bsr(MeshView m, AsmRange ar, const T* __restrict__ Dv,
-                                                                  const T* __restrict__ Ev, int prec,
-                                                                  int n_diag_blocks, T* __restrict__ vals,
-                                                                  T* __restrict__ dinv) {
-  // Nine lanes per work item, lane q owning entry q = 3 r + c of the 3x3
-  // block.  Items: the diagonal slots (the first nV items, one per vertex;
-  // those of the range are [d0, d1)) and the upper slots (i < j; [u0, u1));
-  // an upper slot also writes its mirror (j, i)
-  // transposed, so each pair's records are read once.  The records of
-  // consecutive items are contiguous (vertex-sorted diagonal records,
-  // pair-sorted off-diagonal records, element and contact alike), so a
-  // workgroup stages the records of its items through LDS with 16-byte
-  // loads and the lanes sum from LDS.  Workgroups [0, n_diag_blocks) take
-  // diagonal items, the rest upper items.  Every load before the first wait
-  // is unconditional (lanes without an item load the workgroup's first
-  // item), so the item descriptions (slot, mirror, record range; row with the
-  // Dirichlet flags of its row and column DoFs, environment), the
-  // workgroup's record range and the records are in flight together, and the
-  // environment masks follow the descriptions.
-  __shared__ typename Vec16<T>::type lds_v[kStage<T> * sizeof(T) / 16];
-  T* lds = reinterpret_cast<T*>(lds_v);
-  const int blk = xcd_tile(blockIdx.x, gridDim.x);
-  const bool diag = blk < n_diag_blocks;
-  const int first = diag ? ar.d0 + blk * kGatherItems : ar.u0 + (blk - n_diag_blocks) * kGatherItems;
-  const int last = min(first + kGatherItems, diag ? ar.d1 : ar.u1);  // exclusive
-  const int rec0 = m.item_a[4 * first + 2], rec1 = m.item_a[4 * (last - 1) + 3];
-  const int W = diag ? 6 : 9;
-  const T* src = diag ? Dv : Ev;
-  constexpr int kW = 16 / int(sizeof(T));
-  const int64_t start = (int64_t(rec0) * W) & ~int64_t(kW - 1), end = int64_t(rec1) * W;
-  // the first chunk's records, then the item descriptions, all in flight
-  // together (a workgroup of vertices in no element or triangle has no records)
-  const bool any = start < end;
-  Stage<T, 64 * kGatherWaves, kStage<T>> st;
-  int64_t c1 = start + kStage<T> < end ? start + kStage<T> : end;
-  if (any) st.load(src, start, c1);
-  const int lane = int(threadIdx.x) % 64;
-  const int g = lane / 9, q = lane % 9;
-  const int item0 = first + (int(threadIdx.x) / 64) * 7 + g;
-  const bool mine = g < 7 && item0 < last;
-  const int item = mine ? item0 : first;
-  const int4 ia = reinterpret_cast<const int4*>(m.item_a)[item];
-  const int2 ib = reinterpret_cast<const int2*>(m.item_b)[item];
-  const int msk = m.env_mask[ib.y];
-  const int r = q / 3, c = q % 3;
-  const int e = diag ? sym_entry(r, c) : q;
-  if (any) st.store(lds);
-  __syncthreads();
-  const bool on = mine && msk != 0;
-  T acc = T(0);
-  for (int64_t c0 = start; any;) {
-    if (on) acc += diag ? sum_run<T, 6>(lds, ia.z, ia.w, e, c0, c1) : sum_run<T, 9>(lds, ia.z, ia.w, e, c0, c1);
-    c0 += kStage<T>;
-    if (c0 >= end) break;  // uniform over the workgroup
-    __syncthreads();
-    c1 = c0 + kStage<T> < end ? c0 + kStage<T> : end;
-    stage_chunk<T, 64 * kGatherWaves, kStage<T>>(src, c0, c1, lds);
-    __syncthreads();
-  }
-  if (!on) return;
-  const int nB = m.nB;
-  T v = acc;
-  // Dirichlet rows / columns (mochi_solve.cpp:337-396); the test is symmetric in (i r, j c)
-  const int pins = ib.x >> kItemRowBits;
-  if (((pins >> r) | (pins >> (3 + c))) & 1) v = (diag && r == c) ? T(1) : T(0);
-  vals[(size_t)q * nB + ia.x] = v;
-  if (!diag) {
-    vals[(size_t)(3 * c + r) * nB + ia.y] = v;
-    return;
-  }
-  // diagonal block: every lane of the item gathers the 9 entries and writes
-  // its entry of the preconditioner block (the lanes of one item are all on
-  // or all off, and a diagonal workgroup has no upper items)
-  const int base = g * 9;
-  T blk9[9];
+__global__ void __launch_bounds__(kGatherThreads) k_gather_bsr(MeshView m, AsmRange ar, const T* __restrict__ Dv,
+                                                               const T* __restrict__ Ev, int prec,
+                                                               T* __restrict__ vals, T* __restrict__ dinv) {
+  const int k = ar.s0 + xcd_tile(blockIdx.x, gridDim.x) * kGatherThreads + int(threadIdx.x);
+  if (k >= ar.s1) return;
+  const int4 d = m.slot_desc[k];  // first and end record, row | pins, environment | kind
+  if (!m.env_mask[d.w & 0x3fffffff]) return;
+  const int kind = int(uint32_t(d.w) >> 30);
+  T acc[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) blk9[k] = __shfl(v, base + k, 64);
+  for (int e = 0; e < 9; ++e) acc[e] = T(0);
+  if (kind == kSlotDiag) {
+    for (int r = d.x; r < d.y; ++r) {
+      const T* p = Dv + size_t(r) * 6;
+#pragma unroll
+      for (int e = 0; e < 6; ++e) acc[e] += p[e];
+    }
+  } else {
+    for (int r = d.x; r < d.y; ++r) {
+      const T* p = Ev + size_t(r) * 9;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) acc[e] += p[e];
+    }
+  }
+  // the block, row-major
+  T b[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      b[3 * r + c] = kind == kSlotDiag ? acc[sym_entry(r, c)] : (kind == kSlotLower ? acc[3 * c + r] : acc[3 * r + c]);
+  // Dirichlet rows / columns (mochi_solve.cpp:337-396): row pins in bits 0-2, column pins in bits 3-5
+  const int pins = d.z >> kItemRowBits;
+  const bool diag = kind == kSlotDiag;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (((pins >> r) | (pins >> (3 + c))) & 1) b[3 * r + c] = (diag && r == c) ? T(1) : T(0);
+  const int nB = m.nB;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) vals[(size_t)q * nB + k] = b[q];
+  if (!diag) return;
+  // the preconditioner block, symmetric storage 00 11 22 01 02 12
   T o[9];
   if (prec == 2) {
-    inv3(blk9, o);
+    inv3(b, o);
   } else {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) o[k] = T(0);
+    for (int q = 0; q < 9; ++q) o[q] = T(0);
     if (prec == 1) {
-      o[0] = T(1) / blk9[0];
-      o[4] = T(1) / blk9[4];
-      o[8] = T(1) / blk9[8];
+      o[0] = T(1) / b[0];
+      o[4] = T(1) / b[4];
+      o[8] = T(1) / b[8];
     } else {
       o[0] = o[4] = o[8] = T(1);
     }
   }
-  if (r <= c) dinv[(size_t)sym_entry(r, c) * m.nV + (ib.x & ((1 << kItemRowBits) - 1))] = o[q];
+  const int row = d.z & ((1 << kItemRowBits) - 1);
+  const T sym[6] = {o[0], o[4], o[8], o[1], o[2], o[5]};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) dinv[(size_t)q * m.nV + row] = sym[q];
 }
 
 template <class T>
@@ -1949,11 +1919,9 @@ void launch_contact(hipStream_t s, const MeshView& m, const AsmRange& r, const F
 template <class T>
 void launch_gather_bsr(hipStream_t s, const MeshView& m, const AsmRange& r, const T* Dv, const T* Ev, int prec,
                        T* vals, T* dinv, const KernelEvents* ev) {
-  const int nd = (r.d1 - r.d0 + kGatherItems - 1) / kGatherItems;
-  const int nu = (r.u1 - r.u0 + kGatherItems - 1) / kGatherItems;
-  if (nd + nu == 0) return;
-  SDX_LAUNCH(ev, k_gather_bsr<T>, dim3(unsigned(nd + nu)), dim3(64 * kGatherWaves), s, m, r, Dv, Ev, prec, nd, vals,
-             dinv);
+  if (r.s1 <= r.s0) return;
+  SDX_LAUNCH(ev, k_gather_bsr<T>, dim3(blocks_for(r.s1 - r.s0, kGatherThreads)), dim3(kGatherThreads), s, m, r, Dv,
+             Ev, prec, vals, dinv);
 }
 
 template <class T, class A>

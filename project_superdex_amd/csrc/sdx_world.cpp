@@ -324,33 +324,39 @@ void World<T>::finalize() {
   std::vector<uint32_t> patch_meta;
   patches.finish(pos_lv, pos_le, patch_t0, patch_meta_ptr, patch_meta_mid, patch_meta);
   nP_ = patches.count();
-  // gather work items: the diagonal slots, then the upper slots (i < j),
-  // each with its mirror (j, i), record range, row with the Dirichlet flags of
-  // the row and column DoFs, and environment; lower slots are written by the
-  // item of their upper mirror
-  std::vector<int32_t> item_a, item_b;
-  item_a.reserve(size_t(4) * (size_t(nV_) + size_t(nB_ - nV_) / 2));
-  item_b.reserve(item_a.capacity() / 2);
-  env_u0_.assign(size_t(nEnv) + 1, 0);
-  for (int pass = 0; pass < 2; ++pass)
-    for (int i = 0; i < nV_; ++i) {
-      if (pass == 1 && i == env_v0_[size_t(row_env[size_t(i)])])  // first upper item of the environment
-        env_u0_[size_t(row_env[size_t(i)])] = int32_t(item_b.size() / 2);
-      for (int k = row_ptr[size_t(i)]; k < row_ptr[size_t(i) + 1]; ++k) {
-        const int j = col[size_t(k)];
-        if (pass == 0 ? j != i : j <= i) continue;
+  // gather description of every BSR slot: its record range (the vertex's
+  // diagonal-block records; an upper slot's pair records; a lower slot's
+  // mirror's, read transposed), row with the Dirichlet flags of its row and
+  // column DoFs, environment and kind
+  std::vector<int32_t> slot_desc(size_t(4) * size_t(nB_));
+  for (int i = 0; i < nV_; ++i)
+    for (int k = row_ptr[size_t(i)]; k < row_ptr[size_t(i) + 1]; ++k) {
+      const int j = col[size_t(k)];
+      int r0, r1, kind;
+      if (j == i) {
+        r0 = vtx_ptr[size_t(i)];
+        r1 = vtx_ptr[size_t(i) + 1];
+        kind = kSlotDiag;
+      } else if (j > i) {
+        r0 = eptr[size_t(k)];
+        r1 = eptr[size_t(k) + 1];
+        kind = kSlotUpper;
+      } else {
         auto b = col.begin() + row_ptr[size_t(j)], e2 = col.begin() + row_ptr[size_t(j) + 1];
-        const int mirror = int32_t(std::lower_bound(b, e2, i) - col.begin());  // slot (j, i)
-        const int r0 = pass == 0 ? vtx_ptr[size_t(i)] : eptr[size_t(k)];
-        const int r1 = pass == 0 ? vtx_ptr[size_t(i) + 1] : eptr[size_t(k) + 1];
-        int pins = 0;
-        for (int d = 0; d < 3; ++d)
-          pins |= (pinned[size_t(3 * i + d)] ? 1 : 0) << d | (pinned[size_t(3 * j + d)] ? 1 : 0) << (3 + d);
-        item_a.insert(item_a.end(), {k, mirror, r0, r1});
-        item_b.insert(item_b.end(), {i | (pins << kItemRowBits), row_env[size_t(i)]});
+        const size_t mirror = size_t(std::lower_bound(b, e2, i) - col.begin());  // slot (j, i)
+        r0 = eptr[mirror];
+        r1 = eptr[mirror + 1];
+        kind = kSlotLower;
       }
+      int pins = 0;
+      for (int d = 0; d < 3; ++d)
+        pins |= (pinned[size_t(3 * i + d)] ? 1 : 0) << d | (pinned[size_t(3 * j + d)] ? 1 : 0) << (3 + d);
+      int32_t* sd = &slot_desc[size_t(4) * size_t(k)];
+      sd[0] = r0;
+      sd[1] = r1;
+      sd[2] = i | (pins << kItemRowBits);
+      sd[3] = int32_t(uint32_t(row_env[size_t(i)]) | (uint32_t(kind) << 30));
     }
-  env_u0_[size_t(nEnv)] = int32_t(item_b.size() / 2);
   BuildTiles(env_v0_, nEnv, kVecThreads, tiles_v_);
   BuildTiles(env_v0_, nEnv, kSpmvThreads / sched_.spmv_lanes, tiles_a_);
   max_env_a_ = 0;
@@ -384,9 +390,7 @@ void World<T>::finalize() {
   d_slot_row_.upload(slot_row);
   d_diag_slot_.upload(diag_slot);
   d_eptr_.upload(eptr);
-  d_item_a_.upload(item_a);
-  d_item_b_.upload(item_b);
-  n_items_ = int(item_b.size() / 2);
+  d_slot_desc_.upload(slot_desc);
   d_patch_t0_.upload(patch_t0);
   d_patch_meta_ptr_.upload(patch_meta_ptr);
   d_patch_meta_mid_.upload(patch_meta_mid);
@@ -473,9 +477,7 @@ void World<T>::finalize() {
   mesh_.slot_row = d_slot_row_.p;
   mesh_.diag_slot = d_diag_slot_.p;
   mesh_.eptr = d_eptr_.p;
-  mesh_.item_a = d_item_a_.p;
-  mesh_.item_b = d_item_b_.p;
-  mesh_.n_items = n_items_;
+  mesh_.slot_desc = reinterpret_cast<const int4*>(d_slot_desc_.p);
   mesh_.nP = nP_;
   mesh_.patch_t0 = d_patch_t0_.p;
   mesh_.patch_meta_ptr = d_patch_meta_ptr_.p;
@@ -718,10 +720,8 @@ void World<T>::plan_assembly(int n) {
     G.r.p1 = env_p0_[size_t(e1)];
     G.r.f0 = env_f0_[size_t(e)];
     G.r.f1 = env_f0_[size_t(e1)];
-    G.r.d0 = env_v0_[size_t(e)];
-    G.r.d1 = env_v0_[size_t(e1)];
-    G.r.u0 = env_u0_[size_t(e)];  // item indices: the nV diagonal items come first
-    G.r.u1 = env_u0_[size_t(e1)];
+    G.r.s0 = h_row_ptr_[size_t(env_v0_[size_t(e)])];
+    G.r.s1 = h_row_ptr_[size_t(env_v0_[size_t(e1)])];
     G.r.tile0 = tiles_v_.first[size_t(e)];
     G.r.tile1 = tiles_v_.first[size_t(e1)];
     G.rv0 = env_rv0_[size_t(e)];
@@ -731,7 +731,7 @@ void World<T>::plan_assembly(int n) {
     // every range within its array: the kernels index with these unchecked
     if (G.r.t0 < 0 || G.r.t1 > nT_ || G.r.t0 > G.r.t1 || G.r.p0 < 0 || G.r.p1 > nP_ || G.r.p0 > G.r.p1 ||
         G.r.f0 < 0 || G.r.f1 > nF_ || G.r.f0 > G.r.f1 ||
-        G.r.d0 < 0 || G.r.d1 > nV_ || G.r.d0 > G.r.d1 || G.r.u0 < nV_ || G.r.u1 > n_items_ || G.r.u0 > G.r.u1 ||
+        G.r.s0 < 0 || G.r.s1 > nB_ || G.r.s0 > G.r.s1 ||
         G.r.tile0 < 0 || G.r.tile1 > int(tiles_v_.env.size()) || G.r.tile0 > G.r.tile1)
       throw std::logic_error("assembly group range out of bounds");
     max_rv = std::max<int64_t>(max_rv, int64_t(env_rv0_[size_t(e1)]) - G.rv0);

@@ -532,7 +532,7 @@ class World final : public WorldBase {
   int asm_groups(const sdx_schedule& sc) const;
   void plan_assembly(int n);
   T* group_records(T* buf, int W, int64_t rec0) const;
-  std::vector<int32_t> env_t0_, env_f0_, env_u0_, env_rv0_, env_re0_, env_p0_;
+  std::vector<int32_t> env_t0_, env_f0_, env_rv0_, env_re0_, env_p0_;
   int nP_ = 0;  // element patches
   void issue_cg(const CgConfig& cfg, int m, int par, int k0, int n, bool timed, int cls);
   void issue_cg_tail(const CgConfig& cfg, int par, int k0, int n);
@@ -569,7 +569,7 @@ class World final : public WorldBase {
   void graphs_for(const CgConfig& cfg, int B);
 
   std::vector<DBufBase*> bufs_;
-  int nEnv_ = 0, nV_ = 0, nT_ = 0, nB_ = 0, nF_ = 0, n_items_ = 0;
+  int nEnv_ = 0, nV_ = 0, nT_ = 0, nB_ = 0, nF_ = 0;
   std::vector<int32_t> env_v0_;
   std::vector<int32_t> env_b0_;
   std::vector<int32_t> h_row_ptr_, h_col_;
@@ -590,8 +590,7 @@ class World final : public WorldBase {
   SDX_BUF(int32_t, d_slot_row_)
   SDX_BUF(int32_t, d_diag_slot_)
   SDX_BUF(int32_t, d_eptr_)
-  SDX_BUF(int32_t, d_item_a_)
-  SDX_BUF(int32_t, d_item_b_)
+  SDX_BUF(int32_t, d_slot_desc_)     // [4 nB] gather description of each BSR slot
   SDX_BUF(int32_t, d_patch_t0_)
   SDX_BUF(int32_t, d_patch_meta_ptr_)
   SDX_BUF(int32_t, d_patch_meta_mid_)
