@@ -7,9 +7,10 @@ Workloads (all synthetic, seeded, resident in HBM before the timed region):
                       [1e4, 1e6] Pa, nu ~ U(0.3, 0.45), bottom face pinned,
                       1-3 sphere indenters each (r = 2 cm, random velocities
                       |v| <= 5 cm/s, penalty 1e6), 2 Newton iterations per
-                      step, learned CG warm start, solved together: one set of
-                      launches per CG iteration for all environments, launch
-                      schedule chosen by the autotuner on the first step.
+                      step, CG warm start from the previous Newton steps,
+                      solved together: one set of launches per CG iteration
+                      for all environments, launch schedule chosen by the
+                      autotuner on the first step.
   batch512            512 cubes, n ~ U{6..16}, one indenter pressing down,
                       1 Newton iteration.
   press40             one 40^3-hex cube (384,000 tets) under a 2 mm plate press.
@@ -21,7 +22,8 @@ with FP64 dots, FP64 iterate and an FP64 residual refresh every 10 iterations.
 A "step" is one Scene::Step of the workload.  The JSON line reports
 tet-steps/s, the roofline of the dominant kernel (the CG matvec, timed with
 HIP events at its own dispatch during the timed region), the same steps from
-a zero initial guess, the plain GPU PCG path on a sample of the environments
+a zero initial guess and with the trained network added to the warm start
+(--warm-start learned makes that the timed path), the plain GPU PCG path on a sample of the environments
 (each environment stepped on its own, FP32 PCG from a zero initial guess,
 eager launches), the position error against the CPU FP64 oracle on 32
 sampled environments, and the CPU oracle's throughput on a bounded sample.
@@ -288,9 +290,10 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "mixed", "fp64"], default="mixed")
-    ap.add_argument("--warm-start", choices=["learned", "history", "zero"], default="learned",
-                    help="learned: the trained network of project_superdex_amd/weights (zero if absent); "
-                         "history: the two previous Newton steps alone")
+    ap.add_argument("--warm-start", choices=["learned", "history", "zero"], default="history",
+                    help="history: the combination of the two previous Newton steps of least residual; "
+                         "learned: the same with the trained network of project_superdex_amd/weights "
+                         "as a third guess (history if the weights are absent)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--schedule", default=None,
                     help="fixed launch schedule instead of the autotuner, e.g. "
@@ -298,7 +301,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU PCG comparison")
     ap.add_argument("--no-zero-compare", action="store_true",
-                    help="skip the same steps from a zero initial guess (learned warm start only)")
+                    help="skip the same steps from a zero initial guess and with the other warm start")
     ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP64 position-error runs")
     ap.add_argument("--fp64-envs", type=int, default=32,
                     help="environments the CPU FP64 oracle steps for the position error")
@@ -334,14 +337,18 @@ def main(argv=None) -> int:
     print(f"[bench] {args.workload}: {len(spec.bodies)} envs, {spec.num_tets} tets", file=sys.stderr, flush=True)
     sc, rigid = gs.build_scene(spec, device=0, precision=mode)
     info = sc.info()
+    from project_superdex_amd.learned import DEFAULT_WEIGHTS as weights
+
     learned = False
     history = args.warm_start == "history"
     if args.warm_start == "learned":
-        from project_superdex_amd.learned import DEFAULT_WEIGHTS, WarmStartModel
+        from project_superdex_amd.learned import WarmStartModel
 
-        if os.path.exists(DEFAULT_WEIGHTS):
-            sc.set_warm_start_model(WarmStartModel.load(DEFAULT_WEIGHTS))
+        if os.path.exists(weights):
+            sc.set_warm_start_model(WarmStartModel.load(weights))
             learned = True
+        else:
+            history = True
 
     # warm-up (the first step autotunes the launch schedule)
     if args.schedule:
@@ -426,13 +433,17 @@ def main(argv=None) -> int:
                  "reference_equivalent_GBps": step_b / (elapsed / steps) / 1e9},
         "state_snapshot": snapshot,
     }
-    if learned and not args.no_zero_compare:
-        # the same steps from a zero initial guess and from the two previous Newton steps alone
-        # (the warm start without the network), same schedule
-        def leg(hist):
+    if (learned or history) and not args.no_zero_compare:
+        # the same steps from a zero initial guess, and with the other warm start (the two
+        # previous Newton steps alone, or with the trained network), same schedule
+        def leg(kind):
             z, zr = gs.build_scene(spec, device=0, precision=mode)
             z.set_schedule(**sc.schedule())
-            zp = solver_params(spec, mode, history=hist)
+            if kind == "learned":
+                from project_superdex_amd.learned import WarmStartModel
+
+                z.set_warm_start_model(WarmStartModel.load(weights))
+            zp = solver_params(spec, mode, learned=kind == "learned", history=kind == "history")
             run_steps(z, zr, spec, zp, 0, warmup)
             zst = []
             t0 = time.perf_counter()
@@ -440,28 +451,38 @@ def main(argv=None) -> int:
             zel = time.perf_counter() - t0
             z.close()
             zit = np.array([[s["linear_iters"] for s in st] for st in zst], dtype=np.float64)
-            return nT * steps / zel, float(zit.sum(axis=1).mean())
+            return nT * steps / zel, float(zit.sum(axis=1).mean()), [s for st in zst for s in st]
 
-        zero_value, zero_it = leg(False)
-        hist_value, hist_it = leg(True)
-        kept = sum(s["warm_start_accepted"] for s in per_env)
-        tried = sum(s["warm_start_tried"] for s in per_env)
-        out["learned_vs_zero"] = {
-            "zero_init_value": zero_value, "learned_value": value, "speedup": value / zero_value,
-            "cg_iters_zero": zero_it, "cg_iters_learned": float(it.sum(axis=1).mean()),
-            "history_only_value": hist_value, "cg_iters_history_only": hist_it,
-            "network_vs_history_speedup": value / hist_value,
-            "warm_starts_kept": int(kept), "warm_starts_tried": int(tried),
-            "median_guess_residual": float(np.median([s["warm_start_rel_res"] for s in per_env])),
-        }
-        # the network's own gate: in how many of the last warm starts it ran, and what its guess
-        # left of the residual the previous steps alone leave (geometric mean)
-        net = [(s["warm_start_rel_res"], s["warm_start_history_rel_res"]) for s in per_env
-               if s["warm_start_network"] and s["warm_start_history_rel_res"] > 0]
-        out["learned_vs_zero"]["network_ran_fraction"] = len(net) / max(1, len(per_env))
-        out["learned_vs_zero"]["network_residual_vs_history"] = (
-            float(np.exp(np.mean([np.log(max(r, 1e-300) / rh) for r, rh in net]))) if net else None)
-        print(f"[bench] zero-init and history-only comparisons done", file=sys.stderr, flush=True)
+        zero_value, zero_it, _ = leg("zero")
+        cg_it = float(it.sum(axis=1).mean())
+        cmp = {"warm_start": "learned" if learned else "history", "zero_init_value": zero_value,
+               "speedup": value / zero_value, "cg_iters_zero": zero_it, "cg_iters": cg_it}
+        if learned:
+            hist_value, hist_it, _ = leg("history")
+            learned_value, learned_it, net_env = value, cg_it, per_env
+        elif os.path.exists(weights):
+            learned_value, learned_it, net_env = leg("learned")
+            hist_value, hist_it = value, cg_it
+        else:
+            learned_value = None
+        if learned_value is not None:
+            kept = sum(s["warm_start_accepted"] for s in net_env)
+            tried = sum(s["warm_start_tried"] for s in net_env)
+            # the network's own gate: in how many of the warm starts it ran, and what its guess
+            # left of the residual the previous steps alone leave (geometric mean)
+            net = [(s["warm_start_rel_res"], s["warm_start_history_rel_res"]) for s in net_env
+                   if s["warm_start_network"] and s["warm_start_history_rel_res"] > 0]
+            cmp.update({
+                "learned_value": learned_value, "cg_iters_learned": learned_it,
+                "history_only_value": hist_value, "cg_iters_history_only": hist_it,
+                "network_vs_history_speedup": learned_value / hist_value,
+                "learned_warm_starts_kept": int(kept), "learned_warm_starts_tried": int(tried),
+                "learned_median_guess_residual": float(np.median([s["warm_start_rel_res"] for s in net_env])),
+                "network_ran_fraction": len(net) / max(1, len(net_env)),
+                "network_residual_vs_history": (float(np.exp(np.mean([np.log(max(r, 1e-300) / rh)
+                                                                      for r, rh in net]))) if net else None)})
+        out["warm_start_vs_zero"] = cmp
+        print(f"[bench] zero-init and other warm-start comparisons done", file=sys.stderr, flush=True)
     if batched and not args.no_plain:
         sample = sampled_envs(spec, args.plain_envs, args.seed + 1)
         rate, secs = plain_gpu(spec, steps, warmup, sample)
