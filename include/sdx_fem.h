@@ -343,6 +343,9 @@ sdx_status sdx_add_rigid_sphere(sdx_world* world, const double c[3], double r, i
                                 int32_t* out_collider);
 /* SetRootTransform translation: plane point or sphere centre. */
 sdx_status sdx_set_rigid_position(sdx_world* world, int32_t collider, const double pos[3]);
+/* GetRootTransform translation (mochi_physics/include/mochi_physics/cpp_api/mochi_actor.h:77):
+ * the collider's current target, as set or as restored with a state. */
+sdx_status sdx_get_rigid_position(sdx_world* world, int32_t collider, double out[3]);
 /* The same for n colliders in one call: collider colliders[k] moves to
  * positions[3 k .. 3 k + 2].  No counterpart in the reference (its rigid
  * actors are moved one by one); a batch of environments sets every target of
@@ -362,14 +365,25 @@ sdx_status sdx_set_velocities(sdx_world* world, int32_t actor, const double* in,
 sdx_status sdx_get_positions_world(sdx_world* world, int32_t actor, double* out, int32_t n);
 
 /* State snapshots (Scene::CaptureState / RestoreState / ReleaseState / ReleaseAllStates /
- * IsEqualState / CaptureStateToBytes / RestoreStateFromBytes,
- * mochi_physics/include/mochi_physics/cpp_api/mochi_scene.h:79-91).  A state is the
+ * IsEqualState / CaptureStateToBytes / RestoreStateFromBytes / CaptureStateToFile,
+ * mochi_physics/include/mochi_physics/cpp_api/mochi_scene.h:79-95).  A state is the
  * displacements and velocities of every actor, the collider targets and the Newton-step
- * history of the learned warm start, so steps after a restore repeat the steps after the
+ * history of the warm start, so steps after a restore repeat the steps after the
  * capture.  A handle keeps its state in device memory until it is released, or until actors,
- * pins or materials change (then every handle is dropped).  The byte form (sdx_state_size
- * bytes) starts with a header naming the world it came from; a world of another precision or
- * topology rejects it with SDX_ERR_STATE. */
+ * pins or materials change (then every handle is dropped).
+ *
+ * The byte form (sdx_state_size bytes) has the container layout of the reference's
+ * capture::CaptureState (mochi_physics/src/mochi_capture.cpp:38-75, 456-510): a 24-byte
+ * header {"mcap", uint16 context blocks, uint16 entity blocks, 8 zero bytes, uint64 bytes
+ * after the header}, component blocks each led by a 24-byte header {uint64 type id, uint64 entity
+ * hash, uint32 data size, uint32 entity count}, and the footer "mend".  Type ids are FNV-1a 64
+ * of the component names: context "sdx::CWarmStartGate" (12 int32); per actor
+ * "mochi::CDisplacementSlice<R>", "mochi::CVelocitySlice<R>" (uint64 n = 3 n_verts, then n
+ * reals) and "sdx::CNewtonStepHistory<R>" (uint64 n = 24 n_verts, then n reals); per collider
+ * "sdx::CColliderTarget" (target and stage-start target, 6 doubles); R = float or double.
+ * The entity hashes cover the actors' meshes, materials and permanent pins and the colliders'
+ * kinds, attachments and shapes: a world of another real type or topology rejects the bytes
+ * with SDX_ERR_STATE, malformed bytes give SDX_ERR_INVALID_ARGUMENT. */
 sdx_status sdx_capture_state(sdx_world* world, int32_t* handle);
 sdx_status sdx_restore_state(sdx_world* world, int32_t handle, int32_t release_immediately);
 sdx_status sdx_release_state(sdx_world* world, int32_t handle);  /* 0: every handle */
@@ -377,6 +391,35 @@ sdx_status sdx_is_equal_state(sdx_world* world, int32_t a, int32_t b, int32_t* e
 sdx_status sdx_state_size(sdx_world* world, int64_t* bytes);
 sdx_status sdx_capture_state_to_bytes(sdx_world* world, uint8_t* out, int64_t capacity);
 sdx_status sdx_restore_state_from_bytes(sdx_world* world, const uint8_t* data, int64_t size);
+/* Scene::CaptureStateToFile (mochi_scene.h:95): the byte form written to a file. */
+sdx_status sdx_capture_state_to_file(sdx_world* world, const char* path);
+
+/* Component kinds of a state, for the exclusions of a partial restore. */
+enum {
+  SDX_STATE_DISPLACEMENTS = 1,
+  SDX_STATE_VELOCITIES = 2,
+  SDX_STATE_WARM_START = 4,       /* Newton-step history and warm-start gate counters */
+  SDX_STATE_COLLIDER_TARGETS = 8,
+};
+/* capture::RestorePartialState (mochi_physics/src/mochi_capture.h:47-56): restore every
+ * component of the byte form except the kinds in `excluded` (SDX_STATE_* bits); the excluded
+ * ones are still checked against the world. */
+sdx_status sdx_restore_partial_state_from_bytes(sdx_world* world, const uint8_t* data, int64_t size,
+                                                uint32_t excluded);
+/* capture::IsEqualState on two byte forms (mochi_capture.h:58-66): equal values of every
+ * component (NaN equal to NaN), although the bytes may differ.  Needs no world or device. */
+sdx_status sdx_is_equal_state_bytes(const uint8_t* a, int64_t size_a, const uint8_t* b, int64_t size_b,
+                                    int32_t* equal);
+/* capture::CaptureStateToJson (mochi_capture.h:38-43): a human-readable JSON document of the
+ * state, {"actors": {"Soft": {"_handle", "_name", "_type", "components": {...}}, "Soft2": ...,
+ * "Rigid": ...}, "scene": {"sdx::CWarmStartGate": ...}}, with reals written to read back
+ * exactly (non-finite ones as null).  *length receives the text's length; out (capacity
+ * bytes, NUL-terminated) may be NULL with capacity 0 to query it, else capacity must exceed
+ * the length.  The first form converts a byte form and needs no world or device. */
+sdx_status sdx_state_bytes_to_json(const uint8_t* data, int64_t size, int32_t pretty, char* out, int64_t capacity,
+                                   int64_t* length);
+sdx_status sdx_capture_state_to_json(sdx_world* world, int32_t pretty, char* out, int64_t capacity,
+                                     int64_t* length);
 
 /* World totals: out[0..4] counts over all actors (vertices, tets, BSR blocks, boundary
  * triangles, environments); out[5], out[6] the CG matvec and vector tiles; out[7] the

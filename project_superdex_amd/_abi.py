@@ -222,3 +222,9 @@ def default_soft_params(E: float = 1e5, nu: float = 0.4, density: float = 1000.0
                       penalty_threshold=0.001, max_alignment_normals=0.0,
                       coulomb_friction_coefficient=coulomb, viscous_friction_coefficient=viscous,
                       friction_falloff_vel=falloff_vel, normal_viscous_damping_coefficient=normal_damping)
+
+# Component kinds of a captured state (SDX_STATE_*), for partial restores.
+STATE_DISPLACEMENTS = 1
+STATE_VELOCITIES = 2
+STATE_WARM_START = 4
+STATE_COLLIDER_TARGETS = 8

@@ -40,6 +40,7 @@ SIGNATURES = {
     "sdx_add_rigid_plane": (C.c_int32, [_vp, _dp, _dp, C.c_int32, _ip]),
     "sdx_add_rigid_sphere": (C.c_int32, [_vp, _dp, C.c_double, C.c_int32, _ip]),
     "sdx_set_rigid_position": (C.c_int32, [_vp, C.c_int32, _dp]),
+    "sdx_get_rigid_position": (C.c_int32, [_vp, C.c_int32, _dp]),
     "sdx_set_rigid_positions": (C.c_int32, [_vp, C.c_int32, _ip, _dp]),
     "sdx_step": (C.c_int32, [_vp, C.c_double, C.POINTER(_abi.SolverParams), C.POINTER(_abi.StepStats), C.c_int32]),
     "sdx_get_displacements": (C.c_int32, [_vp, C.c_int32, _dp, C.c_int32]),
@@ -54,6 +55,11 @@ SIGNATURES = {
     "sdx_state_size": (C.c_int32, [_vp, _i64p]),
     "sdx_capture_state_to_bytes": (C.c_int32, [_vp, C.c_void_p, C.c_int64]),
     "sdx_restore_state_from_bytes": (C.c_int32, [_vp, C.c_void_p, C.c_int64]),
+    "sdx_capture_state_to_file": (C.c_int32, [_vp, C.c_char_p]),
+    "sdx_restore_partial_state_from_bytes": (C.c_int32, [_vp, C.c_void_p, C.c_int64, C.c_uint32]),
+    "sdx_is_equal_state_bytes": (C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, _ip]),
+    "sdx_state_bytes_to_json": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, C.c_char_p, C.c_int64, _i64p]),
+    "sdx_capture_state_to_json": (C.c_int32, [_vp, C.c_int32, C.c_char_p, C.c_int64, _i64p]),
     "sdx_world_info": (C.c_int32, [_vp, _i64p]),
     "sdx_actor_bsr_size": (C.c_int32, [_vp, C.c_int32, _ip, _ip]),
     "sdx_assemble": (C.c_int32, [_vp, C.c_int32, C.c_double, C.POINTER(_abi.SolverParams), _dp, _ip, _ip, _dp]),

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "../../include/sdx_fem.h"
+#include "sdx_capture.h"
 #include "sdx_kernels.h"
 #include "sdx_mlp_host.h"
 #include "sdx_world.h"
@@ -283,6 +285,15 @@ sdx_status sdx_set_rigid_position(sdx_world* world, int32_t collider, const doub
   });
 }
 
+sdx_status sdx_get_rigid_position(sdx_world* world, int32_t collider, double out[3]) {
+  SDX_REQUIRE(world && out, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    const sdx::HostCollider& c = world->impl->collider_view(collider);
+    for (int d = 0; d < 3; ++d) out[d] = c.p[d];
+    return SDX_OK;
+  });
+}
+
 sdx_status sdx_set_rigid_positions(sdx_world* world, int32_t n, const int32_t* colliders, const double* positions) {
   SDX_REQUIRE(world && n >= 0 && (n == 0 || (colliders && positions)), "NULL argument");
   return guarded([&]() -> sdx_status {
@@ -416,11 +427,72 @@ sdx_status sdx_capture_state_to_bytes(sdx_world* world, uint8_t* out, int64_t ca
 }
 
 sdx_status sdx_restore_state_from_bytes(sdx_world* world, const uint8_t* data, int64_t size) {
+  return sdx_restore_partial_state_from_bytes(world, data, size, 0);
+}
+
+sdx_status sdx_restore_partial_state_from_bytes(sdx_world* world, const uint8_t* data, int64_t size,
+                                                uint32_t excluded) {
   SDX_REQUIRE(world && data, "NULL argument");
   SDX_REQUIRE(size >= 0, "size must not be negative");
+  SDX_REQUIRE((excluded & ~uint32_t(SDX_STATE_DISPLACEMENTS | SDX_STATE_VELOCITIES | SDX_STATE_WARM_START |
+                                     SDX_STATE_COLLIDER_TARGETS)) == 0,
+              "excluded holds bits that name no state component");
   return guarded([&]() -> sdx_status {
-    world->impl->restore_bytes(data, size_t(size));
+    world->impl->restore_bytes(data, size_t(size), excluded);
     return SDX_OK;
+  });
+}
+
+sdx_status sdx_capture_state_to_file(sdx_world* world, const char* path) {
+  SDX_REQUIRE(world && path, "NULL argument");
+  return guarded([&]() -> sdx_status {
+    std::vector<uint8_t> b(world->impl->state_size());
+    world->impl->capture_bytes(b.data(), b.size());
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return set_error(SDX_ERR_INVALID_ARGUMENT, "cannot open the state file for writing");
+    const size_t put = std::fwrite(b.data(), 1, b.size(), f);
+    const int closed = std::fclose(f);
+    if (put != b.size() || closed != 0) return set_error(SDX_ERR_STATE, "writing the state file failed");
+    return SDX_OK;
+  });
+}
+
+sdx_status sdx_is_equal_state_bytes(const uint8_t* a, int64_t size_a, const uint8_t* b, int64_t size_b,
+                                    int32_t* equal) {
+  SDX_REQUIRE(a && b && equal, "NULL argument");
+  SDX_REQUIRE(size_a >= 0 && size_b >= 0, "size must not be negative");
+  return guarded([&]() -> sdx_status {
+    *equal = sdx::capture::equal(a, size_t(size_a), b, size_t(size_b)) ? 1 : 0;
+    return SDX_OK;
+  });
+}
+
+static sdx_status json_out(const std::string& j, char* out, int64_t capacity, int64_t* length) {
+  *length = int64_t(j.size());
+  if (!out && capacity == 0) return SDX_OK;
+  SDX_REQUIRE(out, "NULL output buffer with a nonzero capacity");
+  SDX_REQUIRE(capacity > int64_t(j.size()), "JSON buffer too small (a call with out = NULL gives the length)");
+  std::memcpy(out, j.c_str(), j.size() + 1);
+  return SDX_OK;
+}
+
+sdx_status sdx_state_bytes_to_json(const uint8_t* data, int64_t size, int32_t pretty, char* out, int64_t capacity,
+                                   int64_t* length) {
+  SDX_REQUIRE(data && length, "NULL argument");
+  SDX_REQUIRE(size >= 0 && capacity >= 0, "sizes must not be negative");
+  return guarded([&]() -> sdx_status {
+    return json_out(sdx::capture::to_json(data, size_t(size), pretty != 0), out, capacity, length);
+  });
+}
+
+sdx_status sdx_capture_state_to_json(sdx_world* world, int32_t pretty, char* out, int64_t capacity,
+                                     int64_t* length) {
+  SDX_REQUIRE(world && length, "NULL argument");
+  SDX_REQUIRE(capacity >= 0, "capacity must not be negative");
+  return guarded([&]() -> sdx_status {
+    std::vector<uint8_t> b(world->impl->state_size());
+    world->impl->capture_bytes(b.data(), b.size());
+    return json_out(sdx::capture::to_json(b.data(), b.size(), pretty != 0), out, capacity, length);
   });
 }
 

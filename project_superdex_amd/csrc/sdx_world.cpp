@@ -29,6 +29,7 @@
 #include <type_traits>
 
 #include "sdx_world.h"
+#include "sdx_capture.h"
 
 namespace sdx {
 
@@ -2147,30 +2148,6 @@ void World<T>::info(int64_t out[8]) {
 
 // ---- state snapshots ----------------------------------------------------
 
-namespace {
-// Header of the byte form of a state: the world it was taken from (real
-// size, precision, vertex, actor and collider counts, a hash of the meshes,
-// materials, pins and colliders) and the warm-start counters, followed by the collider targets
-// and stage-start positions (double), then u, v and the Newton-step history (the world's real
-// type).
-struct StateHeader {
-  char magic[8];
-  uint32_t version;
-  uint32_t real_bytes;
-  int32_t precision;
-  int32_t n_actors;
-  int64_t n_verts;
-  int32_t n_colliders;
-  int32_t n_hist;
-  uint64_t topology;
-  int32_t ws_skip[4];
-  int32_t ws_seen[4];
-  int32_t ws_net_skip[4];
-  int32_t reserved;
-};
-constexpr char kStateMagic[8] = {'S', 'D', 'X', 'S', 'T', 'A', 'T', 0};
-constexpr uint32_t kStateVersion = 4;
-}  // namespace
 
 template <class T>
 typename World<T>::Snapshot& World<T>::snapshot(int32_t handle) {
@@ -2266,25 +2243,25 @@ bool World<T>::equal_states(int32_t a, int32_t b) {
 template <class T>
 size_t World<T>::state_size() {
   finalize();
-  const size_t n3 = size_t(3) * size_t(nV_);
-  return sizeof(StateHeader) + 6 * colliders_.size() * sizeof(double) + (2 + 2 * kNewtonHist) * n3 * sizeof(T);
+  // header, the gate block, three actor blocks (a length per actor), the
+  // collider block and the footer
+  const size_t n3 = size_t(3) * size_t(nV_), na = actors_.size(), nc = colliders_.size();
+  size_t n = 24 + 24 + size_t(capture::kGateCounters) * 4 + 4;
+  if (na) n += 3 * (24 + 8 * na) + (2 + capture::kHistPlanes) * n3 * sizeof(T);
+  if (nc) n += 24 + 6 * nc * sizeof(double);
+  return n;
 }
 
 namespace {
 // FNV-1a over what decides how a state steps: every actor's counts, rest
-// positions, tets, material (homogeneous parameters and per-element fields),
-// permanently pinned DoFs with their values (clearable boundary conditions
-// are targets that change from step to step), and every collider's kind, attachment,
-// normal and radius (collider positions are part of the state itself).
+// positions, tets, material (homogeneous parameters and per-element fields)
+// and permanently pinned DoFs with their values (clearable boundary
+// conditions are targets that change from step to step); and every
+// collider's kind, attachment, normal and radius (collider positions are part
+// of the state itself).
 class TopologyHash {
  public:
-  void bytes(const void* p, size_t n) {
-    const auto* b = static_cast<const uint8_t*>(p);
-    for (size_t k = 0; k < n; ++k) {
-      h_ ^= b[k];
-      h_ *= 1099511628211ull;
-    }
-  }
+  void bytes(const void* p, size_t n) { h_ = capture::fnv1a(p, n, h_); }
   template <class V>
   void vec(const std::vector<V>& v) {
     const uint64_t n = v.size();
@@ -2301,7 +2278,7 @@ class TopologyHash {
   uint64_t h_ = 1469598103934665603ull;
 };
 
-uint64_t topology_hash(const std::vector<HostActor>& actors, const std::vector<HostCollider>& colliders) {
+uint64_t actor_hash(const std::vector<HostActor>& actors) {
   TopologyHash h;
   for (const HostActor& a : actors) {
     h.pod(a.nV);
@@ -2314,6 +2291,11 @@ uint64_t topology_hash(const std::vector<HostActor>& actors, const std::vector<H
     h.vec(a.perm);
     h.vec(a.perm_value);
   }
+  return h.value();
+}
+
+uint64_t collider_hash(const std::vector<HostCollider>& colliders) {
+  TopologyHash h;
   for (const HostCollider& c : colliders) {
     h.pod(c.kind);
     h.pod(c.actor);
@@ -2324,80 +2306,107 @@ uint64_t topology_hash(const std::vector<HostActor>& actors, const std::vector<H
 }
 }  // namespace
 
+// The byte form: the container of include/sdx_fem.h's state formats, built
+// from one download of each device array.
 template <class T>
 void World<T>::capture_bytes(uint8_t* out, size_t cap) {
   const size_t need = state_size();
   if (cap < need) throw std::invalid_argument("state buffer too small (sdx_state_size gives the size)");
-  static_assert(kNewtonHist == 4, "the byte form holds four warm-start counters");
-  StateHeader hd{};
-  std::memcpy(hd.magic, kStateMagic, sizeof(hd.magic));
-  hd.version = kStateVersion;
-  hd.real_bytes = uint32_t(sizeof(T));
-  hd.precision = precision_;
-  hd.n_actors = int32_t(actors_.size());
-  hd.n_verts = nV_;
-  hd.n_colliders = int32_t(colliders_.size());
-  hd.n_hist = kNewtonHist;
-  hd.topology = topology_hash(actors_, colliders_);
-  std::memcpy(hd.ws_skip, wg_.skip, sizeof(hd.ws_skip));
-  std::memcpy(hd.ws_seen, wg_.seen, sizeof(hd.ws_seen));
-  std::memcpy(hd.ws_net_skip, wg_.net_skip, sizeof(hd.ws_net_skip));
-  uint8_t* o = out;
-  std::memcpy(o, &hd, sizeof(hd));
-  o += sizeof(hd);
-  for (const HostCollider& c : colliders_) {
-    std::memcpy(o, c.p, 3 * sizeof(double));
-    std::memcpy(o + 3 * sizeof(double), c.p0, 3 * sizeof(double));
-    o += 6 * sizeof(double);
-  }
+  static_assert(2 * kNewtonHist == capture::kHistPlanes, "the state holds two previous steps per Newton iteration");
+  static_assert(3 * kNewtonHist == capture::kGateCounters, "the state holds three gate counters per Newton iteration");
+  capture::StateParts sp;
+  sp.real_bytes = uint32_t(sizeof(T));
+  sp.actor_hash = actor_hash(actors_);
+  sp.collider_hash = collider_hash(colliders_);
   const size_t n3 = size_t(3) * size_t(nV_);
+  for (size_t a = 0; a < actors_.size(); ++a) sp.actor_dofs.push_back(3 * int64_t(actors_[a].nV));
   std::vector<T> h(n3);
+  auto bytes_of = [](const T* p, size_t n) {
+    const auto* b = reinterpret_cast<const uint8_t*>(p);
+    return std::vector<uint8_t>(b, b + n * sizeof(T));
+  };
   u_.download(h);
-  std::memcpy(o, h.data(), n3 * sizeof(T));
-  o += n3 * sizeof(T);
+  sp.u = bytes_of(h.data(), n3);
   v_.download(h);
-  std::memcpy(o, h.data(), n3 * sizeof(T));
-  o += n3 * sizeof(T);
-  h.resize(size_t(2 * kNewtonHist) * n3);
-  hist_.download(h);
-  std::memcpy(o, h.data(), h.size() * sizeof(T));
+  sp.v = bytes_of(h.data(), n3);
+  // history planes [plane][3 nV] -> per actor [plane][3 nV_a]
+  std::vector<T> hh(size_t(capture::kHistPlanes) * n3), ha(hh.size());
+  hist_.download(hh);
+  size_t o = 0;
+  for (size_t a = 0; a < actors_.size(); ++a) {
+    const size_t d = size_t(3 * actors_[a].nV), v0 = size_t(3 * env_v0_[a]);
+    for (int p = 0; p < capture::kHistPlanes; ++p, o += d)
+      std::memcpy(ha.data() + o, hh.data() + size_t(p) * n3 + v0, d * sizeof(T));
+  }
+  sp.hist = bytes_of(ha.data(), ha.size());
+  for (const HostCollider& c : colliders_) {
+    sp.colliders.insert(sp.colliders.end(), c.p, c.p + 3);
+    sp.colliders.insert(sp.colliders.end(), c.p0, c.p0 + 3);
+  }
+  std::memcpy(sp.gate, wg_.skip, sizeof(wg_.skip));
+  std::memcpy(sp.gate + kNewtonHist, wg_.seen, sizeof(wg_.seen));
+  std::memcpy(sp.gate + 2 * kNewtonHist, wg_.net_skip, sizeof(wg_.net_skip));
+  const std::vector<uint8_t> b = capture::encode(sp);
+  if (b.size() != need) throw std::logic_error("state size does not match its encoding");
+  std::memcpy(out, b.data(), b.size());
 }
 
+// Restore the components of a byte form that `excluded` (SDX_STATE_* bits)
+// does not name (RestorePartialState, mochi_capture.cpp:522-566); every
+// component is checked against this world first, so a rejected state leaves
+// the world as it was.
 template <class T>
-void World<T>::restore_bytes(const uint8_t* in, size_t n) {
-  const size_t need = state_size();
-  StateHeader hd{};
-  if (n < sizeof(hd)) throw std::invalid_argument("not a state: too short");
-  std::memcpy(&hd, in, sizeof(hd));
-  if (std::memcmp(hd.magic, kStateMagic, sizeof(hd.magic)) != 0 || hd.version != kStateVersion)
-    throw std::invalid_argument("not a state of this library version");
-  if (hd.real_bytes != sizeof(T) || hd.precision != precision_ || hd.n_actors != int32_t(actors_.size()) ||
-      hd.n_verts != nV_ || hd.n_colliders != int32_t(colliders_.size()) || hd.n_hist != kNewtonHist ||
-      hd.topology != topology_hash(actors_, colliders_))
-    throw std::runtime_error("state was captured from a world of another precision or topology");
-  if (n != need) throw std::invalid_argument("state size does not match its header");
-  const uint8_t* q = in + sizeof(hd);
-  for (HostCollider& c : colliders_) {
-    std::memcpy(c.p, q, 3 * sizeof(double));
-    std::memcpy(c.p0, q + 3 * sizeof(double), 3 * sizeof(double));
-    q += 6 * sizeof(double);
-  }
-  colliders_dirty_ = true;
+void World<T>::restore_bytes(const uint8_t* in, size_t n, uint32_t excluded) {
+  finalize();
+  uint32_t present = 0;
+  const capture::StateParts sp = capture::decode(in, n, &present);
   const size_t n3 = size_t(3) * size_t(nV_);
-  // one-off transfers: plain uploads, so the pinned staging ring keeps its size
-  std::vector<T> h(n3);
-  std::memcpy(h.data(), q, n3 * sizeof(T));
-  q += n3 * sizeof(T);
-  u_.upload(h);
-  std::memcpy(h.data(), q, n3 * sizeof(T));
-  q += n3 * sizeof(T);
-  v_.upload(h);
-  h.resize(size_t(2 * kNewtonHist) * n3);
-  std::memcpy(h.data(), q, h.size() * sizeof(T));
-  hist_.upload(h);
-  std::memcpy(wg_.skip, hd.ws_skip, sizeof(wg_.skip));
-  std::memcpy(wg_.seen, hd.ws_seen, sizeof(wg_.seen));
-  std::memcpy(wg_.net_skip, hd.ws_net_skip, sizeof(wg_.net_skip));
+  const bool has_actors = !actors_.empty();
+  std::vector<int64_t> dofs;
+  for (const HostActor& a : actors_) dofs.push_back(3 * int64_t(a.nV));
+  const uint32_t actor_kinds = capture::kDisplacements | capture::kVelocities;
+  const bool actors_ok = has_actors ? ((present & actor_kinds) == actor_kinds && sp.real_bytes == sizeof(T) &&
+                                       sp.actor_dofs == dofs && sp.actor_hash == actor_hash(actors_))
+                                    : (present & actor_kinds) == 0;
+  const bool colliders_ok = colliders_.empty() ? (present & capture::kColliderTargets) == 0
+                                               : ((present & capture::kColliderTargets) &&
+                                                  sp.colliders.size() == 6 * colliders_.size() &&
+                                                  sp.collider_hash == collider_hash(colliders_));
+  if (!actors_ok || !colliders_ok)
+    throw std::runtime_error("state was captured from a world of another precision or topology");
+  if (!(excluded & capture::kColliderTargets))
+    for (size_t c = 0; c < colliders_.size(); ++c) {
+      std::memcpy(colliders_[c].p, sp.colliders.data() + 6 * c, 3 * sizeof(double));
+      std::memcpy(colliders_[c].p0, sp.colliders.data() + 6 * c + 3, 3 * sizeof(double));
+      colliders_dirty_ = true;
+    }
+  if (has_actors) {
+    // one-off transfers: plain uploads, so the pinned staging ring keeps its size
+    std::vector<T> h(n3);
+    if (!(excluded & capture::kDisplacements)) {
+      std::memcpy(h.data(), sp.u.data(), n3 * sizeof(T));
+      u_.upload(h);
+    }
+    if (!(excluded & capture::kVelocities)) {
+      std::memcpy(h.data(), sp.v.data(), n3 * sizeof(T));
+      v_.upload(h);
+    }
+    if (!(excluded & capture::kWarmStart)) {
+      std::vector<T> hh(size_t(capture::kHistPlanes) * n3);
+      size_t o = 0;
+      for (size_t a = 0; a < actors_.size(); ++a) {
+        const size_t d = size_t(3 * actors_[a].nV), v0 = size_t(3 * env_v0_[a]);
+        for (int p = 0; p < capture::kHistPlanes; ++p, o += d)
+          std::memcpy(hh.data() + size_t(p) * n3 + v0, sp.hist.data() + o * sizeof(T), d * sizeof(T));
+      }
+      hist_.upload(hh);
+    }
+  }
+  if (!(excluded & capture::kWarmStart)) {
+    std::memcpy(wg_.skip, sp.gate, sizeof(wg_.skip));
+    std::memcpy(wg_.seen, sp.gate + kNewtonHist, sizeof(wg_.seen));
+    std::memcpy(wg_.net_skip, sp.gate + 2 * kNewtonHist, sizeof(wg_.net_skip));
+  }
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 

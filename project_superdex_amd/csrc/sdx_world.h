@@ -235,6 +235,10 @@ class WorldBase {
     colliders_dirty_ = true;
     return colliders_[size_t(c)];
   }
+  const HostCollider& collider_view(int c) const {
+    if (c < 0 || c >= int(colliders_.size())) throw std::out_of_range("no such collider");
+    return colliders_[size_t(c)];
+  }
   // Bring device state back to host mirrors and drop device buffers; the next
   // step rebuilds them (topology or material change).
   void invalidate() {
@@ -289,7 +293,7 @@ class WorldBase {
   virtual bool equal_states(int32_t a, int32_t b) = 0;
   virtual size_t state_size() = 0;
   virtual void capture_bytes(uint8_t* out, size_t cap) = 0;
-  virtual void restore_bytes(const uint8_t* in, size_t n) = 0;
+  virtual void restore_bytes(const uint8_t* in, size_t n, uint32_t excluded) = 0;
   const sdx_schedule& schedule() const { return sched_; }
 
   // kernel timing: class 0 CG matvec, 1 CG update, 2 element, 3 BSR gather.
@@ -432,7 +436,7 @@ class World final : public WorldBase {
   bool equal_states(int32_t a, int32_t b) override;
   size_t state_size() override;
   void capture_bytes(uint8_t* out, size_t cap) override;
-  void restore_bytes(const uint8_t* in, size_t n) override;
+  void restore_bytes(const uint8_t* in, size_t n, uint32_t excluded) override;
   void set_ws_params(const float* params) override;
   void collect(int64_t capacity, int per_env) override;
   int64_t collected(float* x, float* t, int64_t cap_rows) override;

@@ -15,6 +15,7 @@ Python or CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -250,16 +251,25 @@ class Scene:
                 "matvec_tiles": out[5], "vector_tiles": out[6], "cg_groups": out[7]}
 
     # State snapshots (Scene::CaptureState / RestoreState / ReleaseState /
-    # IsEqualState / CaptureStateToBytes / RestoreStateFromBytes,
-    # mochi_physics/include/mochi_physics/cpp_api/mochi_scene.h:79-91)
+    # IsEqualState / CaptureStateToBytes / RestoreStateFromBytes /
+    # CaptureStateToFile, mochi_physics/include/mochi_physics/cpp_api/mochi_scene.h:79-95;
+    # capture::RestorePartialState / CaptureStateToJson, mochi_physics/src/mochi_capture.h:38-56)
     def capture_state(self) -> int:
         """Snapshot of the dynamic state kept in device memory; returns its handle."""
         h = C.c_int32()
         check(lib().sdx_capture_state(self._w, C.byref(h)))
         return h.value
 
+    def _pull_collider_positions(self):
+        """Re-read every collider target from the world (a restore sets them)."""
+        out = np.zeros(3)
+        for k in range(len(self.colliders)):
+            check(lib().sdx_get_rigid_position(self._w, k, out.ctypes.data_as(C.POINTER(C.c_double))))
+            self._coll_pos_buf[k] = out
+
     def restore_state(self, handle: int, release_immediately: bool = False):
         check(lib().sdx_restore_state(self._w, int(handle), int(bool(release_immediately))))
+        self._pull_collider_positions()
 
     def release_state(self, handle: int):
         check(lib().sdx_release_state(self._w, int(handle)))
@@ -282,6 +292,23 @@ class Scene:
     def restore_state_from_bytes(self, data: bytes):
         buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
         check(lib().sdx_restore_state_from_bytes(self._w, buf, len(data)))
+        self._pull_collider_positions()
+
+    def restore_partial_state_from_bytes(self, data: bytes, excluded: int):
+        """Restore every component except the kinds in `excluded` (_abi.STATE_* bits)."""
+        buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+        check(lib().sdx_restore_partial_state_from_bytes(self._w, buf, len(data), int(excluded)))
+        self._pull_collider_positions()
+
+    def capture_state_to_file(self, path: str):
+        check(lib().sdx_capture_state_to_file(self._w, os.fsencode(path)))
+
+    def capture_state_to_json(self, pretty: bool = False) -> str:
+        n = C.c_int64()
+        check(lib().sdx_capture_state_to_json(self._w, int(pretty), None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().sdx_capture_state_to_json(self._w, int(pretty), buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
 
     def set_cg_burst(self, burst: int):
         check(lib().sdx_set_cg_burst(self._w, burst))
@@ -334,6 +361,26 @@ class Scene:
 
 
 _STATS_DTYPE = None
+
+
+def is_equal_state_bytes(a: bytes, b: bytes) -> bool:
+    """capture::IsEqualState on two byte forms: equal values, whatever the bytes."""
+    ba = (C.c_uint8 * max(1, len(a))).from_buffer_copy(a or b"\0")
+    bb = (C.c_uint8 * max(1, len(b))).from_buffer_copy(b or b"\0")
+    eq = C.c_int32()
+    check(lib().sdx_is_equal_state_bytes(ba, len(a), bb, len(b), C.byref(eq)))
+    return bool(eq.value)
+
+
+def state_bytes_to_json(data: bytes, pretty: bool = False) -> str:
+    """The JSON document of a byte form (no world or device needed)."""
+    buf = (C.c_uint8 * max(1, len(data))).from_buffer_copy(data or b"\0")
+    n = C.c_int64()
+    check(lib().sdx_state_bytes_to_json(buf, len(data), int(pretty), None, 0, C.byref(n)))
+    out = C.create_string_buffer(n.value + 1)
+    check(lib().sdx_state_bytes_to_json(buf, len(data), int(pretty), out, n.value + 1, C.byref(n)))
+    return out.value.decode()
+
 
 
 def _stats_dtype() -> np.dtype:

@@ -14,8 +14,17 @@ mochi_physics/include/mochi_physics/cpp_api/mochi_scene.h:79-91).
 - the byte form round-trips exactly (the same bytes after restoring them,
   also in a second world of the same topology, whose steps then match the
   first world's), carries the collider targets, and is rejected by a world
-  of another topology or precision and when cut short.
+  of another topology or real type and when cut short;
+- the byte form has the documented container layout (read back by the plain
+  Python reader of tests/state_format.py: every actor's displacements and
+  velocities, the collider targets); partial restores leave the excluded
+  components as they are (capture::RestorePartialState); the JSON form holds
+  the same values (capture::CaptureStateToJson); the file form is the byte
+  form; IsEqualState on byte forms agrees with IsEqualState on handles.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -160,9 +169,13 @@ def test_byte_form_round_trips_and_is_checked():
         sc.restore_state_from_bytes(b1[:-8])
     with pytest.raises(SdxError):
         sc.restore_state_from_bytes(b"not a state")
-    fp32, _ = gs.build_scene(spec, precision=_abi.PREC_MODE_FP32)
+    fp64, _ = gs.build_scene(spec, precision=_abi.PREC_MODE_FP64)
     with pytest.raises(SdxError, match="another precision or topology"):
-        fp32.restore_state_from_bytes(b1)
+        fp64.restore_state_from_bytes(b1)
+    # the state of a mixed-precision world is one of FP32 reals: an FP32 world takes it
+    fp32, _ = gs.build_scene(spec, precision=_abi.PREC_MODE_FP32)
+    fp32.restore_state_from_bytes(b1)
+    assert fp32.capture_state_to_bytes() == b1
     other, _ = gs.build_scene(spec_of(seed=12), precision=MIXED)
     with pytest.raises(SdxError):
         other.restore_state_from_bytes(b1)
@@ -202,3 +215,68 @@ def test_byte_form_rejects_same_size_worlds_of_another_scene():
     same = variant(lambda s: None)
     same.restore_state_from_bytes(b1)
     assert same.capture_state_to_bytes() == b1
+
+
+def test_byte_form_layout_partial_restore_json_and_file(tmp_path):
+    import state_format as sf
+
+    spec = spec_of(steps=6)
+    p = params()
+    p.warm_start = _abi.WARM_START_HISTORY  # the Newton-step history is written every step
+    sc, rigid = gs.build_scene(spec, precision=MIXED)
+    run(sc, rigid, spec, p, 0, 3)
+    b1 = sc.capture_state_to_bytes()
+    s1 = state(sc)
+    c1 = [r.position for r in rigid]
+    # the documented layout, read by plain Python
+    blocks = sf.read(b1)
+    disp, vel, hst = sf.actor_names(np.float32)
+    assert len(blocks[disp][1]) == len(sc.actors) and blocks[disp][0] == blocks[vel][0] == blocks[hst][0]
+    for a, (u, v) in enumerate(s1):
+        np.testing.assert_array_equal(blocks[disp][1][a], u.astype(np.float32).ravel())
+        np.testing.assert_array_equal(blocks[vel][1][a], v.astype(np.float32).ravel())
+        assert blocks[hst][1][a].size == sf.HIST_PLANES * u.size and np.abs(blocks[hst][1][a]).max() > 0
+    np.testing.assert_array_equal(blocks[sf.COLLIDER][1][:, :3], np.array(c1))
+    # JSON: the same values
+    doc = json.loads(sc.capture_state_to_json())
+    names = ["Soft"] + [f"Soft{a + 1}" for a in range(1, len(sc.actors))]
+    for a, name in enumerate(names):
+        np.testing.assert_array_equal(np.array(doc["actors"][name]["components"][disp], np.float32),
+                                      s1[a][0].astype(np.float32).ravel())
+    assert json.loads(sf_json := sc.capture_state_to_json(pretty=True)) == doc and "\n" in sf_json
+    # the file form is the byte form
+    path = os.path.join(str(tmp_path), "state.mcap")
+    sc.capture_state_to_file(path)
+    assert open(path, "rb").read() == b1
+    # steps move everything on; a partial restore brings back only what it does not exclude
+    run(sc, rigid, spec, p, 3, 5)
+    b2 = sc.capture_state_to_bytes()
+    c2 = [r.position for r in rigid]
+    assert not gs.is_equal_state_bytes(b1, b2)
+    k1, k2 = sf.read(b1), sf.read(b2)
+
+    def same(x, y):
+        return all(np.array_equal(np.asarray(p), np.asarray(q)) for p, q in zip(x[1], y[1]))
+
+    assert not any(same(k1[n], k2[n]) for n in (disp, vel, hst, sf.COLLIDER))
+    sc.restore_partial_state_from_bytes(
+        b1, _abi.STATE_VELOCITIES | _abi.STATE_COLLIDER_TARGETS | _abi.STATE_WARM_START)
+    got = sf.read(sc.capture_state_to_bytes())
+    assert same(got[disp], k1[disp])
+    assert same(got[vel], k2[vel]) and same(got[hst], k2[hst]) and same(got[sf.COLLIDER], k2[sf.COLLIDER])
+    assert np.array_equal(got[sf.GATE][1], k2[sf.GATE][1])
+    for r, c in zip(rigid, c2):
+        np.testing.assert_array_equal(r.position, c)
+    # the rest of b1: now every component is b1's
+    sc.restore_partial_state_from_bytes(b1, _abi.STATE_DISPLACEMENTS)
+    assert gs.is_equal_state_bytes(sc.capture_state_to_bytes(), b1)
+    assert_close(state(sc), s1, tol=0.0)
+    for r, c in zip(rigid, c1):
+        np.testing.assert_array_equal(r.position, c)
+    # IsEqualState on bytes agrees with IsEqualState on handles
+    h1 = sc.capture_state()
+    sc.restore_state_from_bytes(b2)
+    h2 = sc.capture_state()
+    assert sc.is_equal_state(h1, h2) == gs.is_equal_state_bytes(b1, b2) is False
+    with pytest.raises(SdxError):
+        sc.restore_partial_state_from_bytes(b1, 64)
