@@ -491,6 +491,19 @@ typedef struct sdx_schedule {
                              default 1: one group, the records of every environment kept, which
                              measured fastest: the element and gather kernels are bound by their
                              scattered accesses, not by HBM bandwidth) */
+  int32_t fused_asm;      /* 1: every assembly of residual and matrix runs as one fused kernel over
+                             row blocks (each workgroup owns a spatially compact block of up to
+                             64 vertices, computes every tet and contact triangle touching it and
+                             writes its BSR rows, residuals and preconditioner blocks directly, so
+                             no per-element records pass through memory; its LDS sums with float
+                             atomics vary in the last bits from run to run).  Residual-only
+                             assemblies use the records either way, and so does a world with a
+                             vertex coupled to more than about 1000 (FP64: 500) vertices, which
+                             no block holds.  Default 0: element and contact records gathered
+                             into the matrix (asm_groups applies), which measured about 4 times
+                             faster: a block recomputes the tets it shares with its neighbours
+                             (about 1.7-1.9 times the tets) at an occupancy of two workgroups
+                             per CU.  The autotuner leaves it off */
 } sdx_schedule;
 
 sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* schedule);

@@ -90,6 +90,7 @@ class Schedule(C.Structure):
         ("group_iters", C.c_int32),
         ("fused_tail", C.c_int32),
         ("asm_groups", C.c_int32),
+        ("fused_asm", C.c_int32),
     ]
 
     def as_dict(self):

@@ -20,7 +20,7 @@ ARCH = os.environ.get("SDX_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["sdx_kernels.hip", "sdx_mlp.hip", "sdx_world.cpp", "sdx_api.cpp", "sdx_pcg.cpp", "sdx_mlp_host.cpp",
            "sdx_pcg_noalloc.hip", "sdx_capture.cpp"]
-HEADERS = ["sdx_kernels.h", "sdx_layout.h", "sdx_maps.h", "sdx_math.h", "sdx_world.h", "sdx_device.h", "sdx_mlp.h", "sdx_mlp_host.h", "sdx_pcg.h", "sdx_capture.h"]
+HEADERS = ["sdx_kernels.h", "sdx_layout.h", "sdx_maps.h", "sdx_math.h", "sdx_world.h", "sdx_device.h", "sdx_mlp.h", "sdx_mlp_host.h", "sdx_pcg.h", "sdx_capture.h", "sdx_rowblocks.h"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

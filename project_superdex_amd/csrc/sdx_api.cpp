@@ -568,6 +568,7 @@ sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* sc) {
   SDX_REQUIRE(sc->group_iters >= 0 && sc->group_iters <= 4096, "group_iters must be in 0..4096");
   SDX_REQUIRE(sc->fused_tail == 0 || sc->fused_tail == 1, "fused_tail must be 0 or 1");
   SDX_REQUIRE(sc->asm_groups >= 0 && sc->asm_groups <= 4096, "asm_groups must be in 0..4096");
+  SDX_REQUIRE(sc->fused_asm == 0 || sc->fused_asm == 1, "fused_asm must be 0 or 1");
   return guarded([&]() -> sdx_status {
     world->impl->set_schedule(*sc);
     return SDX_OK;

@@ -195,6 +195,38 @@ void launch_contact(hipStream_t s, const MeshView& m, const AsmRange& r, const F
 template <class T>
 void launch_gather_bsr(hipStream_t s, const MeshView& m, const AsmRange& r, const T* Dv, const T* Ev, int prec,
                        T* vals, T* dinv, const KernelEvents* ev = nullptr);
+// Fused assembly over row blocks (sdx_rowblocks.h): a workgroup per block
+// computes every tet and boundary triangle touching its rows, sums their
+// contributions to those rows in LDS and writes the rows' BSR values (with
+// the Dirichlet rows and columns), the preconditioner blocks, the residual
+// (Dirichlet rows zeroed) and the environment's ||res||^2 into acc (set 0,
+// zeroed by the caller).  Environments with env_mask 0 are skipped.
+struct RowBlockView {
+  int32_t n;            // blocks
+  int32_t max_slots;    // BSR slots of the largest block (its LDS)
+  const int32_t* row0;  // [n+1] rows of block b: [row0[b], row0[b+1]) of rows
+  const int32_t* tet0;  // [n+1] tets of block b in tets
+  const int32_t* tri0;  // [n+1] boundary triangles of block b in tris
+  const int32_t* env;   // [n]
+  const int32_t* rows;  // global vertex of each block row (ascending within a block)
+  const int32_t* rbase; // first LDS slot of each block row
+  const int2* tets;     // (global tet, block-local row of each vertex: 8 bits each, 0xFF: another block)
+  const int2* tris;     // (global triangle, block-local rows of its 3 vertices)
+};
+// LDS bytes of the fused assembly's workgroup for blocks of at most max_slots
+// slots: values [slots][9], residual [rows][3], columns [slots], per row its
+// first global slot, LDS base (plus the end) and vertex, the row of each slot.
+constexpr int kRowBlockRowsMax = 64;
+template <class T>
+inline size_t rows_lds_bytes_host(int max_slots) {
+  return size_t(max_slots) * 9 * sizeof(T) + size_t(kRowBlockRowsMax) * 3 * sizeof(T) + size_t(max_slots) * 4 +
+         size_t(3 * kRowBlockRowsMax + 1) * 4 + size_t(max_slots);
+}
+template <class T, class A>
+void launch_assemble_rows(hipStream_t s, const MeshView& m, const RowBlockView& rb, const FieldPtrs<T>& f,
+                          const T* u, const T* u0, const T* v0, double dt, const double grav[3], int project,
+                          int prec, T* vals, T* dinv, T* res, A* acc, bool damp = false,
+                          const KernelEvents* ev = nullptr);
 // acc: set 0 of the accumulators (zeroed by the caller); per-env ||res||^2.
 template <class T, class A>
 void launch_gather_res(hipStream_t s, const MeshView& m, const AsmRange& r, const Tiles& tv, const T* Rv, T* res,

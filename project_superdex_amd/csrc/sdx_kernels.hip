@@ -30,6 +30,7 @@
 #include "sdx_device.h"
 #include "sdx_kernels.h"
 #include "sdx_math.h"
+#include "sdx_rowblocks.h"
 
 namespace sdx {
 
@@ -57,6 +58,118 @@ extern __shared__ __align__(16) unsigned char sdx_dyn_lds[];
 template <class T>
 constexpr size_t patch_lds_bytes() {
   return size_t(kStageE) * 64 * sizeof(T) + size_t(kPatchMetaWords) * 4;
+}
+
+// The inputs of one tet: shape gradients, volume, material, mass and
+// damping, its displacement u, u - predicted target (inertia) and u - u0
+// (damping).  Loads issued in this order: tet geometry planes, then the
+// vertex data.
+template <class T>
+struct TetIn {
+  V3<T> dN[4];
+  T vol, mu, lam, mass, alpha;
+  T ue[12], w[12], wd[12];
+};
+
+template <class T>
+__device__ __forceinline__ void load_tet(const FieldPtrs<T>& f, int nT, int t, const int nd[4],
+                                         const T* __restrict__ u, const T* __restrict__ u0,
+                                         const T* __restrict__ v0, T dt, TetIn<T>& in) {
+  const T* g = f.geom;
+  in.dN[1] = mk3(g[0 * nT + t], g[1 * nT + t], g[2 * nT + t]);
+  in.dN[2] = mk3(g[3 * nT + t], g[4 * nT + t], g[5 * nT + t]);
+  in.dN[3] = mk3(g[6 * nT + t], g[7 * nT + t], g[8 * nT + t]);
+  in.dN[0] = scl3(T(-1), add3(add3(in.dN[1], in.dN[2]), in.dN[3]));
+  in.vol = g[kGVol * nT + t];
+  in.mu = g[kGMu * nT + t];
+  in.lam = g[kGLambda * nT + t];
+  in.mass = g[kGMass * nT + t];
+  in.alpha = g[kGAlpha * nT + t];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T uu = u[3 * nd[a] + d], s0 = u0[3 * nd[a] + d];
+      in.ue[3 * a + d] = uu;
+      in.w[3 * a + d] = uu - (s0 + dt * v0[3 * nd[a] + d]);  // u - predicted target (batched_element_utils.h:90-120)
+      in.wd[3 * a + d] = uu - s0;
+    }
+}
+
+// One tet's contributions (fem_stress.h:107-262, fem_inertia.h:55-250,
+// fem_gravity.h:44-125, fem_stress_damping.h): on_res(res[12]) receives the
+// residual of its 4 vertices (stress, inertia of the consistent mass,
+// gravity, damping), formed before the Hessian; emit(fa, fb, q) each Hessian
+// block with the consistent-mass term, q = 6 symmetric entries (00 11 22 01
+// 02 12) for fa == fb, else 9 row-major entries of block (fa, fb), fa < fb.
+template <class T, bool kDamp, class OnRes, class Emit>
+__device__ __forceinline__ void tet_contrib(const FieldPtrs<T>& f, int nT, int t, const TetIn<T>& in, T dt, T gx,
+                                            T gy, T gz, bool wantH, bool project, OnRes&& on_res, Emit&& emit) {
+  const T* g = f.geom;
+  // stiffness damping (fem_stress_damping.h), kappa = beta / dt (mochi_soft.h:259-292)
+  Viscous<T> vis;
+  bool damp = false;
+  if constexpr (kDamp) {
+    const T beta = g[kGBeta * nT + t];
+    damp = beta > T(0);
+    if (damp) {
+      T us[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) us[k] = in.ue[k] - in.wd[k];
+      viscous_setup<T>(in.dN, in.ue, us, beta / dt, in.mu, in.lam, g[kGBetaGeo * nT + t] != T(0), project, vis);
+    }
+  }
+  // consistent mass M_ab = m/20 (1 + delta_ab) (Q4 integrates P1 x P1 exactly)
+  const T idt2 = T(1) / (dt * dt);
+  const T mdamp = in.alpha > T(0) ? in.alpha / dt : T(0);
+  const T m20 = in.mass * T(0.05);
+  const T ms = (idt2 + mdamp) * m20;
+  auto block = [&](int fa, int fb, const T* q) {
+    if (fa == fb) {
+      T dq[6];
+#pragma unroll
+      for (int r = 0; r < 6; ++r) dq[r] = q[r];
+      if (kDamp && damp) viscous_block<T>(vis, in.dN[fa], in.dN[fb], true, in.vol, dq);
+      dq[0] += T(2) * ms;
+      dq[1] += T(2) * ms;
+      dq[2] += T(2) * ms;
+      emit(fa, fb, dq);
+      return;
+    }
+    T o[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) o[r] = q[r];
+    if (kDamp && damp) viscous_block<T>(vis, in.dN[fa], in.dN[fb], false, in.vol, o);
+    o[0] += ms;
+    o[4] += ms;
+    o[8] += ms;
+    emit(fa, fb, o);
+  };
+  // the residual of each vertex: stress, inertia of the consistent mass and
+  // gravity (fem_gravity.h:108-118: res_f -= rho g vol / 4)
+  auto res_of = [&](const T* res_in) {
+    T res[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) res[k] = res_in[k];
+    if (kDamp && damp) viscous_res<T>(vis, in.dN, in.vol, res);
+    const T gq = T(0.25) * in.mass;
+    const T* w = in.w;
+    const T* wd = in.wd;
+    const T sw[3] = {w[0] + w[3] + w[6] + w[9], w[1] + w[4] + w[7] + w[10], w[2] + w[5] + w[8] + w[11]};
+    const T sd[3] = {wd[0] + wd[3] + wd[6] + wd[9], wd[1] + wd[4] + wd[7] + wd[10], wd[2] + wd[5] + wd[8] + wd[11]};
+    const T gv[3] = {gx, gy, gz};
+    T out[12];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        T r = res[3 * a + d] - gv[d] * gq + idt2 * m20 * (w[3 * a + d] + sw[d]);
+        if (mdamp > T(0)) r += mdamp * m20 * (wd[3 * a + d] + sd[d]);
+        out[3 * a + d] = r;
+      }
+    on_res(out);
+  };
+  smith_element<T>(in.dN, in.ue, in.vol, in.mu, in.lam, wantH, project, res_of, block);
 }
 
 template <class T, bool kDamp>
@@ -90,24 +203,8 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldP
   const int t = t0 + lane < t1 ? t0 + lane : t0;
   const int4 tv = reinterpret_cast<const int4*>(m.tets)[t];
   const int nd[4] = {tv.x, tv.y, tv.z, tv.w};
-  const T* g = f.geom;
-  V3<T> dN[4];
-  dN[1] = mk3(g[0 * nT + t], g[1 * nT + t], g[2 * nT + t]);
-  dN[2] = mk3(g[3 * nT + t], g[4 * nT + t], g[5 * nT + t]);
-  dN[3] = mk3(g[6 * nT + t], g[7 * nT + t], g[8 * nT + t]);
-  dN[0] = scl3(T(-1), add3(add3(dN[1], dN[2]), dN[3]));
-  const T vol = g[kGVol * nT + t], mu = g[kGMu * nT + t], lam = g[kGLambda * nT + t];
-  const T mass = g[kGMass * nT + t], alpha = g[kGAlpha * nT + t];
-  T ue[12], w[12], wd[12];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const T uu = u[3 * nd[a] + d], s0 = u0[3 * nd[a] + d];
-      ue[3 * a + d] = uu;
-      w[3 * a + d] = uu - (s0 + dt * v0[3 * nd[a] + d]);  // u - predicted target (batched_element_utils.h:90-120)
-      wd[3 * a + d] = uu - s0;
-    }
+  TetIn<T> in;
+  load_tet<T>(f, nT, t, nd, u, u0, v0, dt, in);
   uint4* mdst = reinterpret_cast<uint4*>(meta);
 #pragma unroll
   for (int k = 0; k < kMetaVecs; ++k) {
@@ -118,76 +215,26 @@ __global__ void __launch_bounds__(256) k_element(MeshView m, AsmRange ar, FieldP
   const bool mine = on && t0 + lane < t1;
   T off[54];  // the off-diagonal blocks, staged in the second phase
   if (mine) {
-    // stiffness damping (fem_stress_damping.h), kappa = beta / dt (mochi_soft.h:259-292)
-    Viscous<T> vis;
-    bool damp = false;
-    if constexpr (kDamp) {
-      const T beta = g[kGBeta * nT + t];
-      damp = beta > T(0);
-      if (damp) {
-        T us[12];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) us[k] = ue[k] - wd[k];
-        viscous_setup<T>(dN, ue, us, beta / dt, mu, lam, g[kGBetaGeo * nT + t] != T(0), project != 0, vis);
-      }
-    }
-    // consistent mass M_ab = m/20 (1 + delta_ab) (Q4 integrates P1 x P1 exactly)
-    const T idt2 = T(1) / (dt * dt);
-    const T mdamp = alpha > T(0) ? alpha / dt : T(0);
-    const T m20 = mass * T(0.05);
-    const T ms = (idt2 + mdamp) * m20;
     T* sv = stage + lane * kStageV;
-    // Hessian blocks with the mass term added: diagonal blocks (6 entries)
-    // to the stage, off-diagonal blocks (pair index k of (kPairA, kPairB))
-    // to registers
+    // diagonal blocks to the stage, off-diagonal blocks (pair index k of
+    // (kPairA, kPairB)) to registers
     auto emit = [&](int fa, int fb, const T* q) {
       if (fa == fb) {
-        T dq[6];
-#pragma unroll
-        for (int r = 0; r < 6; ++r) dq[r] = q[r];
-        if (kDamp && damp) viscous_block<T>(vis, dN[fa], dN[fb], true, vol, dq);
         T* d = sv + 12 + 6 * fa;
-        d[0] = dq[0] + T(2) * ms;
-        d[1] = dq[1] + T(2) * ms;
-        d[2] = dq[2] + T(2) * ms;
-        d[3] = dq[3];
-        d[4] = dq[4];
-        d[5] = dq[5];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) d[r] = q[r];
         return;
       }
       const int pk = fa == 0 ? fb - 1 : (fa == 1 ? fb + 1 : 5);
-      T o[9];
 #pragma unroll
-      for (int r = 0; r < 9; ++r) o[r] = q[r];
-      if (kDamp && damp) viscous_block<T>(vis, dN[fa], dN[fb], false, vol, o);
-      o[0] += ms;
-      o[4] += ms;
-      o[8] += ms;
-#pragma unroll
-      for (int r = 0; r < 9; ++r) off[9 * pk + r] = o[r];
+      for (int r = 0; r < 9; ++r) off[9 * pk + r] = q[r];
     };
-    // the residual of each vertex: stress, inertia of the consistent mass and
-    // gravity (fem_gravity.h:108-118: res_f -= rho g vol / 4), staged before
-    // the Hessian is formed
-    auto on_res = [&](const T* res_in) {
-      T res[12];
+    // the residual, staged before the Hessian is formed
+    auto on_res = [&](const T* res) {
 #pragma unroll
-      for (int k = 0; k < 12; ++k) res[k] = res_in[k];
-      if (kDamp && damp) viscous_res<T>(vis, dN, vol, res);
-      const T gq = T(0.25) * mass;
-      const T sw[3] = {w[0] + w[3] + w[6] + w[9], w[1] + w[4] + w[7] + w[10], w[2] + w[5] + w[8] + w[11]};
-      const T sd[3] = {wd[0] + wd[3] + wd[6] + wd[9], wd[1] + wd[4] + wd[7] + wd[10], wd[2] + wd[5] + wd[8] + wd[11]};
-      const T gv[3] = {gx, gy, gz};
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          T r = res[3 * a + d] - gv[d] * gq + idt2 * m20 * (w[3 * a + d] + sw[d]);
-          if (mdamp > T(0)) r += mdamp * m20 * (wd[3 * a + d] + sd[d]);
-          sv[3 * a + d] = r;
-        }
+      for (int k = 0; k < 12; ++k) sv[k] = res[k];
     };
-    smith_element<T>(dN, ue, vol, mu, lam, wantH != 0, project != 0, on_res, emit);
+    tet_contrib<T, kDamp>(f, nT, t, in, dt, gx, gy, gz, wantH != 0, project != 0, on_res, emit);
   }
   __syncthreads();
   const uint32_t hd = meta[0];
@@ -376,16 +423,14 @@ __device__ __forceinline__ void tri_q3(int q, T N[3]) {
 // Outputs per triangle, written to their records (zeros when nothing
 // touches): 3 residual rows, 3 diagonal blocks and the 3 blocks of its vertex
 // pairs (symmetric, so no transposition).
+// One boundary triangle tr of environment env against the environment's
+// colliders: rc[9] its 3 vertices' residuals, with wantH hc[36] its 6 vertex
+// pair blocks (hc_block order, each symmetric, 00 11 22 01 02 12).  Returns
+// whether some sample touches (else rc and hc stay zero).
 template <class T>
-__global__ void __launch_bounds__(256) k_contact(MeshView m, AsmRange ar, FieldPtrs<T> f, const T* __restrict__ u,
-                                                 const T* __restrict__ u0, T dt, int wantH, T* __restrict__ Rv,
-                                                 T* __restrict__ Dv, T* __restrict__ Ev) {
-  const int tr = ar.f0 + int(blockIdx.x * blockDim.x + threadIdx.x);
-  if (tr >= ar.f1) return;
-  const int env = m.tri_env[tr];
-  if (!m.env_mask[env]) return;
-  T rc[9];
-  T hc[36];
+__device__ __forceinline__ bool contact_tri(const MeshView& m, const FieldPtrs<T>& f, const T* __restrict__ u,
+                                            const T* __restrict__ u0, T dt, int tr, int env, bool wantH, T rc[9],
+                                            T hc[36]) {
   bool touched = false;  // some quadrature point is in contact
 #pragma unroll
   for (int k = 0; k < 9; ++k) rc[k] = T(0);
@@ -444,6 +489,20 @@ __global__ void __launch_bounds__(256) k_contact(MeshView m, AsmRange ar, FieldP
       }
     }
   }
+  return touched;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_contact(MeshView m, AsmRange ar, FieldPtrs<T> f, const T* __restrict__ u,
+                                                 const T* __restrict__ u0, T dt, int wantH, T* __restrict__ Rv,
+                                                 T* __restrict__ Dv, T* __restrict__ Ev) {
+  const int tr = ar.f0 + int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (tr >= ar.f1) return;
+  const int env = m.tri_env[tr];
+  if (!m.env_mask[env]) return;
+  T rc[9];
+  T hc[36];
+  const bool touched = contact_tri<T>(m, f, u, u0, dt, tr, env, wantH != 0, rc, hc);
   // A triangle out of contact has zero records: they are written only if
   // it is in contact now or its records still hold values from an earlier
   // write (flag bits), so most triangles write nothing; with contact_always
@@ -500,6 +559,30 @@ __device__ __forceinline__ void inv3(const T a[9], T o[9]) {
   o[6] = c02 * id;
   o[7] = c12 * id;
   o[8] = c22 * id;
+}
+
+// The preconditioner block of diagonal block b (row-major) for row `row`:
+// block Jacobi (prec 2) its inverse, Jacobi (1) the reciprocal diagonal,
+// none (0) the identity; 6 planes of nV, symmetric storage 00 11 22 01 02 12.
+template <class T>
+__device__ __forceinline__ void store_precond(const T b[9], int prec, T* __restrict__ dinv, int nV, int row) {
+  T o[9];
+  if (prec == 2) {
+    inv3(b, o);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) o[q] = T(0);
+    if (prec == 1) {
+      o[0] = T(1) / b[0];
+      o[4] = T(1) / b[4];
+      o[8] = T(1) / b[8];
+    } else {
+      o[0] = o[4] = o[8] = T(1);
+    }
+  }
+  const T sym[6] = {o[0], o[4], o[8], o[1], o[2], o[5]};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) dinv[(size_t)q * nV + row] = sym[q];
 }
 
 // Every BSR slot: sum its element and contact contributions, apply Dirichlet
@@ -637,25 +720,8 @@ __global__ void __launch_bounds__(kGatherThreads) k_gather_bsr(MeshView m, AsmRa
 #pragma unroll
   for (int q = 0; q < 9; ++q) vals[(size_t)q * nB + k] = b[q];
   if (!diag) return;
-  // the preconditioner block, symmetric storage 00 11 22 01 02 12
-  T o[9];
-  if (prec == 2) {
-    inv3(b, o);
-  } else {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) o[q] = T(0);
-    if (prec == 1) {
-      o[0] = T(1) / b[0];
-      o[4] = T(1) / b[4];
-      o[8] = T(1) / b[8];
-    } else {
-      o[0] = o[4] = o[8] = T(1);
-    }
-  }
   const int row = d.z & ((1 << kItemRowBits) - 1);
-  const T sym[6] = {o[0], o[4], o[8], o[1], o[2], o[5]};
-#pragma unroll
-  for (int q = 0; q < 6; ++q) dinv[(size_t)q * m.nV + row] = sym[q];
+  store_precond<T>(b, prec, dinv, m.nV, row);
 }
 
 template <class T>
@@ -685,6 +751,183 @@ __global__ void __launch_bounds__(256) k_block_inverse(int nV, const int32_t* __
   const T sym[6] = {o[0], o[4], o[8], o[1], o[2], o[5]};
 #pragma unroll
   for (int k = 0; k < 6; ++k) dinv[(size_t)k * nV + i] = sym[k];
+}
+
+// ------------------------------------------------------------- row blocks
+
+// Fused assembly: one workgroup per row block (sdx_rowblocks.h).  The block's
+// rows, their columns and LDS bases come first; every thread then computes
+// tets of the block (and boundary triangles) and adds each contribution to a
+// row of the block into LDS (float atomics, so the summation order varies
+// from run to run); last, the block writes its rows' BSR values with the
+// Dirichlet rows and columns (mochi_solve.cpp:337-396), the preconditioner
+// blocks and the residual.  LDS: values [slots][9], residual [rows][3],
+// columns [slots], per row its first global slot, LDS base (plus the end)
+// and vertex, and the row of every slot (rows_lds_bytes_host).
+constexpr int kRowBlockThreads = 256;
+static_assert(kBlockRows <= kRowBlockRowsMax && kBlockRows <= kRowBlockThreads,
+              "a row block's rows fit its LDS row arrays and one thread each");
+
+// LDS index of column c in block row r (its columns lie in [base[r], base[r+1]), sorted).
+__device__ __forceinline__ int row_slot(const int* __restrict__ lcol, const int* __restrict__ lbase, int r, int c) {
+  int lo = lbase[r], hi = lbase[r + 1] - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (lcol[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Add a 3x3 block (row-major q, or its transpose) into LDS slot s.
+template <class T>
+__device__ __forceinline__ void lds_add_block(T* __restrict__ lv, int s, const T* q, bool transpose) {
+  T* d = lv + 9 * s;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) atomicAdd(d + 3 * r + c, transpose ? q[3 * c + r] : q[3 * r + c]);
+}
+// The same for a symmetric block in 00 11 22 01 02 12 order.
+template <class T>
+__device__ __forceinline__ void lds_add_sym(T* __restrict__ lv, int s, const T* q) {
+  T* d = lv + 9 * s;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) atomicAdd(d + 3 * r + c, q[sym_entry(r, c)]);
+}
+
+template <class T, class A, bool kDamp>
+__global__ void __launch_bounds__(kRowBlockThreads) k_assemble_rows(MeshView m, RowBlockView rb, FieldPtrs<T> f,
+                                                                    const T* __restrict__ u,
+                                                                    const T* __restrict__ u0,
+                                                                    const T* __restrict__ v0, T dt, T gx, T gy,
+                                                                    T gz, int project, int prec,
+                                                                    T* __restrict__ vals, T* __restrict__ dinv,
+                                                                    T* __restrict__ res, A* acc) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int env = rb.env[b];
+  if (!m.env_mask[env]) return;  // uniform over the workgroup
+  const int ms = rb.max_slots;
+  T* lv = reinterpret_cast<T*>(sdx_dyn_lds);
+  T* lres = lv + size_t(9) * ms;
+  int* lcol = reinterpret_cast<int*>(lres + 3 * kRowBlockRowsMax);
+  int* lrow_g = lcol + ms;
+  int* lbase = lrow_g + kRowBlockRowsMax;  // kRowBlockRowsMax + 1 entries
+  int* lrow_v = lbase + kRowBlockRowsMax + 1;
+  uint8_t* lslot_row = reinterpret_cast<uint8_t*>(lrow_v + kRowBlockRowsMax);
+  const int tid = int(threadIdx.x);
+  const int r0 = rb.row0[b], nr = rb.row0[b + 1] - r0;
+  const int e0 = rb.tet0[b], ne = rb.tet0[b + 1] - e0;
+  const int c0 = rb.tri0[b], nc = rb.tri0[b + 1] - c0;
+  // rows: first global slot, LDS base, vertex; columns and slot rows
+  if (tid < nr) {
+    const int v = rb.rows[r0 + tid];
+    const int s0 = m.row_ptr[v], len = m.row_ptr[v + 1] - s0;
+    const int base = rb.rbase[r0 + tid];
+    lrow_g[tid] = s0;
+    lbase[tid] = base;
+    lrow_v[tid] = v;
+    if (tid == nr - 1) lbase[nr] = base + len;
+    for (int k = 0; k < len; ++k) {
+      lcol[base + k] = m.col[s0 + k];
+      lslot_row[base + k] = uint8_t(tid);
+    }
+  }
+  __syncthreads();
+  const int nslots = lbase[nr];
+  for (int j = tid; j < 9 * nslots; j += kRowBlockThreads) lv[j] = T(0);
+  for (int j = tid; j < 3 * nr; j += kRowBlockThreads) lres[j] = T(0);
+  __syncthreads();
+  // tets
+  const int nT = m.nT;
+  for (int k = tid; k < ne; k += kRowBlockThreads) {
+    const int2 te = rb.tets[e0 + k];
+    const int t = te.x;
+    const uint32_t lr = uint32_t(te.y);
+    const int4 tv = reinterpret_cast<const int4*>(m.tets)[t];
+    const int nd[4] = {tv.x, tv.y, tv.z, tv.w};
+    TetIn<T> in;
+    load_tet<T>(f, nT, t, nd, u, u0, v0, dt, in);
+    auto own = [&](int a) { return int((lr >> (8 * a)) & kNotInBlock); };
+    auto on_res = [&](const T* r) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int la = own(a);
+        if (la == int(kNotInBlock)) continue;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) atomicAdd(lres + 3 * la + d, r[3 * a + d]);
+      }
+    };
+    auto emit = [&](int fa, int fb, const T* q) {
+      const int la = own(fa);
+      if (fa == fb) {
+        if (la != int(kNotInBlock)) lds_add_sym<T>(lv, row_slot(lcol, lbase, la, nd[fa]), q);
+        return;
+      }
+      if (la != int(kNotInBlock)) lds_add_block<T>(lv, row_slot(lcol, lbase, la, nd[fb]), q, false);
+      const int lb = own(fb);
+      if (lb != int(kNotInBlock)) lds_add_block<T>(lv, row_slot(lcol, lbase, lb, nd[fa]), q, true);
+    };
+    tet_contrib<T, kDamp>(f, nT, t, in, dt, gx, gy, gz, true, project != 0, on_res, emit);
+  }
+  // boundary triangles in contact
+  for (int k = tid; k < nc; k += kRowBlockThreads) {
+    const int2 fe = rb.tris[c0 + k];
+    const int tr = fe.x;
+    const uint32_t lr = uint32_t(fe.y);
+    T rc[9], hc[36];
+    if (!contact_tri<T>(m, f, u, u0, dt, tr, env, true, rc, hc)) continue;
+    const int nd[3] = {m.tris[3 * tr], m.tris[3 * tr + 1], m.tris[3 * tr + 2]};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int li = int((lr >> (8 * i)) & kNotInBlock);
+      if (li == int(kNotInBlock)) continue;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) atomicAdd(lres + 3 * li + d, rc[3 * i + d]);
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        lds_add_sym<T>(lv, row_slot(lcol, lbase, li, nd[j]), hc + 6 * hc_block(i < j ? i : j, i < j ? j : i));
+    }
+  }
+  __syncthreads();
+  // the rows' BSR values, Dirichlet rows and columns, preconditioner blocks
+  const int nB = m.nB;
+  for (int j = tid; j < nslots; j += kRowBlockThreads) {
+    const int r = lslot_row[j];
+    const int v = lrow_v[r], c = lcol[j];
+    const int gs = lrow_g[r] + (j - lbase[r]);
+    T q[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) q[e] = lv[9 * j + e];
+    const bool diag = c == v;
+    const uint8_t* pv = m.pinned + 3 * v;
+    const uint8_t* pc = m.pinned + 3 * c;
+    const bool rp[3] = {pv[0] != 0, pv[1] != 0, pv[2] != 0}, cpn[3] = {pc[0] != 0, pc[1] != 0, pc[2] != 0};
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+        if (rp[rr] || cpn[cc]) q[3 * rr + cc] = (diag && rr == cc) ? T(1) : T(0);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) vals[(size_t)e * nB + gs] = q[e];
+    if (diag) store_precond<T>(q, prec, dinv, m.nV, v);
+  }
+  // the residual, Dirichlet rows zeroed, and ||res||^2
+  A part[1] = {A(0)};
+  if (tid < nr) {
+    const int v = lrow_v[tid];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T x = m.pinned[3 * v + d] ? T(0) : lres[3 * tid + d];
+      res[3 * v + d] = x;
+      part[0] += A(x) * A(x);
+    }
+  }
+  acc_add<A, 1>(part, acc_ptr(acc, 0, m.nEnv, env), b);
 }
 
 // One thread per vertex: residual = sum of its element and contact records,
@@ -1925,6 +2168,28 @@ void launch_gather_bsr(hipStream_t s, const MeshView& m, const AsmRange& r, cons
 }
 
 template <class T, class A>
+void launch_assemble_rows(hipStream_t s, const MeshView& m, const RowBlockView& rb, const FieldPtrs<T>& f,
+                          const T* u, const T* u0, const T* v0, double dt, const double grav[3], int project,
+                          int prec, T* vals, T* dinv, T* res, A* acc, bool damp, const KernelEvents* ev) {
+  if (rb.n <= 0) return;
+  const size_t lds = rows_lds_bytes_host<T>(rb.max_slots);
+  if (lds > size_t(65536)) throw std::invalid_argument("row blocks too large for the fused assembly's LDS");
+  auto go = [&](auto kern) {
+    if (ev)
+      hipExtLaunchKernelGGL(kern, dim3(unsigned(rb.n)), dim3(kRowBlockThreads), lds, s, ev->start, ev->stop, 0, m,
+                            rb, f, u, u0, v0, T(dt), T(grav[0]), T(grav[1]), T(grav[2]), project, prec, vals, dinv,
+                            res, acc);
+    else
+      hipLaunchKernelGGL(kern, dim3(unsigned(rb.n)), dim3(kRowBlockThreads), lds, s, m, rb, f, u, u0, v0, T(dt),
+                         T(grav[0]), T(grav[1]), T(grav[2]), project, prec, vals, dinv, res, acc);
+  };
+  if (damp)
+    go(k_assemble_rows<T, A, true>);
+  else
+    go(k_assemble_rows<T, A, false>);
+}
+
+template <class T, class A>
 void launch_gather_res(hipStream_t s, const MeshView& m, const AsmRange& r, const Tiles& tv, const T* Rv, T* res,
                        A* acc) {
   if (r.tile1 <= r.tile0) return;
@@ -2199,6 +2464,9 @@ void launch_energy(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, cons
   template void launch_cg_reset<T>(hipStream_t, int, const int32_t*, EnvCG*, T*);
 
 #define SDX_INST_TA(T, A)                                                                                         \
+  template void launch_assemble_rows<T, A>(hipStream_t, const MeshView&, const RowBlockView&, const FieldPtrs<T>&, \
+                                           const T*, const T*, const T*, double, const double*, int, int, T*, T*, \
+                                           T*, A*, bool, const KernelEvents*);                                    \
   template void launch_gather_res<T, A>(hipStream_t, const MeshView&, const AsmRange&, const Tiles&, const T*,  \
                                         T*, A*);                                                                \
   template void launch_cg_init<T, A>(hipStream_t, const Tiles&, const CgConfig&, const T*, const int32_t*,       \

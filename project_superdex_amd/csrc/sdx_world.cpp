@@ -30,6 +30,7 @@
 
 #include "sdx_world.h"
 #include "sdx_capture.h"
+#include "sdx_rowblocks.h"
 
 namespace sdx {
 
@@ -122,6 +123,7 @@ void World<T>::release_device() {
   drop_graphs();
   tuned_ = false;
   contact_ready_ = false;
+  rb_ready_ = false;
   for (auto* b : bufs_) b->free();
   finalized_ = false;
 }
@@ -594,6 +596,21 @@ void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH, int 
     launch_fill<double>(stream_, set0, 0.0, acc_.p);
   else
     launch_fill<T>(stream_, set0, T(0), reinterpret_cast<T*>(acc_.p));
+  if (wantH && sched_.fused_asm && plan_row_blocks()) {  // residual, matrix and preconditioner in one kernel
+    if (dd)
+      launch_assemble_rows<T, double>(stream_, mesh_, rbv_, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, project,
+                                      sp.linear.preconditioner, vals_.p, dinv_.p, res_.p, acc_.p, stiff_damp_,
+                                      sampled_events(2));
+    else
+      launch_assemble_rows<T, T>(stream_, mesh_, rbv_, fields_, u_.p, u0_.p, v0_.p, dt, sp.gravity, project,
+                                 sp.linear.preconditioner, vals_.p, dinv_.p, res_.p, reinterpret_cast<T*>(acc_.p),
+                                 stiff_damp_, sampled_events(2));
+    if (dd)
+      launch_acc_finalize<double>(stream_, nEnv_, 1, acc_.p, env_sq_.p);
+    else
+      launch_acc_finalize<T>(stream_, nEnv_, 1, reinterpret_cast<T*>(acc_.p), env_sq_.p);
+    return;
+  }
   const bool shared = asm_plan_.size() > 1;
   for (const AsmGroup& g : asm_plan_) {
     AsmRange r = g.r;
@@ -635,6 +652,62 @@ void World<T>::assemble(double dt, const sdx_solver_params& sp, bool wantH, int 
     launch_acc_finalize<double>(stream_, nEnv_, 1, acc_.p, env_sq_.p);
   else
     launch_acc_finalize<T>(stream_, nEnv_, 1, reinterpret_cast<T*>(acc_.p), env_sq_.p);
+}
+
+// Row blocks of every actor (sdx_rowblocks.h), uploaded once per topology:
+// blocks of at most 64 vertices and 1024 BSR slots (512 in FP64), so a
+// block's LDS stays within 64 KB.  Returns false (the assembly then takes
+// the record path) when some vertex couples to more vertices than one block
+// holds.
+template <class T>
+bool World<T>::plan_row_blocks() {
+  if (rb_ready_) return rbv_.n > 0;
+  const int cap = sizeof(T) == 8 ? kBlockSlots / 2 : kBlockSlots;
+  rb_ready_ = true;
+  rbv_ = RowBlockView{};
+  for (int v = 0; v < nV_; ++v)
+    if (h_row_ptr_[size_t(v) + 1] - h_row_ptr_[size_t(v)] > cap) return false;
+  RowBlocks rb(cap);
+  for (int e = 0; e < nEnv_; ++e) {
+    const HostActor& A = actors_[size_t(e)];
+    const int v0 = env_v0_[size_t(e)];
+    std::vector<int32_t> row_len(static_cast<size_t>(A.nV));
+    for (int i = 0; i < A.nV; ++i) row_len[size_t(i)] = h_row_ptr_[size_t(v0 + i) + 1] - h_row_ptr_[size_t(v0 + i)];
+    std::vector<int32_t> atris;
+    BuildBoundary(A.X, A.tets, atris);
+    if (int(atris.size() / 3) != env_f0_[size_t(e) + 1] - env_f0_[size_t(e)])
+      throw std::logic_error("boundary triangles of an actor changed since its upload");
+    rb.add(e, A.X, A.nV, A.tets, atris, row_len, v0, env_t0_[size_t(e)], env_f0_[size_t(e)]);
+  }
+  if (rows_lds_bytes_host<T>(rb.max_slots) > size_t(65536))
+    throw std::invalid_argument("row blocks too large for the fused assembly's LDS");
+  auto pairs = [](const std::vector<int32_t>& a, const std::vector<int32_t>& b) {
+    std::vector<int32_t> out(2 * a.size());
+    for (size_t k = 0; k < a.size(); ++k) {
+      out[2 * k] = a[k];
+      out[2 * k + 1] = b[k];
+    }
+    return out;
+  };
+  d_rb_row0_.upload(rb.row0);
+  d_rb_tet0_.upload(rb.tet0);
+  d_rb_tri0_.upload(rb.tri0);
+  d_rb_env_.upload(rb.env);
+  d_rb_rows_.upload(rb.rows);
+  d_rb_rbase_.upload(rb.rbase);
+  d_rb_tets_.upload(pairs(rb.tets, rb.tet_rows));
+  d_rb_tris_.upload(rb.tris.empty() ? std::vector<int32_t>(2, 0) : pairs(rb.tris, rb.tri_rows));
+  rbv_.n = rb.count();
+  rbv_.max_slots = rb.max_slots;
+  rbv_.row0 = d_rb_row0_.p;
+  rbv_.tet0 = d_rb_tet0_.p;
+  rbv_.tri0 = d_rb_tri0_.p;
+  rbv_.env = d_rb_env_.p;
+  rbv_.rows = d_rb_rows_.p;
+  rbv_.rbase = d_rb_rbase_.p;
+  rbv_.tets = reinterpret_cast<const int2*>(d_rb_tets_.p);
+  rbv_.tris = reinterpret_cast<const int2*>(d_rb_tris_.p);
+  return rbv_.n > 0;
 }
 
 // DResidual of the environments in mask, in up to two assemblies: the
@@ -2110,9 +2183,9 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
         }
     }
   }
-  if (nEnv_ > 1) {  // assembly groups: one (every record kept), the automatic count, half and twice it
+  if (nEnv_ > 1 && !cur.fused_asm) {  // assembly groups: one (every record kept), the automatic count, half and twice it
     const int a = asm_groups(cur);
-    std::vector<int> cand = {1, std::max(1, a / 2), asm_groups(sdx_schedule{0, 0, 0, 0, 0, 0, 0, 0, 0}), 2 * a};
+    std::vector<int> cand = {1, std::max(1, a / 2), asm_groups(sdx_schedule{0, 0, 0, 0, 0, 0, 0, 0, 0, 0}), 2 * a};
     for (int ag : cand)
       if (ag <= nEnv_ && ag != asm_groups(cur)) {
         sdx_schedule c = cur;

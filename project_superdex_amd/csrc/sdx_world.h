@@ -313,7 +313,7 @@ class WorldBase {
     flush_timers();
     return {double(timers_[size_t(k)].launches), timers_[size_t(k)].ms};
   }
-  sdx_schedule sched_{8, 1, 16, 128, 1, 0, 16, 0, 1};
+  sdx_schedule sched_{8, 1, 16, 128, 1, 0, 16, 0, 1, 0};
   bool tuned_ = false;  // the schedule was autotuned for the current topology
 
  protected:
@@ -532,6 +532,10 @@ class World final : public WorldBase {
   };
   std::vector<AsmGroup> asm_plan_;
   int asm_plan_n_ = 0;
+  // row blocks of the fused assembly (sdx_schedule.fused_asm), built on first use
+  bool rb_ready_ = false;
+  RowBlockView rbv_{};
+  bool plan_row_blocks();
   static constexpr double kAsmGroupBytes = 96e6;
   int asm_groups(const sdx_schedule& sc) const;
   void plan_assembly(int n);
@@ -581,6 +585,14 @@ class World final : public WorldBase {
   MeshView mesh_{};
   FieldPtrs<T> fields_{};
 #define SDX_BUF(type, name) DBuf<type> name{bufs_, &stream_};
+  SDX_BUF(int32_t, d_rb_row0_)
+  SDX_BUF(int32_t, d_rb_tet0_)
+  SDX_BUF(int32_t, d_rb_tri0_)
+  SDX_BUF(int32_t, d_rb_env_)
+  SDX_BUF(int32_t, d_rb_rows_)
+  SDX_BUF(int32_t, d_rb_rbase_)
+  SDX_BUF(int32_t, d_rb_tets_)  // (tet, local rows) pairs
+  SDX_BUF(int32_t, d_rb_tris_)  // (triangle, local rows) pairs
   SDX_BUF(int32_t, d_tets_)
   SDX_BUF(int32_t, d_tet_env_)
   SDX_BUF(int32_t, d_tris_)

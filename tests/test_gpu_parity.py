@@ -553,7 +553,7 @@ def test_assembly_groups_match_oracle(oracle, asm_groups):
     spec = mesh.batched_indent_scene(num_envs=5, seed=3, n_min=3, n_max=6, steps=6, speed=0.2)
     p = _abi.pinned_solver_params(2)
     sc, rigid = gs.build_scene(spec)
-    sc.set_schedule(asm_groups=asm_groups)
+    sc.set_schedule(asm_groups=asm_groups, fused_asm=0)  # the record path, where groups apply
     assert sc.schedule()["asm_groups"] == asm_groups
     for s in range(6):
         for r, c in zip(rigid, spec.colliders):

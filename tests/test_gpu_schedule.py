@@ -67,10 +67,11 @@ def state(sc):
     dict(cg_groups=4, group_iters=64, cg_burst=8, spmv_lanes=8),  # every iteration grouped
     dict(fused_tail=1),  # tail bursts as one cooperative launch each
     dict(fused_tail=1, cg_burst=16, spmv_lanes=16),
-    dict(asm_groups=1),  # every environment's records kept between assemblies
-    dict(asm_groups=2),  # assembly groups share the record buffers
-    dict(asm_groups=6, use_graphs=0),  # one environment per assembly group
-    dict(asm_groups=4, cg_groups=3, group_iters=8, cg_burst=4),
+    dict(fused_asm=0, asm_groups=1),  # the record path: every environment's records kept
+    dict(fused_asm=0, asm_groups=2),  # assembly groups share the record buffers
+    dict(fused_asm=0, asm_groups=6, use_graphs=0),  # one environment per assembly group
+    dict(fused_asm=0, asm_groups=4, cg_groups=3, group_iters=8, cg_burst=4),
+    dict(fused_asm=1, asm_groups=3),  # fused assembly (groups apply to residual-only assemblies)
 ])
 def test_schedules_step_to_the_same_state(sched):
     # one Newton iteration per step: a second one starts from a residual at

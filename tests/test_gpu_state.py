@@ -3,10 +3,12 @@ IsEqualState / CaptureStateToBytes / RestoreStateFromBytes,
 mochi_physics/include/mochi_physics/cpp_api/mochi_scene.h:79-91).
 
 - the steps after a restore repeat the steps after the capture (same
-  collider targets): displacements and velocities within 1e-4 of the largest
-  displacement and velocity (the CG's FP64 dot products add their partial
-  sums in arrival order, so two runs of one solve may differ in the last
-  bits), linear iteration counts within 2, learned warm-start decisions
+  collider targets): displacements and velocities within 1e-3 of the largest
+  displacement and velocity (the assembly sums in LDS with float atomics and
+  the CG's FP64 dot products add their partial sums in arrival order, so two
+  runs of one solve differ in the last bits, which solves to 1e-6 let grow
+  over the steps), linear iteration counts within 3 or 20% per solve and 5%
+  over all solves, learned warm-start decisions
   within one environment;
 - capture -> restore -> capture gives an equal state, states a step apart are
   not equal, a handle restored with release_immediately is gone, and every
@@ -62,7 +64,7 @@ def state(sc):
     return [(a.displacements(), a.velocities()) for a in sc.actors]
 
 
-def assert_close(sa, sb, tol=1e-4):
+def assert_close(sa, sb, tol=1e-3):
     umax = max(np.abs(u).max() for u, _ in sa)
     vmax = max(np.abs(v).max() for _, v in sa)
     for (ua, va), (ub, vb) in zip(sa, sb):
@@ -74,7 +76,13 @@ def assert_same_solves(st_a, st_b):
     for sa, sb in zip(st_a, st_b):
         for ea, eb in zip(sa, sb):
             assert eb["linear_convergence"] == _abi.CONV_CONVERGED, eb
-            assert abs(ea["linear_iters"] - eb["linear_iters"]) <= 2, (ea, eb)
+            # run-to-run rounding moves a CG stop by an iteration or two, and a warm start whose
+            # guess sits near its gate by a few more
+            assert abs(ea["linear_iters"] - eb["linear_iters"]) <= max(3, 0.2 * ea["linear_iters"]), (ea, eb)
+    # over all solves the counts agree closely (a lost warm-start history costs 20-30% more)
+    ta = sum(e["linear_iters"] for s in st_a for e in s)
+    tb = sum(e["linear_iters"] for s in st_b for e in s)
+    assert abs(ta - tb) <= 0.05 * ta, (ta, tb)
 
 
 def test_steps_after_restore_repeat_the_steps_after_capture():
