@@ -16,11 +16,56 @@ namespace sdx {
 // almost nothing.  Consumers (the next kernel, or the host) sum the slots in
 // a fixed order; kernel boundaries order producers and consumers.
 
+// Cross-lane sums on the VALU (DPP lane moves) rather than through the LDS
+// crossbar (ds_bpermute).  quad_perm [1,0,3,2] and [2,3,0,1] pair each lane
+// with its xor-1 and xor-2 partner, row_half_mirror reverses every group of 8
+// lanes and row_mirror every row of 16, so after the four steps each lane of
+// a row holds the sum of the row (every lane adds the same two values at each
+// step, so all lanes of a group end bit-identical).  Every lane of the wave
+// must be active where these are called.
+enum : int { kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140 };
+
+template <int kCtrl>
+__device__ __forceinline__ int dpp_move(int v) {
+  return __builtin_amdgcn_mov_dpp(v, kCtrl, 0xF, 0xF, false);
+}
+template <int kCtrl>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(dpp_move<kCtrl>(__float_as_int(v)));
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_move<kCtrl>(int(b)), hi = dpp_move<kCtrl>(int(b >> 32));
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// Sum over each aligned group of L lanes (L = 2, 4, 8 or 16), in every lane of the group.
+template <int L, class A>
+__device__ __forceinline__ A group_sum(A v) {
+  static_assert(L == 2 || L == 4 || L == 8 || L == 16, "DPP sums groups of 2 to 16 lanes");
+  v += dpp_move<kDppXor1>(v);
+  if constexpr (L >= 4) v += dpp_move<kDppXor2>(v);
+  if constexpr (L >= 8) v += dpp_move<kDppHalfMirror>(v);
+  if constexpr (L >= 16) v += dpp_move<kDppMirror>(v);
+  return v;
+}
+
+__device__ __forceinline__ float lane_value(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b), lane), hi = __builtin_amdgcn_readlane(int(b >> 32), lane);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// Sum over the 64 lanes of the wave, in every lane: the 4 row sums read from
+// lanes 0, 16, 32 and 48.
 template <class A>
 __device__ __forceinline__ A wave_sum(A v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v = group_sum<16>(v);
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 
 // Sum NV values over the block; result valid in thread 0.

@@ -1225,9 +1225,8 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(Tiles tv, CgConfig cfg,
 
 // Finish an accumulator read started with one slot load per lane.
 __device__ __forceinline__ double slot_reduce(double v) {
-#pragma unroll
-  for (int off = kSlots / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kSlots);
-  return v;
+  static_assert(kSlots == 16, "one accumulator slot per lane of a row of 16");
+  return group_sum<kSlots>(v);
 }
 
 
@@ -1328,16 +1327,13 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
     return;
   }
   if (lead) acc_zero(acc_ptr(acc, kSetR + ((k + 1) & 1), nEnv, e));
-#pragma unroll
-  for (int off = L / 2; off >= 1; off >>= 1) {
-    a0 += __shfl_xor(a0, off, L);
-    a1 += __shfl_xor(a1, off, L);
-    a2 += __shfl_xor(a2, off, L);
-    if (kRefresh) {
-      q0 += __shfl_xor(q0, off, L);
-      q1 += __shfl_xor(q1, off, L);
-      q2 += __shfl_xor(q2, off, L);
-    }
+  a0 = group_sum<L>(a0);
+  a1 = group_sum<L>(a1);
+  a2 = group_sum<L>(a2);
+  if (kRefresh) {
+    q0 = group_sum<L>(q0);
+    q1 = group_sum<L>(q1);
+    q2 = group_sum<L>(q2);
   }
   A part[1] = {A(0)};
   if (valid && lane == 0) {
@@ -1706,12 +1702,9 @@ __global__ void __launch_bounds__(kSpmvThreads) k_residual64(Tiles ta, int nEnv,
             double(vals[8 * (size_t)nB + kk]) * x2;
     }
   }
-#pragma unroll
-  for (int off = L / 2; off >= 1; off >>= 1) {
-    q0 += __shfl_xor(q0, off, L);
-    q1 += __shfl_xor(q1, off, L);
-    q2 += __shfl_xor(q2, off, L);
-  }
+  q0 = group_sum<L>(q0);
+  q1 = group_sum<L>(q1);
+  q2 = group_sum<L>(q2);
   double part[4] = {0.0, 0.0, 0.0, 0.0};
   if (valid && lane == 0) {
     const double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
@@ -1766,11 +1759,9 @@ __global__ void __launch_bounds__(kSpmvThreads) k_residual64_multi(Tiles ta, int
     }
   }
 #pragma unroll
-  for (int off = L / 2; off >= 1; off >>= 1)
+  for (int q = 0; q < 3; ++q)
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
-#pragma unroll
-      for (int r = 0; r < 3; ++r) p[q][r] += __shfl_xor(p[q][r], off, L);
+    for (int r = 0; r < 3; ++r) p[q][r] = group_sum<L>(p[q][r]);
   double part[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (valid && lane == 0) {
     const double bb[3] = {double(b[3 * i]), double(b[3 * i + 1]), double(b[3 * i + 2])};
