@@ -351,6 +351,9 @@ sdx_status sdx_get_rigid_position(sdx_world* world, int32_t collider, double out
  * actors are moved one by one); a batch of environments sets every target of
  * a step at once.  All indices are checked before any collider moves. */
 sdx_status sdx_set_rigid_positions(sdx_world* world, int32_t n, const int32_t* colliders, const double* positions);
+/* The targets of n colliders in one call: positions[3 k .. 3 k + 2] of collider colliders[k]
+ * (after a state restore, every target at once).  No counterpart in the reference. */
+sdx_status sdx_get_rigid_positions(sdx_world* world, int32_t n, const int32_t* colliders, double* positions);
 
 /* Scene::Step(dt): one backward-Euler stage per actor, each actor its own
  * island and Newton solve.  stats: n_actor entries (may be NULL). */

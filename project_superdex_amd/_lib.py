@@ -41,6 +41,7 @@ SIGNATURES = {
     "sdx_add_rigid_sphere": (C.c_int32, [_vp, _dp, C.c_double, C.c_int32, _ip]),
     "sdx_set_rigid_position": (C.c_int32, [_vp, C.c_int32, _dp]),
     "sdx_get_rigid_position": (C.c_int32, [_vp, C.c_int32, _dp]),
+    "sdx_get_rigid_positions": (C.c_int32, [_vp, C.c_int32, _ip, _dp]),
     "sdx_set_rigid_positions": (C.c_int32, [_vp, C.c_int32, _ip, _dp]),
     "sdx_step": (C.c_int32, [_vp, C.c_double, C.POINTER(_abi.SolverParams), C.POINTER(_abi.StepStats), C.c_int32]),
     "sdx_get_displacements": (C.c_int32, [_vp, C.c_int32, _dp, C.c_int32]),

@@ -294,6 +294,17 @@ sdx_status sdx_get_rigid_position(sdx_world* world, int32_t collider, double out
   });
 }
 
+sdx_status sdx_get_rigid_positions(sdx_world* world, int32_t n, const int32_t* colliders, double* positions) {
+  SDX_REQUIRE(world && n >= 0 && (n == 0 || (colliders && positions)), "NULL argument");
+  return guarded([&]() -> sdx_status {
+    for (int32_t k = 0; k < n; ++k) {
+      const sdx::HostCollider& c = world->impl->collider_view(colliders[k]);
+      for (int d = 0; d < 3; ++d) positions[3 * size_t(k) + size_t(d)] = c.p[d];
+    }
+    return SDX_OK;
+  });
+}
+
 sdx_status sdx_set_rigid_positions(sdx_world* world, int32_t n, const int32_t* colliders, const double* positions) {
   SDX_REQUIRE(world && n >= 0 && (n == 0 || (colliders && positions)), "NULL argument");
   return guarded([&]() -> sdx_status {

@@ -67,23 +67,24 @@ Known lookup(uint64_t id) {
 
 class Writer {
  public:
-  explicit Writer(std::vector<uint8_t>& out) : out_(out) {}
+  explicit Writer(uint8_t* out) : out_(out) {}
   void bytes(const void* p, size_t n) {
-    const auto* b = static_cast<const uint8_t*>(p);
-    out_.insert(out_.end(), b, b + n);
+    if (n) std::memcpy(out_ + at_, p, n);
+    at_ += n;
   }
   template <class V>
   void pod(const V& v) {
     bytes(&v, sizeof(V));
   }
-  size_t pos() const { return out_.size(); }
+  size_t pos() const { return at_; }
   template <class V>
   void pod_at(size_t at, const V& v) {
-    std::memcpy(out_.data() + at, &v, sizeof(V));
+    std::memcpy(out_ + at, &v, sizeof(V));
   }
 
  private:
-  std::vector<uint8_t>& out_;
+  uint8_t* out_;
+  size_t at_ = 0;
 };
 
 class Reader {
@@ -282,9 +283,18 @@ uint64_t fnv1a(const void* p, size_t n, uint64_t h) {
   return h;
 }
 
-std::vector<uint8_t> encode(const StateParts& s) {
+size_t encoded_size(const StateParts& s) {
+  size_t n = sizeof(CaptureHeader) + sizeof(ComponentHeader) + sizeof(s.gate) + 4;
+  if (!s.actor_dofs.empty()) {
+    n += 3 * sizeof(ComponentHeader) + 3 * 8 * s.actor_dofs.size();
+    for (int64_t d : s.actor_dofs) n += size_t(2 + kHistPlanes) * size_t(d) * s.real_bytes;
+  }
+  if (!s.colliders.empty()) n += sizeof(ComponentHeader) + s.colliders.size() * sizeof(double);
+  return n;
+}
+
+void encode_into(const StateParts& s, uint8_t* out) {
   if (s.real_bytes != 4 && s.real_bytes != 8) throw std::invalid_argument("reals are 4 or 8 bytes");
-  std::vector<uint8_t> out;
   Writer w(out);
   CaptureHeader hd{};
   std::memcpy(hd.label, kLabel, 4);
@@ -318,6 +328,12 @@ std::vector<uint8_t> encode(const StateParts& s) {
   w.bytes(kFooter, 4);
   hd.data_size = uint64_t(w.pos() - start);
   w.pod_at(0, hd);
+  if (w.pos() != encoded_size(s)) throw std::logic_error("state encoding size mismatch");
+}
+
+std::vector<uint8_t> encode(const StateParts& s) {
+  std::vector<uint8_t> out(encoded_size(s));
+  encode_into(s, out.data());
   return out;
 }
 

@@ -52,7 +52,9 @@ struct StateParts {
 
 uint64_t fnv1a(const void* p, size_t n, uint64_t h = 1469598103934665603ull);
 
-// The byte form of a state.
+// The byte form of a state: its size, and the bytes written to out.
+size_t encoded_size(const StateParts& s);
+void encode_into(const StateParts& s, uint8_t* out);
 std::vector<uint8_t> encode(const StateParts& s);
 
 // Parse a byte form.  Throws std::invalid_argument when the bytes are not a

@@ -2393,25 +2393,21 @@ void World<T>::capture_bytes(uint8_t* out, size_t cap) {
   sp.collider_hash = collider_hash(colliders_);
   const size_t n3 = size_t(3) * size_t(nV_);
   for (size_t a = 0; a < actors_.size(); ++a) sp.actor_dofs.push_back(3 * int64_t(actors_[a].nV));
-  std::vector<T> h(n3);
-  auto bytes_of = [](const T* p, size_t n) {
-    const auto* b = reinterpret_cast<const uint8_t*>(p);
-    return std::vector<uint8_t>(b, b + n * sizeof(T));
-  };
-  u_.download(h);
-  sp.u = bytes_of(h.data(), n3);
-  v_.download(h);
-  sp.v = bytes_of(h.data(), n3);
-  // history planes [plane][3 nV] -> per actor [plane][3 nV_a]
-  std::vector<T> hh(size_t(capture::kHistPlanes) * n3), ha(hh.size());
-  hist_.download(hh);
+  // u and v straight into the state's arrays; the history planes [plane][3 nV]
+  // regrouped per actor [plane][3 nV_a]
+  sp.u.resize(n3 * sizeof(T));
+  u_.download_raw(sp.u.data(), n3);
+  sp.v.resize(n3 * sizeof(T));
+  v_.download_raw(sp.v.data(), n3);
+  std::vector<T> hh(size_t(capture::kHistPlanes) * n3);
+  hist_.download_raw(hh.data(), hh.size());
+  sp.hist.resize(hh.size() * sizeof(T));
   size_t o = 0;
   for (size_t a = 0; a < actors_.size(); ++a) {
     const size_t d = size_t(3 * actors_[a].nV), v0 = size_t(3 * env_v0_[a]);
     for (int p = 0; p < capture::kHistPlanes; ++p, o += d)
-      std::memcpy(ha.data() + o, hh.data() + size_t(p) * n3 + v0, d * sizeof(T));
+      std::memcpy(sp.hist.data() + o * sizeof(T), hh.data() + size_t(p) * n3 + v0, d * sizeof(T));
   }
-  sp.hist = bytes_of(ha.data(), ha.size());
   for (const HostCollider& c : colliders_) {
     sp.colliders.insert(sp.colliders.end(), c.p, c.p + 3);
     sp.colliders.insert(sp.colliders.end(), c.p0, c.p0 + 3);
@@ -2419,9 +2415,8 @@ void World<T>::capture_bytes(uint8_t* out, size_t cap) {
   std::memcpy(sp.gate, wg_.skip, sizeof(wg_.skip));
   std::memcpy(sp.gate + kNewtonHist, wg_.seen, sizeof(wg_.seen));
   std::memcpy(sp.gate + 2 * kNewtonHist, wg_.net_skip, sizeof(wg_.net_skip));
-  const std::vector<uint8_t> b = capture::encode(sp);
-  if (b.size() != need) throw std::logic_error("state size does not match its encoding");
-  std::memcpy(out, b.data(), b.size());
+  if (capture::encoded_size(sp) != need) throw std::logic_error("state size does not match its encoding");
+  capture::encode_into(sp, out);
 }
 
 // Restore the components of a byte form that `excluded` (SDX_STATE_* bits)
@@ -2455,15 +2450,8 @@ void World<T>::restore_bytes(const uint8_t* in, size_t n, uint32_t excluded) {
     }
   if (has_actors) {
     // one-off transfers: plain uploads, so the pinned staging ring keeps its size
-    std::vector<T> h(n3);
-    if (!(excluded & capture::kDisplacements)) {
-      std::memcpy(h.data(), sp.u.data(), n3 * sizeof(T));
-      u_.upload(h);
-    }
-    if (!(excluded & capture::kVelocities)) {
-      std::memcpy(h.data(), sp.v.data(), n3 * sizeof(T));
-      v_.upload(h);
-    }
+    if (!(excluded & capture::kDisplacements)) u_.upload_raw(sp.u.data(), n3);
+    if (!(excluded & capture::kVelocities)) v_.upload_raw(sp.v.data(), n3);
     if (!(excluded & capture::kWarmStart)) {
       std::vector<T> hh(size_t(capture::kHistPlanes) * n3);
       size_t o = 0;
@@ -2472,7 +2460,7 @@ void World<T>::restore_bytes(const uint8_t* in, size_t n, uint32_t excluded) {
         for (int p = 0; p < capture::kHistPlanes; ++p, o += d)
           std::memcpy(hh.data() + size_t(p) * n3 + v0, sp.hist.data() + o * sizeof(T), d * sizeof(T));
       }
-      hist_.upload(hh);
+      hist_.upload_raw(hh.data(), hh.size());
     }
   }
   if (!(excluded & capture::kWarmStart)) {

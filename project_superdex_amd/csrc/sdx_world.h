@@ -89,6 +89,17 @@ struct DBuf : DBufBase {
     if (!h.empty())
       hip_check(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
   }
+  // Raw transfers of the first n values (n <= the buffer's size), waited for.
+  void download_raw(void* dst, size_t n) const {
+    if (n > this->n) throw std::logic_error("device buffer too small");
+    if (n) hip_check(hipMemcpyAsync(dst, p, n * sizeof(T), hipMemcpyDeviceToHost, *stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(*stream), "hipStreamSynchronize");
+  }
+  void upload_raw(const void* src, size_t n) {
+    if (n > this->n) throw std::logic_error("device buffer too small");
+    if (n) hip_check(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, *stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(*stream), "hipStreamSynchronize");
+  }
   void download(std::vector<T>& h) const {
     if (!h.empty())
       hip_check(hipMemcpyAsync(h.data(), p, h.size() * sizeof(T), hipMemcpyDeviceToHost, *stream), "hipMemcpyAsync");

@@ -261,11 +261,14 @@ class Scene:
         return h.value
 
     def _pull_collider_positions(self):
-        """Re-read every collider target from the world (a restore sets them)."""
-        out = np.zeros(3)
-        for k in range(len(self.colliders)):
-            check(lib().sdx_get_rigid_position(self._w, k, out.ctypes.data_as(C.POINTER(C.c_double))))
-            self._coll_pos_buf[k] = out
+        """Re-read every collider target from the world (a restore sets them), in one call."""
+        n = len(self.colliders)
+        if n == 0:
+            return
+        ids = np.arange(n, dtype=np.int32)
+        out = np.zeros((n, 3))
+        check(lib().sdx_get_rigid_positions(self._w, n, _i(ids), out.ctypes.data_as(C.POINTER(C.c_double))))
+        self._coll_pos_buf[:n] = out
 
     def restore_state(self, handle: int, release_immediately: bool = False):
         check(lib().sdx_restore_state(self._w, int(handle), int(bool(release_immediately))))
@@ -285,19 +288,17 @@ class Scene:
     def capture_state_to_bytes(self) -> bytes:
         n = C.c_int64()
         check(lib().sdx_state_size(self._w, C.byref(n)))
-        buf = (C.c_uint8 * n.value)()
-        check(lib().sdx_capture_state_to_bytes(self._w, buf, n.value))
-        return bytes(buf)
+        out = bytearray(n.value)
+        check(lib().sdx_capture_state_to_bytes(self._w, (C.c_uint8 * n.value).from_buffer(out), n.value))
+        return bytes(out)
 
     def restore_state_from_bytes(self, data: bytes):
-        buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
-        check(lib().sdx_restore_state_from_bytes(self._w, buf, len(data)))
-        self._pull_collider_positions()
+        self.restore_partial_state_from_bytes(data, 0)
 
     def restore_partial_state_from_bytes(self, data: bytes, excluded: int):
         """Restore every component except the kinds in `excluded` (_abi.STATE_* bits)."""
-        buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
-        check(lib().sdx_restore_partial_state_from_bytes(self._w, buf, len(data), int(excluded)))
+        data = bytes(data)  # a bytes object passes its own buffer, uncopied
+        check(lib().sdx_restore_partial_state_from_bytes(self._w, data, len(data), int(excluded)))
         self._pull_collider_positions()
 
     def capture_state_to_file(self, path: str):
