@@ -288,3 +288,30 @@ def test_byte_form_layout_partial_restore_json_and_file(tmp_path):
     assert sc.is_equal_state(h1, h2) == gs.is_equal_state_bytes(b1, b2) is False
     with pytest.raises(SdxError):
         sc.restore_partial_state_from_bytes(b1, 64)
+
+
+def test_fp64_world_bytes_and_json_round_trip():
+    """An FP64 world's byte form holds double components (read back by the plain Python reader),
+    its JSON reals read back exactly, and the bytes restore into a second FP64 world; an FP32
+    world rejects them."""
+    import state_format as sf
+
+    spec = spec_of(steps=3)
+    p = _abi.pinned_solver_params(1, precision_mode=_abi.PREC_MODE_FP64)
+    p.warm_start = _abi.WARM_START_HISTORY
+    sc, rigid = gs.build_scene(spec, precision=_abi.PREC_MODE_FP64)
+    run(sc, rigid, spec, p, 0, 3)
+    b = sc.capture_state_to_bytes()
+    blocks = sf.read(b)
+    disp, vel, hst = sf.actor_names(np.float64)
+    for a, (u, v) in enumerate(state(sc)):
+        np.testing.assert_array_equal(blocks[disp][1][a], u.ravel())
+        np.testing.assert_array_equal(blocks[vel][1][a], v.ravel())
+    doc = json.loads(sc.capture_state_to_json())
+    np.testing.assert_array_equal(np.array(doc["actors"]["Soft"]["components"][disp]), blocks[disp][1][0])
+    sc2, _ = gs.build_scene(spec, precision=_abi.PREC_MODE_FP64)
+    sc2.restore_state_from_bytes(b)
+    assert sc2.capture_state_to_bytes() == b
+    fp32, _ = gs.build_scene(spec, precision=_abi.PREC_MODE_FP32)
+    with pytest.raises(SdxError, match="another precision or topology"):
+        fp32.restore_state_from_bytes(b)
