@@ -1241,7 +1241,9 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
                                              const T* __restrict__ b, T* __restrict__ w,
                                              double* __restrict__ rref, EnvCG* env, A* acc, int32_t* num_active) {
   const int e = ta.env[tile];
-  const int status = env_status(env, e);  // branched on after the row loads are issued
+  // a work list is compacted only at burst starts: the tiles of an environment
+  // that finished inside the burst return before loading their rows
+  if (env_status(env, e) != kCgActive) return;
   const int nEnv = cfg.n_env;
   // raw accumulator slots (one per lane of each 16-lane group), issued first
   const int sl = (threadIdx.x & (kSlots - 1)) * kSlotStride;
@@ -1308,7 +1310,6 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
       zi2 = z[4 * i + 2];
     }
   }
-  if (status != kCgActive) return;
   const double start = slot_reduce(w_start);
   const double latest = slot_reduce(w_latest);
   const double gamma = slot_reduce(w_gamma);
@@ -1362,7 +1363,7 @@ __device__ __forceinline__ void cg_update_tile(int tile, const Tiles& tv, const 
                                                T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
                                                T* __restrict__ s, EnvCG* env, A* acc, int32_t* num_active) {
   const int e = tv.env[tile];
-  const int status = env_status(env, e);  // branched on after the row loads are issued
+  if (env_status(env, e) != kCgActive) return;  // finished inside the burst: load nothing
   const int nEnv = cfg.n_env;
   const int sl = (threadIdx.x & (kSlots - 1)) * kSlotStride;
   const int nsel = cfg.norm_type == 0 ? 0 : (cfg.norm_type == 1 ? 2 : 1);
@@ -1417,7 +1418,6 @@ __device__ __forceinline__ void cg_update_tile(int tile, const Tiles& tv, const 
       ro2 = r[3 * i + 2];
     }
   }
-  if (status != kCgActive) return;
   const double start = slot_reduce(w_start);
   const double latest = slot_reduce(w_latest);
   const double gamma = slot_reduce(w_gamma);
