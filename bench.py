@@ -8,9 +8,10 @@ Workloads (all synthetic, seeded, resident in HBM before the timed region):
                       1-3 sphere indenters each (r = 2 cm, random velocities
                       |v| <= 5 cm/s, penalty 1e6), 2 Newton iterations per
                       step, CG warm start from the previous Newton steps,
-                      solved together: one set of launches per CG iteration
-                      for all environments, launch schedule chosen by the
-                      autotuner on the first step.
+                      solved together: one launch per linear solve for all
+                      environments (each environment's whole CG in one
+                      workgroup), launch schedule chosen by the autotuner
+                      on the first step.
   batch512            512 cubes, n ~ U{6..16}, one indenter pressing down,
                       1 Newton iteration.
   press40             one 40^3-hex cube (384,000 tets) under a 2 mm plate press.
@@ -20,8 +21,9 @@ Common: backward Euler dt = 1e-3, block-Jacobi PCG to relative residual 1e-6
 with FP64 dots, FP64 iterate and an FP64 residual refresh every 10 iterations.
 
 A "step" is one Scene::Step of the workload.  The JSON line reports
-tet-steps/s, the roofline of the dominant kernel (the CG matvec, timed with
-HIP events at its own dispatch during the timed region), the same steps from
+tet-steps/s, the roofline of the dominant kernel (the whole-solve CG kernel
+where the schedule runs it, else the CG matvec; timed with HIP events at its
+own dispatch during the timed region), the same steps from
 a zero initial guess and with the trained network added to the warm start
 (--warm-start learned makes that the timed path), the plain GPU PCG path on a sample of the environments
 (each environment stepped on its own, FP32 PCG from a zero initial guess,
@@ -74,12 +76,37 @@ def step_bytes(nT, nB, nV, cg_iters, ls_trials, newton=1, s=4):
     return (newton * (bytes_asm + 2 * bytes_res) + ls_trials * bytes_res + cg_iters * bytes_cg + bytes_res)
 
 
-def traffic_file(workload):
-    return os.path.join(ROOT, "profiles", f"cg_spmv_{workload}_traffic.txt")
+def body_blocks(tets, nv: int) -> int:
+    """Nonzero 3x3 blocks of one body's BSR pattern: its distinct ordered
+    vertex pairs within a tet, the diagonal included."""
+    t = np.asarray(tets, dtype=np.int64)
+    return int(np.unique((t[:, :, None] * nv + t[:, None, :]).ravel()).size)
 
 
-def read_traffic(workload):
-    path = traffic_file(workload)
+def env_cg_iteration_bytes(spec, s: int = 4, col_bytes: int = 2) -> np.ndarray:
+    """Algorithmic HBM bytes of one whole-solve CG iteration (k_cg_env) of
+    each environment: its block values and column indices ((9s + c) B per
+    block), row pointers (4 B per row), and per row r, p, s and the iterate
+    (increments) read and written plus the 6-plane block-Jacobi inverse read
+    (30s B); z and w stay in LDS.  A refresh iteration's FP64 iterate and
+    true-residual traffic (every 10th) is not counted."""
+    cache = {}
+    out = np.empty(len(spec.bodies))
+    for e, b in enumerate(spec.bodies):
+        nv = b.X.shape[0]
+        key = (nv, hash(np.asarray(b.tets).tobytes()))
+        if key not in cache:
+            cache[key] = body_blocks(b.tets, nv)
+        out[e] = (9 * s + col_bytes) * cache[key] + 4 * (nv + 1) + 30 * s * nv
+    return out
+
+
+def traffic_file(workload, kernel="cg_spmv"):
+    return os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+
+
+def read_traffic(workload, kernel="cg_spmv"):
+    path = traffic_file(workload, kernel)
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
@@ -360,12 +387,15 @@ def main(argv=None) -> int:
     print(f"[bench] warm, schedule {sc.schedule()}", file=sys.stderr, flush=True)
     if args.time_every > 0:
         sc.kernel_timing(0, enable=args.time_every)
+        sc.kernel_timing(4, enable=1)  # the whole-solve CG launch of every solve, where it runs
     stats = []
     t0 = time.perf_counter()
     run_steps(sc, rigid, spec, params, warmup, total, stats)
     elapsed = time.perf_counter() - t0
     launches, ms = sc.kernel_timing(0) if args.time_every > 0 else (0, 0.0)
+    env_launches, env_ms = sc.kernel_timing(4) if args.time_every > 0 else (0, 0.0)
     sc.kernel_timing(0, enable=0)
+    sc.kernel_timing(4, enable=0)
     print(f"[bench] timed {steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
     snapshot = state_snapshot_costs(sc)
 
@@ -386,6 +416,12 @@ def main(argv=None) -> int:
     per_launch = matvec_bytes(nB, nV, s_bytes, column_bytes(spec)) / groups
     avg_ms = ms / launches if launches else None
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms else None
+    if env_launches:
+        # the whole-solve kernel: every launch of the timed steps, bytes of every iteration
+        per_iter = env_cg_iteration_bytes(spec, s_bytes, column_bytes(spec))
+        per_launch = float((it * per_iter[None, :]).sum()) / env_launches
+        avg_ms = env_ms / env_launches
+        achieved = per_launch / (avg_ms * 1e-3) / 1e9
     # reference-equivalent bytes per step, environment by environment
     step_b = 0.0
     for e, b in enumerate(spec.bodies):
@@ -425,6 +461,15 @@ def main(argv=None) -> int:
             "timed": ("every launch of the first CG burst (all environments active; the first burst of "
                       f"each environment group when grouped) of every {args.time_every}-th solve")
             if args.time_every > 0 else None,
+        } if not env_launches else {
+            "bound": "hbm",
+            "kernel": "k_cg_env (whole CG solve of every environment, one workgroup each: w = A z, dots, update)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(args.workload, "cg_env"),
+            "bytes_per_launch": per_launch, "avg_launch_us": 1e3 * avg_ms, "timed_launches": int(env_launches),
+            "timed": "every whole-solve launch of the timed steps (one per linear solve); bytes: every "
+                     "iteration of every environment, matrix rows and vectors once per iteration "
+                     "(re-reads of an environment's rows are served by the last-level cache where it holds them)",
         },
         "step": {"cg_iters_mean_per_env": float(it.mean()),
                  "cg_iters_tet_weighted": float((it * sizes).sum(axis=1).mean() / sizes.sum()),

@@ -450,11 +450,13 @@ sdx_status sdx_objective(sdx_world* world, int32_t actor, double dt, const sdx_s
 /* Timing of one kernel class inside sdx_step, measured with HIP events
  * recorded at the kernel's own dispatch on the world's stream.  kernel: 0 =
  * CG matvec (w = A z + z.w), 1 = CG vector update, 2 = element Hessian, 3 =
- * BSR gather.  enable N > 0 clears the accumulators and times every N-th
+ * BSR gather, 4 = whole-solve CG (sdx_schedule.env_cg: one launch per linear
+ * solve).  enable N > 0 clears the accumulators and times every N-th
  * launch of an assembly class, or every launch of the first CG burst (all
  * environments active; with environment groups, the first burst of each
  * group, whose launches cover that group's rows) of every N-th linear solve
- * (those bursts are launched eagerly, the others as the schedule says); 0
+ * (those bursts are launched eagerly, the others as the schedule says), or
+ * the whole-solve launch of every N-th linear solve; 0
  * stops timing, -1 leaves the setting unchanged.  out (may be NULL): {timed
  * launches, total milliseconds}. */
 sdx_status sdx_kernel_timing(sdx_world* world, int32_t kernel, int32_t enable, double out[2]);
@@ -507,15 +509,26 @@ typedef struct sdx_schedule {
                              faster: a block recomputes the tets it shares with its neighbours
                              (about 1.7-1.9 times the tets) at an occupancy of two workgroups
                              per CU.  The autotuner leaves it off */
+  int32_t env_cg;         /* 1 or 2: each environment's whole CG solve runs in one workgroup of a
+                             single launch per solve (that many workgroups per CU), which takes
+                             environments from a queue, largest first, and iterates each to its
+                             stop decision with z and w in LDS: no kernel boundary, grid barrier,
+                             work-list compaction or host check between iterations.  Applies to
+                             worlds of several environments whose largest one's rows fit the
+                             LDS a workgroup may take (28 bytes per vertex in FP32 and mixed
+                             precision, 56 in FP64: about 5800 / 2900 vertices in 160 KB, 2300 /
+                             1160 in 64 KB); otherwise, and with 0, the CG runs as the two
+                             kernels per iteration the other fields describe (default 1) */
 } sdx_schedule;
 
 sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* schedule);
 sdx_status sdx_get_schedule(sdx_world* world, sdx_schedule* out);
 
 /* Autotuner: times sdx_step of the world's current state under candidate
- * schedules (coordinate search over matvec lanes, element workgroup size,
- * work lists, CG burst length and graph replay, then environment groups; the
- * best of `trials` steps each),
+ * schedules (coordinate search over matvec lanes, the whole-solve CG where
+ * every environment fits it, element workgroup size, and where the solve
+ * still runs as bursts work lists, CG burst length and graph replay, then
+ * environment groups; the best of `trials` steps each),
  * restores the state after every trial and keeps the fastest schedule.
  * out / out_ms (may be NULL): the chosen schedule and its step time. */
 sdx_status sdx_autotune(sdx_world* world, double dt, const sdx_solver_params* params, int32_t trials,

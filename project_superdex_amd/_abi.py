@@ -91,6 +91,7 @@ class Schedule(C.Structure):
         ("fused_tail", C.c_int32),
         ("asm_groups", C.c_int32),
         ("fused_asm", C.c_int32),
+        ("env_cg", C.c_int32),
     ]
 
     def as_dict(self):

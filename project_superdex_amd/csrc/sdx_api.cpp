@@ -545,7 +545,7 @@ sdx_status sdx_objective(sdx_world* world, int32_t actor, double dt, const sdx_s
 
 sdx_status sdx_kernel_timing(sdx_world* world, int32_t kernel, int32_t enable, double out[2]) {
   SDX_REQUIRE(world, "NULL argument");
-  SDX_REQUIRE(kernel >= 0 && kernel < 4, "kernel class must be 0..3");
+  SDX_REQUIRE(kernel >= 0 && kernel < 5, "kernel class must be 0..4");
   return guarded([&]() -> sdx_status {
     if (enable >= 0) world->impl->timing_enable(kernel, enable);
     if (out) {
@@ -580,6 +580,7 @@ sdx_status sdx_set_schedule(sdx_world* world, const sdx_schedule* sc) {
   SDX_REQUIRE(sc->fused_tail == 0 || sc->fused_tail == 1, "fused_tail must be 0 or 1");
   SDX_REQUIRE(sc->asm_groups >= 0 && sc->asm_groups <= 4096, "asm_groups must be in 0..4096");
   SDX_REQUIRE(sc->fused_asm == 0 || sc->fused_asm == 1, "fused_asm must be 0 or 1");
+  SDX_REQUIRE(sc->env_cg >= 0 && sc->env_cg <= 2, "env_cg must be 0, 1 or 2");
   return guarded([&]() -> sdx_status {
     world->impl->set_schedule(*sc);
     return SDX_OK;

@@ -84,6 +84,33 @@ constexpr int kSlots = 16;
 constexpr int kSlotStride = 16;
 constexpr int kCgSets = 6;
 
+// Whole-solve CG (k_cg_env): one workgroup of kEnvThreads runs every iteration
+// of one environment at a time, taking environments from a queue.
+constexpr int kEnvThreads = 1024;
+constexpr int kEnvWaves = kEnvThreads / 64;
+constexpr int kEnvLdsMax = 160 * 1024;  // LDS bytes of a CU: no workgroup takes more
+struct EnvQueue {
+  const int32_t* order;  // environments in the order workgroups take them (largest first)
+  int32_t n;             // entries of order
+  int32_t max_rows;      // LDS rows per workgroup: at least every queued environment's rows
+  int32_t* next;         // queue position, zero before the launch
+  const int32_t* v0;     // [nEnv + 1] first row of each environment
+};
+// LDS of a workgroup: z as Z4 [max_rows], w [3 max_rows], then the two
+// reduction areas (kEnvWaves partials of 1 and of 4 sums).
+template <class T>
+__host__ __device__ inline size_t env_lds_w_offset(int max_rows) {
+  return size_t(max_rows) * 4 * sizeof(T);
+}
+template <class T>
+__host__ __device__ inline size_t env_lds_red_offset(int max_rows) {
+  return (env_lds_w_offset<T>(max_rows) + size_t(max_rows) * 3 * sizeof(T) + 15) & ~size_t(15);
+}
+template <class T, class A>
+__host__ __device__ inline size_t env_lds_bytes(int max_rows) {
+  return env_lds_red_offset<T>(max_rows) + size_t(kEnvWaves) * 5 * sizeof(A);
+}
+
 // Geometry planes of a tet, stored SoA with stride nT.
 // kGBeta: stiffness damping coefficient beta [s]; kGBetaGeo: 1 where its tangent
 // takes the geometric block.
@@ -268,6 +295,19 @@ template <class T, class A>
 void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int grid, const CgConfig& cfg, int k,
                       const T* dinv, int32_t nV, const T* w, const double* rref, A* x, T* r, T* z, T* p, T* sv,
                       EnvCG* env, A* acc, int32_t* num_active, const KernelEvents* ev = nullptr);
+// LDS bytes a workgroup may take on the current device (its reported limit,
+// at most kEnvLdsMax).
+int env_lds_limit();
+
+// The whole CG of every active environment of q in one launch of `grid`
+// workgroups (k_cg_env): the iterations of k_cg_spmv + k_cg_update from the
+// state k_cg_init left, each environment's EnvCG record at the end.  Throws if
+// q.max_rows needs more than lds_limit bytes of LDS.
+template <class T, class A>
+void launch_cg_env(hipStream_t s, const EnvQueue& q, int grid, size_t lds_limit, const CgConfig& cfg, int lanes,
+                   const int32_t* row_ptr, const int32_t* col, const int16_t* col16, const T* vals, int32_t nB,
+                   const T* dinv, int32_t nV, const T* b, const T* z, A* x, T* r, T* p, T* sv, double* rref,
+                   EnvCG* env, const A* acc, int32_t* num_active, const KernelEvents* ev);
 // Work lists of the tiles of active environments among tiles [a0, a1) of ta
 // and [v0, v1) of tv (slot `par` of counts, which must start at zero; slot
 // par ^ 1 is cleared for the next burst).

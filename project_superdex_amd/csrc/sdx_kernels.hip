@@ -1565,6 +1565,313 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, WorkList wl
   });
 }
 
+// ------------------------------------------------------------ whole-solve CG
+//
+// Environments never exchange data, so one workgroup can run every iteration
+// of one environment's PCG by itself (schedule env_cg): workgroups take
+// environments from a queue, largest first, and iterate each to its stop
+// decision with the arithmetic of cg_spmv_tile and cg_update_tile.  z and w
+// live in LDS, an iteration's sums are workgroup reductions in a fixed order,
+// and nothing waits on another workgroup or on a kernel boundary.  The rows of
+// an environment's matrix are re-read every iteration: with one or two
+// workgroups per CU, the matrices in flight stay within the last-level cache.
+
+// Sum N values over the workgroup, in every thread, in a fixed order: wave
+// sums, then the waves' partials (LDS `red`, kEnvWaves * N) in wave order.
+template <int N, class A>
+__device__ __forceinline__ void env_block_sum(A (&v)[N], A* red) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < N; ++c) v[c] = wave_sum(v[c]);
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < N; ++c) red[wave * N + c] = v[c];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    A t = red[c];
+    for (int w = 1; w < kEnvWaves; ++w) t += red[w * N + c];
+    v[c] = t;
+  }
+}
+
+// One environment's CG from the state k_cg_init left (r, z, x, the scale and
+// R_0 accumulators) to its stop decision; the same iterations, scalars and
+// stop tests as k_cg_spmv + k_cg_update, and the same EnvCG record at the end.
+template <class T, class A, int L, bool kC16>
+__device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConfig& cfg,
+                                             const int32_t* __restrict__ row_ptr, const SpmvBufs& bf,
+                                             const T* __restrict__ dinv, int nV, const T* __restrict__ b,
+                                             const T* __restrict__ zin, A* __restrict__ x, T* __restrict__ r,
+                                             T* __restrict__ p, T* __restrict__ s, double* __restrict__ rref,
+                                             EnvCG* env, const A* __restrict__ acc, int32_t* num_active, Z4<T>* zl,
+                                             T* wl, A* red) {
+  const int nsel = cfg.norm_type == 0 ? 0 : (cfg.norm_type == 1 ? 2 : 1);
+  for (int il = threadIdx.x; il < nr; il += kEnvThreads) zl[il] = reinterpret_cast<const Z4<T>*>(zin)[v0 + il];
+  // the scale and R_0 of k_cg_init, summed over the accumulator slots in slot order
+  double start = 0.0, R[4] = {0.0, 0.0, 0.0, 0.0};
+  {
+    const A* sc = acc_ptr(acc, kSetScale, cfg.n_env, e);
+    const A* rs = acc_ptr(acc, kSetR, cfg.n_env, e);
+    A st = A(0), r4[4] = {A(0), A(0), A(0), A(0)};
+    for (int sl = 0; sl < kSlots; ++sl) {
+      st += sc[sl * kSlotStride + nsel];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) r4[c] += rs[sl * kSlotStride + c];
+    }
+    start = double(st);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) R[c] = double(r4[c]);
+  }
+  T* xd = static_cast<T*>(cfg.xd);
+  const int sub = threadIdx.x / L, lane = threadIdx.x % L;
+  double g_prev = 0.0, a_prev = 0.0, m_prev = 0.0;
+  __syncthreads();  // z in LDS
+  for (int k = 0;; ++k) {
+    const double latest = R[nsel], gamma = R[1], rho = R[3];
+    const int st = cg_decide(cfg, e, k, start, latest, gamma);
+    if (st != kCgActive) {
+      if (threadIdx.x == 0) {
+        EnvCG& E = env[e];
+        E.status = st;
+        E.iter = k;
+        E.latest = latest < 0.0 ? 0.0 : latest;
+        E.start = start;
+        E.rz = gamma;
+        atomicSub(num_active, 1);
+      }
+      return;
+    }
+    const bool refresh = cg_refresh(cfg, k);
+    // matvec: w = A z into LDS, z.w; on refresh iterations also the FP64
+    // true residual rref = b - A (x + xd) of the iterate
+    A dz[1] = {A(0)};
+    for (int base = 0; base < nr; base += kEnvThreads / L) {
+      const int il = base + sub;
+      const bool valid = il < nr;
+      const int i = v0 + il;
+      T a0 = T(0), a1 = T(0), a2 = T(0);
+      double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+      if (valid) {
+        const int k0 = row_ptr[i], k1 = row_ptr[i + 1];
+        constexpr int U = kSpmvChunk / L > 0 ? kSpmvChunk / L : 1;
+        for (int kb = k0 + lane; kb < k1; kb += U * L) {
+          int kk[U], jc[U];
+          T wv[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int kr = kb + u * L;
+            kk[u] = kr < k1 ? kr : k1 - 1;
+            wv[u] = kr < k1 ? T(1) : T(0);
+            if (kC16)
+              jc[u] = i + int(short(__builtin_amdgcn_raw_buffer_load_b16(bf.col, uint32_t(kk[u]) * 2u, 0, 0)));
+            else
+              jc[u] = buf_ld<int>(bf.col, uint32_t(kk[u]) * 4u);
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint32_t o = uint32_t(kk[u]) * uint32_t(sizeof(T));
+            const T m0 = buf_ld<T>(bf.v[0], o) * wv[u], m1 = buf_ld<T>(bf.v[1], o) * wv[u];
+            const T m2 = buf_ld<T>(bf.v[2], o) * wv[u], m3 = buf_ld<T>(bf.v[3], o) * wv[u];
+            const T m4 = buf_ld<T>(bf.v[4], o) * wv[u], m5 = buf_ld<T>(bf.v[5], o) * wv[u];
+            const T m6 = buf_ld<T>(bf.v[6], o) * wv[u], m7 = buf_ld<T>(bf.v[7], o) * wv[u];
+            const T m8 = buf_ld<T>(bf.v[8], o) * wv[u];
+            const Z4<T> zj = zl[jc[u] - v0];
+            a0 += m0 * zj.v[0] + m1 * zj.v[1] + m2 * zj.v[2];
+            a1 += m3 * zj.v[0] + m4 * zj.v[1] + m5 * zj.v[2];
+            a2 += m6 * zj.v[0] + m7 * zj.v[1] + m8 * zj.v[2];
+            if (refresh) {
+              double x0 = x[3 * jc[u]], x1 = x[3 * jc[u] + 1], x2 = x[3 * jc[u] + 2];
+              if (xd) {
+                x0 += double(xd[3 * jc[u]]);
+                x1 += double(xd[3 * jc[u] + 1]);
+                x2 += double(xd[3 * jc[u] + 2]);
+              }
+              q0 += double(m0) * x0 + double(m1) * x1 + double(m2) * x2;
+              q1 += double(m3) * x0 + double(m4) * x1 + double(m5) * x2;
+              q2 += double(m6) * x0 + double(m7) * x1 + double(m8) * x2;
+            }
+          }
+        }
+      }
+      a0 = group_sum<L>(a0);
+      a1 = group_sum<L>(a1);
+      a2 = group_sum<L>(a2);
+      if (refresh) {
+        q0 = group_sum<L>(q0);
+        q1 = group_sum<L>(q1);
+        q2 = group_sum<L>(q2);
+      }
+      if (valid && lane == 0) {
+        wl[3 * il] = a0;
+        wl[3 * il + 1] = a1;
+        wl[3 * il + 2] = a2;
+        const Z4<T> zi = zl[il];
+        dz[0] += A(zi.v[0]) * A(a0) + A(zi.v[1]) * A(a1) + A(zi.v[2]) * A(a2);
+        if (refresh) {
+          rref[3 * i] = double(b[3 * i]) - q0;
+          rref[3 * i + 1] = double(b[3 * i + 1]) - q1;
+          rref[3 * i + 2] = double(b[3 * i + 2]) - q2;
+        }
+      }
+    }
+    env_block_sum<1>(dz, red);  // also orders the w and rref writes before the update
+    const double delta = double(dz[0]);
+    double beta = 0.0, mu = delta;
+    if (k >= 1) {
+      beta = (gamma - (k >= 2 ? rho : 0.0)) / g_prev;
+      mu = delta + 2.0 * beta * (rho - gamma) / a_prev + beta * beta * m_prev;
+    }
+    if (mu <= 0.0 && cfg.abort_if_not_spd) {  // pcg.h:152-169: not SPD, abort with iteration k + 1
+      if (threadIdx.x == 0) {
+        EnvCG& E = env[e];
+        E.status = kCgDiverged;
+        E.iter = k + 1;
+        E.latest = latest < 0.0 ? 0.0 : latest;
+        E.start = start;
+        E.rz = gamma;
+        atomicSub(num_active, 1);
+      }
+      return;
+    }
+    const double alpha = gamma / mu;
+    const T al = T(alpha), bt = T(beta);
+    // update: p = z + beta p, s = w + beta s, the iterate, r, z = M^-1 r (LDS)
+    A part[4] = {A(0), A(0), A(0), A(0)};  // r.r, r.z, z.z, r.z_old
+    for (int il = threadIdx.x; il < nr; il += kEnvThreads) {
+      const int i = v0 + il;
+      T d0 = T(0), d1 = T(0), d2 = T(0);
+      A x0 = A(0), x1 = A(0), x2 = A(0);
+      if (xd) {
+        d0 = xd[3 * i];
+        d1 = xd[3 * i + 1];
+        d2 = xd[3 * i + 2];
+      }
+      if (!xd || refresh) {
+        x0 = x[3 * i];
+        x1 = x[3 * i + 1];
+        x2 = x[3 * i + 2];
+      }
+      T p0 = T(0), p1 = T(0), p2 = T(0), s0 = T(0), s1 = T(0), s2 = T(0);
+      if (k > 0) {
+        p0 = p[3 * i];
+        p1 = p[3 * i + 1];
+        p2 = p[3 * i + 2];
+        s0 = s[3 * i];
+        s1 = s[3 * i + 1];
+        s2 = s[3 * i + 2];
+      }
+      double ro0, ro1, ro2;
+      if (refresh) {
+        ro0 = rref[3 * i];
+        ro1 = rref[3 * i + 1];
+        ro2 = rref[3 * i + 2];
+      } else {
+        ro0 = r[3 * i];
+        ro1 = r[3 * i + 1];
+        ro2 = r[3 * i + 2];
+      }
+      Z4<T> zo = zl[il];
+      const T w0 = wl[3 * il], w1 = wl[3 * il + 1], w2 = wl[3 * il + 2];
+      p0 = zo.v[0] + bt * p0;
+      p1 = zo.v[1] + bt * p1;
+      p2 = zo.v[2] + bt * p2;
+      s0 = w0 + bt * s0;
+      s1 = w1 + bt * s1;
+      s2 = w2 + bt * s2;
+      p[3 * i] = p0;
+      p[3 * i + 1] = p1;
+      p[3 * i + 2] = p2;
+      s[3 * i] = s0;
+      s[3 * i + 1] = s1;
+      s[3 * i + 2] = s2;
+      if (xd && !refresh) {
+        xd[3 * i] = d0 + al * p0;
+        xd[3 * i + 1] = d1 + al * p1;
+        xd[3 * i + 2] = d2 + al * p2;
+      } else {
+        x[3 * i] = x0 + A(d0) + A(alpha) * A(p0);
+        x[3 * i + 1] = x1 + A(d1) + A(alpha) * A(p1);
+        x[3 * i + 2] = x2 + A(d2) + A(alpha) * A(p2);
+        if (xd) {
+          xd[3 * i] = T(0);
+          xd[3 * i + 1] = T(0);
+          xd[3 * i + 2] = T(0);
+        }
+      }
+      T r0, r1, r2;
+      if (refresh) {
+        r0 = T(ro0 - alpha * double(s0));
+        r1 = T(ro1 - alpha * double(s1));
+        r2 = T(ro2 - alpha * double(s2));
+      } else {
+        r0 = T(ro0) - al * s0;
+        r1 = T(ro1) - al * s1;
+        r2 = T(ro2) - al * s2;
+      }
+      r[3 * i] = r0;
+      r[3 * i + 1] = r1;
+      r[3 * i + 2] = r2;
+      T n0, n1, n2;
+      apply_dinv(dinv, nV, i, cfg.prec, r0, r1, r2, n0, n1, n2);
+      Z4<T> zn;
+      zn.v[0] = n0;
+      zn.v[1] = n1;
+      zn.v[2] = n2;
+      zn.v[3] = T(0);
+      zl[il] = zn;
+      if (refresh)  // z_k replaced by M^-1 rref for the next mu recurrence
+        apply_dinv(dinv, nV, i, cfg.prec, T(ro0), T(ro1), T(ro2), zo.v[0], zo.v[1], zo.v[2]);
+      part[0] += A(r0) * A(r0) + A(r1) * A(r1) + A(r2) * A(r2);
+      part[1] += A(r0) * A(n0) + A(r1) * A(n1) + A(r2) * A(n2);
+      part[2] += A(n0) * A(n0) + A(n1) * A(n1) + A(n2) * A(n2);
+      part[3] += A(r0) * A(zo.v[0]) + A(r1) * A(zo.v[1]) + A(r2) * A(zo.v[2]);
+    }
+    env_block_sum<4>(part, red + kEnvWaves);  // also orders the LDS z writes before the next matvec
+#pragma unroll
+    for (int c = 0; c < 4; ++c) R[c] = double(part[c]);
+    g_prev = gamma;
+    a_prev = alpha;
+    m_prev = mu;
+  }
+}
+
+template <class T, class A, int L, bool kC16>
+__global__ void __launch_bounds__(kEnvThreads) k_cg_env(EnvQueue q, CgConfig cfg, const int32_t* __restrict__ row_ptr,
+                                                        const void* __restrict__ col, const T* __restrict__ vals,
+                                                        int nB, const T* __restrict__ dinv, int nV,
+                                                        const T* __restrict__ b, const T* __restrict__ zin,
+                                                        A* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
+                                                        T* __restrict__ s, double* __restrict__ rref, EnvCG* env,
+                                                        const A* __restrict__ acc, int32_t* num_active) {
+  extern __shared__ __align__(16) unsigned char env_lds[];
+  Z4<T>* zl = reinterpret_cast<Z4<T>*>(env_lds);
+  T* wl = reinterpret_cast<T*>(env_lds + env_lds_w_offset<T>(q.max_rows));
+  A* red = reinterpret_cast<A*>(env_lds + env_lds_red_offset<T>(q.max_rows));
+  __shared__ int s_next;
+  const SpmvBufs bf = spmv_bufs<T>(col, uint32_t(nB) * (kC16 ? 2u : 4u), vals, nB, zin, nV);
+  for (;;) {
+    if (threadIdx.x == 0) s_next = atomicAdd(q.next, 1);
+    __syncthreads();
+    const int qi = s_next;
+    __syncthreads();  // every thread has read s_next before thread 0 takes the next one
+    if (qi >= q.n) return;
+    const int e = q.order[qi];
+    if (env[e].status != kCgActive) continue;
+    const int v0 = q.v0[e], nr = q.v0[e + 1] - v0;
+    if (nr > q.max_rows) {  // not reached: the launcher sized max_rows for every environment
+      if (threadIdx.x == 0) {
+        env[e].status = kCgDiverged;
+        atomicSub(num_active, 1);
+      }
+      continue;
+    }
+    cg_env_solve<T, A, L, kC16>(e, v0, nr, cfg, row_ptr, bf, dinv, nV, b, zin, x, r, p, s, rref, env, acc,
+                                num_active, zl, wl, red);
+    __syncthreads();  // the next environment reuses the LDS
+  }
+}
+
 // Grid-wide barrier of a cooperative launch (every workgroup resident):
 // arrivals are counted on *bar, zeroed before the launch, and barrier j of
 // the launch waits for j * gridDim.x of them.  The wait is bounded: past
@@ -2272,6 +2579,58 @@ void launch_cg_update(hipStream_t s, const Tiles& tv, const WorkList& wl, int gr
                r, z, p, sv, env, acc, num_active);
 }
 
+int env_lds_limit() {
+  int dev = 0, lim = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0)
+    return 64 * 1024;
+  return std::min(lim, kEnvLdsMax);
+}
+
+template <class T, class A>
+void launch_cg_env(hipStream_t s, const EnvQueue& q, int grid, size_t lds_limit, const CgConfig& cfg, int lanes,
+                   const int32_t* row_ptr, const int32_t* col, const int16_t* col16, const T* vals, int32_t nB,
+                   const T* dinv, int32_t nV, const T* b, const T* z, A* x, T* r, T* p, T* sv, double* rref,
+                   EnvCG* env, const A* acc, int32_t* num_active, const KernelEvents* ev) {
+  if (q.n <= 0 || grid <= 0) return;
+  if (q.max_rows <= 0 || !q.order || !q.next || !q.v0)
+    throw std::invalid_argument("cg_env: empty queue description");
+  const size_t lds = env_lds_bytes<T, A>(q.max_rows);
+  if (lds > lds_limit || lds > size_t(kEnvLdsMax))
+    throw std::invalid_argument("cg_env: an environment's rows need more LDS than a workgroup may take");
+  if (uint64_t(nB) * sizeof(T) >= (uint64_t(1) << 32) || uint64_t(cfg.n_rows) * 4 * sizeof(T) >= (uint64_t(1) << 32))
+    throw std::invalid_argument("matrix too large for 32-bit buffer offsets of the CG matvec");
+  const void* cp = col16 ? static_cast<const void*>(col16) : static_cast<const void*>(col);
+  // above 64 KB a kernel's dynamic LDS must be allowed for it first
+#define SDX_ENV(L, C)                                                                                          \
+  do {                                                                                                         \
+    if (lds > size_t(64 * 1024)) {                                                                             \
+      const hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cg_env<T, A, L, C>),          \
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));          \
+      if (err != hipSuccess) throw std::runtime_error("cg_env: hipFuncSetAttribute refused the workgroup's LDS"); \
+    }                                                                                                          \
+    if (ev)                                                                                                    \
+      hipExtLaunchKernelGGL((k_cg_env<T, A, L, C>), dim3(grid), dim3(kEnvThreads), lds, s, ev->start, ev->stop, \
+                            0, q, cfg, row_ptr, cp, vals, nB, dinv, nV, b, z, x, r, p, sv, rref, env, acc,      \
+                            num_active);                                                                       \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_cg_env<T, A, L, C>), dim3(grid), dim3(kEnvThreads), lds, s, q, cfg, row_ptr, cp,    \
+                         vals, nB, dinv, nV, b, z, x, r, p, sv, rref, env, acc, num_active);                   \
+  } while (0)
+  switch (lanes) {
+    case 4:
+      if (col16) SDX_ENV(4, true); else SDX_ENV(4, false);
+      break;
+    case 8:
+      if (col16) SDX_ENV(8, true); else SDX_ENV(8, false);
+      break;
+    default:
+      if (col16) SDX_ENV(16, true); else SDX_ENV(16, false);
+      break;
+  }
+#undef SDX_ENV
+}
+
 void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, int a0, int a1, int v0, int v1,
                           const EnvCG* env, int32_t* la, int32_t* lv, int32_t* counts, int par,
                           const int32_t* num_active, int32_t* publish, uint32_t* bar) {
@@ -2478,6 +2837,11 @@ void launch_energy(hipStream_t s, const MeshView& m, const FieldPtrs<T>& f, cons
   template void launch_cg_update<T, A>(hipStream_t, const Tiles&, const WorkList&, int, const CgConfig&, int,    \
                                        const T*, int32_t, const T*, const double*, A*, T*, T*, T*, T*, EnvCG*,   \
                                        A*, int32_t*, const KernelEvents*);                                       \
+  template void launch_cg_env<T, A>(hipStream_t, const EnvQueue&, int, size_t, const CgConfig&, int,            \
+                                    const int32_t*,                                                               \
+                                    const int32_t*, const int16_t*, const T*, int32_t, const T*, int32_t, const T*, \
+                                    const T*, A*, T*, T*, T*, double*, EnvCG*, const A*, int32_t*,                \
+                                    const KernelEvents*);                                                         \
   template void launch_cg_tail<T, A>(hipStream_t, const Tiles&, const Tiles&, const WorkList&, const WorkList&,  \
                                      int, const CgConfig&, int, int, int, const int32_t*, const int32_t*,         \
                                      const int16_t*, const T*, int32_t, const T*, int32_t, A*, const T*, T*,      \

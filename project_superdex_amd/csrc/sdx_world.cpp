@@ -451,6 +451,23 @@ void World<T>::finalize() {
   ws_beta_.alloc(static_cast<size_t>(nEnv));
   ws_gamma_.alloc(static_cast<size_t>(nEnv));
   d_env_v0_.upload(env_v0_);
+  {  // whole-solve CG queue: environments by rows, largest first (ties by index)
+    std::vector<int32_t> order(static_cast<size_t>(nEnv));
+    env_max_rows_ = 0;
+    for (int e = 0; e < nEnv; ++e) {
+      order[size_t(e)] = e;
+      env_max_rows_ = std::max(env_max_rows_, env_v0_[size_t(e) + 1] - env_v0_[size_t(e)]);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+      return env_v0_[size_t(a) + 1] - env_v0_[size_t(a)] > env_v0_[size_t(b) + 1] - env_v0_[size_t(b)];
+    });
+    d_env_order_.upload(order);
+    env_next_.alloc(1);
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    env_lds_limit_ = env_lds_limit();
+  }
   num_active_.alloc(1);
   env_mask_.alloc(static_cast<size_t>(nEnv));
   ls_mask_.alloc(static_cast<size_t>(nEnv));
@@ -969,6 +986,21 @@ void World<T>::drop_graphs() {
   grouped_graph_ = nullptr;
 }
 
+// Whether every environment's rows fit a workgroup's LDS (whole-solve CG),
+// in a world of several environments (a single one keeps the bursts, which
+// spread its rows over every CU).
+template <class T>
+bool World<T>::env_cg_fits() const {
+  return nEnv_ > 1 && env_max_rows_ > 0 && env_lds_bytes<T, double>(env_max_rows_) <= size_t(env_lds_limit_);
+}
+
+// Whether a solve runs as one whole-solve CG launch: asked for by the
+// schedule, and env_cg_fits.
+template <class T>
+bool World<T>::env_cg_applies() const {
+  return sched_.env_cg > 0 && env_cg_fits();
+}
+
 // One CG run for the environments with solve[e]; those with warm[e] keep
 // dx_ as the initial guess.  Returns the per-environment CG state.
 template <class T>
@@ -1037,6 +1069,26 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
     launch_cg_init<T, T>(stream_, dt_v_.view, cfg, res_.p, wm, rref_.p, wa, v, rv, wb, w, rw, wc, dinv_.p, nV_,
                          xcg<T>(), r_.p, z_.p, env_cg_.p, accf);
   }
+  tail_used_ = false;
+  if (env_cg_applies()) {
+    // the whole solve as one launch: each workgroup iterates whole
+    // environments from the queue (largest first) to their stop decisions
+    const int grid = std::max(1, std::min(active, std::max(1, num_cus_) * sched_.env_cg));
+    launch_fill<int32_t>(stream_, 1, 0, env_next_.p);
+    const EnvQueue eq{d_env_order_.p, nEnv_, env_max_rows_, env_next_.p, d_env_v0_.p};
+    const KernelEvents* ev = timing_ok ? env_timer_events() : nullptr;
+    if (precision_ != SDX_PREC_MODE_FP32)
+      launch_cg_env<T, double>(stream_, eq, grid, size_t(env_lds_limit_), cfg, sched_.spmv_lanes, d_row_ptr_.p, d_col_.p, d_col16_.p,
+                               vals_.p, nB_, dinv_.p, nV_, res_.p, z_.p, xcg<double>(), r_.p, p_.p, sv_.p, rref_.p,
+                               env_cg_.p, acc_.p, num_active_.p, ev);
+    else
+      launch_cg_env<T, T>(stream_, eq, grid, size_t(env_lds_limit_), cfg, sched_.spmv_lanes, d_row_ptr_.p, d_col_.p, d_col16_.p, vals_.p,
+                          nB_, dinv_.p, nV_, res_.p, z_.p, xcg<T>(), r_.p, p_.p, sv_.p, rref_.p, env_cg_.p,
+                          reinterpret_cast<T*>(acc_.p), num_active_.p, ev);
+    if (cfg.xd)  // the increments since the last refresh, into x
+      launch_cg_fold<T, double>(stream_, int64_t(3) * nV_, xcg<double>(), static_cast<T*>(cfg.xd));
+    return true;
+  }
   // Bursts of B iterations are issued kBurstAhead bursts ahead of the host's
   // check of the active count, so the device does not wait for the host
   // between bursts even when a burst of the last few environments takes less
@@ -1054,7 +1106,6 @@ bool World<T>::cg_run(const CgConfig& cfg, const std::vector<int32_t>& solve, co
   // environments would not cover the rows its launches are counted for
   const bool timed0 = timing_ok && burst_timed(0);
   int q = 0;  // compactions issued: list parity q & 1
-  tail_used_ = false;
   if (max_env_a_ == 0)
     for (int e = 0; e < nEnv_; ++e)
       max_env_a_ = std::max(max_env_a_, tiles_a_.first[size_t(e) + 1] - tiles_a_.first[size_t(e)]);
@@ -2141,7 +2192,26 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
       c.spmv_lanes = L;
       consider(c);
     }
-  {
+  // the whole solve as one launch per solve, where every environment fits
+  // a workgroup's LDS; then its lanes per row, and the burst schedule only
+  // where the solve still runs as bursts
+  if (env_cg_fits()) {
+    for (int ec : {0, 1, 2})
+      if (ec != cur.env_cg) {
+        sdx_schedule c = cur;
+        c.env_cg = ec;
+        consider(c);
+      }
+    if (cur.env_cg > 0)
+      for (int L : {4, 8, 16})
+        if (L != cur.spmv_lanes) {
+          sdx_schedule c = cur;
+          c.spmv_lanes = L;
+          consider(c);
+        }
+  }
+  const bool bursts = !(cur.env_cg > 0 && env_cg_fits());
+  if (bursts) {
     sdx_schedule c = cur;
     c.active_lists = 1 - cur.active_lists;
     consider(c);
@@ -2154,13 +2224,13 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
     }
   for (int g : {1, 0})
     for (int B : {4, 8, 16, 32})
-      if (g != cur.use_graphs || B != cur.cg_burst) {
+      if (bursts && (g != cur.use_graphs || B != cur.cg_burst)) {
         sdx_schedule c = cur;
         c.use_graphs = g;
         c.cg_burst = B;
         consider(c);
       }
-  if (cur.active_lists && nEnv_ > 1) {
+  if (bursts && cur.active_lists && nEnv_ > 1) {
     const sdx_schedule base = cur;
     for (int ng : {1, 2, 3, 4, 6, 8})
       if (ng <= nEnv_ && ng != cg_groups(base)) {
@@ -2185,7 +2255,7 @@ double World<T>::autotune(double dt, const sdx_solver_params& sp_in, int trials)
   }
   if (nEnv_ > 1 && !cur.fused_asm) {  // assembly groups: one (every record kept), the automatic count, half and twice it
     const int a = asm_groups(cur);
-    std::vector<int> cand = {1, std::max(1, a / 2), asm_groups(sdx_schedule{0, 0, 0, 0, 0, 0, 0, 0, 0, 0}), 2 * a};
+    std::vector<int> cand = {1, std::max(1, a / 2), asm_groups(sdx_schedule{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}), 2 * a};
     for (int ag : cand)
       if (ag <= nEnv_ && ag != asm_groups(cur)) {
         sdx_schedule c = cur;

@@ -307,7 +307,8 @@ class WorldBase {
   virtual void restore_bytes(const uint8_t* in, size_t n, uint32_t excluded) = 0;
   const sdx_schedule& schedule() const { return sched_; }
 
-  // kernel timing: class 0 CG matvec, 1 CG update, 2 element, 3 BSR gather.
+  // kernel timing: class 0 CG matvec, 1 CG update, 2 element, 3 BSR gather,
+  // 4 whole-solve CG (env_cg).
   // on > 0: time every on-th launch of an assembly class, and every launch of
   // the first CG burst of every on-th solve (other bursts replay as graphs);
   // 0: off.
@@ -324,7 +325,7 @@ class WorldBase {
     flush_timers();
     return {double(timers_[size_t(k)].launches), timers_[size_t(k)].ms};
   }
-  sdx_schedule sched_{8, 1, 16, 128, 1, 0, 16, 0, 1, 0};
+  sdx_schedule sched_{8, 1, 16, 128, 1, 0, 16, 0, 1, 0, 1};
   bool tuned_ = false;  // the schedule was autotuned for the current topology
 
  protected:
@@ -362,6 +363,12 @@ class WorldBase {
     if (m != 0 || !(timers_[0].enabled || timers_[1].enabled)) return false;
     Timer& t = timers_[0];
     return (t.counter++ % t.every) == 0;
+  }
+  // Events of the whole-solve CG launch of every `every`-th solve.
+  const KernelEvents* env_timer_events() {
+    Timer& t = timers_[4];
+    if (!t.enabled || (t.counter++ % t.every) != 0) return nullptr;
+    return timer_events(4);
   }
   // Call after a stream synchronisation: folds recorded event pairs into totals.
   void flush_timers() {
@@ -417,7 +424,7 @@ class WorldBase {
   std::vector<HostCollider> colliders_;
   bool colliders_dirty_ = true;
   bool finalized_ = false;
-  Timer timers_[4];
+  Timer timers_[5];
 };
 
 template <class T>
@@ -573,6 +580,13 @@ class World final : public WorldBase {
   // matvec tiles of the largest environment (0: not yet counted), and the
   // persistent grid classes (grids of 1, 1/4, 1/16, 1/64 of the full size)
   int max_env_a_ = 0;
+  int env_max_rows_ = 0;   // rows of the largest environment
+  int num_cus_ = 0;        // compute units of the device
+  int env_lds_limit_ = 0;  // LDS bytes a workgroup may take on the device
+  // whether the environments fit the whole-solve CG, and whether this solve
+  // runs as one whole-solve launch (sdx_schedule.env_cg)
+  bool env_cg_fits() const;
+  bool env_cg_applies() const;
   static constexpr int kGridClasses = 4;
   // fused tail bursts (k_cg_tail): workgroups per launch, the barrier counter,
   // the flag a barrier sets when its wait runs out, and whether the current
@@ -672,6 +686,8 @@ class World final : public WorldBase {
   SDX_BUF(double, ws_beta_)      // and of the two previous Newton steps
   SDX_BUF(double, ws_gamma_)
   SDX_BUF(int32_t, d_env_v0_)    // first vertex of each environment [nEnv+1]
+  SDX_BUF(int32_t, d_env_order_)  // environments by rows, largest first (whole-solve CG queue)
+  SDX_BUF(int32_t, env_next_)     // whole-solve CG queue position
   SDX_BUF(int32_t, list_a_)      // work list of active matvec tiles
   SDX_BUF(int32_t, list_v_)      // work list of active vector tiles
   SDX_BUF(int32_t, list_count_)  // list lengths, two parities each

@@ -7,7 +7,7 @@
 - every launch schedule (matvec lanes, element workgroup, burst length,
   graph replay, persistent kernels over work lists of active tiles, the
   first iterations run environment group by group, tail bursts as one
-  cooperative launch) steps to the same state: displacements within 1e-3 of the
+  cooperative launch, the whole solve as one launch) steps to the same state: displacements within 1e-3 of the
   largest displacement (FP32 solves at rel. residual 1e-6 differ at the
   1e-4 level when only the summation order changes; a wrong result is off
   by O(1)), iteration counts within 2;
@@ -55,23 +55,28 @@ def state(sc):
 
 
 @pytest.mark.parametrize("sched", [
-    dict(use_graphs=0),
-    dict(spmv_lanes=4),
-    dict(spmv_lanes=8, cg_burst=3),
-    dict(element_block=64, cg_burst=32),
-    dict(element_block=256, use_graphs=0, cg_burst=1),
-    dict(active_lists=0),
-    dict(active_lists=0, use_graphs=0, spmv_lanes=8),
-    dict(cg_groups=3, group_iters=8, cg_burst=4),
-    dict(cg_groups=6, group_iters=5, cg_burst=2, use_graphs=0),  # 3 bursts per group: list parity flips
-    dict(cg_groups=4, group_iters=64, cg_burst=8, spmv_lanes=8),  # every iteration grouped
-    dict(fused_tail=1),  # tail bursts as one cooperative launch each
-    dict(fused_tail=1, cg_burst=16, spmv_lanes=16),
+    # the CG as bursts of two kernels per iteration (env_cg 0)
+    dict(env_cg=0, use_graphs=0),
+    dict(env_cg=0, spmv_lanes=4),
+    dict(env_cg=0, spmv_lanes=8, cg_burst=3),
+    dict(env_cg=0, element_block=64, cg_burst=32),
+    dict(env_cg=0, element_block=256, use_graphs=0, cg_burst=1),
+    dict(env_cg=0, active_lists=0),
+    dict(env_cg=0, active_lists=0, use_graphs=0, spmv_lanes=8),
+    dict(env_cg=0, cg_groups=3, group_iters=8, cg_burst=4),
+    dict(env_cg=0, cg_groups=6, group_iters=5, cg_burst=2, use_graphs=0),  # 3 bursts per group: list parity flips
+    dict(env_cg=0, cg_groups=4, group_iters=64, cg_burst=8, spmv_lanes=8),  # every iteration grouped
+    dict(env_cg=0, fused_tail=1),  # tail bursts as one cooperative launch each
+    dict(env_cg=0, fused_tail=1, cg_burst=16, spmv_lanes=16),
     dict(fused_asm=0, asm_groups=1),  # the record path: every environment's records kept
     dict(fused_asm=0, asm_groups=2),  # assembly groups share the record buffers
     dict(fused_asm=0, asm_groups=6, use_graphs=0),  # one environment per assembly group
-    dict(fused_asm=0, asm_groups=4, cg_groups=3, group_iters=8, cg_burst=4),
+    dict(env_cg=0, fused_asm=0, asm_groups=4, cg_groups=3, group_iters=8, cg_burst=4),
     dict(fused_asm=1, asm_groups=3),  # fused assembly (groups apply to residual-only assemblies)
+    # the whole solve as one launch, one workgroup per environment at a time
+    dict(env_cg=1, spmv_lanes=4),
+    dict(env_cg=2, spmv_lanes=8, cg_burst=3),
+    dict(env_cg=1, spmv_lanes=16, cg_groups=3, group_iters=8),  # burst fields do not apply
 ])
 def test_schedules_step_to_the_same_state(sched):
     # one Newton iteration per step: a second one starts from a residual at
@@ -108,7 +113,7 @@ def test_graph_replays_match_eager_step_by_step():
     scenes = []
     for graphs in (0, 1):
         sc, rigid = gs.build_scene(spec, precision=MIXED)
-        sc.set_schedule(use_graphs=graphs, cg_burst=4)
+        sc.set_schedule(env_cg=0, use_graphs=graphs, cg_burst=4)
         scenes.append((sc, rigid))
     for s in range(8):
         snaps = []
@@ -136,6 +141,7 @@ def test_autotune_restores_state_and_picks_a_valid_schedule():
         np.testing.assert_array_equal(v0, v1)
     assert ms > 0
     assert sched["spmv_lanes"] in (4, 8, 16) and sched["element_block"] in (64, 128, 256)
+    assert sched["env_cg"] in (0, 1, 2)
     assert sched == sc.schedule()
     # stepping on with the tuned schedule matches a scene that never tuned
     ref, rrig = gs.build_scene(spec, precision=MIXED)
