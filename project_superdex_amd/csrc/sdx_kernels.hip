@@ -1598,16 +1598,28 @@ __device__ __forceinline__ void env_block_sum(A (&v)[N], A* red) {
 // One environment's CG from the state k_cg_init left (r, z, x, the scale and
 // R_0 accumulators) to its stop decision; the same iterations, scalars and
 // stop tests as k_cg_spmv + k_cg_update, and the same EnvCG record at the end.
-template <class T, class A, int L, bool kC16>
+// kVec: r, p and s also live in LDS (rl, pl, sl), r loaded from r at the start.
+template <class T, class A, int L, bool kC16, bool kVec>
 __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConfig& cfg,
                                              const int32_t* __restrict__ row_ptr, const SpmvBufs& bf,
                                              const T* __restrict__ dinv, int nV, const T* __restrict__ b,
                                              const T* __restrict__ zin, A* __restrict__ x, T* __restrict__ r,
                                              T* __restrict__ p, T* __restrict__ s, double* __restrict__ rref,
                                              EnvCG* env, const A* __restrict__ acc, int32_t* num_active, Z4<T>* zl,
-                                             T* wl, A* red) {
+                                             T* wl, T* rl, T* pl, T* sl, A* red) {
   const int nsel = cfg.norm_type == 0 ? 0 : (cfg.norm_type == 1 ? 2 : 1);
-  for (int il = threadIdx.x; il < nr; il += kEnvThreads) zl[il] = reinterpret_cast<const Z4<T>*>(zin)[v0 + il];
+  for (int il = threadIdx.x; il < nr; il += kEnvThreads) {
+    zl[il] = reinterpret_cast<const Z4<T>*>(zin)[v0 + il];
+    if (kVec) {
+      rl[3 * il] = r[3 * (v0 + il)];
+      rl[3 * il + 1] = r[3 * (v0 + il) + 1];
+      rl[3 * il + 2] = r[3 * (v0 + il) + 2];
+    }
+  }
+  // where this thread's rows of r, p and s live: LDS (local row) or global (row)
+  T* const rv = kVec ? rl : r;
+  T* const pv = kVec ? pl : p;
+  T* const sv = kVec ? sl : s;
   // the scale and R_0 of k_cg_init, summed over the accumulator slots in slot order
   double start = 0.0, R[4] = {0.0, 0.0, 0.0, 0.0};
   {
@@ -1740,6 +1752,7 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
     A part[4] = {A(0), A(0), A(0), A(0)};  // r.r, r.z, z.z, r.z_old
     for (int il = threadIdx.x; il < nr; il += kEnvThreads) {
       const int i = v0 + il;
+      const int iv = kVec ? il : i;  // row in rv, pv, sv
       T d0 = T(0), d1 = T(0), d2 = T(0);
       A x0 = A(0), x1 = A(0), x2 = A(0);
       if (xd) {
@@ -1754,12 +1767,12 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
       }
       T p0 = T(0), p1 = T(0), p2 = T(0), s0 = T(0), s1 = T(0), s2 = T(0);
       if (k > 0) {
-        p0 = p[3 * i];
-        p1 = p[3 * i + 1];
-        p2 = p[3 * i + 2];
-        s0 = s[3 * i];
-        s1 = s[3 * i + 1];
-        s2 = s[3 * i + 2];
+        p0 = pv[3 * iv];
+        p1 = pv[3 * iv + 1];
+        p2 = pv[3 * iv + 2];
+        s0 = sv[3 * iv];
+        s1 = sv[3 * iv + 1];
+        s2 = sv[3 * iv + 2];
       }
       double ro0, ro1, ro2;
       if (refresh) {
@@ -1767,9 +1780,9 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
         ro1 = rref[3 * i + 1];
         ro2 = rref[3 * i + 2];
       } else {
-        ro0 = r[3 * i];
-        ro1 = r[3 * i + 1];
-        ro2 = r[3 * i + 2];
+        ro0 = rv[3 * iv];
+        ro1 = rv[3 * iv + 1];
+        ro2 = rv[3 * iv + 2];
       }
       Z4<T> zo = zl[il];
       const T w0 = wl[3 * il], w1 = wl[3 * il + 1], w2 = wl[3 * il + 2];
@@ -1779,12 +1792,12 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
       s0 = w0 + bt * s0;
       s1 = w1 + bt * s1;
       s2 = w2 + bt * s2;
-      p[3 * i] = p0;
-      p[3 * i + 1] = p1;
-      p[3 * i + 2] = p2;
-      s[3 * i] = s0;
-      s[3 * i + 1] = s1;
-      s[3 * i + 2] = s2;
+      pv[3 * iv] = p0;
+      pv[3 * iv + 1] = p1;
+      pv[3 * iv + 2] = p2;
+      sv[3 * iv] = s0;
+      sv[3 * iv + 1] = s1;
+      sv[3 * iv + 2] = s2;
       if (xd && !refresh) {
         xd[3 * i] = d0 + al * p0;
         xd[3 * i + 1] = d1 + al * p1;
@@ -1809,9 +1822,9 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
         r1 = T(ro1) - al * s1;
         r2 = T(ro2) - al * s2;
       }
-      r[3 * i] = r0;
-      r[3 * i + 1] = r1;
-      r[3 * i + 2] = r2;
+      rv[3 * iv] = r0;
+      rv[3 * iv + 1] = r1;
+      rv[3 * iv + 2] = r2;
       T n0, n1, n2;
       apply_dinv(dinv, nV, i, cfg.prec, r0, r1, r2, n0, n1, n2);
       Z4<T> zn;
@@ -1836,7 +1849,7 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
   }
 }
 
-template <class T, class A, int L, bool kC16>
+template <class T, class A, int L, bool kC16, bool kVec>
 __global__ void __launch_bounds__(kEnvThreads) k_cg_env(EnvQueue q, CgConfig cfg, const int32_t* __restrict__ row_ptr,
                                                         const void* __restrict__ col, const T* __restrict__ vals,
                                                         int nB, const T* __restrict__ dinv, int nV,
@@ -1846,8 +1859,11 @@ __global__ void __launch_bounds__(kEnvThreads) k_cg_env(EnvQueue q, CgConfig cfg
                                                         const A* __restrict__ acc, int32_t* num_active) {
   extern __shared__ __align__(16) unsigned char env_lds[];
   Z4<T>* zl = reinterpret_cast<Z4<T>*>(env_lds);
-  T* wl = reinterpret_cast<T*>(env_lds + env_lds_w_offset<T>(q.max_rows));
-  A* red = reinterpret_cast<A*>(env_lds + env_lds_red_offset<T>(q.max_rows));
+  T* wl = reinterpret_cast<T*>(env_lds + env_lds_plane_offset<T>(q.max_rows, 0));
+  T* rl = kVec ? reinterpret_cast<T*>(env_lds + env_lds_plane_offset<T>(q.max_rows, 1)) : nullptr;
+  T* pl = kVec ? reinterpret_cast<T*>(env_lds + env_lds_plane_offset<T>(q.max_rows, 2)) : nullptr;
+  T* sl = kVec ? reinterpret_cast<T*>(env_lds + env_lds_plane_offset<T>(q.max_rows, 3)) : nullptr;
+  A* red = reinterpret_cast<A*>(env_lds + env_lds_red_offset<T>(q.max_rows, kVec ? kEnvPlanesVec : kEnvPlanes));
   __shared__ int s_next;
   const SpmvBufs bf = spmv_bufs<T>(col, uint32_t(nB) * (kC16 ? 2u : 4u), vals, nB, zin, nV);
   for (;;) {
@@ -1866,8 +1882,8 @@ __global__ void __launch_bounds__(kEnvThreads) k_cg_env(EnvQueue q, CgConfig cfg
       }
       continue;
     }
-    cg_env_solve<T, A, L, kC16>(e, v0, nr, cfg, row_ptr, bf, dinv, nV, b, zin, x, r, p, s, rref, env, acc,
-                                num_active, zl, wl, red);
+    cg_env_solve<T, A, L, kC16, kVec>(e, v0, nr, cfg, row_ptr, bf, dinv, nV, b, zin, x, r, p, s, rref, env, acc,
+                                      num_active, zl, wl, rl, pl, sl, red);
     __syncthreads();  // the next environment reuses the LDS
   }
 }
@@ -2595,27 +2611,35 @@ void launch_cg_env(hipStream_t s, const EnvQueue& q, int grid, size_t lds_limit,
   if (q.n <= 0 || grid <= 0) return;
   if (q.max_rows <= 0 || !q.order || !q.next || !q.v0)
     throw std::invalid_argument("cg_env: empty queue description");
-  const size_t lds = env_lds_bytes<T, A>(q.max_rows);
-  if (lds > lds_limit || lds > size_t(kEnvLdsMax))
-    throw std::invalid_argument("cg_env: an environment's rows need more LDS than a workgroup may take");
+  const size_t lim = std::min(lds_limit, size_t(kEnvLdsMax));
+  const bool vec = env_lds_bytes<T, A>(q.max_rows, kEnvPlanesVec) <= lim;  // r, p and s in LDS too
+  const size_t lds = env_lds_bytes<T, A>(q.max_rows, vec ? kEnvPlanesVec : kEnvPlanes);
+  if (lds > lim) throw std::invalid_argument("cg_env: an environment's rows need more LDS than a workgroup may take");
   if (uint64_t(nB) * sizeof(T) >= (uint64_t(1) << 32) || uint64_t(cfg.n_rows) * 4 * sizeof(T) >= (uint64_t(1) << 32))
     throw std::invalid_argument("matrix too large for 32-bit buffer offsets of the CG matvec");
   const void* cp = col16 ? static_cast<const void*>(col16) : static_cast<const void*>(col);
   // above 64 KB a kernel's dynamic LDS must be allowed for it first
-#define SDX_ENV(L, C)                                                                                          \
-  do {                                                                                                         \
-    if (lds > size_t(64 * 1024)) {                                                                             \
-      const hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cg_env<T, A, L, C>),          \
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));          \
-      if (err != hipSuccess) throw std::runtime_error("cg_env: hipFuncSetAttribute refused the workgroup's LDS"); \
-    }                                                                                                          \
-    if (ev)                                                                                                    \
-      hipExtLaunchKernelGGL((k_cg_env<T, A, L, C>), dim3(grid), dim3(kEnvThreads), lds, s, ev->start, ev->stop, \
-                            0, q, cfg, row_ptr, cp, vals, nB, dinv, nV, b, z, x, r, p, sv, rref, env, acc,      \
-                            num_active);                                                                       \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_cg_env<T, A, L, C>), dim3(grid), dim3(kEnvThreads), lds, s, q, cfg, row_ptr, cp,    \
-                         vals, nB, dinv, nV, b, z, x, r, p, sv, rref, env, acc, num_active);                   \
+#define SDX_ENV1(L, C, V)                                                                                       \
+  do {                                                                                                          \
+    if (lds > size_t(64 * 1024)) {                                                                              \
+      const hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cg_env<T, A, L, C, V>),        \
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));           \
+      if (err != hipSuccess) throw std::runtime_error("cg_env: hipFuncSetAttribute refused the workgroup's LDS");  \
+    }                                                                                                           \
+    if (ev)                                                                                                     \
+      hipExtLaunchKernelGGL((k_cg_env<T, A, L, C, V>), dim3(grid), dim3(kEnvThreads), lds, s, ev->start,         \
+                            ev->stop, 0, q, cfg, row_ptr, cp, vals, nB, dinv, nV, b, z, x, r, p, sv, rref, env,  \
+                            acc, num_active);                                                                   \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_cg_env<T, A, L, C, V>), dim3(grid), dim3(kEnvThreads), lds, s, q, cfg, row_ptr, cp,  \
+                         vals, nB, dinv, nV, b, z, x, r, p, sv, rref, env, acc, num_active);                    \
+  } while (0)
+#define SDX_ENV(L, C)             \
+  do {                            \
+    if (vec)                      \
+      SDX_ENV1(L, C, true);       \
+    else                          \
+      SDX_ENV1(L, C, false);      \
   } while (0)
   switch (lanes) {
     case 4:
@@ -2629,6 +2653,7 @@ void launch_cg_env(hipStream_t s, const EnvQueue& q, int grid, size_t lds_limit,
       break;
   }
 #undef SDX_ENV
+#undef SDX_ENV1
 }
 
 void launch_compact_tiles(hipStream_t s, const Tiles& ta, const Tiles& tv, int a0, int a1, int v0, int v1,
