@@ -591,6 +591,7 @@ __device__ __forceinline__ void store_precond(const T b[9], int prec, T* __restr
 // 16-byte vector of T for staging copies: a native vector type, so an
 // unrolled array of them stays in registers (HIP's float4 struct does not).
 typedef float sdx_f4 __attribute__((ext_vector_type(4)));
+typedef float sdx_f2 __attribute__((ext_vector_type(2)));
 typedef double sdx_d2 __attribute__((ext_vector_type(2)));
 template <class T>
 struct Vec16;
@@ -1576,6 +1577,25 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, WorkList wl
 // an environment's matrix are re-read every iteration: with one or two
 // workgroups per CU, the matrices in flight stay within the last-level cache.
 
+// U consecutive floats at byte offset off of a buffer in one load (U = 4, 2
+// or 1; the offset need only be 4-byte aligned).
+template <int U>
+__device__ __forceinline__ void load_run(__amdgpu_buffer_rsrc_t r, uint32_t off, float (&v)[U]) {
+  if constexpr (U == 4) {
+    const sdx_f4 q = __builtin_bit_cast(sdx_f4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  } else if constexpr (U == 2) {
+    const sdx_f2 q = __builtin_bit_cast(sdx_f2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    v[0] = q.x;
+    v[1] = q.y;
+  } else {
+    v[0] = buf_ld<float>(r, off);
+  }
+}
+
 // Sum N values over the workgroup, in every thread, in a fixed order: wave
 // sums, then the waves' partials (LDS `red`, kEnvWaves * N) in wave order.
 template <int N, class A>
@@ -1667,41 +1687,70 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
       if (valid) {
         const int k0 = row_ptr[i], k1 = row_ptr[i + 1];
         constexpr int U = kSpmvChunk / L > 0 ? kSpmvChunk / L : 1;
-        for (int kb = k0 + lane; kb < k1; kb += U * L) {
-          int kk[U], jc[U];
-          T wv[U];
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const int kr = kb + u * L;
-            kk[u] = kr < k1 ? kr : k1 - 1;
-            wv[u] = kr < k1 ? T(1) : T(0);
-            if (kC16)
-              jc[u] = i + int(short(__builtin_amdgcn_raw_buffer_load_b16(bf.col, uint32_t(kk[u]) * 2u, 0, 0)));
-            else
-              jc[u] = buf_ld<int>(bf.col, uint32_t(kk[u]) * 4u);
+        // one block of the row: w_i += A_ij z_j (z from LDS), and on refresh
+        // iterations the FP64 product with the iterate x_j + xd_j
+        auto block = [&](T m0, T m1, T m2, T m3, T m4, T m5, T m6, T m7, T m8, int j) {
+          const Z4<T> zj = zl[j - v0];
+          a0 += m0 * zj.v[0] + m1 * zj.v[1] + m2 * zj.v[2];
+          a1 += m3 * zj.v[0] + m4 * zj.v[1] + m5 * zj.v[2];
+          a2 += m6 * zj.v[0] + m7 * zj.v[1] + m8 * zj.v[2];
+          if (refresh) {
+            double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+            if (xd) {
+              x0 += double(xd[3 * j]);
+              x1 += double(xd[3 * j + 1]);
+              x2 += double(xd[3 * j + 2]);
+            }
+            q0 += double(m0) * x0 + double(m1) * x1 + double(m2) * x2;
+            q1 += double(m3) * x0 + double(m4) * x1 + double(m5) * x2;
+            q2 += double(m6) * x0 + double(m7) * x1 + double(m8) * x2;
           }
+        };
+        if constexpr (sizeof(T) == 4) {
+          // runs of U L blocks: the lane takes U consecutive blocks of each
+          // run, one U-wide load per value plane; its slots past the row's
+          // end weigh zero and name the row itself (loads past the end of a
+          // buffer return zero)
+          for (int kb = k0 + lane * U; kb < k1; kb += U * L) {
+            float m[9][U];
 #pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const uint32_t o = uint32_t(kk[u]) * uint32_t(sizeof(T));
-            const T m0 = buf_ld<T>(bf.v[0], o) * wv[u], m1 = buf_ld<T>(bf.v[1], o) * wv[u];
-            const T m2 = buf_ld<T>(bf.v[2], o) * wv[u], m3 = buf_ld<T>(bf.v[3], o) * wv[u];
-            const T m4 = buf_ld<T>(bf.v[4], o) * wv[u], m5 = buf_ld<T>(bf.v[5], o) * wv[u];
-            const T m6 = buf_ld<T>(bf.v[6], o) * wv[u], m7 = buf_ld<T>(bf.v[7], o) * wv[u];
-            const T m8 = buf_ld<T>(bf.v[8], o) * wv[u];
-            const Z4<T> zj = zl[jc[u] - v0];
-            a0 += m0 * zj.v[0] + m1 * zj.v[1] + m2 * zj.v[2];
-            a1 += m3 * zj.v[0] + m4 * zj.v[1] + m5 * zj.v[2];
-            a2 += m6 * zj.v[0] + m7 * zj.v[1] + m8 * zj.v[2];
-            if (refresh) {
-              double x0 = x[3 * jc[u]], x1 = x[3 * jc[u] + 1], x2 = x[3 * jc[u] + 2];
-              if (xd) {
-                x0 += double(xd[3 * jc[u]]);
-                x1 += double(xd[3 * jc[u] + 1]);
-                x2 += double(xd[3 * jc[u] + 2]);
-              }
-              q0 += double(m0) * x0 + double(m1) * x1 + double(m2) * x2;
-              q1 += double(m3) * x0 + double(m4) * x1 + double(m5) * x2;
-              q2 += double(m6) * x0 + double(m7) * x1 + double(m8) * x2;
+            for (int c = 0; c < 9; ++c) load_run<U>(bf.v[c], uint32_t(kb) * 4u, m[c]);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const bool in = kb + u < k1;
+              const T wu = in ? T(1) : T(0);
+              int j;
+              if (kC16)
+                j = i + int(short(__builtin_amdgcn_raw_buffer_load_b16(bf.col, uint32_t(kb + u) * 2u, 0, 0)));
+              else
+                j = buf_ld<int>(bf.col, uint32_t(kb + u) * 4u);
+              block(m[0][u] * wu, m[1][u] * wu, m[2][u] * wu, m[3][u] * wu, m[4][u] * wu, m[5][u] * wu,
+                    m[6][u] * wu, m[7][u] * wu, m[8][u] * wu, in ? j : i);
+            }
+          }
+        } else {
+          // FP64 values: a lane's blocks strided by L, clamped to the row's
+          // last block (weighted zero) past its end
+          for (int kb = k0 + lane; kb < k1; kb += U * L) {
+            int kk[U], jc[U];
+            T wv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const int kr = kb + u * L;
+              kk[u] = kr < k1 ? kr : k1 - 1;
+              wv[u] = kr < k1 ? T(1) : T(0);
+              if (kC16)
+                jc[u] = i + int(short(__builtin_amdgcn_raw_buffer_load_b16(bf.col, uint32_t(kk[u]) * 2u, 0, 0)));
+              else
+                jc[u] = buf_ld<int>(bf.col, uint32_t(kk[u]) * 4u);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const uint32_t o = uint32_t(kk[u]) * uint32_t(sizeof(T));
+              block(buf_ld<T>(bf.v[0], o) * wv[u], buf_ld<T>(bf.v[1], o) * wv[u], buf_ld<T>(bf.v[2], o) * wv[u],
+                    buf_ld<T>(bf.v[3], o) * wv[u], buf_ld<T>(bf.v[4], o) * wv[u], buf_ld<T>(bf.v[5], o) * wv[u],
+                    buf_ld<T>(bf.v[6], o) * wv[u], buf_ld<T>(bf.v[7], o) * wv[u], buf_ld<T>(bf.v[8], o) * wv[u],
+                    jc[u]);
             }
           }
         }
