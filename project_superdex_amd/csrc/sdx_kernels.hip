@@ -2046,6 +2046,41 @@ __global__ void __launch_bounds__(256) k_compact_tiles(Tiles ta, Tiles tv, int a
   if (keep) list[off + __popcll(m & ((1ull << lane) - 1ull))] = tile;
 }
 
+// The blocks k0..k1-1 of one row as lane `lane` of the row's L lanes takes
+// them: f(a, j) with the block's 9 values in FP64 and its column j.  FP32
+// values: runs of U L blocks (U = kSpmvChunk / L), the lane's U consecutive
+// blocks of a run read with one U-wide load per value plane and one for the
+// columns (bf: descriptors of the planes and the 32-bit columns); FP64
+// values: one load per value, blocks strided by L.
+template <class T, int L, class F>
+__device__ __forceinline__ void row_blocks(const SpmvBufs& bf, const int32_t* __restrict__ col,
+                                           const T* __restrict__ vals, int nB, int k0, int k1, int lane, F&& f) {
+  if constexpr (sizeof(T) == 4) {
+    constexpr int U = kSpmvChunk / L > 0 ? kSpmvChunk / L : 1;
+    for (int kb = k0 + lane * U; kb < k1; kb += U * L) {
+      float m[9][U], cj[U];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) load_run<U>(bf.v[c], uint32_t(kb) * 4u, m[c]);
+      load_run<U>(bf.col, uint32_t(kb) * 4u, cj);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (kb + u < k1) {
+          double a[9];
+#pragma unroll
+          for (int c = 0; c < 9; ++c) a[c] = double(m[c][u]);
+          f(a, __float_as_int(cj[u]));
+        }
+    }
+  } else {
+    for (int kk = k0 + lane; kk < k1; kk += L) {
+      double a[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) a[c] = double(vals[(size_t)c * nB + kk]);
+      f(a, col[kk]);
+    }
+  }
+}
+
 // FP64 residual b - A x of the environments with mask[e]: stored in r64
 // when given, and |b - A x|^2, |b|^2, b.Ax, |Ax|^2 added into set 0 of acc
 // when given (zeroed by the caller).
@@ -2064,15 +2099,13 @@ __global__ void __launch_bounds__(kSpmvThreads) k_residual64(Tiles ta, int nEnv,
   const bool valid = i < ta.row1[tile];
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;
   if (valid) {
-    for (int kk = row_ptr[i] + lane; kk < row_ptr[i + 1]; kk += L) {
-      const int jc = col[kk];
+    const SpmvBufs bf = spmv_bufs<T>(col, uint32_t(nB) * 4u, vals, nB, vals, 0);
+    row_blocks<T, L>(bf, col, vals, nB, row_ptr[i], row_ptr[i + 1], lane, [&](const double (&a)[9], int jc) {
       const double x0 = x[3 * jc], x1 = x[3 * jc + 1], x2 = x[3 * jc + 2];
-      q0 += double(vals[kk]) * x0 + double(vals[(size_t)nB + kk]) * x1 + double(vals[2 * (size_t)nB + kk]) * x2;
-      q1 += double(vals[3 * (size_t)nB + kk]) * x0 + double(vals[4 * (size_t)nB + kk]) * x1 +
-            double(vals[5 * (size_t)nB + kk]) * x2;
-      q2 += double(vals[6 * (size_t)nB + kk]) * x0 + double(vals[7 * (size_t)nB + kk]) * x1 +
-            double(vals[8 * (size_t)nB + kk]) * x2;
-    }
+      q0 += a[0] * x0 + a[1] * x1 + a[2] * x2;
+      q1 += a[3] * x0 + a[4] * x1 + a[5] * x2;
+      q2 += a[6] * x0 + a[7] * x1 + a[8] * x2;
+    });
   }
   q0 = group_sum<L>(q0);
   q1 = group_sum<L>(q1);
@@ -2116,11 +2149,8 @@ __global__ void __launch_bounds__(kSpmvThreads) k_residual64_multi(Tiles ta, int
   const bool valid = i < ta.row1[tile];
   double p[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};  // A u, A v, A w rows
   if (valid) {
-    for (int kk = row_ptr[i] + lane; kk < row_ptr[i + 1]; kk += L) {
-      const int jc = col[kk];
-      double a[9];
-#pragma unroll
-      for (int c = 0; c < 9; ++c) a[c] = double(vals[(size_t)c * nB + kk]);
+    const SpmvBufs bf = spmv_bufs<T>(col, uint32_t(nB) * 4u, vals, nB, vals, 0);
+    row_blocks<T, L>(bf, col, vals, nB, row_ptr[i], row_ptr[i + 1], lane, [&](const double (&a)[9], int jc) {
       const double g[3][3] = {{double(u[3 * jc]), double(u[3 * jc + 1]), double(u[3 * jc + 2])},
                               {double(v[3 * jc]), double(v[3 * jc + 1]), double(v[3 * jc + 2])},
                               {double(w[3 * jc]), double(w[3 * jc + 1]), double(w[3 * jc + 2])}};
@@ -2128,7 +2158,7 @@ __global__ void __launch_bounds__(kSpmvThreads) k_residual64_multi(Tiles ta, int
       for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int r = 0; r < 3; ++r) p[q][r] += a[3 * r] * g[q][0] + a[3 * r + 1] * g[q][1] + a[3 * r + 2] * g[q][2];
-    }
+    });
   }
 #pragma unroll
   for (int q = 0; q < 3; ++q)
@@ -2753,6 +2783,8 @@ void launch_residual64(hipStream_t s, const Tiles& ta, int lanes, int nEnv, cons
                        const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* x,
                        const T* b, double* r64, double* acc) {
   if (ta.n == 0) return;
+  if (uint64_t(nB) * sizeof(T) >= (uint64_t(1) << 32))
+    throw std::invalid_argument("matrix too large for 32-bit buffer offsets of the FP64 residual");
   auto kern = lanes == 4   ? k_residual64<T, X, 4>
               : lanes == 8 ? k_residual64<T, X, 8>
                            : k_residual64<T, X, 16>;
@@ -2765,6 +2797,8 @@ void launch_residual64_multi(hipStream_t s, const Tiles& ta, int lanes, int nEnv
                              const int32_t* row_ptr, const int32_t* col, const T* vals, int32_t nB, const X* u,
                              const T* v, const T* w, const T* b, double* ru, double* rv, double* rw, double* acc) {
   if (ta.n == 0) return;
+  if (uint64_t(nB) * sizeof(T) >= (uint64_t(1) << 32))
+    throw std::invalid_argument("matrix too large for 32-bit buffer offsets of the FP64 residual");
   auto kern = lanes == 4   ? k_residual64_multi<T, X, 4>
               : lanes == 8 ? k_residual64_multi<T, X, 8>
                            : k_residual64_multi<T, X, 16>;
