@@ -1054,6 +1054,25 @@ __device__ __forceinline__ void buf_ld_z(__amdgpu_buffer_rsrc_t r, int j, double
   c = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, uint32_t(j) * 32u + 16u, 0, 0));
 }
 
+// U consecutive floats at byte offset off of a buffer in one load (U = 4, 2
+// or 1; the offset need only be 4-byte aligned).
+template <int U>
+__device__ __forceinline__ void load_run(__amdgpu_buffer_rsrc_t r, uint32_t off, float (&v)[U]) {
+  if constexpr (U == 4) {
+    const sdx_f4 q = __builtin_bit_cast(sdx_f4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  } else if constexpr (U == 2) {
+    const sdx_f2 q = __builtin_bit_cast(sdx_f2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    v[0] = q.x;
+    v[1] = q.y;
+  } else {
+    v[0] = buf_ld<float>(r, off);
+  }
+}
+
 // Descriptors of the matvec operands: the 9 value planes, the column
 // indices and z (each below 4 GB; checked by the launcher).
 struct SpmvBufs {
@@ -1261,47 +1280,73 @@ __device__ __forceinline__ void cg_spmv_tile(int tile, const Tiles& ta, const Cg
   T zi0 = T(0), zi1 = T(0), zi2 = T(0);
   if (valid) {
     const int k0 = row_ptr[i], k1 = row_ptr[i + 1];
-    // a lane's blocks of a chunk of U L blocks are loaded together: indices
-    // past the row end are clamped to its last block (always valid, a row
-    // holds its diagonal) and weighted by zero, so no load waits on a branch
     constexpr int U = kSpmvChunk / L > 0 ? kSpmvChunk / L : 1;
-    for (int kb = k0 + lane; kb < k1; kb += U * L) {
-      int kk[U], jc[U];
-      T wv[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int kr = kb + u * L;
-        kk[u] = kr < k1 ? kr : k1 - 1;
-        wv[u] = kr < k1 ? T(1) : T(0);
-        if (kC16)  // column as a signed 16-bit offset from the row
-          jc[u] = i + int(short(__builtin_amdgcn_raw_buffer_load_b16(bf.col, uint32_t(kk[u]) * 2u, 0, 0)));
-        else
-          jc[u] = buf_ld<int>(bf.col, uint32_t(kk[u]) * 4u);
+    // one block: w_i += A_ij z_j (one 16-byte gather of z_j), and on refresh
+    // iterations the FP64 product with the iterate x_j + xd_j
+    auto block = [&](T v0, T v1, T v2, T v3, T v4, T v5, T v6, T v7, T v8, int j) {
+      T zj0, zj1, zj2;
+      buf_ld_z(bf.z, j, zj0, zj1, zj2);  // FP64: 24 of 32 bytes
+      a0 += v0 * zj0 + v1 * zj1 + v2 * zj2;
+      a1 += v3 * zj0 + v4 * zj1 + v5 * zj2;
+      a2 += v6 * zj0 + v7 * zj1 + v8 * zj2;
+      if (kRefresh) {
+        double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+        if (cfg.xd) {  // the increments not yet folded into x
+          const T* xd = static_cast<const T*>(cfg.xd);
+          x0 += double(xd[3 * j]);
+          x1 += double(xd[3 * j + 1]);
+          x2 += double(xd[3 * j + 2]);
+        }
+        q0 += double(v0) * x0 + double(v1) * x1 + double(v2) * x2;
+        q1 += double(v3) * x0 + double(v4) * x1 + double(v5) * x2;
+        q2 += double(v6) * x0 + double(v7) * x1 + double(v8) * x2;
       }
+    };
+    if constexpr (sizeof(T) == 4) {
+      // runs of U L blocks: the lane takes U consecutive blocks of each run,
+      // one U-wide load per value plane; its slots past the row's end weigh
+      // zero and name the row itself (loads past the end of a buffer return
+      // zero)
+      for (int kb = k0 + lane * U; kb < k1; kb += U * L) {
+        float m[9][U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint32_t o = uint32_t(kk[u]) * uint32_t(sizeof(T));
-        const T v0 = buf_ld<T>(bf.v[0], o) * wv[u], v1 = buf_ld<T>(bf.v[1], o) * wv[u];
-        const T v2 = buf_ld<T>(bf.v[2], o) * wv[u], v3 = buf_ld<T>(bf.v[3], o) * wv[u];
-        const T v4 = buf_ld<T>(bf.v[4], o) * wv[u], v5 = buf_ld<T>(bf.v[5], o) * wv[u];
-        const T v6 = buf_ld<T>(bf.v[6], o) * wv[u], v7 = buf_ld<T>(bf.v[7], o) * wv[u];
-        const T v8 = buf_ld<T>(bf.v[8], o) * wv[u];
-        T zj0, zj1, zj2;
-        buf_ld_z(bf.z, jc[u], zj0, zj1, zj2);  // one 16-byte (FP64: 24 of 32 bytes) gather
-        a0 += v0 * zj0 + v1 * zj1 + v2 * zj2;
-        a1 += v3 * zj0 + v4 * zj1 + v5 * zj2;
-        a2 += v6 * zj0 + v7 * zj1 + v8 * zj2;
-        if (kRefresh) {
-          double x0 = x[3 * jc[u]], x1 = x[3 * jc[u] + 1], x2 = x[3 * jc[u] + 2];
-          if (cfg.xd) {  // the increments not yet folded into x
-            const T* xd = static_cast<const T*>(cfg.xd);
-            x0 += double(xd[3 * jc[u]]);
-            x1 += double(xd[3 * jc[u] + 1]);
-            x2 += double(xd[3 * jc[u] + 2]);
-          }
-          q0 += double(v0) * x0 + double(v1) * x1 + double(v2) * x2;
-          q1 += double(v3) * x0 + double(v4) * x1 + double(v5) * x2;
-          q2 += double(v6) * x0 + double(v7) * x1 + double(v8) * x2;
+        for (int c = 0; c < 9; ++c) load_run<U>(bf.v[c], uint32_t(kb) * 4u, m[c]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool in = kb + u < k1;
+          const T wu = in ? T(1) : T(0);
+          int j;
+          if (kC16)
+            j = i + int(short(__builtin_amdgcn_raw_buffer_load_b16(bf.col, uint32_t(kb + u) * 2u, 0, 0)));
+          else
+            j = buf_ld<int>(bf.col, uint32_t(kb + u) * 4u);
+          block(m[0][u] * wu, m[1][u] * wu, m[2][u] * wu, m[3][u] * wu, m[4][u] * wu, m[5][u] * wu, m[6][u] * wu,
+                m[7][u] * wu, m[8][u] * wu, in ? j : i);
+        }
+      }
+    } else {
+      // FP64 values: a lane's blocks strided by L, indices past the row end
+      // clamped to its last block (a row holds its diagonal) and weighted
+      // by zero, so no load waits on a branch
+      for (int kb = k0 + lane; kb < k1; kb += U * L) {
+        int kk[U], jc[U];
+        T wv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int kr = kb + u * L;
+          kk[u] = kr < k1 ? kr : k1 - 1;
+          wv[u] = kr < k1 ? T(1) : T(0);
+          if (kC16)
+            jc[u] = i + int(short(__builtin_amdgcn_raw_buffer_load_b16(bf.col, uint32_t(kk[u]) * 2u, 0, 0)));
+          else
+            jc[u] = buf_ld<int>(bf.col, uint32_t(kk[u]) * 4u);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t o = uint32_t(kk[u]) * uint32_t(sizeof(T));
+          block(buf_ld<T>(bf.v[0], o) * wv[u], buf_ld<T>(bf.v[1], o) * wv[u], buf_ld<T>(bf.v[2], o) * wv[u],
+                buf_ld<T>(bf.v[3], o) * wv[u], buf_ld<T>(bf.v[4], o) * wv[u], buf_ld<T>(bf.v[5], o) * wv[u],
+                buf_ld<T>(bf.v[6], o) * wv[u], buf_ld<T>(bf.v[7], o) * wv[u], buf_ld<T>(bf.v[8], o) * wv[u], jc[u]);
         }
       }
     }
@@ -1576,25 +1621,6 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(Tiles tv, WorkList wl
 // and nothing waits on another workgroup or on a kernel boundary.  The rows of
 // an environment's matrix are re-read every iteration: with one or two
 // workgroups per CU, the matrices in flight stay within the last-level cache.
-
-// U consecutive floats at byte offset off of a buffer in one load (U = 4, 2
-// or 1; the offset need only be 4-byte aligned).
-template <int U>
-__device__ __forceinline__ void load_run(__amdgpu_buffer_rsrc_t r, uint32_t off, float (&v)[U]) {
-  if constexpr (U == 4) {
-    const sdx_f4 q = __builtin_bit_cast(sdx_f4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-    v[0] = q.x;
-    v[1] = q.y;
-    v[2] = q.z;
-    v[3] = q.w;
-  } else if constexpr (U == 2) {
-    const sdx_f2 q = __builtin_bit_cast(sdx_f2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-    v[0] = q.x;
-    v[1] = q.y;
-  } else {
-    v[0] = buf_ld<float>(r, off);
-  }
-}
 
 // Sum N values over the workgroup, in every thread, in a fixed order: wave
 // sums, then the waves' partials (LDS `red`, kEnvWaves * N) in wave order.
