@@ -15,6 +15,7 @@ of spread atomic accumulators, so the two differ only in summation order:
   FP64 true residual is at most 1e-6), and the burst schedule within
   1e-3 of the largest displacement with CG counts within 2;
 - an iteration budget (max_iter) stops every solve as the bursts do;
+- mixed precision without refreshes keeps the iterate in FP64 (the FP64 oracle's steps);
 - a world whose largest environment needs more LDS than a workgroup may take
   runs the bursts (no whole-solve launch) and steps as the bursts do; an
   environment whose z and w need more than 64 KB of LDS steps as the bursts;
@@ -165,3 +166,31 @@ def test_env_cg_large_environment_matches_the_bursts():
     u1, _ = two_cubes(14, 3, 1)
     for a, b in zip(u0, u1):
         assert np.abs(a - b).max() <= 1e-3 * np.abs(a).max()
+
+
+def test_env_cg_mixed_without_refresh_keeps_an_fp64_iterate(oracle):
+    """Mixed precision with refresh_every = 0 (the library default) on the whole-solve path: every
+    update adds into the FP64 iterate (no FP32 increments), so batched environments match the FP64
+    oracle within 1e-4 relative positions and 2e-3 of the largest displacement, and with the
+    ResidualL2 norm every solve ends at an FP64 true residual <= 1e-6 without a refresh."""
+    print("whole-solve CG, mixed precision without refreshes", flush=True)
+    spec = mesh.batched_indent_scene(num_envs=6, seed=11, n_min=3, n_max=7, steps=6, speed=0.2)
+    p64 = _abi.pinned_solver_params(1, lin_rel_tol=1e-10, precision_mode=FP64)
+    so, _ = oracle.run_scene(spec, p64, 6, fp64=True)
+    for norm in (_abi.NORM_PRECONDITIONED_RESIDUAL_L2, _abi.NORM_RESIDUAL_L2):
+        p = _abi.pinned_solver_params(1, precision_mode=MIXED, refresh_every=0)
+        p.linear.norm_type = norm
+        sc, rigid = gs.build_scene(spec, precision=MIXED)
+        sc.set_schedule(env_cg=1)
+        sc.kernel_timing(ENV_CG_TIMER, enable=1)
+        for s in range(6):
+            for r, c in zip(rigid, spec.colliders):
+                r.set_position(c.position(s, spec.dt))
+            st = sc.step(spec.dt, p)
+            for e in st:
+                assert e["linear_convergence"] == _abi.CONV_CONVERGED, e
+                assert e["fp64_refreshes"] == 0, e
+                if norm == _abi.NORM_RESIDUAL_L2:
+                    assert e["linear_true_rel_res"] <= 1e-6, e
+        assert sc.kernel_timing(ENV_CG_TIMER)[0] >= 6
+        compare(spec, sc, so, 2e-3)
