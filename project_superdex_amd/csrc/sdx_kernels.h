@@ -96,21 +96,22 @@ struct EnvQueue {
   int32_t* next;         // queue position, zero before the launch
   const int32_t* v0;     // [nEnv + 1] first row of each environment
 };
-// LDS of a workgroup: z as Z4 [max_rows], then `planes` vectors of 3 max_rows
-// values (w; with the CG vectors in LDS also r, p and s), then the two
+// LDS of a workgroup: z [max_rows] (as Z4, or as 3 values with the CG
+// vectors in LDS), then `planes` vectors of 3 max_rows values (w; with the CG
+// vectors in LDS also r, p, s and the iterate's increments), then the two
 // reduction areas (kEnvWaves partials of 1 and of 4 sums).
-constexpr int kEnvPlanes = 1, kEnvPlanesVec = 4;
+constexpr int kEnvPlanes = 1, kEnvPlanesVec = 5;
 template <class T>
-__host__ __device__ inline size_t env_lds_plane_offset(int max_rows, int plane) {
-  return size_t(max_rows) * 4 * sizeof(T) + size_t(plane) * size_t(max_rows) * 3 * sizeof(T);
+__host__ __device__ inline size_t env_lds_plane_offset(int max_rows, int plane, bool vec) {
+  return size_t(max_rows) * (vec ? 3 : 4) * sizeof(T) + size_t(plane) * size_t(max_rows) * 3 * sizeof(T);
 }
 template <class T>
-__host__ __device__ inline size_t env_lds_red_offset(int max_rows, int planes) {
-  return (env_lds_plane_offset<T>(max_rows, planes) + 15) & ~size_t(15);
+__host__ __device__ inline size_t env_lds_red_offset(int max_rows, bool vec) {
+  return (env_lds_plane_offset<T>(max_rows, vec ? kEnvPlanesVec : kEnvPlanes, vec) + 15) & ~size_t(15);
 }
 template <class T, class A>
-__host__ __device__ inline size_t env_lds_bytes(int max_rows, int planes) {
-  return env_lds_red_offset<T>(max_rows, planes) + size_t(kEnvWaves) * 5 * sizeof(A);
+__host__ __device__ inline size_t env_lds_bytes(int max_rows, bool vec) {
+  return env_lds_red_offset<T>(max_rows, vec) + size_t(kEnvWaves) * 5 * sizeof(A);
 }
 
 // Geometry planes of a tet, stored SoA with stride nT.
@@ -303,9 +304,10 @@ int env_lds_limit();
 
 // The whole CG of every active environment of q in one launch of `grid`
 // workgroups (k_cg_env): the iterations of k_cg_spmv + k_cg_update from the
-// state k_cg_init left, each environment's EnvCG record at the end.  r, p and
-// s live in LDS too where q.max_rows allows it within lds_limit (they are
-// then not written back: the next solve starts from k_cg_init).  Throws if
+// state k_cg_init left, each environment's EnvCG record at the end.  r, p, s
+// and the iterate's increments live in LDS too where q.max_rows allows it
+// within lds_limit (r, p and s are then not written back, the next solve
+// starting from k_cg_init; the increments are folded into x at the end).  Throws if
 // q.max_rows needs more than lds_limit bytes of LDS even without them.
 template <class T, class A>
 void launch_cg_env(hipStream_t s, const EnvQueue& q, int grid, size_t lds_limit, const CgConfig& cfg, int lanes,

@@ -1652,10 +1652,33 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
                                              const T* __restrict__ zin, A* __restrict__ x, T* __restrict__ r,
                                              T* __restrict__ p, T* __restrict__ s, double* __restrict__ rref,
                                              EnvCG* env, const A* __restrict__ acc, int32_t* num_active, Z4<T>* zl,
-                                             T* wl, T* rl, T* pl, T* sl, A* red) {
+                                             T* z3, T* wl, T* rl, T* pl, T* sl, T* xl, A* red) {
   const int nsel = cfg.norm_type == 0 ? 0 : (cfg.norm_type == 1 ? 2 : 1);
+  T* xd = static_cast<T*>(cfg.xd);
+  // z of local row k: Z4 entries, or 3 values per row with the CG vectors in LDS
+  auto zget = [&](int k) {
+    if (kVec) {
+      Z4<T> q;
+      q.v[0] = z3[3 * k];
+      q.v[1] = z3[3 * k + 1];
+      q.v[2] = z3[3 * k + 2];
+      q.v[3] = T(0);
+      return q;
+    }
+    return zl[k];
+  };
   for (int il = threadIdx.x; il < nr; il += kEnvThreads) {
-    zl[il] = reinterpret_cast<const Z4<T>*>(zin)[v0 + il];
+    const Z4<T> q = reinterpret_cast<const Z4<T>*>(zin)[v0 + il];
+    if (kVec) {
+      z3[3 * il] = q.v[0];
+      z3[3 * il + 1] = q.v[1];
+      z3[3 * il + 2] = q.v[2];
+      xl[3 * il] = T(0);
+      xl[3 * il + 1] = T(0);
+      xl[3 * il + 2] = T(0);
+    } else {
+      zl[il] = q;
+    }
     if (kVec) {
       rl[3 * il] = r[3 * (v0 + il)];
       rl[3 * il + 1] = r[3 * (v0 + il) + 1];
@@ -1681,7 +1704,20 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
 #pragma unroll
     for (int c = 0; c < 4; ++c) R[c] = double(r4[c]);
   }
-  T* xd = static_cast<T*>(cfg.xd);
+  // the iterate's increments of row i (local row il): LDS with the CG vectors there
+  T* const xdv = kVec ? xl : xd;
+  const bool has_xd = xd != nullptr;
+  // at the end of the solve, with the increments in LDS: this thread's rows (the
+  // update's) fold theirs into x
+  auto fold = [&]() {
+    if (kVec && has_xd)
+      for (int il = threadIdx.x; il < nr; il += kEnvThreads) {
+        const int i = v0 + il;
+        x[3 * i] += A(xl[3 * il]);
+        x[3 * i + 1] += A(xl[3 * il + 1]);
+        x[3 * i + 2] += A(xl[3 * il + 2]);
+      }
+  };
   const int sub = threadIdx.x / L, lane = threadIdx.x % L;
   double g_prev = 0.0, a_prev = 0.0, m_prev = 0.0;
   __syncthreads();  // z in LDS
@@ -1689,6 +1725,7 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
     const double latest = R[nsel], gamma = R[1], rho = R[3];
     const int st = cg_decide(cfg, e, k, start, latest, gamma);
     if (st != kCgActive) {
+      fold();
       if (threadIdx.x == 0) {
         EnvCG& E = env[e];
         E.status = st;
@@ -1716,16 +1753,17 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
         // one block of the row: w_i += A_ij z_j (z from LDS), and on refresh
         // iterations the FP64 product with the iterate x_j + xd_j
         auto block = [&](T m0, T m1, T m2, T m3, T m4, T m5, T m6, T m7, T m8, int j) {
-          const Z4<T> zj = zl[j - v0];
+          const Z4<T> zj = zget(j - v0);
           a0 += m0 * zj.v[0] + m1 * zj.v[1] + m2 * zj.v[2];
           a1 += m3 * zj.v[0] + m4 * zj.v[1] + m5 * zj.v[2];
           a2 += m6 * zj.v[0] + m7 * zj.v[1] + m8 * zj.v[2];
           if (refresh) {
             double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
-            if (xd) {
-              x0 += double(xd[3 * j]);
-              x1 += double(xd[3 * j + 1]);
-              x2 += double(xd[3 * j + 2]);
+            if (has_xd) {
+              const int jv = kVec ? j - v0 : j;
+              x0 += double(xdv[3 * jv]);
+              x1 += double(xdv[3 * jv + 1]);
+              x2 += double(xdv[3 * jv + 2]);
             }
             q0 += double(m0) * x0 + double(m1) * x1 + double(m2) * x2;
             q1 += double(m3) * x0 + double(m4) * x1 + double(m5) * x2;
@@ -1793,7 +1831,7 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
         wl[3 * il] = a0;
         wl[3 * il + 1] = a1;
         wl[3 * il + 2] = a2;
-        const Z4<T> zi = zl[il];
+        const Z4<T> zi = zget(il);
         dz[0] += A(zi.v[0]) * A(a0) + A(zi.v[1]) * A(a1) + A(zi.v[2]) * A(a2);
         if (refresh) {
           rref[3 * i] = double(b[3 * i]) - q0;
@@ -1810,6 +1848,7 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
       mu = delta + 2.0 * beta * (rho - gamma) / a_prev + beta * beta * m_prev;
     }
     if (mu <= 0.0 && cfg.abort_if_not_spd) {  // pcg.h:152-169: not SPD, abort with iteration k + 1
+      fold();
       if (threadIdx.x == 0) {
         EnvCG& E = env[e];
         E.status = kCgDiverged;
@@ -1830,12 +1869,12 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
       const int iv = kVec ? il : i;  // row in rv, pv, sv
       T d0 = T(0), d1 = T(0), d2 = T(0);
       A x0 = A(0), x1 = A(0), x2 = A(0);
-      if (xd) {
-        d0 = xd[3 * i];
-        d1 = xd[3 * i + 1];
-        d2 = xd[3 * i + 2];
+      if (has_xd) {
+        d0 = xdv[3 * iv];
+        d1 = xdv[3 * iv + 1];
+        d2 = xdv[3 * iv + 2];
       }
-      if (!xd || refresh) {
+      if (!has_xd || refresh) {
         x0 = x[3 * i];
         x1 = x[3 * i + 1];
         x2 = x[3 * i + 2];
@@ -1859,7 +1898,7 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
         ro1 = rv[3 * iv + 1];
         ro2 = rv[3 * iv + 2];
       }
-      Z4<T> zo = zl[il];
+      Z4<T> zo = zget(il);
       const T w0 = wl[3 * il], w1 = wl[3 * il + 1], w2 = wl[3 * il + 2];
       p0 = zo.v[0] + bt * p0;
       p1 = zo.v[1] + bt * p1;
@@ -1873,18 +1912,18 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
       sv[3 * iv] = s0;
       sv[3 * iv + 1] = s1;
       sv[3 * iv + 2] = s2;
-      if (xd && !refresh) {
-        xd[3 * i] = d0 + al * p0;
-        xd[3 * i + 1] = d1 + al * p1;
-        xd[3 * i + 2] = d2 + al * p2;
+      if (has_xd && !refresh) {
+        xdv[3 * iv] = d0 + al * p0;
+        xdv[3 * iv + 1] = d1 + al * p1;
+        xdv[3 * iv + 2] = d2 + al * p2;
       } else {
         x[3 * i] = x0 + A(d0) + A(alpha) * A(p0);
         x[3 * i + 1] = x1 + A(d1) + A(alpha) * A(p1);
         x[3 * i + 2] = x2 + A(d2) + A(alpha) * A(p2);
-        if (xd) {
-          xd[3 * i] = T(0);
-          xd[3 * i + 1] = T(0);
-          xd[3 * i + 2] = T(0);
+        if (has_xd) {
+          xdv[3 * iv] = T(0);
+          xdv[3 * iv + 1] = T(0);
+          xdv[3 * iv + 2] = T(0);
         }
       }
       T r0, r1, r2;
@@ -1907,7 +1946,13 @@ __device__ __forceinline__ void cg_env_solve(int e, int v0, int nr, const CgConf
       zn.v[1] = n1;
       zn.v[2] = n2;
       zn.v[3] = T(0);
-      zl[il] = zn;
+      if (kVec) {
+        z3[3 * il] = n0;
+        z3[3 * il + 1] = n1;
+        z3[3 * il + 2] = n2;
+      } else {
+        zl[il] = zn;
+      }
       if (refresh)  // z_k replaced by M^-1 rref for the next mu recurrence
         apply_dinv(dinv, nV, i, cfg.prec, T(ro0), T(ro1), T(ro2), zo.v[0], zo.v[1], zo.v[2]);
       part[0] += A(r0) * A(r0) + A(r1) * A(r1) + A(r2) * A(r2);
@@ -1933,12 +1978,15 @@ __global__ void __launch_bounds__(kEnvThreads) k_cg_env(EnvQueue q, CgConfig cfg
                                                         T* __restrict__ s, double* __restrict__ rref, EnvCG* env,
                                                         const A* __restrict__ acc, int32_t* num_active) {
   extern __shared__ __align__(16) unsigned char env_lds[];
-  Z4<T>* zl = reinterpret_cast<Z4<T>*>(env_lds);
-  T* wl = reinterpret_cast<T*>(env_lds + env_lds_plane_offset<T>(q.max_rows, 0));
-  T* rl = kVec ? reinterpret_cast<T*>(env_lds + env_lds_plane_offset<T>(q.max_rows, 1)) : nullptr;
-  T* pl = kVec ? reinterpret_cast<T*>(env_lds + env_lds_plane_offset<T>(q.max_rows, 2)) : nullptr;
-  T* sl = kVec ? reinterpret_cast<T*>(env_lds + env_lds_plane_offset<T>(q.max_rows, 3)) : nullptr;
-  A* red = reinterpret_cast<A*>(env_lds + env_lds_red_offset<T>(q.max_rows, kVec ? kEnvPlanesVec : kEnvPlanes));
+  Z4<T>* zl = kVec ? nullptr : reinterpret_cast<Z4<T>*>(env_lds);
+  T* z3 = kVec ? reinterpret_cast<T*>(env_lds) : nullptr;
+  auto plane = [&](int k) { return reinterpret_cast<T*>(env_lds + env_lds_plane_offset<T>(q.max_rows, k, kVec)); };
+  T* wl = plane(0);
+  T* rl = kVec ? plane(1) : nullptr;
+  T* pl = kVec ? plane(2) : nullptr;
+  T* sl = kVec ? plane(3) : nullptr;
+  T* xl = kVec ? plane(4) : nullptr;
+  A* red = reinterpret_cast<A*>(env_lds + env_lds_red_offset<T>(q.max_rows, kVec));
   __shared__ int s_next;
   const SpmvBufs bf = spmv_bufs<T>(col, uint32_t(nB) * (kC16 ? 2u : 4u), vals, nB, zin, nV);
   for (;;) {
@@ -1958,7 +2006,7 @@ __global__ void __launch_bounds__(kEnvThreads) k_cg_env(EnvQueue q, CgConfig cfg
       continue;
     }
     cg_env_solve<T, A, L, kC16, kVec>(e, v0, nr, cfg, row_ptr, bf, dinv, nV, b, zin, x, r, p, s, rref, env, acc,
-                                      num_active, zl, wl, rl, pl, sl, red);
+                                      num_active, zl, z3, wl, rl, pl, sl, xl, red);
     __syncthreads();  // the next environment reuses the LDS
   }
 }
@@ -2717,8 +2765,8 @@ void launch_cg_env(hipStream_t s, const EnvQueue& q, int grid, size_t lds_limit,
   if (q.max_rows <= 0 || !q.order || !q.next || !q.v0)
     throw std::invalid_argument("cg_env: empty queue description");
   const size_t lim = std::min(lds_limit, size_t(kEnvLdsMax));
-  const bool vec = env_lds_bytes<T, A>(q.max_rows, kEnvPlanesVec) <= lim;  // r, p and s in LDS too
-  const size_t lds = env_lds_bytes<T, A>(q.max_rows, vec ? kEnvPlanesVec : kEnvPlanes);
+  const bool vec = env_lds_bytes<T, A>(q.max_rows, true) <= lim;  // r, p, s and increments in LDS too
+  const size_t lds = env_lds_bytes<T, A>(q.max_rows, vec);
   if (lds > lim) throw std::invalid_argument("cg_env: an environment's rows need more LDS than a workgroup may take");
   if (uint64_t(nB) * sizeof(T) >= (uint64_t(1) << 32) || uint64_t(cfg.n_rows) * 4 * sizeof(T) >= (uint64_t(1) << 32))
     throw std::invalid_argument("matrix too large for 32-bit buffer offsets of the CG matvec");

@@ -991,7 +991,7 @@ void World<T>::drop_graphs() {
 // spread its rows over every CU).
 template <class T>
 bool World<T>::env_cg_fits() const {
-  return nEnv_ > 1 && env_max_rows_ > 0 && env_lds_bytes<T, double>(env_max_rows_, kEnvPlanes) <= size_t(env_lds_limit_);
+  return nEnv_ > 1 && env_max_rows_ > 0 && env_lds_bytes<T, double>(env_max_rows_, false) <= size_t(env_lds_limit_);
 }
 
 // Whether a solve runs as one whole-solve CG launch: asked for by the
