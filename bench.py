@@ -84,12 +84,14 @@ def body_blocks(tets, nv: int) -> int:
 
 
 def env_cg_iteration_bytes(spec, s: int = 4, col_bytes: int = 2) -> np.ndarray:
-    """Algorithmic HBM bytes of one whole-solve CG iteration (k_cg_env) of
-    each environment: its block values and column indices ((9s + c) B per
-    block), row pointers (4 B per row), and per row r, p, s and the iterate
-    (increments) read and written plus the 6-plane block-Jacobi inverse read
-    (30s B); z and w stay in LDS.  A refresh iteration's FP64 iterate and
-    true-residual traffic (every 10th) is not counted."""
+    """Algorithmic bytes of one CG iteration of each environment, as the
+    algorithm moves them (k_cg_env's roofline): its block values and column
+    indices ((9s + c) B per block), row pointers (4 B per row), and per row
+    r, p, s and the iterate's increments read and written plus the 6-plane
+    block-Jacobi inverse read (30s B).  The kernel keeps z, w, r, p, s and
+    the increments in LDS where the largest environment allows it, so its
+    measured memory traffic is below this count.  A refresh iteration's FP64
+    iterate and true-residual traffic (every 10th) is not counted."""
     cache = {}
     out = np.empty(len(spec.bodies))
     for e, b in enumerate(spec.bodies):
